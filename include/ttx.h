@@ -1,0 +1,193 @@
+/*
+ * ttx -- C ABI of the MI355X (gfx950) tensor-train solver kernels.
+ *
+ * Every entry point takes device pointers (FP64, element strides in units of
+ * elements, int64 sizes), a HIP stream, and returns a status code.  No entry
+ * point allocates, frees or retains a caller buffer; scratch memory comes in
+ * through an explicit `work` argument whose size the matching *_work_size
+ * query returns (in doubles).
+ *
+ * The reference library (ttnumpy 0.3.0) is pure Python over NumPy/SciPy and
+ * has no FFI of its own; each entry below names the reference operation it
+ * replaces (file:line in the reference tree).  The Python package
+ * `ttnumpy_amd` binds these through ctypes and exposes the reference's
+ * Python API (`amen`, `als`, `TensorTrain.truncate`, `SolverInfo`).
+ */
+#ifndef TTX_H_
+#define TTX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* ttx_stream_t; /* hipStream_t */
+
+enum {
+  TTX_OK = 0,
+  TTX_EINVAL = 1,      /* argument error            -> ValueError            */
+  TTX_ENONFINITE = 2,  /* NaN/inf input             -> ValueError("Core i has NaN or infinite entries") */
+  TTX_ENOCONV = 3,     /* iteration did not converge -> warning + retry      */
+  TTX_EHIP = 4,        /* HIP launch/runtime error   -> RuntimeError          */
+  TTX_ESINGULAR = 5    /* exactly singular pivot     -> numpy LinAlgError    */
+};
+
+/* Library version (major*10000 + minor*100 + patch). */
+int ttx_version(void);
+
+/*
+ * Strided batched FP64 GEMM on MFMA (v_mfma_f64_16x16x4f64):
+ *   C[b](m,n) = alpha * sum_k A[b](m,k) * B[b](k,n) + beta * C[b](m,n)
+ * with A[b](m,k) = A[b*a_bs + m*a_ms + k*a_ks] etc.  beta == 0 never reads C.
+ * Replaces every np.tensordot / @ / einsum of the hot path:
+ *   environment updates  ttnumpy/solvers/common.py:178-217
+ *   one-site local apply  ttnumpy/solvers/one_site.py:98-108
+ *   residual core         ttnumpy/solvers/amen.py:142-159
+ *   core absorb           ttnumpy/utils.py:72-83
+ *   TT matvec cores       ttnumpy/tensor_train.py:483-495
+ */
+int ttx_gemm_f64(ttx_stream_t stream, int64_t batch, int64_t M, int64_t N, int64_t K,
+                 double alpha,
+                 const double* A, int64_t a_ms, int64_t a_ks, int64_t a_bs,
+                 const double* B, int64_t b_ks, int64_t b_ns, int64_t b_bs,
+                 double beta,
+                 double* C, int64_t c_ms, int64_t c_ns, int64_t c_bs);
+
+/*
+ * dst[i0..i5] = alpha * src[i0..i5] over an ndim-dimensional box (ndim <= 6),
+ * with independent element strides.  Permutes, slices, block placement and
+ * zero padding of cores (ttnumpy/tensor_train.py:254-322,
+ * ttnumpy/solvers/amen.py:348-356, one_site.py:75 transpose).
+ */
+int ttx_copy_f64(ttx_stream_t stream, int ndim, const int64_t* dims,
+                 const double* src, const int64_t* src_strides,
+                 double* dst, const int64_t* dst_strides, double alpha);
+
+/* dst[0..n) = value. */
+int ttx_fill_f64(ttx_stream_t stream, double* dst, int64_t n, double value);
+
+/* z = a*x + b*y (z may alias x or y). */
+int ttx_axpby_f64(ttx_stream_t stream, int64_t n, double a, const double* x, double b,
+                  const double* y, double* z);
+
+/* out[0] = ||x||_2 (deterministic).  np.linalg.norm at common.py:787,852, amen.py:250. */
+int ttx_nrm2_f64(ttx_stream_t stream, int64_t n, const double* x, double* out);
+
+/* out[0] = ||a - b||_2 (deterministic). */
+int ttx_diff_nrm2_f64(ttx_stream_t stream, int64_t n, const double* a, const double* b, double* out);
+
+/*
+ * Economic Householder QR of the m x n matrix A(i,j) = A[i*a_rs + j*a_cs]:
+ * Q (m x k) and R (k x n), k = min(m, n), written with their own strides.
+ * scipy.linalg.qr(mode="economic") at ttnumpy/utils.py:101,130 and
+ * ttnumpy/solvers/one_site.py:188,204.  Q may be NULL (only R is formed, as the
+ * norm sweep of ttnumpy/tensor_train.py:509-540 needs).  Work: ttx_qr_work_size(m, n).
+ */
+int64_t ttx_qr_work_size(int64_t m, int64_t n);
+int ttx_qr_f64(ttx_stream_t stream, int64_t m, int64_t n,
+               const double* A, int64_t a_rs, int64_t a_cs,
+               double* Q, int64_t q_rs, int64_t q_cs,
+               double* R, int64_t r_rs, int64_t r_cs, double* work);
+
+/*
+ * Thin SVD A = U diag(S) Vt of the m x n matrix A (strided), k = min(m, n),
+ * singular values descending.  One-sided Jacobi (after a QR of the tall
+ * side).  scipy.linalg.svd at ttnumpy/utils.py:347,359 (gesdd/gesvd) and
+ * ttnumpy/solvers/amen.py:254,301 (gesvd).  info[0] receives TTX_OK,
+ * TTX_ENOCONV (sweep limit) or TTX_ENONFINITE.  Work: ttx_svd_work_size.
+ */
+int64_t ttx_svd_work_size(int64_t m, int64_t n);
+int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n,
+                const double* A, int64_t a_rs, int64_t a_cs,
+                double* U, int64_t u_rs, int64_t u_cs, double* S,
+                double* Vt, int64_t v_rs, int64_t v_cs, double* work, int32_t* info);
+
+/*
+ * Solve M u = b for a dense n x n system by LU with partial pivoting.
+ * `Mb` holds [M | b] column-major (n rows, n+1 columns, leading dim n) and is
+ * destroyed; `u` receives the solution.  scipy.linalg.solve at
+ * ttnumpy/solvers/common.py:767,802 (getrf/getrs).  info[0]: TTX_OK or
+ * TTX_ESINGULAR.  Work: ttx_lu_work_size(n) doubles.
+ */
+int64_t ttx_lu_work_size(int64_t n);
+int ttx_lu_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, double* u, double* work,
+                     int32_t* info);
+
+/*
+ * MINRES (the recurrences and stopping tests of scipy.sparse.linalg.minres,
+ * called at ttnumpy/solvers/common.py:784) with the operator applied by the
+ * caller between steps.  State lives in `st` (TTX_MINRES_STATE doubles) and
+ * the vectors in `vec` (8 * n doubles: x, v, y, r1, r2, w, w1, w2).
+ *   init : x = x0 (or 0), r1 = b - A x0 given `ax0` = A x0 (or NULL), sets v
+ *   step : given y_in = A v, one iteration; writes the next v
+ * st[0] = istop, st[1] = itn; the host polls them.  See krylov.hip.
+ */
+#define TTX_MINRES_STATE 32
+int ttx_minres_init_f64(ttx_stream_t stream, int64_t n, const double* b, const double* x0,
+                        const double* ax0, double rtol, int64_t maxiter, double* vec, double* st);
+int ttx_minres_step_f64(ttx_stream_t stream, int64_t n, const double* y_in, double* vec,
+                        double* st);
+
+/*
+ * Rank rules.
+ * truncated_bond: ttnumpy/utils.py:22-69 -> out[0] = bond.
+ */
+int ttx_truncated_bond_f64(ttx_stream_t stream, const double* S, int64_t len, double delta,
+                           int64_t max_rank, int32_t* out);
+/*
+ * Residual-budget rank (ttnumpy/solvers/common.py:812-866): given
+ * contrib (N x limit, column-major, column r = A_loc vec(s_r u_r v_r^T)),
+ * b_loc, the singular values S (S[0] == 0 gives rank 1), the scalars
+ * exact_residual = *exact (device) and half_trunc_tol = 0.5*truncation_tol,
+ * damp: out[0] = rank, out[1] = 1 if the budget was never met.  Work: N doubles.
+ */
+int ttx_rank_by_residual_f64(ttx_stream_t stream, int64_t N, int64_t limit, const double* S,
+                             const double* contrib, const double* b_loc, const double* exact,
+                             double damp, double half_trunc_tol, double* work, int32_t* out);
+/*
+ * terms (N x limit, column-major, N = p*q): column r = vec(S[r] * U[:, r] * Vt[r, :])
+ * with the row index (i, j) -> i*q + j, i < p, j < q (common.py:855-857).
+ */
+int ttx_svd_terms_f64(ttx_stream_t stream, int64_t p, int64_t q, int64_t limit,
+                      const double* U, int64_t u_rs, int64_t u_cs, const double* S,
+                      const double* Vt, int64_t v_rs, int64_t v_cs, double* terms);
+
+/*
+ * Explicit one-site local matrix, column-major into `out` (leading dim ld):
+ *   M[(f_l,i,f_r), (t_l,j,t_r)] = sum_{a,b} psi[t_l,a,f_l] A[a,i,j,b] phi[t_r,b,f_r]
+ * psi (rl, Ra, fl), A (Ra, nr, nc, Rb), phi (rr, Rb, fr), all contiguous.
+ * ttnumpy/solvers/one_site.py:49-75 (einsum + transpose).
+ */
+int ttx_local_matrix_f64(ttx_stream_t stream, const double* psi, int64_t rl, int64_t Ra,
+                         int64_t fl, const double* A, int64_t nr, int64_t nc, int64_t Rb,
+                         const double* phi, int64_t rr, int64_t fr, double* out, int64_t ld);
+
+/*
+ * One core of the TTM @ TT product (ttnumpy/tensor_train.py:483-495):
+ *   out[(a*rx + m)*o_l + i*o_i + (b*rx2 + n)*o_r] = sum_j A[a,i,j,b] x[m,j,n]
+ */
+int ttx_ttm_core_f64(ttx_stream_t stream, const double* A, int64_t Ra, int64_t nr, int64_t nc,
+                     int64_t Rb, const double* x, int64_t rx, int64_t rx2, double* out,
+                     int64_t o_l, int64_t o_i, int64_t o_r);
+
+/* X[r*rs + c*cs] *= s[c] (U*S absorbs, ttnumpy/utils.py:400-404,449-453). */
+int ttx_scale_cols_f64(ttx_stream_t stream, int64_t rows, int64_t cols, double* X, int64_t rs,
+                       int64_t cs, const double* s);
+
+/* truncated_bond with delta = scale * nrm[0] read on the device (ttnumpy/utils.py:388-390,437-440). */
+int ttx_truncated_bond_dev_f64(ttx_stream_t stream, const double* S, int64_t len,
+                               const double* nrm, double scale, int64_t max_rank, int32_t* out);
+
+/*
+ * sigma-cutoff rank (ttnumpy/solvers/common.py:872-885): out[0] = max(#{S/S[0] > thr}, 1)
+ * capped by max_rank (<= 0: no cap); out[1] = 1 when the cap bound it.
+ */
+int ttx_rank_by_threshold_f64(ttx_stream_t stream, const double* S, int64_t len, double thr,
+                              int64_t max_rank, int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TTX_H_ */

@@ -1,0 +1,846 @@
+"""CPU oracle of the tensor-train solver hot path (test infrastructure only).
+
+This module is the checker for the device implementation in ``ttnumpy_amd``:
+only the tests, the smoke check and the benchmark's CPU-baseline leg import
+it.  The product path never calls into it.
+
+It restates, in FP64 NumPy/SciPy, the algorithm of ttnumpy 0.3.0 for
+
+* ``TensorTrain.truncate`` -- QR sweep then truncated-SVD sweep
+  (``ttnumpy/tensor_train.py:705-940``, ``ttnumpy/utils.py:22-136,336-460``);
+* the one-site AMEn and ALS solvers (``ttnumpy/solvers/amen.py:80-725``,
+  ``ttnumpy/solvers/als.py:56-212``, ``ttnumpy/solvers/one_site.py:27-209``,
+  ``ttnumpy/solvers/common.py:118-929``);
+* the per-sweep residual ``||A x - b|| / ||b||`` in TT arithmetic
+  (``common.py:118-146`` via ``tensor_train.py:254-540``).
+
+Every core is a 4-D array ``(left_bond, row_mode, col_mode, right_bond)``; a
+plain train has ``col_mode == 1``.  Trains are :class:`Train` records.
+
+Pinning (what checks each function, beyond retyping its formula):
+
+* ``truncated_bond``: the known-answer values of the reference's tests
+  (``tests/test_utils.py:57-130``) -- see ``tests/test_oracle.py``.
+* ``truncate``: golden fixtures written by the reference itself
+  (``tests/golden/c2_round_d40.npz``, ``round_small_*.npz``), the two-core
+  closed form ``error = ||sigma[2:]||/||sigma||`` and orthonormality checks.
+* environments / local operator / local rhs: brute force against dense
+  interface matrices on tiny trains (``x^T A x`` projections).
+* ``amen`` / ``als`` / ``relative_residual``: the golden trajectories written
+  by the reference (bonds, residual history, stop reason, solution), and the
+  dense solution of small systems.
+"""
+
+from __future__ import annotations
+
+import copy as _copy
+from dataclasses import dataclass, field
+from math import prod
+from typing import List, Optional, Tuple
+
+import numpy as np
+import scipy.linalg as sla
+import scipy.sparse.linalg as spla
+
+
+# --------------------------------------------------------------------------
+# container
+# --------------------------------------------------------------------------
+
+
+@dataclass
+class Train:
+    """List of 4-D cores plus the canonical-centre bookkeeping.
+
+    Mirrors the fields of the reference container that the hot path reads
+    (``tensor_train.py:150-244``): ``bonds`` (d+1 ints), ``orth_center``.
+    """
+
+    cores: List[np.ndarray]
+    orth_center: Optional[int] = None
+
+    def __post_init__(self):
+        cores = []
+        for c in self.cores:
+            c = np.asarray(c, dtype=np.float64)
+            if c.ndim == 3:
+                c = c.reshape(c.shape[0], c.shape[1], 1, c.shape[2])
+            cores.append(c)
+        self.cores = cores
+
+    @property
+    def d(self) -> int:
+        return len(self.cores)
+
+    @property
+    def bonds(self) -> List[int]:
+        return [c.shape[0] for c in self.cores] + [self.cores[-1].shape[3]]
+
+    @property
+    def row_modes(self) -> List[int]:
+        return [c.shape[1] for c in self.cores]
+
+    @property
+    def col_modes(self) -> List[int]:
+        return [c.shape[2] for c in self.cores]
+
+    def copy(self) -> "Train":
+        return Train([c.copy() for c in self.cores], self.orth_center)
+
+    def full(self) -> np.ndarray:
+        """Dense tensor (row modes then col modes interleaved per core), small trains only."""
+        t = self.cores[0].reshape(-1, self.cores[0].shape[3])
+        for c in self.cores[1:]:
+            t = (t @ c.reshape(c.shape[0], -1)).reshape(-1, c.shape[3])
+        return t.reshape(-1)
+
+    def dense_matrix(self) -> np.ndarray:
+        """Dense matrix of a TTM (rows = row multi-index, cols = col multi-index)."""
+        rows, cols = self.row_modes, self.col_modes
+        d = self.d
+        t = self.full().reshape([m for pair in zip(rows, cols) for m in pair])
+        perm = list(range(0, 2 * d, 2)) + list(range(1, 2 * d, 2))
+        return t.transpose(perm).reshape(prod(rows), prod(cols))
+
+
+# --------------------------------------------------------------------------
+# rounding: QR sweeps, SVD sweeps, truncate  (utils.py, tensor_train.py)
+# --------------------------------------------------------------------------
+
+
+def truncated_bond(s: np.ndarray, delta: float, max_rank: Optional[int] = None) -> int:
+    """Smallest rank whose discarded tail has Frobenius norm <= delta.
+
+    ``utils.py:22-69``: values <= ``len(s) * eps * s[0]`` count as zero, the
+    tail is the reversed cumulative sum of squares, floor 1, cap max_rank.
+    """
+    if max_rank is not None and max_rank <= 0:
+        raise ValueError("max_rank must be a positive integer.")
+    s = np.asarray(s)
+    eps = np.finfo(s.dtype).eps if np.issubdtype(s.dtype, np.inexact) else np.finfo(np.float64).eps
+    v = s.astype(float).copy()
+    if v.size:
+        v[v <= eps * v.size * v[0]] = 0.0
+    tail = np.cumsum((v ** 2)[::-1])[::-1]
+    r = max(1, int(np.count_nonzero(tail > delta * delta)))
+    if max_rank is not None:
+        r = min(r, max_rank)
+    return r
+
+
+def _absorb_left(m: np.ndarray, core: np.ndarray) -> np.ndarray:
+    """``m @ core`` over the core's first axis (``utils.py:72-76``)."""
+    return (m @ core.reshape(core.shape[0], -1)).reshape((m.shape[0],) + core.shape[1:])
+
+
+def _absorb_right(core: np.ndarray, m: np.ndarray) -> np.ndarray:
+    """``core @ m`` over the core's last axis (``utils.py:79-83``)."""
+    return (core.reshape(-1, core.shape[-1]) @ m).reshape(core.shape[:-1] + (m.shape[1],))
+
+
+def qr_sweep_left(cores: List[np.ndarray], start: int, end: int) -> None:
+    """Left-orthonormalise cores ``start..end-1`` in place (``utils.py:86-112``)."""
+    for i in range(start, end):
+        s = cores[i].shape
+        q, r = sla.qr(cores[i].reshape(s[0] * s[1] * s[2], s[3]), mode="economic")
+        cores[i] = q.reshape(s[0], s[1], s[2], -1)
+        cores[i + 1] = _absorb_left(r, cores[i + 1])
+
+
+def qr_sweep_right(cores: List[np.ndarray], start: int, end: int) -> None:
+    """Right-orthonormalise cores ``start..end+1`` in place (``utils.py:115-136``)."""
+    for i in range(start, end, -1):
+        s = cores[i].shape
+        q, r = sla.qr(cores[i].reshape(s[0], s[1] * s[2] * s[3]).T, mode="economic")
+        cores[i] = q.T.reshape(-1, s[1], s[2], s[3])
+        cores[i - 1] = _absorb_right(cores[i - 1], r.T)
+
+
+def _thin_svd(m: np.ndarray, core: int):
+    """``gesdd`` with a ``gesvd`` retry (``utils.py:336-361``)."""
+    try:
+        return sla.svd(m, full_matrices=False, check_finite=False)
+    except np.linalg.LinAlgError as err:
+        if not np.isfinite(m).all():
+            raise ValueError(f"Core {core} has NaN or infinite entries") from err
+        return sla.svd(m, full_matrices=False, check_finite=False, lapack_driver="gesvd")
+
+
+def svd_sweep_left(cores, error, max_rank=None, spectra=None):
+    """Left-to-right truncated SVD sweep (``utils.py:364-411``)."""
+    c0 = cores[0]
+    nrm = np.linalg.norm(c0.reshape(-1))
+    delta = error / np.sqrt(len(cores) - 1) * nrm
+    bond = 1
+    for i in range(len(cores) - 1):
+        s = cores[i].shape
+        u, sv, vt = _thin_svd(cores[i].reshape(bond * s[1] * s[2], s[3]), i)
+        if spectra is not None:
+            spectra.append(sv.copy())
+        nb = truncated_bond(sv, delta, max_rank)
+        cores[i] = u[:, :nb].reshape(bond, s[1], s[2], nb)
+        cores[i + 1] = _absorb_left(sv[:nb, None] * vt[:nb], cores[i + 1])
+        bond = nb
+    return cores
+
+
+def svd_sweep_right(cores, error, max_rank=None, spectra=None):
+    """Right-to-left truncated SVD sweep (``utils.py:414-460``)."""
+    cl = cores[-1]
+    nrm = np.linalg.norm(cl.reshape(-1))
+    delta = error / np.sqrt(len(cores) - 1) * nrm
+    bond = 1
+    for i in range(len(cores) - 1, 0, -1):
+        s = cores[i].shape
+        u, sv, vt = _thin_svd(cores[i].reshape(s[0], s[1] * s[2] * bond), i)
+        if spectra is not None:
+            spectra.append(sv.copy())
+        nb = truncated_bond(sv, delta, max_rank)
+        cores[i] = vt[:nb].reshape(nb, s[1], s[2], bond)
+        cores[i - 1] = _absorb_right(cores[i - 1], u[:, :nb] * sv[:nb])
+        bond = nb
+    return cores
+
+
+def orthonormalize(t: Train, canonical: str = "left", center: Optional[int] = None) -> Train:
+    """In-place QR sweep with the centre bookkeeping of ``tensor_train.py:705-787``."""
+    d = t.d
+    if center is not None and not 0 <= center < d:
+        raise ValueError(f"center_position must be in [0, {d - 1}], but got {center}")
+    had = t.orth_center is not None
+    if canonical == "left":
+        end = center if center is not None else d - 1
+        start = 0
+        if t.orth_center is not None:
+            start = t.orth_center
+            t.orth_center = end if start <= end else t.orth_center
+        if start >= end:
+            return t
+        qr_sweep_left(t.cores, start, end)
+    elif canonical == "right":
+        end = center if center is not None else 0
+        start = d - 1
+        if t.orth_center is not None:
+            start = t.orth_center
+            t.orth_center = end if start >= end else t.orth_center
+        if start <= end:
+            return t
+        qr_sweep_right(t.cores, start, end)
+    elif canonical == "mix":
+        if center is None:
+            center = d // 2
+        end = center
+        qr_sweep_left(t.cores, 0, center)
+        qr_sweep_right(t.cores, d - 1, center)
+    else:
+        raise ValueError(f"Canonical form must be 'left', 'right' or 'mix', but got {canonical}")
+    t.orth_center = end if (had or canonical == "mix" or end in (0, d - 1)) else None
+    return t
+
+
+def compress_svd(t: Train, error: float = 0.0, max_rank=None, canonical="auto", spectra=None) -> Train:
+    """In-place SVD sweep, direction by centre (``tensor_train.py:789-850``)."""
+    if canonical == "auto":
+        canonical = "left" if (t.orth_center is None or t.orth_center <= (t.d - 1) / 2) else "right"
+    if canonical == "right":
+        svd_sweep_right(t.cores, error, max_rank, spectra)
+        t.orth_center = 0
+    elif canonical == "left":
+        svd_sweep_left(t.cores, error, max_rank, spectra)
+        t.orth_center = t.d - 1
+    else:
+        raise ValueError(f"Canonical form must be 'left', 'right' or 'auto', but got {canonical}")
+    return t
+
+
+def truncate(t: Train, error: float = 0.0, max_rank=None, canonical="auto", spectra=None) -> Train:
+    """Copy, QR sweep, SVD sweep (``tensor_train.py:852-940``, ``method="direct"``)."""
+    if canonical not in ("auto", "left", "right"):
+        raise ValueError(f"Canonical form must be 'left', 'right' or 'auto', but got {canonical}")
+    out = t.copy()
+    form = "left" if canonical == "right" else "right"
+    if canonical == "auto":
+        form = "left" if (out.orth_center is None or out.orth_center >= out.d / 2) else "right"
+    orthonormalize(out, form)
+    return compress_svd(out, error, max_rank, spectra=spectra)
+
+
+# --------------------------------------------------------------------------
+# TT arithmetic used by the residual check (tensor_train.py:254-540)
+# --------------------------------------------------------------------------
+
+
+def tt_add(a: Train, b: Train) -> Train:
+    """Block-placed sum (``tensor_train.py:254-322``); centre unknown."""
+    d = a.d
+    out = []
+    for i in range(d):
+        ca, cb = a.cores[i], b.cores[i]
+        left = 1 if i == 0 else ca.shape[0] + cb.shape[0]
+        right = 1 if i == d - 1 else ca.shape[3] + cb.shape[3]
+        c = np.zeros((left, ca.shape[1], ca.shape[2], right))
+        c[: ca.shape[0], :, :, : ca.shape[3]] = ca
+        c[left - cb.shape[0]:, :, :, right - cb.shape[3]:] = cb
+        out.append(c)
+    return Train(out)
+
+
+def tt_scale(a: Train, alpha: float) -> Train:
+    """Scalar multiple on the centre core, or core 0 (``tensor_train.py:396-401``)."""
+    out = a.copy()
+    k = out.orth_center if out.orth_center is not None else 0
+    out.cores[k] = alpha * out.cores[k]
+    return out
+
+
+def tt_matvec(A: Train, x: Train) -> Train:
+    """TTM @ TT cores ``(Ra*rx, n, 1, Ra'*rx')`` (``tensor_train.py:427-499``)."""
+    out = []
+    for ca, cx in zip(A.cores, x.cores):
+        c = np.einsum("aijb,mjkn->amikbn", ca, cx)
+        out.append(c.reshape(ca.shape[0] * cx.shape[0], ca.shape[1], cx.shape[2], ca.shape[3] * cx.shape[3]))
+    return Train(out)
+
+
+def tt_norm(t: Train) -> float:
+    """Frobenius norm: centre core if known, else right QR sweep (``tensor_train.py:509-540``)."""
+    if t.orth_center is not None:
+        return float(np.linalg.norm(t.cores[t.orth_center].reshape(-1)))
+    c = [x.reshape(x.shape[0], x.shape[1] * x.shape[2], 1, x.shape[3]).copy() for x in t.cores]
+    qr_sweep_right(c, len(c) - 1, 0)
+    return float(np.linalg.norm(c[0].reshape(-1)))
+
+
+def relative_residual(A: Train, b: Train, x: Train, b_norm: Optional[float] = None) -> float:
+    """``||A x - b|| / ||b||`` in TT arithmetic (``common.py:118-146``)."""
+    r = tt_add(tt_matvec(A, x), tt_scale(b, -1.0))
+    if b_norm is None:
+        b_norm = tt_norm(b)
+    rn = tt_norm(r)
+    return rn if b_norm == 0.0 else rn / b_norm
+
+
+# --------------------------------------------------------------------------
+# environments and the one-site local system (common.py, one_site.py)
+# --------------------------------------------------------------------------
+
+
+def env_left(prev: np.ndarray, frame_core: np.ndarray, target_core: np.ndarray,
+             op_core: Optional[np.ndarray] = None) -> np.ndarray:
+    """One step of the left environment recursion (``common.py:149-188``).
+
+    With the operator: ``E'[t',a',f'] = sum E[t,a,f] F[f,i,f'] A[a,i,j,a'] T[t,j,t']``;
+    without: ``E'[t',f'] = sum E[t,f] F[f,i,f'] T[t,i,t']``.  ``F`` is the
+    frame core (conjugated; real here), ``T`` the target core, both 3-D.
+    """
+    if op_core is not None:
+        s = np.tensordot(prev, frame_core, axes=([2], [0]))          # t a i f'
+        s = np.tensordot(s, op_core, axes=([1, 2], [0, 1]))          # t f' j a'
+        return np.tensordot(s, target_core, axes=([0, 2], [0, 1])).transpose(2, 1, 0)
+    s = np.tensordot(prev, frame_core, axes=([1], [0]))              # t i f'
+    return np.tensordot(s, target_core, axes=([0, 1], [0, 1])).T
+
+
+def env_right(nxt: np.ndarray, frame_core: np.ndarray, target_core: np.ndarray,
+              op_core: Optional[np.ndarray] = None) -> np.ndarray:
+    """Mirror recursion (``common.py:191-217``), indexed (target, op, frame)."""
+    if op_core is not None:
+        s = np.tensordot(target_core, nxt, axes=([2], [0]))          # t j a' f'
+        s = np.tensordot(s, op_core, axes=([1, 2], [2, 3]))          # t f' a i
+        return np.tensordot(s, frame_core, axes=([1, 3], [2, 1]))    # t a f
+    s = np.tensordot(target_core, nxt, axes=([2], [0]))              # t i f'
+    return np.tensordot(s, frame_core, axes=([1, 2], [1, 2]))
+
+
+def _c3(core4: np.ndarray) -> np.ndarray:
+    return core4[:, :, 0, :]
+
+
+def build_env(envs: list, k: int, side: str, frame: Train, target: Train, A: Optional[Train] = None):
+    """Set ``envs[k]`` from its neighbour (boundary = ones), as the reference does."""
+    with_op = A is not None
+    if side == "left":
+        if k == 0:
+            envs[0] = np.ones((1, 1, 1)) if with_op else np.ones((1, 1))
+            return
+        envs[k] = env_left(envs[k - 1], _c3(frame.cores[k - 1]).conj(), _c3(target.cores[k - 1]),
+                           A.cores[k - 1] if with_op else None)
+    else:
+        if k == frame.d - 1:
+            envs[-1] = np.ones((1, 1, 1)) if with_op else np.ones((1, 1))
+            return
+        envs[k] = env_right(envs[k + 1], _c3(frame.cores[k + 1]).conj(), _c3(target.cores[k + 1]),
+                            A.cores[k + 1] if with_op else None)
+
+
+def local_rhs(psi_b: np.ndarray, b_core3: np.ndarray, phi_b: np.ndarray) -> np.ndarray:
+    """``b_loc[f_l,i,f_r] = sum Psi[t,f_l] b[t,i,t'] Phi[t',f_r]`` (``one_site.py:27-46``)."""
+    return np.einsum("ai,ajc,cl->ijl", psi_b, b_core3, phi_b, optimize="optimal").reshape(-1)
+
+
+def local_matrix(psi: np.ndarray, op_core: np.ndarray, phi: np.ndarray) -> np.ndarray:
+    """Explicit one-site operator, rows (f_l,i,f_r), cols (t_l,j,t_r) (``one_site.py:49-75``)."""
+    m = np.einsum("iaj,aklb,mbn->ijklmn", psi, op_core, phi, optimize="optimal")
+    n_row, n_col = op_core.shape[1], op_core.shape[2]
+    rows = psi.shape[2] * n_row * phi.shape[2]
+    cols = psi.shape[0] * n_col * phi.shape[0]
+    return m.transpose([1, 2, 5, 0, 3, 4]).reshape(rows, cols)
+
+
+def local_apply(psi: np.ndarray, op_core: np.ndarray, phi: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """Matrix-free one-site apply on columns of ``y`` (``one_site.py:78-115``).
+
+    ``y`` has shape ``(N,)`` or ``(N, B)`` with rows indexed (t_l, j, t_r);
+    the result rows are (f_l, i, f_r).
+    """
+    rl, rr = psi.shape[0], phi.shape[0]
+    vec = y.ndim == 1
+    yy = y.reshape(rl, op_core.shape[2], rr, -1)
+    t = np.tensordot(yy, phi, axes=(2, 0))                      # i l z b n
+    t = np.tensordot(t, op_core, axes=((1, 3), (2, 3)))          # i z n a k
+    t = np.tensordot(t, psi, axes=((0, 3), (0, 1)))              # z n k j
+    out = t.transpose(3, 2, 1, 0).reshape(psi.shape[2] * op_core.shape[1] * phi.shape[2], -1)
+    return out.ravel() if vec else out
+
+
+class _LocalOp(spla.LinearOperator):
+    def __init__(self, psi, op_core, phi):
+        n = psi.shape[0] * op_core.shape[1] * phi.shape[0]
+        super().__init__(dtype=np.float64, shape=(n, n))
+        self.psi, self.op_core, self.phi = psi, op_core, phi
+
+    def _matvec(self, v):
+        return local_apply(self.psi, self.op_core, self.phi, np.asarray(v).reshape(-1))
+
+    def _matmat(self, m):
+        return local_apply(self.psi, self.op_core, self.phi, np.asarray(m))
+
+
+# constants of common.py:220-241
+LOCAL_TOL_FACTOR = 0.1
+DEFAULT_LOCAL_RTOL = 1e-8
+MAX_LOCAL_RESTARTS = 4
+RESTART_RTOL_SHRINK = 1e-2
+DEFAULT_STAGNATION = 0.9
+
+
+@dataclass
+class Settings:
+    """Resolved options (``common.py:244-400``)."""
+
+    max_sweeps: int
+    residual_tol: Optional[float]
+    stagnation_factor: Optional[float]
+    iterative: Optional[str]
+    direct_solve_size: int
+    track_residual: bool
+    local_rtol: Optional[float]
+    allow_dense_fallback: bool = False
+    truncation_steps: int = 1
+
+    @property
+    def residual_mode(self) -> bool:
+        return self.residual_tol is not None
+
+    @property
+    def truncation_tol(self) -> Optional[float]:
+        return None if self.residual_tol is None else self.residual_tol / np.sqrt(self.truncation_steps)
+
+    def use_iterative(self, n: int) -> bool:
+        return self.iterative is not None and n > self.direct_solve_size
+
+
+def resolve_settings(max_sweeps, residual_tol, stagnation_factor, local_solver, symmetric,
+                     direct_solve_size, return_info, local_rtol=None, allow_dense_fallback=False,
+                     truncation_steps=1) -> Settings:
+    """Validation and derived fields (``common.py:301-400``)."""
+    if residual_tol is not None and residual_tol <= 0.0:
+        raise ValueError(f"residual_tol must be positive, got {residual_tol}")
+    if truncation_steps < 1:
+        raise ValueError(f"truncation_steps must be positive, got {truncation_steps}")
+    if isinstance(stagnation_factor, str):
+        if stagnation_factor != "auto":
+            raise ValueError("stagnation_factor must be a float in (0, 1], None or 'auto'")
+        stagnation_factor = DEFAULT_STAGNATION if (residual_tol is not None or return_info) else None
+    if stagnation_factor is not None and not 0.0 < stagnation_factor <= 1.0:
+        raise ValueError(f"stagnation_factor must be in (0, 1], got {stagnation_factor}")
+    track = return_info or residual_tol is not None or (stagnation_factor is not None and max_sweeps > 1)
+    if local_rtol is not None and local_rtol <= 0.0:
+        raise ValueError(f"local_rtol must be positive, got {local_rtol}")
+    ls = str(local_solver)
+    if ls not in ("auto", "direct", "minres", "gmres"):
+        raise ValueError(f"local_solver must be one of 'auto', 'direct', 'minres', 'gmres', got {local_solver!r}")
+    iterative = ("minres" if symmetric else "gmres") if ls == "auto" else (None if ls == "direct" else ls)
+    if iterative is None:
+        lr = None
+    elif local_rtol is not None:
+        lr = local_rtol
+    elif residual_tol is not None:
+        lr = LOCAL_TOL_FACTOR * residual_tol
+    else:
+        lr = DEFAULT_LOCAL_RTOL
+    return Settings(max_sweeps, residual_tol, stagnation_factor, iterative, direct_solve_size,
+                    track, lr, allow_dense_fallback, truncation_steps)
+
+
+@dataclass
+class Info:
+    """Same fields as the reference ``SolverInfo`` (``common.py:85-115``)."""
+
+    residuals: List[float] = field(default_factory=list)
+    num_sweeps_run: int = 0
+    stop_reason: str = "max_sweeps"
+    max_bonds: List[int] = field(default_factory=list)
+    local_solves: int = 0
+    local_fallbacks: int = 0
+    local_inexact: int = 0
+
+
+def solve_local(op, b_vec, iterative, local_rtol, warm=None, fallback=None, info: Optional[Info] = None):
+    """Direct LU or restarted Krylov solve (``common.py:729-809``)."""
+    if iterative is None:
+        return np.asarray(sla.solve(op, b_vec, check_finite=False)).ravel(), op
+    target = local_rtol * np.linalg.norm(b_vec)
+    u = warm
+    rt = local_rtol
+    for _ in range(MAX_LOCAL_RESTARTS):
+        if iterative == "minres":
+            u, _ = spla.minres(op, b_vec, x0=u, rtol=rt)
+        else:
+            u, _ = spla.gmres(op, b_vec, x0=u, rtol=rt, atol=0.0)
+        if np.linalg.norm(op @ u - b_vec) <= target:
+            if info is not None:
+                info.local_solves += 1
+            return np.asarray(u).ravel(), op
+        rt = max(rt * RESTART_RTOL_SHRINK, 1e-15)
+    if fallback is None:
+        if info is not None:
+            info.local_solves += 1
+            info.local_inexact += 1
+        return np.asarray(u).ravel(), op
+    if info is not None:
+        info.local_solves += 1
+        info.local_fallbacks += 1
+    op = fallback()
+    return np.asarray(sla.solve(op, b_vec, check_finite=False)).ravel(), op
+
+
+def rank_by_residual(op, b_loc, U, S, Vt, exact_res, trunc_tol, damp, max_rank) -> Tuple[int, bool]:
+    """Smallest rank whose local residual meets the budget (``common.py:812-866``)."""
+    if S[0] == 0.0:
+        return 1, False
+    limit = len(S) if max_rank is None else min(len(S), max_rank)
+    thr = max(damp * exact_res, 0.5 * trunc_tol * np.linalg.norm(b_loc))
+    terms = np.einsum("ik,kj->kij", U[:, :limit] * S[:limit], Vt[:limit, :], optimize="optimal")
+    contrib = op @ terms.reshape(limit, -1).T
+    partial = -b_loc.copy()
+    for r in range(limit):
+        partial += contrib[:, r]
+        if np.linalg.norm(partial) <= thr:
+            return r + 1, False
+    return limit, max_rank is not None and max_rank < len(S)
+
+
+def rank_by_threshold(S, thr, max_rank) -> Tuple[int, bool]:
+    """sigma_i / sigma_1 > thr count (``common.py:872-885``)."""
+    if S.size == 0 or S[0] == 0.0:
+        return 1, False
+    r = max(int(np.count_nonzero(S / S[0] > thr)), 1)
+    if max_rank is not None and r > max_rank:
+        return max_rank, True
+    return r, False
+
+
+def _choose_rank(settings, op, b_loc, U, S, Vt, exact, error_threshold, max_rank, damp):
+    if settings.residual_mode:
+        return rank_by_residual(op, b_loc, U, S, Vt, exact, settings.truncation_tol, damp, max_rank)[0]
+    return rank_by_threshold(S, error_threshold, max_rank)[0]
+
+
+class _System:
+    """Environments of one sweep solver, held per site (``amen.py:538-566``)."""
+
+    def __init__(self, d):
+        self.L = {}
+        self.R = {}
+
+    def env(self, side, name, d):
+        store = self.L if side == "left" else self.R
+        if name not in store:
+            store[name] = [None] * d
+        return store[name]
+
+
+def _solve_site(k, x, A, b, eL, eR, bL, bR, settings, info):
+    """Assemble and solve the one-site system at ``k`` (``one_site.py:118-178``)."""
+    n = x.bonds[k] * x.row_modes[k] * x.bonds[k + 1]
+    b_loc = local_rhs(bL[k], _c3(b.cores[k]), bR[k])
+    if settings.use_iterative(n):
+        op = _LocalOp(eL[k], A.cores[k], eR[k])
+        warm = _c3(x.cores[k]).reshape(-1)
+        if warm.size != n:
+            warm = None
+        fb = (lambda: local_matrix(eL[k], A.cores[k], eR[k])) if settings.allow_dense_fallback else None
+        u, op = solve_local(op, b_loc, settings.iterative, settings.local_rtol, warm, fb, info)
+    else:
+        op = local_matrix(eL[k], A.cores[k], eR[k])
+        u, op = solve_local(op, b_loc, None, None)
+    return u, op, b_loc
+
+
+def shift_right(x: Train, k: int, core3: np.ndarray) -> None:
+    """QR of (r_l n x r_r), Q stored at k, R into k+1 (``one_site.py:181-193``)."""
+    q, r = sla.qr(core3.reshape(-1, core3.shape[-1]), mode="economic")
+    x.cores[k] = q.reshape(core3.shape[0], core3.shape[1], 1, q.shape[1])
+    x.cores[k + 1] = _absorb_left(r, x.cores[k + 1])
+    x.orth_center = k + 1
+
+
+def shift_left(x: Train, k: int, core3: np.ndarray) -> None:
+    """QR of the transposed right unfolding, R^T into k-1 (``one_site.py:196-209``)."""
+    q, r = sla.qr(core3.reshape(core3.shape[0], -1).T, mode="economic")
+    x.cores[k] = q.T.reshape(q.shape[1], core3.shape[1], 1, core3.shape[2])
+    x.cores[k - 1] = _absorb_right(x.cores[k - 1], r.T)
+    x.orth_center = k - 1
+
+
+def residual_core(k, zbL, zbR, zAL, zAR, A, b, x_core3):
+    """Projection of ``b - A x`` onto z's frame at ``k`` (``amen.py:109-159``)."""
+    b_part = local_rhs(zbL[k], _c3(b.cores[k]), zbR[k]).reshape(zbL[k].shape[1], b.cores[k].shape[1], zbR[k].shape[1])
+    left, core, right = zAL[k], A.cores[k], zAR[k]
+    xl, al, zl = left.shape
+    _, rows, cols, ar = core.shape
+    xr, _, zr = right.shape
+    s = left.transpose(2, 1, 0).reshape(zl * al, xl) @ x_core3.reshape(xl, cols * xr)
+    s = s.reshape(zl, al, cols, xr).transpose(0, 3, 1, 2).reshape(zl * xr, al * cols)
+    s = s @ core.transpose(0, 2, 1, 3).reshape(al * cols, rows * ar)
+    s = s.reshape(zl, xr, rows, ar).transpose(0, 2, 1, 3).reshape(zl * rows, xr * ar) @ right.reshape(xr * ar, zr)
+    return b_part - s.reshape(zl, rows, zr)
+
+
+def fit_residual(A, b, x, z, sweeps=1):
+    """ALS fit of z to ``b - A x`` (``amen.py:162-224``)."""
+    d = z.d
+    zbR, zAR, zbL, zAL = [None] * d, [None] * d, [None] * d, [None] * d
+    for i in range(d - 1, -1, -1):
+        build_env(zbR, i, "right", z, b)
+        build_env(zAR, i, "right", z, x, A)
+    for _ in range(sweeps):
+        for k in range(d - 1):
+            build_env(zbL, k, "left", z, b)
+            build_env(zAL, k, "left", z, x, A)
+            shift_right(z, k, residual_core(k, zbL, zbR, zAL, zAR, A, b, _c3(x.cores[k])))
+        build_env(zbL, d - 1, "left", z, b)
+        build_env(zAL, d - 1, "left", z, x, A)
+        for k in range(d - 1, -1, -1):
+            build_env(zbR, k, "right", z, b)
+            build_env(zAR, k, "right", z, x, A)
+            c = residual_core(k, zbL, zbR, zAL, zAR, A, b, _c3(x.cores[k]))
+            if k > 0:
+                shift_left(z, k, c)
+            else:
+                z.cores[0] = c.reshape(z.bonds[0], c.shape[1], 1, z.bonds[1])
+                z.orth_center = 0
+    return z
+
+
+def _exact_residual(op, u, b_loc):
+    return float(np.linalg.norm(op @ u - b_loc))
+
+
+def truncate_core_right(core3, settings, error_threshold, max_rank, op, b_loc, damp):
+    """SVD split of (r_l n x r_r) (``amen.py:227-275``): (U_r, S V_r)."""
+    rl, n, rr = core3.shape
+    exact = _exact_residual(op, core3.ravel(), b_loc) if settings.residual_mode else 0.0
+    U, S, Vt = sla.svd(core3.reshape(rl * n, rr), full_matrices=False, check_finite=False, lapack_driver="gesvd")
+    r = _choose_rank(settings, op, b_loc, U, S, Vt, exact, error_threshold, max_rank, damp)
+    return U[:, :r].reshape(rl, n, r), S[:r, None] * Vt[:r]
+
+
+def truncate_core_left(core3, settings, error_threshold, max_rank, op, b_loc, damp):
+    """Mirror (``amen.py:278-322``): (V_r as (r,n,r_r), U_r S)."""
+    rl, n, rr = core3.shape
+    exact = _exact_residual(op, core3.ravel(), b_loc) if settings.residual_mode else 0.0
+    U, S, Vt = sla.svd(core3.reshape(rl, n * rr), full_matrices=False, check_finite=False, lapack_driver="gesvd")
+    r = _choose_rank(settings, op, b_loc, U, S, Vt, exact, error_threshold, max_rank, damp)
+    return Vt[:r].reshape(r, n, rr), U[:, :r] * S[:r]
+
+
+def enrich_right(x, k, core, carry, block, max_rank):
+    """Append ``block`` on the right bond, zero-padded carry into k+1 (``amen.py:325-360``)."""
+    enriched = np.concatenate([core, block], axis=2)
+    pad = np.zeros((block.shape[2], carry.shape[1]))
+    x.cores[k + 1] = _absorb_left(np.concatenate([carry, pad], axis=0), x.cores[k + 1])
+    shift_right(x, k, enriched)
+    if max_rank is not None and x.bonds[k + 1] > max_rank:
+        x.cores[k] = x.cores[k][:, :, :, :max_rank]
+        x.cores[k + 1] = x.cores[k + 1][:max_rank]
+
+
+def enrich_left(x, k, core, carry, block, max_rank):
+    """Mirror (``amen.py:363-392``)."""
+    enriched = np.concatenate([core, block], axis=0)
+    pad = np.zeros((carry.shape[0], block.shape[0]))
+    x.cores[k - 1] = _absorb_right(x.cores[k - 1], np.concatenate([carry, pad], axis=1))
+    shift_left(x, k, enriched)
+    if max_rank is not None and x.bonds[k] > max_rank:
+        x.cores[k] = x.cores[k][:max_rank]
+        x.cores[k - 1] = x.cores[k - 1][:, :, :, :max_rank]
+
+
+def _record(settings, info, A, b, x, b_norm, sweep, max_bond=None) -> bool:
+    """Per-sweep residual and stop test (``common.py:504-592``)."""
+    info.num_sweeps_run = sweep + 1
+    if max_bond is not None:
+        info.max_bonds.append(max_bond)
+    if not settings.track_residual:
+        return False
+    res = relative_residual(A, b, x, b_norm)
+    info.residuals.append(res)
+    if settings.residual_tol is not None and res <= settings.residual_tol:
+        info.stop_reason = "converged"
+        return True
+    if (settings.stagnation_factor is not None and len(info.residuals) >= 2
+            and info.residuals[-1] > settings.stagnation_factor * info.residuals[-2]):
+        info.stop_reason = "stagnated"
+        return True
+    return False
+
+
+def _validate(A: Train, b: Train, x0: Train):
+    """Structural checks (``common.py:684-726``)."""
+    if any(m != 1 for m in x0.col_modes):
+        raise TypeError("Initial guess x0 must be a TensorTrain, not a TTM.")
+    if any(m != 1 for m in b.col_modes):
+        raise TypeError("Right-hand side b must be a TensorTrain, not a TTM.")
+    if not (A.d == b.d == x0.d):
+        raise ValueError("A, b and x0 must have the same number of cores")
+    if A.row_modes != A.col_modes or A.row_modes != b.row_modes or A.col_modes != x0.row_modes:
+        raise ValueError("dimension mismatch")
+
+
+def amen(A: Train, b: Train, x0: Optional[Train] = None, kickrank=4, error_threshold=1e-7,
+         max_rank=None, max_sweeps=1, residual_tol=None, residual_damp=2.0, stagnation_factor="auto",
+         local_solver="auto", direct_solve_size=1000, local_rtol=None, allow_dense_fallback=False,
+         symmetric=False, seed=None, return_info=True):
+    """One-site AMEn (``amen.py:395-725``).  Returns ``(x, Info)``."""
+    if kickrank <= 0:
+        raise ValueError(f"kickrank must be a positive integer, got {kickrank}")
+    if max_rank is not None and max_rank <= 0:
+        raise ValueError(f"max_rank must be a positive integer, got {max_rank}")
+    if residual_damp < 1.0:
+        raise ValueError(f"residual_damp must be >= 1, got {residual_damp}")
+    st = resolve_settings(max_sweeps, residual_tol, stagnation_factor, local_solver, symmetric,
+                          direct_solve_size, return_info, local_rtol, allow_dense_fallback,
+                          max(2 * (A.d - 1), 1))
+    if x0 is None:
+        x0 = Train([np.ones((b.bonds[i], b.row_modes[i], 1, b.bonds[i + 1])) for i in range(b.d)])
+    _validate(A, b, x0)
+    x = x0.copy()
+    if x.orth_center != 0:
+        orthonormalize(x, "right")
+    d = x.d
+    eA_L, eA_R, eb_L, eb_R = [None] * d, [None] * d, [None] * d, [None] * d
+    rng = np.random.default_rng(seed)
+    z = Train([rng.random((([1] + [kickrank] * (d - 1) + [1])[i], b.row_modes[i], 1,
+                           ([1] + [kickrank] * (d - 1) + [1])[i + 1])) for i in range(d)])
+    orthonormalize(z, "right")
+    fit_residual(A, b, x, z, 1)
+    zbL, zbR, zAL, zAR, xzL, xzR = [None] * d, [None] * d, [None] * d, [None] * d, [None] * d, [None] * d
+    for i in range(d - 1, -1, -1):
+        build_env(eA_R, i, "right", x, x, A)
+        build_env(eb_R, i, "right", x, b)
+        build_env(zbR, i, "right", z, b)
+        build_env(zAR, i, "right", z, x, A)
+        build_env(xzR, i, "right", x, z)
+    info = Info()
+    b_norm = tt_norm(b) if st.track_residual else None
+
+    def left_envs(k):
+        build_env(eA_L, k, "left", x, x, A)
+        build_env(eb_L, k, "left", x, b)
+        build_env(zbL, k, "left", z, b)
+        build_env(zAL, k, "left", z, x, A)
+        build_env(xzL, k, "left", x, z)
+
+    def right_envs(k):
+        build_env(eA_R, k, "right", x, x, A)
+        build_env(eb_R, k, "right", x, b)
+        build_env(zbR, k, "right", z, b)
+        build_env(zAR, k, "right", z, x, A)
+        build_env(xzR, k, "right", x, z)
+
+    for sweep in range(st.max_sweeps):
+        for k in range(d - 1):
+            left_envs(k)
+            u, op, b_loc = _solve_site(k, x, A, b, eA_L, eA_R, eb_L, eb_R, st, info)
+            core3 = u.reshape(x.bonds[k], x.row_modes[k], x.bonds[k + 1])
+            z_new = residual_core(k, zbL, zbR, zAL, zAR, A, b, core3)
+            core, carry = truncate_core_right(core3, st, error_threshold, max_rank, op, b_loc, residual_damp)
+            block = np.einsum("ba,bin->ain", xzL[k], z_new)
+            enrich_right(x, k, core, carry, block, max_rank)
+            shift_right(z, k, z_new)
+        left_envs(d - 1)
+        for k in range(d - 1, -1, -1):
+            right_envs(k)
+            u, op, b_loc = _solve_site(k, x, A, b, eA_L, eA_R, eb_L, eb_R, st, info)
+            core3 = u.reshape(x.bonds[k], x.row_modes[k], x.bonds[k + 1])
+            z_new = residual_core(k, zbL, zbR, zAL, zAR, A, b, core3)
+            if k == 0:
+                x.cores[0] = core3.reshape(x.bonds[0], core3.shape[1], 1, x.bonds[1])
+                x.orth_center = 0
+                z.cores[0] = z_new.reshape(z.bonds[0], z_new.shape[1], 1, z.bonds[1])
+                z.orth_center = 0
+            else:
+                core, carry = truncate_core_left(core3, st, error_threshold, max_rank, op, b_loc, residual_damp)
+                block = _absorb_right(z_new, xzR[k])
+                enrich_left(x, k, core, carry, block, max_rank)
+                shift_left(z, k, z_new)
+        if _record(st, info, A, b, x, b_norm, sweep, max(x.bonds)):
+            break
+    redundant = [k for k in range(1, d) if x.bonds[k] > prod(x.row_modes[:k])]
+    if redundant:
+        orthonormalize(x, "left", redundant[-1])
+        orthonormalize(x, "right", 0)
+    return x, info
+
+
+def als(A: Train, b: Train, x0: Optional[Train] = None, max_sweeps=4, residual_tol=None,
+        stagnation_factor="auto", local_solver="auto", direct_solve_size=1000, local_rtol=None,
+        allow_dense_fallback=False, symmetric=False, return_info=True):
+    """Fixed-rank one-site ALS (``als.py:56-212``).  Returns ``(x, Info)``."""
+    st = resolve_settings(max_sweeps, residual_tol, stagnation_factor, local_solver, symmetric,
+                          direct_solve_size, return_info, local_rtol, allow_dense_fallback)
+    if x0 is None:
+        x0 = Train([np.ones((b.bonds[i], b.row_modes[i], 1, b.bonds[i + 1])) for i in range(b.d)])
+    _validate(A, b, x0)
+    x = x0.copy()
+    if x.orth_center != 0:
+        orthonormalize(x, "right")
+    d = x.d
+    eA_L, eA_R, eb_L, eb_R = [None] * d, [None] * d, [None] * d, [None] * d
+    for i in range(d - 1, -1, -1):
+        build_env(eA_R, i, "right", x, x, A)
+        build_env(eb_R, i, "right", x, b)
+    info = Info()
+    b_norm = tt_norm(b) if st.track_residual else None
+    for sweep in range(st.max_sweeps):
+        for k in range(d - 1):
+            build_env(eA_L, k, "left", x, x, A)
+            build_env(eb_L, k, "left", x, b)
+            u, _, _ = _solve_site(k, x, A, b, eA_L, eA_R, eb_L, eb_R, st, info)
+            shift_right(x, k, u.reshape(x.bonds[k], x.row_modes[k], x.bonds[k + 1]))
+        build_env(eA_L, d - 1, "left", x, x, A)
+        build_env(eb_L, d - 1, "left", x, b)
+        for k in range(d - 1, -1, -1):
+            build_env(eA_R, k, "right", x, x, A)
+            build_env(eb_R, k, "right", x, b)
+            u, _, _ = _solve_site(k, x, A, b, eA_L, eA_R, eb_L, eb_R, st, info)
+            if k == 0:
+                x.cores[0] = u.reshape(x.bonds[0], x.row_modes[0], 1, x.bonds[1])
+                x.orth_center = 0
+            else:
+                shift_left(x, k, u.reshape(x.bonds[k], x.row_modes[k], x.bonds[k + 1]))
+        if _record(st, info, A, b, x, b_norm, sweep):
+            break
+    return x, info

@@ -1,0 +1,43 @@
+"""Shared pytest configuration: the ``gpu`` marker and fixture helpers."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libttx.so")
+
+
+def load_golden(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+
+
+def unpack_cores(g):
+    shapes = g["core_shapes"]
+    flat = g["core_data"]
+    out, off = [], 0
+    for s in shapes:
+        n = int(np.prod(s))
+        out.append(flat[off:off + n].reshape(tuple(int(v) for v in s)))
+        off += n
+    return out
+
+
+# The reference's AMEn trajectory depends on BLAS round-off (its own C1 run
+# changes ranks with the OpenBLAS thread count), so the oracle runs with one
+# BLAS thread, the setting of the reference's CI.
+try:
+    from threadpoolctl import threadpool_limits
+
+    _BLAS_LIMIT = threadpool_limits(limits=1, user_api="blas")
+except Exception:  # pragma: no cover
+    _BLAS_LIMIT = None

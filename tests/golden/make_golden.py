@@ -1,0 +1,179 @@
+"""Write the golden fixtures of the solver hot path from the reference library.
+
+Run in a container where the reference checkout is available:
+
+    PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py /path/to/reference
+
+The reference is imported only here, to record input/output vectors of the
+functions its own tests leave unpinned at the workload sizes (the full AMEn
+trajectory, the C2 rounding spectra).  Inputs are rebuilt from the generator
+parameters stored beside the outputs, so each ``.npz`` holds only small
+arrays: bonds, residual histories, singular values, and solution cores where
+they are small.  Nothing here runs at test time; tests read the ``.npz``.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.abspath(os.path.join(HERE, "..", "..")))
+
+from ttnumpy_amd import problems  # noqa: E402
+
+
+def _pack_cores(cores):
+    shapes = np.array([c.shape for c in cores], dtype=np.int64)
+    flat = np.concatenate([np.asarray(c, dtype=np.float64).reshape(-1) for c in cores])
+    return shapes, flat
+
+
+def _info_dict(info):
+    return dict(
+        residuals=np.asarray(info.residuals, dtype=np.float64),
+        max_bonds=np.asarray(info.max_bonds, dtype=np.int64),
+        num_sweeps_run=np.int64(info.num_sweeps_run),
+        stop_reason=np.array(str(info.stop_reason)),
+        local_solves=np.int64(info.local_solves),
+        local_fallbacks=np.int64(info.local_fallbacks),
+        local_inexact=np.int64(info.local_inexact),
+    )
+
+
+class _SvdRecorder:
+    """Wraps the SVD the rounding sweep calls and keeps every spectrum."""
+
+    def __init__(self, svd):
+        self.svd = svd
+        self.spectra = []
+
+    def __call__(self, *args, **kwargs):
+        out = self.svd(*args, **kwargs)
+        self.spectra.append(np.array(out[1]))
+        return out
+
+
+def main(reference_root: str) -> None:
+    sys.path.insert(0, reference_root)
+    import ttnumpy as tt  # noqa: F401
+    from ttnumpy import tensor_train, utils
+    from ttnumpy.solvers.amen import amen
+    from ttnumpy.solvers.als import als
+
+    TT = tensor_train.TensorTrain
+    meta = {}
+
+    # ---- C1: 2D Poisson 256^2, plain FP64 AMEn --------------------------------
+    A = TT(problems.poisson_2d_cores(8))
+    b = TT(problems.ones_cores([2] * 16))
+    x0 = tensor_train.rand([2] * 16, ranks=2, seed=0)
+    t0 = time.perf_counter()
+    x, info = amen(A, b, x0=x0, symmetric=True, residual_tol=1e-8, max_sweeps=20,
+                   seed=0, return_info=True)
+    elapsed = time.perf_counter() - t0
+    shapes, flat = _pack_cores(x.cores)
+    np.savez_compressed(
+        os.path.join(HERE, "c1_poisson256_amen.npz"),
+        bonds=np.asarray(x.bonds, dtype=np.int64), core_shapes=shapes, core_data=flat,
+        orth_center=np.int64(x.orth_center), **_info_dict(info))
+    meta["c1_poisson256_amen"] = dict(
+        n_bits=8, x0="rand([2]*16, ranks=2, seed=0)",
+        call="amen(symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0, return_info=True)",
+        seconds_1core_reference=elapsed, bonds=list(map(int, x.bonds)),
+        sweeps=int(info.num_sweeps_run), stop=str(info.stop_reason),
+        residuals=[float(r) for r in info.residuals])
+    print("C1", elapsed, x.bonds, info, flush=True)
+
+    # ---- C1 at 64^2 (N=6) with ALS, and AMEn on the 1D QTT Laplacian ----------
+    A6 = TT(problems.poisson_2d_cores(6))
+    b6 = TT(problems.ones_cores([2] * 12))
+    x06 = tensor_train.rand([2] * 12, ranks=2, seed=0)
+    x, info = amen(A6, b6, x0=x06, symmetric=True, residual_tol=1e-8, max_sweeps=20,
+                   seed=0, return_info=True)
+    shapes, flat = _pack_cores(x.cores)
+    np.savez_compressed(os.path.join(HERE, "poisson64_amen.npz"),
+                        bonds=np.asarray(x.bonds, dtype=np.int64), core_shapes=shapes,
+                        core_data=flat, **_info_dict(info))
+    meta["poisson64_amen"] = dict(n_bits=6, x0="rand([2]*12, ranks=2, seed=0)",
+                                  call="amen(symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0)")
+
+    # ALS from an orthonormalised rank-4 ones guess on the 1D Laplacian of order 6
+    h = 1.0 / (2**6 + 1)
+    lap = TT(problems.laplacian_1d_cores(6, h))
+    ones6 = TT(problems.ones_cores([2] * 6))
+    x0 = tensor_train.ones([2] * 6, ranks=4).orthonormalize(canonical="right")
+    x, info = als(lap, ones6, x0=x0, max_sweeps=4, residual_tol=1e-10, return_info=True)
+    shapes, flat = _pack_cores(x.cores)
+    np.savez_compressed(os.path.join(HERE, "lap1d6_als.npz"),
+                        bonds=np.asarray(x.bonds, dtype=np.int64), core_shapes=shapes,
+                        core_data=flat, **_info_dict(info))
+    meta["lap1d6_als"] = dict(order=6, x0="ones([2]*6, ranks=4).orthonormalize('right')",
+                              call="als(max_sweeps=4, residual_tol=1e-10, return_info=True)")
+
+    # AMEn forced onto the Krylov path (MINRES) by a small direct_solve_size
+    x0 = tensor_train.rand([2] * 12, ranks=2, seed=0)
+    x, info = amen(A6, b6, x0=x0, symmetric=True, residual_tol=1e-8, max_sweeps=20,
+                   seed=0, direct_solve_size=16, return_info=True)
+    shapes, flat = _pack_cores(x.cores)
+    np.savez_compressed(os.path.join(HERE, "poisson64_amen_minres.npz"),
+                        bonds=np.asarray(x.bonds, dtype=np.int64), core_shapes=shapes,
+                        core_data=flat, **_info_dict(info))
+    meta["poisson64_amen_minres"] = dict(
+        n_bits=6, call="amen(symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0, direct_solve_size=16)")
+
+    # ---- C2: rounding of a sum of two d=40 normal trains --------------------
+    ca = problems.random_train_cores([4] * 40, 64, seed=1, normal=True)
+    cb = problems.random_train_cores([4] * 40, 64, seed=2, normal=True)
+    s = TT([c.copy() for c in ca]).add(TT([c.copy() for c in cb]))
+    rec = _SvdRecorder(utils.svd)
+    utils.svd = rec
+    try:
+        t0 = time.perf_counter()
+        out = s.truncate(error=1e-10, max_rank=96)
+        elapsed = time.perf_counter() - t0
+    finally:
+        utils.svd = rec.svd
+    kept = [sp[: out.bonds[len(out.bonds) - 2 - i]] for i, sp in enumerate(rec.spectra)]
+    np.savez_compressed(
+        os.path.join(HERE, "c2_round_d40.npz"),
+        bonds=np.asarray(out.bonds, dtype=np.int64),
+        spectra_lengths=np.array([len(sp) for sp in rec.spectra], dtype=np.int64),
+        spectra=np.concatenate(rec.spectra), kept_lengths=np.array([len(k) for k in kept]),
+        orth_center=np.int64(out.orth_center))
+    meta["c2_round_d40"] = dict(
+        inputs="cores default_rng(1|2).standard_normal((r,4,1,r')) d=40 rank 64, summed",
+        call="truncate(error=1e-10, max_rank=96)", seconds_1core_reference=elapsed,
+        bonds=list(map(int, out.bonds)),
+        note="spectra[i] is the full spectrum of the i-th SVD of the right-to-left sweep (bond d-1-i)")
+    print("C2", elapsed, out.bonds, flush=True)
+
+    # ---- small rounding cases with the full output cores ----------------------
+    for name, modes, rank, err, mr in (
+        ("round_small_a", [3, 4, 2, 5, 3, 2], 6, 1e-6, None),
+        ("round_small_b", [2] * 8, 7, 0.0, 5),
+        ("round_small_c", [4] * 5, 9, 1e-2, None),
+    ):
+        c1 = problems.random_train_cores(modes, rank, seed=11, normal=True)
+        c2 = problems.random_train_cores(modes, rank, seed=12, normal=True)
+        s = TT([c.copy() for c in c1]).add(TT([c.copy() for c in c2]))
+        out = s.truncate(error=err, max_rank=mr)
+        full = np.asarray(out.get_full_tensor()).reshape(-1)
+        shapes, flat = _pack_cores(out.cores)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"),
+                            bonds=np.asarray(out.bonds, dtype=np.int64), full=full,
+                            core_shapes=shapes, core_data=flat,
+                            orth_center=np.int64(out.orth_center))
+        meta[name] = dict(modes=modes, rank=rank, seeds=[11, 12], normal=True, error=err,
+                          max_rank=mr, call="(x1 + x2).truncate(error, max_rank)")
+
+    with open(os.path.join(HERE, "golden_meta.json"), "w") as fh:
+        json.dump(meta, fh, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

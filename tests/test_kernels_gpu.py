@@ -1,0 +1,192 @@
+"""Kernel-level parity of libttx against NumPy FP64 on the same seeded inputs.
+
+Every case prints its name and shape before it touches the device.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev():
+    from ttnumpy_amd import device as D
+    return D
+
+
+def _t(a):
+    return _dev().to_device(a)
+
+
+def _h(t):
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("M,N,K,batch", [(1, 1, 1, 1), (5, 7, 3, 1), (64, 64, 16, 1), (65, 129, 33, 3),
+                                         (200, 17, 300, 2), (3, 500, 7, 5)])
+@pytest.mark.parametrize("layout", ["nn", "tn", "nt", "tt"])
+def test_gemm_strided(M, N, K, batch, layout):
+    print(f"gemm M={M} N={N} K={K} batch={batch} layout={layout}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(M * 1000 + N * 10 + K)
+    A = rng.standard_normal((batch, M, K)) if layout[0] == "n" else rng.standard_normal((batch, K, M))
+    B = rng.standard_normal((batch, K, N)) if layout[1] == "n" else rng.standard_normal((batch, N, K))
+    C0 = rng.standard_normal((batch, M, N))
+    Ad, Bd, Cd = _t(A), _t(B), _t(C0)
+    a_ms, a_ks = (K, 1) if layout[0] == "n" else (1, M)
+    b_ks, b_ns = (N, 1) if layout[1] == "n" else (1, K)
+    D.gemm(batch, M, N, K, 1.5, D.P(Ad), a_ms, a_ks, M * K, D.P(Bd), b_ks, b_ns, K * N, -0.5,
+           D.P(Cd), N, 1, M * N)
+    Am = A if layout[0] == "n" else A.transpose(0, 2, 1)
+    Bm = B if layout[1] == "n" else B.transpose(0, 2, 1)
+    ref = 1.5 * Am @ Bm - 0.5 * C0
+    np.testing.assert_allclose(_h(Cd), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_gemm_beta_zero_ignores_nan():
+    D = _dev()
+    A = np.ones((4, 3))
+    B = np.ones((3, 5))
+    C = np.full((4, 5), np.nan)
+    Cd = _t(C)
+    D.gemm(1, 4, 5, 3, 1.0, D.P(_t(A)), 3, 1, 0, D.P(_t(B)), 5, 1, 0, 0.0, D.P(Cd), 5, 1, 0)
+    np.testing.assert_array_equal(_h(Cd), np.full((4, 5), 3.0))
+
+
+def test_copy_nd_permute_and_norm():
+    D = _dev()
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((3, 4, 5))
+    Xd = _t(X)
+    Y = D.empty(5, 3, 4)
+    D.copy_nd([3, 4, 5], D.P(Xd), [20, 5, 1], D.P(Y), [4, 1, 12], 2.0)
+    np.testing.assert_array_equal(_h(Y), 2.0 * X.transpose(2, 0, 1))
+    assert float(D.norm(Xd).item()) == pytest.approx(np.linalg.norm(X), rel=1e-14)
+    big = rng.standard_normal(300001)
+    assert float(D.norm(_t(big)).item()) == pytest.approx(np.linalg.norm(big), rel=1e-13)
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (6, 3), (3, 6), (64, 64), (130, 33), (33, 130), (512, 128),
+                                 (257, 200)])
+def test_qr(m, n):
+    print(f"qr {m}x{n}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(m + 7 * n)
+    A = rng.standard_normal((m, n))
+    k = min(m, n)
+    Q = D.empty(m, k)
+    R = D.empty(k, n)
+    D.qr(m, n, D.P(_t(A)), n, 1, D.P(Q), k, 1, D.P(R), n, 1)
+    Qh, Rh = _h(Q), _h(R)
+    np.testing.assert_allclose(Qh @ Rh, A, atol=1e-13 * np.abs(A).max() * max(m, n))
+    np.testing.assert_allclose(Qh.T @ Qh, np.eye(k), atol=1e-13 * max(m, n))
+    assert np.allclose(np.tril(Rh, -1), 0.0)
+    # gauge-invariant: |diag R| equals LAPACK's
+    Rref = np.linalg.qr(A, mode="reduced")[1]
+    np.testing.assert_allclose(np.abs(np.diag(Rh)), np.abs(np.diag(Rref)), rtol=1e-11)
+
+
+def test_qr_rank_deficient_and_transposed_view():
+    D = _dev()
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((40, 3)) @ rng.standard_normal((3, 10))  # rank 3
+    A[:, 4] = 0.0
+    Ad = _t(A)
+    # QR of A^T read through strides (the right-sweep unfolding)
+    Q = D.empty(10, 10)
+    R = D.empty(10, 40)
+    D.qr(10, 40, D.P(Ad), 1, 10, D.P(Q), 10, 1, D.P(R), 40, 1)
+    np.testing.assert_allclose(_h(Q) @ _h(R), A.T, atol=1e-12)
+    np.testing.assert_allclose(_h(Q).T @ _h(Q), np.eye(10), atol=1e-12)
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (1, 5), (5, 1), (8, 8), (66, 33), (33, 66), (128, 384),
+                                 (384, 128), (200, 150), (2, 2)])
+def test_svd(m, n):
+    print(f"svd {m}x{n}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(3 * m + n)
+    A = rng.standard_normal((m, n)) * np.logspace(0, -9, n)[None, :]
+    info = torch.zeros(2, dtype=torch.int32, device=D.device())
+    U, S, Vt = D.svd(m, n, D.P(_t(A)), n, 1, info)
+    assert int(info[0].item()) == 0
+    Uh, Sh, Vh = _h(U), _h(S), _h(Vt)
+    Sref = np.linalg.svd(A, compute_uv=False)
+    np.testing.assert_allclose(Sh, Sref, rtol=1e-11, atol=1e-14 * Sref[0])
+    assert np.all(np.diff(Sh) <= 0)
+    np.testing.assert_allclose((Uh * Sh) @ Vh, A, atol=1e-13 * Sref[0] * max(m, n))
+    k = min(m, n)
+    np.testing.assert_allclose(Uh.T @ Uh, np.eye(k), atol=1e-12 * max(m, n))
+    np.testing.assert_allclose(Vh @ Vh.T, np.eye(k), atol=1e-12 * max(m, n))
+
+
+def test_svd_nonfinite_reports():
+    D = _dev()
+    A = np.ones((4, 3))
+    A[1, 2] = np.nan
+    info = torch.zeros(2, dtype=torch.int32, device=D.device())
+    D.svd(4, 3, D.P(_t(A)), 3, 1, info)
+    assert int(info[0].item()) == 2
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 64, 100, 333, 800, 1000])
+def test_lu_solve(n):
+    print(f"lu n={n}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(n)
+    M = rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)
+    b = rng.standard_normal(n)
+    Mb = np.concatenate([M.T.reshape(-1), b])  # column-major [M | b]
+    u = D.empty(n)
+    info = torch.zeros(1, dtype=torch.int32, device=D.device())
+    D.lu_solve(_t(Mb), n, u, info)
+    ref = np.linalg.solve(M, b)
+    assert int(info.item()) == 0
+    np.testing.assert_allclose(_h(u), ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_lu_needs_pivoting():
+    D = _dev()
+    M = np.array([[0.0, 1.0, 2.0], [1.0, 0.0, 3.0], [4.0, 5.0, 0.0]])
+    b = np.array([1.0, 2.0, 3.0])
+    Mb = np.concatenate([M.T.reshape(-1), b])
+    u = D.empty(3)
+    info = torch.zeros(1, dtype=torch.int32, device=D.device())
+    D.lu_solve(_t(Mb), 3, u, info)
+    np.testing.assert_allclose(_h(u), np.linalg.solve(M, b), rtol=1e-13)
+
+
+def test_minres_matches_scipy():
+    """Iterates and stop iteration of the device MINRES equal SciPy's on an SPD matrix."""
+    import scipy.sparse.linalg as spla
+    from ttnumpy_amd.solvers.common import minres_device
+    D = _dev()
+    rng = np.random.default_rng(0)
+    n = 200
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    M = (Q * np.logspace(0, 3, n)) @ Q.T
+    M = 0.5 * (M + M.T)
+    b = rng.standard_normal(n)
+    x0 = rng.standard_normal(n)
+    op = D.ExplicitOp(_t(M.T.reshape(-1)), n)  # column-major storage
+    for rtol in (1e-4, 1e-8, 1e-10):
+        ref, _ = spla.minres(M, b, x0=x0, rtol=rtol)
+        got = _h(minres_device(op, _t(b), _t(x0), rtol))
+        np.testing.assert_allclose(got, ref, rtol=1e-7, atol=1e-9 * np.abs(ref).max())
+
+
+def test_truncated_bond_kernel_known_answers():
+    from oracle import tt_oracle as O
+    D = _dev()
+    out = torch.zeros(2, dtype=torch.int32, device=D.device())
+    cases = [([4.0, 2.0, 1.0, 0.5], 1.0, None), ([1.0] + [0.099] * 100, 0.1, None),
+             ([1.0, 0.05], 0.1, None), ([4.0, 2.0, 1.0, 0.5], 100.0, None),
+             ([4.0, 2.0, 1.0, 0.5], 0.0, 2), ([1.0, 1e-17, 3e-18], 0.0, None),
+             ([1e8, 1e-9], 0.0, None), ([1e8, 1e-5], 0.0, None), ([1.0, 1e-9], 0.0, None)]
+    for s, delta, mr in cases:
+        S = _t(np.array(s))
+        one = _t(np.array([1.0]))
+        D.truncated_bond_dev(S, one, delta, mr, out, 1)
+        assert int(out[1].item()) == O.truncated_bond(np.array(s), delta, mr), (s, delta, mr)

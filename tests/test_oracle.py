@@ -1,0 +1,232 @@
+"""Pins of the CPU oracle: reference-written golden vectors, known answers, brute force.
+
+Everything here runs on the CPU.  The golden ``.npz`` files were written by
+the reference library itself (``tests/golden/make_golden.py``); the known
+answers are the values the reference's own tests assert
+(``tests/test_utils.py:57-130``, ``tests/test_tensor_train.py:1789-1811``).
+"""
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, unpack_cores
+from oracle import tt_oracle as O
+from ttnumpy_amd import problems as P
+
+
+# ---------------------------------------------------------------- truncated_bond
+SV = np.array([4.0, 2.0, 1.0, 0.5])
+
+
+@pytest.mark.parametrize(
+    "values,delta,max_rank,expected",
+    [
+        (SV, 1.0, None, 3),
+        (np.concatenate([[1.0], np.full(100, 0.099)]), 0.1, None, 100),
+        (np.array([1.0, 0.05]), 0.1, None, 1),
+        (np.array([1.0, 0.5]), 0.1, None, 2),
+        (SV, 100.0, None, 1),
+        (SV, 0.0, 2, 2),
+        (np.array([1.0, 1e-17, 3e-18]), 0.0, None, 1),
+        (np.array([1.0, 0.0, 0.0]), 0.0, None, 1),
+        (np.array([1.0, 5e-17, 0.0]), 0.0, None, 1),
+        (np.array([1.0, 1e-20, 1e-160]), 0.0, None, 1),
+        (np.array([1e8, 1e-9]), 0.0, None, 1),
+        (np.array([1e8, 1e-5]), 0.0, None, 2),
+        (SV, 0.0, None, 4),
+        (np.array([1.0, 1e-9]), 0.0, None, 2),
+        (np.array([1.0, 1e-9], dtype=np.float32), 0.0, None, 1),
+    ],
+)
+def test_truncated_bond_known_answers(values, delta, max_rank, expected):
+    assert O.truncated_bond(values, delta, max_rank) == expected
+
+
+def test_truncated_bond_rejects_bad_max_rank():
+    for mr in (0, -3):
+        with pytest.raises(ValueError):
+            O.truncated_bond(SV, 1.0, mr)
+
+
+# ---------------------------------------------------------------- truncate
+def test_truncate_two_core_closed_form():
+    rng = np.random.default_rng(19)
+    left, _ = np.linalg.qr(rng.standard_normal((30, 6)))
+    right, _ = np.linalg.qr(rng.standard_normal((30, 6)))
+    sigma, mix = 0.5 ** np.arange(6), rng.standard_normal((6, 6))
+    t = O.Train([((left * sigma) @ mix).reshape(1, 30, 1, 6),
+                 np.linalg.solve(mix, right.T).reshape(6, 30, 1, 1)])
+    full = t.full()
+    best = np.linalg.norm(sigma[2:]) / np.linalg.norm(sigma)
+    for canonical in ("auto", "left", "right"):
+        capped = O.truncate(t, max_rank=2, canonical=canonical)
+        err = np.linalg.norm(capped.full() - full) / np.linalg.norm(full)
+        assert err == pytest.approx(best, rel=1e-10)
+        assert O.truncate(t, error=0.2, canonical=canonical).bonds == [1, 3, 1]
+
+
+@pytest.mark.parametrize("name,modes,rank,err,mr", [
+    ("round_small_a", [3, 4, 2, 5, 3, 2], 6, 1e-6, None),
+    ("round_small_b", [2] * 8, 7, 0.0, 5),
+    ("round_small_c", [4] * 5, 9, 1e-2, None),
+])
+def test_truncate_matches_reference_small(name, modes, rank, err, mr):
+    g = load_golden(name)
+    s = O.tt_add(O.Train(P.random_train_cores(modes, rank, 11, normal=True)),
+                 O.Train(P.random_train_cores(modes, rank, 12, normal=True)))
+    out = O.truncate(s, err, mr)
+    assert out.bonds == list(g["bonds"])
+    assert out.orth_center == int(g["orth_center"])
+    np.testing.assert_allclose(out.full(), g["full"], rtol=1e-9, atol=1e-11 * np.abs(g["full"]).max())
+    # right-orthonormal cores after the right-to-left SVD sweep
+    for c in out.cores[1:]:
+        m = c.reshape(c.shape[0], -1)
+        np.testing.assert_allclose(m @ m.T, np.eye(c.shape[0]), atol=1e-12)
+
+
+def test_truncate_c2_spectra_match_reference():
+    g = load_golden("c2_round_d40")
+    s = O.tt_add(O.Train(P.random_train_cores([4] * 40, 64, 1, normal=True)),
+                 O.Train(P.random_train_cores([4] * 40, 64, 2, normal=True)))
+    spectra = []
+    out = O.truncate(s, 1e-10, 96, spectra=spectra)
+    assert out.bonds == list(g["bonds"])
+    lens = g["spectra_lengths"]
+    ref = np.split(g["spectra"], np.cumsum(lens)[:-1])
+    assert len(spectra) == len(ref)
+    for a, r in zip(spectra, ref):
+        np.testing.assert_allclose(a, r, rtol=1e-10)
+
+
+# ---------------------------------------------------------------- environments / local system
+def _tiny_system(seed=0):
+    rng = np.random.default_rng(seed)
+    d, n = 4, 3
+    A = O.Train([rng.standard_normal((r0, n, n, r1)) for r0, r1 in zip([1, 2, 3, 2], [2, 3, 2, 1])])
+    x = O.Train([rng.standard_normal((r0, n, 1, r1)) for r0, r1 in zip([1, 3, 4, 2], [3, 4, 2, 1])])
+    y = O.Train([rng.standard_normal((r0, n, 1, r1)) for r0, r1 in zip([1, 2, 3, 2], [2, 3, 2, 1])])
+    return A, x, y, d, n
+
+
+def _interface(t: O.Train, k: int, side: str) -> np.ndarray:
+    """Dense left interface (prod modes[:k], bond k) or right (bond k+1, prod modes[k+1:])."""
+    if side == "left":
+        m = np.ones((1, 1))
+        for c in t.cores[:k]:
+            m = (m @ c[:, :, 0, :].reshape(c.shape[0], -1)).reshape(-1, c.shape[3])
+        return m
+    m = np.ones((1, 1))
+    for c in reversed(t.cores[k + 1:]):
+        m = (c[:, :, 0, :].reshape(-1, c.shape[3]) @ m).reshape(c.shape[0], -1)
+    return m
+
+
+def test_envs_and_local_operator_brute_force():
+    """Psi . A_k . Phi equals X_{!=k}^T A X_{!=k} from dense interface matrices."""
+    A, x, y, d, n = _tiny_system()
+    Ad = A.dense_matrix()
+    for k in range(d):
+        eL = [None] * d
+        eR = [None] * d
+        for i in range(k + 1):
+            O.build_env(eL, i, "left", x, y, A)
+        for i in range(d - 1, k - 1, -1):
+            O.build_env(eR, i, "right", x, y, A)
+        # frame x (rows), target y (cols): M = Xf^T A Xt restricted to site k
+        XL, XR = _interface(x, k, "left"), _interface(x, k, "right")
+        YL, YR = _interface(y, k, "left"), _interface(y, k, "right")
+        nl, nr = XL.shape[0], XR.shape[1]
+        frame = np.einsum("pa,bq->paq b".replace(" ", ""), XL, XR) if False else None
+        # rows (f_l, i, f_r) -> full index (p, i, q)
+        Pf = np.einsum("pa,ij,bq->aibpjq", XL, np.eye(n), XR).reshape(XL.shape[1] * n * XR.shape[0], -1)
+        Pt = np.einsum("pa,ij,bq->aibpjq", YL, np.eye(n), YR).reshape(YL.shape[1] * n * YR.shape[0], -1)
+        dense = Pf @ Ad @ Pt.T
+        M = O.local_matrix(eL[k], A.cores[k], eR[k])
+        np.testing.assert_allclose(M, dense, rtol=1e-11, atol=1e-11 * np.abs(dense).max())
+        v = np.random.default_rng(k).standard_normal((M.shape[1], 3))
+        np.testing.assert_allclose(O.local_apply(eL[k], A.cores[k], eR[k], v), M @ v,
+                                   rtol=1e-11, atol=1e-11 * np.abs(M).max())
+        # overlap envs and the local rhs: b_loc = Pf @ y_full
+        bl = [None] * d
+        br = [None] * d
+        for i in range(k + 1):
+            O.build_env(bl, i, "left", x, y)
+        for i in range(d - 1, k - 1, -1):
+            O.build_env(br, i, "right", x, y)
+        rhs = O.local_rhs(bl[k], y.cores[k][:, :, 0, :], br[k])
+        Py = np.einsum("pa,ij,bq->pjq", YL, np.eye(n), YR) if False else None
+        yfull = y.full()
+        np.testing.assert_allclose(rhs, Pf @ yfull, rtol=1e-11, atol=1e-11 * np.abs(yfull).max())
+
+
+def test_relative_residual_dense():
+    A, x, y, d, n = _tiny_system(3)
+    Ad = A.dense_matrix()
+    r = np.linalg.norm(Ad @ x.full() - y.full()) / np.linalg.norm(y.full())
+    assert O.relative_residual(A, y, x) == pytest.approx(r, rel=1e-10)
+
+
+# ---------------------------------------------------------------- solver trajectories
+def _check_traj(x, info, g, rtol_res=1e-5):
+    assert x.bonds == list(g["bonds"])
+    assert info.num_sweeps_run == int(g["num_sweeps_run"])
+    assert info.stop_reason == str(g["stop_reason"])
+    assert info.max_bonds == list(g["max_bonds"])
+    np.testing.assert_allclose(info.residuals, g["residuals"], rtol=rtol_res)
+    ref = O.Train(unpack_cores(g))
+    diff = O.tt_norm(O.tt_add(x, O.tt_scale(ref, -1.0))) / O.tt_norm(ref)
+    assert diff < 1e-10
+
+
+def test_amen_poisson64_matches_reference():
+    g = load_golden("poisson64_amen")
+    A = O.Train(P.poisson_2d_cores(6))
+    b = O.Train(P.ones_cores([2] * 12))
+    x0 = O.Train(P.random_train_cores([2] * 12, 2, 0))
+    x, info = O.amen(A, b, x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0)
+    _check_traj(x, info, g)
+
+
+def test_amen_poisson64_minres_matches_reference():
+    g = load_golden("poisson64_amen_minres")
+    A = O.Train(P.poisson_2d_cores(6))
+    b = O.Train(P.ones_cores([2] * 12))
+    x0 = O.Train(P.random_train_cores([2] * 12, 2, 0))
+    x, info = O.amen(A, b, x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0,
+                     direct_solve_size=16)
+    _check_traj(x, info, g)
+    assert info.local_solves == int(g["local_solves"])
+
+
+def test_als_lap1d_matches_reference():
+    g = load_golden("lap1d6_als")
+    h = 1.0 / (2 ** 6 + 1)
+    A = O.Train(P.laplacian_1d_cores(6, h))
+    b = O.Train(P.ones_cores([2] * 6))
+    x0 = O.orthonormalize(O.Train(P.ones_cores([2] * 6, 4)), "right")
+    x, info = O.als(A, b, x0, max_sweeps=4, residual_tol=1e-10)
+    _check_traj(x, info, g, rtol_res=1e-3)
+    # dense pin: the solve reaches the exact discrete solution
+    xd = np.linalg.solve(A.dense_matrix(), b.full())
+    assert np.linalg.norm(x.full() - xd) / np.linalg.norm(xd) < 1e-8
+
+
+def test_amen_c1_poisson256_matches_reference():
+    g = load_golden("c1_poisson256_amen")
+    A = O.Train(P.poisson_2d_cores(8))
+    b = O.Train(P.ones_cores([2] * 16))
+    x0 = O.Train(P.random_train_cores([2] * 16, 2, 0))
+    x, info = O.amen(A, b, x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0)
+    _check_traj(x, info, g)
+
+
+def test_poisson_operator_is_the_kronecker_sum():
+    """The QTT builder reproduces the dense 5-point Laplacian (brute force, 8x8 grid)."""
+    nb = 3
+    A = O.Train(P.poisson_2d_cores(nb)).dense_matrix()
+    m = 2 ** nb
+    h = 1.0 / (m + 1)
+    L = (2 * np.eye(m) - np.eye(m, k=1) - np.eye(m, k=-1)) / h ** 2
+    dense = np.kron(L, np.eye(m)) + np.kron(np.eye(m), L)
+    np.testing.assert_allclose(A, dense, rtol=1e-12, atol=1e-9)
+    assert O.Train(P.poisson_2d_cores(nb)).bonds == [1, 4, 4, 2, 4, 4, 1]

@@ -1,0 +1,203 @@
+"""Parity of the device hot path with the oracle and the reference-written goldens.
+
+Contractions (environments, local operator, local rhs, residual core) are
+compared with the oracle on seeded random trains; ``truncate`` with the
+reference's golden bonds and singular values; AMEn/ALS with the reference's
+golden trajectories.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, unpack_cores
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import tt_oracle as O  # noqa: E402
+from ttnumpy_amd import problems as P  # noqa: E402
+
+
+def _tt(cores):
+    import ttnumpy_amd as tx
+    return tx.TensorTrain([np.asarray(c) for c in cores])
+
+
+def _h(t):
+    return t.cpu().numpy()
+
+
+def _rand_system(seed):
+    rng = np.random.default_rng(seed)
+    n = 3
+    A = [rng.standard_normal((r0, n, n, r1)) for r0, r1 in zip([1, 2, 3, 2], [2, 3, 2, 1])]
+    x = [rng.standard_normal((r0, n, 1, r1)) for r0, r1 in zip([1, 3, 4, 2], [3, 4, 2, 1])]
+    y = [rng.standard_normal((r0, n, 1, r1)) for r0, r1 in zip([1, 2, 3, 2], [2, 3, 2, 1])]
+    return A, x, y
+
+
+def test_environments_and_local_ops_match_oracle():
+    from ttnumpy_amd import device as D
+    from ttnumpy_amd.solvers import common as C
+    A, x, y = _rand_system(0)
+    d = 4
+    Ad, xd, yd = _tt(A), _tt(x), _tt(y)
+    Ao, xo, yo = O.Train(A), O.Train(x), O.Train(y)
+    for k in range(d):
+        print(f"envs site {k}", flush=True)
+        eL, eR, oL, oR = [None] * d, [None] * d, [None] * d, [None] * d
+        bL, bR, obL, obR = [None] * d, [None] * d, [None] * d, [None] * d
+        for i in range(k + 1):
+            C.prepare_left_environment(i, eL, xd, yd, Ad)
+            O.build_env(oL, i, "left", xo, yo, Ao)
+            C.prepare_left_environment(i, bL, xd, yd)
+            O.build_env(obL, i, "left", xo, yo)
+        for i in range(d - 1, k - 1, -1):
+            C.prepare_right_environment(i, eR, xd, yd, Ad)
+            O.build_env(oR, i, "right", xo, yo, Ao)
+            C.prepare_right_environment(i, bR, xd, yd)
+            O.build_env(obR, i, "right", xo, yo)
+        for a, b in ((eL[k], oL[k]), (eR[k], oR[k]), (bL[k], obL[k]), (bR[k], obR[k])):
+            np.testing.assert_allclose(_h(a), b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+        op = D.LocalOp(eL[k], Ad.cores[k], eR[k])
+        Mo = O.local_matrix(oL[k], Ao.cores[k], oR[k])
+        M = D.empty(op.rows * op.cols)
+        op.explicit(M, op.rows)
+        np.testing.assert_allclose(_h(M).reshape(op.cols, op.rows).T, Mo, rtol=1e-12,
+                                   atol=1e-12 * np.abs(Mo).max())
+        v = np.random.default_rng(k).standard_normal((3, op.cols))
+        out = D.empty(3 * op.rows)
+        op.apply(D.P(D.to_device(v)), 3, D.P(out))
+        np.testing.assert_allclose(_h(out).reshape(3, op.rows), (Mo @ v.T).T, rtol=1e-11,
+                                   atol=1e-12 * np.abs(Mo).max() * np.abs(v).max())
+        rhs = D.empty(op.rows)
+        D.local_rhs(bL[k], yd.cores[k], bR[k], D.P(rhs))
+        np.testing.assert_allclose(_h(rhs), O.local_rhs(obL[k], yo.cores[k][:, :, 0, :], obR[k]),
+                                   rtol=1e-12, atol=1e-12)
+
+
+def test_relative_residual_matches_oracle():
+    from ttnumpy_amd.solvers.common import relative_residual
+    A, x, y = _rand_system(3)
+    got = relative_residual(_tt(A), _tt(y), _tt(x))
+    assert got == pytest.approx(O.relative_residual(O.Train(A), O.Train(y), O.Train(x)), rel=1e-12)
+
+
+@pytest.mark.parametrize("name,modes,rank,err,mr", [
+    ("round_small_a", [3, 4, 2, 5, 3, 2], 6, 1e-6, None),
+    ("round_small_b", [2] * 8, 7, 0.0, 5),
+    ("round_small_c", [4] * 5, 9, 1e-2, None),
+])
+def test_truncate_small_matches_reference(name, modes, rank, err, mr):
+    print(f"truncate {name}", flush=True)
+    g = load_golden(name)
+    s = _tt(P.random_train_cores(modes, rank, 11, normal=True)).add(
+        _tt(P.random_train_cores(modes, rank, 12, normal=True)))
+    out = s.truncate(error=err, max_rank=mr)
+    assert out.bonds == list(g["bonds"])
+    assert out.orth_center == int(g["orth_center"])
+    full = out.get_full_tensor().reshape(-1)
+    np.testing.assert_allclose(full, g["full"], rtol=1e-9, atol=1e-11 * np.abs(g["full"]).max())
+
+
+def test_truncate_two_core_closed_form():
+    rng = np.random.default_rng(19)
+    left, _ = np.linalg.qr(rng.standard_normal((30, 6)))
+    right, _ = np.linalg.qr(rng.standard_normal((30, 6)))
+    sigma, mix = 0.5 ** np.arange(6), rng.standard_normal((6, 6))
+    cores = [((left * sigma) @ mix).reshape(1, 30, 1, 6), np.linalg.solve(mix, right.T).reshape(6, 30, 1, 1)]
+    full = O.Train(cores).full()
+    best = np.linalg.norm(sigma[2:]) / np.linalg.norm(sigma)
+    for canonical in ("auto", "left", "right"):
+        capped = _tt(cores).truncate(max_rank=2, canonical=canonical)
+        err = np.linalg.norm(capped.get_full_tensor().reshape(-1) - full) / np.linalg.norm(full)
+        assert err == pytest.approx(best, rel=1e-10)
+        assert _tt(cores).truncate(error=0.2, canonical=canonical).bonds == [1, 3, 1]
+
+
+def test_truncate_c2_matches_reference():
+    """d=40 rounding: identical bonds, every kept singular value within 1e-10."""
+    print("truncate C2 d=40", flush=True)
+    g = load_golden("c2_round_d40")
+    s = _tt(P.random_train_cores([4] * 40, 64, 1, normal=True)).add(
+        _tt(P.random_train_cores([4] * 40, 64, 2, normal=True)))
+    spectra = []
+    out = s.truncate(error=1e-10, max_rank=96, spectra=spectra)
+    assert out.bonds == list(g["bonds"])
+    ref = np.split(g["spectra"], np.cumsum(g["spectra_lengths"])[:-1])
+    assert len(spectra) == len(ref)
+    for i, (a, r) in enumerate(zip(spectra, ref)):
+        a = _h(a)
+        kept = out.bonds[len(out.bonds) - 2 - i]
+        np.testing.assert_allclose(a[:kept], r[:kept], rtol=1e-10)
+
+
+def _amen_case(n_bits, **kw):
+    import ttnumpy_amd as tx
+    d = 2 * n_bits
+    A = _tt(P.poisson_2d_cores(n_bits))
+    b = _tt(P.ones_cores([2] * d))
+    x0 = tx.rand([2] * d, ranks=2, seed=0)
+    return tx.amen(A, b, x0=x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0,
+                   return_info=True, **kw)
+
+
+def _oracle_solution(g):
+    return O.Train(unpack_cores(g))
+
+
+def _rel_diff(x, ref: O.Train):
+    xo = O.Train(x.to_numpy())
+    return O.tt_norm(O.tt_add(xo, O.tt_scale(ref, -1.0))) / O.tt_norm(ref)
+
+
+def test_amen_poisson64_direct_matches_reference():
+    print("amen poisson 64x64 direct", flush=True)
+    g = load_golden("poisson64_amen")
+    x, info = _amen_case(6)
+    assert str(info.stop_reason) == "converged"
+    assert info.residuals[-1] <= 1e-8
+    assert info.num_sweeps_run == int(g["num_sweeps_run"])
+    assert x.bonds == list(g["bonds"])
+    assert info.max_bonds == list(g["max_bonds"])
+    assert _rel_diff(x, _oracle_solution(g)) < 1e-7
+
+
+def test_amen_poisson64_minres_path():
+    print("amen poisson 64x64 minres", flush=True)
+    g = load_golden("poisson64_amen_minres")
+    x, info = _amen_case(6, direct_solve_size=16)
+    assert str(info.stop_reason) == "converged"
+    assert info.residuals[-1] <= 1e-8
+    assert info.local_solves > 0
+    assert _rel_diff(x, _oracle_solution(g)) < 1e-6
+
+
+def test_als_lap1d_matches_reference():
+    import ttnumpy_amd as tx
+    g = load_golden("lap1d6_als")
+    h = 1.0 / (2 ** 6 + 1)
+    A = _tt(P.laplacian_1d_cores(6, h))
+    b = _tt(P.ones_cores([2] * 6))
+    x0 = tx.ones([2] * 6, ranks=4).orthonormalize(canonical="right")
+    x, info = tx.als(A, b, x0=x0, max_sweeps=4, residual_tol=1e-10, return_info=True)
+    assert x.bonds == list(g["bonds"])
+    assert info.num_sweeps_run == int(g["num_sweeps_run"])
+    assert str(info.stop_reason) == str(g["stop_reason"])
+    assert _rel_diff(x, _oracle_solution(g)) < 1e-9
+
+
+def test_amen_c1_poisson256_converges_like_reference():
+    """C1: converged at 1e-8, solution within the residual-implied distance of the reference's.
+
+    The reference's own trajectory on C1 changes with BLAS round-off (bond 32
+    vs 33, 6 sweeps, even 'stagnated' with 8 OpenBLAS threads), so ranks are
+    compared within that band rather than exactly.
+    """
+    print("amen C1 poisson 256x256", flush=True)
+    g = load_golden("c1_poisson256_amen")
+    x, info = _amen_case(8)
+    assert info.residuals[-1] <= 1e-8 * 1.05
+    assert abs(max(x.bonds) - max(g["bonds"])) <= 2
+    assert _rel_diff(x, _oracle_solution(g)) < 1e-5

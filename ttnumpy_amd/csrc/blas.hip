@@ -1,0 +1,237 @@
+// Dense building blocks: strided batched FP64 GEMM on MFMA, strided N-d copy,
+// fills, axpby and deterministic norms.
+//
+// The GEMM is the one contraction engine of the hot path: every tensordot of
+// the environment recursion, the one-site local apply, the residual core and
+// the core absorbs is issued as a strided batched GEMM, so no permuted copy of
+// a core or environment is ever materialised.
+
+#include "ttx_common.h"
+
+namespace ttx {
+
+struct GemmArgs {
+  int64_t M, N, K;
+  double alpha, beta;
+  const double* A;
+  int64_t a_ms, a_ks, a_bs;
+  const double* B;
+  int64_t b_ks, b_ns, b_bs;
+  double* C;
+  int64_t c_ms, c_ns, c_bs;
+};
+
+// 64x64 output tile per 256-thread block; each of the 4 waves owns a 32x32
+// quadrant = 2x2 MFMA 16x16 blocks.  K is staged 16 at a time through LDS,
+// k-major so that the MFMA operand reads are consecutive across lanes.
+// Operand layouts of v_mfma_f64_16x16x4f64 (wave64):
+//   A (16x4): lane l holds A[l % 16][l / 16]
+//   B (4x16): lane l holds B[l / 16][l % 16]
+//   D (16x16): lane l, element i holds D[4 * (l / 16) + i][l % 16]
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs p) {
+  __shared__ double As[GBK][GBM + 1];
+  __shared__ double Bs[GBK][GBN + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int64_t b = blockIdx.z;
+  const int64_t m0 = (int64_t)blockIdx.y * GBM;
+  const int64_t n0 = (int64_t)blockIdx.x * GBN;
+  const double* A = p.A + b * p.a_bs;
+  const double* B = p.B + b * p.b_bs;
+  double* C = p.C + b * p.c_bs;
+  // pick the fastest-varying tile index so that global reads coalesce
+  const bool a_m_fast = p.a_ms <= p.a_ks;
+  const bool b_n_fast = p.b_ns <= p.b_ks;
+  const int wm = (wave >> 1) * 32;
+  const int wn = (wave & 1) * 32;
+  double4_t acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+
+  for (int64_t k0 = 0; k0 < p.K; k0 += GBK) {
+#pragma unroll
+    for (int e0 = 0; e0 < GBM * GBK; e0 += 256) {
+      const int e = e0 + tid;
+      int mm, kk;
+      if (a_m_fast) { mm = e % GBM; kk = e / GBM; } else { kk = e % GBK; mm = e / GBK; }
+      const int64_t gm = m0 + mm, gk = k0 + kk;
+      As[kk][mm] = (gm < p.M && gk < p.K) ? A[gm * p.a_ms + gk * p.a_ks] : 0.0;
+    }
+#pragma unroll
+    for (int e0 = 0; e0 < GBN * GBK; e0 += 256) {
+      const int e = e0 + tid;
+      int nn, kk;
+      if (b_n_fast) { nn = e % GBN; kk = e / GBN; } else { kk = e % GBK; nn = e / GBK; }
+      const int64_t gn = n0 + nn, gk = k0 + kk;
+      Bs[kk][nn] = (gn < p.N && gk < p.K) ? B[gk * p.b_ks + gn * p.b_ns] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GBK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      const int c = lane & 15;
+      const double a0 = As[kr][wm + c];
+      const double a1 = As[kr][wm + 16 + c];
+      const double b0 = Bs[kr][wn + c];
+      const double b1 = Bs[kr][wn + 16 + c];
+      acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
+      acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
+      acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+      acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  const int col = lane & 15;
+  const int rq = 4 * (lane >> 4);
+  double4_t* accs[4] = {&acc00, &acc01, &acc10, &acc11};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t gn = n0 + wn + 16 * (q & 1) + col;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t gm = m0 + wm + 16 * (q >> 1) + rq + i;
+      if (gm < p.M && gn < p.N) {
+        double* dst = C + gm * p.c_ms + gn * p.c_ns;
+        const double v = p.alpha * (*accs[q])[i];
+        *dst = (p.beta == 0.0) ? v : v + p.beta * (*dst);
+      }
+    }
+  }
+}
+
+struct CopyArgs {
+  int ndim;
+  int64_t dims[6];
+  int64_t ss[6];
+  int64_t ds[6];
+  int64_t total;
+  double alpha;
+  const double* src;
+  double* dst;
+};
+
+__global__ __launch_bounds__(256) void copy_nd_kernel(CopyArgs a) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.total; i += stride) {
+    int64_t rem = i, so = 0, dof = 0;
+    for (int d = a.ndim - 1; d >= 0; --d) {
+      const int64_t c = rem % a.dims[d];
+      rem /= a.dims[d];
+      so += c * a.ss[d];
+      dof += c * a.ds[d];
+    }
+    a.dst[dof] = (a.alpha == 0.0) ? 0.0 : a.alpha * a.src[so];
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(double* dst, int64_t n, double v) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = v;
+}
+
+__global__ __launch_bounds__(256) void axpby_kernel(int64_t n, double a, const double* x, double b,
+                                                    const double* y, double* z) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    z[i] = a * x[i] + b * y[i];
+}
+
+// Deterministic sum of squares in one 1024-thread block (fixed reduction order).
+__global__ __launch_bounds__(1024) void sumsq_single_kernel(int64_t n, const double* a,
+                                                           const double* b, double* out) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = b ? a[i] - b[i] : a[i];
+    s += v * v;
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[0] = sqrt(s);
+}
+
+static int norm_impl(hipStream_t st, int64_t n, const double* a, const double* b, double* out) {
+  if (n < 0) return TTX_EINVAL;
+  hipLaunchKernelGGL(sumsq_single_kernel, dim3(1), dim3(1024), 0, st, n, a, b, out);
+  return launch_status();
+}
+
+static inline unsigned grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g < 1) g = 1;
+  if (g > 8192) g = 8192;
+  return (unsigned)g;
+}
+
+}  // namespace ttx
+
+using namespace ttx;
+
+extern "C" int ttx_version(void) { return 100; }
+
+extern "C" int ttx_gemm_f64(ttx_stream_t stream, int64_t batch, int64_t M, int64_t N, int64_t K,
+                            double alpha, const double* A, int64_t a_ms, int64_t a_ks,
+                            int64_t a_bs, const double* B, int64_t b_ks, int64_t b_ns,
+                            int64_t b_bs, double beta, double* C, int64_t c_ms, int64_t c_ns,
+                            int64_t c_bs) {
+  if (batch < 0 || M < 0 || N < 0 || K < 0) return TTX_EINVAL;
+  if (batch == 0 || M == 0 || N == 0) return TTX_OK;
+  if (batch > 65535 || (M + GBM - 1) / GBM > 65535) return TTX_EINVAL;
+  if (K == 0) {
+    // empty contraction: C = beta * C (beta == 0 writes zeros without reading C)
+    int64_t dims[3] = {batch, M, N};
+    int64_t ds[3] = {c_bs, c_ms, c_ns};
+    return ttx_copy_f64(stream, 3, dims, C, ds, C, ds, beta);
+  }
+  GemmArgs p{M, N, K, alpha, beta, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, C, c_ms, c_ns, c_bs};
+  dim3 grid((unsigned)((N + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM), (unsigned)batch);
+  hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, (hipStream_t)stream, p);
+  return launch_status();
+}
+
+extern "C" int ttx_copy_f64(ttx_stream_t stream, int ndim, const int64_t* dims, const double* src,
+                            const int64_t* src_strides, double* dst, const int64_t* dst_strides,
+                            double alpha) {
+  if (ndim < 1 || ndim > 6) return TTX_EINVAL;
+  CopyArgs a;
+  a.ndim = ndim;
+  a.total = 1;
+  for (int d = 0; d < 6; ++d) {
+    a.dims[d] = d < ndim ? dims[d] : 1;
+    a.ss[d] = d < ndim ? src_strides[d] : 0;
+    a.ds[d] = d < ndim ? dst_strides[d] : 0;
+    if (a.dims[d] < 0) return TTX_EINVAL;
+    a.total *= a.dims[d];
+  }
+  if (a.total == 0) return TTX_OK;
+  a.alpha = alpha;
+  a.src = src;
+  a.dst = dst;
+  hipLaunchKernelGGL(copy_nd_kernel, dim3(grid_for(a.total)), dim3(256), 0, (hipStream_t)stream, a);
+  return launch_status();
+}
+
+extern "C" int ttx_fill_f64(ttx_stream_t stream, double* dst, int64_t n, double value) {
+  if (n < 0) return TTX_EINVAL;
+  if (n == 0) return TTX_OK;
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, dst, n, value);
+  return launch_status();
+}
+
+extern "C" int ttx_axpby_f64(ttx_stream_t stream, int64_t n, double a, const double* x, double b,
+                             const double* y, double* z) {
+  if (n < 0) return TTX_EINVAL;
+  if (n == 0) return TTX_OK;
+  hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, a, x, b, y, z);
+  return launch_status();
+}
+
+extern "C" int ttx_nrm2_f64(ttx_stream_t stream, int64_t n, const double* x, double* out) {
+  return norm_impl((hipStream_t)stream, n, x, nullptr, out);
+}
+
+extern "C" int ttx_diff_nrm2_f64(ttx_stream_t stream, int64_t n, const double* a, const double* b,
+                                 double* out) {
+  return norm_impl((hipStream_t)stream, n, a, b, out);
+}

@@ -1,0 +1,234 @@
+// Device-resident MINRES for the matrix-free local solve.
+//
+// The recurrences, the rotation updates and the stopping tests restate
+// scipy.sparse.linalg.minres (SciPy 1.15, _isolve/minres.py, no
+// preconditioner, shift 0), which the reference calls at
+// ttnumpy/solvers/common.py:784.  The operator is applied between steps by
+// the caller (the MFMA GEMM chain of the one-site apply); all scalar state
+// stays on the device, so the host only polls `istop` every few iterations.
+//
+// vec layout (n doubles each): x, v, y, r1, r2, w, w1, w2.
+
+#include <float.h>
+
+#include "ttx_common.h"
+
+namespace ttx {
+
+enum {
+  S_ISTOP = 0, S_ITN, S_BETA1, S_BETA, S_OLDB, S_DBAR, S_EPSLN, S_PHIBAR, S_RHS1, S_RHS2,
+  S_TNORM2, S_GMAX, S_GMIN, S_CS, S_SN, S_RTOL, S_MAXIT, S_ANORM, S_YNORM, S_ACOND, S_RNORM,
+  S_BNORM
+};
+
+// istop value for the early returns of scipy (beta1 == 0 or ||b|| == 0)
+constexpr double kIstopEarly = 100.0;
+
+__global__ __launch_bounds__(1024) void minres_init_kernel(int64_t n, const double* b,
+                                                          const double* x0, const double* ax0,
+                                                          double rtol, int64_t maxiter,
+                                                          double* vec, double* st) {
+  __shared__ double red[16];
+  double* x = vec;
+  double* v = vec + n;
+  double* r1 = vec + 3 * n;
+  double* r2 = vec + 4 * n;
+  double* w = vec + 5 * n;
+  double* w1 = vec + 6 * n;
+  double* w2 = vec + 7 * n;
+  const int tid = threadIdx.x;
+  double sb = 0.0, sr = 0.0;
+  for (int64_t i = tid; i < n; i += blockDim.x) {
+    const double xi = x0 ? x0[i] : 0.0;
+    const double ri = ax0 ? b[i] - ax0[i] : b[i];
+    x[i] = xi;
+    r1[i] = ri;
+    r2[i] = ri;
+    w[i] = 0.0;
+    w1[i] = 0.0;
+    w2[i] = 0.0;
+    sb += b[i] * b[i];
+    sr += ri * ri;
+  }
+  const double beta1sq = block_sum(sr, red);
+  const double bnorm = sqrt(block_sum(sb, red));
+  double istop = 0.0;
+  if (beta1sq == 0.0) {
+    istop = kIstopEarly;  // x stays x0
+  } else if (bnorm == 0.0) {
+    istop = kIstopEarly;  // x = b
+    for (int64_t i = tid; i < n; i += blockDim.x) x[i] = b[i];
+  }
+  const double beta1 = sqrt(beta1sq);
+  if (istop == 0.0) {
+    const double s = 1.0 / beta1;
+    for (int64_t i = tid; i < n; i += blockDim.x) v[i] = s * r1[i];
+  }
+  if (tid == 0) {
+    for (int i = 0; i < TTX_MINRES_STATE; ++i) st[i] = 0.0;
+    st[S_ISTOP] = istop;
+    st[S_ITN] = 0.0;
+    st[S_BETA1] = beta1;
+    st[S_BETA] = beta1;
+    st[S_OLDB] = 0.0;
+    st[S_DBAR] = 0.0;
+    st[S_EPSLN] = 0.0;
+    st[S_PHIBAR] = beta1;
+    st[S_RHS1] = beta1;
+    st[S_RHS2] = 0.0;
+    st[S_TNORM2] = 0.0;
+    st[S_GMAX] = 0.0;
+    st[S_GMIN] = DBL_MAX;
+    st[S_CS] = -1.0;
+    st[S_SN] = 0.0;
+    st[S_RTOL] = rtol;
+    st[S_MAXIT] = (double)maxiter;
+    st[S_BNORM] = bnorm;
+  }
+}
+
+__global__ __launch_bounds__(1024) void minres_step_kernel(int64_t n, const double* y_in,
+                                                          double* vec, double* st) {
+  __shared__ double red[16];
+  __shared__ double sc[8];
+  if (st[S_ISTOP] != 0.0) return;  // uniform across the block
+  const int tid = threadIdx.x;
+  double* x = vec;
+  double* v = vec + n;
+  double* y = vec + 2 * n;
+  double* r1 = vec + 3 * n;
+  double* r2 = vec + 4 * n;
+  double* w = vec + 5 * n;
+  double* w1 = vec + 6 * n;
+  double* w2 = vec + 7 * n;
+  const double eps = DBL_EPSILON;
+  const double itn = st[S_ITN] + 1.0;
+  const double beta = st[S_BETA];
+  const double oldb0 = st[S_OLDB];
+  // y = A v - (beta/oldb) r1 ; alfa = <v, y>
+  const double f1 = itn >= 2.0 ? beta / oldb0 : 0.0;
+  double s = 0.0;
+  for (int64_t i = tid; i < n; i += blockDim.x) {
+    double yi = y_in[i];
+    if (itn >= 2.0) yi = yi - f1 * r1[i];
+    y[i] = yi;
+    s += v[i] * yi;
+  }
+  const double alfa = block_sum(s, red);
+  // y -= (alfa/beta) r2 ; r1 = r2 ; r2 = y ; beta' = ||r2||
+  const double f2 = alfa / beta;
+  s = 0.0;
+  for (int64_t i = tid; i < n; i += blockDim.x) {
+    const double yi = y[i] - f2 * r2[i];
+    r1[i] = r2[i];
+    r2[i] = yi;
+    s += yi * yi;
+  }
+  const double betasq = block_sum(s, red);
+  const double oldb = beta;
+  const double nbeta = sqrt(betasq);
+  // scalar recurrence (identical on every thread)
+  double istop = st[S_ISTOP];
+  const double beta1 = st[S_BETA1];
+  double tnorm2 = st[S_TNORM2] + alfa * alfa + oldb * oldb + nbeta * nbeta;
+  if (itn == 1.0 && nbeta / beta1 <= 10.0 * eps) istop = -1.0;
+  const double cs0 = st[S_CS], sn0 = st[S_SN], dbar0 = st[S_DBAR];
+  const double oldeps = st[S_EPSLN];
+  const double delta = cs0 * dbar0 + sn0 * alfa;
+  const double gbar = sn0 * dbar0 - cs0 * alfa;
+  const double epsln = sn0 * nbeta;
+  const double dbar = -cs0 * nbeta;
+  const double root = sqrt(gbar * gbar + dbar * dbar);
+  double gamma = sqrt(gbar * gbar + nbeta * nbeta);
+  gamma = gamma > eps ? gamma : eps;
+  const double cs = gbar / gamma;
+  const double sn = nbeta / gamma;
+  const double phibar0 = st[S_PHIBAR];
+  const double phi = cs * phibar0;
+  const double phibar = sn * phibar0;
+  const double denom = 1.0 / gamma;
+  // w1 = w2 ; w2 = w ; w = (v - oldeps w1 - delta w2) / gamma ; x += phi w
+  s = 0.0;
+  for (int64_t i = tid; i < n; i += blockDim.x) {
+    const double ow2 = w2[i];
+    const double ow = w[i];
+    const double nw = (v[i] - oldeps * ow2 - delta * ow) * denom;
+    w1[i] = ow2;
+    w2[i] = ow;
+    w[i] = nw;
+    const double xi = x[i] + phi * nw;
+    x[i] = xi;
+    s += xi * xi;
+  }
+  const double ynorm = sqrt(block_sum(s, red));
+  const double gmax = fmax(st[S_GMAX], gamma);
+  const double gmin = fmin(st[S_GMIN], gamma);
+  const double z = st[S_RHS1] / gamma;
+  const double rhs1 = st[S_RHS2] - delta * z;
+  const double rhs2 = -epsln * z;
+  const double Anorm = sqrt(tnorm2);
+  const double epsx = Anorm * ynorm * eps;
+  const double rnorm = phibar;
+  const double test1 = (ynorm == 0.0 || Anorm == 0.0) ? INFINITY : rnorm / (Anorm * ynorm);
+  const double test2 = (Anorm == 0.0) ? INFINITY : root / Anorm;
+  const double Acond = gmax / gmin;
+  const double rtol = st[S_RTOL];
+  if (istop == 0.0) {
+    const double t1 = 1.0 + test1;
+    const double t2 = 1.0 + test2;
+    if (t2 <= 1.0) istop = 2.0;
+    if (t1 <= 1.0) istop = 1.0;
+    if (itn >= st[S_MAXIT]) istop = 6.0;
+    if (Acond >= 0.1 / eps) istop = 4.0;
+    if (epsx >= beta1) istop = 3.0;
+    if (test2 <= rtol) istop = 2.0;
+    if (test1 <= rtol) istop = 1.0;
+  }
+  if (istop == 0.0) {
+    const double sv = 1.0 / nbeta;
+    for (int64_t i = tid; i < n; i += blockDim.x) v[i] = sv * r2[i];
+  }
+  __syncthreads();  // every thread has read st[] before thread 0 rewrites it
+  if (tid == 0) {
+    st[S_ISTOP] = istop;
+    st[S_ITN] = itn;
+    st[S_BETA] = nbeta;
+    st[S_OLDB] = oldb;
+    st[S_DBAR] = dbar;
+    st[S_EPSLN] = epsln;
+    st[S_PHIBAR] = phibar;
+    st[S_RHS1] = rhs1;
+    st[S_RHS2] = rhs2;
+    st[S_TNORM2] = tnorm2;
+    st[S_GMAX] = gmax;
+    st[S_GMIN] = gmin;
+    st[S_CS] = cs;
+    st[S_SN] = sn;
+    st[S_ANORM] = Anorm;
+    st[S_YNORM] = ynorm;
+    st[S_ACOND] = Acond;
+    st[S_RNORM] = rnorm;
+  }
+  (void)sc;
+}
+
+}  // namespace ttx
+
+using namespace ttx;
+
+extern "C" int ttx_minres_init_f64(ttx_stream_t stream, int64_t n, const double* b,
+                                   const double* x0, const double* ax0, double rtol,
+                                   int64_t maxiter, double* vec, double* st) {
+  if (n < 1) return TTX_EINVAL;
+  hipLaunchKernelGGL(minres_init_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, n, b, x0,
+                     ax0, rtol, maxiter, vec, st);
+  return launch_status();
+}
+
+extern "C" int ttx_minres_step_f64(ttx_stream_t stream, int64_t n, const double* y_in, double* vec,
+                                   double* st) {
+  if (n < 1) return TTX_EINVAL;
+  hipLaunchKernelGGL(minres_step_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, n, y_in,
+                     vec, st);
+  return launch_status();
+}

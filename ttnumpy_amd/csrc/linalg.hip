@@ -1,0 +1,626 @@
+// Dense FP64 factorizations of core unfoldings and local systems:
+//   * Householder QR (economic), one workgroup per matrix;
+//   * thin SVD: QR of the tall side, then one-sided (Hestenes) Jacobi on the
+//     square factor with a round-robin pair schedule, one workgroup;
+//   * LU with partial pivoting for the dense local system: a one-workgroup
+//     panel kernel (panel staged in LDS, pivoting, row swaps, triangular
+//     solve of the U12 block row) alternating with the MFMA GEMM for the
+//     trailing update, then a blocked back substitution;
+//   * the rank rules of the truncations.
+//
+// Numerics, not bits, follow LAPACK: QR/SVD factors are unique up to column
+// signs, which every consumer in the solver path is invariant to.
+
+#include <float.h>
+
+#include "ttx_common.h"
+
+namespace ttx {
+
+constexpr double kEps = DBL_EPSILON;
+
+// ---------------------------------------------------------------------------
+// Householder QR on a column-major m x n matrix W (leading dim m) held in
+// global memory.  On return W holds R in its upper triangle and the reflector
+// vectors below the diagonal; tau[j] the reflector scales.  Q (m x k,
+// column-major, leading dim m) is formed in Qw.  Block-cooperative.
+// ---------------------------------------------------------------------------
+__device__ void block_householder_qr(int64_t m, int64_t n, double* W, double* tau, double* Qw,
+                                     double* red, bool form_q = true) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwave = blockDim.x >> 6;
+  const int64_t k = m < n ? m : n;
+  __shared__ double s_tau;
+  for (int64_t j = 0; j < k; ++j) {
+    double* col = W + j * m;
+    double s = 0.0;
+    for (int64_t i = j + 1 + tid; i < m; i += blockDim.x) s += col[i] * col[i];
+    s = block_sum(s, red);
+    const double alpha = col[j];
+    __syncthreads();
+    if (s == 0.0) {
+      if (tid == 0) { s_tau = 0.0; tau[j] = 0.0; }
+    } else {
+      const double beta = -copysign(sqrt(alpha * alpha + s), alpha);
+      const double scal = 1.0 / (alpha - beta);
+      for (int64_t i = j + 1 + tid; i < m; i += blockDim.x) col[i] *= scal;
+      if (tid == 0) {
+        s_tau = (beta - alpha) / beta;
+        tau[j] = s_tau;
+        col[j] = beta;
+      }
+    }
+    __syncthreads();
+    const double t = s_tau;
+    if (t != 0.0) {
+      for (int64_t c = j + 1 + wave; c < n; c += nwave) {
+        double* cc = W + c * m;
+        double d = 0.0;
+        for (int64_t i = j + 1 + lane; i < m; i += 64) d += col[i] * cc[i];
+        d = wave_sum(d) + cc[j];
+        d *= t;
+        if (lane == 0) cc[j] -= d;
+        for (int64_t i = j + 1 + lane; i < m; i += 64) cc[i] -= d * col[i];
+      }
+    }
+    __syncthreads();
+  }
+  if (!form_q) return;
+  // Q = H_0 ... H_{k-1} [I_k; 0], accumulated backwards
+  for (int64_t e = tid; e < m * k; e += blockDim.x) {
+    const int64_t i = e % m, c = e / m;
+    Qw[e] = (i == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int64_t j = k - 1; j >= 0; --j) {
+    const double t = tau[j];
+    const double* v = W + j * m;
+    if (t != 0.0) {
+      for (int64_t c = j + wave; c < k; c += nwave) {
+        double* qc = Qw + c * m;
+        double d = 0.0;
+        for (int64_t i = j + 1 + lane; i < m; i += 64) d += v[i] * qc[i];
+        d = wave_sum(d) + qc[j];
+        d *= t;
+        if (lane == 0) qc[j] -= d;
+        for (int64_t i = j + 1 + lane; i < m; i += 64) qc[i] -= d * v[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+struct QrArgs {
+  int64_t m, n;
+  const double* A;
+  int64_t a_rs, a_cs;
+  double* Q;
+  int64_t q_rs, q_cs;
+  double* R;
+  int64_t r_rs, r_cs;
+  double* work;
+};
+
+__global__ __launch_bounds__(1024) void qr_kernel(QrArgs p) {
+  __shared__ double red[16];
+  const int64_t m = p.m, n = p.n, k = m < n ? m : n;
+  double* W = p.work;
+  double* tau = W + m * n;
+  double* Qw = tau + k;
+  for (int64_t e = threadIdx.x; e < m * n; e += blockDim.x) {
+    const int64_t i = e % m, j = e / m;
+    W[e] = p.A[i * p.a_rs + j * p.a_cs];
+  }
+  __syncthreads();
+  block_householder_qr(m, n, W, tau, Qw, red, p.Q != nullptr);
+  if (p.Q) {
+    for (int64_t e = threadIdx.x; e < m * k; e += blockDim.x) {
+      const int64_t i = e % m, c = e / m;
+      p.Q[i * p.q_rs + c * p.q_cs] = Qw[e];
+    }
+  }
+  for (int64_t e = threadIdx.x; e < k * n; e += blockDim.x) {
+    const int64_t i = e % k, c = e / k;
+    p.R[i * p.r_rs + c * p.r_cs] = (i <= c) ? W[i + c * m] : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SVD
+// ---------------------------------------------------------------------------
+struct SvdArgs {
+  int64_t m, n;
+  const double* A;
+  int64_t a_rs, a_cs;
+  double* U;
+  int64_t u_rs, u_cs;
+  double* S;
+  double* Vt;
+  int64_t v_rs, v_cs;
+  double* work;
+  int32_t* info;
+};
+
+__device__ __forceinline__ int64_t rr_player(int64_t t, int64_t round, int64_t qq) {
+  return t == 0 ? 0 : 1 + (t - 1 + round) % (qq - 1);
+}
+
+__global__ __launch_bounds__(1024) void svd_kernel(SvdArgs p) {
+  __shared__ double red[16];
+  __shared__ int s_rot;
+  __shared__ int s_bad;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwave = blockDim.x >> 6;
+  const bool wide = p.m < p.n;
+  // X = A (tall) or A^T (wide): P x Q with P >= Q
+  const int64_t P = wide ? p.n : p.m;
+  const int64_t Q = wide ? p.m : p.n;
+  const int64_t x_rs = wide ? p.a_cs : p.a_rs;
+  const int64_t x_cs = wide ? p.a_rs : p.a_cs;
+  double* Wq = p.work;            // P x Q   (QR working matrix)
+  double* tau = Wq + P * Q;       // Q
+  double* Qx = tau + Q;           // P x Q   (orthonormal factor)
+  double* G = Qx + P * Q;         // Q x Q   (Jacobi matrix)
+  double* V = G + Q * Q;          // Q x Q   (accumulated rotations)
+  double* sig = V + Q * Q;        // Q
+  double* ordd = sig + Q;         // Q       (sort permutation, stored as doubles)
+
+  if (tid == 0) s_bad = 0;
+  __syncthreads();
+  for (int64_t e = tid; e < P * Q; e += blockDim.x) {
+    const int64_t i = e % P, j = e / P;
+    const double v = p.A[i * x_rs + j * x_cs];
+    if (!isfinite(v)) s_bad = 1;
+    Wq[e] = v;
+  }
+  __syncthreads();
+  if (s_bad) {
+    if (tid == 0) p.info[0] = TTX_ENONFINITE;
+    return;
+  }
+  const bool use_qr = P > Q;
+  if (use_qr) {
+    block_householder_qr(P, Q, Wq, tau, Qx, red);
+    for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
+      const int64_t i = e % Q, c = e / Q;
+      G[e] = (i <= c) ? Wq[i + c * P] : 0.0;
+    }
+  } else {
+    for (int64_t e = tid; e < Q * Q; e += blockDim.x) G[e] = Wq[e];
+  }
+  for (int64_t e = tid; e < Q * Q; e += blockDim.x) V[e] = ((e % Q) == (e / Q)) ? 1.0 : 0.0;
+  __syncthreads();
+
+  // one-sided Jacobi on the columns of G, round-robin pairs per round
+  const int64_t QQ = Q + (Q & 1);
+  const double tol = (double)Q * kEps;
+  int converged = (Q < 2);
+  for (int sweep = 0; sweep < 80 && !converged; ++sweep) {
+    if (tid == 0) s_rot = 0;
+    __syncthreads();
+    for (int64_t round = 0; round < QQ - 1; ++round) {
+      for (int64_t pk = wave; pk < QQ / 2; pk += nwave) {
+        int64_t i = rr_player(pk, round, QQ);
+        int64_t j = rr_player(QQ - 1 - pk, round, QQ);
+        if (i >= Q || j >= Q) continue;
+        if (i > j) { const int64_t t = i; i = j; j = t; }
+        double* gi = G + i * Q;
+        double* gj = G + j * Q;
+        double a = 0.0, b = 0.0, c = 0.0;
+        for (int64_t r = lane; r < Q; r += 64) {
+          const double x = gi[r], y = gj[r];
+          a += x * x;
+          b += y * y;
+          c += x * y;
+        }
+        a = wave_sum(a);
+        b = wave_sum(b);
+        c = wave_sum(c);
+        if (c == 0.0 || a == 0.0 || b == 0.0) continue;
+        if (fabs(c) <= tol * sqrt(a) * sqrt(b)) continue;
+        const double zeta = (b - a) / (2.0 * c);
+        double t;
+        if (fabs(zeta) > 1e150)
+          t = 0.5 / zeta;
+        else
+          t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + t * t);
+        const double sn = cs * t;
+        for (int64_t r = lane; r < Q; r += 64) {
+          const double x = gi[r], y = gj[r];
+          gi[r] = cs * x - sn * y;
+          gj[r] = sn * x + cs * y;
+        }
+        double* vi = V + i * Q;
+        double* vj = V + j * Q;
+        for (int64_t r = lane; r < Q; r += 64) {
+          const double x = vi[r], y = vj[r];
+          vi[r] = cs * x - sn * y;
+          vj[r] = sn * x + cs * y;
+        }
+        if (lane == 0) s_rot = 1;
+      }
+      __syncthreads();
+    }
+    converged = (s_rot == 0);
+    __syncthreads();
+  }
+  if (tid == 0) p.info[0] = converged ? TTX_OK : TTX_ENOCONV;
+
+  // singular values = column norms
+  for (int64_t j = wave; j < Q; j += nwave) {
+    const double* gj = G + j * Q;
+    double s = 0.0;
+    for (int64_t r = lane; r < Q; r += 64) s += gj[r] * gj[r];
+    s = wave_sum(s);
+    if (lane == 0) sig[j] = sqrt(s);
+  }
+  __syncthreads();
+  // descending order, stable on ties (selection by rank count)
+  for (int64_t j = tid; j < Q; j += blockDim.x) {
+    const double sj = sig[j];
+    int64_t pos = 0;
+    for (int64_t l = 0; l < Q; ++l) {
+      const double sl = sig[l];
+      if (sl > sj || (sl == sj && l < j)) ++pos;
+    }
+    ordd[pos] = (double)j;
+  }
+  __syncthreads();
+  // normalise the columns of G in place: G[:, j] /= sig[j]
+  for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
+    const int64_t j = e / Q;
+    const double s = sig[j];
+    G[e] = s > 0.0 ? G[e] / s : 0.0;
+  }
+  __syncthreads();
+  // Ux (P x Q) column o = (use_qr ? Qx : I) * G[:, ord[o]];  V column ord[o]
+  // tall: U_A = Ux, Vt_A = V^T.  wide: U_A = V, Vt_A = Ux^T.
+  for (int64_t e = tid; e < P * Q; e += blockDim.x) {
+    const int64_t r = e % P, o = e / P;
+    const int64_t src = (int64_t)ordd[o];
+    double v;
+    if (use_qr) {
+      v = 0.0;
+      const double* gcol = G + src * Q;
+      for (int64_t l = 0; l < Q; ++l) v += Qx[r + l * P] * gcol[l];
+    } else {
+      v = G[r + src * Q];
+    }
+    if (!wide)
+      p.U[r * p.u_rs + o * p.u_cs] = v;
+    else
+      p.Vt[o * p.v_rs + r * p.v_cs] = v;
+  }
+  for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
+    const int64_t r = e % Q, o = e / Q;
+    const int64_t src = (int64_t)ordd[o];
+    const double v = V[r + src * Q];
+    if (!wide)
+      p.Vt[o * p.v_rs + r * p.v_cs] = v;
+    else
+      p.U[r * p.u_rs + o * p.u_cs] = v;
+  }
+  for (int64_t o = tid; o < Q; o += blockDim.x) p.S[o] = sig[(int64_t)ordd[o]];
+}
+
+// ---------------------------------------------------------------------------
+// LU with partial pivoting on [M | b], column-major, leading dim n, n+1 cols.
+// ---------------------------------------------------------------------------
+struct PanelArgs {
+  int64_t n, k, nb;
+  double* W;
+  int32_t* piv;
+  int32_t* info;
+};
+
+__global__ __launch_bounds__(512) void lu_panel_kernel(PanelArgs p) {
+  extern __shared__ double P[];  // nb columns x mp rows, column-major
+  __shared__ double rv[8];
+  __shared__ int64_t ri[8];
+  __shared__ int64_t s_piv;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwave = blockDim.x >> 6;
+  const int64_t n = p.n, k = p.k, nb = p.nb, ncols = n + 1;
+  const int64_t mp = n - k;
+  double* W = p.W;
+  for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
+    const int64_t r = e % mp, c = e / mp;
+    P[e] = W[(k + r) + (k + c) * n];
+  }
+  __syncthreads();
+  for (int64_t j = 0; j < nb; ++j) {
+    double* pj = P + j * mp;
+    // pivot: first index of the largest magnitude (idamax semantics)
+    double best = -1.0;
+    int64_t bi = j;
+    for (int64_t r = j + tid; r < mp; r += blockDim.x) {
+      const double v = fabs(pj[r]);
+      if (v > best) { best = v; bi = r; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(best, off, 64);
+      const int64_t oi = __shfl_xor(bi, off, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if (lane == 0) { rv[wave] = best; ri[wave] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      double b0 = rv[0];
+      int64_t i0 = ri[0];
+      for (int w = 1; w < nwave; ++w)
+        if (rv[w] > b0 || (rv[w] == b0 && ri[w] < i0)) { b0 = rv[w]; i0 = ri[w]; }
+      s_piv = i0;
+      p.piv[k + j] = (int32_t)(k + i0);
+      if (b0 == 0.0) p.info[0] = TTX_ESINGULAR;
+    }
+    __syncthreads();
+    const int64_t pr = s_piv;
+    if (pr != j) {
+      for (int64_t c = tid; c < nb; c += blockDim.x) {
+        double* pc = P + c * mp;
+        const double t = pc[j];
+        pc[j] = pc[pr];
+        pc[pr] = t;
+      }
+    }
+    __syncthreads();
+    const double d = pj[j];
+    if (d != 0.0) {
+      for (int64_t r = j + 1 + tid; r < mp; r += blockDim.x) pj[r] /= d;
+    }
+    __syncthreads();
+    for (int64_t c = j + 1; c < nb; ++c) {
+      double* pc = P + c * mp;
+      const double u = pc[j];
+      for (int64_t r = j + 1 + tid; r < mp; r += blockDim.x) pc[r] -= pj[r] * u;
+    }
+    __syncthreads();
+  }
+  for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
+    const int64_t r = e % mp, c = e / mp;
+    W[(k + r) + (k + c) * n] = P[e];
+  }
+  // row swaps on every column outside the panel, then U12 = L11^-1 A12 on the
+  // trailing columns; one thread owns a column, so no barrier is needed
+  for (int64_t c = tid; c < ncols; c += blockDim.x) {
+    if (c >= k && c < k + nb) continue;
+    double* wc = W + c * n;
+    for (int64_t j = 0; j < nb; ++j) {
+      const int64_t pr = (int64_t)p.piv[k + j];
+      if (pr != k + j) {
+        const double t = wc[k + j];
+        wc[k + j] = wc[pr];
+        wc[pr] = t;
+      }
+    }
+    if (c >= k + nb) {
+      for (int64_t i = 1; i < nb; ++i) {
+        double s = wc[k + i];
+        for (int64_t l = 0; l < i; ++l) s -= P[l * mp + i] * wc[k + l];
+        wc[k + i] = s;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void lu_backsub_kernel(int64_t n, const double* W, double* u,
+                                                         int32_t* info) {
+  __shared__ double xb[64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const double* y = W + n * n;  // column n
+  for (int64_t i = tid; i < n; i += blockDim.x) u[i] = y[i];
+  __syncthreads();
+  for (int64_t j1 = n; j1 > 0; j1 -= 64) {
+    const int64_t j0 = j1 - 64 > 0 ? j1 - 64 : 0;
+    if (wave == 0) {
+      const int64_t row = j0 + lane;
+      double yv = row < j1 ? u[row] : 0.0;
+      for (int64_t j = j1 - 1; j >= j0; --j) {
+        const double ujj = W[j + j * n];
+        const double xj = __shfl(yv, (int)(j - j0), 64) / ujj;
+        if (row == j) yv = xj;
+        if (row < j) yv -= xj * W[row + j * n];
+      }
+      if (row < j1) { u[row] = yv; xb[lane] = yv; }
+    }
+    __syncthreads();
+    const int64_t nbk = j1 - j0;
+    for (int64_t r = tid; r < j0; r += blockDim.x) {
+      double s = u[r];
+      for (int64_t c = 0; c < nbk; ++c) s -= W[r + (j0 + c) * n] * xb[c];
+      u[r] = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void lu_info_init_kernel(int32_t* info) { info[0] = TTX_OK; }
+
+// ---------------------------------------------------------------------------
+// rank rules
+// ---------------------------------------------------------------------------
+__global__ void truncated_bond_kernel(const double* S, int64_t len, double delta, int64_t max_rank,
+                                      int32_t* out) {
+  if (threadIdx.x != 0) return;
+  // zero the rounding tail, then count tail norms above delta
+  const double cutoff = kEps * (double)len * (len > 0 ? S[0] : 0.0);
+  double tail = 0.0;
+  int64_t count = 0;
+  const double d2 = delta * delta;
+  for (int64_t i = len - 1; i >= 0; --i) {
+    const double v = S[i] <= cutoff ? 0.0 : S[i];
+    tail += v * v;
+    if (tail > d2) ++count;
+  }
+  int64_t bond = count > 1 ? count : 1;
+  if (max_rank > 0 && bond > max_rank) bond = max_rank;
+  out[0] = (int32_t)bond;
+}
+
+__global__ __launch_bounds__(1024) void rank_by_residual_kernel(int64_t N, int64_t limit,
+                                                               const double* S,
+                                                               const double* contrib,
+                                                               const double* b, const double* exact,
+                                                               double damp, double half_tol,
+                                                               double* partial, int32_t* out) {
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  if (S[0] == 0.0) {
+    if (tid == 0) { out[0] = 1; out[1] = 0; }
+    return;
+  }
+  double s = 0.0;
+  for (int64_t i = tid; i < N; i += blockDim.x) {
+    const double v = b[i];
+    partial[i] = -v;
+    s += v * v;
+  }
+  const double bnorm = sqrt(block_sum(s, red));
+  const double a = damp * exact[0];
+  const double f = half_tol * bnorm;
+  const double thr = a > f ? a : f;
+  for (int64_t r = 0; r < limit; ++r) {
+    const double* cr = contrib + r * N;
+    double q = 0.0;
+    for (int64_t i = tid; i < N; i += blockDim.x) {
+      const double v = partial[i] + cr[i];
+      partial[i] = v;
+      q += v * v;
+    }
+    const double nrm = sqrt(block_sum(q, red));
+    if (nrm <= thr) {
+      if (tid == 0) { out[0] = (int32_t)(r + 1); out[1] = 0; }
+      return;
+    }
+  }
+  if (tid == 0) { out[0] = (int32_t)limit; out[1] = 1; }
+}
+
+__global__ __launch_bounds__(256) void svd_terms_kernel(int64_t p, int64_t q, int64_t limit,
+                                                       const double* U, int64_t u_rs, int64_t u_cs,
+                                                       const double* S, const double* Vt,
+                                                       int64_t v_rs, int64_t v_cs, double* T) {
+  const int64_t N = p * q;
+  const int64_t total = N * limit;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int64_t row = e % N, r = e / N;
+    const int64_t i = row / q, j = row % q;
+    T[e] = U[i * u_rs + r * u_cs] * S[r] * Vt[r * v_rs + j * v_cs];
+  }
+}
+
+}  // namespace ttx
+
+using namespace ttx;
+
+extern "C" int64_t ttx_qr_work_size(int64_t m, int64_t n) {
+  const int64_t k = m < n ? m : n;
+  return m * n + k + m * k;
+}
+
+extern "C" int ttx_qr_f64(ttx_stream_t stream, int64_t m, int64_t n, const double* A, int64_t a_rs,
+                          int64_t a_cs, double* Q, int64_t q_rs, int64_t q_cs, double* R,
+                          int64_t r_rs, int64_t r_cs, double* work) {
+  if (m < 1 || n < 1) return TTX_EINVAL;
+  QrArgs p{m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work};
+  hipLaunchKernelGGL(qr_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, p);
+  return launch_status();
+}
+
+extern "C" int64_t ttx_svd_work_size(int64_t m, int64_t n) {
+  const int64_t P = m > n ? m : n, Q = m < n ? m : n;
+  return 2 * P * Q + 2 * Q * Q + 3 * Q + 8;
+}
+
+extern "C" int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n, const double* A,
+                           int64_t a_rs, int64_t a_cs, double* U, int64_t u_rs, int64_t u_cs,
+                           double* S, double* Vt, int64_t v_rs, int64_t v_cs, double* work,
+                           int32_t* info) {
+  if (m < 1 || n < 1) return TTX_EINVAL;
+  SvdArgs p{m, n, A, a_rs, a_cs, U, u_rs, u_cs, S, Vt, v_rs, v_cs, work, info};
+  hipLaunchKernelGGL(svd_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, p);
+  return launch_status();
+}
+
+static int64_t lu_block(int64_t n) {
+  // the panel (n rows x nb cols) is staged in LDS; keep it under 144 KiB
+  int64_t nb = (144 * 1024) / (8 * (n > 0 ? n : 1));
+  if (nb > 16) nb = 16;
+  return nb;
+}
+
+extern "C" int64_t ttx_lu_work_size(int64_t n) { return (n + 1) / 2 + 1; /* pivots as int32 */ }
+
+extern "C" int ttx_lu_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, double* u, double* work,
+                                int32_t* info) {
+  if (n < 1) return TTX_EINVAL;
+  const int64_t nb0 = lu_block(n);
+  if (nb0 < 1) return TTX_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  static bool attr_set = false;
+  if (!attr_set) {
+    if (hipFuncSetAttribute((const void*)lu_panel_kernel,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024) != hipSuccess)
+      return TTX_EHIP;
+    attr_set = true;
+  }
+  int32_t* piv = reinterpret_cast<int32_t*>(work);
+  hipLaunchKernelGGL(lu_info_init_kernel, dim3(1), dim3(1), 0, st, info);
+  for (int64_t k = 0; k < n; k += nb0) {
+    const int64_t nb = (n - k) < nb0 ? (n - k) : nb0;
+    PanelArgs pa{n, k, nb, Mb, piv, info};
+    const size_t shm = (size_t)(nb * (n - k)) * sizeof(double);
+    hipLaunchKernelGGL(lu_panel_kernel, dim3(1), dim3(512), shm, st, pa);
+    const int64_t rows = n - k - nb;
+    const int64_t cols = n + 1 - k - nb;
+    if (rows > 0 && cols > 0) {
+      int rc = ttx_gemm_f64(stream, 1, rows, cols, nb, -1.0, Mb + (k + nb) + k * n, 1, n, 0,
+                            Mb + k + (k + nb) * n, 1, n, 0, 1.0, Mb + (k + nb) + (k + nb) * n, 1,
+                            n, 0);
+      if (rc != TTX_OK) return rc;
+    }
+  }
+  hipLaunchKernelGGL(lu_backsub_kernel, dim3(1), dim3(1024), 0, st, n, Mb, u, info);
+  return launch_status();
+}
+
+extern "C" int ttx_truncated_bond_f64(ttx_stream_t stream, const double* S, int64_t len,
+                                      double delta, int64_t max_rank, int32_t* out) {
+  if (len < 1) return TTX_EINVAL;
+  hipLaunchKernelGGL(truncated_bond_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, S, len,
+                     delta, max_rank, out);
+  return launch_status();
+}
+
+extern "C" int ttx_rank_by_residual_f64(ttx_stream_t stream, int64_t N, int64_t limit,
+                                        const double* S, const double* contrib,
+                                        const double* b_loc, const double* exact, double damp,
+                                        double half_trunc_tol, double* work, int32_t* out) {
+  if (N < 1 || limit < 1) return TTX_EINVAL;
+  hipLaunchKernelGGL(rank_by_residual_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, N,
+                     limit, S, contrib, b_loc, exact, damp, half_trunc_tol, work, out);
+  return launch_status();
+}
+
+extern "C" int ttx_svd_terms_f64(ttx_stream_t stream, int64_t p, int64_t q, int64_t limit,
+                                 const double* U, int64_t u_rs, int64_t u_cs, const double* S,
+                                 const double* Vt, int64_t v_rs, int64_t v_cs, double* terms) {
+  if (p < 1 || q < 1 || limit < 1) return TTX_EINVAL;
+  int64_t total = p * q * limit;
+  int64_t g = (total + 255) / 256;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(svd_terms_kernel, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream, p, q,
+                     limit, U, u_rs, u_cs, S, Vt, v_rs, v_cs, terms);
+  return launch_status();
+}

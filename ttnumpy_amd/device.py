@@ -1,0 +1,292 @@
+"""Device operations of the solver hot path, issued to libttx on torch buffers.
+
+PyTorch provides device memory and the stream; every arithmetic operation is
+a libttx kernel.  Cores are contiguous FP64 device tensors
+``(left_bond, row_mode, col_mode, right_bond)``; environments are contiguous
+``(target, operator, frame)`` or ``(target, frame)`` tensors, the index order
+of ``ttnumpy/solvers/common.py:149-217``.
+
+Contractions are issued as strided batched GEMMs whose strides read the
+operands in place, so no permuted copy of a core or environment is made.
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Optional, Tuple
+
+import torch
+
+from . import _lib
+from ._lib import check
+
+F64 = torch.float64
+_DEV = None
+
+
+def device() -> torch.device:
+    global _DEV
+    if _DEV is None:
+        if not torch.cuda.is_available():
+            raise RuntimeError("ttnumpy_amd needs a HIP device (MI355X); none is visible")
+        _lib.load()
+        _DEV = torch.device("cuda", torch.cuda.current_device())
+    return _DEV
+
+
+def lib():
+    return _lib.load()
+
+
+def stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def empty(*shape) -> torch.Tensor:
+    return torch.empty(*shape, dtype=F64, device=device())
+
+
+def zeros(*shape) -> torch.Tensor:
+    t = torch.empty(*shape, dtype=F64, device=device())
+    fill(t, 0.0)
+    return t
+
+
+def to_device(a) -> torch.Tensor:
+    return torch.as_tensor(a, dtype=F64).to(device()).contiguous()
+
+
+def P(t: torch.Tensor, off: int = 0) -> int:
+    """Device address of element ``off`` of a contiguous FP64 tensor."""
+    return t.data_ptr() + 8 * off
+
+
+# ---------------------------------------------------------------- primitives
+def gemm(batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, beta, C, c_ms, c_ns, c_bs):
+    """C = alpha A B + beta C on MFMA; A/B/C are addresses (ints)."""
+    rc = lib().ttx_gemm_f64(stream(), batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns,
+                            b_bs, beta, C, c_ms, c_ns, c_bs)
+    check(rc, "ttx_gemm_f64")
+
+
+def copy_nd(dims, src, src_strides, dst, dst_strides, alpha=1.0):
+    nd = len(dims)
+    arr = (ctypes_i64 * nd)
+    rc = lib().ttx_copy_f64(stream(), nd, arr(*dims), src, arr(*src_strides), dst,
+                            arr(*dst_strides), alpha)
+    check(rc, "ttx_copy_f64")
+
+
+def fill(t: torch.Tensor, value: float) -> None:
+    check(lib().ttx_fill_f64(stream(), P(t), t.numel(), value), "ttx_fill_f64")
+
+
+def nrm2(n: int, x: int, out: int) -> None:
+    check(lib().ttx_nrm2_f64(stream(), n, x, out), "ttx_nrm2_f64")
+
+
+def diff_nrm2(n: int, a: int, b: int, out: int) -> None:
+    check(lib().ttx_diff_nrm2_f64(stream(), n, a, b, out), "ttx_diff_nrm2_f64")
+
+
+def norm(t: torch.Tensor) -> torch.Tensor:
+    """Device scalar ||t||_2."""
+    out = empty(1)
+    nrm2(t.numel(), P(t), P(out))
+    return out
+
+
+import ctypes as _ct  # noqa: E402
+
+ctypes_i64 = _ct.c_int64
+
+
+# ---------------------------------------------------------------- environments
+def env_left(E: torch.Tensor, F: torch.Tensor, T: torch.Tensor, A: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Next left environment (``common.py:170-188``); F frame core, T target core (4-D)."""
+    rf, nr, _, rf2 = F.shape
+    rt, nc, _, rt2 = T.shape
+    if A is None:
+        s1 = empty(rt * nr * rf2)
+        gemm(1, rt, nr * rf2, rf, 1.0, P(E), rf, 1, 0, P(F), nr * rf2, 1, 0, 0.0, P(s1), nr * rf2, 1, 0)
+        out = empty(rt2, rf2)
+        gemm(1, rt2, rf2, rt * nr, 1.0, P(T), 1, rt2, 0, P(s1), rf2, 1, 0, 0.0, P(out), rf2, 1, 0)
+        return out
+    Ra, _, _, Ra2 = A.shape
+    s1 = empty(rt * Ra * nr * rf2)       # [t, a, i, f']
+    gemm(1, rt * Ra, nr * rf2, rf, 1.0, P(E), rf, 1, 0, P(F), nr * rf2, 1, 0, 0.0, P(s1), nr * rf2, 1, 0)
+    s2 = empty(rt * nc * Ra2 * rf2)      # [t, j, a', f']
+    gemm(rt, rf2, nc * Ra2, Ra * nr, 1.0, P(s1), 1, rf2, Ra * nr * rf2, P(A), nc * Ra2, 1, 0, 0.0,
+         P(s2), 1, rf2, nc * Ra2 * rf2)
+    out = empty(rt2, Ra2, rf2)
+    gemm(1, rt2, Ra2 * rf2, rt * nc, 1.0, P(T), 1, rt2, 0, P(s2), Ra2 * rf2, 1, 0, 0.0, P(out),
+         Ra2 * rf2, 1, 0)
+    return out
+
+
+def env_right(Phi: torch.Tensor, F: torch.Tensor, T: torch.Tensor, A: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Next right environment (``common.py:203-217``); F, T are the cores right of it."""
+    rf, nr, _, rf2 = F.shape
+    rt, nc, _, rt2 = T.shape
+    if A is None:
+        s1 = empty(rt * nc * rf2)        # [t, i, f']
+        gemm(1, rt * nc, rf2, rt2, 1.0, P(T), rt2, 1, 0, P(Phi), rf2, 1, 0, 0.0, P(s1), rf2, 1, 0)
+        out = empty(rt, rf)
+        gemm(1, rt, rf, nc * rf2, 1.0, P(s1), nc * rf2, 1, 0, P(F), 1, nr * rf2, 0, 0.0, P(out), rf, 1, 0)
+        return out
+    Ra, _, _, Ra2 = A.shape
+    s1 = empty(rt * nc * Ra2 * rf2)      # [t, j, a', f']
+    gemm(1, rt * nc, Ra2 * rf2, rt2, 1.0, P(T), rt2, 1, 0, P(Phi), Ra2 * rf2, 1, 0, 0.0, P(s1),
+         Ra2 * rf2, 1, 0)
+    s2 = empty(rt * Ra * nr * rf2)       # [t, a, i, f']
+    gemm(rt, rf2, Ra * nr, nc * Ra2, 1.0, P(s1), 1, rf2, nc * Ra2 * rf2, P(A), 1, nc * Ra2, 0, 0.0,
+         P(s2), 1, rf2, Ra * nr * rf2)
+    out = empty(rt, Ra, rf)
+    gemm(1, rt * Ra, rf, nr * rf2, 1.0, P(s2), nr * rf2, 1, 0, P(F), 1, nr * rf2, 0, 0.0, P(out), rf, 1, 0)
+    return out
+
+
+def ones_env(with_op: bool) -> torch.Tensor:
+    t = empty(1, 1, 1) if with_op else empty(1, 1)
+    fill(t, 1.0)
+    return t
+
+
+# ---------------------------------------------------------------- local system
+class LocalOp:
+    """Matrix-free one-site operator Psi . A_k . Phi (``one_site.py:78-115``).
+
+    Rows are indexed by the frame bonds (fl, i, fr), columns by the target
+    bonds (rl, j, rr).  ``apply`` takes B column vectors stored one after
+    the other (column-major N x B).
+    """
+
+    def __init__(self, psi: torch.Tensor, A: torch.Tensor, phi: torch.Tensor):
+        self.psi, self.A, self.phi = psi, A, phi
+        self.rl, self.Ra, self.fl = psi.shape
+        self.rr, self.Rb, self.fr = phi.shape
+        _, self.nr, self.nc, _ = A.shape
+        self.rows = self.fl * self.nr * self.fr
+        self.cols = self.rl * self.nc * self.rr
+        self._scratch = {}
+
+    def _bufs(self, B):
+        if B not in self._scratch:
+            self._scratch[B] = (empty(B * self.rl * self.nc * self.Rb * self.fr),
+                                empty(B * self.rl * self.Ra * self.nr * self.fr))
+        return self._scratch[B]
+
+    def apply(self, y: int, B: int, out: int, alpha: float = 1.0, beta: float = 0.0) -> None:
+        rl, Ra, fl, rr, Rb, fr, nr, nc = self.rl, self.Ra, self.fl, self.rr, self.Rb, self.fr, self.nr, self.nc
+        s1, s2 = self._bufs(B)
+        # s1[z, tl, j, b, fr] = sum_tr y[z, tl, j, tr] phi[tr, b, fr]
+        gemm(1, B * rl * nc, Rb * fr, rr, 1.0, y, rr, 1, 0, P(self.phi), Rb * fr, 1, 0, 0.0, P(s1),
+             Rb * fr, 1, 0)
+        # s2[(z, tl), (a, i), fr] = sum_{j, b} A[a, i, j, b] s1[(z, tl), j, b, fr]
+        gemm(B * rl, Ra * nr, fr, nc * Rb, 1.0, P(self.A), nc * Rb, 1, 0, P(s1), fr, 1, nc * Rb * fr,
+             0.0, P(s2), fr, 1, Ra * nr * fr)
+        # out[z, fl, (i, fr)] = sum_{tl, a} psi[tl, a, fl] s2[z, (tl, a), (i, fr)]
+        gemm(B, fl, nr * fr, rl * Ra, alpha, P(self.psi), 1, fl, 0, P(s2), nr * fr, 1,
+             rl * Ra * nr * fr, beta, out, nr * fr, 1, fl * nr * fr)
+
+    def explicit(self, out: torch.Tensor, ld: int) -> None:
+        """Write the explicit matrix column-major into ``out`` (``one_site.py:49-75``)."""
+        rc = lib().ttx_local_matrix_f64(stream(), P(self.psi), self.rl, self.Ra, self.fl, P(self.A),
+                                        self.nr, self.nc, self.Rb, P(self.phi), self.rr, self.fr,
+                                        P(out), ld)
+        check(rc, "ttx_local_matrix_f64")
+
+
+class ExplicitOp:
+    """The explicit local matrix (column-major N x N) as an operator."""
+
+    def __init__(self, M: torch.Tensor, n: int):
+        self.M, self.rows, self.cols = M, n, n
+
+    def apply(self, y: int, B: int, out: int, alpha: float = 1.0, beta: float = 0.0) -> None:
+        n = self.rows
+        gemm(1, n, B, n, alpha, P(self.M), 1, n, 0, y, 1, n, 0, beta, out, 1, n, 0)
+
+
+def local_rhs(psi_b: torch.Tensor, b_core: torch.Tensor, phi_b: torch.Tensor, out: int) -> None:
+    """``b_loc[f_l, i, f_r] = sum Psi[t, f_l] b[t, i, t'] Phi[t', f_r]`` (``one_site.py:27-46``)."""
+    rt, fl = psi_b.shape
+    _, n, _, rt2 = b_core.shape
+    _, fr = phi_b.shape
+    s1 = empty(fl * n * rt2)
+    gemm(1, fl, n * rt2, rt, 1.0, P(psi_b), 1, fl, 0, P(b_core), n * rt2, 1, 0, 0.0, P(s1), n * rt2, 1, 0)
+    gemm(1, fl * n, fr, rt2, 1.0, P(s1), rt2, 1, 0, P(phi_b), fr, 1, 0, 0.0, out, fr, 1, 0)
+
+
+def lu_solve(Mb: torch.Tensor, n: int, u: torch.Tensor, info: torch.Tensor) -> None:
+    """Solve the [M | b] system in ``Mb`` (destroyed) into ``u`` (``common.py:767``)."""
+    work = torch.empty(int(lib().ttx_lu_work_size(n)) + 1, dtype=F64, device=device())
+    check(lib().ttx_lu_solve_f64(stream(), n, P(Mb), P(u), P(work), info.data_ptr()), "ttx_lu_solve_f64")
+
+
+# ---------------------------------------------------------------- factorizations
+def qr(m: int, n: int, A: int, a_rs: int, a_cs: int, Q: Optional[int], q_rs: int, q_cs: int,
+       R: int, r_rs: int, r_cs: int) -> None:
+    work = empty(int(lib().ttx_qr_work_size(m, n)))
+    rc = lib().ttx_qr_f64(stream(), m, n, A, a_rs, a_cs, Q if Q is not None else None, q_rs, q_cs,
+                          R, r_rs, r_cs, P(work))
+    check(rc, "ttx_qr_f64")
+
+
+def svd(m: int, n: int, A: int, a_rs: int, a_cs: int, info: torch.Tensor, info_off: int = 0
+        ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Thin SVD; returns contiguous U (m x k), S (k), Vt (k x n)."""
+    k = min(m, n)
+    U = empty(m, k)
+    S = empty(k)
+    Vt = empty(k, n)
+    work = empty(int(lib().ttx_svd_work_size(m, n)))
+    rc = lib().ttx_svd_f64(stream(), m, n, A, a_rs, a_cs, P(U), k, 1, P(S), P(Vt), n, 1, P(work),
+                           info.data_ptr() + 4 * info_off)
+    check(rc, "ttx_svd_f64")
+    return U, S, Vt
+
+
+def scale_cols(rows, cols, X: int, rs, cs, s: int) -> None:
+    check(lib().ttx_scale_cols_f64(stream(), rows, cols, X, rs, cs, s), "ttx_scale_cols_f64")
+
+
+def truncated_bond_dev(S: torch.Tensor, nrm: torch.Tensor, scale: float, max_rank: Optional[int],
+                       out: torch.Tensor, out_off: int = 0) -> None:
+    rc = lib().ttx_truncated_bond_dev_f64(stream(), P(S), S.numel(), P(nrm), scale,
+                                          0 if max_rank is None else int(max_rank),
+                                          out.data_ptr() + 4 * out_off)
+    check(rc, "ttx_truncated_bond_dev_f64")
+
+
+def ttm_core(A: torch.Tensor, x: torch.Tensor, out: int, o_l: int, o_i: int, o_r: int) -> None:
+    Ra, nr, nc, Rb = A.shape
+    rx, _, _, rx2 = x.shape
+    rc = lib().ttx_ttm_core_f64(stream(), P(A), Ra, nr, nc, Rb, P(x), rx, rx2, out, o_l, o_i, o_r)
+    check(rc, "ttx_ttm_core_f64")
+
+
+def absorb_left(Mx: int, m: int, r: int, core: torch.Tensor, out: Optional[torch.Tensor] = None,
+                m_rs: Optional[int] = None, m_cs: int = 1) -> torch.Tensor:
+    """``(m x r) @ core.reshape(r, -1)`` (``utils.py:72-76``); Mx row-major unless strides given."""
+    rest = core.numel() // core.shape[0]
+    if out is None:
+        out = empty((m,) + tuple(core.shape[1:]))
+    gemm(1, m, rest, r, 1.0, Mx, r if m_rs is None else m_rs, m_cs, 0, P(core), rest, 1, 0, 0.0,
+         P(out), rest, 1, 0)
+    return out
+
+
+def absorb_right(core: torch.Tensor, Mx: int, r: int, m: int, m_rs: Optional[int] = None,
+                 m_cs: int = 1) -> torch.Tensor:
+    """``core.reshape(-1, r) @ (r x m)`` (``utils.py:79-83``)."""
+    rows = core.numel() // r
+    out = empty(tuple(core.shape[:-1]) + (m,))
+    gemm(1, rows, m, r, 1.0, P(core), r, 1, 0, Mx, m if m_rs is None else m_rs, m_cs, 0, 0.0,
+         P(out), m, 1, 0)
+    return out
+
+
+def sync_ints(t: torch.Tensor):
+    """Bring a small int32 device array to the host (the one sync of a decision)."""
+    return t.cpu().tolist()

@@ -1,0 +1,6 @@
+"""Device-resident TT linear solvers (AMEn, ALS) with the reference vocabulary."""
+
+from .common import (LocalSolver, SolverInfo, StopReason, TruncationMode,  # noqa: F401
+                     relative_residual)
+from .amen import amen  # noqa: F401
+from .als import als  # noqa: F401

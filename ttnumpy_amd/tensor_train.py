@@ -1,0 +1,465 @@
+"""Device-resident tensor train with the reference container's interface.
+
+Mirrors the parts of ``ttnumpy.tensor_train.TensorTrain`` (ttnumpy 0.3.0,
+``ttnumpy/tensor_train.py:150-940``) on the solver hot path: layout and
+validation, ``copy``, ``orthonormalize``, ``compress_svd``, ``truncate``,
+``add``, scalar ``multiply``, ``dot`` (TTM @ TT) and ``tt_norm``, plus the
+``rand``/``ones``/``zeros``/``eye`` constructors.  Cores are contiguous FP64
+device tensors ``(r, n_row, n_col, r')``; every sweep runs on libttx kernels.
+"""
+
+from __future__ import annotations
+
+import math
+import numbers
+import warnings
+from enum import Enum
+from typing import List, Optional, Sequence, Union
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import device as D
+from .device import P
+
+
+class DimensionMismatch(Exception):
+    """Exception for dimension errors (``tensor_train.py:21-26``)."""
+
+    def __init__(self, msg="Dimension mismatch"):
+        self.msg = msg
+        super().__init__(msg)
+
+
+class WrongTTFormat(Exception):
+    """Exception for errors in tensor train structure (``tensor_train.py:29-34``)."""
+
+    def __init__(self, msg="Wrong tensor train format"):
+        self.msg = msg
+        super().__init__(msg)
+
+
+class WrongTTType(Exception):
+    """Exception for errors in tensor train type (``tensor_train.py:37-42``)."""
+
+    def __init__(self, msg="Wrong tensor train type"):
+        self.msg = msg
+        super().__init__(msg)
+
+
+class TensorTrainType(Enum):
+    TT = "Tensor Train"
+    TTM = "Tensor Train Matrix"
+
+
+def _as_core(c) -> torch.Tensor:
+    if isinstance(c, torch.Tensor):
+        t = c.to(device=D.device(), dtype=D.F64)
+    else:
+        t = D.to_device(np.asarray(c, dtype=np.float64))
+    return t.contiguous()
+
+
+# ---------------------------------------------------------------- sweeps
+def qr_sweep_left(cores: List[torch.Tensor], start: int, end: int) -> None:
+    """Left QR sweep ``start..end-1`` (``utils.py:86-112``)."""
+    for i in range(start, end):
+        c = cores[i]
+        r0, nr, nc, r1 = c.shape
+        m = r0 * nr * nc
+        k = min(m, r1)
+        Q = D.empty(r0, nr, nc, k)
+        R = D.empty(k, r1)
+        D.qr(m, r1, P(c), r1, 1, P(Q), k, 1, P(R), r1, 1)
+        cores[i] = Q
+        cores[i + 1] = D.absorb_left(P(R), k, r1, cores[i + 1])
+
+
+def qr_sweep_right(cores: List[torch.Tensor], start: int, end: int, form_q: bool = True) -> None:
+    """Right QR sweep ``start..end+1`` on the transposed unfoldings (``utils.py:115-136``)."""
+    for i in range(start, end, -1):
+        c = cores[i]
+        r0, nr, nc, r1 = c.shape
+        w = nr * nc * r1
+        k = min(w, r0)
+        R = D.empty(k, r0)
+        if form_q:
+            Qt = D.empty(k, nr, nc, r1)
+            D.qr(w, r0, P(c), 1, w, P(Qt), 1, w, P(R), r0, 1)
+            cores[i] = Qt
+        else:
+            D.qr(w, r0, P(c), 1, w, None, 1, w, P(R), r0, 1)
+        cores[i - 1] = D.absorb_right(cores[i - 1], P(R), r0, k, m_rs=1, m_cs=r0)
+
+
+def _sweep_status(buf: torch.Tensor, core: int) -> int:
+    info, bond = D.sync_ints(buf)
+    if info == _lib.TTX_ENONFINITE:
+        raise ValueError(f"Core {core} has NaN or infinite entries")
+    if info == _lib.TTX_ENOCONV:
+        warnings.warn(f"SVD of core {core} did not converge within the sweep limit", RuntimeWarning)
+    return bond
+
+
+def svd_sweep_left(cores: List[torch.Tensor], error: float, max_rank=None, spectra=None) -> None:
+    """Left-to-right truncated SVD sweep (``utils.py:364-411``)."""
+    d = len(cores)
+    nrm = D.norm(cores[0])
+    scale = error / math.sqrt(d - 1)
+    bond = 1
+    buf = torch.zeros(2, dtype=torch.int32, device=D.device())
+    for i in range(d - 1):
+        c = cores[i]
+        _, nr, nc, r1 = c.shape
+        m = bond * nr * nc
+        U, S, Vt = D.svd(m, r1, P(c), r1, 1, buf, 0)
+        k = min(m, r1)
+        D.truncated_bond_dev(S, nrm, scale, max_rank, buf, 1)
+        nb = _sweep_status(buf, i)
+        if spectra is not None:
+            spectra.append(S.clone())
+        new = D.empty(bond, nr, nc, nb)
+        D.copy_nd([m, nb], P(U), [k, 1], P(new), [nb, 1])
+        cores[i] = new
+        D.scale_cols(r1, nb, P(Vt), 1, r1, P(S))
+        cores[i + 1] = D.absorb_left(P(Vt), nb, r1, cores[i + 1])
+        bond = nb
+
+
+def svd_sweep_right(cores: List[torch.Tensor], error: float, max_rank=None, spectra=None) -> None:
+    """Right-to-left truncated SVD sweep (``utils.py:414-460``)."""
+    d = len(cores)
+    nrm = D.norm(cores[-1])
+    scale = error / math.sqrt(d - 1)
+    bond = 1
+    buf = torch.zeros(2, dtype=torch.int32, device=D.device())
+    for i in range(d - 1, 0, -1):
+        c = cores[i]
+        r0, nr, nc, _ = c.shape
+        w = nr * nc * bond
+        U, S, Vt = D.svd(r0, w, P(c), w, 1, buf, 0)
+        k = min(r0, w)
+        D.truncated_bond_dev(S, nrm, scale, max_rank, buf, 1)
+        nb = _sweep_status(buf, i)
+        if spectra is not None:
+            spectra.append(S.clone())
+        cores[i] = Vt[:nb].reshape(nb, nr, nc, bond)
+        D.scale_cols(r0, nb, P(U), k, 1, P(S))
+        cores[i - 1] = D.absorb_right(cores[i - 1], P(U), r0, nb, m_rs=k, m_cs=1)
+        bond = nb
+
+
+# ---------------------------------------------------------------- container
+class TensorTrain:
+    """Tensor train / tensor train matrix with device cores (``tensor_train.py:150-244``)."""
+
+    def __init__(self, cores: Sequence):
+        if len(cores) < 2:
+            raise WrongTTFormat("Tensor train must has at least 2 sites")
+        shapes = [tuple(c.shape) for c in cores]
+        ndims = {len(s) for s in shapes}
+        if ndims == {3}:
+            tt_type = TensorTrainType.TT
+        elif ndims == {4}:
+            tt_type = TensorTrainType.TT if all(s[2] == 1 for s in shapes) else TensorTrainType.TTM
+        else:
+            raise WrongTTFormat("All cores must be 3 or 4 rank tensors")
+        bonds, rows, cols, out = [1], [], [], []
+        for i, (c, s) in enumerate(zip(cores, shapes)):
+            if i == 0 and s[0] != 1:
+                raise WrongTTFormat(
+                    "Only open boundary conditions is supported, first core must has first index equal to 1")
+            if i == len(cores) - 1 and s[-1] != 1:
+                raise WrongTTFormat(
+                    "Only open boundary conditions is supported, last core must has last index equal to 1")
+            if s[0] != bonds[i]:
+                raise DimensionMismatch(f"Connected sites bonds are mismatching: {s[0]} != {bonds[i]}")
+            t = _as_core(c)
+            if len(s) == 3:
+                t = t.reshape(s[0], s[1], 1, s[2])
+            out.append(t)
+            rows.append(s[1])
+            cols.append(s[2] if tt_type == TensorTrainType.TTM else 1)
+            bonds.append(s[-1])
+        self.n_cores = len(out)
+        self.cores = out
+        self.max_rank = max(bonds)
+        self.orth_center = None
+        self.row_modes = rows
+        self.col_modes = cols
+        self.bonds = bonds
+        self.tt_type = tt_type
+
+    # -- bookkeeping
+    def __len__(self) -> int:
+        return self.n_cores
+
+    def __getitem__(self, idx):
+        return self.cores[idx]
+
+    def _sync_bonds(self) -> None:
+        self.bonds = [c.shape[0] for c in self.cores] + [1]
+        self.max_rank = max(self.bonds)
+
+    def is_ttm(self) -> bool:
+        return self.tt_type == TensorTrainType.TTM
+
+    def physical_dims(self):
+        if self.tt_type is TensorTrainType.TT:
+            return list(self.row_modes)
+        return list(zip(self.row_modes, self.col_modes))
+
+    def copy(self) -> "TensorTrain":
+        t = TensorTrain.__new__(TensorTrain)
+        t.n_cores = self.n_cores
+        t.cores = [c.clone() for c in self.cores]
+        t.max_rank = self.max_rank
+        t.orth_center = self.orth_center
+        t.row_modes = list(self.row_modes)
+        t.col_modes = list(self.col_modes)
+        t.bonds = list(self.bonds)
+        t.tt_type = self.tt_type
+        return t
+
+    def to_numpy(self) -> List[np.ndarray]:
+        """Host copies of the cores."""
+        return [c.detach().cpu().numpy() for c in self.cores]
+
+    # -- canonical forms
+    def orthonormalize(self, canonical: str = "left", center_position: Optional[int] = None,
+                       inplace: bool = False) -> "TensorTrain":
+        """QR sweep with the reference's centre bookkeeping (``tensor_train.py:705-787``)."""
+        d = self.n_cores
+        if center_position is not None and not 0 <= center_position < d:
+            raise ValueError(
+                f"center_position must be in [0, {d - 1}], but got {center_position}")
+        t = self if inplace else self.copy()
+        had = t.orth_center is not None
+        if canonical == "left":
+            end = center_position if center_position is not None else d - 1
+            start = 0
+            if t.orth_center is not None:
+                start = t.orth_center
+                t.orth_center = end if start <= end else t.orth_center
+            if start >= end:
+                return t
+            qr_sweep_left(t.cores, start, end)
+        elif canonical == "right":
+            end = center_position if center_position is not None else 0
+            start = d - 1
+            if t.orth_center is not None:
+                start = t.orth_center
+                t.orth_center = end if start >= end else t.orth_center
+            if start <= end:
+                return t
+            qr_sweep_right(t.cores, start, end)
+        elif canonical == "mix":
+            if center_position is None:
+                center_position = d // 2
+            end = center_position
+            qr_sweep_left(t.cores, 0, center_position)
+            qr_sweep_right(t.cores, d - 1, center_position)
+        else:
+            raise ValueError(f"Canonical form must be 'left', 'right' or 'mix', but got {canonical}")
+        t.orth_center = end if (had or canonical == "mix" or end in (0, d - 1)) else None
+        t._sync_bonds()
+        return t
+
+    def compress_svd(self, error: float = 0.0, max_rank: Optional[int] = None, canonical: str = "auto",
+                     inplace: bool = False, spectra=None) -> "TensorTrain":
+        """SVD sweep, direction from the centre (``tensor_train.py:789-850``)."""
+        t = self if inplace else self.copy()
+        if canonical == "auto":
+            canonical = "left" if (t.orth_center is None or t.orth_center <= (t.n_cores - 1) / 2) else "right"
+        if canonical == "right":
+            svd_sweep_right(t.cores, error, max_rank, spectra)
+            t.orth_center = 0
+        elif canonical == "left":
+            svd_sweep_left(t.cores, error, max_rank, spectra)
+            t.orth_center = t.n_cores - 1
+        else:
+            raise ValueError(f"Canonical form must be 'left', 'right' or 'auto', but got {canonical}")
+        t._sync_bonds()
+        return t
+
+    def truncate(self, error: float = 0.0, max_rank: Optional[int] = None, canonical: str = "auto",
+                 inplace: bool = False, method: str = "direct", seed: Optional[int] = None,
+                 options: Optional[dict] = None, spectra=None) -> "TensorTrain":
+        """Round to minimal ranks: QR sweep then SVD sweep (``tensor_train.py:852-940``)."""
+        if canonical not in ("auto", "left", "right"):
+            raise ValueError(f"Canonical form must be 'left', 'right' or 'auto', but got {canonical}")
+        if method not in ("direct", "randomized"):
+            raise ValueError(f"Method must be 'direct' or 'randomized', but got {method!r}")
+        if method == "randomized":
+            raise NotImplementedError("method='randomized' is not part of this build")
+        if max_rank is not None and max_rank <= 0:
+            raise ValueError("max_rank must be a positive integer.")
+        t = self if inplace else self.copy()
+        form = "left" if canonical == "right" else "right"
+        if canonical == "auto":
+            form = "left" if (t.orth_center is None or t.orth_center >= t.n_cores / 2) else "right"
+        t.orthonormalize(canonical=form, inplace=True)
+        t.compress_svd(error, max_rank, inplace=True, spectra=spectra)
+        return t
+
+    # -- arithmetic used by the residual check
+    def add(self, other: "TensorTrain", truncate: bool = False, error=None, max_rank=None,
+            method: str = "direct", seed=None, options=None) -> "TensorTrain":
+        """Block-placed sum (``tensor_train.py:254-322``)."""
+        if not isinstance(other, TensorTrain):
+            raise TypeError(f"unsupported operand type(s) for +: 'TensorTrain' and '{type(other)}'")
+        if self.physical_dims() != other.physical_dims():
+            raise DimensionMismatch("Tensor trains must have the same dimensions")
+        d = self.n_cores
+        cores = []
+        for i in range(d):
+            a, b = self.cores[i], other.cores[i]
+            L = 1 if i == 0 else a.shape[0] + b.shape[0]
+            R = 1 if i == d - 1 else a.shape[3] + b.shape[3]
+            nr, nc = a.shape[1], a.shape[2]
+            c = D.zeros(L, nr, nc, R)
+            st = [nr * nc * R, nc * R, R, 1]
+            D.copy_nd(list(a.shape), P(a), [nr * nc * a.shape[3], nc * a.shape[3], a.shape[3], 1], P(c), st)
+            off = (L - b.shape[0]) * nr * nc * R + (R - b.shape[3])
+            D.copy_nd(list(b.shape), P(b), [nr * nc * b.shape[3], nc * b.shape[3], b.shape[3], 1], P(c, off), st)
+            cores.append(c)
+        res = TensorTrain(cores)
+        return _round(res, truncate, error, max_rank, method)
+
+    def __add__(self, other):
+        return self.add(other)
+
+    def multiply(self, other, truncate: bool = False, error=None, max_rank=None, method="direct",
+                 seed=None, options=None) -> "TensorTrain":
+        """Scalar multiple on the centre core, or core 0 (``tensor_train.py:396-401``)."""
+        if isinstance(other, TensorTrain):
+            raise NotImplementedError("the Hadamard product of two trains is not part of this build")
+        if not isinstance(other, numbers.Number):
+            raise TypeError(f"unsupported operand type(s) for *: 'TensorTrain' and '{type(other)}'")
+        res = self.copy()
+        k = res.orth_center if res.orth_center is not None else 0
+        c = res.cores[k]
+        D.copy_nd([c.numel()], P(c), [1], P(c), [1], float(other))
+        return _round(res, truncate, error, max_rank, method)
+
+    def __mul__(self, other):
+        return self.multiply(other)
+
+    def __rmul__(self, other):
+        return self.multiply(other)
+
+    def __neg__(self):
+        return self.multiply(-1)
+
+    def __sub__(self, other):
+        return self.add(-other)
+
+    def dot(self, other: "TensorTrain", truncate: bool = False, error=None, max_rank=None,
+            method="direct", seed=None, options=None) -> "TensorTrain":
+        """TTM @ TT cores ``(Ra r, n, 1, Ra' r')`` (``tensor_train.py:427-499``)."""
+        if not isinstance(other, TensorTrain):
+            raise TypeError(f"unsupported operand type(s) for @: 'TensorTrain' and '{type(other)}'")
+        if self.col_modes != other.row_modes:
+            raise DimensionMismatch(
+                "Tensor trains must have the same last and first physical dimensions respectively")
+        if other.is_ttm():
+            raise NotImplementedError("TTM @ TTM is not part of this build")
+        cores = []
+        for a, x in zip(self.cores, other.cores):
+            Ra, nr, _, Rb = a.shape
+            rx, _, _, rx2 = x.shape
+            c = D.empty(Ra * rx, nr, 1, Rb * rx2)
+            D.ttm_core(a, x, P(c), nr * Rb * rx2, Rb * rx2, 1)
+            cores.append(c)
+        return _round(TensorTrain(cores), truncate, error, max_rank, method)
+
+    def __matmul__(self, other):
+        return self.dot(other)
+
+    def tt_norm_device(self) -> torch.Tensor:
+        """Frobenius norm as a device scalar (``tensor_train.py:509-540``)."""
+        if self.orth_center is not None:
+            return D.norm(self.cores[self.orth_center])
+        cores = [c.reshape(c.shape[0], c.shape[1] * c.shape[2], 1, c.shape[3]) for c in self.cores]
+        cores = list(cores)
+        qr_sweep_right(cores, len(cores) - 1, 0, form_q=False)
+        return D.norm(cores[0])
+
+    def tt_norm(self) -> float:
+        return float(self.tt_norm_device().item())
+
+    def get_full_tensor(self) -> np.ndarray:
+        """Dense tensor of a small train, contracted on the device (test helper)."""
+        t = self.cores[0].reshape(-1, self.cores[0].shape[3])
+        for c in self.cores[1:]:
+            m, r = t.shape
+            rest = c.numel() // c.shape[0]
+            out = D.empty(m, rest)
+            D.gemm(1, m, rest, r, 1.0, P(t), r, 1, 0, P(c), rest, 1, 0, 0.0, P(out), rest, 1, 0)
+            t = out.reshape(-1, c.shape[3])
+        shape = []
+        for r, c in zip(self.row_modes, self.col_modes):
+            shape += [r] if not self.is_ttm() else [r, c]
+        return t.reshape(shape).cpu().numpy()
+
+
+def _round(res: TensorTrain, truncate: bool, error, max_rank, method) -> TensorTrain:
+    """``_round_result`` (``tensor_train.py:94-119``)."""
+    if not truncate:
+        if method != "direct" or error is not None or max_rank is not None:
+            raise ValueError("error, max_rank and method need truncate=True")
+        return res
+    return res.truncate(error=0.0 if error is None else error, max_rank=max_rank, inplace=True, method=method)
+
+
+def _ranks(d: int, ranks) -> List[int]:
+    if not isinstance(ranks, list):
+        return [1] + [int(ranks)] * (d - 1) + [1]
+    return list(ranks)
+
+
+def ones(physical_indexes, ranks=1) -> TensorTrain:
+    """All-ones train (``tensor_train.py:1250-1285``)."""
+    if len(physical_indexes) < 2:
+        raise WrongTTFormat("Tensor train must have at least two cores.")
+    r = _ranks(len(physical_indexes), ranks)
+    cores = []
+    for i, p in enumerate(physical_indexes):
+        nr, nc = (p, 1) if isinstance(p, (int, np.integer)) else (p[0], p[1])
+        c = D.empty(r[i], nr, nc, r[i + 1])
+        D.fill(c, 1.0)
+        cores.append(c)
+    return TensorTrain(cores)
+
+
+def zeros(physical_indexes, ranks=1) -> TensorTrain:
+    """All-zeros train (``tensor_train.py:1288-1318``)."""
+    t = ones(physical_indexes, ranks)
+    for c in t.cores:
+        D.fill(c, 0.0)
+    return t
+
+
+def eye(physical_indexes) -> TensorTrain:
+    """Identity TTM of bond 1 (``tensor_train.py:1321-1335``)."""
+    if len(physical_indexes) < 2:
+        raise WrongTTFormat("Tensor train must have at least two cores.")
+    return TensorTrain([np.eye(p).reshape(1, p, p, 1) for p in physical_indexes])
+
+
+def rand(physical_indexes, ranks=1, seed=None) -> TensorTrain:
+    """Uniform [0, 1) cores in the reference's draw order (``tensor_train.py:1350-1387``).
+
+    The numbers come from NumPy's PCG64 ``default_rng(seed)`` on the host (the
+    same generator the reference calls), then live on the device.
+    """
+    if len(physical_indexes) < 2:
+        raise WrongTTFormat("Tensor train must have at least two cores.")
+    rng = np.random.default_rng(seed)
+    r = _ranks(len(physical_indexes), ranks)
+    cores = []
+    for i, p in enumerate(physical_indexes):
+        shape = (r[i], p, 1, r[i + 1]) if isinstance(p, (int, np.integer)) else (r[i], p[0], p[1], r[i + 1])
+        cores.append(rng.random(shape))
+    return TensorTrain(cores)
