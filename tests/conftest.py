@@ -3,7 +3,13 @@
 import os
 import sys
 
+# one BLAS thread for the oracle (see the note at the end of this file); set
+# before NumPy/SciPy load their OpenBLAS builds
+os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+os.environ.setdefault("OMP_NUM_THREADS", "1")
+
 import numpy as np
+import scipy.linalg  # noqa: F401  (loads SciPy's own OpenBLAS before the limit below)
 import pytest
 
 ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
