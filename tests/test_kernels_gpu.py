@@ -16,8 +16,14 @@ def _dev():
     return D
 
 
+_KEEP = []
+
+
 def _t(a):
-    return _dev().to_device(a)
+    """Device copy that stays alive for the whole module (kernels read it after enqueue)."""
+    t = _dev().to_device(a)
+    _KEEP.append(t)
+    return t
 
 
 def _h(t):
@@ -172,8 +178,13 @@ def test_minres_matches_scipy():
     x0 = rng.standard_normal(n)
     op = D.ExplicitOp(_t(M.T.reshape(-1)), n)  # column-major storage
     for rtol in (1e-4, 1e-8, 1e-10):
-        ref, _ = spla.minres(M, b, x0=x0, rtol=rtol)
-        got = _h(minres_device(op, _t(b), _t(x0), rtol))
+        its = []
+        ref, _ = spla.minres(M, b, x0=x0, rtol=rtol, callback=lambda xk: its.append(1))
+        st = []
+        got = _h(minres_device(op, _t(b), _t(x0), rtol, state_out=st))
+        print(f"rtol {rtol}: scipy iterations {len(its)}, device iterations {int(st[0][1])} "
+              f"istop {int(st[0][0])}", flush=True)
+        assert int(st[0][1]) == len(its)
         np.testing.assert_allclose(got, ref, rtol=1e-7, atol=1e-9 * np.abs(ref).max())
 
 

@@ -27,7 +27,7 @@ struct GemmArgs {
 // Operand layouts of v_mfma_f64_16x16x4f64 (wave64):
 //   A (16x4): lane l holds A[l % 16][l / 16]
 //   B (4x16): lane l holds B[l / 16][l % 16]
-//   D (16x16): lane l, element i holds D[4 * (l / 16) + i][l % 16]
+//   D (16x16): lane l, element i holds D[4 * i + l / 16][l % 16]
 constexpr int GBM = 64, GBN = 64, GBK = 16;
 
 __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs p) {
@@ -84,14 +84,14 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs p) {
   }
 
   const int col = lane & 15;
-  const int rq = 4 * (lane >> 4);
+  const int rq = lane >> 4;
   double4_t* accs[4] = {&acc00, &acc01, &acc10, &acc11};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int64_t gn = n0 + wn + 16 * (q & 1) + col;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int64_t gm = m0 + wm + 16 * (q >> 1) + rq + i;
+      const int64_t gm = m0 + wm + 16 * (q >> 1) + 4 * i + rq;
       if (gm < p.M && gn < p.N) {
         double* dst = C + gm * p.c_ms + gn * p.c_ns;
         const double v = p.alpha * (*accs[q])[i];

@@ -344,7 +344,8 @@ def _validate_solver_inputs(A: TensorTrain, b: TensorTrain, x0: TensorTrain) -> 
 _POLL = 16  # MINRES steps queued between polls of the device stop flag
 
 
-def minres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float) -> torch.Tensor:
+def minres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float,
+                  state_out: Optional[list] = None) -> torch.Tensor:
     """``scipy.sparse.linalg.minres(op, b, x0=x0, rtol=rtol)`` on the device.
 
     The recurrence and stopping tests run in ``ttx_minres_step_f64``; the
@@ -374,6 +375,8 @@ def minres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float) 
             op.apply(P(vec, n), 1, P(yA))
             _lib.check(L.ttx_minres_step_f64(s, n, P(yA), P(vec), P(st)), "ttx_minres_step_f64")
         done += chunk
+    if state_out is not None:
+        state_out.append(st.cpu().numpy())
     return vec[:n].clone()
 
 
