@@ -105,14 +105,14 @@ int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n,
 
 /*
  * Solve M u = b for a dense n x n system by LU with partial pivoting.
- * `Mb` holds [M | b] column-major (n rows, n+1 columns, leading dim n) and is
- * destroyed; `u` receives the solution.  scipy.linalg.solve at
+ * `Mb` holds [M | b] row-major (n rows, n+1 columns, row stride ld >= n+1)
+ * and is destroyed; `u` receives the solution.  scipy.linalg.solve at
  * ttnumpy/solvers/common.py:767,802 (getrf/getrs).  info[0]: TTX_OK or
  * TTX_ESINGULAR.  Work: ttx_lu_work_size(n) doubles.
  */
 int64_t ttx_lu_work_size(int64_t n);
-int ttx_lu_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, double* u, double* work,
-                     int32_t* info);
+int ttx_lu_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, int64_t ld, double* u,
+                     double* work, int32_t* info);
 
 /*
  * MINRES (the recurrences and stopping tests of scipy.sparse.linalg.minres,
@@ -154,14 +154,15 @@ int ttx_svd_terms_f64(ttx_stream_t stream, int64_t p, int64_t q, int64_t limit,
                       const double* Vt, int64_t v_rs, int64_t v_cs, double* terms);
 
 /*
- * Explicit one-site local matrix, column-major into `out` (leading dim ld):
+ * Explicit one-site local matrix into out[row*o_rs + col*o_cs]:
  *   M[(f_l,i,f_r), (t_l,j,t_r)] = sum_{a,b} psi[t_l,a,f_l] A[a,i,j,b] phi[t_r,b,f_r]
  * psi (rl, Ra, fl), A (Ra, nr, nc, Rb), phi (rr, Rb, fr), all contiguous.
  * ttnumpy/solvers/one_site.py:49-75 (einsum + transpose).
  */
 int ttx_local_matrix_f64(ttx_stream_t stream, const double* psi, int64_t rl, int64_t Ra,
                          int64_t fl, const double* A, int64_t nr, int64_t nc, int64_t Rb,
-                         const double* phi, int64_t rr, int64_t fr, double* out, int64_t ld);
+                         const double* phi, int64_t rr, int64_t fr, double* out, int64_t o_rs,
+                         int64_t o_cs);
 
 /*
  * One core of the TTM @ TT product (ttnumpy/tensor_train.py:483-495):

@@ -63,8 +63,8 @@ def test_environments_and_local_ops_match_oracle():
         op = D.LocalOp(eL[k], Ad.cores[k], eR[k])
         Mo = O.local_matrix(oL[k], Ao.cores[k], oR[k])
         M = D.empty(op.rows * op.cols)
-        op.explicit(M, op.rows)
-        np.testing.assert_allclose(_h(M).reshape(op.cols, op.rows).T, Mo, rtol=1e-12,
+        op.explicit(M, op.cols)
+        np.testing.assert_allclose(_h(M).reshape(op.rows, op.cols), Mo, rtol=1e-12,
                                    atol=1e-12 * np.abs(Mo).max())
         v = np.random.default_rng(k).standard_normal((3, op.cols))
         out = D.empty(3 * op.rows)

@@ -42,7 +42,7 @@ _SIGNATURES = {
     "ttx_svd_f64": (_i32, [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _i64,
                            _p, _p]),
     "ttx_lu_work_size": (_i64, [_i64]),
-    "ttx_lu_solve_f64": (_i32, [_p, _i64, _p, _p, _p, _p]),
+    "ttx_lu_solve_f64": (_i32, [_p, _i64, _p, _i64, _p, _p, _p]),
     "ttx_minres_init_f64": (_i32, [_p, _i64, _p, _p, _p, _f64, _i64, _p, _p]),
     "ttx_minres_step_f64": (_i32, [_p, _i64, _p, _p, _p]),
     "ttx_truncated_bond_f64": (_i32, [_p, _p, _i64, _f64, _i64, _p]),
@@ -50,7 +50,7 @@ _SIGNATURES = {
     "ttx_rank_by_residual_f64": (_i32, [_p, _i64, _i64, _p, _p, _p, _p, _f64, _f64, _p, _p]),
     "ttx_svd_terms_f64": (_i32, [_p, _i64, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _i64, _p]),
     "ttx_local_matrix_f64": (_i32, [_p, _p, _i64, _i64, _i64, _p, _i64, _i64, _i64, _p, _i64,
-                                    _i64, _p, _i64]),
+                                    _i64, _p, _i64, _i64]),
     "ttx_ttm_core_f64": (_i32, [_p, _p, _i64, _i64, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64,
                                 _i64]),
     "ttx_scale_cols_f64": (_i32, [_p, _i64, _i64, _p, _i64, _i64, _p]),

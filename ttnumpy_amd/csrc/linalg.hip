@@ -103,10 +103,14 @@ struct QrArgs {
   double* work;
 };
 
+// IN_LDS: the working matrix, tau and Q live in dynamic LDS (small cores);
+// otherwise in the caller's global workspace.
+template <bool IN_LDS>
 __global__ __launch_bounds__(1024) void qr_kernel(QrArgs p) {
+  extern __shared__ double smem[];
   __shared__ double red[16];
   const int64_t m = p.m, n = p.n, k = m < n ? m : n;
-  double* W = p.work;
+  double* W = IN_LDS ? smem : p.work;
   double* tau = W + m * n;
   double* Qw = tau + k;
   for (int64_t e = threadIdx.x; e < m * n; e += blockDim.x) {
@@ -115,6 +119,7 @@ __global__ __launch_bounds__(1024) void qr_kernel(QrArgs p) {
   }
   __syncthreads();
   block_householder_qr(m, n, W, tau, Qw, red, p.Q != nullptr);
+  __syncthreads();
   if (p.Q) {
     for (int64_t e = threadIdx.x; e < m * k; e += blockDim.x) {
       const int64_t i = e % m, c = e / m;
@@ -147,7 +152,9 @@ __device__ __forceinline__ int64_t rr_player(int64_t t, int64_t round, int64_t q
   return t == 0 ? 0 : 1 + (t - 1 + round) % (qq - 1);
 }
 
+template <bool IN_LDS>
 __global__ __launch_bounds__(1024) void svd_kernel(SvdArgs p) {
+  extern __shared__ double smem[];
   __shared__ double red[16];
   __shared__ int s_rot;
   __shared__ int s_bad;
@@ -161,7 +168,7 @@ __global__ __launch_bounds__(1024) void svd_kernel(SvdArgs p) {
   const int64_t Q = wide ? p.m : p.n;
   const int64_t x_rs = wide ? p.a_cs : p.a_rs;
   const int64_t x_cs = wide ? p.a_rs : p.a_cs;
-  double* Wq = p.work;            // P x Q   (QR working matrix)
+  double* Wq = IN_LDS ? smem : p.work;  // P x Q   (QR working matrix)
   double* tau = Wq + P * Q;       // Q
   double* Qx = tau + Q;           // P x Q   (orthonormal factor)
   double* G = Qx + P * Q;         // Q x Q   (Jacobi matrix)
@@ -250,6 +257,7 @@ __global__ __launch_bounds__(1024) void svd_kernel(SvdArgs p) {
     __syncthreads();
   }
   if (tid == 0) p.info[0] = converged ? TTX_OK : TTX_ENOCONV;
+  __syncthreads();
 
   // singular values = column norms
   for (int64_t j = wave; j < Q; j += nwave) {
@@ -312,27 +320,31 @@ __global__ __launch_bounds__(1024) void svd_kernel(SvdArgs p) {
 // LU with partial pivoting on [M | b], column-major, leading dim n, n+1 cols.
 // ---------------------------------------------------------------------------
 struct PanelArgs {
-  int64_t n, k, nb;
-  double* W;
+  int64_t n, ld, k, nb;
+  double* W;  // row-major [M | b], row stride ld >= n + 1
   int32_t* piv;
   int32_t* info;
 };
 
+// One panel of the right-looking LU: factor rows k.. of columns k..k+nb-1 in
+// LDS with partial pivoting, apply the row swaps to every other column and
+// solve the U12 block row.  Row-major storage makes the swaps and the
+// triangular solve coalesced: consecutive threads own consecutive columns.
 __global__ __launch_bounds__(512) void lu_panel_kernel(PanelArgs p) {
   extern __shared__ double P[];  // nb columns x mp rows, column-major
   __shared__ double rv[8];
   __shared__ int64_t ri[8];
-  __shared__ int64_t s_piv;
+  __shared__ int64_t s_piv[16];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int nwave = blockDim.x >> 6;
-  const int64_t n = p.n, k = p.k, nb = p.nb, ncols = n + 1;
+  const int64_t n = p.n, ld = p.ld, k = p.k, nb = p.nb, ncols = n + 1;
   const int64_t mp = n - k;
   double* W = p.W;
   for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
-    const int64_t r = e % mp, c = e / mp;
-    P[e] = W[(k + r) + (k + c) * n];
+    const int64_t c = e % nb, r = e / nb;
+    P[c * mp + r] = W[(k + r) * ld + (k + c)];
   }
   __syncthreads();
   for (int64_t j = 0; j < nb; ++j) {
@@ -357,12 +369,12 @@ __global__ __launch_bounds__(512) void lu_panel_kernel(PanelArgs p) {
       int64_t i0 = ri[0];
       for (int w = 1; w < nwave; ++w)
         if (rv[w] > b0 || (rv[w] == b0 && ri[w] < i0)) { b0 = rv[w]; i0 = ri[w]; }
-      s_piv = i0;
+      s_piv[j] = i0;
       p.piv[k + j] = (int32_t)(k + i0);
       if (b0 == 0.0) p.info[0] = TTX_ESINGULAR;
     }
     __syncthreads();
-    const int64_t pr = s_piv;
+    const int64_t pr = s_piv[j];
     if (pr != j) {
       for (int64_t c = tid; c < nb; c += blockDim.x) {
         double* pc = P + c * mp;
@@ -385,59 +397,64 @@ __global__ __launch_bounds__(512) void lu_panel_kernel(PanelArgs p) {
     __syncthreads();
   }
   for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
-    const int64_t r = e % mp, c = e / mp;
-    W[(k + r) + (k + c) * n] = P[e];
+    const int64_t c = e % nb, r = e / nb;
+    W[(k + r) * ld + (k + c)] = P[c * mp + r];
   }
-  // row swaps on every column outside the panel, then U12 = L11^-1 A12 on the
-  // trailing columns; one thread owns a column, so no barrier is needed
   for (int64_t c = tid; c < ncols; c += blockDim.x) {
     if (c >= k && c < k + nb) continue;
-    double* wc = W + c * n;
     for (int64_t j = 0; j < nb; ++j) {
-      const int64_t pr = (int64_t)p.piv[k + j];
+      const int64_t pr = k + s_piv[j];
       if (pr != k + j) {
-        const double t = wc[k + j];
-        wc[k + j] = wc[pr];
-        wc[pr] = t;
+        const double t = W[(k + j) * ld + c];
+        W[(k + j) * ld + c] = W[pr * ld + c];
+        W[pr * ld + c] = t;
       }
     }
     if (c >= k + nb) {
       for (int64_t i = 1; i < nb; ++i) {
-        double s = wc[k + i];
-        for (int64_t l = 0; l < i; ++l) s -= P[l * mp + i] * wc[k + l];
-        wc[k + i] = s;
+        double s = W[(k + i) * ld + c];
+        for (int64_t l = 0; l < i; ++l) s -= P[l * mp + i] * W[(k + l) * ld + c];
+        W[(k + i) * ld + c] = s;
       }
     }
   }
 }
 
-__global__ __launch_bounds__(1024) void lu_backsub_kernel(int64_t n, const double* W, double* u,
-                                                         int32_t* info) {
+// U x = y with y the last column (n) of the row-major factor; 64-row blocks,
+// the diagonal block staged in LDS and solved by one wave with shuffles.
+__global__ __launch_bounds__(1024) void lu_backsub_kernel(int64_t n, int64_t ld, const double* W,
+                                                         double* u) {
+  __shared__ double T[64][65];
   __shared__ double xb[64];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const double* y = W + n * n;  // column n
-  for (int64_t i = tid; i < n; i += blockDim.x) u[i] = y[i];
+  for (int64_t i = tid; i < n; i += blockDim.x) u[i] = W[i * ld + n];
   __syncthreads();
   for (int64_t j1 = n; j1 > 0; j1 -= 64) {
     const int64_t j0 = j1 - 64 > 0 ? j1 - 64 : 0;
+    const int64_t nbk = j1 - j0;
+    for (int64_t e = tid; e < nbk * nbk; e += blockDim.x) {
+      const int64_t r = e / nbk, c = e % nbk;
+      T[r][c] = W[(j0 + r) * ld + j0 + c];
+    }
+    __syncthreads();
     if (wave == 0) {
       const int64_t row = j0 + lane;
       double yv = row < j1 ? u[row] : 0.0;
       for (int64_t j = j1 - 1; j >= j0; --j) {
-        const double ujj = W[j + j * n];
-        const double xj = __shfl(yv, (int)(j - j0), 64) / ujj;
+        const int jj = (int)(j - j0);
+        const double xj = __shfl(yv, jj, 64) / T[jj][jj];
         if (row == j) yv = xj;
-        if (row < j) yv -= xj * W[row + j * n];
+        if (row < j) yv -= xj * T[lane][jj];
       }
       if (row < j1) { u[row] = yv; xb[lane] = yv; }
     }
     __syncthreads();
-    const int64_t nbk = j1 - j0;
     for (int64_t r = tid; r < j0; r += blockDim.x) {
       double s = u[r];
-      for (int64_t c = 0; c < nbk; ++c) s -= W[r + (j0 + c) * n] * xb[c];
+      const double* wr = W + r * ld + j0;
+      for (int64_t c = 0; c < nbk; ++c) s -= wr[c] * xb[c];
       u[r] = s;
     }
     __syncthreads();
@@ -529,12 +546,30 @@ extern "C" int64_t ttx_qr_work_size(int64_t m, int64_t n) {
   return m * n + k + m * k;
 }
 
+constexpr size_t kLdsLimit = 150 * 1024;
+
+static int set_lds_attr(const void* fn) {
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit) ==
+                 hipSuccess
+             ? TTX_OK
+             : TTX_EHIP;
+}
+
 extern "C" int ttx_qr_f64(ttx_stream_t stream, int64_t m, int64_t n, const double* A, int64_t a_rs,
                           int64_t a_cs, double* Q, int64_t q_rs, int64_t q_cs, double* R,
                           int64_t r_rs, int64_t r_cs, double* work) {
   if (m < 1 || n < 1) return TTX_EINVAL;
+  static bool attr = false;
+  if (!attr) {
+    if (set_lds_attr((const void*)qr_kernel<true>) != TTX_OK) return TTX_EHIP;
+    attr = true;
+  }
   QrArgs p{m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work};
-  hipLaunchKernelGGL(qr_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, p);
+  const size_t bytes = (size_t)ttx_qr_work_size(m, n) * sizeof(double);
+  if (bytes <= kLdsLimit)
+    hipLaunchKernelGGL(qr_kernel<true>, dim3(1), dim3(1024), bytes, (hipStream_t)stream, p);
+  else
+    hipLaunchKernelGGL(qr_kernel<false>, dim3(1), dim3(1024), 0, (hipStream_t)stream, p);
   return launch_status();
 }
 
@@ -548,50 +583,58 @@ extern "C" int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n, const doub
                            double* S, double* Vt, int64_t v_rs, int64_t v_cs, double* work,
                            int32_t* info) {
   if (m < 1 || n < 1) return TTX_EINVAL;
+  static bool attr = false;
+  if (!attr) {
+    if (set_lds_attr((const void*)svd_kernel<true>) != TTX_OK) return TTX_EHIP;
+    attr = true;
+  }
   SvdArgs p{m, n, A, a_rs, a_cs, U, u_rs, u_cs, S, Vt, v_rs, v_cs, work, info};
-  hipLaunchKernelGGL(svd_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, p);
+  const size_t bytes = (size_t)ttx_svd_work_size(m, n) * sizeof(double);
+  if (bytes <= kLdsLimit)
+    hipLaunchKernelGGL(svd_kernel<true>, dim3(1), dim3(1024), bytes, (hipStream_t)stream, p);
+  else
+    hipLaunchKernelGGL(svd_kernel<false>, dim3(1), dim3(1024), 0, (hipStream_t)stream, p);
   return launch_status();
 }
 
 static int64_t lu_block(int64_t n) {
-  // the panel (n rows x nb cols) is staged in LDS; keep it under 144 KiB
-  int64_t nb = (144 * 1024) / (8 * (n > 0 ? n : 1));
+  // the panel (n rows x nb cols) is staged in LDS
+  int64_t nb = (int64_t)(kLdsLimit / (8 * (n > 0 ? n : 1)));
   if (nb > 16) nb = 16;
   return nb;
 }
 
 extern "C" int64_t ttx_lu_work_size(int64_t n) { return (n + 1) / 2 + 1; /* pivots as int32 */ }
 
-extern "C" int ttx_lu_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, double* u, double* work,
-                                int32_t* info) {
-  if (n < 1) return TTX_EINVAL;
+extern "C" int ttx_lu_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, int64_t ld, double* u,
+                                double* work, int32_t* info) {
+  if (n < 1 || ld < n + 1) return TTX_EINVAL;
   const int64_t nb0 = lu_block(n);
   if (nb0 < 1) return TTX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
   static bool attr_set = false;
   if (!attr_set) {
-    if (hipFuncSetAttribute((const void*)lu_panel_kernel,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024) != hipSuccess)
-      return TTX_EHIP;
+    if (set_lds_attr((const void*)lu_panel_kernel) != TTX_OK) return TTX_EHIP;
     attr_set = true;
   }
   int32_t* piv = reinterpret_cast<int32_t*>(work);
   hipLaunchKernelGGL(lu_info_init_kernel, dim3(1), dim3(1), 0, st, info);
   for (int64_t k = 0; k < n; k += nb0) {
     const int64_t nb = (n - k) < nb0 ? (n - k) : nb0;
-    PanelArgs pa{n, k, nb, Mb, piv, info};
+    PanelArgs pa{n, ld, k, nb, Mb, piv, info};
     const size_t shm = (size_t)(nb * (n - k)) * sizeof(double);
     hipLaunchKernelGGL(lu_panel_kernel, dim3(1), dim3(512), shm, st, pa);
     const int64_t rows = n - k - nb;
     const int64_t cols = n + 1 - k - nb;
     if (rows > 0 && cols > 0) {
-      int rc = ttx_gemm_f64(stream, 1, rows, cols, nb, -1.0, Mb + (k + nb) + k * n, 1, n, 0,
-                            Mb + k + (k + nb) * n, 1, n, 0, 1.0, Mb + (k + nb) + (k + nb) * n, 1,
-                            n, 0);
+      // A22 -= L21 * U12, all row-major with row stride ld
+      int rc = ttx_gemm_f64(stream, 1, rows, cols, nb, -1.0, Mb + (k + nb) * ld + k, ld, 1, 0,
+                            Mb + k * ld + (k + nb), ld, 1, 0, 1.0, Mb + (k + nb) * ld + (k + nb),
+                            ld, 1, 0);
       if (rc != TTX_OK) return rc;
     }
   }
-  hipLaunchKernelGGL(lu_backsub_kernel, dim3(1), dim3(1024), 0, st, n, Mb, u, info);
+  hipLaunchKernelGGL(lu_backsub_kernel, dim3(1), dim3(1024), 0, st, n, ld, Mb, u);
   return launch_status();
 }
 

@@ -20,8 +20,8 @@ struct LocalMatrixArgs {
   const double* A;    // (Ra, nr, nc, Rb)
   const double* phi;  // (rr, Rb, fr)
   int64_t rl, Ra, fl, nr, nc, Rb, rr, fr;
-  double* out;        // column-major, leading dim ld
-  int64_t ld;
+  double* out;
+  int64_t o_rs, o_cs;
 };
 
 __global__ __launch_bounds__(256) void local_matrix_kernel(LocalMatrixArgs p) {
@@ -30,7 +30,9 @@ __global__ __launch_bounds__(256) void local_matrix_kernel(LocalMatrixArgs p) {
   const int64_t total = rows * cols;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
-    const int64_t row = e % rows, col = e / rows;
+    // the index with the unit output stride varies fastest (coalesced stores)
+    int64_t row, col;
+    if (p.o_cs < p.o_rs) { col = e % cols; row = e / cols; } else { row = e % rows; col = e / rows; }
     const int64_t f_r = row % p.fr;
     const int64_t i = (row / p.fr) % p.nr;
     const int64_t f_l = row / (p.fr * p.nr);
@@ -46,7 +48,7 @@ __global__ __launch_bounds__(256) void local_matrix_kernel(LocalMatrixArgs p) {
       for (int64_t b = 0; b < p.Rb; ++b) t += arow[b] * p.phi[(t_r * p.Rb + b) * p.fr + f_r];
       s += ps * t;
     }
-    p.out[row + col * p.ld] = s;
+    p.out[row * p.o_rs + col * p.o_cs] = s;
   }
 }
 
@@ -132,10 +134,9 @@ using namespace ttx;
 extern "C" int ttx_local_matrix_f64(ttx_stream_t stream, const double* psi, int64_t rl, int64_t Ra,
                                     int64_t fl, const double* A, int64_t nr, int64_t nc,
                                     int64_t Rb, const double* phi, int64_t rr, int64_t fr,
-                                    double* out, int64_t ld) {
+                                    double* out, int64_t o_rs, int64_t o_cs) {
   if (rl < 1 || Ra < 1 || fl < 1 || nr < 1 || nc < 1 || Rb < 1 || rr < 1 || fr < 1) return TTX_EINVAL;
-  if (ld < fl * nr * fr) return TTX_EINVAL;
-  LocalMatrixArgs p{psi, A, phi, rl, Ra, fl, nr, nc, Rb, rr, fr, out, ld};
+  LocalMatrixArgs p{psi, A, phi, rl, Ra, fl, nr, nc, Rb, rr, fr, out, o_rs, o_cs};
   const int64_t total = fl * nr * fr * rl * nc * rr;
   hipLaunchKernelGGL(local_matrix_kernel, dim3(grid_of(total)), dim3(256), 0, (hipStream_t)stream, p);
   return launch_status();

@@ -38,7 +38,13 @@ def lib():
     return _lib.load()
 
 
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream() -> int:
+    """Raw handle of torch's current stream on the current device (all ttx work goes there)."""
+    if _RAW_STREAM is not None:
+        return _RAW_STREAM(device().index)
     return torch.cuda.current_stream().cuda_stream
 
 
@@ -190,22 +196,22 @@ class LocalOp:
              rl * Ra * nr * fr, beta, out, nr * fr, 1, fl * nr * fr)
 
     def explicit(self, out: torch.Tensor, ld: int) -> None:
-        """Write the explicit matrix column-major into ``out`` (``one_site.py:49-75``)."""
+        """Write the explicit matrix row-major (row stride ``ld``) into ``out`` (``one_site.py:49-75``)."""
         rc = lib().ttx_local_matrix_f64(stream(), P(self.psi), self.rl, self.Ra, self.fl, P(self.A),
                                         self.nr, self.nc, self.Rb, P(self.phi), self.rr, self.fr,
-                                        P(out), ld)
+                                        P(out), ld, 1)
         check(rc, "ttx_local_matrix_f64")
 
 
 class ExplicitOp:
-    """The explicit local matrix (column-major N x N) as an operator."""
+    """The explicit local matrix (row-major N x N) as an operator."""
 
     def __init__(self, M: torch.Tensor, n: int):
         self.M, self.rows, self.cols = M, n, n
 
     def apply(self, y: int, B: int, out: int, alpha: float = 1.0, beta: float = 0.0) -> None:
         n = self.rows
-        gemm(1, n, B, n, alpha, P(self.M), 1, n, 0, y, 1, n, 0, beta, out, 1, n, 0)
+        gemm(1, n, B, n, alpha, P(self.M), n, 1, 0, y, 1, n, 0, beta, out, 1, n, 0)
 
 
 def local_rhs(psi_b: torch.Tensor, b_core: torch.Tensor, phi_b: torch.Tensor, out: int) -> None:
@@ -218,10 +224,11 @@ def local_rhs(psi_b: torch.Tensor, b_core: torch.Tensor, phi_b: torch.Tensor, ou
     gemm(1, fl * n, fr, rt2, 1.0, P(s1), rt2, 1, 0, P(phi_b), fr, 1, 0, 0.0, out, fr, 1, 0)
 
 
-def lu_solve(Mb: torch.Tensor, n: int, u: torch.Tensor, info: torch.Tensor) -> None:
-    """Solve the [M | b] system in ``Mb`` (destroyed) into ``u`` (``common.py:767``)."""
+def lu_solve(Mb: torch.Tensor, n: int, u: torch.Tensor, info: torch.Tensor, ld: Optional[int] = None) -> None:
+    """Solve the row-major [M | b] system in ``Mb`` (destroyed) into ``u`` (``common.py:767``)."""
     work = torch.empty(int(lib().ttx_lu_work_size(n)) + 1, dtype=F64, device=device())
-    check(lib().ttx_lu_solve_f64(stream(), n, P(Mb), P(u), P(work), info.data_ptr()), "ttx_lu_solve_f64")
+    check(lib().ttx_lu_solve_f64(stream(), n, P(Mb), n + 1 if ld is None else ld, P(u), P(work),
+                                 info.data_ptr()), "ttx_lu_solve_f64")
 
 
 # ---------------------------------------------------------------- factorizations

@@ -421,8 +421,8 @@ def direct_solve(op_explicit, b_loc: torch.Tensor) -> torch.Tensor:
     """LU with partial pivoting on [M | b] (``common.py:766-774``)."""
     n = op_explicit.rows
     Mb = D.empty(n * (n + 1))
-    D.copy_nd([n * n], P(op_explicit.M), [1], P(Mb), [1])
-    D.copy_nd([n], P(b_loc), [1], P(Mb, n * n), [1])
+    D.copy_nd([n, n], P(op_explicit.M), [n, 1], P(Mb), [n + 1, 1])
+    D.copy_nd([n], P(b_loc), [1], P(Mb, n), [n + 1])
     u = D.empty(n)
     info = torch.zeros(1, dtype=torch.int32, device=D.device())
     D.lu_solve(Mb, n, u, info)
