@@ -1,0 +1,100 @@
+"""The batched engine against the oracle, and the same request alone vs in a mix.
+
+Every case prints its name before it runs.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, unpack_cores
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import tt_oracle as O  # noqa: E402
+from ttnumpy_amd import problems as P  # noqa: E402
+
+
+def _tt(cores):
+    import ttnumpy_amd as tx
+    return tx.TensorTrain([np.asarray(c) for c in cores])
+
+
+def _poisson_request(n_bits, seed=0, x0_seed=0, **kw):
+    import ttnumpy_amd as tx
+    from ttnumpy_amd.engine import Request
+    d = 2 * n_bits
+    opts = dict(symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=seed)
+    opts.update(kw)
+    return Request(A=_tt(P.poisson_2d_cores(n_bits)), b=_tt(P.ones_cores([2] * d)),
+                   x0=tx.rand([2] * d, ranks=2, seed=x0_seed), **opts)
+
+
+def _rel_diff(x, ref: O.Train):
+    xo = O.Train(x.to_numpy())
+    return O.tt_norm(O.tt_add(xo, O.tt_scale(ref, -1.0))) / O.tt_norm(ref)
+
+
+def test_engine_poisson16_matches_oracle():
+    """16 x 16 grid (d = 8): direct path only; converged, same sweeps as the oracle."""
+    from ttnumpy_amd.engine import amen_batched
+    print("engine poisson 16x16", flush=True)
+    req = _poisson_request(4)
+    (out,) = amen_batched([req], rmax=32, poll=4)
+    assert out.error is None, out.error
+    A = O.Train(P.poisson_2d_cores(4))
+    b = O.Train(P.ones_cores([2] * 8))
+    x0 = O.Train(P.random_train_cores([2] * 8, 2, 0))
+    xo, io = O.amen(A, b, x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0)
+    print(out.info, io, flush=True)
+    assert str(out.info.stop_reason) == "converged"
+    assert out.info.residuals[-1] <= 1e-8
+    assert out.info.num_sweeps_run == io.num_sweeps_run
+    assert out.info.residuals[0] == pytest.approx(io.residuals[0], rel=1e-8)
+    assert _rel_diff(out.result, xo) < 1e-7
+
+
+def test_engine_poisson64_minres_path():
+    """64 x 64 grid with direct_solve_size=16: the MINRES path of the engine."""
+    from ttnumpy_amd.engine import amen_batched
+    print("engine poisson 64x64 minres", flush=True)
+    g = load_golden("poisson64_amen_minres")
+    (out,) = amen_batched([_poisson_request(6, direct_solve_size=16)], rmax=40, poll=8)
+    assert out.error is None, out.error
+    print(out.info, flush=True)
+    assert str(out.info.stop_reason) == "converged"
+    assert out.info.residuals[-1] <= 1e-8
+    assert out.info.local_solves > 0
+    assert _rel_diff(out.result, O.Train(unpack_cores(g))) < 1e-6
+
+
+def test_engine_c1_poisson256():
+    """C1 on the engine: converged at 1e-8 and the reference's solution to 1e-5."""
+    from ttnumpy_amd.engine import amen_batched
+    print("engine C1 poisson 256x256", flush=True)
+    g = load_golden("c1_poisson256_amen")
+    (out,) = amen_batched([_poisson_request(8)], rmax=48, poll=16)
+    assert out.error is None, out.error
+    print(out.info, out.result.bonds, flush=True)
+    assert out.info.residuals[-1] <= 1e-8 * 1.05
+    assert abs(max(out.result.bonds) - max(g["bonds"])) <= 2
+    assert _rel_diff(out.result, O.Train(unpack_cores(g))) < 1e-5
+
+
+def test_engine_alone_equals_in_mix():
+    """A request's result is bit-identical alone and batched with different requests."""
+    from ttnumpy_amd.engine import Engine
+    print("engine alone vs mix", flush=True)
+    alone = Engine(1, rmax=40).run([_poisson_request(5, seed=1, x0_seed=1)], poll=4)[0]
+    mix_reqs = [_poisson_request(4, seed=2), _poisson_request(5, seed=1, x0_seed=1),
+                _poisson_request(6, seed=3, x0_seed=5), _poisson_request(5, seed=7, x0_seed=2)]
+    mix = Engine(3, rmax=40).run(mix_reqs, poll=4)
+    same = [r for r in mix if r is mix_reqs[1]][0]
+    assert alone.error is None and same.error is None
+    assert alone.info.residuals == same.info.residuals
+    assert alone.result.bonds == same.result.bonds
+    for a, b in zip(alone.result.to_numpy(), same.result.to_numpy()):
+        np.testing.assert_array_equal(a, b)
+    for r in mix:
+        assert r.error is None and r.info.residuals[-1] <= 1e-8

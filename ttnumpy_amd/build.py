@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["blas.hip", "linalg.hip", "krylov.hip", "tt_kernels.hip"]
+SOURCES = ["blas.hip", "linalg.hip", "krylov.hip", "tt_kernels.hip", "engine.hip"]
 LIB = os.path.join(HERE, "libttx.so")
 
 HIPCC_FLAGS = [

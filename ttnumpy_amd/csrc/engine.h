@@ -1,0 +1,70 @@
+// Slot descriptor of the batched AMEn engine (shared by the kernels and the
+// host planner).  Every field is 8 bytes wide so the host mirror of the
+// layout is the plain sequence of fields.
+#pragma once
+
+#include <stdint.h>
+
+#define TTX_MAXD 64
+#define TTX_MAXSWEEPS 64
+
+enum ttx_phase {
+  TTX_PH_XORTH = 1,     // right-orthonormalise x0, k = d-1 .. 1
+  TTX_PH_ZORTH = 2,     // right-orthonormalise the random z0, k = d-1 .. 1
+  TTX_PH_FIT_INIT = 3,  // right environments of the z fit, k = d-1 .. 0
+  TTX_PH_FIT_FWD = 4,   // z fit, forward, k = 0 .. d-2
+  TTX_PH_FIT_LAST = 5,  // left z environments at d-1
+  TTX_PH_FIT_BWD = 6,   // z fit, backward, k = d-1 .. 0
+  TTX_PH_INIT_R = 7,    // all right environments, k = d-1 .. 0
+  TTX_PH_FWD = 8,       // AMEn forward half-sweep, k = 0 .. d-2
+  TTX_PH_LAST = 9,      // left environments at d-1
+  TTX_PH_BWD = 10,      // AMEn backward half-sweep, k = d-1 .. 0
+  TTX_PH_CHECK = 11,    // per-sweep residual and stopping test
+  TTX_PH_RED_L = 12,    // redundant-bond reduction, left QR sweep
+  TTX_PH_RED_R = 13,    // redundant-bond reduction, right QR sweep
+  TTX_PH_DONE = 14
+};
+
+enum ttx_slot_status {
+  TTX_SLOT_RUNNING = 0,
+  TTX_SLOT_DONE = 1,
+  TTX_SLOT_ERR_CAPACITY = -1,  // a bond outgrew the slot's rank capacity
+  TTX_SLOT_ERR_NONFINITE = -2,
+  TTX_SLOT_ERR_SINGULAR = -3,
+  TTX_SLOT_EMPTY = -100
+};
+
+// scratch regions (indices into scr_off)
+enum {
+  TTX_SCR_U = 0, TTX_SCR_BLOC, TTX_SCR_TMP, TTX_SCR_ZNEW, TTX_SCR_MAT, TTX_SCR_LU, TTX_SCR_MR,
+  TTX_SCR_S1, TTX_SCR_S2, TTX_SCR_SVDW, TTX_SCR_USVD, TTX_SCR_SSVD, TTX_SCR_VSVD, TTX_SCR_ENR,
+  TTX_SCR_CHK, TTX_SCR_ENVS, TTX_SCR_SCAL, TTX_SCR_COUNT
+};
+
+typedef struct ttx_slot {
+  // ---- problem (host-written)
+  int64_t d, kick, max_rank, direct_size, iterative, residual_mode, max_sweeps, track;
+  int64_t rmax, nmax, ramax, rbmax, bneg, allow_fallback, chunk, pad0;
+  int64_t n[TTX_MAXD];
+  int64_t ra[TTX_MAXD + 1];
+  int64_t rb[TTX_MAXD + 1];
+  double residual_tol, trunc_tol, local_rtol, damp, error_threshold, stagnation, bnorm, pad1;
+  // ---- layout (planner-written, doubles)
+  int64_t cap_core, cap_zcore, cap_loc, direct_cap, lmax, scr_size, env_size, pad2;
+  int64_t cap_env[5], pad3[3];
+  int64_t env_off[10], pad4[6];
+  int64_t scr_off[TTX_SCR_COUNT], pad5[32 - TTX_SCR_COUNT];
+  const double* A[TTX_MAXD];
+  const double* b[TTX_MAXD];
+  double* x;
+  double* z;
+  double* env;
+  double* scr;
+  // ---- state (device-written, host-initialised)
+  int64_t rx[TTX_MAXD + 1];
+  int64_t rz[TTX_MAXD + 1];
+  int64_t phase, k, sweep, status, stop_reason, local_solves, local_inexact, local_fallbacks;
+  int64_t rank_decisions, rank_capped, red_last, svd_noconv, ticks, pad6[3];
+  double residuals[TTX_MAXSWEEPS];
+  int64_t max_bonds[TTX_MAXSWEEPS];
+} ttx_slot;

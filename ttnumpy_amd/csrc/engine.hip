@@ -1,0 +1,994 @@
+// Batched, device-resident AMEn engine.
+//
+// Every request ("slot") owns its trains, environments and scratch in HBM and
+// a small descriptor (engine.h).  A tick launches four kernels over all slots,
+// one workgroup per slot:
+//
+//   env   : the environment updates of the slot's current site (grid slots x 5,
+//           one workgroup per environment kind);
+//   solve : the one-site local solve (LU on the explicit matrix, or MINRES
+//           with the true-residual restart loop);
+//   post  : residual core, SVD truncation with the residual-budget rank rule,
+//           enrichment, QR centre shifts (or the QR sweeps of set-up and the
+//           residual-train fit);
+//   check : the per-sweep relative residual in TT arithmetic and the stopping
+//           test, then the advance of the slot's phase machine.
+//
+// All rank decisions and the stopping test run on the device, so the host
+// only replays ticks and polls the slot status; requests at different sites,
+// sweeps and ranks share every launch (continuous batching).  The algorithm
+// and its order of operations are those of ttnumpy/solvers/amen.py:395-725
+// with the helpers of one_site.py and common.py; each phase below cites the
+// reference lines it follows.
+
+#include "block_ops.h"
+#include "engine.h"
+
+namespace ttx {
+
+constexpr int kEngThreads = 512;
+constexpr size_t kEngLds = 150 * 1024;
+
+__device__ __forceinline__ double* xcore(const ttx_slot* s, int64_t k) { return s->x + k * s->cap_core; }
+__device__ __forceinline__ double* zcore(const ttx_slot* s, int64_t k) { return s->z + k * s->cap_zcore; }
+__device__ __forceinline__ double* envp(const ttx_slot* s, int kind, int side, int64_t k) {
+  return s->env + s->env_off[kind * 2 + side] + k * s->cap_env[kind];
+}
+__device__ __forceinline__ double* scr(const ttx_slot* s, int id) { return s->scr + s->scr_off[id]; }
+
+__device__ __forceinline__ void bfill(double* p, int64_t n, double v) {
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+__device__ __forceinline__ void bcopy(double* dst, const double* src, int64_t n) {
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+__device__ __forceinline__ double bnorm2(const double* a, const double* b, int64_t n, double* red) {
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = b ? a[i] - b[i] : a[i];
+    s += v * v;
+  }
+  return sqrt(block_sum(s, red));
+}
+
+// ---------------------------------------------------------------- contractions
+// left environment step (common.py:170-188): out from E with frame core F
+// (rf, nr, rf2), target core T (rt, nc, rt2), operator core A (Ra, nr, nc, Ra2)
+__device__ void env_left_blk(const double* E, const double* F, int64_t rf, int64_t nr, int64_t rf2,
+                             const double* T, int64_t rt, int64_t nc, int64_t rt2, const double* A,
+                             int64_t Ra, int64_t Ra2, double* out, double* s1, double* s2) {
+  if (!A) {
+    bgemm(1, rt, nr * rf2, rf, 1.0, E, rf, 1, 0, F, nr * rf2, 1, 0, 0.0, s1, nr * rf2, 1, 0);
+    __syncthreads();
+    bgemm(1, rt2, rf2, rt * nc, 1.0, T, 1, rt2, 0, s1, rf2, 1, 0, 0.0, out, rf2, 1, 0);
+    return;
+  }
+  bgemm(1, rt * Ra, nr * rf2, rf, 1.0, E, rf, 1, 0, F, nr * rf2, 1, 0, 0.0, s1, nr * rf2, 1, 0);
+  __syncthreads();
+  bgemm(rt, rf2, nc * Ra2, Ra * nr, 1.0, s1, 1, rf2, Ra * nr * rf2, A, nc * Ra2, 1, 0, 0.0, s2, 1,
+        rf2, nc * Ra2 * rf2);
+  __syncthreads();
+  bgemm(1, rt2, Ra2 * rf2, rt * nc, 1.0, T, 1, rt2, 0, s2, Ra2 * rf2, 1, 0, 0.0, out, Ra2 * rf2, 1, 0);
+}
+
+// right environment step (common.py:203-217)
+__device__ void env_right_blk(const double* Phi, const double* F, int64_t rf, int64_t nr, int64_t rf2,
+                              const double* T, int64_t rt, int64_t nc, int64_t rt2, const double* A,
+                              int64_t Ra, int64_t Ra2, double* out, double* s1, double* s2) {
+  if (!A) {
+    bgemm(1, rt * nc, rf2, rt2, 1.0, T, rt2, 1, 0, Phi, rf2, 1, 0, 0.0, s1, rf2, 1, 0);
+    __syncthreads();
+    bgemm(1, rt, rf, nc * rf2, 1.0, s1, nc * rf2, 1, 0, F, 1, nr * rf2, 0, 0.0, out, rf, 1, 0);
+    return;
+  }
+  bgemm(1, rt * nc, Ra2 * rf2, rt2, 1.0, T, rt2, 1, 0, Phi, Ra2 * rf2, 1, 0, 0.0, s1, Ra2 * rf2, 1, 0);
+  __syncthreads();
+  bgemm(rt, rf2, Ra * nr, nc * Ra2, 1.0, s1, 1, rf2, nc * Ra2 * rf2, A, 1, nc * Ra2, 0, 0.0, s2, 1,
+        rf2, Ra * nr * rf2);
+  __syncthreads();
+  bgemm(1, rt * Ra, rf, nr * rf2, 1.0, s2, nr * rf2, 1, 0, F, 1, nr * rf2, 0, 0.0, out, rf, 1, 0);
+}
+
+// b_loc[f_l, i, f_r] = sum Psi[t, f_l] b[t, i, t'] Phi[t', f_r] (one_site.py:27-46)
+__device__ void local_rhs_blk(const double* psi, int64_t rt, int64_t fl, const double* bc, int64_t n,
+                              int64_t rt2, const double* phi, int64_t fr, double* out, double* s1) {
+  bgemm(1, fl, n * rt2, rt, 1.0, psi, 1, fl, 0, bc, n * rt2, 1, 0, 0.0, s1, n * rt2, 1, 0);
+  __syncthreads();
+  bgemm(1, fl * n, fr, rt2, 1.0, s1, rt2, 1, 0, phi, fr, 1, 0, 0.0, out, fr, 1, 0);
+}
+
+// one-site operator Psi . A_k . Phi, explicit (row-major N x N) or matrix-free
+// (one_site.py:78-115); rows (fl, i, fr), columns (rl, j, rr)
+struct LocOp {
+  int explicit_;
+  const double* M;
+  const double* psi;
+  const double* A;
+  const double* phi;
+  int64_t rl, Ra, fl, nr, nc, Rb, rr, fr;
+  double* s1;
+  double* s2;
+  __device__ int64_t rows() const { return fl * nr * fr; }
+  __device__ int64_t cols() const { return rl * nc * rr; }
+  // out = alpha * op(y) + beta * out for B columns (column-major); ends synced
+  __device__ void apply(const double* y, int64_t B, double* out, double alpha, double beta) const {
+    if (explicit_) {
+      const int64_t N = rows();
+      if (B == 1)
+        bgemv(N, N, alpha, M, N, y, beta, out);
+      else
+        bgemm(1, N, B, N, alpha, M, N, 1, 0, y, 1, N, 0, beta, out, 1, N, 0);
+      __syncthreads();
+      return;
+    }
+    bgemm(1, B * rl * nc, Rb * fr, rr, 1.0, y, rr, 1, 0, phi, Rb * fr, 1, 0, 0.0, s1, Rb * fr, 1, 0);
+    __syncthreads();
+    bgemm(B * rl, Ra * nr, fr, nc * Rb, 1.0, A, nc * Rb, 1, 0, s1, fr, 1, nc * Rb * fr, 0.0, s2, fr,
+          1, Ra * nr * fr);
+    __syncthreads();
+    bgemm(B, fl, nr * fr, rl * Ra, alpha, psi, 1, fl, 0, s2, nr * fr, 1, rl * Ra * nr * fr, beta,
+          out, nr * fr, 1, fl * nr * fr);
+    __syncthreads();
+  }
+};
+
+__device__ void local_matrix_blk(const LocOp& op, double* out) {
+  const int64_t rows = op.rows(), cols = op.cols();
+  for (int64_t e = threadIdx.x; e < rows * cols; e += blockDim.x) {
+    const int64_t col = e % cols, row = e / cols;
+    const int64_t f_r = row % op.fr, i = (row / op.fr) % op.nr, f_l = row / (op.fr * op.nr);
+    const int64_t t_r = col % op.rr, j = (col / op.rr) % op.nc, t_l = col / (op.rr * op.nc);
+    double sum = 0.0;
+    for (int64_t a = 0; a < op.Ra; ++a) {
+      const double ps = op.psi[(t_l * op.Ra + a) * op.fl + f_l];
+      if (ps == 0.0) continue;
+      const double* arow = op.A + ((a * op.nr + i) * op.nc + j) * op.Rb;
+      double t = 0.0;
+      for (int64_t b = 0; b < op.Rb; ++b) t += arow[b] * op.phi[(t_r * op.Rb + b) * op.fr + f_r];
+      sum += ps * t;
+    }
+    out[e] = sum;
+  }
+}
+
+// ---------------------------------------------------------------- QR helpers
+// Economic QR of the strided m x n matrix A; Q (m x k) and R (k x n) written
+// with their strides (Q may be null).  Input is copied first, so outputs may
+// overlap A.  work: m*n + k + m*k doubles.  Ends synced.
+__device__ void block_qr(int64_t m, int64_t n, const double* A, int64_t a_rs, int64_t a_cs, double* Q,
+                         int64_t q_rs, int64_t q_cs, double* R, int64_t r_rs, int64_t r_cs,
+                         double* work, double* red) {
+  const int64_t k = m < n ? m : n;
+  double* W = work;
+  double* tau = W + m * n;
+  double* Qw = tau + k;
+  for (int64_t e = threadIdx.x; e < m * n; e += blockDim.x) {
+    const int64_t i = e % m, j = e / m;
+    W[e] = A[i * a_rs + j * a_cs];
+  }
+  __syncthreads();
+  block_householder_qr(m, n, W, tau, Qw, red, Q != nullptr);
+  __syncthreads();
+  if (Q) {
+    for (int64_t e = threadIdx.x; e < m * k; e += blockDim.x) {
+      const int64_t i = e % m, c = e / m;
+      Q[i * q_rs + c * q_cs] = Qw[e];
+    }
+  }
+  for (int64_t e = threadIdx.x; e < k * n; e += blockDim.x) {
+    const int64_t i = e % k, c = e / k;
+    R[i * r_rs + c * r_cs] = (i <= c) ? W[i + c * m] : 0.0;
+  }
+  __syncthreads();
+}
+
+// train[k] <- Q of (rl n x rr), train[k+1] <- R train[k+1]; returns the new
+// bond (one_site.py:181-193)
+__device__ int64_t shift_right_blk(double* core_k, double* core_k1, const double* src, int64_t rl,
+                                   int64_t n, int64_t rr, int64_t rest, double* qb, double* rb,
+                                   double* tmp, double* work, double* red) {
+  const int64_t m = rl * n;
+  const int64_t kk = m < rr ? m : rr;
+  block_qr(m, rr, src, rr, 1, qb, kk, 1, rb, rr, 1, work, red);
+  bgemm(1, kk, rest, rr, 1.0, rb, rr, 1, 0, core_k1, rest, 1, 0, 0.0, tmp, rest, 1, 0);
+  __syncthreads();
+  bcopy(core_k, qb, m * kk);
+  bcopy(core_k1, tmp, kk * rest);
+  __syncthreads();
+  return kk;
+}
+
+// train[k] <- Q^T of the transposed right unfolding, train[k-1] <- train[k-1] R^T
+// (one_site.py:196-209)
+__device__ int64_t shift_left_blk(double* core_k, double* core_km1, const double* src, int64_t rl,
+                                  int64_t n, int64_t rr, int64_t rows_prev, double* qb, double* rb,
+                                  double* tmp, double* work, double* red) {
+  const int64_t w = n * rr;
+  const int64_t kk = w < rl ? w : rl;
+  block_qr(w, rl, src, 1, w, qb, 1, w, rb, rl, 1, work, red);  // qb holds Q^T (kk x w)
+  bgemm(1, rows_prev, kk, rl, 1.0, core_km1, rl, 1, 0, rb, 1, rl, 0, 0.0, tmp, kk, 1, 0);
+  __syncthreads();
+  bcopy(core_k, qb, kk * w);
+  bcopy(core_km1, tmp, rows_prev * kk);
+  __syncthreads();
+  return kk;
+}
+
+// ---------------------------------------------------------------- MINRES
+// scipy.sparse.linalg.minres (no preconditioner, shift 0) by one block, the
+// recurrences of krylov.hip; V holds 9 n doubles (x, v, y, r1, r2, w, w1, w2,
+// yA).  x0 may alias V's x.  Returns with x in V[0..n).
+__device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const double* x0, double rtol,
+                           int64_t maxiter, double* V, double* red) {
+  double* x = V;
+  double* v = V + n;
+  double* y = V + 2 * n;
+  double* r1 = V + 3 * n;
+  double* r2 = V + 4 * n;
+  double* w = V + 5 * n;
+  double* w1 = V + 6 * n;
+  double* w2 = V + 7 * n;
+  double* yA = V + 8 * n;
+  const double eps = DBL_EPSILON;
+  if (x0) op.apply(x0, 1, yA, 1.0, 0.0);
+  double sb = 0.0, sr = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double xi = x0 ? x0[i] : 0.0;
+    const double ri = x0 ? b[i] - yA[i] : b[i];
+    x[i] = xi;
+    r1[i] = ri;
+    r2[i] = ri;
+    w[i] = 0.0;
+    w1[i] = 0.0;
+    w2[i] = 0.0;
+    sb += b[i] * b[i];
+    sr += ri * ri;
+  }
+  const double beta1sq = block_sum(sr, red);
+  const double bn = sqrt(block_sum(sb, red));
+  if (beta1sq == 0.0) return;
+  if (bn == 0.0) {
+    bcopy(x, b, n);
+    __syncthreads();
+    return;
+  }
+  const double beta1 = sqrt(beta1sq);
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = r1[i] / beta1;
+  __syncthreads();
+  double beta = beta1, oldb = 0.0, dbar = 0.0, epsln = 0.0, phibar = beta1, tnorm2 = 0.0;
+  double gmax = 0.0, gmin = DBL_MAX, cs = -1.0, sn = 0.0;
+  int istop = 0;
+  int64_t itn = 0;
+  while (itn < maxiter) {
+    ++itn;
+    op.apply(v, 1, yA, 1.0, 0.0);
+    const double f1 = itn >= 2 ? beta / oldb : 0.0;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      double yi = yA[i];
+      if (itn >= 2) yi -= f1 * r1[i];
+      y[i] = yi;
+      s += v[i] * yi;
+    }
+    const double alfa = block_sum(s, red);
+    const double f2 = alfa / beta;
+    s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const double yi = y[i] - f2 * r2[i];
+      r1[i] = r2[i];
+      r2[i] = yi;
+      s += yi * yi;
+    }
+    const double nbeta = sqrt(block_sum(s, red));
+    oldb = beta;
+    tnorm2 += alfa * alfa + oldb * oldb + nbeta * nbeta;
+    if (itn == 1 && nbeta / beta1 <= 10.0 * eps) istop = -1;
+    const double oldeps = epsln;
+    const double delta = cs * dbar + sn * alfa;
+    const double gbar = sn * dbar - cs * alfa;
+    epsln = sn * nbeta;
+    dbar = -cs * nbeta;
+    const double root = sqrt(gbar * gbar + dbar * dbar);
+    double gamma = sqrt(gbar * gbar + nbeta * nbeta);
+    gamma = gamma > eps ? gamma : eps;
+    cs = gbar / gamma;
+    sn = nbeta / gamma;
+    const double phi = cs * phibar;
+    phibar = sn * phibar;
+    const double denom = 1.0 / gamma;
+    s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const double ow2 = w2[i], ow = w[i];
+      const double nw = (v[i] - oldeps * ow2 - delta * ow) * denom;
+      w1[i] = ow2;
+      w2[i] = ow;
+      w[i] = nw;
+      const double xi = x[i] + phi * nw;
+      x[i] = xi;
+      s += xi * xi;
+    }
+    const double ynorm = sqrt(block_sum(s, red));
+    beta = nbeta;
+    gmax = fmax(gmax, gamma);
+    gmin = fmin(gmin, gamma);
+    const double Anorm = sqrt(tnorm2);
+    const double epsx = Anorm * ynorm * eps;
+    const double test1 = (ynorm == 0.0 || Anorm == 0.0) ? INFINITY : phibar / (Anorm * ynorm);
+    const double test2 = (Anorm == 0.0) ? INFINITY : root / Anorm;
+    const double Acond = gmax / gmin;
+    if (istop == 0) {
+      if (1.0 + test2 <= 1.0) istop = 2;
+      if (1.0 + test1 <= 1.0) istop = 1;
+      if (itn >= maxiter) istop = 6;
+      if (Acond >= 0.1 / eps) istop = 4;
+      if (epsx >= beta1) istop = 3;
+      if (test2 <= rtol) istop = 2;
+      if (test1 <= rtol) istop = 1;
+    }
+    if (istop != 0) break;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = r2[i] / beta;
+    __syncthreads();
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------- slot phases
+struct Env {
+  const double* ptr;
+  int64_t t, a, f;  // target, operator (0: none), frame bonds
+};
+
+__device__ LocOp make_mfree(const ttx_slot* s, int64_t k, const double* psi, int64_t rl, int64_t Ra,
+                            int64_t fl, const double* phi, int64_t rr, int64_t Rb, int64_t fr) {
+  LocOp op;
+  op.explicit_ = 0;
+  op.M = nullptr;
+  op.psi = psi;
+  op.A = s->A[k];
+  op.phi = phi;
+  op.rl = rl;
+  op.Ra = Ra;
+  op.fl = fl;
+  op.nr = s->n[k];
+  op.nc = s->n[k];
+  op.Rb = Rb;
+  op.rr = rr;
+  op.fr = fr;
+  op.s1 = scr(s, TTX_SCR_S1);
+  op.s2 = scr(s, TTX_SCR_S2);
+  return op;
+}
+
+// the x/x/A local operator at k, explicit when the solve built the matrix
+__device__ LocOp system_op(const ttx_slot* s, int64_t k, bool explicit_) {
+  const int64_t rl = s->rx[k], rr = s->rx[k + 1];
+  LocOp op = make_mfree(s, k, envp(s, 0, 0, k), rl, s->ra[k], rl, envp(s, 0, 1, k), rr, s->ra[k + 1], rr);
+  if (explicit_) {
+    op.explicit_ = 1;
+    op.M = scr(s, TTX_SCR_MAT);
+  }
+  return op;
+}
+
+// z_k = b_part - Psi_zx . A_k . Phi_zx x_core (amen.py:109-159), into ZNEW
+__device__ void residual_core_blk(ttx_slot* s, int64_t k, const double* x_core, double* red) {
+  const int64_t zl = s->rz[k], zr = s->rz[k + 1], n = s->n[k];
+  double* out = scr(s, TTX_SCR_ZNEW);
+  local_rhs_blk(envp(s, 2, 0, k), s->rb[k], zl, s->b[k], n, s->rb[k + 1], envp(s, 2, 1, k), zr, out,
+                scr(s, TTX_SCR_S1));
+  __syncthreads();
+  LocOp op = make_mfree(s, k, envp(s, 3, 0, k), s->rx[k], s->ra[k], zl, envp(s, 3, 1, k), s->rx[k + 1],
+                        s->ra[k + 1], zr);
+  op.apply(x_core, 1, out, -1.0, 1.0);
+  (void)red;
+}
+
+// rank of the solved core (common.py:812-929); same value on every thread
+__device__ int64_t choose_rank_blk(ttx_slot* s, const LocOp& op, const double* bloc, const double* U,
+                                   const double* S, const double* Vt, int64_t p, int64_t q, int64_t K,
+                                   double exact, int* capped, double* red) {
+  const int64_t mr = s->max_rank;
+  if (!s->residual_mode) {
+    // sigma_i / sigma_1 > error_threshold, floor 1, cap (common.py:872-885)
+    if (K == 0 || S[0] == 0.0) { *capped = 0; return 1; }
+    int64_t c = 0;
+    for (int64_t i = 0; i < K; ++i) c += (S[i] / S[0] > s->error_threshold) ? 1 : 0;
+    int64_t r = c > 1 ? c : 1;
+    *capped = 0;
+    if (mr > 0 && r > mr) { r = mr; *capped = 1; }
+    return r;
+  }
+  *capped = 0;
+  if (S[0] == 0.0) return 1;
+  int64_t limit = K;
+  if (mr > 0 && mr < limit) limit = mr;
+  const int64_t N = p * q;
+  const double bn = bnorm2(bloc, nullptr, N, red);
+  const double a = s->damp * exact, f = 0.5 * s->trunc_tol * bn;
+  const double thr = a > f ? a : f;
+  double* partial = scr(s, TTX_SCR_TMP);
+  double* terms = scr(s, TTX_SCR_CHK);
+  double* contrib = terms + s->chunk * s->cap_loc;
+  for (int64_t i = threadIdx.x; i < N; i += blockDim.x) partial[i] = -bloc[i];
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < limit; c0 += s->chunk) {
+    const int64_t cb = (limit - c0) < s->chunk ? (limit - c0) : s->chunk;
+    for (int64_t e = threadIdx.x; e < N * cb; e += blockDim.x) {
+      const int64_t row = e % N, j = e / N;
+      const int64_t i = row / q, jj = row % q, r = c0 + j;
+      terms[e] = U[i * K + r] * S[r] * Vt[r * q + jj];
+    }
+    __syncthreads();
+    op.apply(terms, cb, contrib, 1.0, 0.0);
+    for (int64_t j = 0; j < cb; ++j) {
+      const double* cj = contrib + j * N;
+      double t = 0.0;
+      for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+        const double v = partial[i] + cj[i];
+        partial[i] = v;
+        t += v * v;
+      }
+      const double nrm = sqrt(block_sum(t, red));
+      if (nrm <= thr) return c0 + j + 1;
+    }
+  }
+  *capped = (mr > 0 && mr < K) ? 1 : 0;
+  return limit;
+}
+
+__device__ void fail(ttx_slot* s, int64_t code) {
+  if (threadIdx.x == 0) s->status = code;
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ __launch_bounds__(kEngThreads) void eng_env_kernel(ttx_slot* slots) {
+  ttx_slot* s = slots + blockIdx.x;
+  const int kind = blockIdx.y;
+  if (s->status != TTX_SLOT_RUNNING) return;
+  const int64_t ph = s->phase, d = s->d;
+  int side;
+  int64_t k = s->k;
+  bool zonly = false;
+  switch (ph) {
+    case TTX_PH_FIT_INIT: side = 1; zonly = true; break;
+    case TTX_PH_FIT_FWD: side = 0; zonly = true; break;
+    case TTX_PH_FIT_LAST: side = 0; zonly = true; k = d - 1; break;
+    case TTX_PH_FIT_BWD: side = 1; zonly = true; break;
+    case TTX_PH_INIT_R: side = 1; break;
+    case TTX_PH_FWD: side = 0; break;
+    case TTX_PH_LAST: side = 0; k = d - 1; break;
+    case TTX_PH_BWD: side = 1; break;
+    default: return;
+  }
+  if (zonly && kind != 2 && kind != 3) return;
+  double* out = envp(s, kind, side, k);
+  if ((side == 0 && k == 0) || (side == 1 && k == d - 1)) {
+    if (threadIdx.x == 0) out[0] = 1.0;
+    return;
+  }
+  const int64_t c = side == 0 ? k - 1 : k + 1;
+  // frame/target trains of the kind: 0 (x,x,A) 1 (x,b) 2 (z,b) 3 (z,x,A) 4 (x,z)
+  const double* F;
+  const int64_t* fb;
+  const double* T;
+  const int64_t* tb;
+  const bool frame_z = (kind == 2 || kind == 3);
+  F = frame_z ? zcore(s, c) : xcore(s, c);
+  fb = frame_z ? s->rz : s->rx;
+  if (kind == 0 || kind == 3) { T = xcore(s, c); tb = s->rx; }
+  else if (kind == 4) { T = zcore(s, c); tb = s->rz; }
+  else { T = s->b[c]; tb = s->rb; }
+  const bool with_op = (kind == 0 || kind == 3);
+  const double* A = with_op ? s->A[c] : nullptr;
+  const int64_t n = s->n[c];
+  double* s1 = scr(s, TTX_SCR_ENVS) + (int64_t)kind * 2 * s->cap_core * s->ramax;
+  double* s2 = s1 + s->cap_core * s->ramax;
+  if (side == 0)
+    env_left_blk(envp(s, kind, 0, k - 1), F, fb[c], n, fb[c + 1], T, tb[c], n, tb[c + 1], A,
+                 with_op ? s->ra[c] : 1, with_op ? s->ra[c + 1] : 1, out, s1, s2);
+  else
+    env_right_blk(envp(s, kind, 1, k + 1), F, fb[c], n, fb[c + 1], T, tb[c], n, tb[c + 1], A,
+                  with_op ? s->ra[c] : 1, with_op ? s->ra[c + 1] : 1, out, s1, s2);
+}
+
+// one-site local solve (one_site.py:118-178, common.py:729-809)
+__global__ __launch_bounds__(kEngThreads) void eng_solve_kernel(ttx_slot* slots) {
+  extern __shared__ double lds[];
+  __shared__ double red[16];
+  __shared__ int32_t s_info;
+  ttx_slot* s = slots + blockIdx.x;
+  if (s->status != TTX_SLOT_RUNNING) return;
+  if (s->phase != TTX_PH_FWD && s->phase != TTX_PH_BWD) return;
+  const int64_t k = s->k;
+  const int64_t rl = s->rx[k], n = s->n[k], rr = s->rx[k + 1];
+  const int64_t N = rl * n * rr;
+  double* u = scr(s, TTX_SCR_U);
+  double* bloc = scr(s, TTX_SCR_BLOC);
+  double* tmp = scr(s, TTX_SCR_TMP);
+  double* scal = scr(s, TTX_SCR_SCAL);
+  local_rhs_blk(envp(s, 1, 0, k), s->rb[k], rl, s->b[k], n, s->rb[k + 1], envp(s, 1, 1, k), rr, bloc,
+                scr(s, TTX_SCR_S1));
+  __syncthreads();
+  const bool iter = s->iterative && N > s->direct_size;
+  if (iter) {
+    LocOp op = system_op(s, k, false);
+    const double target = s->local_rtol * bnorm2(bloc, nullptr, N, red);
+    double rt = s->local_rtol;
+    double* V = scr(s, TTX_SCR_MR);
+    const double* x0 = xcore(s, k);  // warm start: the current core (one_site.py:151-153)
+    bool ok = false;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+      minres_blk(op, N, bloc, x0, rt, 5 * N, V, red);
+      x0 = V;
+      op.apply(V, 1, tmp, 1.0, 0.0);
+      const double res = bnorm2(tmp, bloc, N, red);
+      if (res <= target) { ok = true; break; }
+      rt = fmax(rt * 1e-2, 1e-15);
+    }
+    bcopy(u, V, N);
+    if (threadIdx.x == 0) {
+      s->local_solves += 1;
+      if (!ok) s->local_inexact += 1;
+      scal[1] = 0.0;
+    }
+  } else {
+    if (N > s->direct_cap) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
+    LocOp op = system_op(s, k, false);
+    double* M = scr(s, TTX_SCR_MAT);
+    double* W = scr(s, TTX_SCR_LU);
+    local_matrix_blk(op, M);
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < N * (N + 1); e += blockDim.x) {
+      const int64_t r = e / (N + 1), c = e % (N + 1);
+      W[e] = c < N ? M[r * N + c] : bloc[r];
+    }
+    if (threadIdx.x == 0) s_info = TTX_OK;
+    __syncthreads();
+    int64_t nb0 = (int64_t)(kEngLds / (8 * N));
+    if (nb0 > 16) nb0 = 16;
+    if (nb0 < 1) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
+    block_lu_solve(N, N + 1, W, u, lds, nb0, red, &s_info);
+    __syncthreads();
+    if (s_info != TTX_OK) { fail(s, TTX_SLOT_ERR_SINGULAR); return; }
+    if (threadIdx.x == 0) scal[1] = 1.0;
+  }
+  __syncthreads();
+  if (s->residual_mode) {
+    LocOp op = system_op(s, k, !iter);
+    op.apply(u, 1, tmp, 1.0, 0.0);
+    const double ex = bnorm2(tmp, bloc, N, red);
+    if (threadIdx.x == 0) scal[0] = ex;
+  } else if (threadIdx.x == 0) {
+    scal[0] = 0.0;
+  }
+}
+
+// truncation, enrichment and shifts after a solve; QR sweeps and z fit
+__global__ __launch_bounds__(kEngThreads) void eng_post_kernel(ttx_slot* slots) {
+  __shared__ double red[16];
+  ttx_slot* s = slots + blockIdx.x;
+  if (s->status != TTX_SLOT_RUNNING) return;
+  const int64_t ph = s->phase, d = s->d, k = s->k;
+  double* enr = scr(s, TTX_SCR_ENR);
+  const int64_t wcap = (s->rmax + s->kick) * s->nmax * (s->rmax + s->kick);
+  double* nb_buf = enr + wcap;         // neighbour update
+  double* qb = nb_buf + wcap;          // Q
+  double* rbuf = qb + wcap;            // R
+  double* tmp = rbuf + wcap;           // GEMM output before copy-back
+  double* qrw = tmp + wcap;            // QR work (3 wcap)
+  const int64_t n = s->n[k];
+
+  if (ph == TTX_PH_XORTH || ph == TTX_PH_ZORTH || ph == TTX_PH_RED_R) {
+    // right-orthonormalising QR step at k (utils.py:115-136)
+    const bool onz = ph == TTX_PH_ZORTH;
+    double* ck = onz ? zcore(s, k) : xcore(s, k);
+    double* cp = onz ? zcore(s, k - 1) : xcore(s, k - 1);
+    int64_t* bond = onz ? s->rz : s->rx;
+    const int64_t rl = bond[k], rr = bond[k + 1];
+    const int64_t rows_prev = bond[k - 1] * s->n[k - 1];
+    const int64_t kk = shift_left_blk(ck, cp, ck, rl, n, rr, rows_prev, qb, rbuf, tmp, qrw, red);
+    if (threadIdx.x == 0) bond[k] = kk;
+    return;
+  }
+  if (ph == TTX_PH_RED_L) {
+    // left-orthonormalising QR step at k (utils.py:86-112)
+    const int64_t rl = s->rx[k], rr = s->rx[k + 1];
+    const int64_t rest = s->n[k + 1] * s->rx[k + 2];
+    const int64_t kk = shift_right_blk(xcore(s, k), xcore(s, k + 1), xcore(s, k), rl, n, rr, rest, qb,
+                                       rbuf, tmp, qrw, red);
+    if (threadIdx.x == 0) s->rx[k + 1] = kk;
+    return;
+  }
+  if (ph == TTX_PH_FIT_FWD || ph == TTX_PH_FIT_BWD) {
+    // residual-train fit (amen.py:162-224)
+    residual_core_blk(s, k, xcore(s, k), red);
+    double* zn = scr(s, TTX_SCR_ZNEW);
+    const int64_t zl = s->rz[k], zr = s->rz[k + 1];
+    if (ph == TTX_PH_FIT_FWD) {
+      const int64_t kk = shift_right_blk(zcore(s, k), zcore(s, k + 1), zn, zl, n, zr,
+                                         s->n[k + 1] * s->rz[k + 2], qb, rbuf, tmp, qrw, red);
+      if (threadIdx.x == 0) s->rz[k + 1] = kk;
+    } else if (k > 0) {
+      const int64_t kk = shift_left_blk(zcore(s, k), zcore(s, k - 1), zn, zl, n, zr,
+                                        s->rz[k - 1] * s->n[k - 1], qb, rbuf, tmp, qrw, red);
+      if (threadIdx.x == 0) s->rz[k] = kk;
+    } else {
+      bcopy(zcore(s, 0), zn, zl * n * zr);
+    }
+    return;
+  }
+  if (ph != TTX_PH_FWD && ph != TTX_PH_BWD) return;
+
+  const int64_t rl = s->rx[k], rr = s->rx[k + 1];
+  const int64_t N = rl * n * rr;
+  const double* u = scr(s, TTX_SCR_U);
+  const double* bloc = scr(s, TTX_SCR_BLOC);
+  const double* scal = scr(s, TTX_SCR_SCAL);
+  const double exact = scal[0];
+  const bool expl = scal[1] != 0.0;
+  residual_core_blk(s, k, u, red);  // z from the untruncated solution (amen.py:601-613)
+  double* zn = scr(s, TTX_SCR_ZNEW);
+  const int64_t zl = s->rz[k], zr = s->rz[k + 1];
+  __syncthreads();
+
+  if (ph == TTX_PH_BWD && k == 0) {
+    // last core of the backward sweep is stored as solved (amen.py:654-677)
+    bcopy(xcore(s, 0), u, N);
+    bcopy(zcore(s, 0), zn, zl * n * zr);
+    return;
+  }
+  const LocOp op = system_op(s, k, expl);
+  const bool fwd = ph == TTX_PH_FWD;
+  // SVD of (rl n x rr) forward or (rl x n rr) backward (amen.py:254-260,301-307)
+  const int64_t p = fwd ? rl * n : rl;
+  const int64_t q = fwd ? rr : n * rr;
+  const int64_t K = p < q ? p : q;
+  double* U = scr(s, TTX_SCR_USVD);
+  double* S = scr(s, TTX_SCR_SSVD);
+  double* Vt = scr(s, TTX_SCR_VSVD);
+  __shared__ int32_t s_svd;
+  if (threadIdx.x == 0) s_svd = TTX_OK;
+  __syncthreads();
+  SvdArgs sa{p, q, u, q, 1, U, K, 1, S, Vt, q, 1, scr(s, TTX_SCR_SVDW), &s_svd};
+  block_svd(sa, scr(s, TTX_SCR_SVDW), red);
+  __syncthreads();
+  if (s_svd == TTX_ENONFINITE) { fail(s, TTX_SLOT_ERR_NONFINITE); return; }
+  if (s_svd == TTX_ENOCONV && threadIdx.x == 0) s->svd_noconv += 1;
+  int capped = 0;
+  const int64_t rank = choose_rank_blk(s, op, bloc, U, S, Vt, p, q, K, exact, &capped, red);
+  __syncthreads();
+  const int64_t mr = s->max_rank;
+  if (fwd) {
+    // core = U_r, carry = S V_r, block = Psi_xz^T z (amen.py:614-627), enrich (amen.py:325-360)
+    const int64_t W = rank + zr;
+    const int64_t m = rl * n;
+    for (int64_t e = threadIdx.x; e < m * rank; e += blockDim.x) {
+      const int64_t row = e / rank, c = e % rank;
+      enr[row * W + c] = U[row * K + c];
+    }
+    bgemm(n, rl, zr, zl, 1.0, envp(s, 4, 0, k), 1, rl, 0, zn, n * zr, 1, zr, 0.0, enr + rank, n * W,
+          1, W);
+    double* carry = tmp;
+    for (int64_t e = threadIdx.x; e < rank * rr; e += blockDim.x) {
+      const int64_t c = e / rr;
+      carry[e] = S[c] * Vt[e];
+    }
+    __syncthreads();
+    const int64_t rest = s->n[k + 1] * s->rx[k + 2];
+    bgemm(1, rank, rest, rr, 1.0, carry, rr, 1, 0, xcore(s, k + 1), rest, 1, 0, 0.0, nb_buf, rest, 1, 0);
+    bfill(nb_buf + rank * rest, zr * rest, 0.0);
+    __syncthreads();
+    const int64_t kk = m < W ? m : W;
+    block_qr(m, W, enr, W, 1, qb, kk, 1, rbuf, W, 1, qrw, red);
+    const int64_t kc = (mr > 0 && kk > mr) ? mr : kk;
+    if (kc > s->rmax) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
+    double* xk = xcore(s, k);
+    for (int64_t e = threadIdx.x; e < m * kc; e += blockDim.x) {
+      const int64_t row = e / kc, c = e % kc;
+      xk[e] = qb[row * kk + c];
+    }
+    bgemm(1, kc, rest, W, 1.0, rbuf, W, 1, 0, nb_buf, rest, 1, 0, 0.0, xcore(s, k + 1), rest, 1, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s->rx[k + 1] = kc;
+      s->rank_decisions += 2;
+      s->rank_capped += capped + (kc < kk ? 1 : 0);
+    }
+    // z centre moves right with the fresh core (amen.py:628)
+    const int64_t kz = shift_right_blk(zcore(s, k), zcore(s, k + 1), zn, zl, n, zr,
+                                       s->n[k + 1] * s->rz[k + 2], qb, rbuf, tmp, qrw, red);
+    if (threadIdx.x == 0) s->rz[k + 1] = kz;
+    return;
+  }
+  // backward: core = V_r, carry = U_r S, block = z Phi_xz (amen.py:679-703), enrich (amen.py:363-392)
+  const int64_t W = rank + zl;
+  const int64_t w = n * rr;
+  bcopy(enr, Vt, rank * w);
+  bgemm(1, zl * n, rr, zr, 1.0, zn, zr, 1, 0, envp(s, 4, 1, k), rr, 1, 0, 0.0, enr + rank * w, rr, 1, 0);
+  double* carry = tmp;
+  for (int64_t e = threadIdx.x; e < rl * rank; e += blockDim.x) {
+    const int64_t i = e / rank, c = e % rank;
+    carry[e] = U[i * K + c] * S[c];
+  }
+  const int64_t rows_prev = s->rx[k - 1] * s->n[k - 1];
+  bfill(nb_buf, rows_prev * W, 0.0);
+  __syncthreads();
+  bgemm(1, rows_prev, rank, rl, 1.0, xcore(s, k - 1), rl, 1, 0, carry, rank, 1, 0, 0.0, nb_buf, W, 1, 0);
+  __syncthreads();
+  const int64_t kk = w < W ? w : W;
+  block_qr(w, W, enr, 1, w, qb, 1, w, rbuf, W, 1, qrw, red);  // qb = Q^T (kk x w)
+  const int64_t kc = (mr > 0 && kk > mr) ? mr : kk;
+  if (kc > s->rmax) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
+  bcopy(xcore(s, k), qb, kc * w);
+  bgemm(1, rows_prev, kc, W, 1.0, nb_buf, W, 1, 0, rbuf, 1, W, 0, 0.0, xcore(s, k - 1), kc, 1, 0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s->rx[k] = kc;
+    s->rank_decisions += 2;
+    s->rank_capped += capped + (kc < kk ? 1 : 0);
+  }
+  const int64_t kz = shift_left_blk(zcore(s, k), zcore(s, k - 1), zn, zl, n, zr,
+                                    s->rz[k - 1] * s->n[k - 1], qb, rbuf, tmp, qrw, red);
+  if (threadIdx.x == 0) s->rz[k] = kz;
+}
+
+// ||A x - b|| / ||b|| in TT arithmetic (common.py:118-146): the residual
+// train is formed core by core and normed by a Q-less right QR sweep.
+__device__ double residual_norm_blk(ttx_slot* s, double* red) {
+  const int64_t d = s->d;
+  double* base = scr(s, TTX_SCR_CHK);
+  const int64_t L2 = s->lmax * s->lmax;
+  double* T = base;                       // L x n x R
+  double* Wb = T + L2 * s->nmax;          // L x n x c
+  double* C = Wb + L2 * s->nmax;          // R x c
+  double* Rb = C + L2;                    // R factor
+  double* qw = Rb + L2;                   // QR work
+  int64_t cdim = 1;
+  if (threadIdx.x == 0) C[0] = 1.0;
+  __syncthreads();
+  double nrm = 0.0;
+  for (int64_t i = d - 1; i >= 0; --i) {
+    const int64_t n = s->n[i];
+    const int64_t Ra = s->ra[i], Rb2 = s->ra[i + 1], rx = s->rx[i], rx2 = s->rx[i + 1];
+    const int64_t rbl = s->rb[i], rbr = s->rb[i + 1];
+    const int64_t L = i == 0 ? 1 : Ra * rx + rbl;
+    const int64_t R = i == d - 1 ? 1 : Rb2 * rx2 + rbr;
+    bfill(T, L * n * R, 0.0);
+    __syncthreads();
+    const double* A = s->A[i];
+    const double* xc = xcore(s, i);
+    const int64_t Lx = Ra * rx, Rx = Rb2 * rx2;
+    for (int64_t e = threadIdx.x; e < Lx * n * Rx; e += blockDim.x) {
+      const int64_t r = e % Rx, j = (e / Rx) % n, l = e / (Rx * n);
+      const int64_t a = l / rx, m = l % rx, b = r / rx2, nn = r % rx2;
+      double t = 0.0;
+      for (int64_t jj = 0; jj < n; ++jj) t += A[((a * n + j) * n + jj) * Rb2 + b] * xc[(m * n + jj) * rx2 + nn];
+      T[(l * n + j) * R + r] = t;
+    }
+    const double sg = (i == s->bneg) ? -1.0 : 1.0;
+    const double* bc = s->b[i];
+    const int64_t offl = L - rbl, offr = R - rbr;
+    for (int64_t e = threadIdx.x; e < rbl * n * rbr; e += blockDim.x) {
+      const int64_t r = e % rbr, j = (e / rbr) % n, l = e / (rbr * n);
+      T[((offl + l) * n + j) * R + offr + r] = sg * bc[e];
+    }
+    __syncthreads();
+    bgemm(1, L * n, cdim, R, 1.0, T, R, 1, 0, C, cdim, 1, 0, 0.0, Wb, cdim, 1, 0);
+    __syncthreads();
+    if (i == 0) {
+      nrm = bnorm2(Wb, nullptr, L * n * cdim, red);
+      break;
+    }
+    // QR of the transposed unfolding ((n c) x L), keep R^T as the next carry
+    const int64_t m = n * cdim;
+    const int64_t kk = m < L ? m : L;
+    block_qr(m, L, Wb, 1, m, nullptr, 0, 0, Rb, L, 1, qw, red);
+    for (int64_t e = threadIdx.x; e < L * kk; e += blockDim.x) {
+      const int64_t j = e / kk, c = e % kk;
+      C[e] = Rb[c * L + j];
+    }
+    cdim = kk;
+    __syncthreads();
+  }
+  return nrm;
+}
+
+__device__ void advance(ttx_slot* s) {
+  const int64_t d = s->d;
+  switch (s->phase) {
+    case TTX_PH_XORTH:
+      if (--s->k < 1) { s->phase = TTX_PH_ZORTH; s->k = d - 1; }
+      break;
+    case TTX_PH_ZORTH:
+      if (--s->k < 1) { s->phase = TTX_PH_FIT_INIT; s->k = d - 1; }
+      break;
+    case TTX_PH_FIT_INIT:
+      if (--s->k < 0) { s->phase = TTX_PH_FIT_FWD; s->k = 0; }
+      break;
+    case TTX_PH_FIT_FWD:
+      if (++s->k >= d - 1) { s->phase = TTX_PH_FIT_LAST; s->k = d - 1; }
+      break;
+    case TTX_PH_FIT_LAST: s->phase = TTX_PH_FIT_BWD; s->k = d - 1; break;
+    case TTX_PH_FIT_BWD:
+      if (--s->k < 0) { s->phase = TTX_PH_INIT_R; s->k = d - 1; }
+      break;
+    case TTX_PH_INIT_R:
+      if (--s->k < 0) { s->phase = TTX_PH_FWD; s->k = 0; s->sweep = 0; }
+      break;
+    case TTX_PH_FWD:
+      if (++s->k >= d - 1) { s->phase = TTX_PH_LAST; s->k = d - 1; }
+      break;
+    case TTX_PH_LAST: s->phase = TTX_PH_BWD; s->k = d - 1; break;
+    case TTX_PH_BWD:
+      if (--s->k < 0) { s->phase = TTX_PH_CHECK; s->k = 0; }
+      break;
+    case TTX_PH_RED_L:
+      if (++s->k >= s->red_last) { s->phase = TTX_PH_RED_R; s->k = s->red_last; }
+      break;
+    case TTX_PH_RED_R:
+      if (--s->k < 1) { s->phase = TTX_PH_DONE; s->status = TTX_SLOT_DONE; }
+      break;
+    default: break;
+  }
+}
+
+// per-sweep monitor (common.py:504-592) and the phase advance of every slot
+__global__ __launch_bounds__(kEngThreads) void eng_check_kernel(ttx_slot* slots) {
+  __shared__ double red[16];
+  ttx_slot* s = slots + blockIdx.x;
+  if (s->status != TTX_SLOT_RUNNING) return;
+  if (s->phase != TTX_PH_CHECK) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s->ticks += 1;
+      advance(s);
+    }
+    return;
+  }
+  const int64_t sw = s->sweep;
+  double res = 0.0;
+  if (s->track) {
+    const double rn = residual_norm_blk(s, red);
+    res = s->bnorm == 0.0 ? rn : rn / s->bnorm;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  s->ticks += 1;
+  int64_t mb = 0;
+  for (int64_t i = 0; i <= s->d; ++i) mb = s->rx[i] > mb ? s->rx[i] : mb;
+  if (sw < TTX_MAXSWEEPS) {
+    s->max_bonds[sw] = mb;
+    s->residuals[sw] = res;
+  }
+  int64_t stop = 0;
+  if (s->track) {
+    if (s->residual_tol > 0.0 && res <= s->residual_tol) stop = 1;
+    else if (s->stagnation > 0.0 && sw >= 1 && res > s->stagnation * s->residuals[sw - 1]) stop = 2;
+  }
+  if (!stop && sw + 1 >= s->max_sweeps) stop = 3;
+  if (!stop) {
+    s->sweep = sw + 1;
+    s->phase = TTX_PH_FWD;
+    s->k = 0;
+    return;
+  }
+  s->stop_reason = stop;
+  // bonds above the full-rank bound left by the last enrichment (amen.py:710-719)
+  int64_t last = 0;
+  double prodn = 1.0;
+  for (int64_t i = 1; i < s->d; ++i) {
+    prodn *= (double)s->n[i - 1];
+    if ((double)s->rx[i] > prodn) last = i;
+  }
+  if (last > 0) {
+    s->red_last = last;
+    s->phase = TTX_PH_RED_L;
+    s->k = 0;
+  } else {
+    s->phase = TTX_PH_DONE;
+    s->status = TTX_SLOT_DONE;
+  }
+}
+
+}  // namespace ttx
+
+using namespace ttx;
+
+extern "C" int64_t ttx_engine_slot_bytes(void) { return (int64_t)sizeof(ttx_slot); }
+
+extern "C" int64_t ttx_engine_field_offset(int which) {
+  switch (which) {
+    case 0: return (int64_t)offsetof(ttx_slot, n);
+    case 1: return (int64_t)offsetof(ttx_slot, residual_tol);
+    case 2: return (int64_t)offsetof(ttx_slot, cap_core);
+    case 3: return (int64_t)offsetof(ttx_slot, A);
+    case 4: return (int64_t)offsetof(ttx_slot, rx);
+    case 5: return (int64_t)offsetof(ttx_slot, residuals);
+    case 6: return (int64_t)offsetof(ttx_slot, max_bonds);
+    default: return -1;
+  }
+}
+
+// Fill the layout fields of a host-side slot from its problem fields.
+extern "C" int ttx_engine_plan(ttx_slot* s) {
+  if (s->d < 2 || s->d > TTX_MAXD || s->rmax < 1 || s->kick < 1 || s->chunk < 1) return TTX_EINVAL;
+  int64_t nmax = 1, ramax = 1, rbmax = 1;
+  for (int64_t i = 0; i < s->d; ++i) nmax = s->n[i] > nmax ? s->n[i] : nmax;
+  for (int64_t i = 0; i <= s->d; ++i) {
+    ramax = s->ra[i] > ramax ? s->ra[i] : ramax;
+    rbmax = s->rb[i] > rbmax ? s->rb[i] : rbmax;
+  }
+  const int64_t rmax = s->rmax, kick = s->kick;
+  s->nmax = nmax;
+  s->ramax = ramax;
+  s->rbmax = rbmax;
+  s->cap_core = rmax * nmax * rmax;
+  s->cap_zcore = kick * nmax * kick;
+  s->cap_loc = s->cap_core;
+  s->direct_cap = s->direct_size < s->cap_loc ? s->direct_size : s->cap_loc;
+  if (s->direct_cap < 1) s->direct_cap = 1;
+  const int64_t zr = kick > rmax ? kick : rmax;
+  s->cap_env[0] = rmax * ramax * rmax;
+  s->cap_env[1] = rbmax * rmax;
+  s->cap_env[2] = rbmax * kick;
+  s->cap_env[3] = rmax * ramax * kick;
+  s->cap_env[4] = kick * zr;
+  int64_t off = 0;
+  for (int kind = 0; kind < 5; ++kind)
+    for (int side = 0; side < 2; ++side) {
+      s->env_off[kind * 2 + side] = off;
+      off += s->d * s->cap_env[kind];
+    }
+  s->env_size = off;
+  s->lmax = ramax * rmax + rbmax;
+  const int64_t L2 = s->lmax * s->lmax;
+  const int64_t cl = s->cap_loc;
+  const int64_t wcap = (rmax + kick) * nmax * (rmax + kick);
+  int64_t sz[TTX_SCR_COUNT];
+  sz[TTX_SCR_U] = cl;
+  sz[TTX_SCR_BLOC] = cl;
+  sz[TTX_SCR_TMP] = cl > wcap ? cl : wcap;
+  sz[TTX_SCR_ZNEW] = kick * nmax * kick;
+  sz[TTX_SCR_MAT] = s->direct_cap * s->direct_cap;
+  sz[TTX_SCR_LU] = s->direct_cap * (s->direct_cap + 1);
+  sz[TTX_SCR_MR] = 9 * cl;
+  const int64_t sapply = s->chunk * cl * ramax;
+  sz[TTX_SCR_S1] = sapply > cl * ramax * 2 ? sapply : cl * ramax * 2;
+  sz[TTX_SCR_S2] = sz[TTX_SCR_S1];
+  sz[TTX_SCR_SVDW] = 2 * cl + 2 * rmax * rmax * nmax * nmax + 3 * rmax * nmax + 8;
+  sz[TTX_SCR_USVD] = cl;
+  sz[TTX_SCR_SSVD] = rmax * nmax;
+  sz[TTX_SCR_VSVD] = cl;
+  sz[TTX_SCR_ENR] = 8 * wcap;
+  const int64_t chk_res = (4 * nmax + 3) * L2 + 2 * s->lmax + 8;
+  const int64_t chk_rank = 2 * s->chunk * cl;
+  sz[TTX_SCR_CHK] = chk_res > chk_rank ? chk_res : chk_rank;
+  sz[TTX_SCR_ENVS] = 5 * 2 * s->cap_core * ramax;
+  sz[TTX_SCR_SCAL] = 64;
+  off = 0;
+  for (int i = 0; i < TTX_SCR_COUNT; ++i) {
+    s->scr_off[i] = off;
+    off += (sz[i] + 31) & ~(int64_t)31;  // 256-byte aligned regions
+  }
+  s->scr_size = off;
+  return TTX_OK;
+}
+
+extern "C" int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks) {
+  if (nslots < 1 || ticks < 1) return TTX_EINVAL;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute((const void*)eng_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kEngLds) != hipSuccess)
+      return TTX_EHIP;
+    attr = true;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  for (int64_t t = 0; t < ticks; ++t) {
+    hipLaunchKernelGGL(eng_env_kernel, dim3((unsigned)nslots, 5), dim3(kEngThreads), 0, st, slots);
+    hipLaunchKernelGGL(eng_solve_kernel, dim3((unsigned)nslots), dim3(kEngThreads), kEngLds, st, slots);
+    hipLaunchKernelGGL(eng_post_kernel, dim3((unsigned)nslots), dim3(kEngThreads), 0, st, slots);
+    hipLaunchKernelGGL(eng_check_kernel, dim3((unsigned)nslots), dim3(kEngThreads), 0, st, slots);
+  }
+  return launch_status();
+}

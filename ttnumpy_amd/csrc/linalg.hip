@@ -13,84 +13,9 @@
 
 #include <float.h>
 
-#include "ttx_common.h"
+#include "block_ops.h"
 
 namespace ttx {
-
-constexpr double kEps = DBL_EPSILON;
-
-// ---------------------------------------------------------------------------
-// Householder QR on a column-major m x n matrix W (leading dim m) held in
-// global memory.  On return W holds R in its upper triangle and the reflector
-// vectors below the diagonal; tau[j] the reflector scales.  Q (m x k,
-// column-major, leading dim m) is formed in Qw.  Block-cooperative.
-// ---------------------------------------------------------------------------
-__device__ void block_householder_qr(int64_t m, int64_t n, double* W, double* tau, double* Qw,
-                                     double* red, bool form_q = true) {
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int nwave = blockDim.x >> 6;
-  const int64_t k = m < n ? m : n;
-  __shared__ double s_tau;
-  for (int64_t j = 0; j < k; ++j) {
-    double* col = W + j * m;
-    double s = 0.0;
-    for (int64_t i = j + 1 + tid; i < m; i += blockDim.x) s += col[i] * col[i];
-    s = block_sum(s, red);
-    const double alpha = col[j];
-    __syncthreads();
-    if (s == 0.0) {
-      if (tid == 0) { s_tau = 0.0; tau[j] = 0.0; }
-    } else {
-      const double beta = -copysign(sqrt(alpha * alpha + s), alpha);
-      const double scal = 1.0 / (alpha - beta);
-      for (int64_t i = j + 1 + tid; i < m; i += blockDim.x) col[i] *= scal;
-      if (tid == 0) {
-        s_tau = (beta - alpha) / beta;
-        tau[j] = s_tau;
-        col[j] = beta;
-      }
-    }
-    __syncthreads();
-    const double t = s_tau;
-    if (t != 0.0) {
-      for (int64_t c = j + 1 + wave; c < n; c += nwave) {
-        double* cc = W + c * m;
-        double d = 0.0;
-        for (int64_t i = j + 1 + lane; i < m; i += 64) d += col[i] * cc[i];
-        d = wave_sum(d) + cc[j];
-        d *= t;
-        if (lane == 0) cc[j] -= d;
-        for (int64_t i = j + 1 + lane; i < m; i += 64) cc[i] -= d * col[i];
-      }
-    }
-    __syncthreads();
-  }
-  if (!form_q) return;
-  // Q = H_0 ... H_{k-1} [I_k; 0], accumulated backwards
-  for (int64_t e = tid; e < m * k; e += blockDim.x) {
-    const int64_t i = e % m, c = e / m;
-    Qw[e] = (i == c) ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  for (int64_t j = k - 1; j >= 0; --j) {
-    const double t = tau[j];
-    const double* v = W + j * m;
-    if (t != 0.0) {
-      for (int64_t c = j + wave; c < k; c += nwave) {
-        double* qc = Qw + c * m;
-        double d = 0.0;
-        for (int64_t i = j + 1 + lane; i < m; i += 64) d += v[i] * qc[i];
-        d = wave_sum(d) + qc[j];
-        d *= t;
-        if (lane == 0) qc[j] -= d;
-        for (int64_t i = j + 1 + lane; i < m; i += 64) qc[i] -= d * v[i];
-      }
-    }
-    __syncthreads();
-  }
-}
 
 struct QrArgs {
   int64_t m, n;
@@ -135,185 +60,11 @@ __global__ __launch_bounds__(1024) void qr_kernel(QrArgs p) {
 // ---------------------------------------------------------------------------
 // SVD
 // ---------------------------------------------------------------------------
-struct SvdArgs {
-  int64_t m, n;
-  const double* A;
-  int64_t a_rs, a_cs;
-  double* U;
-  int64_t u_rs, u_cs;
-  double* S;
-  double* Vt;
-  int64_t v_rs, v_cs;
-  double* work;
-  int32_t* info;
-};
-
-__device__ __forceinline__ int64_t rr_player(int64_t t, int64_t round, int64_t qq) {
-  return t == 0 ? 0 : 1 + (t - 1 + round) % (qq - 1);
-}
-
 template <bool IN_LDS>
 __global__ __launch_bounds__(1024) void svd_kernel(SvdArgs p) {
   extern __shared__ double smem[];
   __shared__ double red[16];
-  __shared__ int s_rot;
-  __shared__ int s_bad;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int nwave = blockDim.x >> 6;
-  const bool wide = p.m < p.n;
-  // X = A (tall) or A^T (wide): P x Q with P >= Q
-  const int64_t P = wide ? p.n : p.m;
-  const int64_t Q = wide ? p.m : p.n;
-  const int64_t x_rs = wide ? p.a_cs : p.a_rs;
-  const int64_t x_cs = wide ? p.a_rs : p.a_cs;
-  double* Wq = IN_LDS ? smem : p.work;  // P x Q   (QR working matrix)
-  double* tau = Wq + P * Q;       // Q
-  double* Qx = tau + Q;           // P x Q   (orthonormal factor)
-  double* G = Qx + P * Q;         // Q x Q   (Jacobi matrix)
-  double* V = G + Q * Q;          // Q x Q   (accumulated rotations)
-  double* sig = V + Q * Q;        // Q
-  double* ordd = sig + Q;         // Q       (sort permutation, stored as doubles)
-
-  if (tid == 0) s_bad = 0;
-  __syncthreads();
-  for (int64_t e = tid; e < P * Q; e += blockDim.x) {
-    const int64_t i = e % P, j = e / P;
-    const double v = p.A[i * x_rs + j * x_cs];
-    if (!isfinite(v)) s_bad = 1;
-    Wq[e] = v;
-  }
-  __syncthreads();
-  if (s_bad) {
-    if (tid == 0) p.info[0] = TTX_ENONFINITE;
-    return;
-  }
-  const bool use_qr = P > Q;
-  if (use_qr) {
-    block_householder_qr(P, Q, Wq, tau, Qx, red);
-    for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
-      const int64_t i = e % Q, c = e / Q;
-      G[e] = (i <= c) ? Wq[i + c * P] : 0.0;
-    }
-  } else {
-    for (int64_t e = tid; e < Q * Q; e += blockDim.x) G[e] = Wq[e];
-  }
-  for (int64_t e = tid; e < Q * Q; e += blockDim.x) V[e] = ((e % Q) == (e / Q)) ? 1.0 : 0.0;
-  __syncthreads();
-
-  // one-sided Jacobi on the columns of G, round-robin pairs per round
-  const int64_t QQ = Q + (Q & 1);
-  const double tol = (double)Q * kEps;
-  int converged = (Q < 2);
-  for (int sweep = 0; sweep < 80 && !converged; ++sweep) {
-    if (tid == 0) s_rot = 0;
-    __syncthreads();
-    for (int64_t round = 0; round < QQ - 1; ++round) {
-      for (int64_t pk = wave; pk < QQ / 2; pk += nwave) {
-        int64_t i = rr_player(pk, round, QQ);
-        int64_t j = rr_player(QQ - 1 - pk, round, QQ);
-        if (i >= Q || j >= Q) continue;
-        if (i > j) { const int64_t t = i; i = j; j = t; }
-        double* gi = G + i * Q;
-        double* gj = G + j * Q;
-        double a = 0.0, b = 0.0, c = 0.0;
-        for (int64_t r = lane; r < Q; r += 64) {
-          const double x = gi[r], y = gj[r];
-          a += x * x;
-          b += y * y;
-          c += x * y;
-        }
-        a = wave_sum(a);
-        b = wave_sum(b);
-        c = wave_sum(c);
-        if (c == 0.0 || a == 0.0 || b == 0.0) continue;
-        if (fabs(c) <= tol * sqrt(a) * sqrt(b)) continue;
-        const double zeta = (b - a) / (2.0 * c);
-        double t;
-        if (fabs(zeta) > 1e150)
-          t = 0.5 / zeta;
-        else
-          t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        const double cs = 1.0 / sqrt(1.0 + t * t);
-        const double sn = cs * t;
-        for (int64_t r = lane; r < Q; r += 64) {
-          const double x = gi[r], y = gj[r];
-          gi[r] = cs * x - sn * y;
-          gj[r] = sn * x + cs * y;
-        }
-        double* vi = V + i * Q;
-        double* vj = V + j * Q;
-        for (int64_t r = lane; r < Q; r += 64) {
-          const double x = vi[r], y = vj[r];
-          vi[r] = cs * x - sn * y;
-          vj[r] = sn * x + cs * y;
-        }
-        if (lane == 0) s_rot = 1;
-      }
-      __syncthreads();
-    }
-    converged = (s_rot == 0);
-    __syncthreads();
-  }
-  if (tid == 0) p.info[0] = converged ? TTX_OK : TTX_ENOCONV;
-  __syncthreads();
-
-  // singular values = column norms
-  for (int64_t j = wave; j < Q; j += nwave) {
-    const double* gj = G + j * Q;
-    double s = 0.0;
-    for (int64_t r = lane; r < Q; r += 64) s += gj[r] * gj[r];
-    s = wave_sum(s);
-    if (lane == 0) sig[j] = sqrt(s);
-  }
-  __syncthreads();
-  // descending order, stable on ties (selection by rank count)
-  for (int64_t j = tid; j < Q; j += blockDim.x) {
-    const double sj = sig[j];
-    int64_t pos = 0;
-    for (int64_t l = 0; l < Q; ++l) {
-      const double sl = sig[l];
-      if (sl > sj || (sl == sj && l < j)) ++pos;
-    }
-    ordd[pos] = (double)j;
-  }
-  __syncthreads();
-  // normalise the columns of G in place: G[:, j] /= sig[j]
-  for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
-    const int64_t j = e / Q;
-    const double s = sig[j];
-    G[e] = s > 0.0 ? G[e] / s : 0.0;
-  }
-  __syncthreads();
-  // Ux (P x Q) column o = (use_qr ? Qx : I) * G[:, ord[o]];  V column ord[o]
-  // tall: U_A = Ux, Vt_A = V^T.  wide: U_A = V, Vt_A = Ux^T.
-  for (int64_t e = tid; e < P * Q; e += blockDim.x) {
-    const int64_t r = e % P, o = e / P;
-    const int64_t src = (int64_t)ordd[o];
-    double v;
-    if (use_qr) {
-      v = 0.0;
-      const double* gcol = G + src * Q;
-      for (int64_t l = 0; l < Q; ++l) v += Qx[r + l * P] * gcol[l];
-    } else {
-      v = G[r + src * Q];
-    }
-    if (!wide)
-      p.U[r * p.u_rs + o * p.u_cs] = v;
-    else
-      p.Vt[o * p.v_rs + r * p.v_cs] = v;
-  }
-  for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
-    const int64_t r = e % Q, o = e / Q;
-    const int64_t src = (int64_t)ordd[o];
-    const double v = V[r + src * Q];
-    if (!wide)
-      p.Vt[o * p.v_rs + r * p.v_cs] = v;
-    else
-      p.U[r * p.u_rs + o * p.u_cs] = v;
-  }
-  for (int64_t o = tid; o < Q; o += blockDim.x) p.S[o] = sig[(int64_t)ordd[o]];
+  block_svd(p, IN_LDS ? smem : p.work, red);
 }
 
 // ---------------------------------------------------------------------------

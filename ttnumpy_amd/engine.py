@@ -1,0 +1,343 @@
+"""Batched, device-resident AMEn with continuous batching.
+
+Each request occupies a *slot*: a descriptor (``csrc/engine.h``) plus one
+arena in HBM holding its solution and residual trains at a fixed rank
+capacity, its environments and its scratch.  ``ttx_engine_tick`` advances
+every active slot by one site step (environments, local solve, truncation and
+enrichment, or the per-sweep residual check); rank decisions and the stopping
+test are taken on the device.  The host replays ticks, polls the slot status
+every few ticks, retires finished requests and admits queued ones into free
+slots when HBM allows -- requests at different sites, sweeps and ranks share
+every launch.
+
+The arithmetic and its order follow ``ttnumpy/solvers/amen.py:395-725``; the
+result of one slot does not depend on the other slots (one workgroup per slot,
+no cross-slot reductions), so a request gives the same bits alone and in a mix.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import device as D
+from .solvers.common import SolverInfo, StopReason, _SolverSettings
+from .tensor_train import TensorTrain
+
+MAXD = 64
+MAXSWEEPS = 64
+SCR_COUNT = 17
+
+PH_XORTH, PH_ZORTH, PH_DONE = 1, 2, 14
+SLOT_RUNNING, SLOT_DONE, SLOT_EMPTY = 0, 1, -100
+_STATUS_TEXT = {-1: "rank capacity exceeded", -2: "NaN or infinite entries",
+                -3: "singular local matrix"}
+
+_i64 = ctypes.c_int64
+_f64 = ctypes.c_double
+_vp = ctypes.c_void_p
+
+
+class Slot(ctypes.Structure):
+    """Host mirror of ``ttx_slot`` (every field 8 bytes wide)."""
+
+    _fields_ = (
+        [(nm, _i64) for nm in ("d", "kick", "max_rank", "direct_size", "iterative", "residual_mode",
+                               "max_sweeps", "track", "rmax", "nmax", "ramax", "rbmax", "bneg",
+                               "allow_fallback", "chunk", "pad0")]
+        + [("n", _i64 * MAXD), ("ra", _i64 * (MAXD + 1)), ("rb", _i64 * (MAXD + 1))]
+        + [(nm, _f64) for nm in ("residual_tol", "trunc_tol", "local_rtol", "damp", "error_threshold",
+                                 "stagnation", "bnorm", "pad1")]
+        + [(nm, _i64) for nm in ("cap_core", "cap_zcore", "cap_loc", "direct_cap", "lmax", "scr_size",
+                                 "env_size", "pad2")]
+        + [("cap_env", _i64 * 5), ("pad3", _i64 * 3), ("env_off", _i64 * 10), ("pad4", _i64 * 6),
+           ("scr_off", _i64 * 32)]
+        + [("A", _vp * MAXD), ("b", _vp * MAXD), ("x", _vp), ("z", _vp), ("env", _vp), ("scr", _vp)]
+        + [("rx", _i64 * (MAXD + 1)), ("rz", _i64 * (MAXD + 1))]
+        + [(nm, _i64) for nm in ("phase", "k", "sweep", "status", "stop_reason", "local_solves",
+                                 "local_inexact", "local_fallbacks", "rank_decisions", "rank_capped",
+                                 "red_last", "svd_noconv", "ticks", "pad6a", "pad6b", "pad6c")]
+        + [("residuals", _f64 * MAXSWEEPS), ("max_bonds", _i64 * MAXSWEEPS)]
+    )
+
+
+def _check_layout(lib) -> None:
+    lib.ttx_engine_slot_bytes.restype = _i64
+    lib.ttx_engine_field_offset.restype = _i64
+    lib.ttx_engine_field_offset.argtypes = [ctypes.c_int]
+    lib.ttx_engine_plan.argtypes = [ctypes.c_void_p]
+    lib.ttx_engine_plan.restype = ctypes.c_int
+    lib.ttx_engine_tick.argtypes = [_vp, _vp, _i64, _i64]
+    lib.ttx_engine_tick.restype = ctypes.c_int
+    if lib.ttx_engine_slot_bytes() != ctypes.sizeof(Slot):
+        raise RuntimeError("ttx_slot layout mismatch between libttx and the host mirror")
+    for i, name in enumerate(("n", "residual_tol", "cap_core", "A", "rx", "residuals", "max_bonds")):
+        if lib.ttx_engine_field_offset(i) != getattr(Slot, name).offset:
+            raise RuntimeError(f"ttx_slot field {name} offset mismatch")
+
+
+@dataclass
+class Request:
+    """One AMEn solve: operator A, right-hand side b, initial guess x0 and options."""
+
+    A: TensorTrain
+    b: TensorTrain
+    x0: Optional[TensorTrain] = None
+    kickrank: int = 4
+    error_threshold: float = 1e-7
+    max_rank: Optional[int] = None
+    max_sweeps: int = 1
+    residual_tol: Optional[float] = None
+    residual_damp: float = 2.0
+    stagnation_factor: object = "auto"
+    local_solver: str = "auto"
+    direct_solve_size: int = 1000
+    local_rtol: Optional[float] = None
+    symmetric: bool = False
+    seed: Optional[int] = None
+    tag: object = None
+    arrival: float = 0.0
+    # filled by the engine
+    result: Optional[TensorTrain] = None
+    info: Optional[SolverInfo] = None
+    error: Optional[str] = None
+    admitted: float = 0.0
+    finished: float = 0.0
+
+
+class Engine:
+    """A fixed set of slots on one GPU plus the continuous-batching loop."""
+
+    def __init__(self, nslots: int, rmax: int, chunk: int = 8, hbm_reserve_bytes: int = 2 << 30):
+        self.lib = _lib.load()
+        _check_layout(self.lib)
+        self.dev = D.device()
+        self.nslots = int(nslots)
+        self.rmax = int(rmax)
+        self.chunk = int(chunk)
+        self.reserve = int(hbm_reserve_bytes)
+        self.host = (Slot * self.nslots)()
+        for s in self.host:
+            s.status = SLOT_EMPTY
+        self.nbytes = ctypes.sizeof(Slot)
+        self.dev_slots = torch.zeros(self.nslots * self.nbytes, dtype=torch.uint8, device=self.dev)
+        self.pinned = torch.zeros(self.nslots * self.nbytes, dtype=torch.uint8).pin_memory()
+        self.slot_req: List[Optional[Request]] = [None] * self.nslots
+        self.slot_mem: List[Optional[tuple]] = [None] * self.nslots
+        self._push_all()
+
+    # ------------------------------------------------------------ host <-> device slots
+    def _host_bytes(self) -> np.ndarray:
+        return np.frombuffer(self.host, dtype=np.uint8)
+
+    def _push_all(self) -> None:
+        self.pinned.numpy()[:] = self._host_bytes()
+        self.dev_slots.copy_(self.pinned, non_blocking=False)
+
+    def _push(self, i: int) -> None:
+        a, b = i * self.nbytes, (i + 1) * self.nbytes
+        self.pinned.numpy()[a:b] = self._host_bytes()[a:b]
+        self.dev_slots[a:b].copy_(self.pinned[a:b], non_blocking=False)
+
+    def _pull(self) -> None:
+        self.pinned.copy_(self.dev_slots, non_blocking=False)
+        self._host_bytes()[:] = self.pinned.numpy()
+
+    # ------------------------------------------------------------ admission
+    def _plan(self, req: Request) -> Slot:
+        A, b = req.A, req.b
+        d = len(A)
+        if d > MAXD:
+            raise ValueError(f"at most {MAXD} cores per request")
+        s = Slot()
+        s.d = d
+        s.kick = req.kickrank
+        s.max_rank = 0 if req.max_rank is None else int(req.max_rank)
+        s.direct_size = int(req.direct_solve_size)
+        s.rmax = self.rmax
+        s.chunk = self.chunk
+        for i in range(d):
+            s.n[i] = A.row_modes[i]
+        for i in range(d + 1):
+            s.ra[i] = A.bonds[i]
+            s.rb[i] = b.bonds[i]
+        rc = self.lib.ttx_engine_plan(ctypes.byref(s))
+        if rc != 0:
+            raise ValueError("engine planner rejected the request")
+        return s
+
+    def slot_bytes_needed(self, req: Request) -> int:
+        s = self._plan(req)
+        d = s.d
+        return 8 * (d * s.cap_core + d * s.cap_zcore + s.env_size + s.scr_size)
+
+    def admit(self, i: int, req: Request) -> None:
+        """Place ``req`` in free slot ``i``: arena, descriptor, initial trains."""
+        if req.kickrank <= 0:
+            raise ValueError(f"kickrank must be a positive integer, got {req.kickrank}")
+        if req.max_rank is not None and req.max_rank <= 0:
+            raise ValueError(f"max_rank must be a positive integer, got {req.max_rank}")
+        if req.residual_damp < 1.0:
+            raise ValueError(f"residual_damp must be >= 1, got {req.residual_damp}")
+        A, b = req.A, req.b
+        d = len(A)
+        st = _SolverSettings.resolve(
+            max_sweeps=req.max_sweeps, residual_tol=req.residual_tol,
+            stagnation_factor=req.stagnation_factor, local_solver=req.local_solver,
+            symmetric=req.symmetric, direct_solve_size=req.direct_solve_size, return_info=True,
+            local_rtol=req.local_rtol, truncation_steps=max(2 * (d - 1), 1))
+        if st.iterative_solver is not None and str(st.iterative_solver) == "gmres":
+            raise NotImplementedError("the engine's iterative local solver is MINRES (symmetric=True)")
+        s = self._plan(req)
+        s.iterative = 1 if st.iterative_solver is not None else 0
+        s.residual_mode = 1 if st.residual_tol is not None else 0
+        s.max_sweeps = st.max_sweeps
+        s.track = 1 if st.track_residual else 0
+        s.residual_tol = st.residual_tol if st.residual_tol is not None else 0.0
+        s.trunc_tol = st.truncation_tol if st.truncation_tol is not None else 0.0
+        s.local_rtol = st.local_rtol if st.local_rtol is not None else 0.0
+        s.damp = req.residual_damp
+        s.error_threshold = req.error_threshold
+        s.stagnation = st.stagnation_factor if st.stagnation_factor is not None else 0.0
+        s.bneg = b.orth_center if b.orth_center is not None else 0
+        s.bnorm = b.tt_norm() if st.track_residual else 0.0
+        x0 = req.x0
+        if x0 is None:
+            from .tensor_train import ones
+            x0 = ones(b.physical_dims(), ranks=b.bonds)
+        if max(x0.bonds) > self.rmax:
+            raise ValueError(f"x0 bond {max(x0.bonds)} exceeds the slot capacity {self.rmax}")
+        # arena: x | z | env | scratch
+        nx, nz = d * s.cap_core, d * s.cap_zcore
+        arena = torch.zeros(nx + nz + s.env_size + s.scr_size, dtype=D.F64, device=self.dev)
+        base = arena.data_ptr()
+        s.x = base
+        s.z = base + 8 * nx
+        s.env = base + 8 * (nx + nz)
+        s.scr = base + 8 * (nx + nz + s.env_size)
+        keep = [arena, A, b]
+        for k in range(d):
+            s.A[k] = A.cores[k].data_ptr()
+            s.b[k] = b.cores[k].data_ptr()
+            D.copy_nd([x0.cores[k].numel()], D.P(x0.cores[k]), [1], s.x + 8 * k * s.cap_core, [1])
+        # z0 = rand(modes, kickrank, seed) in the reference's draw order (amen.py:80-106)
+        rng = np.random.default_rng(req.seed)
+        zr = [1] + [req.kickrank] * (d - 1) + [1]
+        for k in range(d):
+            core = rng.random((zr[k], s.n[k], 1, zr[k + 1]))
+            t = D.to_device(core)
+            keep.append(t)
+            D.copy_nd([t.numel()], D.P(t), [1], s.z + 8 * k * s.cap_zcore, [1])
+        for k in range(d + 1):
+            s.rx[k] = x0.bonds[k]
+            s.rz[k] = zr[k]
+        # x0 is right-orthonormalised first unless its centre is already core 0
+        if x0.orth_center == 0:
+            s.phase, s.k = PH_ZORTH, d - 1
+        else:
+            s.phase, s.k = PH_XORTH, (x0.orth_center if x0.orth_center is not None else d - 1)
+            if s.k < 1:
+                s.phase, s.k = PH_ZORTH, d - 1
+        s.status = SLOT_RUNNING
+        ctypes.memmove(ctypes.addressof(self.host[i]), ctypes.addressof(s), ctypes.sizeof(Slot))
+        self.slot_req[i] = req
+        self.slot_mem[i] = tuple(keep)
+        req.admitted = time.perf_counter()
+        self._push(i)
+
+    # ------------------------------------------------------------ retirement
+    def _retire(self, i: int) -> None:
+        s = self.host[i]
+        req = self.slot_req[i]
+        if s.status == SLOT_DONE:
+            d = s.d
+            arena = self.slot_mem[i][0]
+            cores = []
+            for k in range(d):
+                r0, n, r1 = s.rx[k], s.n[k], s.rx[k + 1]
+                off = k * s.cap_core
+                cores.append(arena[off:off + r0 * n * r1].reshape(r0, n, 1, r1).clone())
+            x = TensorTrain(cores)
+            x.orth_center = 0
+            x.max_rank = max(x.bonds)
+            info = SolverInfo()
+            nsw = int(s.sweep) + 1
+            info.num_sweeps_run = nsw
+            if s.track:
+                info.residuals = [float(s.residuals[j]) for j in range(min(nsw, MAXSWEEPS))]
+            info.max_bonds = [int(s.max_bonds[j]) for j in range(min(nsw, MAXSWEEPS))]
+            info.stop_reason = {1: StopReason.CONVERGED, 2: StopReason.STAGNATED}.get(
+                int(s.stop_reason), StopReason.MAX_SWEEPS)
+            info.local_solves = int(s.local_solves)
+            info.local_inexact = int(s.local_inexact)
+            info.local_fallbacks = int(s.local_fallbacks)
+            req.result, req.info = x, info
+        else:
+            req.error = _STATUS_TEXT.get(int(s.status), f"status {int(s.status)}")
+        req.finished = time.perf_counter()
+        s.status = SLOT_EMPTY
+        self.slot_req[i] = None
+        self.slot_mem[i] = None
+        self._push(i)
+
+    # ------------------------------------------------------------ loop
+    def tick(self, n: int = 1) -> None:
+        rc = self.lib.ttx_engine_tick(D.stream(), self.dev_slots.data_ptr(), self.nslots, n)
+        _lib.check(rc, "ttx_engine_tick")
+
+    def active(self) -> int:
+        return sum(1 for r in self.slot_req if r is not None)
+
+    def run(self, queue: Sequence[Request], poll: int = 8, clock=None, max_ticks: int = 10 ** 7,
+            on_done=None) -> List[Request]:
+        """Serve ``queue`` (ordered by arrival time) with continuous batching.
+
+        A request is admitted once its arrival time has passed (by ``clock``,
+        default: immediately), a slot is free and HBM has room; every ``poll``
+        ticks finished requests are retired.
+        """
+        pending = list(queue)
+        done: List[Request] = []
+        ticks = 0
+        t0 = time.perf_counter()
+        now = clock or (lambda: float("inf"))
+        while pending or self.active():
+            # admit
+            free = [i for i in range(self.nslots) if self.slot_req[i] is None]
+            while pending and free and pending[0].arrival <= now() - 0.0:
+                need = self.slot_bytes_needed(pending[0])
+                free_b, _ = torch.cuda.mem_get_info(self.dev)
+                if free_b - need < self.reserve and self.active():
+                    break
+                self.admit(free.pop(0), pending.pop(0))
+            if not self.active():
+                if pending:
+                    time.sleep(max(0.0, min(0.001, pending[0].arrival - now())))
+                continue
+            self.tick(poll)
+            ticks += poll
+            self._pull()
+            for i in range(self.nslots):
+                if self.slot_req[i] is not None and self.host[i].status != SLOT_RUNNING:
+                    req = self.slot_req[i]
+                    self._retire(i)
+                    done.append(req)
+                    if on_done is not None:
+                        on_done(req)
+            if ticks > max_ticks:
+                raise RuntimeError("engine did not finish within the tick limit")
+        return done
+
+
+def amen_batched(requests: Sequence[Request], nslots: Optional[int] = None, rmax: int = 64,
+                 poll: int = 8) -> List[Request]:
+    """Solve all ``requests`` on the engine; returns them with ``result``/``info`` set."""
+    eng = Engine(nslots or len(requests), rmax)
+    return eng.run(list(requests), poll=poll)
