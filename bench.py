@@ -1,12 +1,15 @@
 """Benchmark: AMEn solves/s and latency on the 2D Poisson 256x256 QTT system, one MI355X.
 
-Workload (one "step" = one full solve, inputs resident in HBM before the
-clock starts): A = -Laplace on a 256 x 256 grid (Dirichlet, h = 1/257) as
-the Kronecker-sum QTT operator of 16 cores with 2x2 modes and TTM bonds <= 4,
+Workload (C1): A = -Laplace on a 256 x 256 grid (Dirichlet, h = 1/257) as the
+Kronecker-sum QTT operator of 16 cores with 2x2 modes and TTM bonds <= 4,
 b = all-ones rank-1 train, x0 = rand([2]*16, ranks=2, seed=0);
 amen(symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0) in FP64.
 
-Prints one JSON line (rank 0).  ``--gpus N`` with N > 1 exits: this build
+One step = one batch of ``--batch`` such solves served concurrently by the
+device-resident engine (one slot per request, continuous batching); inputs
+are resident in HBM before the clock starts.  ``value`` is solves per second
+over the timed steps; p50/p99 are per-request latencies (admission to
+result).  Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build
 runs on one GPU.
 """
 
@@ -24,8 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 N_BITS = 8  # 256 x 256 grid
+RMAX = 48   # slot rank capacity (the C1 solution reaches bond 33)
 PEAK_FP64_TFLOPS = 78.6
-PEAK_HBM_GBS = 8000.0
+KERNELS = ("ttx::eng_env_kernel", "ttx::eng_solve_kernel", "ttx::eng_post_kernel",
+           "ttx::eng_check_kernel")
 
 
 def build_inputs():
@@ -38,53 +43,14 @@ def build_inputs():
     return A, b, x0
 
 
-def solve(A, b, x0):
-    import ttnumpy_amd as tx
-    return tx.amen(A, b, x0=x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0,
-                   return_info=True)
-
-
-def kernel_roofline(A, b, x0, kernel: str = "gemm"):
-    """Per-launch timing of the dominant kernel over one extra solve, with HIP events.
-
-    Events are recorded on the stream the kernels are launched on (torch's
-    current stream, which ttx launches onto).  Algorithmic flops per launch
-    are 2*M*N*K*batch for the GEMM.
-    """
-    import torch
-    from ttnumpy_amd import device as D
-    rec = []
-    orig = D.gemm
-
-    def traced(batch, M, N, K, *args):
-        s = torch.cuda.Event(enable_timing=True)
-        e = torch.cuda.Event(enable_timing=True)
-        s.record()
-        orig(batch, M, N, K, *args)
-        e.record()
-        rec.append((2.0 * batch * M * N * K, s, e))
-
-    D.gemm = traced
-    try:
-        solve(A, b, x0)
-    finally:
-        D.gemm = orig
-    torch.cuda.synchronize()
-    flops = sum(r[0] for r in rec)
-    ms = sum(r[1].elapsed_time(r[2]) for r in rec)
-    launches = len(rec)
-    avg_us = 1e3 * ms / max(launches, 1)
-    achieved = (flops / launches) / (avg_us * 1e-6) / 1e12 if launches else 0.0
-    return {
-        "bound": "mfma", "kernel": "ttx::gemm_f64_kernel", "achieved": achieved,
-        "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-        "traffic": None, "launches_per_solve": launches, "avg_launch_us": avg_us,
-        "flops_per_launch": flops / max(launches, 1),
-    }
+def requests(A, b, x0, n):
+    from ttnumpy_amd.engine import Request
+    return [Request(A=A, b=b, x0=x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0)
+            for _ in range(n)]
 
 
 def cpu_baseline(budget_s: float = 12.0):
-    """The oracle's AMEn on the same C1 inputs on the host, 1 BLAS thread."""
+    """The oracle's AMEn (FP64 NumPy/SciPy) on the same C1 inputs, one BLAS thread."""
     from threadpoolctl import threadpool_limits
     from oracle import tt_oracle as O
     from ttnumpy_amd import problems as P
@@ -102,50 +68,90 @@ def cpu_baseline(budget_s: float = 12.0):
             if time.perf_counter() > t_end or len(times) >= 20:
                 break
     return {"value": len(times) / sum(times), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"{len(times)} C1 solves (oracle FP64 NumPy/SciPy, OpenBLAS 1 thread), "
-                      f"median {np.median(times):.3f}s"}
+            "sample": f"{len(times)} C1 solves, oracle FP64 NumPy/SciPy with OpenBLAS on 1 thread, "
+                      f"median {np.median(times):.3f} s per solve"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--poll", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is documented, not built)", file=sys.stderr)
+              "(how the path would shard is described in the design notes)", file=sys.stderr)
         sys.exit(2)
     import torch
+    from ttnumpy_amd.engine import Engine
     assert torch.cuda.is_available(), "bench.py needs the MI355X"
     A, b, x0 = build_inputs()
+    eng = Engine(args.batch, rmax=RMAX)
     for _ in range(args.warmup):
-        solve(A, b, x0)
+        eng.run(requests(A, b, x0, args.batch), poll=args.poll)
     torch.cuda.synchronize()
-    lat = []
-    info = None
+    lat, done = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        s = time.perf_counter()
-        x, info = solve(A, b, x0)
-        torch.cuda.synchronize()
-        lat.append(time.perf_counter() - s)
+        out = eng.run(requests(A, b, x0, args.batch), poll=args.poll)
+        done += out
+        lat += [r.finished - r.admitted for r in out]
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
-    value = args.steps / total
-    roof = kernel_roofline(A, b, x0)
+    errors = [r.error for r in done if r.error]
+    ok = [r for r in done if r.error is None]
+    nsolved = len(ok)
+    value = nsolved / total
+
+    # one request alone (latency floor of a single solve on this engine)
+    single = Engine(1, rmax=RMAX)
+    single.run(requests(A, b, x0, 1), poll=args.poll)
+    t1 = time.perf_counter()
+    single.run(requests(A, b, x0, 1), poll=args.poll)
+    torch.cuda.synchronize()
+    single_ms = 1e3 * (time.perf_counter() - t1)
+
+    # dominant-kernel roofline: one more batch replayed tick by tick with HIP
+    # events around each kernel on the engine's stream; algorithmic flops are
+    # banked per kernel by the device counters of each slot
+    prof = Engine(args.batch, rmax=RMAX)
+    prof.profiling = True
+    prof.run(requests(A, b, x0, args.batch), poll=args.poll)
+    j = int(np.argmax(prof.kernel_ms))
+    launches = prof.kernel_launches
+    avg_ms = prof.kernel_ms[j] / launches
+    flops_per_launch = prof.kernel_flops[j] / launches
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "kernel": KERNELS[j], "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+            "avg_launch_us": 1e3 * avg_ms, "flops_per_launch": flops_per_launch,
+            "launches": launches, "kernel_ms_share": {
+                k.split("::")[1]: round(float(v / prof.kernel_ms.sum()), 4)
+                for k, v in zip(KERNELS, prof.kernel_ms)}}
+    r0 = ok[0] if ok else None
     line = {
         "metric": "AMEn solves/s + p99 latency, mixed vs FP64 on same GPU at residual 1e-8",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "C1: 2D Poisson 256x256 QTT (d=16, TTM bonds<=4), b=ones, "
-                               "x0=rand(ranks=2,seed=0), AMEn residual_tol=1e-8 FP64",
-                   "precision": "fp64-plain"},
+                               "x0=rand(ranks=2,seed=0), amen(symmetric, residual_tol=1e-8, "
+                               "max_sweeps=20, seed=0), FP64",
+                   "batch": args.batch, "engine": "device-resident, one slot per request",
+                   "rank_capacity": RMAX},
         "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
-        "final_residual": info.residuals[-1], "sweeps": info.num_sweeps_run,
-        "stop_reason": str(info.stop_reason), "max_bond": max(x.bonds),
+        "single_request_ms": single_ms,
+        "solves_ok": nsolved, "errors": errors[:3],
+        "final_residual": r0.info.residuals[-1] if r0 else None,
+        "sweeps": r0.info.num_sweeps_run if r0 else None,
+        "stop_reason": str(r0.info.stop_reason) if r0 else None,
+        "max_bond": max(r0.result.bonds) if r0 else None,
+        "minres_iters_per_solve": prof.minres_iters / max(args.batch, 1),
+        "lu_solves_per_solve": prof.lu_solves / max(args.batch, 1),
+        "gflop_per_solve": float(prof.kernel_flops.sum()) / max(args.batch, 1) / 1e9,
         "roofline": roof,
     }
     if not args.no_cpu:

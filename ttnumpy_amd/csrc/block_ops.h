@@ -10,6 +10,13 @@ namespace ttx {
 
 constexpr double kEps = DBL_EPSILON;
 
+// Algorithmic flop tally of the block-level routines of one workgroup
+// (thread 0 adds; the engine kernels zero it on entry and bank it per slot).
+static __shared__ double g_flops;
+__device__ __forceinline__ void add_flops(double f) {
+  if (threadIdx.x == 0) g_flops += f;
+}
+
 // ---------------------------------------------------------------------------
 // Householder QR on a column-major m x n matrix W (leading dim m) held in
 // global memory.  On return W holds R in its upper triangle and the reflector
@@ -24,6 +31,11 @@ __device__ __forceinline__ void block_householder_qr(int64_t m, int64_t n, doubl
   const int nwave = blockDim.x >> 6;
   const int64_t k = m < n ? m : n;
   __shared__ double s_tau;
+  {
+    const double dm = (double)m, dn = (double)n, dk = (double)k;
+    add_flops(2.0 * dm * dn * dk - (dm + dn) * dk * dk + 2.0 * dk * dk * dk / 3.0);
+    if (form_q) add_flops(4.0 * dm * dk * dk - 4.0 * dk * dk * dk / 3.0);
+  }
   for (int64_t j = 0; j < k; ++j) {
     double* col = W + j * m;
     double s = 0.0;
@@ -118,6 +130,8 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   const int64_t Q = wide ? p.m : p.n;
   const int64_t x_rs = wide ? p.a_cs : p.a_rs;
   const int64_t x_cs = wide ? p.a_rs : p.a_cs;
+  // Jacobi on the Q x Q factor and U = Q_x U_r (the QR of the tall side counts itself)
+  add_flops(20.0 * (double)Q * (double)Q * (double)Q + 2.0 * (double)P * (double)Q * (double)Q);
   double* Wq = work;              // P x Q   (QR working matrix)
   double* tau = Wq + P * Q;       // Q
   double* Qx = tau + Q;           // P x Q   (orthonormal factor)
@@ -283,6 +297,7 @@ __device__ __forceinline__ void bgemm(int64_t batch, int64_t M, int64_t N, int64
   const int r = lane & 15, kq = lane >> 4;
   const int64_t tm = (M + 31) >> 5, tn = (N + 31) >> 5;
   const int64_t per = tm * tn;
+  add_flops(2.0 * (double)batch * (double)M * (double)N * (double)K);
   for (int64_t t = wave; t < batch * per; t += nw) {
     const int64_t bt = t / per, tt = t % per;
     const int64_t m0 = (tt / tn) << 5, n0 = (tt % tn) << 5;
@@ -335,6 +350,7 @@ __device__ __forceinline__ void bgemv(int64_t rows, int64_t cols, double alpha, 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
+  add_flops(2.0 * (double)rows * (double)cols);
   for (int64_t r = wave; r < rows; r += nw) {
     const double* mr = M + r * ld;
     double s = 0.0;
@@ -362,6 +378,10 @@ __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W,
   const int wave = tid >> 6;
   const int nwave = blockDim.x >> 6;
   const int64_t ncols = n + 1;
+  {
+    const double dn = (double)n;
+    add_flops(2.0 * dn * dn * dn / 3.0 + 2.0 * dn * dn);
+  }
   for (int64_t k = 0; k < n; k += nb0) {
     const int64_t nb = (n - k) < nb0 ? (n - k) : nb0;
     const int64_t mp = n - k;

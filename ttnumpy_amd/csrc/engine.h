@@ -64,7 +64,8 @@ typedef struct ttx_slot {
   int64_t rx[TTX_MAXD + 1];
   int64_t rz[TTX_MAXD + 1];
   int64_t phase, k, sweep, status, stop_reason, local_solves, local_inexact, local_fallbacks;
-  int64_t rank_decisions, rank_capped, red_last, svd_noconv, ticks, pad6[3];
+  int64_t rank_decisions, rank_capped, red_last, svd_noconv, ticks, minres_iters, lu_solves, pad6;
+  double flops[4];  // algorithmic flops banked per kernel: env, solve, post, check
   double residuals[TTX_MAXSWEEPS];
   int64_t max_bonds[TTX_MAXSWEEPS];
 } ttx_slot;

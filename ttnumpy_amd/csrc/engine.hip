@@ -134,6 +134,7 @@ struct LocOp {
 
 __device__ void local_matrix_blk(const LocOp& op, double* out) {
   const int64_t rows = op.rows(), cols = op.cols();
+  add_flops(2.0 * (double)rows * (double)cols * (double)op.Ra * (double)op.Rb);
   for (int64_t e = threadIdx.x; e < rows * cols; e += blockDim.x) {
     const int64_t col = e % cols, row = e / cols;
     const int64_t f_r = row % op.fr, i = (row / op.fr) % op.nr, f_l = row / (op.fr * op.nr);
@@ -219,7 +220,7 @@ __device__ int64_t shift_left_blk(double* core_k, double* core_km1, const double
 // recurrences of krylov.hip; V holds 9 n doubles (x, v, y, r1, r2, w, w1, w2,
 // yA).  x0 may alias V's x.  Returns with x in V[0..n).
 __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const double* x0, double rtol,
-                           int64_t maxiter, double* V, double* red) {
+                           int64_t maxiter, double* V, double* red, int64_t* iters) {
   double* x = V;
   double* v = V + n;
   double* y = V + 2 * n;
@@ -325,11 +326,13 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
       if (test2 <= rtol) istop = 2;
       if (test1 <= rtol) istop = 1;
     }
+    add_flops(18.0 * (double)n);
     if (istop != 0) break;
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = r2[i] / beta;
     __syncthreads();
   }
   __syncthreads();
+  if (iters) *iters += itn;
 }
 
 // ---------------------------------------------------------------- slot phases
@@ -441,9 +444,7 @@ __device__ void fail(ttx_slot* s, int64_t code) {
 }
 
 // ---------------------------------------------------------------- kernels
-__global__ __launch_bounds__(kEngThreads) void eng_env_kernel(ttx_slot* slots) {
-  ttx_slot* s = slots + blockIdx.x;
-  const int kind = blockIdx.y;
+__device__ void env_body(ttx_slot* s, int kind) {
   if (s->status != TTX_SLOT_RUNNING) return;
   const int64_t ph = s->phase, d = s->d;
   int side;
@@ -491,12 +492,19 @@ __global__ __launch_bounds__(kEngThreads) void eng_env_kernel(ttx_slot* slots) {
                   with_op ? s->ra[c] : 1, with_op ? s->ra[c + 1] : 1, out, s1, s2);
 }
 
+__global__ __launch_bounds__(kEngThreads) void eng_env_kernel(ttx_slot* slots) {
+  ttx_slot* s = slots + blockIdx.x;
+  if (threadIdx.x == 0) g_flops = 0.0;
+  __syncthreads();
+  env_body(s, blockIdx.y);
+  __syncthreads();
+  if (threadIdx.x == 0 && g_flops != 0.0) atomicAdd(&s->flops[0], g_flops);
+}
+
 // one-site local solve (one_site.py:118-178, common.py:729-809)
-__global__ __launch_bounds__(kEngThreads) void eng_solve_kernel(ttx_slot* slots) {
-  extern __shared__ double lds[];
+__device__ void solve_body(ttx_slot* s, double* lds) {
   __shared__ double red[16];
   __shared__ int32_t s_info;
-  ttx_slot* s = slots + blockIdx.x;
   if (s->status != TTX_SLOT_RUNNING) return;
   if (s->phase != TTX_PH_FWD && s->phase != TTX_PH_BWD) return;
   const int64_t k = s->k;
@@ -518,7 +526,7 @@ __global__ __launch_bounds__(kEngThreads) void eng_solve_kernel(ttx_slot* slots)
     const double* x0 = xcore(s, k);  // warm start: the current core (one_site.py:151-153)
     bool ok = false;
     for (int attempt = 0; attempt < 4; ++attempt) {
-      minres_blk(op, N, bloc, x0, rt, 5 * N, V, red);
+      minres_blk(op, N, bloc, x0, rt, 5 * N, V, red, threadIdx.x == 0 ? &s->minres_iters : nullptr);
       x0 = V;
       op.apply(V, 1, tmp, 1.0, 0.0);
       const double res = bnorm2(tmp, bloc, N, red);
@@ -550,7 +558,10 @@ __global__ __launch_bounds__(kEngThreads) void eng_solve_kernel(ttx_slot* slots)
     block_lu_solve(N, N + 1, W, u, lds, nb0, red, &s_info);
     __syncthreads();
     if (s_info != TTX_OK) { fail(s, TTX_SLOT_ERR_SINGULAR); return; }
-    if (threadIdx.x == 0) scal[1] = 1.0;
+    if (threadIdx.x == 0) {
+      scal[1] = 1.0;
+      s->lu_solves += 1;
+    }
   }
   __syncthreads();
   if (s->residual_mode) {
@@ -563,10 +574,19 @@ __global__ __launch_bounds__(kEngThreads) void eng_solve_kernel(ttx_slot* slots)
   }
 }
 
-// truncation, enrichment and shifts after a solve; QR sweeps and z fit
-__global__ __launch_bounds__(kEngThreads) void eng_post_kernel(ttx_slot* slots) {
-  __shared__ double red[16];
+__global__ __launch_bounds__(kEngThreads) void eng_solve_kernel(ttx_slot* slots) {
+  extern __shared__ double lds[];
   ttx_slot* s = slots + blockIdx.x;
+  if (threadIdx.x == 0) g_flops = 0.0;
+  __syncthreads();
+  solve_body(s, lds);
+  __syncthreads();
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[1] += g_flops;
+}
+
+// truncation, enrichment and shifts after a solve; QR sweeps and z fit
+__device__ void post_body(ttx_slot* s) {
+  __shared__ double red[16];
   if (s->status != TTX_SLOT_RUNNING) return;
   const int64_t ph = s->phase, d = s->d, k = s->k;
   double* enr = scr(s, TTX_SCR_ENR);
@@ -732,6 +752,15 @@ __global__ __launch_bounds__(kEngThreads) void eng_post_kernel(ttx_slot* slots) 
   if (threadIdx.x == 0) s->rz[k] = kz;
 }
 
+__global__ __launch_bounds__(kEngThreads) void eng_post_kernel(ttx_slot* slots) {
+  ttx_slot* s = slots + blockIdx.x;
+  if (threadIdx.x == 0) g_flops = 0.0;
+  __syncthreads();
+  post_body(s);
+  __syncthreads();
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[2] += g_flops;
+}
+
 // ||A x - b|| / ||b|| in TT arithmetic (common.py:118-146): the residual
 // train is formed core by core and normed by a Q-less right QR sweep.
 __device__ double residual_norm_blk(ttx_slot* s, double* red) {
@@ -793,6 +822,8 @@ __device__ double residual_norm_blk(ttx_slot* s, double* red) {
   return nrm;
 }
 
+__device__ void check_decide(ttx_slot* s, int64_t sw, double res);
+
 __device__ void advance(ttx_slot* s) {
   const int64_t d = s->d;
   switch (s->phase) {
@@ -833,9 +864,8 @@ __device__ void advance(ttx_slot* s) {
 }
 
 // per-sweep monitor (common.py:504-592) and the phase advance of every slot
-__global__ __launch_bounds__(kEngThreads) void eng_check_kernel(ttx_slot* slots) {
+__device__ void check_body(ttx_slot* s) {
   __shared__ double red[16];
-  ttx_slot* s = slots + blockIdx.x;
   if (s->status != TTX_SLOT_RUNNING) return;
   if (s->phase != TTX_PH_CHECK) {
     __syncthreads();
@@ -852,7 +882,20 @@ __global__ __launch_bounds__(kEngThreads) void eng_check_kernel(ttx_slot* slots)
     res = s->bnorm == 0.0 ? rn : rn / s->bnorm;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x == 0) check_decide(s, sw, res);
+}
+
+__global__ __launch_bounds__(kEngThreads) void eng_check_kernel(ttx_slot* slots) {
+  ttx_slot* s = slots + blockIdx.x;
+  if (threadIdx.x == 0) g_flops = 0.0;
+  __syncthreads();
+  check_body(s);
+  __syncthreads();
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[3] += g_flops;
+}
+
+// the stopping test and the next phase after a completed sweep (thread 0)
+__device__ void check_decide(ttx_slot* s, int64_t sw, double res) {
   s->ticks += 1;
   int64_t mb = 0;
   for (int64_t i = 0; i <= s->d; ++i) mb = s->rx[i] > mb ? s->rx[i] : mb;
@@ -975,7 +1018,7 @@ extern "C" int ttx_engine_plan(ttx_slot* s) {
 }
 
 extern "C" int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks) {
-  if (nslots < 1 || ticks < 1) return TTX_EINVAL;
+  if (nslots < 1 || ticks < 0) return TTX_EINVAL;
   static bool attr = false;
   if (!attr) {
     if (hipFuncSetAttribute((const void*)eng_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -991,4 +1034,36 @@ extern "C" int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nsl
     hipLaunchKernelGGL(eng_check_kernel, dim3((unsigned)nslots), dim3(kEngThreads), 0, st, slots);
   }
   return launch_status();
+}
+
+// One-tick-at-a-time replay with HIP events around each of the four kernels;
+// ms_out[0..3] accumulate the elapsed time of env, solve, post and check.
+extern "C" int ttx_engine_profile(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
+                                  double* ms_out) {
+  if (nslots < 1 || ticks < 1) return TTX_EINVAL;
+  if (ttx_engine_tick(stream, slots, nslots, 0) == TTX_EHIP) return TTX_EHIP;
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t ev[5];
+  for (int i = 0; i < 5; ++i)
+    if (hipEventCreate(&ev[i]) != hipSuccess) return TTX_EHIP;
+  int rc = TTX_OK;
+  for (int64_t t = 0; t < ticks && rc == TTX_OK; ++t) {
+    hipEventRecord(ev[0], st);
+    hipLaunchKernelGGL(eng_env_kernel, dim3((unsigned)nslots, 5), dim3(kEngThreads), 0, st, slots);
+    hipEventRecord(ev[1], st);
+    hipLaunchKernelGGL(eng_solve_kernel, dim3((unsigned)nslots), dim3(kEngThreads), kEngLds, st, slots);
+    hipEventRecord(ev[2], st);
+    hipLaunchKernelGGL(eng_post_kernel, dim3((unsigned)nslots), dim3(kEngThreads), 0, st, slots);
+    hipEventRecord(ev[3], st);
+    hipLaunchKernelGGL(eng_check_kernel, dim3((unsigned)nslots), dim3(kEngThreads), 0, st, slots);
+    hipEventRecord(ev[4], st);
+    if (hipEventSynchronize(ev[4]) != hipSuccess) { rc = TTX_EHIP; break; }
+    for (int j = 0; j < 4; ++j) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev[j], ev[j + 1]) != hipSuccess) { rc = TTX_EHIP; break; }
+      ms_out[j] += (double)ms;
+    }
+  }
+  for (int i = 0; i < 5; ++i) hipEventDestroy(ev[i]);
+  return rc == TTX_OK ? launch_status() : rc;
 }

@@ -63,7 +63,8 @@ class Slot(ctypes.Structure):
         + [("rx", _i64 * (MAXD + 1)), ("rz", _i64 * (MAXD + 1))]
         + [(nm, _i64) for nm in ("phase", "k", "sweep", "status", "stop_reason", "local_solves",
                                  "local_inexact", "local_fallbacks", "rank_decisions", "rank_capped",
-                                 "red_last", "svd_noconv", "ticks", "pad6a", "pad6b", "pad6c")]
+                                 "red_last", "svd_noconv", "ticks", "minres_iters", "lu_solves", "pad6")]
+        + [("flops", _f64 * 4)]
         + [("residuals", _f64 * MAXSWEEPS), ("max_bonds", _i64 * MAXSWEEPS)]
     )
 
@@ -76,6 +77,8 @@ def _check_layout(lib) -> None:
     lib.ttx_engine_plan.restype = ctypes.c_int
     lib.ttx_engine_tick.argtypes = [_vp, _vp, _i64, _i64]
     lib.ttx_engine_tick.restype = ctypes.c_int
+    lib.ttx_engine_profile.argtypes = [_vp, _vp, _i64, _i64, ctypes.POINTER(_f64)]
+    lib.ttx_engine_profile.restype = ctypes.c_int
     if lib.ttx_engine_slot_bytes() != ctypes.sizeof(Slot):
         raise RuntimeError("ttx_slot layout mismatch between libttx and the host mirror")
     for i, name in enumerate(("n", "residual_tol", "cap_core", "A", "rx", "residuals", "max_bonds")):
@@ -131,6 +134,12 @@ class Engine:
         self.pinned = torch.zeros(self.nslots * self.nbytes, dtype=torch.uint8).pin_memory()
         self.slot_req: List[Optional[Request]] = [None] * self.nslots
         self.slot_mem: List[Optional[tuple]] = [None] * self.nslots
+        self.profiling = False
+        self.kernel_ms = np.zeros(4)     # env, solve, post, check (profiling runs only)
+        self.kernel_flops = np.zeros(4)  # algorithmic flops of retired requests
+        self.kernel_launches = 0
+        self.minres_iters = 0
+        self.lu_solves = 0
         self._push_all()
 
     # ------------------------------------------------------------ host <-> device slots
@@ -256,6 +265,9 @@ class Engine:
     def _retire(self, i: int) -> None:
         s = self.host[i]
         req = self.slot_req[i]
+        self.kernel_flops += np.array(list(s.flops))
+        self.minres_iters += int(s.minres_iters)
+        self.lu_solves += int(s.lu_solves)
         if s.status == SLOT_DONE:
             d = s.d
             arena = self.slot_mem[i][0]
@@ -289,6 +301,13 @@ class Engine:
 
     # ------------------------------------------------------------ loop
     def tick(self, n: int = 1) -> None:
+        if self.profiling:
+            ms = (_f64 * 4)(*([0.0] * 4))
+            rc = self.lib.ttx_engine_profile(D.stream(), self.dev_slots.data_ptr(), self.nslots, n, ms)
+            _lib.check(rc, "ttx_engine_profile")
+            self.kernel_ms += np.array(list(ms))
+            self.kernel_launches += n
+            return
         rc = self.lib.ttx_engine_tick(D.stream(), self.dev_slots.data_ptr(), self.nslots, n)
         _lib.check(rc, "ttx_engine_tick")
 
