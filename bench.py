@@ -80,6 +80,10 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--poll", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--single", action="store_true",
+                    help="also time one request alone (its launches are not part of the roofline)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -88,6 +92,16 @@ def main():
     import torch
     from ttnumpy_amd.engine import Engine
     assert torch.cuda.is_available(), "bench.py needs the MI355X"
+    if args.copy_calibration:
+        from ttnumpy_amd import device as D
+        n = args.copy_calibration * (1 << 20) // 8
+        src, dst = D.empty(n), D.empty(n)
+        D.fill(src, 1.0)
+        for _ in range(3):
+            D.copy_nd([n], D.P(src), [1], D.P(dst), [1])
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": 8 * n, "launches": 3}))
+        return
     A, b, x0 = build_inputs()
     eng = Engine(args.batch, rmax=RMAX)
     for _ in range(args.warmup):
@@ -106,13 +120,15 @@ def main():
     nsolved = len(ok)
     value = nsolved / total
 
-    # one request alone (latency floor of a single solve on this engine)
-    single = Engine(1, rmax=RMAX)
-    single.run(requests(A, b, x0, 1), poll=args.poll)
-    t1 = time.perf_counter()
-    single.run(requests(A, b, x0, 1), poll=args.poll)
-    torch.cuda.synchronize()
-    single_ms = 1e3 * (time.perf_counter() - t1)
+    single_ms = None
+    if args.single:
+        # one request alone (latency floor of a single solve on this engine)
+        single = Engine(1, rmax=RMAX)
+        single.run(requests(A, b, x0, 1), poll=args.poll)
+        t1 = time.perf_counter()
+        single.run(requests(A, b, x0, 1), poll=args.poll)
+        torch.cuda.synchronize()
+        single_ms = 1e3 * (time.perf_counter() - t1)
 
     # dominant-kernel roofline: one more batch replayed tick by tick with HIP
     # events around each kernel on the engine's stream; algorithmic flops are
