@@ -49,6 +49,21 @@ def requests(A, b, x0, n):
             for _ in range(n)]
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` from the committed FETCH_SIZE/WRITE_SIZE summary.
+
+    The counters come from separate ``rocprofv3 --pmc`` runs of this benchmark
+    (a counter run cannot time); the summary names the commit it measured.
+    """
+    short = kernel.split("::")[-1]
+    path = os.path.join(ROOT, "profiles", f"{short}_c1_batch256_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    t = json.load(open(path))
+    return t["bytes_per_launch"], {"file": os.path.relpath(path, ROOT), "commit": t.get("commit"),
+                                   "read_bytes": t["read_bytes"], "write_bytes": t["write_bytes"]}
+
+
 def cpu_baseline(budget_s: float = 12.0):
     """The oracle's AMEn (FP64 NumPy/SciPy) on the same C1 inputs, one BLAS thread."""
     from threadpoolctl import threadpool_limits
@@ -141,8 +156,11 @@ def main():
     avg_ms = prof.kernel_ms[j] / launches
     flops_per_launch = prof.kernel_flops[j] / launches
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    traffic, traffic_src = measured_traffic(KERNELS[j])
     roof = {"bound": "mfma", "kernel": KERNELS[j], "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+            "traffic_source": traffic_src,
+            "hbm_gbs_at_traffic": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
             "avg_launch_us": 1e3 * avg_ms, "flops_per_launch": flops_per_launch,
             "launches": launches, "kernel_ms_share": {
                 k.split("::")[1]: round(float(v / prof.kernel_ms.sum()), 4)

@@ -5,6 +5,8 @@
 
 #include <stdint.h>
 
+#include "ttx.h"
+
 #define TTX_MAXD 64
 #define TTX_MAXSWEEPS 64
 
@@ -69,3 +71,42 @@ typedef struct ttx_slot {
   double residuals[TTX_MAXSWEEPS];
   int64_t max_bonds[TTX_MAXSWEEPS];
 } ttx_slot;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * Batched device-resident AMEn (the sweep of ttnumpy/solvers/amen.py:395-725
+ * with the residual check of ttnumpy/solvers/common.py:118-146), one slot per
+ * request.  The reference runs one solve per Python call; these entry points
+ * serve many at once and have no single counterpart there.
+ */
+
+/* sizeof(ttx_slot), for the host mirror's layout check. */
+int64_t ttx_engine_slot_bytes(void);
+/* byte offset of a field: 0 n, 1 residual_tol, 2 cap_core, 3 A, 4 rx, 5 residuals, 6 max_bonds. */
+int64_t ttx_engine_field_offset(int which);
+/*
+ * Fill the layout block of a host-side slot whose problem block is set:
+ * per-core, environment and scratch capacities and offsets (in doubles) of the
+ * slot's arena for rank capacity rmax.  Returns TTX_EINVAL on bad sizes.
+ */
+int ttx_engine_plan(ttx_slot* s);
+/*
+ * Advance every slot of the device array `slots` by `ticks` ticks (four
+ * kernels each: environments, local solve, truncation/enrichment/shift,
+ * residual check).  Slots with status != RUNNING are skipped.  ticks == 0
+ * only prepares the kernels' LDS attributes.
+ */
+int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks);
+/*
+ * ttx_engine_tick with a HIP event pair around each of the four kernels and a
+ * synchronisation per tick; ms_out[0..3] accumulate the kernels' milliseconds.
+ */
+int ttx_engine_profile(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
+                       double* ms_out);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,4 +1,4 @@
-"""The C-ABI boundary: the library loads and exports every symbol ``include/ttx.h`` declares.
+"""The C-ABI boundary: the library loads and exports every symbol ``include/*.h`` declares.
 
 CPU only: nothing here launches a kernel.
 """
@@ -12,7 +12,9 @@ ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
 
 
 def _declared():
-    text = open(os.path.join(ROOT, "include", "ttx.h")).read()
+    inc = os.path.join(ROOT, "include")
+    text = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc))
+                   if f.endswith(".h"))
     return sorted(set(re.findall(r"^\s*(?:int|int64_t)\s+(ttx_\w+)\s*\(", text, flags=re.M)))
 
 

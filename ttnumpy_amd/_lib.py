@@ -1,4 +1,4 @@
-"""ctypes binding of ``libttx.so`` (declared in ``include/ttx.h``).
+"""ctypes binding of ``libttx.so`` (declared in ``include/ttx.h`` and ``include/ttx_engine.h``).
 
 The library is the only compute path: if it is missing, or no HIP device is
 present, every operation raises instead of falling back to the CPU.
@@ -55,6 +55,11 @@ _SIGNATURES = {
                                 _i64]),
     "ttx_scale_cols_f64": (_i32, [_p, _i64, _i64, _p, _i64, _i64, _p]),
     "ttx_rank_by_threshold_f64": (_i32, [_p, _p, _i64, _f64, _i64, _p]),
+    "ttx_engine_slot_bytes": (_i64, []),
+    "ttx_engine_field_offset": (_i64, [_i32]),
+    "ttx_engine_plan": (_i32, [_p]),
+    "ttx_engine_tick": (_i32, [_p, _p, _i64, _i64]),
+    "ttx_engine_profile": (_i32, [_p, _p, _i64, _i64, ctypes.POINTER(_f64)]),
 }
 
 _lib = None

@@ -1,7 +1,7 @@
 // Batched, device-resident AMEn engine.
 //
 // Every request ("slot") owns its trains, environments and scratch in HBM and
-// a small descriptor (engine.h).  A tick launches four kernels over all slots,
+// a small descriptor (include/ttx_engine.h).  A tick launches four kernels over all slots,
 // one workgroup per slot:
 //
 //   env   : the environment updates of the slot's current site (grid slots x 5,
@@ -22,7 +22,7 @@
 // reference lines it follows.
 
 #include "block_ops.h"
-#include "engine.h"
+#include "ttx_engine.h"
 
 namespace ttx {
 

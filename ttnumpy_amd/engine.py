@@ -1,6 +1,6 @@
 """Batched, device-resident AMEn with continuous batching.
 
-Each request occupies a *slot*: a descriptor (``csrc/engine.h``) plus one
+Each request occupies a *slot*: a descriptor (``include/ttx_engine.h``) plus one
 arena in HBM holding its solution and residual trains at a fixed rank
 capacity, its environments and its scratch.  ``ttx_engine_tick`` advances
 every active slot by one site step (environments, local solve, truncation and
