@@ -186,6 +186,9 @@ def main():
         "minres_iters_per_solve": prof.minres_iters / max(args.batch, 1),
         "lu_solves_per_solve": prof.lu_solves / max(args.batch, 1),
         "gflop_per_solve": float(prof.kernel_flops.sum()) / max(args.batch, 1) / 1e9,
+        "stage_ms_per_solve": {nm: round(float(v) / max(args.batch, 1), 3) for nm, v in zip(
+            ("rhs", "matrix", "lu", "minres", "exact_res", "post", "check", "env", "post.residual_core",
+             "post.svd", "post.rank", "post.enrich_qr", "post.zshift"), prof.stage_ms)},
         "roofline": roof,
     }
     if not args.no_cpu:

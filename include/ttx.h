@@ -108,7 +108,9 @@ int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n,
  * `Mb` holds [M | b] row-major (n rows, n+1 columns, row stride ld >= n+1)
  * and is destroyed; `u` receives the solution.  scipy.linalg.solve at
  * ttnumpy/solvers/common.py:767,802 (getrf/getrs).  info[0]: TTX_OK or
- * TTX_ESINGULAR.  Work: ttx_lu_work_size(n) doubles.
+ * TTX_ESINGULAR.  One workgroup with the row permutation and the panels in
+ * its 150 KiB of LDS: n <= 12800 (TTX_EINVAL above).  Work:
+ * ttx_lu_work_size(n) doubles (reserved, may be NULL).
  */
 int64_t ttx_lu_work_size(int64_t n);
 int ttx_lu_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, int64_t ld, double* u,

@@ -70,6 +70,11 @@ typedef struct ttx_slot {
   double flops[4];  // algorithmic flops banked per kernel: env, solve, post, check
   double residuals[TTX_MAXSWEEPS];
   int64_t max_bonds[TTX_MAXSWEEPS];
+  // wall-clock ticks (100 MHz) spent per stage, thread 0's view:
+  // 0 local rhs, 1 explicit matrix, 2 LU, 3 MINRES, 4 exact residual, 5 post, 6 check, 7 env,
+  // inside post (sweep steps only): 8 residual core, 9 SVD, 10 rank choice, 11 enrichment
+  // and QR, 12 z shift; 13-15 spare
+  int64_t clk[16];
 } ttx_slot;
 
 #ifdef __cplusplus

@@ -153,6 +153,47 @@ def test_lu_solve(n):
     np.testing.assert_allclose(_h(u), ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("n,kind", [(129, "gauss"), (300, "gauss"), (1000, "gauss"),
+                                    (257, "reversed"), (700, "reversed")])
+def test_lu_solve_pivots_across_blocks(n, kind):
+    """Pivots that come from far below the current block (the 128-column outer
+    panels and 16-column sub-panels must see the same row order)."""
+    print(f"lu pivoting n={n} {kind}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(1000 + n)
+    if kind == "gauss":
+        M = rng.standard_normal((n, n))  # partial pivoting swaps most rows
+    else:
+        # diagonally dominant rows stored bottom-up: every pivot lies in the last rows
+        M = (rng.standard_normal((n, n)) + 4.0 * n ** 0.5 * np.eye(n))[::-1].copy()
+    b = rng.standard_normal(n)
+    ld = n + 3  # a padded row stride
+    Mb = np.zeros((n, ld))
+    Mb[:, :n] = M
+    Mb[:, n] = b
+    u = D.empty(n)
+    info = torch.zeros(1, dtype=torch.int32, device=D.device())
+    D.lu_solve(_t(Mb.reshape(-1)), n, u, info, ld=ld)
+    assert int(info.item()) == 0
+    x = _h(u)
+    ref = np.linalg.solve(M, b)
+    assert np.linalg.norm(M @ x - b) <= 1e-12 * np.linalg.norm(M, 1) * np.linalg.norm(x)
+    np.testing.assert_allclose(x, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+
+
+def test_lu_singular_reports():
+    D = _dev()
+    n = 200
+    rng = np.random.default_rng(7)
+    M = rng.standard_normal((n, n))
+    M[:, 150] = 0.0  # an exactly zero column: the pivot of column 150 is 0
+    Mb = np.concatenate([M, np.ones((n, 1))], axis=1).reshape(-1)
+    u = D.empty(n)
+    info = torch.zeros(1, dtype=torch.int32, device=D.device())
+    D.lu_solve(_t(Mb), n, u, info)
+    assert int(info.item()) == 5  # TTX_ESINGULAR
+
+
 def test_lu_needs_pivoting():
     D = _dev()
     M = np.array([[0.0, 1.0, 2.0], [1.0, 0.0, 3.0], [4.0, 5.0, 0.0]])

@@ -361,123 +361,269 @@ __device__ __forceinline__ void bgemv(int64_t rows, int64_t cols, double alpha, 
 }
 
 // ---------------------------------------------------------------------------
-// LU with partial pivoting and the solve of [M | b] (row-major, row stride
-// ld >= n + 1) by one block.  `lds` holds the panel (nb x n doubles) and is
-// reused for the 64 x 64 diagonal blocks of the back substitution; `u`
-// receives the solution.  Returns TTX_ESINGULAR through *info on a zero
-// pivot (the factorization continues, as getrf does).
+// Blocked LU with partial pivoting and the solve of [M | b] by one 512-thread
+// block (ttnumpy/solvers/common.py:767-773 solves the same system with
+// LAPACK's getrf/getrs).
+//
+// W is row-major with row stride ld >= n + 1; column n holds b.  Row
+// interchanges are logical: perm[i] is the storage row of logical row i, so a
+// pivot swaps two ints instead of two rows, and every access goes through
+// perm.  Pivots are chosen as getrf does (largest |entry|, first on ties, in
+// the current logical order).  Because b rides along as column n, the
+// updates leave y = L^{-1} P b there and L is never needed again, so the
+// columns left of the current block are not permuted.
+//
+// Blocking: outer panels of kLuOuter = 8 * nb columns; inside, nb-column
+// panels (nb <= 16, staged in LDS column-major) are factored right-looking
+// with their updates confined to the outer panel and to its block row; the
+// rest of the matrix takes one rank-kLuOuter update per outer panel through
+// the LDS-staged MFMA tile routine, which reads and writes the trailing matrix
+// once per 128 columns instead of once per 16.
+//
+// lds: lds_bytes of shared memory (the panel, the GEMM staging, the back
+// substitution's diagonal blocks) with perm in its last 4 n bytes; needs
+// lds_bytes >= 4 n + 8 max(n, 4224).  *info = TTX_ESINGULAR on a zero pivot
+// (the factorization continues, as getrf does).  Ends synced.
 // ---------------------------------------------------------------------------
+constexpr int kLuTile = 128;  // trailing-update output tile (rows and columns)
+constexpr int kLuK = 16;      // K slab staged per step
+
+// For logical rows i < Mr and columns j < Nc:
+//   W[perm[r0+i]][c0+j] -= sum_{t<K} W[perm[r0+i]][a0+t] * W[perm[b0+t]][c0+j]
+// 128 x 128 output tiles; each of the 8 waves owns 32 x 64 (2 x 4 MFMA 16x16
+// blocks) accumulated in registers; A and B are staged kLuK deep in LDS,
+// k-major, with the next slab's global loads in flight during the MFMAs.
+// Uses 2 * kLuK * kLuTile doubles of lds.  Ends synced.
+__device__ __forceinline__ void lu_tile_update(int64_t Mr, int64_t Nc, int64_t K, double* W,
+                                               int64_t ld, const int* perm, int64_t r0,
+                                               int64_t a0, int64_t b0, int64_t c0, double* lds) {
+  if (Mr <= 0 || Nc <= 0 || K <= 0) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+  double* As = lds;                    // [kLuK][kLuTile]
+  double* Bs = lds + kLuK * kLuTile;   // [kLuK][kLuTile]
+  const int ar = tid >> 2, ak = (tid & 3) * 4;   // A loader: row ar, k ak..ak+3
+  const int bk = tid >> 5, bc = (tid & 31) * 4;  // B loader: k row bk, columns bc..bc+3
+  const int64_t tm = (Mr + kLuTile - 1) / kLuTile, tn = (Nc + kLuTile - 1) / kLuTile;
+  for (int64_t t = 0; t < tm * tn; ++t) {
+    const int64_t m0 = (t / tn) * kLuTile, n0 = (t % tn) * kLuTile;
+    // the 8 output rows of this lane: row m0 + wm + 16 mt + 4 e + lq
+    double* crow[2][4];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t row = m0 + wm + 16 * mt + 4 * e + lq;
+        crow[mt][e] = row < Mr ? W + (int64_t)perm[r0 + row] * ld + c0 : nullptr;
+      }
+    double4_t acc[2][4];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const int64_t col = n0 + wn + 16 * nt + l16;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          acc[mt][nt][e] = (crow[mt][e] && col < Nc) ? crow[mt][e][col] : 0.0;
+      }
+    const int64_t arow = m0 + ar;
+    const double* ap = arow < Mr ? W + (int64_t)perm[r0 + arow] * ld + a0 : nullptr;
+    double ra[4], rb[4];
+    auto gload = [&](int64_t k0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t kk = k0 + ak + i;
+        ra[i] = (ap && kk < K) ? ap[kk] : 0.0;
+      }
+      const int64_t kb = k0 + bk;
+      const double* bp = kb < K ? W + (int64_t)perm[b0 + kb] * ld + c0 + n0 : nullptr;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t cc = bc + i;
+        rb[i] = (bp && n0 + cc < Nc) ? bp[cc] : 0.0;
+      }
+    };
+    gload(0);
+    for (int64_t k0 = 0; k0 < K; k0 += kLuK) {
+      __syncthreads();  // the previous slab's MFMAs are done with As/Bs
+#pragma unroll
+      for (int i = 0; i < 4; ++i) As[(ak + i) * kLuTile + ar] = ra[i];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Bs[bk * kLuTile + bc + i] = rb[i];
+      __syncthreads();
+      if (k0 + kLuK < K) gload(k0 + kLuK);
+#pragma unroll
+      for (int ks = 0; ks < kLuK; ks += 4) {
+        const int kr = ks + lq;
+        double a[2], b[4];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) a[mt] = -As[kr * kLuTile + wm + 16 * mt + l16];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) b[nt] = Bs[kr * kLuTile + wn + 16 * nt + l16];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt)
+            acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const int64_t col = n0 + wn + 16 * nt + l16;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (crow[mt][e] && col < Nc) crow[mt][e][col] = acc[mt][nt][e];
+      }
+  }
+  __syncthreads();
+}
+
+// Rows k..k+nb (logical) of columns [cb, ce): x <- L_kk^{-1} x with the unit
+// lower L_kk given as L[l * ls + i] = L_kk[i][l] (i > l).  A thread per column,
+// so the row reads coalesce.  nb <= 16.  No barrier inside.
+__device__ __forceinline__ void lu_trsm_rows(int64_t k, int64_t nb, int64_t cb, int64_t ce, double* W,
+                                             int64_t ld, const int* perm, const double* L,
+                                             int64_t ls) {
+  for (int64_t c = cb + threadIdx.x; c < ce; c += blockDim.x) {
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = i < nb ? W[(int64_t)perm[k + i] * ld + c] : 0.0;
+#pragma unroll
+    for (int i = 1; i < 16; ++i) {
+      if (i < nb) {
+        double s = x[i];
+#pragma unroll
+        for (int l = 0; l < i; ++l) s -= L[l * ls + i] * x[l];
+        x[i] = s;
+      }
+    }
+#pragma unroll
+    for (int i = 1; i < 16; ++i)
+      if (i < nb) W[(int64_t)perm[k + i] * ld + c] = x[i];
+  }
+}
+
 __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W, double* u,
-                                               double* lds, int64_t nb0, double* red,
-                                               int32_t* info) {
+                                               double* lds, int64_t lds_bytes, int32_t* info) {
   __shared__ double rv[16];
-  __shared__ int64_t ri[16];
-  __shared__ int64_t s_piv[16];
+  __shared__ int ri[16];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int nwave = blockDim.x >> 6;
-  const int64_t ncols = n + 1;
+  int* perm = reinterpret_cast<int*>(reinterpret_cast<char*>(lds) + lds_bytes) - n;
+  int64_t nb0 = (lds_bytes - 4 * n) / (8 * n);
+  if (nb0 > 16) nb0 = 16;
+  const int64_t nbo = 8 * nb0;
   {
     const double dn = (double)n;
     add_flops(2.0 * dn * dn * dn / 3.0 + 2.0 * dn * dn);
   }
-  for (int64_t k = 0; k < n; k += nb0) {
-    const int64_t nb = (n - k) < nb0 ? (n - k) : nb0;
-    const int64_t mp = n - k;
-    double* P = lds;
-    for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
-      const int64_t c = e % nb, r = e / nb;
-      P[c * mp + r] = W[(k + r) * ld + (k + c)];
-    }
-    __syncthreads();
-    for (int64_t j = 0; j < nb; ++j) {
-      double* pj = P + j * mp;
-      double best = -1.0;
-      int64_t bi = j;
-      for (int64_t r = j + tid; r < mp; r += blockDim.x) {
-        const double v = fabs(pj[r]);
-        if (v > best) { best = v; bi = r; }
+  for (int64_t i = tid; i < n; i += blockDim.x) perm[i] = (int)i;
+  __syncthreads();
+  for (int64_t K0 = 0; K0 < n; K0 += nbo) {
+    const int64_t ko = (n - K0) < nbo ? (n - K0) : nbo;
+    const int64_t K1 = K0 + ko;
+    for (int64_t k = K0; k < K1; k += nb0) {
+      const int64_t nb = (K1 - k) < nb0 ? (K1 - k) : nb0;
+      const int64_t mp = n - k;
+      double* P = lds;
+      for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
+        const int64_t c = e % nb, r = e / nb;
+        P[c * mp + r] = W[(int64_t)perm[k + r] * ld + (k + c)];
       }
+      __syncthreads();
+      for (int64_t j = 0; j < nb; ++j) {
+        double* pj = P + j * mp;
+        double best = -1.0;
+        int bi = (int)j;
+        for (int64_t r = j + tid; r < mp; r += blockDim.x) {
+          const double v = fabs(pj[r]);
+          if (v > best) { best = v; bi = (int)r; }
+        }
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(best, off, 64);
-        const int64_t oi = __shfl_xor(bi, off, 64);
-        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-      }
-      if (lane == 0) { rv[wave] = best; ri[wave] = bi; }
-      __syncthreads();
-      if (tid == 0) {
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ov = __shfl_xor(best, off, 64);
+          const int oi = __shfl_xor(bi, off, 64);
+          if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        }
+        if (lane == 0) { rv[wave] = best; ri[wave] = bi; }
+        __syncthreads();
         double b0 = rv[0];
-        int64_t i0 = ri[0];
+        int pr = ri[0];
         for (int w = 1; w < nwave; ++w)
-          if (rv[w] > b0 || (rv[w] == b0 && ri[w] < i0)) { b0 = rv[w]; i0 = ri[w]; }
-        s_piv[j] = i0;
-        if (b0 == 0.0) info[0] = TTX_ESINGULAR;
-      }
-      __syncthreads();
-      const int64_t pr = s_piv[j];
-      if (pr != j) {
-        for (int64_t c = tid; c < nb; c += blockDim.x) {
-          double* pc = P + c * mp;
+          if (rv[w] > b0 || (rv[w] == b0 && ri[w] < pr)) { b0 = rv[w]; pr = ri[w]; }
+        if (tid == 0) {
+          if (b0 == 0.0) info[0] = TTX_ESINGULAR;
+          if (pr != j) {
+            const int t = perm[k + j];
+            perm[k + j] = perm[k + pr];
+            perm[k + pr] = t;
+          }
+        }
+        if (pr != j && tid < nb) {
+          double* pc = P + tid * mp;
           const double t = pc[j];
           pc[j] = pc[pr];
           pc[pr] = t;
         }
-      }
-      __syncthreads();
-      const double dj = pj[j];
-      if (dj != 0.0) {
-        for (int64_t r = j + 1 + tid; r < mp; r += blockDim.x) pj[r] /= dj;
-      }
-      __syncthreads();
-      for (int64_t c = j + 1; c < nb; ++c) {
-        double* pc = P + c * mp;
-        const double uu = pc[j];
-        for (int64_t r = j + 1 + tid; r < mp; r += blockDim.x) pc[r] -= pj[r] * uu;
-      }
-      __syncthreads();
-    }
-    for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
-      const int64_t c = e % nb, r = e / nb;
-      W[(k + r) * ld + (k + c)] = P[c * mp + r];
-    }
-    for (int64_t c = tid; c < ncols; c += blockDim.x) {
-      if (c >= k && c < k + nb) continue;
-      for (int64_t j = 0; j < nb; ++j) {
-        const int64_t pr = k + s_piv[j];
-        if (pr != k + j) {
-          const double t = W[(k + j) * ld + c];
-          W[(k + j) * ld + c] = W[pr * ld + c];
-          W[pr * ld + c] = t;
+        __syncthreads();
+        const double dj = pj[j];
+        for (int64_t r = j + 1 + tid; r < mp; r += blockDim.x) {
+          double l = pj[r];
+          if (dj != 0.0) {
+            l /= dj;
+            pj[r] = l;
+          }
+          for (int64_t c = j + 1; c < nb; ++c) P[c * mp + r] -= l * P[c * mp + j];
         }
+        __syncthreads();
       }
-      if (c >= k + nb) {
-        for (int64_t i = 1; i < nb; ++i) {
-          double s = W[(k + i) * ld + c];
-          for (int64_t l = 0; l < i; ++l) s -= P[l * mp + i] * W[(k + l) * ld + c];
-          W[(k + i) * ld + c] = s;
-        }
+      for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
+        const int64_t c = e % nb, r = e / nb;
+        W[(int64_t)perm[k + r] * ld + (k + c)] = P[c * mp + r];
       }
+      // U rows k..k+nb of the panel columns right of this sub-panel, then the
+      // rank-nb update of the rest of the outer panel (all rows below)
+      lu_trsm_rows(k, nb, k + nb, K1, W, ld, perm, P, mp);
+      __syncthreads();
+      lu_tile_update(n - (k + nb), K1 - (k + nb), nb, W, ld, perm, k + nb, k, k, k + nb, lds);
     }
-    __syncthreads();
-    const int64_t rows = n - k - nb, cols = ncols - k - nb;
-    if (rows > 0 && cols > 0)
-      bgemm(1, rows, cols, nb, -1.0, W + (k + nb) * ld + k, ld, 1, 0, W + k * ld + (k + nb), ld, 1, 0,
-            1.0, W + (k + nb) * ld + (k + nb), ld, 1, 0);
-    __syncthreads();
+    // the outer panel is final (its pivots are all known), so its block row
+    // U12 = L11^{-1} A12 can be formed: per sub-panel a triangular solve with
+    // L_kk (staged in LDS) and a rank-nb update of the block rows below it
+    for (int64_t k = K0; k < K1; k += nb0) {
+      const int64_t nb = (K1 - k) < nb0 ? (K1 - k) : nb0;
+      double* L = lds;  // nb x nb, L[l * 16 + i] = L_kk[i][l]
+      for (int64_t e = tid; e < nb * nb; e += blockDim.x) {
+        const int64_t i = e / nb, l = e % nb;
+        L[l * 16 + i] = W[(int64_t)perm[k + i] * ld + k + l];
+      }
+      __syncthreads();
+      lu_trsm_rows(k, nb, K1, n + 1, W, ld, perm, L, 16);
+      __syncthreads();
+      lu_tile_update(K1 - (k + nb), n + 1 - K1, nb, W, ld, perm, k + nb, k, k, K1, lds);
+    }
+    // the rest of the matrix: one rank-ko update
+    lu_tile_update(n - K1, n + 1 - K1, ko, W, ld, perm, K1, K0, K0, K1, lds);
   }
-  // back substitution U u = y, y = column n
-  double* T = lds;          // 64 x 65
+  // back substitution U u = y in 64-column blocks from the bottom; the
+  // diagonal block is solved by one wave, the rows above are updated one
+  // wave per row so each row's reads coalesce
+  double* T = lds;  // 64 x 65
   double* xb = lds + 64 * 65;
-  for (int64_t i = tid; i < n; i += blockDim.x) u[i] = W[i * ld + n];
+  for (int64_t i = tid; i < n; i += blockDim.x) u[i] = W[(int64_t)perm[i] * ld + n];
   __syncthreads();
   for (int64_t j1 = n; j1 > 0; j1 -= 64) {
     const int64_t j0 = j1 - 64 > 0 ? j1 - 64 : 0;
     const int64_t nbk = j1 - j0;
     for (int64_t e = tid; e < nbk * nbk; e += blockDim.x) {
       const int64_t r = e / nbk, c = e % nbk;
-      T[r * 65 + c] = W[(j0 + r) * ld + j0 + c];
+      T[r * 65 + c] = W[(int64_t)perm[j0 + r] * ld + j0 + c];
     }
     __syncthreads();
     if (wave == 0) {
@@ -492,15 +638,14 @@ __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W,
       if (row < j1) { u[row] = yv; xb[lane] = yv; }
     }
     __syncthreads();
-    for (int64_t r = tid; r < j0; r += blockDim.x) {
-      double s = u[r];
-      const double* wr = W + r * ld + j0;
-      for (int64_t c = 0; c < nbk; ++c) s -= wr[c] * xb[c];
-      u[r] = s;
+    for (int64_t r = wave; r < j0; r += nwave) {
+      const double* wr = W + (int64_t)perm[r] * ld + j0;
+      double s = lane < nbk ? wr[lane] * xb[lane] : 0.0;
+      s = wave_sum(s);
+      if (lane == 0) u[r] -= s;
     }
     __syncthreads();
   }
-  (void)red;
 }
 
 }  // namespace ttx

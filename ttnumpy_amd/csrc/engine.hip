@@ -97,11 +97,9 @@ __device__ void local_rhs_blk(const double* psi, int64_t rt, int64_t fl, const d
   bgemm(1, fl * n, fr, rt2, 1.0, s1, rt2, 1, 0, phi, fr, 1, 0, 0.0, out, fr, 1, 0);
 }
 
-// one-site operator Psi . A_k . Phi, explicit (row-major N x N) or matrix-free
-// (one_site.py:78-115); rows (fl, i, fr), columns (rl, j, rr)
+// one-site operator Psi . A_k . Phi, matrix-free (one_site.py:78-115);
+// rows (fl, i, fr), columns (rl, j, rr)
 struct LocOp {
-  int explicit_;
-  const double* M;
   const double* psi;
   const double* A;
   const double* phi;
@@ -112,15 +110,6 @@ struct LocOp {
   __device__ int64_t cols() const { return rl * nc * rr; }
   // out = alpha * op(y) + beta * out for B columns (column-major); ends synced
   __device__ void apply(const double* y, int64_t B, double* out, double alpha, double beta) const {
-    if (explicit_) {
-      const int64_t N = rows();
-      if (B == 1)
-        bgemv(N, N, alpha, M, N, y, beta, out);
-      else
-        bgemm(1, N, B, N, alpha, M, N, 1, 0, y, 1, N, 0, beta, out, 1, N, 0);
-      __syncthreads();
-      return;
-    }
     bgemm(1, B * rl * nc, Rb * fr, rr, 1.0, y, rr, 1, 0, phi, Rb * fr, 1, 0, 0.0, s1, Rb * fr, 1, 0);
     __syncthreads();
     bgemm(B * rl, Ra * nr, fr, nc * Rb, 1.0, A, nc * Rb, 1, 0, s1, fr, 1, nc * Rb * fr, 0.0, s2, fr,
@@ -132,24 +121,38 @@ struct LocOp {
   }
 };
 
-__device__ void local_matrix_blk(const LocOp& op, double* out) {
-  const int64_t rows = op.rows(), cols = op.cols();
+// explicit matrix into out (row-major, row stride ld).  Each thread owns a
+// column and a slice of the rows, so the stores of a row are coalesced; psi
+// and phi are staged in LDS when they fit in lds_cap doubles.  Ends synced.
+__device__ void local_matrix_blk(const LocOp& op, double* out, int64_t ld, double* lds,
+                                 int64_t lds_cap) {
+  const int rows = (int)op.rows(), cols = (int)op.cols();
   add_flops(2.0 * (double)rows * (double)cols * (double)op.Ra * (double)op.Rb);
-  for (int64_t e = threadIdx.x; e < rows * cols; e += blockDim.x) {
-    const int64_t col = e % cols, row = e / cols;
-    const int64_t f_r = row % op.fr, i = (row / op.fr) % op.nr, f_l = row / (op.fr * op.nr);
-    const int64_t t_r = col % op.rr, j = (col / op.rr) % op.nc, t_l = col / (op.rr * op.nc);
-    double sum = 0.0;
-    for (int64_t a = 0; a < op.Ra; ++a) {
-      const double ps = op.psi[(t_l * op.Ra + a) * op.fl + f_l];
-      if (ps == 0.0) continue;
-      const double* arow = op.A + ((a * op.nr + i) * op.nc + j) * op.Rb;
-      double t = 0.0;
-      for (int64_t b = 0; b < op.Rb; ++b) t += arow[b] * op.phi[(t_r * op.Rb + b) * op.fr + f_r];
-      sum += ps * t;
-    }
-    out[e] = sum;
+  const double* psi = op.psi;
+  const double* phi = op.phi;
+  const int npsi = (int)(op.rl * op.Ra * op.fl), nphi = (int)(op.rr * op.Rb * op.fr);
+  if ((int64_t)(npsi + nphi) <= lds_cap) {
+    for (int e = threadIdx.x; e < npsi; e += blockDim.x) lds[e] = psi[e];
+    for (int e = threadIdx.x; e < nphi; e += blockDim.x) lds[npsi + e] = phi[e];
+    __syncthreads();
+    psi = lds;
+    phi = lds + npsi;
   }
+  const int fr = (int)op.fr, nr = (int)op.nr, rr = (int)op.rr, nc = (int)op.nc;
+  const int S = cols >= (int)blockDim.x ? 1 : (int)blockDim.x / cols;
+  const int64_t a_stride = op.nr * op.nc * op.Rb;
+  for (int it = threadIdx.x; it < cols * S; it += blockDim.x) {
+    const int c = it % cols, sl = it / cols;
+    const int t_r = c % rr, j = (c / rr) % nc, t_l = c / (rr * nc);
+    const double* ps = psi + (int64_t)t_l * op.Ra * op.fl;
+    const double* ph = phi + (int64_t)t_r * op.Rb * op.fr;
+    for (int row = sl; row < rows; row += S) {
+      const int f_r = row % fr, i = (row / fr) % nr, f_l = row / (fr * nr);
+      out[(int64_t)row * ld + c] = local_matrix_entry(ps, op.fl, f_l, op.A + (i * op.nc + j) * op.Rb,
+                                                      a_stride, op.Ra, op.Rb, ph, op.fr, f_r);
+    }
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------- QR helpers
@@ -344,8 +347,6 @@ struct Env {
 __device__ LocOp make_mfree(const ttx_slot* s, int64_t k, const double* psi, int64_t rl, int64_t Ra,
                             int64_t fl, const double* phi, int64_t rr, int64_t Rb, int64_t fr) {
   LocOp op;
-  op.explicit_ = 0;
-  op.M = nullptr;
   op.psi = psi;
   op.A = s->A[k];
   op.phi = phi;
@@ -362,15 +363,12 @@ __device__ LocOp make_mfree(const ttx_slot* s, int64_t k, const double* psi, int
   return op;
 }
 
-// the x/x/A local operator at k, explicit when the solve built the matrix
-__device__ LocOp system_op(const ttx_slot* s, int64_t k, bool explicit_) {
+// the x/x/A local operator at k, matrix-free (the direct solve's residual and
+// rank tests apply it too: the same operator as the explicit matrix, at the
+// cost of three small GEMMs instead of a pass over N^2 entries)
+__device__ LocOp system_op(const ttx_slot* s, int64_t k) {
   const int64_t rl = s->rx[k], rr = s->rx[k + 1];
-  LocOp op = make_mfree(s, k, envp(s, 0, 0, k), rl, s->ra[k], rl, envp(s, 0, 1, k), rr, s->ra[k + 1], rr);
-  if (explicit_) {
-    op.explicit_ = 1;
-    op.M = scr(s, TTX_SCR_MAT);
-  }
-  return op;
+  return make_mfree(s, k, envp(s, 0, 0, k), rl, s->ra[k], rl, envp(s, 0, 1, k), rr, s->ra[k + 1], rr);
 }
 
 // z_k = b_part - Psi_zx . A_k . Phi_zx x_core (amen.py:109-159), into ZNEW
@@ -439,6 +437,13 @@ __device__ int64_t choose_rank_blk(ttx_slot* s, const LocOp& op, const double* b
   return limit;
 }
 
+// charge the ticks since *t0 to stage i and restart the stopwatch
+__device__ __forceinline__ void stamp(ttx_slot* s, int i, int64_t& t0) {
+  const int64_t t = wall_clock64();
+  if (threadIdx.x == 0) s->clk[i] += t - t0;
+  t0 = t;
+}
+
 __device__ void fail(ttx_slot* s, int64_t code) {
   if (threadIdx.x == 0) s->status = code;
 }
@@ -496,8 +501,11 @@ __global__ __launch_bounds__(kEngThreads) void eng_env_kernel(ttx_slot* slots) {
   ttx_slot* s = slots + blockIdx.x;
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
+  const int64_t t0 = wall_clock64();
   env_body(s, blockIdx.y);
   __syncthreads();
+  if (threadIdx.x == 0 && s->status == TTX_SLOT_RUNNING) atomicAdd((unsigned long long*)&s->clk[7],
+                                                                   (unsigned long long)(wall_clock64() - t0));
   if (threadIdx.x == 0 && g_flops != 0.0) atomicAdd(&s->flops[0], g_flops);
 }
 
@@ -510,6 +518,7 @@ __device__ void solve_body(ttx_slot* s, double* lds) {
   const int64_t k = s->k;
   const int64_t rl = s->rx[k], n = s->n[k], rr = s->rx[k + 1];
   const int64_t N = rl * n * rr;
+  int64_t t0 = wall_clock64();
   double* u = scr(s, TTX_SCR_U);
   double* bloc = scr(s, TTX_SCR_BLOC);
   double* tmp = scr(s, TTX_SCR_TMP);
@@ -517,9 +526,10 @@ __device__ void solve_body(ttx_slot* s, double* lds) {
   local_rhs_blk(envp(s, 1, 0, k), s->rb[k], rl, s->b[k], n, s->rb[k + 1], envp(s, 1, 1, k), rr, bloc,
                 scr(s, TTX_SCR_S1));
   __syncthreads();
+  stamp(s, 0, t0);
   const bool iter = s->iterative && N > s->direct_size;
   if (iter) {
-    LocOp op = system_op(s, k, false);
+    LocOp op = system_op(s, k);
     const double target = s->local_rtol * bnorm2(bloc, nullptr, N, red);
     double rt = s->local_rtol;
     double* V = scr(s, TTX_SCR_MR);
@@ -534,6 +544,7 @@ __device__ void solve_body(ttx_slot* s, double* lds) {
       rt = fmax(rt * 1e-2, 1e-15);
     }
     bcopy(u, V, N);
+    stamp(s, 3, t0);
     if (threadIdx.x == 0) {
       s->local_solves += 1;
       if (!ok) s->local_inexact += 1;
@@ -541,22 +552,17 @@ __device__ void solve_body(ttx_slot* s, double* lds) {
     }
   } else {
     if (N > s->direct_cap) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
-    LocOp op = system_op(s, k, false);
-    double* M = scr(s, TTX_SCR_MAT);
+    LocOp op = system_op(s, k);
     double* W = scr(s, TTX_SCR_LU);
-    local_matrix_blk(op, M);
-    __syncthreads();
-    for (int64_t e = threadIdx.x; e < N * (N + 1); e += blockDim.x) {
-      const int64_t r = e / (N + 1), c = e % (N + 1);
-      W[e] = c < N ? M[r * N + c] : bloc[r];
-    }
+    local_matrix_blk(op, W, N + 1, lds, kEngLds / 8);
+    for (int64_t r = threadIdx.x; r < N; r += blockDim.x) W[r * (N + 1) + N] = bloc[r];
     if (threadIdx.x == 0) s_info = TTX_OK;
     __syncthreads();
-    int64_t nb0 = (int64_t)(kEngLds / (8 * N));
-    if (nb0 > 16) nb0 = 16;
-    if (nb0 < 1) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
-    block_lu_solve(N, N + 1, W, u, lds, nb0, red, &s_info);
+    stamp(s, 1, t0);
+    if ((int64_t)kEngLds < 4 * N + 8 * (N > 4224 ? N : 4224)) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
+    block_lu_solve(N, N + 1, W, u, lds, kEngLds, &s_info);
     __syncthreads();
+    stamp(s, 2, t0);
     if (s_info != TTX_OK) { fail(s, TTX_SLOT_ERR_SINGULAR); return; }
     if (threadIdx.x == 0) {
       scal[1] = 1.0;
@@ -565,10 +571,11 @@ __device__ void solve_body(ttx_slot* s, double* lds) {
   }
   __syncthreads();
   if (s->residual_mode) {
-    LocOp op = system_op(s, k, !iter);
+    LocOp op = system_op(s, k);
     op.apply(u, 1, tmp, 1.0, 0.0);
     const double ex = bnorm2(tmp, bloc, N, red);
     if (threadIdx.x == 0) scal[0] = ex;
+    stamp(s, 4, t0);
   } else if (threadIdx.x == 0) {
     scal[0] = 0.0;
   }
@@ -645,11 +652,12 @@ __device__ void post_body(ttx_slot* s) {
   const double* bloc = scr(s, TTX_SCR_BLOC);
   const double* scal = scr(s, TTX_SCR_SCAL);
   const double exact = scal[0];
-  const bool expl = scal[1] != 0.0;
+  int64_t t0 = wall_clock64();
   residual_core_blk(s, k, u, red);  // z from the untruncated solution (amen.py:601-613)
   double* zn = scr(s, TTX_SCR_ZNEW);
   const int64_t zl = s->rz[k], zr = s->rz[k + 1];
   __syncthreads();
+  stamp(s, 8, t0);
 
   if (ph == TTX_PH_BWD && k == 0) {
     // last core of the backward sweep is stored as solved (amen.py:654-677)
@@ -657,7 +665,7 @@ __device__ void post_body(ttx_slot* s) {
     bcopy(zcore(s, 0), zn, zl * n * zr);
     return;
   }
-  const LocOp op = system_op(s, k, expl);
+  const LocOp op = system_op(s, k);
   const bool fwd = ph == TTX_PH_FWD;
   // SVD of (rl n x rr) forward or (rl x n rr) backward (amen.py:254-260,301-307)
   const int64_t p = fwd ? rl * n : rl;
@@ -672,11 +680,13 @@ __device__ void post_body(ttx_slot* s) {
   SvdArgs sa{p, q, u, q, 1, U, K, 1, S, Vt, q, 1, scr(s, TTX_SCR_SVDW), &s_svd};
   block_svd(sa, scr(s, TTX_SCR_SVDW), red);
   __syncthreads();
+  stamp(s, 9, t0);
   if (s_svd == TTX_ENONFINITE) { fail(s, TTX_SLOT_ERR_NONFINITE); return; }
   if (s_svd == TTX_ENOCONV && threadIdx.x == 0) s->svd_noconv += 1;
   int capped = 0;
   const int64_t rank = choose_rank_blk(s, op, bloc, U, S, Vt, p, q, K, exact, &capped, red);
   __syncthreads();
+  stamp(s, 10, t0);
   const int64_t mr = s->max_rank;
   if (fwd) {
     // core = U_r, carry = S V_r, block = Psi_xz^T z (amen.py:614-627), enrich (amen.py:325-360)
@@ -714,10 +724,12 @@ __device__ void post_body(ttx_slot* s) {
       s->rank_decisions += 2;
       s->rank_capped += capped + (kc < kk ? 1 : 0);
     }
+    stamp(s, 11, t0);
     // z centre moves right with the fresh core (amen.py:628)
     const int64_t kz = shift_right_blk(zcore(s, k), zcore(s, k + 1), zn, zl, n, zr,
                                        s->n[k + 1] * s->rz[k + 2], qb, rbuf, tmp, qrw, red);
     if (threadIdx.x == 0) s->rz[k + 1] = kz;
+    stamp(s, 12, t0);
     return;
   }
   // backward: core = V_r, carry = U_r S, block = z Phi_xz (amen.py:679-703), enrich (amen.py:363-392)
@@ -747,17 +759,21 @@ __device__ void post_body(ttx_slot* s) {
     s->rank_decisions += 2;
     s->rank_capped += capped + (kc < kk ? 1 : 0);
   }
+  stamp(s, 11, t0);
   const int64_t kz = shift_left_blk(zcore(s, k), zcore(s, k - 1), zn, zl, n, zr,
                                     s->rz[k - 1] * s->n[k - 1], qb, rbuf, tmp, qrw, red);
   if (threadIdx.x == 0) s->rz[k] = kz;
+  stamp(s, 12, t0);
 }
 
 __global__ __launch_bounds__(kEngThreads) void eng_post_kernel(ttx_slot* slots) {
   ttx_slot* s = slots + blockIdx.x;
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
+  int64_t t0 = wall_clock64();
   post_body(s);
   __syncthreads();
+  if (s->status == TTX_SLOT_RUNNING || s->status == TTX_SLOT_DONE) stamp(s, 5, t0);
   if (threadIdx.x == 0 && g_flops != 0.0) s->flops[2] += g_flops;
 }
 
@@ -889,8 +905,10 @@ __global__ __launch_bounds__(kEngThreads) void eng_check_kernel(ttx_slot* slots)
   ttx_slot* s = slots + blockIdx.x;
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
+  int64_t t0 = wall_clock64();
   check_body(s);
   __syncthreads();
+  if (s->status == TTX_SLOT_RUNNING || s->status == TTX_SLOT_DONE) stamp(s, 6, t0);
   if (threadIdx.x == 0 && g_flops != 0.0) s->flops[3] += g_flops;
 }
 
@@ -992,7 +1010,7 @@ extern "C" int ttx_engine_plan(ttx_slot* s) {
   sz[TTX_SCR_BLOC] = cl;
   sz[TTX_SCR_TMP] = cl > wcap ? cl : wcap;
   sz[TTX_SCR_ZNEW] = kick * nmax * kick;
-  sz[TTX_SCR_MAT] = s->direct_cap * s->direct_cap;
+  sz[TTX_SCR_MAT] = 0;  // the explicit matrix is built straight into the LU buffer
   sz[TTX_SCR_LU] = s->direct_cap * (s->direct_cap + 1);
   sz[TTX_SCR_MR] = 9 * cl;
   const int64_t sapply = s->chunk * cl * ramax;

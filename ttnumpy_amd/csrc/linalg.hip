@@ -68,151 +68,16 @@ __global__ __launch_bounds__(1024) void svd_kernel(SvdArgs p) {
 }
 
 // ---------------------------------------------------------------------------
-// LU with partial pivoting on [M | b], column-major, leading dim n, n+1 cols.
+// LU with partial pivoting and the solve of [M | b] (row-major) by one block:
+// the same blocked routine the solver engine runs per request.
 // ---------------------------------------------------------------------------
-struct PanelArgs {
-  int64_t n, ld, k, nb;
-  double* W;  // row-major [M | b], row stride ld >= n + 1
-  int32_t* piv;
-  int32_t* info;
-};
-
-// One panel of the right-looking LU: factor rows k.. of columns k..k+nb-1 in
-// LDS with partial pivoting, apply the row swaps to every other column and
-// solve the U12 block row.  Row-major storage makes the swaps and the
-// triangular solve coalesced: consecutive threads own consecutive columns.
-__global__ __launch_bounds__(512) void lu_panel_kernel(PanelArgs p) {
-  extern __shared__ double P[];  // nb columns x mp rows, column-major
-  __shared__ double rv[8];
-  __shared__ int64_t ri[8];
-  __shared__ int64_t s_piv[16];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int nwave = blockDim.x >> 6;
-  const int64_t n = p.n, ld = p.ld, k = p.k, nb = p.nb, ncols = n + 1;
-  const int64_t mp = n - k;
-  double* W = p.W;
-  for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
-    const int64_t c = e % nb, r = e / nb;
-    P[c * mp + r] = W[(k + r) * ld + (k + c)];
-  }
-  __syncthreads();
-  for (int64_t j = 0; j < nb; ++j) {
-    double* pj = P + j * mp;
-    // pivot: first index of the largest magnitude (idamax semantics)
-    double best = -1.0;
-    int64_t bi = j;
-    for (int64_t r = j + tid; r < mp; r += blockDim.x) {
-      const double v = fabs(pj[r]);
-      if (v > best) { best = v; bi = r; }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(best, off, 64);
-      const int64_t oi = __shfl_xor(bi, off, 64);
-      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-    }
-    if (lane == 0) { rv[wave] = best; ri[wave] = bi; }
-    __syncthreads();
-    if (tid == 0) {
-      double b0 = rv[0];
-      int64_t i0 = ri[0];
-      for (int w = 1; w < nwave; ++w)
-        if (rv[w] > b0 || (rv[w] == b0 && ri[w] < i0)) { b0 = rv[w]; i0 = ri[w]; }
-      s_piv[j] = i0;
-      p.piv[k + j] = (int32_t)(k + i0);
-      if (b0 == 0.0) p.info[0] = TTX_ESINGULAR;
-    }
-    __syncthreads();
-    const int64_t pr = s_piv[j];
-    if (pr != j) {
-      for (int64_t c = tid; c < nb; c += blockDim.x) {
-        double* pc = P + c * mp;
-        const double t = pc[j];
-        pc[j] = pc[pr];
-        pc[pr] = t;
-      }
-    }
-    __syncthreads();
-    const double d = pj[j];
-    if (d != 0.0) {
-      for (int64_t r = j + 1 + tid; r < mp; r += blockDim.x) pj[r] /= d;
-    }
-    __syncthreads();
-    for (int64_t c = j + 1; c < nb; ++c) {
-      double* pc = P + c * mp;
-      const double u = pc[j];
-      for (int64_t r = j + 1 + tid; r < mp; r += blockDim.x) pc[r] -= pj[r] * u;
-    }
-    __syncthreads();
-  }
-  for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
-    const int64_t c = e % nb, r = e / nb;
-    W[(k + r) * ld + (k + c)] = P[c * mp + r];
-  }
-  for (int64_t c = tid; c < ncols; c += blockDim.x) {
-    if (c >= k && c < k + nb) continue;
-    for (int64_t j = 0; j < nb; ++j) {
-      const int64_t pr = k + s_piv[j];
-      if (pr != k + j) {
-        const double t = W[(k + j) * ld + c];
-        W[(k + j) * ld + c] = W[pr * ld + c];
-        W[pr * ld + c] = t;
-      }
-    }
-    if (c >= k + nb) {
-      for (int64_t i = 1; i < nb; ++i) {
-        double s = W[(k + i) * ld + c];
-        for (int64_t l = 0; l < i; ++l) s -= P[l * mp + i] * W[(k + l) * ld + c];
-        W[(k + i) * ld + c] = s;
-      }
-    }
-  }
+__global__ __launch_bounds__(512) void lu_solve_kernel(int64_t n, int64_t ld, double* W, double* u,
+                                                       int32_t* info, int64_t lds_bytes) {
+  extern __shared__ double lds[];
+  if (threadIdx.x == 0) info[0] = TTX_OK;
+  block_lu_solve(n, ld, W, u, lds, lds_bytes, info);
 }
 
-// U x = y with y the last column (n) of the row-major factor; 64-row blocks,
-// the diagonal block staged in LDS and solved by one wave with shuffles.
-__global__ __launch_bounds__(1024) void lu_backsub_kernel(int64_t n, int64_t ld, const double* W,
-                                                         double* u) {
-  __shared__ double T[64][65];
-  __shared__ double xb[64];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  for (int64_t i = tid; i < n; i += blockDim.x) u[i] = W[i * ld + n];
-  __syncthreads();
-  for (int64_t j1 = n; j1 > 0; j1 -= 64) {
-    const int64_t j0 = j1 - 64 > 0 ? j1 - 64 : 0;
-    const int64_t nbk = j1 - j0;
-    for (int64_t e = tid; e < nbk * nbk; e += blockDim.x) {
-      const int64_t r = e / nbk, c = e % nbk;
-      T[r][c] = W[(j0 + r) * ld + j0 + c];
-    }
-    __syncthreads();
-    if (wave == 0) {
-      const int64_t row = j0 + lane;
-      double yv = row < j1 ? u[row] : 0.0;
-      for (int64_t j = j1 - 1; j >= j0; --j) {
-        const int jj = (int)(j - j0);
-        const double xj = __shfl(yv, jj, 64) / T[jj][jj];
-        if (row == j) yv = xj;
-        if (row < j) yv -= xj * T[lane][jj];
-      }
-      if (row < j1) { u[row] = yv; xb[lane] = yv; }
-    }
-    __syncthreads();
-    for (int64_t r = tid; r < j0; r += blockDim.x) {
-      double s = u[r];
-      const double* wr = W + r * ld + j0;
-      for (int64_t c = 0; c < nbk; ++c) s -= wr[c] * xb[c];
-      u[r] = s;
-    }
-    __syncthreads();
-  }
-}
-
-__global__ void lu_info_init_kernel(int32_t* info) { info[0] = TTX_OK; }
 
 // ---------------------------------------------------------------------------
 // rank rules
@@ -348,44 +213,22 @@ extern "C" int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n, const doub
   return launch_status();
 }
 
-static int64_t lu_block(int64_t n) {
-  // the panel (n rows x nb cols) is staged in LDS
-  int64_t nb = (int64_t)(kLdsLimit / (8 * (n > 0 ? n : 1)));
-  if (nb > 16) nb = 16;
-  return nb;
-}
-
-extern "C" int64_t ttx_lu_work_size(int64_t n) { return (n + 1) / 2 + 1; /* pivots as int32 */ }
+extern "C" int64_t ttx_lu_work_size(int64_t n) { return (n + 1) / 2 + 1; /* reserved */ }
 
 extern "C" int ttx_lu_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, int64_t ld, double* u,
                                 double* work, int32_t* info) {
+  (void)work;
   if (n < 1 || ld < n + 1) return TTX_EINVAL;
-  const int64_t nb0 = lu_block(n);
-  if (nb0 < 1) return TTX_EINVAL;
-  hipStream_t st = (hipStream_t)stream;
+  // the panel (nb >= 1 columns of n rows), the row permutation and the
+  // staging buffers share the block's LDS
+  if ((int64_t)kLdsLimit < 4 * n + 8 * (n > 4224 ? n : 4224)) return TTX_EINVAL;
   static bool attr_set = false;
   if (!attr_set) {
-    if (set_lds_attr((const void*)lu_panel_kernel) != TTX_OK) return TTX_EHIP;
+    if (set_lds_attr((const void*)lu_solve_kernel) != TTX_OK) return TTX_EHIP;
     attr_set = true;
   }
-  int32_t* piv = reinterpret_cast<int32_t*>(work);
-  hipLaunchKernelGGL(lu_info_init_kernel, dim3(1), dim3(1), 0, st, info);
-  for (int64_t k = 0; k < n; k += nb0) {
-    const int64_t nb = (n - k) < nb0 ? (n - k) : nb0;
-    PanelArgs pa{n, ld, k, nb, Mb, piv, info};
-    const size_t shm = (size_t)(nb * (n - k)) * sizeof(double);
-    hipLaunchKernelGGL(lu_panel_kernel, dim3(1), dim3(512), shm, st, pa);
-    const int64_t rows = n - k - nb;
-    const int64_t cols = n + 1 - k - nb;
-    if (rows > 0 && cols > 0) {
-      // A22 -= L21 * U12, all row-major with row stride ld
-      int rc = ttx_gemm_f64(stream, 1, rows, cols, nb, -1.0, Mb + (k + nb) * ld + k, ld, 1, 0,
-                            Mb + k * ld + (k + nb), ld, 1, 0, 1.0, Mb + (k + nb) * ld + (k + nb),
-                            ld, 1, 0);
-      if (rc != TTX_OK) return rc;
-    }
-  }
-  hipLaunchKernelGGL(lu_backsub_kernel, dim3(1), dim3(1024), 0, st, n, ld, Mb, u);
+  hipLaunchKernelGGL(lu_solve_kernel, dim3(1), dim3(512), kLdsLimit, (hipStream_t)stream, n, ld, Mb,
+                     u, info, (int64_t)kLdsLimit);
   return launch_status();
 }
 

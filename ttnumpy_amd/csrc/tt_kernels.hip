@@ -39,16 +39,9 @@ __global__ __launch_bounds__(256) void local_matrix_kernel(LocalMatrixArgs p) {
     const int64_t t_r = col % p.rr;
     const int64_t j = (col / p.rr) % p.nc;
     const int64_t t_l = col / (p.rr * p.nc);
-    double s = 0.0;
-    for (int64_t a = 0; a < p.Ra; ++a) {
-      const double ps = p.psi[(t_l * p.Ra + a) * p.fl + f_l];
-      if (ps == 0.0) continue;
-      const double* arow = p.A + ((a * p.nr + i) * p.nc + j) * p.Rb;
-      double t = 0.0;
-      for (int64_t b = 0; b < p.Rb; ++b) t += arow[b] * p.phi[(t_r * p.Rb + b) * p.fr + f_r];
-      s += ps * t;
-    }
-    p.out[row * p.o_rs + col * p.o_cs] = s;
+    p.out[row * p.o_rs + col * p.o_cs] =
+        local_matrix_entry(p.psi + t_l * p.Ra * p.fl, p.fl, f_l, p.A + (i * p.nc + j) * p.Rb,
+                           p.nr * p.nc * p.Rb, p.Ra, p.Rb, p.phi + t_r * p.Rb * p.fr, p.fr, f_r);
   }
 }
 

@@ -44,4 +44,25 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return t;
 }
 
+// One entry of the explicit one-site matrix (ttnumpy/solvers/one_site.py:49-75):
+//   sum_a psi[t_l, a, f_l] * sum_b A[a, i, j, b] * phi[t_r, b, f_r]
+// psi_t = psi + t_l*Ra*fl, A_ij = A + (i*nc + j)*Rb (a stride nr*nc*Rb),
+// phi_t = phi + t_r*Rb*fr.  Every caller sums in this order, so the direct
+// launches and the engine build bit-identical matrices.
+__device__ __forceinline__ double local_matrix_entry(const double* psi_t, int64_t fl, int64_t f_l,
+                                                    const double* A_ij, int64_t a_stride, int64_t Ra,
+                                                    int64_t Rb, const double* phi_t, int64_t fr,
+                                                    int64_t f_r) {
+  double s = 0.0;
+  for (int64_t a = 0; a < Ra; ++a) {
+    const double ps = psi_t[a * fl + f_l];
+    if (ps == 0.0) continue;
+    const double* arow = A_ij + a * a_stride;
+    double t = 0.0;
+    for (int64_t b = 0; b < Rb; ++b) t += arow[b] * phi_t[b * fr + f_r];
+    s += ps * t;
+  }
+  return s;
+}
+
 }  // namespace ttx

@@ -66,6 +66,7 @@ class Slot(ctypes.Structure):
                                  "red_last", "svd_noconv", "ticks", "minres_iters", "lu_solves", "pad6")]
         + [("flops", _f64 * 4)]
         + [("residuals", _f64 * MAXSWEEPS), ("max_bonds", _i64 * MAXSWEEPS)]
+        + [("clk", _i64 * 16)]
     )
 
 
@@ -134,12 +135,20 @@ class Engine:
         self.pinned = torch.zeros(self.nslots * self.nbytes, dtype=torch.uint8).pin_memory()
         self.slot_req: List[Optional[Request]] = [None] * self.nslots
         self.slot_mem: List[Optional[tuple]] = [None] * self.nslots
+        self.arena: List[Optional[torch.Tensor]] = [None] * self.nslots  # reused across requests
+        # per-input caches: requests that share A, b, x0 or a seeded z0 reuse
+        # the plan, ||b|| and the packed device images of x0 and z0
+        self._plans = {}
+        self._bnorms = {}
+        self._x0_images = {}
+        self._z0_images = {}
         self.profiling = False
         self.kernel_ms = np.zeros(4)     # env, solve, post, check (profiling runs only)
         self.kernel_flops = np.zeros(4)  # algorithmic flops of retired requests
         self.kernel_launches = 0
         self.minres_iters = 0
         self.lu_solves = 0
+        self.stage_ms = np.zeros(16)  # per-stage device wall clock summed over retired requests
         self._push_all()
 
     # ------------------------------------------------------------ host <-> device slots
@@ -165,6 +174,13 @@ class Engine:
         d = len(A)
         if d > MAXD:
             raise ValueError(f"at most {MAXD} cores per request")
+        key = (tuple(A.row_modes), tuple(A.bonds), tuple(b.bonds), req.kickrank,
+               req.max_rank, int(req.direct_solve_size))
+        hit = self._plans.get(key)
+        if hit is not None:
+            s = Slot()
+            ctypes.memmove(ctypes.addressof(s), ctypes.addressof(hit), ctypes.sizeof(Slot))
+            return s
         s = Slot()
         s.d = d
         s.kick = req.kickrank
@@ -180,15 +196,64 @@ class Engine:
         rc = self.lib.ttx_engine_plan(ctypes.byref(s))
         if rc != 0:
             raise ValueError("engine planner rejected the request")
+        if len(self._plans) < 256:
+            keep = Slot()
+            ctypes.memmove(ctypes.addressof(keep), ctypes.addressof(s), ctypes.sizeof(Slot))
+            self._plans[key] = keep
         return s
+
+    @staticmethod
+    def _train_key(t: TensorTrain):
+        """Identity of a train's current contents: object, storage and the
+        in-place version counter of every core."""
+        return (id(t),) + tuple((c.data_ptr(), c._version) for c in t.cores)
+
+    @staticmethod
+    def _cached(cache: dict, key, make):
+        hit = cache.get(key)
+        if hit is None:
+            if len(cache) >= 64:
+                cache.pop(next(iter(cache)))
+            hit = cache[key] = make()
+        return hit[-1]
+
+    def _x0_image(self, x0: TensorTrain, cap: int) -> torch.Tensor:
+        """x0's cores at stride ``cap`` in one device buffer (the arena's x block)."""
+        def make():
+            img = torch.zeros(len(x0) * cap, dtype=D.F64, device=self.dev)
+            for k, c in enumerate(x0.cores):
+                img[k * cap:k * cap + c.numel()].copy_(c.reshape(-1))
+            return (x0, img)
+        return self._cached(self._x0_images, (self._train_key(x0), cap), make)
+
+    def _z0_image(self, seed, n: list, kick: int, cap: int) -> torch.Tensor:
+        """z0 = rand(modes, kickrank, seed) in the reference's draw order
+        (amen.py:80-106), packed at stride ``cap``."""
+        d = len(n)
+
+        def make():
+            rng = np.random.default_rng(seed)
+            zr = [1] + [kick] * (d - 1) + [1]
+            img = np.zeros(d * cap)
+            for k in range(d):
+                core = rng.random((zr[k], n[k], 1, zr[k + 1]))
+                img[k * cap:k * cap + core.size] = core.reshape(-1)
+            return (D.to_device(img),)
+        if seed is None:
+            return make()[0]
+        return self._cached(self._z0_images, (seed, tuple(n), kick, cap), make)
 
     def slot_bytes_needed(self, req: Request) -> int:
         s = self._plan(req)
         d = s.d
         return 8 * (d * s.cap_core + d * s.cap_zcore + s.env_size + s.scr_size)
 
-    def admit(self, i: int, req: Request) -> None:
-        """Place ``req`` in free slot ``i``: arena, descriptor, initial trains."""
+    def admit(self, i: int, req: Request, push: bool = True) -> None:
+        """Place ``req`` in free slot ``i``: arena, descriptor, initial trains.
+
+        With ``push=False`` the descriptor stays on the host until the caller
+        pushes the slot array (one copy for a whole admission round).
+        """
         if req.kickrank <= 0:
             raise ValueError(f"kickrank must be a positive integer, got {req.kickrank}")
         if req.max_rank is not None and req.max_rank <= 0:
@@ -216,34 +281,35 @@ class Engine:
         s.error_threshold = req.error_threshold
         s.stagnation = st.stagnation_factor if st.stagnation_factor is not None else 0.0
         s.bneg = b.orth_center if b.orth_center is not None else 0
-        s.bnorm = b.tt_norm() if st.track_residual else 0.0
+        s.bnorm = (self._cached(self._bnorms, self._train_key(b), lambda: (b, b.tt_norm()))
+                   if st.track_residual else 0.0)
         x0 = req.x0
         if x0 is None:
             from .tensor_train import ones
             x0 = ones(b.physical_dims(), ranks=b.bonds)
         if max(x0.bonds) > self.rmax:
             raise ValueError(f"x0 bond {max(x0.bonds)} exceeds the slot capacity {self.rmax}")
-        # arena: x | z | env | scratch
+        # arena: x | z | env | scratch, kept by the slot across requests
         nx, nz = d * s.cap_core, d * s.cap_zcore
-        arena = torch.zeros(nx + nz + s.env_size + s.scr_size, dtype=D.F64, device=self.dev)
+        need = nx + nz + s.env_size + s.scr_size
+        arena = self.arena[i]
+        if arena is None or arena.numel() < need:
+            self.arena[i] = None
+            arena = self.arena[i] = torch.empty(need, dtype=D.F64, device=self.dev)
+        arena[nx + nz:need].zero_()
+        arena[:nx].copy_(self._x0_image(x0, s.cap_core))
+        zn = [int(s.n[k]) for k in range(d)]
+        arena[nx:nx + nz].copy_(self._z0_image(req.seed, zn, req.kickrank, s.cap_zcore))
         base = arena.data_ptr()
         s.x = base
         s.z = base + 8 * nx
         s.env = base + 8 * (nx + nz)
         s.scr = base + 8 * (nx + nz + s.env_size)
-        keep = [arena, A, b]
+        keep = [A, b]
         for k in range(d):
             s.A[k] = A.cores[k].data_ptr()
             s.b[k] = b.cores[k].data_ptr()
-            D.copy_nd([x0.cores[k].numel()], D.P(x0.cores[k]), [1], s.x + 8 * k * s.cap_core, [1])
-        # z0 = rand(modes, kickrank, seed) in the reference's draw order (amen.py:80-106)
-        rng = np.random.default_rng(req.seed)
         zr = [1] + [req.kickrank] * (d - 1) + [1]
-        for k in range(d):
-            core = rng.random((zr[k], s.n[k], 1, zr[k + 1]))
-            t = D.to_device(core)
-            keep.append(t)
-            D.copy_nd([t.numel()], D.P(t), [1], s.z + 8 * k * s.cap_zcore, [1])
         for k in range(d + 1):
             s.rx[k] = x0.bonds[k]
             s.rz[k] = zr[k]
@@ -259,23 +325,25 @@ class Engine:
         self.slot_req[i] = req
         self.slot_mem[i] = tuple(keep)
         req.admitted = time.perf_counter()
-        self._push(i)
+        if push:
+            self._push(i)
 
     # ------------------------------------------------------------ retirement
-    def _retire(self, i: int) -> None:
+    def _retire(self, i: int, push: bool = True) -> None:
         s = self.host[i]
         req = self.slot_req[i]
         self.kernel_flops += np.array(list(s.flops))
         self.minres_iters += int(s.minres_iters)
         self.lu_solves += int(s.lu_solves)
+        self.stage_ms += np.array(list(s.clk), dtype=np.float64) * 1e-5  # 100 MHz ticks
         if s.status == SLOT_DONE:
             d = s.d
-            arena = self.slot_mem[i][0]
+            xs = self.arena[i][:d * s.cap_core].clone()
             cores = []
             for k in range(d):
                 r0, n, r1 = s.rx[k], s.n[k], s.rx[k + 1]
                 off = k * s.cap_core
-                cores.append(arena[off:off + r0 * n * r1].reshape(r0, n, 1, r1).clone())
+                cores.append(xs[off:off + r0 * n * r1].view(r0, n, 1, r1))
             x = TensorTrain(cores)
             x.orth_center = 0
             x.max_rank = max(x.bonds)
@@ -297,7 +365,8 @@ class Engine:
         s.status = SLOT_EMPTY
         self.slot_req[i] = None
         self.slot_mem[i] = None
-        self._push(i)
+        if push:
+            self._push(i)
 
     # ------------------------------------------------------------ loop
     def tick(self, n: int = 1) -> None:
@@ -330,12 +399,20 @@ class Engine:
         while pending or self.active():
             # admit
             free = [i for i in range(self.nslots) if self.slot_req[i] is None]
-            while pending and free and pending[0].arrival <= now() - 0.0:
-                need = self.slot_bytes_needed(pending[0])
+            admitted = False
+            if pending and free and pending[0].arrival <= now():
                 free_b, _ = torch.cuda.mem_get_info(self.dev)
-                if free_b - need < self.reserve and self.active():
-                    break
-                self.admit(free.pop(0), pending.pop(0))
+                while pending and free and pending[0].arrival <= now():
+                    i = free[0]
+                    held = self.arena[i].numel() * 8 if self.arena[i] is not None else 0
+                    need = max(self.slot_bytes_needed(pending[0]) - held, 0)
+                    if free_b - need < self.reserve and self.active():
+                        break
+                    self.admit(free.pop(0), pending.pop(0), push=False)
+                    free_b -= need
+                    admitted = True
+            if admitted:
+                self._push_all()
             if not self.active():
                 if pending:
                     time.sleep(max(0.0, min(0.001, pending[0].arrival - now())))
@@ -343,13 +420,17 @@ class Engine:
             self.tick(poll)
             ticks += poll
             self._pull()
+            retired = False
             for i in range(self.nslots):
                 if self.slot_req[i] is not None and self.host[i].status != SLOT_RUNNING:
                     req = self.slot_req[i]
-                    self._retire(i)
+                    self._retire(i, push=False)
+                    retired = True
                     done.append(req)
                     if on_done is not None:
                         on_done(req)
+            if retired:
+                self._push_all()
             if ticks > max_ticks:
                 raise RuntimeError("engine did not finish within the tick limit")
         return done

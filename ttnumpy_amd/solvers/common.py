@@ -385,7 +385,9 @@ def solve_local_system(op, b_loc: torch.Tensor, *, iterative_solver, local_rtol,
     """Restarted Krylov solve with the true-residual check (``common.py:776-809``).
 
     Returns ``(u, op)``.  The direct path is handled by the caller (LU on the
-    explicit matrix), as the reference does at ``common.py:766-774``.
+    explicit matrix), as the reference does at ``common.py:766-774``; with
+    ``explicit_fallback`` true a solve that misses its target after the
+    restarts is redone that way (``common.py:794-809``).
     """
     if iterative_solver is LocalSolver.GMRES:
         raise NotImplementedError("the GMRES local solver is not part of this build; "
@@ -406,25 +408,28 @@ def solve_local_system(op, b_loc: torch.Tensor, *, iterative_solver, local_rtol,
                 stats.record_local_solve(fell_back=False)
             return u, op
         rt = max(rt * _RESTART_RTOL_SHRINK, 1e-15)
-    if explicit_fallback is None:
+    if not explicit_fallback:
         if stats is not None:
             stats.record_local_solve(inexact=True)
         return u, op
     if stats is not None:
         stats.record_local_solve(fell_back=True)
-    M = explicit_fallback()
-    u = direct_solve(M, b_loc)
-    return u, M
+    u = direct_solve(op, b_loc)
+    return u, op
 
 
-def direct_solve(op_explicit, b_loc: torch.Tensor) -> torch.Tensor:
-    """LU with partial pivoting on [M | b] (``common.py:766-774``)."""
-    n = op_explicit.rows
+def direct_solve(op, b_loc: torch.Tensor) -> torch.Tensor:
+    """LU with partial pivoting on [M | b], M the explicit matrix of ``op`` (``common.py:766-774``).
+
+    The LU status is left on ``op.lu_info`` (device int32) for the caller's
+    next synchronisation.
+    """
+    n = op.rows
     Mb = D.empty(n * (n + 1))
-    D.copy_nd([n, n], P(op_explicit.M), [n, 1], P(Mb), [n + 1, 1])
+    op.explicit(Mb, n + 1)
     D.copy_nd([n], P(b_loc), [1], P(Mb, n), [n + 1])
     u = D.empty(n)
     info = torch.zeros(1, dtype=torch.int32, device=D.device())
     D.lu_solve(Mb, n, u, info)
-    op_explicit.lu_info = info
+    op.lu_info = info
     return u

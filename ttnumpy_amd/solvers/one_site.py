@@ -47,15 +47,13 @@ def _solve_local_one_site(k: int, x: TensorTrain, A: TensorTrain, b: TensorTrain
         warm = x.cores[k].reshape(-1)
         if warm.numel() != size:
             warm = None
-        fallback = None
-        if settings.allow_dense_fallback:
-            def fallback():
-                return local_explicit_matrix_one_site(k, left_env_A, right_env_A, A)
         u, op = solve_local_system(op, b_loc, iterative_solver=settings.iterative_solver,
                                    local_rtol=settings.local_rtol, warm_start=warm,
-                                   explicit_fallback=fallback, stats=stats)
+                                   explicit_fallback=settings.allow_dense_fallback, stats=stats)
         return u, op, b_loc
-    op = local_explicit_matrix_one_site(k, left_env_A, right_env_A, A)
+    # the explicit matrix goes straight into the LU buffer; the residual and
+    # rank tests that follow apply the same operator matrix-free
+    op = D.LocalOp(left_env_A[k], A.cores[k], right_env_A[k])
     u = direct_solve(op, b_loc)
     return u, op, b_loc
 
