@@ -5,8 +5,9 @@ Kronecker-sum QTT operator of 16 cores with 2x2 modes and TTM bonds <= 4,
 b = all-ones rank-1 train, x0 = rand([2]*16, ranks=2, seed=0);
 amen(symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0) in FP64.
 
-One step = one batch of ``--batch`` such solves served concurrently by the
-device-resident engine (one slot per request, continuous batching); inputs
+One step = one batch of ``--batch`` (default 512: two request slots per CU)
+such solves served concurrently by the device-resident engine (one slot per
+request, continuous batching); inputs
 are resident in HBM before the clock starts.  ``value`` is solves per second
 over the timed steps; p50/p99 are per-request latencies (admission to
 result).  Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build
@@ -29,6 +30,7 @@ sys.path.insert(0, ROOT)
 N_BITS = 8  # 256 x 256 grid
 RMAX = 48   # slot rank capacity (the C1 solution reaches bond 33)
 PEAK_FP64_TFLOPS = 78.6
+BATCH_PROFILED = 512  # batch of the committed counter summary
 KERNELS = ("ttx::eng_env_kernel", "ttx::eng_solve_kernel", "ttx::eng_post_kernel",
            "ttx::eng_check_kernel")
 
@@ -56,7 +58,7 @@ def measured_traffic(kernel: str):
     (a counter run cannot time); the summary names the commit it measured.
     """
     short = kernel.split("::")[-1]
-    path = os.path.join(ROOT, "profiles", f"{short}_c1_batch256_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"{short}_c1_batch{BATCH_PROFILED}_traffic.json")
     if not os.path.exists(path):
         return None, None
     t = json.load(open(path))
@@ -92,7 +94,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--poll", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--single", action="store_true",
@@ -188,7 +190,8 @@ def main():
         "gflop_per_solve": float(prof.kernel_flops.sum()) / max(args.batch, 1) / 1e9,
         "stage_ms_per_solve": {nm: round(float(v) / max(args.batch, 1), 3) for nm, v in zip(
             ("rhs", "matrix", "lu", "minres", "exact_res", "post", "check", "env", "post.residual_core",
-             "post.svd", "post.rank", "post.enrich_qr", "post.zshift"), prof.stage_ms)},
+             "post.svd", "post.rank", "post.enrich_qr", "post.zshift", "lu.panels", "lu.in_panel",
+             "lu.block_rows", "lu.trailing", "lu.backsub"), prof.stage_ms)},
         "roofline": roof,
     }
     if not args.no_cpu:

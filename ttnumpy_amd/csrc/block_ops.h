@@ -73,7 +73,7 @@ __device__ __forceinline__ void block_householder_qr(int64_t m, int64_t n, doubl
   if (!form_q) return;
   // Q = H_0 ... H_{k-1} [I_k; 0], accumulated backwards
   for (int64_t e = tid; e < m * k; e += blockDim.x) {
-    const int64_t i = e % m, c = e / m;
+    const int64_t c = q32(e, m), i = e - c * m;
     Qw[e] = (i == c) ? 1.0 : 0.0;
   }
   __syncthreads();
@@ -143,7 +143,7 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   if (tid == 0) s_bad = 0;
   __syncthreads();
   for (int64_t e = tid; e < P * Q; e += blockDim.x) {
-    const int64_t i = e % P, j = e / P;
+    const int64_t j = q32(e, P), i = e - j * P;
     const double v = p.A[i * x_rs + j * x_cs];
     if (!isfinite(v)) s_bad = 1;
     Wq[e] = v;
@@ -157,13 +157,13 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   if (use_qr) {
     block_householder_qr(P, Q, Wq, tau, Qx, red);
     for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
-      const int64_t i = e % Q, c = e / Q;
+      const int64_t c = q32(e, Q), i = e - c * Q;
       G[e] = (i <= c) ? Wq[i + c * P] : 0.0;
     }
   } else {
     for (int64_t e = tid; e < Q * Q; e += blockDim.x) G[e] = Wq[e];
   }
-  for (int64_t e = tid; e < Q * Q; e += blockDim.x) V[e] = ((e % Q) == (e / Q)) ? 1.0 : 0.0;
+  for (int64_t e = tid; e < Q * Q; e += blockDim.x) V[e] = (e == q32(e, Q) * (Q + 1)) ? 1.0 : 0.0;
   __syncthreads();
 
   // one-sided Jacobi on the columns of G, round-robin pairs per round
@@ -245,7 +245,7 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   __syncthreads();
   // normalise the columns of G in place: G[:, j] /= sig[j]
   for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
-    const int64_t j = e / Q;
+    const int64_t j = q32(e, Q);
     const double s = sig[j];
     G[e] = s > 0.0 ? G[e] / s : 0.0;
   }
@@ -253,7 +253,7 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   // Ux (P x Q) column o = (use_qr ? Qx : I) * G[:, ord[o]];  V column ord[o]
   // tall: U_A = Ux, Vt_A = V^T.  wide: U_A = V, Vt_A = Ux^T.
   for (int64_t e = tid; e < P * Q; e += blockDim.x) {
-    const int64_t r = e % P, o = e / P;
+    const int64_t o = q32(e, P), r = e - o * P;
     const int64_t src = (int64_t)ordd[o];
     double v;
     if (use_qr) {
@@ -269,7 +269,7 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
       p.Vt[o * p.v_rs + r * p.v_cs] = v;
   }
   for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
-    const int64_t r = e % Q, o = e / Q;
+    const int64_t o = q32(e, Q), r = e - o * Q;
     const int64_t src = (int64_t)ordd[o];
     const double v = V[r + src * Q];
     if (!wide)
@@ -382,33 +382,39 @@ __device__ __forceinline__ void bgemv(int64_t rows, int64_t cols, double alpha, 
 //
 // lds: lds_bytes of shared memory (the panel, the GEMM staging, the back
 // substitution's diagonal blocks) with perm in its last 4 n bytes; needs
-// lds_bytes >= 4 n + 8 max(n, 4224).  *info = TTX_ESINGULAR on a zero pivot
+// lds_bytes >= 4 n + 8 max(n, 4224).  NT = blockDim.x, 256 or 512.  *info = TTX_ESINGULAR on a zero pivot
 // (the factorization continues, as getrf does).  Ends synced.
 // ---------------------------------------------------------------------------
-constexpr int kLuTile = 128;  // trailing-update output tile (rows and columns)
-constexpr int kLuK = 16;      // K slab staged per step
+constexpr int kLuTileM = 128;  // trailing-update output tile rows
+constexpr int kLuK = 16;       // K slab staged per step
 
 // For logical rows i < Mr and columns j < Nc:
 //   W[perm[r0+i]][c0+j] -= sum_{t<K} W[perm[r0+i]][a0+t] * W[perm[b0+t]][c0+j]
-// 128 x 128 output tiles; each of the 8 waves owns 32 x 64 (2 x 4 MFMA 16x16
-// blocks) accumulated in registers; A and B are staged kLuK deep in LDS,
-// k-major, with the next slab's global loads in flight during the MFMAs.
-// Uses 2 * kLuK * kLuTile doubles of lds.  Ends synced.
-__device__ __forceinline__ void lu_tile_update(int64_t Mr, int64_t Nc, int64_t K, double* W,
-                                               int64_t ld, const int* perm, int64_t r0,
-                                               int64_t a0, int64_t b0, int64_t c0, double* lds) {
+// NT threads (256 or 512): output tiles of 128 x (64 NT / 256); each wave owns
+// 32 x 64 (2 x 4 MFMA 16x16 blocks) accumulated in registers; A and B are
+// staged kLuK deep in LDS, k-major, with the next slab's global loads in
+// flight during the MFMAs.  Uses kLuK * (128 + tile columns) doubles of lds.
+// Ends synced.
+template <int NT>
+__device__ __forceinline__ void lu_tile_update_t(int64_t Mr, int64_t Nc, int64_t K, double* W,
+                                                 int64_t ld, const int* perm, int64_t r0,
+                                                 int64_t a0, int64_t b0, int64_t c0, double* lds) {
+  constexpr int TN = 64 * NT / 256;       // tile columns
+  constexpr int EA = kLuTileM * kLuK / NT;  // A slab elements per thread (consecutive k)
+  constexpr int TPA = kLuK / EA;            // threads per A row
+  constexpr int TPB = TN / 4;               // threads per B k-row (4 columns each)
   if (Mr <= 0 || Nc <= 0 || K <= 0) return;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, lq = lane >> 4;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
-  double* As = lds;                    // [kLuK][kLuTile]
-  double* Bs = lds + kLuK * kLuTile;   // [kLuK][kLuTile]
-  const int ar = tid >> 2, ak = (tid & 3) * 4;   // A loader: row ar, k ak..ak+3
-  const int bk = tid >> 5, bc = (tid & 31) * 4;  // B loader: k row bk, columns bc..bc+3
-  const int64_t tm = (Mr + kLuTile - 1) / kLuTile, tn = (Nc + kLuTile - 1) / kLuTile;
+  const int wm = (wave & 3) * 32, wn = (wave >> 2) * 64;
+  double* As = lds;                     // [kLuK][kLuTileM]
+  double* Bs = lds + kLuK * kLuTileM;   // [kLuK][TN]
+  const int ar = tid / TPA, ak = (tid % TPA) * EA;
+  const int bk = tid / TPB, bc = (tid % TPB) * 4;
+  const int64_t tm = (Mr + kLuTileM - 1) / kLuTileM, tn = (Nc + TN - 1) / TN;
   for (int64_t t = 0; t < tm * tn; ++t) {
-    const int64_t m0 = (t / tn) * kLuTile, n0 = (t % tn) * kLuTile;
+    const int64_t m0 = (t / tn) * kLuTileM, n0 = (t % tn) * TN;
     // the 8 output rows of this lane: row m0 + wm + 16 mt + 4 e + lq
     double* crow[2][4];
 #pragma unroll
@@ -430,10 +436,10 @@ __device__ __forceinline__ void lu_tile_update(int64_t Mr, int64_t Nc, int64_t K
       }
     const int64_t arow = m0 + ar;
     const double* ap = arow < Mr ? W + (int64_t)perm[r0 + arow] * ld + a0 : nullptr;
-    double ra[4], rb[4];
+    double ra[EA], rb[4];
     auto gload = [&](int64_t k0) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < EA; ++i) {
         const int64_t kk = k0 + ak + i;
         ra[i] = (ap && kk < K) ? ap[kk] : 0.0;
       }
@@ -449,9 +455,9 @@ __device__ __forceinline__ void lu_tile_update(int64_t Mr, int64_t Nc, int64_t K
     for (int64_t k0 = 0; k0 < K; k0 += kLuK) {
       __syncthreads();  // the previous slab's MFMAs are done with As/Bs
 #pragma unroll
-      for (int i = 0; i < 4; ++i) As[(ak + i) * kLuTile + ar] = ra[i];
+      for (int i = 0; i < EA; ++i) As[(ak + i) * kLuTileM + ar] = ra[i];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) Bs[bk * kLuTile + bc + i] = rb[i];
+      for (int i = 0; i < 4; ++i) Bs[bk * TN + bc + i] = rb[i];
       __syncthreads();
       if (k0 + kLuK < K) gload(k0 + kLuK);
 #pragma unroll
@@ -459,9 +465,9 @@ __device__ __forceinline__ void lu_tile_update(int64_t Mr, int64_t Nc, int64_t K
         const int kr = ks + lq;
         double a[2], b[4];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) a[mt] = -As[kr * kLuTile + wm + 16 * mt + l16];
+        for (int mt = 0; mt < 2; ++mt) a[mt] = -As[kr * kLuTileM + wm + 16 * mt + l16];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) b[nt] = Bs[kr * kLuTile + wn + 16 * nt + l16];
+        for (int nt = 0; nt < 4; ++nt) b[nt] = Bs[kr * TN + wn + 16 * nt + l16];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
@@ -507,8 +513,21 @@ __device__ __forceinline__ void lu_trsm_rows(int64_t k, int64_t nb, int64_t cb, 
   }
 }
 
+// clk (may be null): wall-clock ticks added per phase: [0] panels, [1] the
+// in-panel triangular solves and updates, [2] block rows, [3] trailing updates,
+// [4] back substitution.
+__device__ __forceinline__ void lu_clock(int64_t* clk, int i, int64_t& t0) {
+  if (clk == nullptr) return;
+  const int64_t t = wall_clock64();
+  if (threadIdx.x == 0) clk[i] += t - t0;
+  t0 = t;
+}
+
+template <int NT>
 __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W, double* u,
-                                               double* lds, int64_t lds_bytes, int32_t* info) {
+                                               double* lds, int64_t lds_bytes, int32_t* info,
+                                               int64_t* clk = nullptr) {
+  int64_t t0 = wall_clock64();
   __shared__ double rv[16];
   __shared__ int ri[16];
   const int tid = threadIdx.x;
@@ -532,9 +551,23 @@ __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W,
       const int64_t nb = (K1 - k) < nb0 ? (K1 - k) : nb0;
       const int64_t mp = n - k;
       double* P = lds;
-      for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
-        const int64_t c = e % nb, r = e / nb;
-        P[c * mp + r] = W[(int64_t)perm[k + r] * ld + (k + c)];
+      if ((tid & 15) < nb) {
+        // 16 lanes per row (one 128-byte read per row), 8 rows in flight per lane
+        const int c = tid & 15;
+        const int64_t rs = blockDim.x >> 4;
+        for (int64_t rb = tid >> 4; rb < mp; rb += 8 * rs) {
+          double v[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int64_t r = rb + q * rs;
+            v[q] = r < mp ? W[(int64_t)perm[k + r] * ld + (k + c)] : 0.0;
+          }
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int64_t r = rb + q * rs;
+            if (r < mp) P[c * mp + r] = v[q];
+          }
+        }
       }
       __syncthreads();
       for (int64_t j = 0; j < nb; ++j) {
@@ -583,15 +616,18 @@ __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W,
         }
         __syncthreads();
       }
-      for (int64_t e = tid; e < nb * mp; e += blockDim.x) {
-        const int64_t c = e % nb, r = e / nb;
-        W[(int64_t)perm[k + r] * ld + (k + c)] = P[c * mp + r];
+      if ((tid & 15) < nb) {
+        const int c = tid & 15;
+        for (int64_t r = tid >> 4; r < mp; r += blockDim.x >> 4)
+          W[(int64_t)perm[k + r] * ld + (k + c)] = P[c * mp + r];
       }
+      lu_clock(clk, 0, t0);
       // U rows k..k+nb of the panel columns right of this sub-panel, then the
       // rank-nb update of the rest of the outer panel (all rows below)
       lu_trsm_rows(k, nb, k + nb, K1, W, ld, perm, P, mp);
       __syncthreads();
-      lu_tile_update(n - (k + nb), K1 - (k + nb), nb, W, ld, perm, k + nb, k, k, k + nb, lds);
+      lu_tile_update_t<NT>(n - (k + nb), K1 - (k + nb), nb, W, ld, perm, k + nb, k, k, k + nb, lds);
+      lu_clock(clk, 1, t0);
     }
     // the outer panel is final (its pivots are all known), so its block row
     // U12 = L11^{-1} A12 can be formed: per sub-panel a triangular solve with
@@ -600,16 +636,18 @@ __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W,
       const int64_t nb = (K1 - k) < nb0 ? (K1 - k) : nb0;
       double* L = lds;  // nb x nb, L[l * 16 + i] = L_kk[i][l]
       for (int64_t e = tid; e < nb * nb; e += blockDim.x) {
-        const int64_t i = e / nb, l = e % nb;
+        const int64_t i = q32(e, nb), l = e - i * nb;
         L[l * 16 + i] = W[(int64_t)perm[k + i] * ld + k + l];
       }
       __syncthreads();
       lu_trsm_rows(k, nb, K1, n + 1, W, ld, perm, L, 16);
       __syncthreads();
-      lu_tile_update(K1 - (k + nb), n + 1 - K1, nb, W, ld, perm, k + nb, k, k, K1, lds);
+      lu_tile_update_t<NT>(K1 - (k + nb), n + 1 - K1, nb, W, ld, perm, k + nb, k, k, K1, lds);
     }
+    lu_clock(clk, 2, t0);
     // the rest of the matrix: one rank-ko update
-    lu_tile_update(n - K1, n + 1 - K1, ko, W, ld, perm, K1, K0, K0, K1, lds);
+    lu_tile_update_t<NT>(n - K1, n + 1 - K1, ko, W, ld, perm, K1, K0, K0, K1, lds);
+    lu_clock(clk, 3, t0);
   }
   // back substitution U u = y in 64-column blocks from the bottom; the
   // diagonal block is solved by one wave, the rows above are updated one
@@ -621,9 +659,9 @@ __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W,
   for (int64_t j1 = n; j1 > 0; j1 -= 64) {
     const int64_t j0 = j1 - 64 > 0 ? j1 - 64 : 0;
     const int64_t nbk = j1 - j0;
-    for (int64_t e = tid; e < nbk * nbk; e += blockDim.x) {
-      const int64_t r = e / nbk, c = e % nbk;
-      T[r * 65 + c] = W[(int64_t)perm[j0 + r] * ld + j0 + c];
+    if (lane < nbk) {
+      for (int64_t r = wave; r < nbk; r += nwave)
+        T[r * 65 + lane] = W[(int64_t)perm[j0 + r] * ld + j0 + lane];
     }
     __syncthreads();
     if (wave == 0) {
@@ -638,14 +676,24 @@ __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W,
       if (row < j1) { u[row] = yv; xb[lane] = yv; }
     }
     __syncthreads();
-    for (int64_t r = wave; r < j0; r += nwave) {
-      const double* wr = W + (int64_t)perm[r] * ld + j0;
-      double s = lane < nbk ? wr[lane] * xb[lane] : 0.0;
-      s = wave_sum(s);
-      if (lane == 0) u[r] -= s;
+    const double xl = lane < nbk ? xb[lane] : 0.0;
+    for (int64_t r = wave; r < j0; r += 4 * nwave) {  // 4 rows in flight per wave
+      double sv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t rq = r + q * nwave;
+        sv[q] = (rq < j0 && lane < nbk) ? W[(int64_t)perm[rq] * ld + j0 + lane] * xl : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double t = wave_sum(sv[q]);
+        const int64_t rq = r + q * nwave;
+        if (lane == 0 && rq < j0) u[rq] -= t;
+      }
     }
     __syncthreads();
   }
+  lu_clock(clk, 4, t0);
 }
 
 }  // namespace ttx

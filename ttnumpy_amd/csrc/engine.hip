@@ -26,8 +26,11 @@
 
 namespace ttx {
 
-constexpr int kEngThreads = 512;
-constexpr size_t kEngLds = 150 * 1024;
+// One 256-thread workgroup per request slot with 75 KiB of LDS, so two slots
+// share a CU (160 KiB of LDS, 2 waves per SIMD at <= 256 VGPRs): while one
+// slot waits at a barrier or on memory, the other one issues.
+constexpr int kEngThreads = 256;
+constexpr size_t kEngLds = 75 * 1024;
 
 __device__ __forceinline__ double* xcore(const ttx_slot* s, int64_t k) { return s->x + k * s->cap_core; }
 __device__ __forceinline__ double* zcore(const ttx_slot* s, int64_t k) { return s->z + k * s->cap_zcore; }
@@ -121,35 +124,81 @@ struct LocOp {
   }
 };
 
-// explicit matrix into out (row-major, row stride ld).  Each thread owns a
-// column and a slice of the rows, so the stores of a row are coalesced; psi
-// and phi are staged in LDS when they fit in lds_cap doubles.  Ends synced.
+// explicit matrix into out (row-major, row stride ld):
+//   M[(f_l,i,f_r),(t_l,j,t_r)] = sum_a psi[t_l,a,f_l] * T_a,  T_a = sum_b A[a,i,j,b] phi[t_r,b,f_r]
+// Each thread owns a column (t_l, j, t_r) and a share of the (i, f_r) pairs;
+// T_a does not depend on f_l, so it is formed once per pair and reused down
+// the fl rows that share it.  Stores of a row stay coalesced across threads.
+// psi, A_k and phi are staged in LDS when they fit in lds_cap doubles.
+// Matches local_matrix_entry's sums term for term (a ascending, zero psi
+// skipped, b ascending).  Ends synced.
 __device__ void local_matrix_blk(const LocOp& op, double* out, int64_t ld, double* lds,
                                  int64_t lds_cap) {
+  constexpr int kMaxRa = 16;
   const int rows = (int)op.rows(), cols = (int)op.cols();
   add_flops(2.0 * (double)rows * (double)cols * (double)op.Ra * (double)op.Rb);
+  const int Ra = (int)op.Ra, Rb = (int)op.Rb, fl = (int)op.fl, fr = (int)op.fr, nr = (int)op.nr,
+            nc = (int)op.nc, rr = (int)op.rr;
+  const int npsi = (int)(op.rl * Ra * fl), nphi = (int)(rr * Rb * fr), nA = Ra * nr * nc * Rb;
   const double* psi = op.psi;
   const double* phi = op.phi;
-  const int npsi = (int)(op.rl * op.Ra * op.fl), nphi = (int)(op.rr * op.Rb * op.fr);
-  if ((int64_t)(npsi + nphi) <= lds_cap) {
+  const double* A = op.A;
+  if ((int64_t)(npsi + nphi + nA) <= lds_cap) {
     for (int e = threadIdx.x; e < npsi; e += blockDim.x) lds[e] = psi[e];
     for (int e = threadIdx.x; e < nphi; e += blockDim.x) lds[npsi + e] = phi[e];
+    for (int e = threadIdx.x; e < nA; e += blockDim.x) lds[npsi + nphi + e] = A[e];
     __syncthreads();
     psi = lds;
     phi = lds + npsi;
+    A = lds + npsi + nphi;
   }
-  const int fr = (int)op.fr, nr = (int)op.nr, rr = (int)op.rr, nc = (int)op.nc;
+  if (Ra > kMaxRa) {  // wide operator bonds: entry by entry
+    for (int it = threadIdx.x; it < cols; it += blockDim.x) {
+      const int cq = it / rr, t_r = it - cq * rr, t_l = cq / nc, j = cq - t_l * nc;
+      for (int row = 0; row < rows; ++row) {
+        const int rq = row / fr, f_r = row - rq * fr, f_l = rq / nr, i = rq - f_l * nr;
+        out[(int64_t)row * ld + it] = local_matrix_entry(
+            psi + (int64_t)t_l * Ra * fl, fl, f_l, A + (i * nc + j) * Rb, (int64_t)nr * nc * Rb, Ra,
+            Rb, phi + (int64_t)t_r * Rb * fr, fr, f_r);
+      }
+    }
+    __syncthreads();
+    return;
+  }
+  const int npairs = nr * fr;
   const int S = cols >= (int)blockDim.x ? 1 : (int)blockDim.x / cols;
-  const int64_t a_stride = op.nr * op.nc * op.Rb;
+  const int a_stride = nr * nc * Rb;
   for (int it = threadIdx.x; it < cols * S; it += blockDim.x) {
-    const int c = it % cols, sl = it / cols;
-    const int t_r = c % rr, j = (c / rr) % nc, t_l = c / (rr * nc);
-    const double* ps = psi + (int64_t)t_l * op.Ra * op.fl;
-    const double* ph = phi + (int64_t)t_r * op.Rb * op.fr;
-    for (int row = sl; row < rows; row += S) {
-      const int f_r = row % fr, i = (row / fr) % nr, f_l = row / (fr * nr);
-      out[(int64_t)row * ld + c] = local_matrix_entry(ps, op.fl, f_l, op.A + (i * op.nc + j) * op.Rb,
-                                                      a_stride, op.Ra, op.Rb, ph, op.fr, f_r);
+    const int sl = it / cols, c = it - sl * cols;
+    const int cq = c / rr, t_r = c - cq * rr, t_l = cq / nc, j = cq - t_l * nc;
+    const double* ps = psi + t_l * Ra * fl;
+    const double* ph = phi + t_r * Rb * fr;
+    const int pbeg = npairs * sl / S, pend = npairs * (sl + 1) / S;
+    for (int pq = pbeg; pq < pend; ++pq) {
+      const int i = pq / fr, f_r = pq - i * fr;
+      double T[kMaxRa];
+#pragma unroll
+      for (int a = 0; a < kMaxRa; ++a) {
+        if (a < Ra) {
+          const double* arow = A + a * a_stride + (i * nc + j) * Rb;
+          double t = 0.0;
+          for (int b = 0; b < Rb; ++b) t += arow[b] * ph[b * fr + f_r];
+          T[a] = t;
+        }
+      }
+      double* o = out + (int64_t)(i * fr + f_r) * ld + c;
+      const int64_t ostep = (int64_t)nr * fr * ld;
+      for (int f_l = 0; f_l < fl; ++f_l) {
+        double sum = 0.0;
+#pragma unroll
+        for (int a = 0; a < kMaxRa; ++a) {
+          if (a < Ra) {
+            const double p = ps[a * fl + f_l];
+            if (p != 0.0) sum += p * T[a];
+          }
+        }
+        o[f_l * ostep] = sum;
+      }
     }
   }
   __syncthreads();
@@ -167,7 +216,7 @@ __device__ void block_qr(int64_t m, int64_t n, const double* A, int64_t a_rs, in
   double* tau = W + m * n;
   double* Qw = tau + k;
   for (int64_t e = threadIdx.x; e < m * n; e += blockDim.x) {
-    const int64_t i = e % m, j = e / m;
+    const int64_t j = q32(e, m), i = e - j * m;
     W[e] = A[i * a_rs + j * a_cs];
   }
   __syncthreads();
@@ -175,12 +224,12 @@ __device__ void block_qr(int64_t m, int64_t n, const double* A, int64_t a_rs, in
   __syncthreads();
   if (Q) {
     for (int64_t e = threadIdx.x; e < m * k; e += blockDim.x) {
-      const int64_t i = e % m, c = e / m;
+      const int64_t c = q32(e, m), i = e - c * m;
       Q[i * q_rs + c * q_cs] = Qw[e];
     }
   }
   for (int64_t e = threadIdx.x; e < k * n; e += blockDim.x) {
-    const int64_t i = e % k, c = e / k;
+    const int64_t c = q32(e, k), i = e - c * k;
     R[i * r_rs + c * r_cs] = (i <= c) ? W[i + c * m] : 0.0;
   }
   __syncthreads();
@@ -415,8 +464,8 @@ __device__ int64_t choose_rank_blk(ttx_slot* s, const LocOp& op, const double* b
   for (int64_t c0 = 0; c0 < limit; c0 += s->chunk) {
     const int64_t cb = (limit - c0) < s->chunk ? (limit - c0) : s->chunk;
     for (int64_t e = threadIdx.x; e < N * cb; e += blockDim.x) {
-      const int64_t row = e % N, j = e / N;
-      const int64_t i = row / q, jj = row % q, r = c0 + j;
+      const int64_t j = q32(e, N), row = e - j * N;
+      const int64_t i = q32(row, q), jj = row - i * q, r = c0 + j;
       terms[e] = U[i * K + r] * S[r] * Vt[r * q + jj];
     }
     __syncthreads();
@@ -497,7 +546,7 @@ __device__ void env_body(ttx_slot* s, int kind) {
                   with_op ? s->ra[c] : 1, with_op ? s->ra[c + 1] : 1, out, s1, s2);
 }
 
-__global__ __launch_bounds__(kEngThreads) void eng_env_kernel(ttx_slot* slots) {
+__global__ __launch_bounds__(kEngThreads, 2) void eng_env_kernel(ttx_slot* slots) {
   ttx_slot* s = slots + blockIdx.x;
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
@@ -560,7 +609,7 @@ __device__ void solve_body(ttx_slot* s, double* lds) {
     __syncthreads();
     stamp(s, 1, t0);
     if ((int64_t)kEngLds < 4 * N + 8 * (N > 4224 ? N : 4224)) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
-    block_lu_solve(N, N + 1, W, u, lds, kEngLds, &s_info);
+    block_lu_solve<kEngThreads>(N, N + 1, W, u, lds, kEngLds, &s_info, &s->clk[13]);
     __syncthreads();
     stamp(s, 2, t0);
     if (s_info != TTX_OK) { fail(s, TTX_SLOT_ERR_SINGULAR); return; }
@@ -581,7 +630,7 @@ __device__ void solve_body(ttx_slot* s, double* lds) {
   }
 }
 
-__global__ __launch_bounds__(kEngThreads) void eng_solve_kernel(ttx_slot* slots) {
+__global__ __launch_bounds__(kEngThreads, 2) void eng_solve_kernel(ttx_slot* slots) {
   extern __shared__ double lds[];
   ttx_slot* s = slots + blockIdx.x;
   if (threadIdx.x == 0) g_flops = 0.0;
@@ -693,14 +742,14 @@ __device__ void post_body(ttx_slot* s) {
     const int64_t W = rank + zr;
     const int64_t m = rl * n;
     for (int64_t e = threadIdx.x; e < m * rank; e += blockDim.x) {
-      const int64_t row = e / rank, c = e % rank;
+      const int64_t row = q32(e, rank), c = e - row * rank;
       enr[row * W + c] = U[row * K + c];
     }
     bgemm(n, rl, zr, zl, 1.0, envp(s, 4, 0, k), 1, rl, 0, zn, n * zr, 1, zr, 0.0, enr + rank, n * W,
           1, W);
     double* carry = tmp;
     for (int64_t e = threadIdx.x; e < rank * rr; e += blockDim.x) {
-      const int64_t c = e / rr;
+      const int64_t c = q32(e, rr);
       carry[e] = S[c] * Vt[e];
     }
     __syncthreads();
@@ -714,7 +763,7 @@ __device__ void post_body(ttx_slot* s) {
     if (kc > s->rmax) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
     double* xk = xcore(s, k);
     for (int64_t e = threadIdx.x; e < m * kc; e += blockDim.x) {
-      const int64_t row = e / kc, c = e % kc;
+      const int64_t row = q32(e, kc), c = e - row * kc;
       xk[e] = qb[row * kk + c];
     }
     bgemm(1, kc, rest, W, 1.0, rbuf, W, 1, 0, nb_buf, rest, 1, 0, 0.0, xcore(s, k + 1), rest, 1, 0);
@@ -739,7 +788,7 @@ __device__ void post_body(ttx_slot* s) {
   bgemm(1, zl * n, rr, zr, 1.0, zn, zr, 1, 0, envp(s, 4, 1, k), rr, 1, 0, 0.0, enr + rank * w, rr, 1, 0);
   double* carry = tmp;
   for (int64_t e = threadIdx.x; e < rl * rank; e += blockDim.x) {
-    const int64_t i = e / rank, c = e % rank;
+    const int64_t i = q32(e, rank), c = e - i * rank;
     carry[e] = U[i * K + c] * S[c];
   }
   const int64_t rows_prev = s->rx[k - 1] * s->n[k - 1];
@@ -766,7 +815,7 @@ __device__ void post_body(ttx_slot* s) {
   stamp(s, 12, t0);
 }
 
-__global__ __launch_bounds__(kEngThreads) void eng_post_kernel(ttx_slot* slots) {
+__global__ __launch_bounds__(kEngThreads, 2) void eng_post_kernel(ttx_slot* slots) {
   ttx_slot* s = slots + blockIdx.x;
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
@@ -804,8 +853,8 @@ __device__ double residual_norm_blk(ttx_slot* s, double* red) {
     const double* xc = xcore(s, i);
     const int64_t Lx = Ra * rx, Rx = Rb2 * rx2;
     for (int64_t e = threadIdx.x; e < Lx * n * Rx; e += blockDim.x) {
-      const int64_t r = e % Rx, j = (e / Rx) % n, l = e / (Rx * n);
-      const int64_t a = l / rx, m = l % rx, b = r / rx2, nn = r % rx2;
+      const int64_t e_q = q32(e, Rx), r = e - e_q * Rx, l = q32(e_q, n), j = e_q - l * n;
+      const int64_t a = q32(l, rx), m = l - a * rx, b = q32(r, rx2), nn = r - b * rx2;
       double t = 0.0;
       for (int64_t jj = 0; jj < n; ++jj) t += A[((a * n + j) * n + jj) * Rb2 + b] * xc[(m * n + jj) * rx2 + nn];
       T[(l * n + j) * R + r] = t;
@@ -814,7 +863,7 @@ __device__ double residual_norm_blk(ttx_slot* s, double* red) {
     const double* bc = s->b[i];
     const int64_t offl = L - rbl, offr = R - rbr;
     for (int64_t e = threadIdx.x; e < rbl * n * rbr; e += blockDim.x) {
-      const int64_t r = e % rbr, j = (e / rbr) % n, l = e / (rbr * n);
+      const int64_t e_q = q32(e, rbr), r = e - e_q * rbr, l = q32(e_q, n), j = e_q - l * n;
       T[((offl + l) * n + j) * R + offr + r] = sg * bc[e];
     }
     __syncthreads();
@@ -829,7 +878,7 @@ __device__ double residual_norm_blk(ttx_slot* s, double* red) {
     const int64_t kk = m < L ? m : L;
     block_qr(m, L, Wb, 1, m, nullptr, 0, 0, Rb, L, 1, qw, red);
     for (int64_t e = threadIdx.x; e < L * kk; e += blockDim.x) {
-      const int64_t j = e / kk, c = e % kk;
+      const int64_t j = q32(e, kk), c = e - j * kk;
       C[e] = Rb[c * L + j];
     }
     cdim = kk;
@@ -901,7 +950,7 @@ __device__ void check_body(ttx_slot* s) {
   if (threadIdx.x == 0) check_decide(s, sw, res);
 }
 
-__global__ __launch_bounds__(kEngThreads) void eng_check_kernel(ttx_slot* slots) {
+__global__ __launch_bounds__(kEngThreads, 2) void eng_check_kernel(ttx_slot* slots) {
   ttx_slot* s = slots + blockIdx.x;
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(512) void lu_solve_kernel(int64_t n, int64_t ld, do
                                                        int32_t* info, int64_t lds_bytes) {
   extern __shared__ double lds[];
   if (threadIdx.x == 0) info[0] = TTX_OK;
-  block_lu_solve(n, ld, W, u, lds, lds_bytes, info);
+  block_lu_solve<512>(n, ld, W, u, lds, lds_bytes, info);
 }
 
 

@@ -10,6 +10,13 @@ namespace ttx {
 
 constexpr int kWave = 64;
 
+// e / m in 32-bit arithmetic for 0 <= e < 2^31, 0 < m < 2^31: 64-bit integer
+// division expands to a long instruction sequence on the GPU, and the flat
+// index loops over per-request arrays never leave 32-bit range.
+__device__ __forceinline__ int64_t q32(int64_t e, int64_t m) {
+  return (int64_t)((uint32_t)e / (uint32_t)m);
+}
+
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 inline int launch_status() {

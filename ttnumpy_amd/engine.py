@@ -66,7 +66,7 @@ class Slot(ctypes.Structure):
                                  "red_last", "svd_noconv", "ticks", "minres_iters", "lu_solves", "pad6")]
         + [("flops", _f64 * 4)]
         + [("residuals", _f64 * MAXSWEEPS), ("max_bonds", _i64 * MAXSWEEPS)]
-        + [("clk", _i64 * 16)]
+        + [("clk", _i64 * 24)]
     )
 
 
@@ -148,7 +148,7 @@ class Engine:
         self.kernel_launches = 0
         self.minres_iters = 0
         self.lu_solves = 0
-        self.stage_ms = np.zeros(16)  # per-stage device wall clock summed over retired requests
+        self.stage_ms = np.zeros(24)  # per-stage device wall clock summed over retired requests
         self._push_all()
 
     # ------------------------------------------------------------ host <-> device slots
