@@ -31,8 +31,8 @@ N_BITS = 8  # 256 x 256 grid
 RMAX = 48   # slot rank capacity (the C1 solution reaches bond 33)
 PEAK_FP64_TFLOPS = 78.6
 BATCH_PROFILED = 512  # batch of the committed counter summary
-KERNELS = ("ttx::eng_env_kernel", "ttx::eng_solve_kernel", "ttx::eng_post_kernel",
-           "ttx::eng_check_kernel")
+KERNELS = ("ttx::eng_env_kernel", "ttx::eng_solve_kernel", "ttx::eng_chol_kernel",
+           "ttx::eng_lu_kernel", "ttx::eng_post_kernel", "ttx::eng_check_kernel")
 
 
 def build_inputs():

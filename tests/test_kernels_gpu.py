@@ -181,6 +181,46 @@ def test_lu_solve_pivots_across_blocks(n, kind):
     np.testing.assert_allclose(x, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("n", [1, 2, 15, 16, 17, 100, 129, 300, 1000])
+def test_chol_solve_spd(n):
+    """Cholesky on the lower triangle; the upper triangle holds garbage on purpose."""
+    print(f"chol n={n}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(2000 + n)
+    G = rng.standard_normal((n, n))
+    M = G @ G.T / n + 0.5 * np.eye(n)  # SPD, condition ~ 1e1..1e3
+    b = rng.standard_normal(n)
+    ld = n + 2
+    Mb = np.zeros((n, ld))
+    Mb[:, :n] = np.tril(M) + np.triu(np.full((n, n), 7.0), 1)
+    Mb[:, n] = b
+    u = D.empty(n)
+    info = torch.zeros(1, dtype=torch.int32, device=D.device())
+    D.chol_solve(_t(Mb.reshape(-1)), n, u, info, ld=ld)
+    assert int(info.item()) == 0
+    x = _h(u)
+    ref = np.linalg.solve(M, b)
+    assert np.linalg.norm(M @ x - b) <= 1e-13 * np.linalg.norm(M, 1) * np.linalg.norm(x) * n ** 0.5
+    np.testing.assert_allclose(x, ref, rtol=0, atol=1e-10 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n,where", [(50, 0), (300, 137), (700, 699)])
+def test_chol_reports_not_spd(n, where):
+    """An indefinite symmetric matrix: the pivot at `where` turns negative."""
+    D = _dev()
+    rng = np.random.default_rng(3000 + n)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.linspace(1.0, 2.0, n)
+    M = (Q * lam) @ Q.T
+    # make the leading (where+1) x (where+1) block indefinite at its last pivot
+    M[where, where] -= 10.0 * n
+    Mb = np.concatenate([M, np.ones((n, 1))], axis=1).reshape(-1)
+    u = D.empty(n)
+    info = torch.zeros(1, dtype=torch.int32, device=D.device())
+    D.chol_solve(_t(Mb), n, u, info)
+    assert int(info.item()) == 6  # TTX_ENOTSPD
+
+
 def test_lu_singular_reports():
     D = _dev()
     n = 200

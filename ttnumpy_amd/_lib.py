@@ -19,6 +19,7 @@ TTX_ENONFINITE = 2
 TTX_ENOCONV = 3
 TTX_EHIP = 4
 TTX_ESINGULAR = 5
+TTX_ENOTSPD = 6
 TTX_MINRES_STATE = 32
 
 _i64 = ctypes.c_int64
@@ -43,6 +44,7 @@ _SIGNATURES = {
                            _p, _p]),
     "ttx_lu_work_size": (_i64, [_i64]),
     "ttx_lu_solve_f64": (_i32, [_p, _i64, _p, _i64, _p, _p, _p]),
+    "ttx_chol_solve_f64": (_i32, [_p, _i64, _p, _i64, _p, _p]),
     "ttx_minres_init_f64": (_i32, [_p, _i64, _p, _p, _p, _f64, _i64, _p, _p]),
     "ttx_minres_step_f64": (_i32, [_p, _i64, _p, _p, _p]),
     "ttx_truncated_bond_f64": (_i32, [_p, _p, _i64, _f64, _i64, _p]),

@@ -696,4 +696,261 @@ __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W,
   lu_clock(clk, 4, t0);
 }
 
+// ---------------------------------------------------------------------------
+// Blocked Cholesky solve of a symmetric positive definite [M | b] (row-major,
+// row stride ld >= n + 1, only the lower triangle of M is read) by one block
+// of NT (256 or 512) threads.
+//
+// Right-looking with two levels: per 16-column sub-panel the diagonal block
+// is factored by one wave, the rows below are solved against it one row per
+// thread (rows are contiguous, no pivoting, so no block-wide reduction per
+// column), and the forward substitution of b rides along; the rest of the
+// 128-column outer panel takes rank-16 updates, the trailing lower triangle
+// one rank-128 update per outer panel through the LDS-staged MFMA tiles.
+// Then L^T u = y from the bottom in 64-row blocks.  Needs
+// 8 * 4352 + 256 bytes of lds at most (independent of n).
+// *info = TTX_ENOTSPD on a pivot <= 0 or not finite: the routine stops early
+// and W is left partly overwritten.  Ends synced.
+// ---------------------------------------------------------------------------
+
+// For i < Mr, j < Nc with r0 + i >= c0 + j (tiles wholly above the diagonal
+// are skipped; entries above it inside a crossing tile are written but never
+// read):  W[r0+i][c0+j] -= sum_{t<K} W[r0+i][a0+t] * W[c0+j][a0+t]
+template <int NT>
+__device__ __forceinline__ void chol_tile_update_t(int64_t Mr, int64_t Nc, int64_t K, double* W,
+                                                   int64_t ld, int64_t r0, int64_t c0, int64_t a0,
+                                                   double* lds) {
+  constexpr int TN = 64 * NT / 256;
+  constexpr int EA = kLuTileM * kLuK / NT;  // A slab: consecutive k per thread
+  constexpr int TPA = kLuK / EA;
+  constexpr int EB = TN * kLuK / NT;        // B slab: consecutive k of one B row per thread
+  constexpr int TPB = kLuK / EB;
+  if (Mr <= 0 || Nc <= 0 || K <= 0) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int wm = (wave & 3) * 32, wn = (wave >> 2) * 64;
+  double* As = lds;                    // [kLuK][kLuTileM]
+  double* Bs = lds + kLuK * kLuTileM;  // [kLuK][TN]
+  const int ar = tid / TPA, ak = (tid % TPA) * EA;
+  const int br = tid / TPB, bk = (tid % TPB) * EB;
+  const int64_t tm = (Mr + kLuTileM - 1) / kLuTileM, tn = (Nc + TN - 1) / TN;
+  for (int64_t t = 0; t < tm * tn; ++t) {
+    const int64_t m0 = (t / tn) * kLuTileM, n0 = (t % tn) * TN;
+    // skip tiles strictly above the diagonal: last row < first column
+    if (r0 + m0 + kLuTileM - 1 < c0 + n0) continue;
+    double* crow[2][4];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t row = m0 + wm + 16 * mt + 4 * e + lq;
+        crow[mt][e] = row < Mr ? W + (r0 + row) * ld + c0 : nullptr;
+      }
+    double4_t acc[2][4];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const int64_t col = n0 + wn + 16 * nt + l16;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          acc[mt][nt][e] = (crow[mt][e] && col < Nc) ? crow[mt][e][col] : 0.0;
+      }
+    const double* ap = (m0 + ar) < Mr ? W + (r0 + m0 + ar) * ld + a0 : nullptr;
+    const double* bp = (n0 + br) < Nc ? W + (c0 + n0 + br) * ld + a0 : nullptr;
+    double ra[EA], rb[EB];
+    auto gload = [&](int64_t k0) {
+#pragma unroll
+      for (int i = 0; i < EA; ++i) {
+        const int64_t kk = k0 + ak + i;
+        ra[i] = (ap && kk < K) ? ap[kk] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < EB; ++i) {
+        const int64_t kk = k0 + bk + i;
+        rb[i] = (bp && kk < K) ? bp[kk] : 0.0;
+      }
+    };
+    gload(0);
+    for (int64_t k0 = 0; k0 < K; k0 += kLuK) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < EA; ++i) As[(ak + i) * kLuTileM + ar] = ra[i];
+#pragma unroll
+      for (int i = 0; i < EB; ++i) Bs[(bk + i) * TN + br] = rb[i];
+      __syncthreads();
+      if (k0 + kLuK < K) gload(k0 + kLuK);
+#pragma unroll
+      for (int ks = 0; ks < kLuK; ks += 4) {
+        const int kr = ks + lq;
+        double a[2], b[4];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) a[mt] = -As[kr * kLuTileM + wm + 16 * mt + l16];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) b[nt] = Bs[kr * TN + wn + 16 * nt + l16];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt)
+            acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const int64_t col = n0 + wn + 16 * nt + l16;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (crow[mt][e] && col < Nc) crow[mt][e][col] = acc[mt][nt][e];
+      }
+  }
+  __syncthreads();
+}
+
+template <int NT>
+__device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* W, double* u,
+                                                 double* lds, int32_t* info,
+                                                 int64_t* clk = nullptr) {
+  __shared__ int s_bad;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwave = NT >> 6;
+  int64_t t0 = wall_clock64();
+  {
+    const double dn = (double)n;
+    add_flops(dn * dn * dn / 3.0 + 2.0 * dn * dn);
+  }
+  if (tid == 0) s_bad = 0;
+  double* Lk = lds;        // 16 x 17, the factored diagonal block
+  double* yk = lds + 272;  // its 16 entries of y
+  double* stage = lds + 320;
+  constexpr int64_t kOuter = 128;
+  for (int64_t K0 = 0; K0 < n; K0 += kOuter) {
+    const int64_t ko = (n - K0) < kOuter ? (n - K0) : kOuter;
+    const int64_t K1 = K0 + ko;
+    for (int64_t k = K0; k < K1; k += 16) {
+      const int64_t nb = (K1 - k) < 16 ? (K1 - k) : 16;
+      // 1. the diagonal block and y_k by wave 0 (lane = row)
+      if (wave == 0) {
+        double row[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+          row[c] = (lane < nb && c <= lane && c < nb) ? W[(k + lane) * ld + k + c] : 0.0;
+        double yv = lane < nb ? W[(k + lane) * ld + n] : 0.0;
+        bool bad = false;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          if (j < nb) {  // uniform: every shuffle below runs on all 64 lanes
+            const double djj = __shfl(row[j], j, 64);
+            bad = bad || !(djj > 0.0) || !(djj < INFINITY);
+            const double d = sqrt(djj);
+            const double lj = lane >= j ? row[j] / d : 0.0;  // column j of L, rows >= j
+            if (lane >= j) row[j] = lj;
+            const double yj = __shfl(yv, j, 64) / d;
+            if (lane == j) yv = yj;
+#pragma unroll
+            for (int c = j + 1; c < 16; ++c) {
+              const double lcj = __shfl(lj, c, 64);
+              if (lane > j && c <= lane) row[c] -= lj * lcj;
+            }
+            if (lane > j) yv -= lj * yj;
+          }
+        }
+        if (lane < nb) {
+#pragma unroll
+          for (int c = 0; c < 16; ++c)
+            if (c <= lane && c < nb) {
+              W[(k + lane) * ld + k + c] = row[c];
+              Lk[lane * 17 + c] = row[c];
+            }
+          W[(k + lane) * ld + n] = yv;
+          yk[lane] = yv;
+        }
+        if (lane == 0 && bad) s_bad = 1;
+      }
+      __syncthreads();
+      if (s_bad) {
+        if (tid == 0) info[0] = TTX_ENOTSPD;
+        return;
+      }
+      // 2. rows below: x L_kk^T = w (forward substitution along the row),
+      //    then b_i -= x . y_k
+      for (int64_t i = k + nb + tid; i < n; i += NT) {
+        double* wr = W + i * ld + k;
+        double x[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) x[c] = c < nb ? wr[c] : 0.0;
+        double bi = W[i * ld + n];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+          if (c < nb) {
+            double v = x[c];
+#pragma unroll
+            for (int t = 0; t < c; ++t) v -= x[t] * Lk[c * 17 + t];
+            v /= Lk[c * 17 + c];
+            x[c] = v;
+            bi -= v * yk[c];
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+          if (c < nb) wr[c] = x[c];
+        W[i * ld + n] = bi;
+      }
+      __syncthreads();
+      lu_clock(clk, 0, t0);
+      // 3. the rest of the outer panel (lower part), rank nb
+      chol_tile_update_t<NT>(n - (k + nb), K1 - (k + nb), nb, W, ld, k + nb, k + nb, k, stage);
+      lu_clock(clk, 1, t0);
+    }
+    // trailing lower triangle, rank ko
+    chol_tile_update_t<NT>(n - K1, n - K1, ko, W, ld, K1, K1, K0, stage);
+    lu_clock(clk, 3, t0);
+  }
+  // L^T u = y, 64-row blocks from the bottom
+  double* T = lds;  // 64 x 65: T[r][c] = L[j0 + r][j0 + c], r >= c
+  double* ub = lds + 64 * 65;
+  for (int64_t i = tid; i < n; i += NT) u[i] = W[i * ld + n];
+  __syncthreads();
+  for (int64_t j1 = n; j1 > 0; j1 -= 64) {
+    const int64_t j0 = j1 - 64 > 0 ? j1 - 64 : 0;
+    const int64_t nbk = j1 - j0;
+    if (lane < nbk) {
+      for (int64_t r = wave; r < nbk; r += nwave)
+        T[r * 65 + lane] = lane <= r ? W[(j0 + r) * ld + j0 + lane] : 0.0;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const int64_t row = j0 + lane;
+      double yv = row < j1 ? u[row] : 0.0;
+      for (int64_t j = j1 - 1; j >= j0; --j) {
+        const int jj = (int)(j - j0);
+        const double xj = __shfl(yv, jj, 64) / T[jj * 65 + jj];
+        if (row == j) yv = xj;
+        if (row < j) yv -= xj * T[jj * 65 + lane];  // L[j][row] = L^T[row][j]
+      }
+      if (row < j1) { u[row] = yv; ub[lane] = yv; }
+    }
+    __syncthreads();
+    // y_i -= sum_{c in block} L[c][i] u_c for the rows above (coalesced over i)
+    for (int64_t i = tid; i < j0; i += NT) {
+      double acc = 0.0;
+      for (int64_t c = 0; c < nbk; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = (c + q < nbk) ? W[(j0 + c + q) * ld + i] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (c + q < nbk) acc += v[q] * ub[c + q];
+      }
+      u[i] -= acc;
+    }
+    __syncthreads();
+  }
+  lu_clock(clk, 4, t0);
+}
+
 }  // namespace ttx

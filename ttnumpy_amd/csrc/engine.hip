@@ -129,11 +129,12 @@ struct LocOp {
 // Each thread owns a column (t_l, j, t_r) and a share of the (i, f_r) pairs;
 // T_a does not depend on f_l, so it is formed once per pair and reused down
 // the fl rows that share it.  Stores of a row stay coalesced across threads.
-// psi, A_k and phi are staged in LDS when they fit in lds_cap doubles.
+// psi, A_k and phi are staged in LDS when they fit in lds_cap doubles.  With
+// lower, only entries on or below the diagonal are written.
 // Matches local_matrix_entry's sums term for term (a ascending, zero psi
 // skipped, b ascending).  Ends synced.
 __device__ void local_matrix_blk(const LocOp& op, double* out, int64_t ld, double* lds,
-                                 int64_t lds_cap) {
+                                 int64_t lds_cap, bool lower) {
   constexpr int kMaxRa = 16;
   const int rows = (int)op.rows(), cols = (int)op.cols();
   add_flops(2.0 * (double)rows * (double)cols * (double)op.Ra * (double)op.Rb);
@@ -157,6 +158,7 @@ __device__ void local_matrix_blk(const LocOp& op, double* out, int64_t ld, doubl
       const int cq = it / rr, t_r = it - cq * rr, t_l = cq / nc, j = cq - t_l * nc;
       for (int row = 0; row < rows; ++row) {
         const int rq = row / fr, f_r = row - rq * fr, f_l = rq / nr, i = rq - f_l * nr;
+        if (lower && row < it) continue;
         out[(int64_t)row * ld + it] = local_matrix_entry(
             psi + (int64_t)t_l * Ra * fl, fl, f_l, A + (i * nc + j) * Rb, (int64_t)nr * nc * Rb, Ra,
             Rb, phi + (int64_t)t_r * Rb * fr, fr, f_r);
@@ -188,7 +190,13 @@ __device__ void local_matrix_blk(const LocOp& op, double* out, int64_t ld, doubl
       }
       double* o = out + (int64_t)(i * fr + f_r) * ld + c;
       const int64_t ostep = (int64_t)nr * fr * ld;
-      for (int f_l = 0; f_l < fl; ++f_l) {
+      // with lower, only rows >= c: f_l >= ceil((c - i*fr - f_r) / (nr*fr))
+      int f0 = 0;
+      if (lower) {
+        const int need = c - (i * fr + f_r);
+        f0 = need > 0 ? (need + nr * fr - 1) / (nr * fr) : 0;
+      }
+      for (int f_l = f0; f_l < fl; ++f_l) {
         double sum = 0.0;
 #pragma unroll
         for (int a = 0; a < kMaxRa; ++a) {
@@ -558,12 +566,37 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_env_kernel(ttx_slot* slots
   if (threadIdx.x == 0 && g_flops != 0.0) atomicAdd(&s->flops[0], g_flops);
 }
 
-// one-site local solve (one_site.py:118-178, common.py:729-809)
-__device__ void solve_body(ttx_slot* s, double* lds) {
+// The local solve of a sweep step (one_site.py:118-178, common.py:729-809)
+// runs as up to three kernels of one tick, each with its own register
+// budget: eng_solve_kernel forms b_loc and, for a large symmetric system,
+// runs restarted MINRES; a direct solve is handed on through s->sstage to
+// eng_chol_kernel (symmetric: Cholesky) and, when that finds the matrix not
+// positive definite or the problem is not symmetric, to eng_lu_kernel (LU
+// with partial pivoting).  scal[0] receives the exact local residual.
+enum { kSolveNone = 0, kSolveChol = 1, kSolveLu = 2 };
+
+__device__ bool solve_step_active(const ttx_slot* s) {
+  return s->status == TTX_SLOT_RUNNING && (s->phase == TTX_PH_FWD || s->phase == TTX_PH_BWD);
+}
+
+// ||A_loc u - b_loc|| into scal[0] when the residual budget needs it
+__device__ void exact_local_residual(ttx_slot* s, int64_t k, int64_t N, double* red, int64_t& t0) {
+  double* scal = scr(s, TTX_SCR_SCAL);
+  if (s->residual_mode) {
+    LocOp op = system_op(s, k);
+    double* tmp = scr(s, TTX_SCR_TMP);
+    op.apply(scr(s, TTX_SCR_U), 1, tmp, 1.0, 0.0);
+    const double ex = bnorm2(tmp, scr(s, TTX_SCR_BLOC), N, red);
+    if (threadIdx.x == 0) scal[0] = ex;
+    stamp(s, 4, t0);
+  } else if (threadIdx.x == 0) {
+    scal[0] = 0.0;
+  }
+}
+
+__device__ void solve_body(ttx_slot* s) {
   __shared__ double red[16];
-  __shared__ int32_t s_info;
-  if (s->status != TTX_SLOT_RUNNING) return;
-  if (s->phase != TTX_PH_FWD && s->phase != TTX_PH_BWD) return;
+  if (!solve_step_active(s)) return;
   const int64_t k = s->k;
   const int64_t rl = s->rx[k], n = s->n[k], rr = s->rx[k + 1];
   const int64_t N = rl * n * rr;
@@ -577,67 +610,125 @@ __device__ void solve_body(ttx_slot* s, double* lds) {
   __syncthreads();
   stamp(s, 0, t0);
   const bool iter = s->iterative && N > s->direct_size;
-  if (iter) {
-    LocOp op = system_op(s, k);
-    const double target = s->local_rtol * bnorm2(bloc, nullptr, N, red);
-    double rt = s->local_rtol;
-    double* V = scr(s, TTX_SCR_MR);
-    const double* x0 = xcore(s, k);  // warm start: the current core (one_site.py:151-153)
-    bool ok = false;
-    for (int attempt = 0; attempt < 4; ++attempt) {
-      minres_blk(op, N, bloc, x0, rt, 5 * N, V, red, threadIdx.x == 0 ? &s->minres_iters : nullptr);
-      x0 = V;
-      op.apply(V, 1, tmp, 1.0, 0.0);
-      const double res = bnorm2(tmp, bloc, N, red);
-      if (res <= target) { ok = true; break; }
-      rt = fmax(rt * 1e-2, 1e-15);
-    }
-    bcopy(u, V, N);
-    stamp(s, 3, t0);
-    if (threadIdx.x == 0) {
-      s->local_solves += 1;
-      if (!ok) s->local_inexact += 1;
-      scal[1] = 0.0;
-    }
-  } else {
+  if (!iter) {
     if (N > s->direct_cap) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
-    LocOp op = system_op(s, k);
-    double* W = scr(s, TTX_SCR_LU);
-    local_matrix_blk(op, W, N + 1, lds, kEngLds / 8);
-    for (int64_t r = threadIdx.x; r < N; r += blockDim.x) W[r * (N + 1) + N] = bloc[r];
-    if (threadIdx.x == 0) s_info = TTX_OK;
-    __syncthreads();
-    stamp(s, 1, t0);
-    if ((int64_t)kEngLds < 4 * N + 8 * (N > 4224 ? N : 4224)) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
-    block_lu_solve<kEngThreads>(N, N + 1, W, u, lds, kEngLds, &s_info, &s->clk[13]);
-    __syncthreads();
-    stamp(s, 2, t0);
-    if (s_info != TTX_OK) { fail(s, TTX_SLOT_ERR_SINGULAR); return; }
-    if (threadIdx.x == 0) {
-      scal[1] = 1.0;
-      s->lu_solves += 1;
-    }
+    if (threadIdx.x == 0) s->sstage = s->chol ? kSolveChol : kSolveLu;
+    return;
+  }
+  LocOp op = system_op(s, k);
+  const double target = s->local_rtol * bnorm2(bloc, nullptr, N, red);
+  double rt = s->local_rtol;
+  double* V = scr(s, TTX_SCR_MR);
+  const double* x0 = xcore(s, k);  // warm start: the current core (one_site.py:151-153)
+  bool ok = false;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    minres_blk(op, N, bloc, x0, rt, 5 * N, V, red, threadIdx.x == 0 ? &s->minres_iters : nullptr);
+    x0 = V;
+    op.apply(V, 1, tmp, 1.0, 0.0);
+    const double res = bnorm2(tmp, bloc, N, red);
+    if (res <= target) { ok = true; break; }
+    rt = fmax(rt * 1e-2, 1e-15);
+  }
+  bcopy(u, V, N);
+  stamp(s, 3, t0);
+  if (threadIdx.x == 0) {
+    s->local_solves += 1;
+    if (!ok) s->local_inexact += 1;
+    scal[1] = 0.0;
+    s->sstage = kSolveNone;
   }
   __syncthreads();
-  if (s->residual_mode) {
-    LocOp op = system_op(s, k);
-    op.apply(u, 1, tmp, 1.0, 0.0);
-    const double ex = bnorm2(tmp, bloc, N, red);
-    if (threadIdx.x == 0) scal[0] = ex;
-    stamp(s, 4, t0);
-  } else if (threadIdx.x == 0) {
-    scal[0] = 0.0;
+  exact_local_residual(s, k, N, red, t0);
+}
+
+__device__ void chol_body(ttx_slot* s, double* lds) {
+  __shared__ double red[16];
+  __shared__ int32_t s_info;
+  if (!solve_step_active(s) || s->sstage != kSolveChol) return;
+  const int64_t k = s->k;
+  const int64_t N = s->rx[k] * s->n[k] * s->rx[k + 1];
+  int64_t t0 = wall_clock64();
+  double* W = scr(s, TTX_SCR_LU);
+  const double* bloc = scr(s, TTX_SCR_BLOC);
+  local_matrix_blk(system_op(s, k), W, N + 1, lds, kEngLds / 8, true);
+  for (int64_t r = threadIdx.x; r < N; r += blockDim.x) W[r * (N + 1) + N] = bloc[r];
+  if (threadIdx.x == 0) s_info = TTX_OK;
+  __syncthreads();
+  stamp(s, 1, t0);
+  block_chol_solve<kEngThreads>(N, N + 1, W, scr(s, TTX_SCR_U), lds, &s_info, &s->clk[13]);
+  __syncthreads();
+  stamp(s, 2, t0);
+  if (s_info == TTX_ENOTSPD) {  // not positive definite: LU takes over this tick
+    if (threadIdx.x == 0) {
+      s->chol_fallbacks += 1;
+      s->sstage = kSolveLu;
+    }
+    return;
   }
+  if (threadIdx.x == 0) {
+    scr(s, TTX_SCR_SCAL)[1] = 1.0;
+    s->lu_solves += 1;
+    s->sstage = kSolveNone;
+  }
+  __syncthreads();
+  exact_local_residual(s, k, N, red, t0);
+}
+
+__device__ void lu_body(ttx_slot* s, double* lds) {
+  __shared__ double red[16];
+  __shared__ int32_t s_info;
+  if (!solve_step_active(s) || s->sstage != kSolveLu) return;
+  const int64_t k = s->k;
+  const int64_t N = s->rx[k] * s->n[k] * s->rx[k + 1];
+  int64_t t0 = wall_clock64();
+  double* W = scr(s, TTX_SCR_LU);
+  const double* bloc = scr(s, TTX_SCR_BLOC);
+  if ((int64_t)kEngLds < 4 * N + 8 * (N > 4224 ? N : 4224)) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
+  local_matrix_blk(system_op(s, k), W, N + 1, lds, kEngLds / 8, false);
+  for (int64_t r = threadIdx.x; r < N; r += blockDim.x) W[r * (N + 1) + N] = bloc[r];
+  if (threadIdx.x == 0) s_info = TTX_OK;
+  __syncthreads();
+  stamp(s, 1, t0);
+  block_lu_solve<kEngThreads>(N, N + 1, W, scr(s, TTX_SCR_U), lds, kEngLds, &s_info, &s->clk[13]);
+  __syncthreads();
+  stamp(s, 2, t0);
+  if (s_info != TTX_OK) { fail(s, TTX_SLOT_ERR_SINGULAR); return; }
+  if (threadIdx.x == 0) {
+    scr(s, TTX_SCR_SCAL)[1] = 1.0;
+    s->lu_solves += 1;
+    s->sstage = kSolveNone;
+  }
+  __syncthreads();
+  exact_local_residual(s, k, N, red, t0);
 }
 
 __global__ __launch_bounds__(kEngThreads, 2) void eng_solve_kernel(ttx_slot* slots) {
+  ttx_slot* s = slots + blockIdx.x;
+  if (threadIdx.x == 0) g_flops = 0.0;
+  __syncthreads();
+  solve_body(s);
+  __syncthreads();
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[1] += g_flops;
+}
+
+__global__ __launch_bounds__(kEngThreads, 2) void eng_chol_kernel(ttx_slot* slots) {
   extern __shared__ double lds[];
   ttx_slot* s = slots + blockIdx.x;
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
-  solve_body(s, lds);
+  chol_body(s, lds);
   __syncthreads();
-  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[1] += g_flops;
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[2] += g_flops;
+}
+
+__global__ __launch_bounds__(kEngThreads, 2) void eng_lu_kernel(ttx_slot* slots) {
+  extern __shared__ double lds[];
+  ttx_slot* s = slots + blockIdx.x;
+  if (threadIdx.x == 0) g_flops = 0.0;
+  __syncthreads();
+  lu_body(s, lds);
+  __syncthreads();
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[3] += g_flops;
 }
 
 // truncation, enrichment and shifts after a solve; QR sweeps and z fit
@@ -823,7 +914,7 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_post_kernel(ttx_slot* slot
   post_body(s);
   __syncthreads();
   if (s->status == TTX_SLOT_RUNNING || s->status == TTX_SLOT_DONE) stamp(s, 5, t0);
-  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[2] += g_flops;
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[4] += g_flops;
 }
 
 // ||A x - b|| / ||b|| in TT arithmetic (common.py:118-146): the residual
@@ -958,7 +1049,7 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_check_kernel(ttx_slot* slo
   check_body(s);
   __syncthreads();
   if (s->status == TTX_SLOT_RUNNING || s->status == TTX_SLOT_DONE) stamp(s, 6, t0);
-  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[3] += g_flops;
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[5] += g_flops;
 }
 
 // the stopping test and the next phase after a completed sweep (thread 0)
@@ -1084,53 +1175,64 @@ extern "C" int ttx_engine_plan(ttx_slot* s) {
   return TTX_OK;
 }
 
-extern "C" int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks) {
-  if (nslots < 1 || ticks < 0) return TTX_EINVAL;
+static int engine_prepare() {
   static bool attr = false;
   if (!attr) {
-    if (hipFuncSetAttribute((const void*)eng_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)kEngLds) != hipSuccess)
-      return TTX_EHIP;
+    const void* fns[2] = {(const void*)eng_chol_kernel, (const void*)eng_lu_kernel};
+    for (const void* f : fns)
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kEngLds) !=
+          hipSuccess)
+        return TTX_EHIP;
     attr = true;
   }
-  hipStream_t st = (hipStream_t)stream;
-  for (int64_t t = 0; t < ticks; ++t) {
-    hipLaunchKernelGGL(eng_env_kernel, dim3((unsigned)nslots, 5), dim3(kEngThreads), 0, st, slots);
-    hipLaunchKernelGGL(eng_solve_kernel, dim3((unsigned)nslots), dim3(kEngThreads), kEngLds, st, slots);
-    hipLaunchKernelGGL(eng_post_kernel, dim3((unsigned)nslots), dim3(kEngThreads), 0, st, slots);
-    hipLaunchKernelGGL(eng_check_kernel, dim3((unsigned)nslots), dim3(kEngThreads), 0, st, slots);
+  return TTX_OK;
+}
+
+// kernel i of a tick: env, solve (rhs + MINRES), Cholesky, LU, post, check
+static void launch_stage(int i, hipStream_t st, ttx_slot* slots, int64_t nslots) {
+  const dim3 g((unsigned)nslots), b(kEngThreads);
+  switch (i) {
+    case 0: hipLaunchKernelGGL(eng_env_kernel, dim3((unsigned)nslots, 5), b, 0, st, slots); break;
+    case 1: hipLaunchKernelGGL(eng_solve_kernel, g, b, 0, st, slots); break;
+    case 2: hipLaunchKernelGGL(eng_chol_kernel, g, b, kEngLds, st, slots); break;
+    case 3: hipLaunchKernelGGL(eng_lu_kernel, g, b, kEngLds, st, slots); break;
+    case 4: hipLaunchKernelGGL(eng_post_kernel, g, b, 0, st, slots); break;
+    default: hipLaunchKernelGGL(eng_check_kernel, g, b, 0, st, slots); break;
   }
+}
+
+extern "C" int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks) {
+  if (nslots < 1 || ticks < 0) return TTX_EINVAL;
+  if (engine_prepare() != TTX_OK) return TTX_EHIP;
+  hipStream_t st = (hipStream_t)stream;
+  for (int64_t t = 0; t < ticks; ++t)
+    for (int i = 0; i < TTX_ENGINE_KERNELS; ++i) launch_stage(i, st, slots, nslots);
   return launch_status();
 }
 
-// One-tick-at-a-time replay with HIP events around each of the four kernels;
-// ms_out[0..3] accumulate the elapsed time of env, solve, post and check.
 extern "C" int ttx_engine_profile(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
                                   double* ms_out) {
   if (nslots < 1 || ticks < 1) return TTX_EINVAL;
-  if (ttx_engine_tick(stream, slots, nslots, 0) == TTX_EHIP) return TTX_EHIP;
+  if (engine_prepare() != TTX_OK) return TTX_EHIP;
   hipStream_t st = (hipStream_t)stream;
-  hipEvent_t ev[5];
-  for (int i = 0; i < 5; ++i)
+  constexpr int K = TTX_ENGINE_KERNELS;
+  hipEvent_t ev[K + 1];
+  for (int i = 0; i <= K; ++i)
     if (hipEventCreate(&ev[i]) != hipSuccess) return TTX_EHIP;
   int rc = TTX_OK;
   for (int64_t t = 0; t < ticks && rc == TTX_OK; ++t) {
     hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(eng_env_kernel, dim3((unsigned)nslots, 5), dim3(kEngThreads), 0, st, slots);
-    hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL(eng_solve_kernel, dim3((unsigned)nslots), dim3(kEngThreads), kEngLds, st, slots);
-    hipEventRecord(ev[2], st);
-    hipLaunchKernelGGL(eng_post_kernel, dim3((unsigned)nslots), dim3(kEngThreads), 0, st, slots);
-    hipEventRecord(ev[3], st);
-    hipLaunchKernelGGL(eng_check_kernel, dim3((unsigned)nslots), dim3(kEngThreads), 0, st, slots);
-    hipEventRecord(ev[4], st);
-    if (hipEventSynchronize(ev[4]) != hipSuccess) { rc = TTX_EHIP; break; }
-    for (int j = 0; j < 4; ++j) {
+    for (int i = 0; i < K; ++i) {
+      launch_stage(i, st, slots, nslots);
+      hipEventRecord(ev[i + 1], st);
+    }
+    if (hipEventSynchronize(ev[K]) != hipSuccess) { rc = TTX_EHIP; break; }
+    for (int j = 0; j < K; ++j) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, ev[j], ev[j + 1]) != hipSuccess) { rc = TTX_EHIP; break; }
       ms_out[j] += (double)ms;
     }
   }
-  for (int i = 0; i < 5; ++i) hipEventDestroy(ev[i]);
+  for (int i = 0; i <= K; ++i) hipEventDestroy(ev[i]);
   return rc == TTX_OK ? launch_status() : rc;
 }

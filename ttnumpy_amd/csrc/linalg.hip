@@ -71,6 +71,14 @@ __global__ __launch_bounds__(1024) void svd_kernel(SvdArgs p) {
 // LU with partial pivoting and the solve of [M | b] (row-major) by one block:
 // the same blocked routine the solver engine runs per request.
 // ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void chol_solve_kernel(int64_t n, int64_t ld, double* W,
+                                                         double* u, int32_t* info) {
+  __shared__ double lds[4480];
+  if (threadIdx.x == 0) info[0] = TTX_OK;
+  __syncthreads();
+  block_chol_solve<512>(n, ld, W, u, lds, info);
+}
+
 __global__ __launch_bounds__(512) void lu_solve_kernel(int64_t n, int64_t ld, double* W, double* u,
                                                        int32_t* info, int64_t lds_bytes) {
   extern __shared__ double lds[];
@@ -229,6 +237,13 @@ extern "C" int ttx_lu_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, int6
   }
   hipLaunchKernelGGL(lu_solve_kernel, dim3(1), dim3(512), kLdsLimit, (hipStream_t)stream, n, ld, Mb,
                      u, info, (int64_t)kLdsLimit);
+  return launch_status();
+}
+
+extern "C" int ttx_chol_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, int64_t ld, double* u,
+                                  int32_t* info) {
+  if (n < 1 || ld < n + 1) return TTX_EINVAL;
+  hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(512), 0, (hipStream_t)stream, n, ld, Mb, u, info);
   return launch_status();
 }
 

@@ -231,6 +231,14 @@ def lu_solve(Mb: torch.Tensor, n: int, u: torch.Tensor, info: torch.Tensor, ld: 
                                  info.data_ptr()), "ttx_lu_solve_f64")
 
 
+def chol_solve(Mb: torch.Tensor, n: int, u: torch.Tensor, info: torch.Tensor,
+               ld: Optional[int] = None) -> None:
+    """Cholesky solve of the SPD row-major [M | b] in ``Mb`` (lower triangle read,
+    destroyed); ``info`` gets TTX_ENOTSPD when M is not positive definite."""
+    check(lib().ttx_chol_solve_f64(stream(), n, P(Mb), n + 1 if ld is None else ld, P(u),
+                                   info.data_ptr()), "ttx_chol_solve_f64")
+
+
 # ---------------------------------------------------------------- factorizations
 def qr(m: int, n: int, A: int, a_rs: int, a_cs: int, Q: Optional[int], q_rs: int, q_cs: int,
        R: int, r_rs: int, r_cs: int) -> None:

@@ -33,6 +33,9 @@ from .tensor_train import TensorTrain
 
 MAXD = 64
 MAXSWEEPS = 64
+NKERNELS = 6  # per tick: env, solve (rhs + MINRES), chol, lu, post, check
+KERNEL_NAMES = ("eng_env_kernel", "eng_solve_kernel", "eng_chol_kernel", "eng_lu_kernel",
+                "eng_post_kernel", "eng_check_kernel")
 SCR_COUNT = 17
 
 PH_XORTH, PH_ZORTH, PH_DONE = 1, 2, 14
@@ -51,7 +54,7 @@ class Slot(ctypes.Structure):
     _fields_ = (
         [(nm, _i64) for nm in ("d", "kick", "max_rank", "direct_size", "iterative", "residual_mode",
                                "max_sweeps", "track", "rmax", "nmax", "ramax", "rbmax", "bneg",
-                               "allow_fallback", "chunk", "pad0")]
+                               "allow_fallback", "chunk", "chol")]
         + [("n", _i64 * MAXD), ("ra", _i64 * (MAXD + 1)), ("rb", _i64 * (MAXD + 1))]
         + [(nm, _f64) for nm in ("residual_tol", "trunc_tol", "local_rtol", "damp", "error_threshold",
                                  "stagnation", "bnorm", "pad1")]
@@ -63,8 +66,9 @@ class Slot(ctypes.Structure):
         + [("rx", _i64 * (MAXD + 1)), ("rz", _i64 * (MAXD + 1))]
         + [(nm, _i64) for nm in ("phase", "k", "sweep", "status", "stop_reason", "local_solves",
                                  "local_inexact", "local_fallbacks", "rank_decisions", "rank_capped",
-                                 "red_last", "svd_noconv", "ticks", "minres_iters", "lu_solves", "pad6")]
-        + [("flops", _f64 * 4)]
+                                 "red_last", "svd_noconv", "ticks", "minres_iters", "lu_solves",
+                                 "chol_fallbacks", "sstage", "pad6")]
+        + [("flops", _f64 * NKERNELS)]
         + [("residuals", _f64 * MAXSWEEPS), ("max_bonds", _i64 * MAXSWEEPS)]
         + [("clk", _i64 * 24)]
     )
@@ -143,8 +147,8 @@ class Engine:
         self._x0_images = {}
         self._z0_images = {}
         self.profiling = False
-        self.kernel_ms = np.zeros(4)     # env, solve, post, check (profiling runs only)
-        self.kernel_flops = np.zeros(4)  # algorithmic flops of retired requests
+        self.kernel_ms = np.zeros(NKERNELS)     # per tick kernel (profiling runs only)
+        self.kernel_flops = np.zeros(NKERNELS)  # algorithmic flops of retired requests
         self.kernel_launches = 0
         self.minres_iters = 0
         self.lu_solves = 0
@@ -271,6 +275,7 @@ class Engine:
             raise NotImplementedError("the engine's iterative local solver is MINRES (symmetric=True)")
         s = self._plan(req)
         s.iterative = 1 if st.iterative_solver is not None else 0
+        s.chol = 1 if req.symmetric else 0
         s.residual_mode = 1 if st.residual_tol is not None else 0
         s.max_sweeps = st.max_sweeps
         s.track = 1 if st.track_residual else 0
@@ -371,7 +376,7 @@ class Engine:
     # ------------------------------------------------------------ loop
     def tick(self, n: int = 1) -> None:
         if self.profiling:
-            ms = (_f64 * 4)(*([0.0] * 4))
+            ms = (_f64 * NKERNELS)(*([0.0] * NKERNELS))
             rc = self.lib.ttx_engine_profile(D.stream(), self.dev_slots.data_ptr(), self.nslots, n, ms)
             _lib.check(rc, "ttx_engine_profile")
             self.kernel_ms += np.array(list(ms))

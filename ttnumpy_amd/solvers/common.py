@@ -178,6 +178,7 @@ class _SolverSettings:
     local_rtol: Optional[float]
     allow_dense_fallback: bool = False
     truncation_steps: int = 1
+    symmetric: bool = False
 
     @property
     def truncation_mode(self) -> TruncationMode:
@@ -238,7 +239,8 @@ class _SolverSettings:
         return cls(max_sweeps=max_sweeps, residual_tol=residual_tol,
                    stagnation_factor=stagnation_factor, iterative_solver=it,
                    direct_solve_size=direct_solve_size, track_residual=track, local_rtol=lr,
-                   allow_dense_fallback=allow_dense_fallback, truncation_steps=truncation_steps)
+                   allow_dense_fallback=allow_dense_fallback, truncation_steps=truncation_steps,
+                   symmetric=bool(symmetric))
 
 
 @dataclass
@@ -381,7 +383,8 @@ def minres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float,
 
 
 def solve_local_system(op, b_loc: torch.Tensor, *, iterative_solver, local_rtol, warm_start=None,
-                       explicit_fallback=None, stats: Optional[_SweepStats] = None):
+                       explicit_fallback=None, stats: Optional[_SweepStats] = None,
+                       spd_first: bool = False):
     """Restarted Krylov solve with the true-residual check (``common.py:776-809``).
 
     Returns ``(u, op)``.  The direct path is handled by the caller (LU on the
@@ -414,22 +417,36 @@ def solve_local_system(op, b_loc: torch.Tensor, *, iterative_solver, local_rtol,
         return u, op
     if stats is not None:
         stats.record_local_solve(fell_back=True)
-    u = direct_solve(op, b_loc)
+    u = direct_solve(op, b_loc, spd_first=spd_first)
     return u, op
 
 
-def direct_solve(op, b_loc: torch.Tensor) -> torch.Tensor:
-    """LU with partial pivoting on [M | b], M the explicit matrix of ``op`` (``common.py:766-774``).
+def direct_solve(op, b_loc: torch.Tensor, spd_first: bool = False) -> torch.Tensor:
+    """Dense solve of [M | b], M the explicit matrix of ``op`` (``common.py:766-774``).
 
-    The LU status is left on ``op.lu_info`` (device int32) for the caller's
-    next synchronisation.
+    LU with partial pivoting; with ``spd_first`` (symmetric problems) a
+    Cholesky factorization is tried first and LU runs only when M turns out
+    not to be positive definite.  Both give the solution scipy.linalg.solve
+    gives, up to rounding.  The factorization status is left on
+    ``op.lu_info`` (device int32) for the caller's next synchronisation.
     """
     n = op.rows
     Mb = D.empty(n * (n + 1))
-    op.explicit(Mb, n + 1)
-    D.copy_nd([n], P(b_loc), [1], P(Mb, n), [n + 1])
     u = D.empty(n)
     info = torch.zeros(1, dtype=torch.int32, device=D.device())
+
+    def assemble():
+        op.explicit(Mb, n + 1)
+        D.copy_nd([n], P(b_loc), [1], P(Mb, n), [n + 1])
+
+    assemble()
+    if spd_first:
+        D.chol_solve(Mb, n, u, info)
+        if int(info.item()) != _lib.TTX_ENOTSPD:
+            op.lu_info = info
+            return u
+        assemble()
+        info.zero_()
     D.lu_solve(Mb, n, u, info)
     op.lu_info = info
     return u

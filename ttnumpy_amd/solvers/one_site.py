@@ -49,12 +49,13 @@ def _solve_local_one_site(k: int, x: TensorTrain, A: TensorTrain, b: TensorTrain
             warm = None
         u, op = solve_local_system(op, b_loc, iterative_solver=settings.iterative_solver,
                                    local_rtol=settings.local_rtol, warm_start=warm,
-                                   explicit_fallback=settings.allow_dense_fallback, stats=stats)
+                                   explicit_fallback=settings.allow_dense_fallback, stats=stats,
+                                   spd_first=settings.symmetric)
         return u, op, b_loc
     # the explicit matrix goes straight into the LU buffer; the residual and
     # rank tests that follow apply the same operator matrix-free
     op = D.LocalOp(left_env_A[k], A.cores[k], right_env_A[k])
-    u = direct_solve(op, b_loc)
+    u = direct_solve(op, b_loc, spd_first=settings.symmetric)
     return u, op, b_loc
 
 
