@@ -66,6 +66,20 @@ def measured_traffic(kernel: str):
                                    "read_bytes": t["read_bytes"], "write_bytes": t["write_bytes"]}
 
 
+def verify_result(A, b, req):
+    """FP64 check of one returned solution: ||A x - b|| / ||b|| in TT arithmetic on
+    the GPU (the device relative_residual) and, on the CPU, by the oracle."""
+    from ttnumpy_amd.solvers.common import relative_residual
+    from oracle import tt_oracle as O
+    from ttnumpy_amd import problems as P
+    gpu = float(relative_residual(A, b, req.result))
+    d = 2 * N_BITS
+    cpu = O.relative_residual(O.Train(P.poisson_2d_cores(N_BITS)), O.Train(P.ones_cores([2] * d)),
+                              O.Train(req.result.to_numpy()))
+    return {"relative_residual_gpu_fp64": gpu, "relative_residual_cpu_oracle": float(cpu),
+            "engine_reported": req.info.residuals[-1]}
+
+
 def cpu_baseline(budget_s: float = 12.0):
     """The oracle's AMEn (FP64 NumPy/SciPy) on the same C1 inputs, one BLAS thread."""
     from threadpoolctl import threadpool_limits
@@ -168,6 +182,7 @@ def main():
                 k.split("::")[1]: round(float(v / prof.kernel_ms.sum()), 4)
                 for k, v in zip(KERNELS, prof.kernel_ms)}}
     r0 = ok[0] if ok else None
+    verify = verify_result(A, b, r0) if r0 is not None else None
     line = {
         "metric": "AMEn solves/s + p99 latency, mixed vs FP64 on same GPU at residual 1e-8",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -193,6 +208,7 @@ def main():
              "post.svd", "post.rank", "post.enrich_qr", "post.zshift", "lu.panels", "lu.in_panel",
              "lu.block_rows", "lu.trailing", "lu.backsub"), prof.stage_ms)},
         "roofline": roof,
+        "verify": verify,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()

@@ -307,17 +307,36 @@ __device__ __forceinline__ void bgemm(int64_t batch, int64_t M, int64_t N, int64
     const int64_t ma = m0 + r, mb = m0 + 16 + r, na = n0 + r, nb = n0 + 16 + r;
     const bool va = ma < M, vb = mb < M, wa = na < N, wb = nb < N;
     double4_t c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
-    for (int64_t k0 = 0; k0 < K; k0 += 4) {
-      const int64_t kk = k0 + kq;
-      const bool kin = kk < K;
-      const double a0 = (kin && va) ? Ab[ma * a_ms + kk * a_ks] : 0.0;
-      const double a1 = (kin && vb) ? Ab[mb * a_ms + kk * a_ks] : 0.0;
-      const double b0 = (kin && wa) ? Bb[kk * b_ks + na * b_ns] : 0.0;
-      const double b1 = (kin && wb) ? Bb[kk * b_ks + nb * b_ns] : 0.0;
-      c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c00, 0, 0, 0);
-      c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c01, 0, 0, 0);
-      c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c10, 0, 0, 0);
-      c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c11, 0, 0, 0);
+    // two k-steps of operands are loaded before their MFMAs (8 loads per
+    // lane in flight); the accumulation order is k ascending
+    const double* pa0 = Ab + ma * a_ms + kq * a_ks;
+    const double* pa1 = Ab + mb * a_ms + kq * a_ks;
+    const double* pb0 = Bb + kq * b_ks + na * b_ns;
+    const double* pb1 = Bb + kq * b_ks + nb * b_ns;
+    const int64_t sa = 4 * a_ks, sb = 4 * b_ks;
+    for (int64_t k0 = 0; k0 < K; k0 += 8) {
+      double a0[2], a1[2], b0[2], b1[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const bool kin = k0 + 4 * q + kq < K;
+        a0[q] = (kin && va) ? pa0[q * sa] : 0.0;
+        a1[q] = (kin && vb) ? pa1[q * sa] : 0.0;
+        b0[q] = (kin && wa) ? pb0[q * sb] : 0.0;
+        b1[q] = (kin && wb) ? pb1[q * sb] : 0.0;
+      }
+      pa0 += 2 * sa;
+      pa1 += 2 * sa;
+      pb0 += 2 * sb;
+      pb1 += 2 * sb;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (k0 + 4 * q < K) {
+          c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b0[q], c00, 0, 0, 0);
+          c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b1[q], c01, 0, 0, 0);
+          c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b0[q], c10, 0, 0, 0);
+          c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b1[q], c11, 0, 0, 0);
+        }
+      }
     }
     // D slot (lane, e) of a 16x16 block is row 4e + lane/16, column lane%16
 #pragma unroll

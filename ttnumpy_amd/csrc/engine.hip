@@ -281,26 +281,30 @@ __device__ int64_t shift_left_blk(double* core_k, double* core_km1, const double
 // yA).  x0 may alias V's x.  Returns with x in V[0..n).
 __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const double* x0, double rtol,
                            int64_t maxiter, double* V, double* red, int64_t* iters) {
+  // SciPy's MINRES recurrences.  The vector passes touch element i only from
+  // thread i mod blockDim, so w1/w2/w and r1/r2 rotate by pointer instead of
+  // by copy, and each pass loads kB elements per thread before it computes
+  // (the vectors live in global memory: batching keeps kB loads in flight).
+  constexpr int kB = 4;
   double* x = V;
   double* v = V + n;
   double* y = V + 2 * n;
   double* r1 = V + 3 * n;
   double* r2 = V + 4 * n;
   double* w = V + 5 * n;
-  double* w1 = V + 6 * n;
   double* w2 = V + 7 * n;
   double* yA = V + 8 * n;
   const double eps = DBL_EPSILON;
+  const int64_t T = blockDim.x;
   if (x0) op.apply(x0, 1, yA, 1.0, 0.0);
   double sb = 0.0, sr = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int64_t i = threadIdx.x; i < n; i += T) {
     const double xi = x0 ? x0[i] : 0.0;
     const double ri = x0 ? b[i] - yA[i] : b[i];
     x[i] = xi;
     r1[i] = ri;
     r2[i] = ri;
     w[i] = 0.0;
-    w1[i] = 0.0;
     w2[i] = 0.0;
     sb += b[i] * b[i];
     sr += ri * ri;
@@ -314,7 +318,7 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
     return;
   }
   const double beta1 = sqrt(beta1sq);
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = r1[i] / beta1;
+  for (int64_t i = threadIdx.x; i < n; i += T) v[i] = r1[i] / beta1;
   __syncthreads();
   double beta = beta1, oldb = 0.0, dbar = 0.0, epsln = 0.0, phibar = beta1, tnorm2 = 0.0;
   double gmax = 0.0, gmin = DBL_MAX, cs = -1.0, sn = 0.0;
@@ -325,21 +329,51 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
     op.apply(v, 1, yA, 1.0, 0.0);
     const double f1 = itn >= 2 ? beta / oldb : 0.0;
     double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-      double yi = yA[i];
-      if (itn >= 2) yi -= f1 * r1[i];
-      y[i] = yi;
-      s += v[i] * yi;
+    for (int64_t i0 = threadIdx.x; i0 < n; i0 += kB * T) {
+      double ya[kB], rr[kB], vv[kB];
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        const int64_t i = i0 + q * T;
+        ya[q] = i < n ? yA[i] : 0.0;
+        rr[q] = i < n ? r1[i] : 0.0;
+        vv[q] = i < n ? v[i] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        const int64_t i = i0 + q * T;
+        if (i < n) {
+          double yi = ya[q];
+          if (itn >= 2) yi -= f1 * rr[q];
+          y[i] = yi;
+          s += vv[q] * yi;
+        }
+      }
     }
     const double alfa = block_sum(s, red);
     const double f2 = alfa / beta;
     s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-      const double yi = y[i] - f2 * r2[i];
-      r1[i] = r2[i];
-      r2[i] = yi;
-      s += yi * yi;
+    // r1 <- r2, r2 <- y - f2 r2 (the new r2 goes into the old r1's storage)
+    double* nr2 = r1;
+    for (int64_t i0 = threadIdx.x; i0 < n; i0 += kB * T) {
+      double yy[kB], rr[kB];
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        const int64_t i = i0 + q * T;
+        yy[q] = i < n ? y[i] : 0.0;
+        rr[q] = i < n ? r2[i] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        const int64_t i = i0 + q * T;
+        if (i < n) {
+          const double yi = yy[q] - f2 * rr[q];
+          nr2[i] = yi;
+          s += yi * yi;
+        }
+      }
     }
+    r1 = r2;
+    r2 = nr2;
     const double nbeta = sqrt(block_sum(s, red));
     oldb = beta;
     tnorm2 += alfa * alfa + oldb * oldb + nbeta * nbeta;
@@ -358,16 +392,35 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
     phibar = sn * phibar;
     const double denom = 1.0 / gamma;
     s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-      const double ow2 = w2[i], ow = w[i];
-      const double nw = (v[i] - oldeps * ow2 - delta * ow) * denom;
-      w1[i] = ow2;
-      w2[i] = ow;
-      w[i] = nw;
-      const double xi = x[i] + phi * nw;
-      x[i] = xi;
-      s += xi * xi;
+    // w1 <- w2, w2 <- w, w <- (v - oldeps w1 - delta w2) / gamma (into the old
+    // w2's storage), x += phi w, and the next Lanczos vector v <- r2 / beta
+    double* nw_ = w2;
+    for (int64_t i0 = threadIdx.x; i0 < n; i0 += kB * T) {
+      double o2[kB], o1[kB], vv[kB], xx[kB], rr[kB];
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        const int64_t i = i0 + q * T;
+        o2[q] = i < n ? w2[i] : 0.0;
+        o1[q] = i < n ? w[i] : 0.0;
+        vv[q] = i < n ? v[i] : 0.0;
+        xx[q] = i < n ? x[i] : 0.0;
+        rr[q] = i < n ? r2[i] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        const int64_t i = i0 + q * T;
+        if (i < n) {
+          const double nw = (vv[q] - oldeps * o2[q] - delta * o1[q]) * denom;
+          nw_[i] = nw;
+          const double xi = xx[q] + phi * nw;
+          x[i] = xi;
+          s += xi * xi;
+          v[i] = rr[q] / nbeta;
+        }
+      }
     }
+    w2 = w;
+    w = nw_;
     const double ynorm = sqrt(block_sum(s, red));
     beta = nbeta;
     gmax = fmax(gmax, gamma);
@@ -388,8 +441,6 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
     }
     add_flops(18.0 * (double)n);
     if (istop != 0) break;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = r2[i] / beta;
-    __syncthreads();
   }
   __syncthreads();
   if (iters) *iters += itn;
