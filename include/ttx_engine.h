@@ -70,8 +70,9 @@ typedef struct ttx_slot {
   int64_t phase, k, sweep, status, stop_reason, local_solves, local_inexact, local_fallbacks;
   int64_t rank_decisions, rank_capped, red_last, svd_noconv, ticks, minres_iters, lu_solves,
       chol_fallbacks;
-  int64_t sstage;   // direct solve handed from the solve kernel: 1 Cholesky, 2 LU
-  int64_t pad6;
+  int64_t sstage;   // local solve in flight: 1 Cholesky, 2 LU (this tick), 3 MINRES (resumes)
+  int64_t chk_core; // residual check in flight: next core (-1: not started)
+  int64_t chk_cdim; // and the width of its carried R factor
   double flops[TTX_ENGINE_KERNELS];  // algorithmic flops banked per kernel
   double residuals[TTX_MAXSWEEPS];
   int64_t max_bonds[TTX_MAXSWEEPS];

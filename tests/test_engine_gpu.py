@@ -123,3 +123,26 @@ def test_engine_alone_equals_in_mix():
         np.testing.assert_array_equal(a, b)
     for r in mix:
         assert r.error is None and r.info.residuals[-1] <= 1e-8
+
+
+def test_serving_diffusion_alone_equals_in_stream():
+    """Parametric diffusion requests of different grid sizes served as a timed
+    stream (continuous admission, MINRES slices over ticks) give the same bits
+    as the same request served alone."""
+    from ttnumpy_amd.engine import Engine
+    from ttnumpy_amd.serving import ServingConfig, ServingItem, build_requests, serve
+    print("serving diffusion alone vs stream", flush=True)
+    cfg = ServingConfig(n_requests=6, levels=(4, 5, 6), rate=200.0, nslots=3, rmax=40, poll=4,
+                        seed=11)
+    rep, done = serve(cfg)
+    print(rep, flush=True)
+    assert rep.errors == 0 and rep.solves == 6
+    target = done[0]
+    it = target.tag
+    (alone,) = build_requests(cfg, [ServingItem(it.index, it.level, it.theta, 0.0)])
+    Engine(1, rmax=40).run([alone], poll=4)
+    assert alone.error is None
+    assert alone.info.residuals == target.info.residuals
+    assert alone.result.bonds == target.result.bonds
+    for a, b in zip(alone.result.to_numpy(), target.result.to_numpy()):
+        np.testing.assert_array_equal(a, b)

@@ -18,13 +18,200 @@ __device__ __forceinline__ void add_flops(double f) {
 }
 
 // ---------------------------------------------------------------------------
+// Batched strided GEMM by one block: C[b] = alpha A[b] B[b] + beta C[b].
+// Each wave owns 32x32 output tiles (2x2 MFMA 16x16x4 f64 blocks) and reads
+// its operands straight from memory with the given strides.  No barrier
+// inside: callers separate dependent calls with __syncthreads().
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bgemm(int64_t batch, int64_t M, int64_t N, int64_t K, double alpha,
+                                      const double* A, int64_t a_ms, int64_t a_ks, int64_t a_bs,
+                                      const double* B, int64_t b_ks, int64_t b_ns, int64_t b_bs,
+                                      double beta, double* C, int64_t c_ms, int64_t c_ns,
+                                      int64_t c_bs, bool count_flops = true) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  const int r = lane & 15, kq = lane >> 4;
+  const int64_t tm = (M + 31) >> 5, tn = (N + 31) >> 5;
+  const int64_t per = tm * tn;
+  if (count_flops) add_flops(2.0 * (double)batch * (double)M * (double)N * (double)K);
+  for (int64_t t = wave; t < batch * per; t += nw) {
+    const int64_t bt = t / per, tt = t % per;
+    const int64_t m0 = (tt / tn) << 5, n0 = (tt % tn) << 5;
+    const double* Ab = A + bt * a_bs;
+    const double* Bb = B + bt * b_bs;
+    double* Cb = C + bt * c_bs;
+    const int64_t ma = m0 + r, mb = m0 + 16 + r, na = n0 + r, nb = n0 + 16 + r;
+    const bool va = ma < M, vb = mb < M, wa = na < N, wb = nb < N;
+    double4_t c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
+    // two k-steps of operands are loaded before their MFMAs (8 loads per
+    // lane in flight); the accumulation order is k ascending
+    const double* pa0 = Ab + ma * a_ms + kq * a_ks;
+    const double* pa1 = Ab + mb * a_ms + kq * a_ks;
+    const double* pb0 = Bb + kq * b_ks + na * b_ns;
+    const double* pb1 = Bb + kq * b_ks + nb * b_ns;
+    const int64_t sa = 4 * a_ks, sb = 4 * b_ks;
+    for (int64_t k0 = 0; k0 < K; k0 += 8) {
+      double a0[2], a1[2], b0[2], b1[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const bool kin = k0 + 4 * q + kq < K;
+        a0[q] = (kin && va) ? pa0[q * sa] : 0.0;
+        a1[q] = (kin && vb) ? pa1[q * sa] : 0.0;
+        b0[q] = (kin && wa) ? pb0[q * sb] : 0.0;
+        b1[q] = (kin && wb) ? pb1[q * sb] : 0.0;
+      }
+      pa0 += 2 * sa;
+      pa1 += 2 * sa;
+      pb0 += 2 * sb;
+      pb1 += 2 * sb;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (k0 + 4 * q < K) {
+          c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b0[q], c00, 0, 0, 0);
+          c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b1[q], c01, 0, 0, 0);
+          c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b0[q], c10, 0, 0, 0);
+          c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b1[q], c11, 0, 0, 0);
+        }
+      }
+    }
+    // D slot (lane, e) of a 16x16 block is row 4e + lane/16, column lane%16
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t row0 = m0 + 4 * e + kq, row1 = row0 + 16;
+      if (row0 < M && wa) {
+        double* d = Cb + row0 * c_ms + na * c_ns;
+        *d = beta == 0.0 ? alpha * c00[e] : alpha * c00[e] + beta * (*d);
+      }
+      if (row0 < M && wb) {
+        double* d = Cb + row0 * c_ms + nb * c_ns;
+        *d = beta == 0.0 ? alpha * c01[e] : alpha * c01[e] + beta * (*d);
+      }
+      if (row1 < M && wa) {
+        double* d = Cb + row1 * c_ms + na * c_ns;
+        *d = beta == 0.0 ? alpha * c10[e] : alpha * c10[e] + beta * (*d);
+      }
+      if (row1 < M && wb) {
+        double* d = Cb + row1 * c_ms + nb * c_ns;
+        *d = beta == 0.0 ? alpha * c11[e] : alpha * c11[e] + beta * (*d);
+      }
+    }
+  }
+}
+
+// y = alpha * M x + beta * y for a row-major M (row stride ld); one wave per
+// row so the reads of a row coalesce.  No barrier inside.
+__device__ __forceinline__ void bgemv(int64_t rows, int64_t cols, double alpha, const double* M,
+                                      int64_t ld, const double* x, double beta, double* y) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  add_flops(2.0 * (double)rows * (double)cols);
+  for (int64_t r = wave; r < rows; r += nw) {
+    const double* mr = M + r * ld;
+    double s = 0.0;
+    for (int64_t c = lane; c < cols; c += 64) s += mr[c] * x[c];
+    s = wave_sum(s);
+    if (lane == 0) y[r] = beta == 0.0 ? alpha * s : alpha * s + beta * y[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Householder QR on a column-major m x n matrix W (leading dim m) held in
 // global memory.  On return W holds R in its upper triangle and the reflector
 // vectors below the diagonal; tau[j] the reflector scales.  Q (m x k,
 // column-major, leading dim m) is formed in Qw.  Block-cooperative.
 // ---------------------------------------------------------------------------
+// Blocked form (compact WY, LAPACK's geqrf/larft/larfb scheme restated for
+// one block): nb-column panels are factored in LDS, T (nb x nb, upper) is
+// built from V^T V, and the trailing columns take C -= V (T^T (V^T C)) as two
+// bgemm calls per 64-column chunk instead of one pass per column; Q is formed
+// backwards the same way.  W keeps R above and the reflectors below the
+// diagonal as in the unblocked form.  lds: lds_cap doubles; the panel must fit
+// (m nb + 16 * 64 + 272 doubles with nb >= 4), else the unblocked form runs.
+constexpr int64_t kQrChunk = 64;
+// The blocked form runs for k >= kQrBlockMin with the panel width chosen
+// against kQrPanelCap doubles whatever LDS the caller has beyond that, so
+// that a given shape takes the same arithmetic in every kernel that calls it.
+constexpr int64_t kQrBlockMin = 32;
+constexpr int64_t kQrPanelCap = 9600;
+
+__device__ __forceinline__ void qr_wy_block(int64_t m, int64_t j0, int64_t jb, const double* Vp,
+                                            int64_t mp, double* T, const double* tau, double* red) {
+  // G = V^T V (jb x jb) by waves, then T by thread 0 (dlarft, forward, columnwise)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  double* G = T + 16 * 16;  // 16 x 16
+  for (int64_t p = wave; p < jb * jb; p += nwave) {
+    const int64_t i = p / jb, c = p - i * jb;
+    double d = 0.0;
+    if (i < c) {
+      for (int64_t r = lane; r < mp; r += 64) d += Vp[i * mp + r] * Vp[c * mp + r];
+      d = wave_sum(d);
+    }
+    if (lane == 0) G[i * 16 + c] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int64_t c = 0; c < jb; ++c) {
+      const double tc = tau[j0 + c];
+      T[c * 16 + c] = tc;
+      for (int64_t i = 0; i < c; ++i) {
+        double acc = 0.0;
+        for (int64_t l = i; l < c; ++l) acc += T[i * 16 + l] * G[l * 16 + c];
+        T[i * 16 + c] = -tc * acc;
+      }
+      for (int64_t i = c + 1; i < 16; ++i) T[i * 16 + c] = 0.0;
+    }
+  }
+  __syncthreads();
+  (void)m;
+  (void)red;
+}
+
+// C (mp x cols, column stride ldc) <- (I - V op(T) V^T) C, op(T) = T^T when trans
+__device__ __forceinline__ void qr_wy_apply(int64_t mp, int64_t jb, const double* Vp, const double* T,
+                                            double* C, int64_t ldc, int64_t cols, bool trans,
+                                            double* Y) {
+  for (int64_t c0 = 0; c0 < cols; c0 += kQrChunk) {
+    const int64_t cw = (cols - c0) < kQrChunk ? (cols - c0) : kQrChunk;
+    double* Cc = C + c0 * ldc;
+    // Y (jb x cw, row-major) = V^T C
+    bgemm(1, jb, cw, mp, 1.0, Vp, mp, 1, 0, Cc, 1, ldc, 0, 0.0, Y, cw, 1, 0, false);
+    __syncthreads();
+    // Y <- op(T) Y: each thread forms its (at most 8) outputs, then all write
+    {
+      double o[8];
+      const int64_t tot = jb * cw;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int64_t e = threadIdx.x + (int64_t)q * blockDim.x;
+        double acc = 0.0;
+        if (e < tot) {
+          const int64_t i = e / cw, c = e - i * cw;
+          for (int64_t t = 0; t < jb; ++t) {
+            const double tv = trans ? T[t * 16 + i] : T[i * 16 + t];  // T upper
+            acc += tv * Y[t * cw + c];
+          }
+        }
+        o[q] = acc;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int64_t e = threadIdx.x + (int64_t)q * blockDim.x;
+        if (e < tot) Y[e] = o[q];
+      }
+    }
+    __syncthreads();
+    // C -= V Y
+    bgemm(1, mp, cw, jb, -1.0, Vp, 1, mp, 0, Y, cw, 1, 0, 1.0, Cc, 1, ldc, 0, false);
+    __syncthreads();
+  }
+}
+
 __device__ __forceinline__ void block_householder_qr(int64_t m, int64_t n, double* W, double* tau, double* Qw,
-                                     double* red, bool form_q = true) {
+                                     double* red, bool form_q = true, double* lds = nullptr,
+                                     int64_t lds_cap = 0) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -35,6 +222,93 @@ __device__ __forceinline__ void block_householder_qr(int64_t m, int64_t n, doubl
     const double dm = (double)m, dn = (double)n, dk = (double)k;
     add_flops(2.0 * dm * dn * dk - (dm + dn) * dk * dk + 2.0 * dk * dk * dk / 3.0);
     if (form_q) add_flops(4.0 * dm * dk * dk - 4.0 * dk * dk * dk / 3.0);
+  }
+  const int64_t cap = lds_cap < kQrPanelCap ? lds_cap : kQrPanelCap;
+  int64_t nbq = 16;
+  while (nbq >= 4 && m * nbq + 16 * kQrChunk + 2 * 256 > cap) nbq /= 2;
+  if (lds != nullptr && nbq >= 4 && k >= kQrBlockMin) {
+    double* Vp = lds;                     // m x nb, column-major (leading dim mp)
+    double* T = lds + m * nbq;            // 16 x 16, then G 16 x 16
+    double* Y = T + 2 * 256;              // nb x kQrChunk
+    for (int64_t j0 = 0; j0 < k; j0 += nbq) {
+      const int64_t jb = (k - j0) < nbq ? (k - j0) : nbq;
+      const int64_t mp = m - j0;
+      for (int64_t e = tid; e < jb * mp; e += blockDim.x) {
+        const int64_t c = q32(e, mp), r = e - c * mp;
+        Vp[c * mp + r] = W[(j0 + c) * m + j0 + r];
+      }
+      __syncthreads();
+      // unblocked factorization of the panel in LDS
+      for (int64_t c = 0; c < jb; ++c) {
+        double* col = Vp + c * mp;
+        double sq = 0.0;
+        for (int64_t i = c + 1 + tid; i < mp; i += blockDim.x) sq += col[i] * col[i];
+        sq = block_sum(sq, red);
+        const double alpha = col[c];
+        __syncthreads();
+        if (sq == 0.0) {
+          if (tid == 0) { s_tau = 0.0; tau[j0 + c] = 0.0; }
+        } else {
+          const double beta = -copysign(sqrt(alpha * alpha + sq), alpha);
+          const double scal = 1.0 / (alpha - beta);
+          for (int64_t i = c + 1 + tid; i < mp; i += blockDim.x) col[i] *= scal;
+          if (tid == 0) {
+            s_tau = (beta - alpha) / beta;
+            tau[j0 + c] = s_tau;
+            col[c] = beta;
+          }
+        }
+        __syncthreads();
+        const double t = s_tau;
+        if (t != 0.0) {
+          for (int64_t cc = c + 1 + wave; cc < jb; cc += nwave) {
+            double* pc = Vp + cc * mp;
+            double d = 0.0;
+            for (int64_t i = c + 1 + lane; i < mp; i += 64) d += col[i] * pc[i];
+            d = wave_sum(d) + pc[c];
+            d *= t;
+            if (lane == 0) pc[c] -= d;
+            for (int64_t i = c + 1 + lane; i < mp; i += 64) pc[i] -= d * col[i];
+          }
+        }
+        __syncthreads();
+      }
+      // R and the reflectors back to W; V made explicit (unit diagonal, zeros above)
+      for (int64_t e = tid; e < jb * mp; e += blockDim.x) {
+        const int64_t c = q32(e, mp), r = e - c * mp;
+        W[(j0 + c) * m + j0 + r] = Vp[c * mp + r];
+      }
+      __syncthreads();
+      for (int64_t e = tid; e < jb * mp; e += blockDim.x) {
+        const int64_t c = q32(e, mp), r = e - c * mp;
+        if (r <= c) Vp[c * mp + r] = r == c ? 1.0 : 0.0;
+      }
+      __syncthreads();
+      if (j0 + jb < n) {
+        qr_wy_block(m, j0, jb, Vp, mp, T, tau, red);
+        qr_wy_apply(mp, jb, Vp, T, W + (j0 + jb) * m + j0, m, n - j0 - jb, true, Y);
+      }
+    }
+    if (!form_q) return;
+    // Q = H_0 ... H_{k-1} [I_k; 0]: block reflectors applied from the last panel back
+    for (int64_t e = tid; e < m * k; e += blockDim.x) {
+      const int64_t c = q32(e, m), i = e - c * m;
+      Qw[e] = (i == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int64_t last = ((k - 1) / nbq) * nbq;
+    for (int64_t j0 = last; j0 >= 0; j0 -= nbq) {
+      const int64_t jb = (k - j0) < nbq ? (k - j0) : nbq;
+      const int64_t mp = m - j0;
+      for (int64_t e = tid; e < jb * mp; e += blockDim.x) {
+        const int64_t c = q32(e, mp), r = e - c * mp;
+        Vp[c * mp + r] = r < c ? 0.0 : (r == c ? 1.0 : W[(j0 + c) * m + j0 + r]);
+      }
+      __syncthreads();
+      qr_wy_block(m, j0, jb, Vp, mp, T, tau, red);
+      qr_wy_apply(mp, jb, Vp, T, Qw + j0 * m + j0, m, k - j0, false, Y);
+    }
+    return;
   }
   for (int64_t j = 0; j < k; ++j) {
     double* col = W + j * m;
@@ -117,7 +391,8 @@ __device__ __forceinline__ int64_t rr_player(int64_t t, int64_t round, int64_t q
 // one-sided Jacobi on the square factor (round-robin pairs, one wave per
 // pair), columns sorted by descending singular value.  `work` holds
 // 2*P*Q + 2*Q*Q + 3*Q doubles (P >= Q the long and short sides); it may be LDS.
-__device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double* red) {
+__device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double* red,
+                                          double* lds = nullptr, int64_t lds_cap = 0) {
   __shared__ int s_rot;
   __shared__ int s_bad;
   const int tid = threadIdx.x;
@@ -155,7 +430,7 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   }
   const bool use_qr = P > Q;
   if (use_qr) {
-    block_householder_qr(P, Q, Wq, tau, Qx, red);
+    block_householder_qr(P, Q, Wq, tau, Qx, red, true, lds, lds_cap);
     for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
       const int64_t c = q32(e, Q), i = e - c * Q;
       G[e] = (i <= c) ? Wq[i + c * P] : 0.0;
@@ -278,105 +553,6 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
       p.U[r * p.u_rs + o * p.u_cs] = v;
   }
   for (int64_t o = tid; o < Q; o += blockDim.x) p.S[o] = sig[(int64_t)ordd[o]];
-}
-
-// ---------------------------------------------------------------------------
-// Batched strided GEMM by one block: C[b] = alpha A[b] B[b] + beta C[b].
-// Each wave owns 32x32 output tiles (2x2 MFMA 16x16x4 f64 blocks) and reads
-// its operands straight from memory with the given strides.  No barrier
-// inside: callers separate dependent calls with __syncthreads().
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void bgemm(int64_t batch, int64_t M, int64_t N, int64_t K, double alpha,
-                                      const double* A, int64_t a_ms, int64_t a_ks, int64_t a_bs,
-                                      const double* B, int64_t b_ks, int64_t b_ns, int64_t b_bs,
-                                      double beta, double* C, int64_t c_ms, int64_t c_ns,
-                                      int64_t c_bs) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nw = blockDim.x >> 6;
-  const int r = lane & 15, kq = lane >> 4;
-  const int64_t tm = (M + 31) >> 5, tn = (N + 31) >> 5;
-  const int64_t per = tm * tn;
-  add_flops(2.0 * (double)batch * (double)M * (double)N * (double)K);
-  for (int64_t t = wave; t < batch * per; t += nw) {
-    const int64_t bt = t / per, tt = t % per;
-    const int64_t m0 = (tt / tn) << 5, n0 = (tt % tn) << 5;
-    const double* Ab = A + bt * a_bs;
-    const double* Bb = B + bt * b_bs;
-    double* Cb = C + bt * c_bs;
-    const int64_t ma = m0 + r, mb = m0 + 16 + r, na = n0 + r, nb = n0 + 16 + r;
-    const bool va = ma < M, vb = mb < M, wa = na < N, wb = nb < N;
-    double4_t c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
-    // two k-steps of operands are loaded before their MFMAs (8 loads per
-    // lane in flight); the accumulation order is k ascending
-    const double* pa0 = Ab + ma * a_ms + kq * a_ks;
-    const double* pa1 = Ab + mb * a_ms + kq * a_ks;
-    const double* pb0 = Bb + kq * b_ks + na * b_ns;
-    const double* pb1 = Bb + kq * b_ks + nb * b_ns;
-    const int64_t sa = 4 * a_ks, sb = 4 * b_ks;
-    for (int64_t k0 = 0; k0 < K; k0 += 8) {
-      double a0[2], a1[2], b0[2], b1[2];
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const bool kin = k0 + 4 * q + kq < K;
-        a0[q] = (kin && va) ? pa0[q * sa] : 0.0;
-        a1[q] = (kin && vb) ? pa1[q * sa] : 0.0;
-        b0[q] = (kin && wa) ? pb0[q * sb] : 0.0;
-        b1[q] = (kin && wb) ? pb1[q * sb] : 0.0;
-      }
-      pa0 += 2 * sa;
-      pa1 += 2 * sa;
-      pb0 += 2 * sb;
-      pb1 += 2 * sb;
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        if (k0 + 4 * q < K) {
-          c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b0[q], c00, 0, 0, 0);
-          c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b1[q], c01, 0, 0, 0);
-          c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b0[q], c10, 0, 0, 0);
-          c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b1[q], c11, 0, 0, 0);
-        }
-      }
-    }
-    // D slot (lane, e) of a 16x16 block is row 4e + lane/16, column lane%16
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int64_t row0 = m0 + 4 * e + kq, row1 = row0 + 16;
-      if (row0 < M && wa) {
-        double* d = Cb + row0 * c_ms + na * c_ns;
-        *d = beta == 0.0 ? alpha * c00[e] : alpha * c00[e] + beta * (*d);
-      }
-      if (row0 < M && wb) {
-        double* d = Cb + row0 * c_ms + nb * c_ns;
-        *d = beta == 0.0 ? alpha * c01[e] : alpha * c01[e] + beta * (*d);
-      }
-      if (row1 < M && wa) {
-        double* d = Cb + row1 * c_ms + na * c_ns;
-        *d = beta == 0.0 ? alpha * c10[e] : alpha * c10[e] + beta * (*d);
-      }
-      if (row1 < M && wb) {
-        double* d = Cb + row1 * c_ms + nb * c_ns;
-        *d = beta == 0.0 ? alpha * c11[e] : alpha * c11[e] + beta * (*d);
-      }
-    }
-  }
-}
-
-// y = alpha * M x + beta * y for a row-major M (row stride ld); one wave per
-// row so the reads of a row coalesce.  No barrier inside.
-__device__ __forceinline__ void bgemv(int64_t rows, int64_t cols, double alpha, const double* M,
-                                      int64_t ld, const double* x, double beta, double* y) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nw = blockDim.x >> 6;
-  add_flops(2.0 * (double)rows * (double)cols);
-  for (int64_t r = wave; r < rows; r += nw) {
-    const double* mr = M + r * ld;
-    double s = 0.0;
-    for (int64_t c = lane; c < cols; c += 64) s += mr[c] * x[c];
-    s = wave_sum(s);
-    if (lane == 0) y[r] = beta == 0.0 ? alpha * s : alpha * s + beta * y[r];
-  }
 }
 
 // ---------------------------------------------------------------------------

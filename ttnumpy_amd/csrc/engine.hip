@@ -32,6 +32,13 @@ namespace ttx {
 constexpr int kEngThreads = 256;
 constexpr size_t kEngLds = 75 * 1024;
 
+// local solve in flight (slot.sstage): a direct solve is handed from the
+// solve kernel to the Cholesky or LU kernel of the same tick; MINRES resumes
+// over ticks, kMinresSlice iterations at a time, so that a long solve of one
+// request does not hold up the tick of every other request
+enum { kSolveNone = 0, kSolveChol = 1, kSolveLu = 2, kSolveMinres = 3 };
+constexpr int64_t kMinresSlice = 128;  // MINRES iterations per tick for one slot
+
 __device__ __forceinline__ double* xcore(const ttx_slot* s, int64_t k) { return s->x + k * s->cap_core; }
 __device__ __forceinline__ double* zcore(const ttx_slot* s, int64_t k) { return s->z + k * s->cap_zcore; }
 __device__ __forceinline__ double* envp(const ttx_slot* s, int kind, int side, int64_t k) {
@@ -216,6 +223,10 @@ __device__ void local_matrix_blk(const LocOp& op, double* out, int64_t ld, doubl
 // Economic QR of the strided m x n matrix A; Q (m x k) and R (k x n) written
 // with their strides (Q may be null).  Input is copied first, so outputs may
 // overlap A.  work: m*n + k + m*k doubles.  Ends synced.
+// the dynamic LDS of the kernel that runs the step (post and check kernels are
+// launched with kEngLds bytes of it); the QRs use it for their panels
+extern __shared__ double eng_lds[];
+
 __device__ void block_qr(int64_t m, int64_t n, const double* A, int64_t a_rs, int64_t a_cs, double* Q,
                          int64_t q_rs, int64_t q_cs, double* R, int64_t r_rs, int64_t r_cs,
                          double* work, double* red) {
@@ -228,7 +239,7 @@ __device__ void block_qr(int64_t m, int64_t n, const double* A, int64_t a_rs, in
     W[e] = A[i * a_rs + j * a_cs];
   }
   __syncthreads();
-  block_householder_qr(m, n, W, tau, Qw, red, Q != nullptr);
+  block_householder_qr(m, n, W, tau, Qw, red, Q != nullptr, eng_lds, (int64_t)(kEngLds / 8));
   __syncthreads();
   if (Q) {
     for (int64_t e = threadIdx.x; e < m * k; e += blockDim.x) {
@@ -279,54 +290,98 @@ __device__ int64_t shift_left_blk(double* core_k, double* core_km1, const double
 // scipy.sparse.linalg.minres (no preconditioner, shift 0) by one block, the
 // recurrences of krylov.hip; V holds 9 n doubles (x, v, y, r1, r2, w, w1, w2,
 // yA).  x0 may alias V's x.  Returns with x in V[0..n).
-__device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const double* x0, double rtol,
-                           int64_t maxiter, double* V, double* red, int64_t* iters) {
-  // SciPy's MINRES recurrences.  The vector passes touch element i only from
-  // thread i mod blockDim, so w1/w2/w and r1/r2 rotate by pointer instead of
-  // by copy, and each pass loads kB elements per thread before it computes
-  // (the vectors live in global memory: batching keeps kB loads in flight).
-  constexpr int kB = 4;
-  double* x = V;
-  double* v = V + n;
-  double* y = V + 2 * n;
-  double* r1 = V + 3 * n;
-  double* r2 = V + 4 * n;
-  double* w = V + 5 * n;
-  double* w2 = V + 7 * n;
-  double* yA = V + 8 * n;
+// Resumable MINRES with SciPy's recurrences.  The iteration state lives in
+// the slot: vectors in V (x, v, y, r1/r2, w/w2, A v at fixed offsets; r and w
+// rotate by parity bits instead of copies) and the scalars in st[], so an
+// attempt can span several ticks.  Element i of every vector pass is touched
+// only by thread i mod blockDim; each pass loads kB elements per thread before
+// it computes.
+enum {
+  kMrBeta = 0, kMrOldb, kMrDbar, kMrEpsln, kMrPhibar, kMrTnorm2, kMrGmax, kMrGmin, kMrCs, kMrSn,
+  kMrBeta1, kMrItn, kMrIstop, kMrPr, kMrPw, kMrRtol, kMrMaxit, kMrAttempt, kMrTarget, kMrCount
+};
+
+struct MrVec {
+  double *x, *v, *y, *r1, *r2, *w, *w2, *yA;
+  __device__ MrVec(double* V, int64_t n, int pr, int pw)
+      : x(V), v(V + n), y(V + 2 * n), r1(V + (3 + pr) * n), r2(V + (4 - pr) * n),
+        w(V + (5 + 2 * pw) * n), w2(V + (7 - 2 * pw) * n), yA(V + 8 * n) {}
+};
+
+// begin an attempt from x0 (x0 may alias V's x); st[kMrRtol], st[kMrMaxit] set
+__device__ void minres_start(const LocOp& op, int64_t n, const double* b, const double* x0, double* V,
+                             double* red, double* st) {
+  MrVec m(V, n, 0, 0);
   const double eps = DBL_EPSILON;
-  const int64_t T = blockDim.x;
-  if (x0) op.apply(x0, 1, yA, 1.0, 0.0);
+  if (x0) op.apply(x0, 1, m.yA, 1.0, 0.0);
   double sb = 0.0, sr = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += T) {
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
     const double xi = x0 ? x0[i] : 0.0;
-    const double ri = x0 ? b[i] - yA[i] : b[i];
-    x[i] = xi;
-    r1[i] = ri;
-    r2[i] = ri;
-    w[i] = 0.0;
-    w2[i] = 0.0;
+    const double ri = x0 ? b[i] - m.yA[i] : b[i];
+    m.x[i] = xi;
+    m.r1[i] = ri;
+    m.r2[i] = ri;
+    m.w[i] = 0.0;
+    m.w2[i] = 0.0;
     sb += b[i] * b[i];
     sr += ri * ri;
   }
   const double beta1sq = block_sum(sr, red);
   const double bn = sqrt(block_sum(sb, red));
-  if (beta1sq == 0.0) return;
-  if (bn == 0.0) {
-    bcopy(x, b, n);
-    __syncthreads();
-    return;
+  double istop = 0.0;
+  if (beta1sq == 0.0) {
+    istop = 100.0;  // x0 already solves it
+  } else if (bn == 0.0) {
+    bcopy(m.x, b, n);
+    istop = 100.0;
+  } else {
+    const double beta1 = sqrt(beta1sq);
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m.v[i] = m.r1[i] / beta1;
+    if (threadIdx.x == 0) {
+      st[kMrBeta] = beta1;
+      st[kMrBeta1] = beta1;
+      st[kMrPhibar] = beta1;
+    }
   }
-  const double beta1 = sqrt(beta1sq);
-  for (int64_t i = threadIdx.x; i < n; i += T) v[i] = r1[i] / beta1;
+  if (threadIdx.x == 0) {
+    st[kMrOldb] = 0.0;
+    st[kMrDbar] = 0.0;
+    st[kMrEpsln] = 0.0;
+    st[kMrTnorm2] = 0.0;
+    st[kMrGmax] = 0.0;
+    st[kMrGmin] = DBL_MAX;
+    st[kMrCs] = -1.0;
+    st[kMrSn] = 0.0;
+    st[kMrItn] = 0.0;
+    st[kMrIstop] = istop;
+    st[kMrPr] = 0.0;
+    st[kMrPw] = 0.0;
+  }
+  (void)eps;
   __syncthreads();
-  double beta = beta1, oldb = 0.0, dbar = 0.0, epsln = 0.0, phibar = beta1, tnorm2 = 0.0;
-  double gmax = 0.0, gmin = DBL_MAX, cs = -1.0, sn = 0.0;
-  int istop = 0;
-  int64_t itn = 0;
-  while (itn < maxiter) {
+}
+
+// up to `budget` iterations of the current attempt; returns the number done
+__device__ int64_t minres_run(const LocOp& op, int64_t n, double* V, double* red, double* st,
+                              int64_t budget, int64_t* iters) {
+  constexpr int kB = 4;
+  const double eps = DBL_EPSILON;
+  const int64_t T = blockDim.x;
+  double beta = st[kMrBeta], oldb = st[kMrOldb], dbar = st[kMrDbar], epsln = st[kMrEpsln];
+  double phibar = st[kMrPhibar], tnorm2 = st[kMrTnorm2], gmax = st[kMrGmax], gmin = st[kMrGmin];
+  double cs = st[kMrCs], sn = st[kMrSn];
+  const double beta1 = st[kMrBeta1], rtol = st[kMrRtol];
+  const int64_t maxiter = (int64_t)st[kMrMaxit];
+  int64_t itn = (int64_t)st[kMrItn];
+  int istop = (int)st[kMrIstop];
+  int pr = (int)st[kMrPr], pw = (int)st[kMrPw];
+  __syncthreads();  // every thread has read st before thread 0 rewrites it
+  MrVec m(V, n, pr, pw);
+  int64_t done = 0;
+  while (istop == 0 && itn < maxiter && done < budget) {
     ++itn;
-    op.apply(v, 1, yA, 1.0, 0.0);
+    ++done;
+    op.apply(m.v, 1, m.yA, 1.0, 0.0);
     const double f1 = itn >= 2 ? beta / oldb : 0.0;
     double s = 0.0;
     for (int64_t i0 = threadIdx.x; i0 < n; i0 += kB * T) {
@@ -334,9 +389,9 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
 #pragma unroll
       for (int q = 0; q < kB; ++q) {
         const int64_t i = i0 + q * T;
-        ya[q] = i < n ? yA[i] : 0.0;
-        rr[q] = i < n ? r1[i] : 0.0;
-        vv[q] = i < n ? v[i] : 0.0;
+        ya[q] = i < n ? m.yA[i] : 0.0;
+        rr[q] = i < n ? m.r1[i] : 0.0;
+        vv[q] = i < n ? m.v[i] : 0.0;
       }
 #pragma unroll
       for (int q = 0; q < kB; ++q) {
@@ -344,7 +399,7 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
         if (i < n) {
           double yi = ya[q];
           if (itn >= 2) yi -= f1 * rr[q];
-          y[i] = yi;
+          m.y[i] = yi;
           s += vv[q] * yi;
         }
       }
@@ -353,14 +408,14 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
     const double f2 = alfa / beta;
     s = 0.0;
     // r1 <- r2, r2 <- y - f2 r2 (the new r2 goes into the old r1's storage)
-    double* nr2 = r1;
+    double* nr2 = m.r1;
     for (int64_t i0 = threadIdx.x; i0 < n; i0 += kB * T) {
       double yy[kB], rr[kB];
 #pragma unroll
       for (int q = 0; q < kB; ++q) {
         const int64_t i = i0 + q * T;
-        yy[q] = i < n ? y[i] : 0.0;
-        rr[q] = i < n ? r2[i] : 0.0;
+        yy[q] = i < n ? m.y[i] : 0.0;
+        rr[q] = i < n ? m.r2[i] : 0.0;
       }
 #pragma unroll
       for (int q = 0; q < kB; ++q) {
@@ -372,8 +427,9 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
         }
       }
     }
-    r1 = r2;
-    r2 = nr2;
+    m.r1 = m.r2;
+    m.r2 = nr2;
+    pr ^= 1;
     const double nbeta = sqrt(block_sum(s, red));
     oldb = beta;
     tnorm2 += alfa * alfa + oldb * oldb + nbeta * nbeta;
@@ -394,17 +450,17 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
     s = 0.0;
     // w1 <- w2, w2 <- w, w <- (v - oldeps w1 - delta w2) / gamma (into the old
     // w2's storage), x += phi w, and the next Lanczos vector v <- r2 / beta
-    double* nw_ = w2;
+    double* nw_ = m.w2;
     for (int64_t i0 = threadIdx.x; i0 < n; i0 += kB * T) {
       double o2[kB], o1[kB], vv[kB], xx[kB], rr[kB];
 #pragma unroll
       for (int q = 0; q < kB; ++q) {
         const int64_t i = i0 + q * T;
-        o2[q] = i < n ? w2[i] : 0.0;
-        o1[q] = i < n ? w[i] : 0.0;
-        vv[q] = i < n ? v[i] : 0.0;
-        xx[q] = i < n ? x[i] : 0.0;
-        rr[q] = i < n ? r2[i] : 0.0;
+        o2[q] = i < n ? m.w2[i] : 0.0;
+        o1[q] = i < n ? m.w[i] : 0.0;
+        vv[q] = i < n ? m.v[i] : 0.0;
+        xx[q] = i < n ? m.x[i] : 0.0;
+        rr[q] = i < n ? m.r2[i] : 0.0;
       }
 #pragma unroll
       for (int q = 0; q < kB; ++q) {
@@ -413,14 +469,15 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
           const double nw = (vv[q] - oldeps * o2[q] - delta * o1[q]) * denom;
           nw_[i] = nw;
           const double xi = xx[q] + phi * nw;
-          x[i] = xi;
+          m.x[i] = xi;
           s += xi * xi;
-          v[i] = rr[q] / nbeta;
+          m.v[i] = rr[q] / nbeta;
         }
       }
     }
-    w2 = w;
-    w = nw_;
+    m.w2 = m.w;
+    m.w = nw_;
+    pw ^= 1;
     const double ynorm = sqrt(block_sum(s, red));
     beta = nbeta;
     gmax = fmax(gmax, gamma);
@@ -440,10 +497,27 @@ __device__ void minres_blk(const LocOp& op, int64_t n, const double* b, const do
       if (test1 <= rtol) istop = 1;
     }
     add_flops(18.0 * (double)n);
-    if (istop != 0) break;
+  }
+  if (istop == 0 && itn >= maxiter) istop = 6;
+  if (threadIdx.x == 0) {
+    st[kMrBeta] = beta;
+    st[kMrOldb] = oldb;
+    st[kMrDbar] = dbar;
+    st[kMrEpsln] = epsln;
+    st[kMrPhibar] = phibar;
+    st[kMrTnorm2] = tnorm2;
+    st[kMrGmax] = gmax;
+    st[kMrGmin] = gmin;
+    st[kMrCs] = cs;
+    st[kMrSn] = sn;
+    st[kMrItn] = (double)itn;
+    st[kMrIstop] = (double)istop;
+    st[kMrPr] = (double)pr;
+    st[kMrPw] = (double)pw;
+    if (iters) *iters += done;
   }
   __syncthreads();
-  if (iters) *iters += itn;
+  return done;
 }
 
 // ---------------------------------------------------------------- slot phases
@@ -558,7 +632,7 @@ __device__ void fail(ttx_slot* s, int64_t code) {
 
 // ---------------------------------------------------------------- kernels
 __device__ void env_body(ttx_slot* s, int kind) {
-  if (s->status != TTX_SLOT_RUNNING) return;
+  if (s->status != TTX_SLOT_RUNNING || s->sstage != kSolveNone) return;
   const int64_t ph = s->phase, d = s->d;
   int side;
   int64_t k = s->k;
@@ -624,7 +698,6 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_env_kernel(ttx_slot* slots
 // eng_chol_kernel (symmetric: Cholesky) and, when that finds the matrix not
 // positive definite or the problem is not symmetric, to eng_lu_kernel (LU
 // with partial pivoting).  scal[0] receives the exact local residual.
-enum { kSolveNone = 0, kSolveChol = 1, kSolveLu = 2 };
 
 __device__ bool solve_step_active(const ttx_slot* s) {
   return s->status == TTX_SLOT_RUNNING && (s->phase == TTX_PH_FWD || s->phase == TTX_PH_BWD);
@@ -656,40 +729,66 @@ __device__ void solve_body(ttx_slot* s) {
   double* bloc = scr(s, TTX_SCR_BLOC);
   double* tmp = scr(s, TTX_SCR_TMP);
   double* scal = scr(s, TTX_SCR_SCAL);
-  local_rhs_blk(envp(s, 1, 0, k), s->rb[k], rl, s->b[k], n, s->rb[k + 1], envp(s, 1, 1, k), rr, bloc,
-                scr(s, TTX_SCR_S1));
-  __syncthreads();
-  stamp(s, 0, t0);
-  const bool iter = s->iterative && N > s->direct_size;
-  if (!iter) {
-    if (N > s->direct_cap) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
-    if (threadIdx.x == 0) s->sstage = s->chol ? kSolveChol : kSolveLu;
-    return;
-  }
-  LocOp op = system_op(s, k);
-  const double target = s->local_rtol * bnorm2(bloc, nullptr, N, red);
-  double rt = s->local_rtol;
+  double* st = scal + 8;  // MINRES state (kMrCount doubles)
   double* V = scr(s, TTX_SCR_MR);
-  const double* x0 = xcore(s, k);  // warm start: the current core (one_site.py:151-153)
-  bool ok = false;
-  for (int attempt = 0; attempt < 4; ++attempt) {
-    minres_blk(op, N, bloc, x0, rt, 5 * N, V, red, threadIdx.x == 0 ? &s->minres_iters : nullptr);
-    x0 = V;
+  const int64_t stage = s->sstage;
+  if (stage != kSolveNone && stage != kSolveMinres) return;  // a direct solve is pending
+  LocOp op = system_op(s, k);
+  if (stage == kSolveNone) {
+    local_rhs_blk(envp(s, 1, 0, k), s->rb[k], rl, s->b[k], n, s->rb[k + 1], envp(s, 1, 1, k), rr,
+                  bloc, scr(s, TTX_SCR_S1));
+    __syncthreads();
+    stamp(s, 0, t0);
+    const bool iter = s->iterative && N > s->direct_size;
+    if (!iter) {
+      if (N > s->direct_cap) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
+      if (threadIdx.x == 0) s->sstage = s->chol ? kSolveChol : kSolveLu;
+      return;
+    }
+    // first attempt, warm-started from the current core (one_site.py:151-153)
+    const double target = s->local_rtol * bnorm2(bloc, nullptr, N, red);
+    if (threadIdx.x == 0) {
+      st[kMrRtol] = s->local_rtol;
+      st[kMrMaxit] = (double)(5 * N);
+      st[kMrAttempt] = 0.0;
+      st[kMrTarget] = target;
+      s->sstage = kSolveMinres;
+    }
+    minres_start(op, N, bloc, xcore(s, k), V, red, st);
+  }
+  // restarted MINRES with the true-residual check (common.py:776-800), at most
+  // kMinresSlice iterations in this tick
+  int64_t budget = kMinresSlice;
+  while (true) {
+    budget -= minres_run(op, N, V, red, st, budget, threadIdx.x == 0 ? &s->minres_iters : nullptr);
+    if (st[kMrIstop] == 0.0) break;  // out of budget: resume next tick
     op.apply(V, 1, tmp, 1.0, 0.0);
     const double res = bnorm2(tmp, bloc, N, red);
-    if (res <= target) { ok = true; break; }
-    rt = fmax(rt * 1e-2, 1e-15);
+    const double target = st[kMrTarget];
+    const int attempt = (int)st[kMrAttempt];
+    const bool ok = res <= target;
+    if (ok || attempt >= 3) {
+      bcopy(u, V, N);
+      if (threadIdx.x == 0) {
+        s->local_solves += 1;
+        if (!ok) s->local_inexact += 1;
+        scal[1] = 0.0;
+        s->sstage = kSolveNone;
+      }
+      __syncthreads();
+      stamp(s, 3, t0);
+      exact_local_residual(s, k, N, red, t0);
+      return;
+    }
+    __syncthreads();  // everyone has read st before it is rewritten
+    if (threadIdx.x == 0) {
+      st[kMrRtol] = fmax(st[kMrRtol] * 1e-2, 1e-15);
+      st[kMrAttempt] = (double)(attempt + 1);
+    }
+    minres_start(op, N, bloc, V, V, red, st);  // warm start from the last iterate
+    if (budget <= 0) break;
   }
-  bcopy(u, V, N);
   stamp(s, 3, t0);
-  if (threadIdx.x == 0) {
-    s->local_solves += 1;
-    if (!ok) s->local_inexact += 1;
-    scal[1] = 0.0;
-    s->sstage = kSolveNone;
-  }
-  __syncthreads();
-  exact_local_residual(s, k, N, red, t0);
 }
 
 __device__ void chol_body(ttx_slot* s, double* lds) {
@@ -836,6 +935,7 @@ __device__ void post_body(ttx_slot* s) {
     return;
   }
   if (ph != TTX_PH_FWD && ph != TTX_PH_BWD) return;
+  if (s->sstage != kSolveNone) return;  // the local solve is still running
 
   const int64_t rl = s->rx[k], rr = s->rx[k + 1];
   const int64_t N = rl * n * rr;
@@ -869,7 +969,7 @@ __device__ void post_body(ttx_slot* s) {
   if (threadIdx.x == 0) s_svd = TTX_OK;
   __syncthreads();
   SvdArgs sa{p, q, u, q, 1, U, K, 1, S, Vt, q, 1, scr(s, TTX_SCR_SVDW), &s_svd};
-  block_svd(sa, scr(s, TTX_SCR_SVDW), red);
+  block_svd(sa, scr(s, TTX_SCR_SVDW), red, eng_lds, (int64_t)(kEngLds / 8));
   __syncthreads();
   stamp(s, 9, t0);
   if (s_svd == TTX_ENONFINITE) { fail(s, TTX_SLOT_ERR_NONFINITE); return; }
@@ -969,8 +1069,14 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_post_kernel(ttx_slot* slot
 }
 
 // ||A x - b|| / ||b|| in TT arithmetic (common.py:118-146): the residual
-// train is formed core by core and normed by a Q-less right QR sweep.
-__device__ double residual_norm_blk(ttx_slot* s, double* red) {
+// train is formed core by core and normed by a Q-less right QR sweep.  The
+// sweep resumes across ticks: s->chk_core is the next core (d-1 .. 0) and the
+// carried R^T factor (s->chk_cdim columns) stays in scratch; cores are taken
+// until kCheckSliceWork (m L^2 of the QRs) is spent.  Returns true when the
+// norm is complete (in *nrm).
+constexpr double kCheckSliceWork = 1.5e8;
+
+__device__ bool residual_norm_blk(ttx_slot* s, double* red, double* nrm) {
   const int64_t d = s->d;
   double* base = scr(s, TTX_SCR_CHK);
   const int64_t L2 = s->lmax * s->lmax;
@@ -979,16 +1085,22 @@ __device__ double residual_norm_blk(ttx_slot* s, double* red) {
   double* C = Wb + L2 * s->nmax;          // R x c
   double* Rb = C + L2;                    // R factor
   double* qw = Rb + L2;                   // QR work
-  int64_t cdim = 1;
-  if (threadIdx.x == 0) C[0] = 1.0;
+  int64_t i = s->chk_core, cdim = s->chk_cdim;
+  if (i < 0) {
+    i = d - 1;
+    cdim = 1;
+    if (threadIdx.x == 0) C[0] = 1.0;
+  }
   __syncthreads();
-  double nrm = 0.0;
-  for (int64_t i = d - 1; i >= 0; --i) {
+  double work = 0.0;
+  for (; i >= 0; --i) {
     const int64_t n = s->n[i];
     const int64_t Ra = s->ra[i], Rb2 = s->ra[i + 1], rx = s->rx[i], rx2 = s->rx[i + 1];
     const int64_t rbl = s->rb[i], rbr = s->rb[i + 1];
     const int64_t L = i == 0 ? 1 : Ra * rx + rbl;
     const int64_t R = i == d - 1 ? 1 : Rb2 * rx2 + rbr;
+    if (work > 0.0 && work + (double)(n * cdim) * L * L > kCheckSliceWork) break;  // next tick
+    work += (double)(n * cdim) * L * L;
     bfill(T, L * n * R, 0.0);
     __syncthreads();
     const double* A = s->A[i];
@@ -1012,8 +1124,9 @@ __device__ double residual_norm_blk(ttx_slot* s, double* red) {
     bgemm(1, L * n, cdim, R, 1.0, T, R, 1, 0, C, cdim, 1, 0, 0.0, Wb, cdim, 1, 0);
     __syncthreads();
     if (i == 0) {
-      nrm = bnorm2(Wb, nullptr, L * n * cdim, red);
-      break;
+      *nrm = bnorm2(Wb, nullptr, L * n * cdim, red);
+      if (threadIdx.x == 0) s->chk_core = -1;
+      return true;
     }
     // QR of the transposed unfolding ((n c) x L), keep R^T as the next carry
     const int64_t m = n * cdim;
@@ -1026,7 +1139,11 @@ __device__ double residual_norm_blk(ttx_slot* s, double* red) {
     cdim = kk;
     __syncthreads();
   }
-  return nrm;
+  if (threadIdx.x == 0) {
+    s->chk_core = i;
+    s->chk_cdim = cdim;
+  }
+  return false;
 }
 
 __device__ void check_decide(ttx_slot* s, int64_t sw, double res);
@@ -1075,6 +1192,7 @@ __device__ void check_body(ttx_slot* s) {
   __shared__ double red[16];
   if (s->status != TTX_SLOT_RUNNING) return;
   if (s->phase != TTX_PH_CHECK) {
+    if (s->sstage != kSolveNone) return;  // the local solve resumes next tick
     __syncthreads();
     if (threadIdx.x == 0) {
       s->ticks += 1;
@@ -1085,7 +1203,8 @@ __device__ void check_body(ttx_slot* s) {
   const int64_t sw = s->sweep;
   double res = 0.0;
   if (s->track) {
-    const double rn = residual_norm_blk(s, red);
+    double rn = 0.0;
+    if (!residual_norm_blk(s, red, &rn)) return;  // the sweep resumes next tick
     res = s->bnorm == 0.0 ? rn : rn / s->bnorm;
   }
   __syncthreads();
@@ -1229,7 +1348,8 @@ extern "C" int ttx_engine_plan(ttx_slot* s) {
 static int engine_prepare() {
   static bool attr = false;
   if (!attr) {
-    const void* fns[2] = {(const void*)eng_chol_kernel, (const void*)eng_lu_kernel};
+    const void* fns[4] = {(const void*)eng_chol_kernel, (const void*)eng_lu_kernel,
+                          (const void*)eng_post_kernel, (const void*)eng_check_kernel};
     for (const void* f : fns)
       if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kEngLds) !=
           hipSuccess)
@@ -1247,8 +1367,8 @@ static void launch_stage(int i, hipStream_t st, ttx_slot* slots, int64_t nslots)
     case 1: hipLaunchKernelGGL(eng_solve_kernel, g, b, 0, st, slots); break;
     case 2: hipLaunchKernelGGL(eng_chol_kernel, g, b, kEngLds, st, slots); break;
     case 3: hipLaunchKernelGGL(eng_lu_kernel, g, b, kEngLds, st, slots); break;
-    case 4: hipLaunchKernelGGL(eng_post_kernel, g, b, 0, st, slots); break;
-    default: hipLaunchKernelGGL(eng_check_kernel, g, b, 0, st, slots); break;
+    case 4: hipLaunchKernelGGL(eng_post_kernel, g, b, kEngLds, st, slots); break;
+    default: hipLaunchKernelGGL(eng_check_kernel, g, b, kEngLds, st, slots); break;
   }
 }
 

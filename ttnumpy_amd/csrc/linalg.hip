@@ -17,6 +17,9 @@
 
 namespace ttx {
 
+// dynamic LDS of the global-workspace QR/SVD kernels (blocked QR panels)
+constexpr size_t kQrLds = 150 * 1024;
+
 struct QrArgs {
   int64_t m, n;
   const double* A;
@@ -43,7 +46,9 @@ __global__ __launch_bounds__(1024) void qr_kernel(QrArgs p) {
     W[e] = p.A[i * p.a_rs + j * p.a_cs];
   }
   __syncthreads();
-  block_householder_qr(m, n, W, tau, Qw, red, p.Q != nullptr);
+  // global workspace: LDS holds the blocked factorization's panels
+  block_householder_qr(m, n, W, tau, Qw, red, p.Q != nullptr, IN_LDS ? nullptr : smem,
+                       IN_LDS ? 0 : (int64_t)(kQrLds / sizeof(double)));
   __syncthreads();
   if (p.Q) {
     for (int64_t e = threadIdx.x; e < m * k; e += blockDim.x) {
@@ -64,7 +69,8 @@ template <bool IN_LDS>
 __global__ __launch_bounds__(1024) void svd_kernel(SvdArgs p) {
   extern __shared__ double smem[];
   __shared__ double red[16];
-  block_svd(p, IN_LDS ? smem : p.work, red);
+  block_svd(p, IN_LDS ? smem : p.work, red, IN_LDS ? nullptr : smem,
+            IN_LDS ? 0 : (int64_t)(kQrLds / sizeof(double)));
 }
 
 // ---------------------------------------------------------------------------
@@ -190,10 +196,18 @@ extern "C" int ttx_qr_f64(ttx_stream_t stream, int64_t m, int64_t n, const doubl
   }
   QrArgs p{m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work};
   const size_t bytes = (size_t)ttx_qr_work_size(m, n) * sizeof(double);
-  if (bytes <= kLdsLimit)
+  const int64_t k = m < n ? m : n;
+  // k >= kQrBlockMin: the blocked factorization (panels in LDS), as in the engine
+  if (bytes <= kLdsLimit && k < kQrBlockMin) {
     hipLaunchKernelGGL(qr_kernel<true>, dim3(1), dim3(1024), bytes, (hipStream_t)stream, p);
-  else
-    hipLaunchKernelGGL(qr_kernel<false>, dim3(1), dim3(1024), 0, (hipStream_t)stream, p);
+  } else {
+    static bool attr_g = false;
+    if (!attr_g) {
+      if (set_lds_attr((const void*)qr_kernel<false>) != TTX_OK) return TTX_EHIP;
+      attr_g = true;
+    }
+    hipLaunchKernelGGL(qr_kernel<false>, dim3(1), dim3(1024), kQrLds, (hipStream_t)stream, p);
+  }
   return launch_status();
 }
 
@@ -214,10 +228,17 @@ extern "C" int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n, const doub
   }
   SvdArgs p{m, n, A, a_rs, a_cs, U, u_rs, u_cs, S, Vt, v_rs, v_cs, work, info};
   const size_t bytes = (size_t)ttx_svd_work_size(m, n) * sizeof(double);
-  if (bytes <= kLdsLimit)
+  const int64_t k = m < n ? m : n;
+  if (bytes <= kLdsLimit && k < kQrBlockMin) {
     hipLaunchKernelGGL(svd_kernel<true>, dim3(1), dim3(1024), bytes, (hipStream_t)stream, p);
-  else
-    hipLaunchKernelGGL(svd_kernel<false>, dim3(1), dim3(1024), 0, (hipStream_t)stream, p);
+  } else {
+    static bool attr_g = false;
+    if (!attr_g) {
+      if (set_lds_attr((const void*)svd_kernel<false>) != TTX_OK) return TTX_EHIP;
+      attr_g = true;
+    }
+    hipLaunchKernelGGL(svd_kernel<false>, dim3(1), dim3(1024), kQrLds, (hipStream_t)stream, p);
+  }
   return launch_status();
 }
 

@@ -67,7 +67,7 @@ class Slot(ctypes.Structure):
         + [(nm, _i64) for nm in ("phase", "k", "sweep", "status", "stop_reason", "local_solves",
                                  "local_inexact", "local_fallbacks", "rank_decisions", "rank_capped",
                                  "red_last", "svd_noconv", "ticks", "minres_iters", "lu_solves",
-                                 "chol_fallbacks", "sstage", "pad6")]
+                                 "chol_fallbacks", "sstage", "chk_core", "chk_cdim")]
         + [("flops", _f64 * NKERNELS)]
         + [("residuals", _f64 * MAXSWEEPS), ("max_bonds", _i64 * MAXSWEEPS)]
         + [("clk", _i64 * 24)]
@@ -326,6 +326,8 @@ class Engine:
             if s.k < 1:
                 s.phase, s.k = PH_ZORTH, d - 1
         s.status = SLOT_RUNNING
+        s.sstage = 0
+        s.chk_core = -1
         ctypes.memmove(ctypes.addressof(self.host[i]), ctypes.addressof(s), ctypes.sizeof(Slot))
         self.slot_req[i] = req
         self.slot_mem[i] = tuple(keep)

@@ -1,0 +1,150 @@
+"""Serving loop for parametric 2-D diffusion solves on the batched engine.
+
+A stream of requests arrives with exponential inter-arrival times (Poisson
+arrivals at ``rate`` per second).  Request ``i`` solves
+``-div(k grad u) = 1`` on a ``2**L x 2**L`` grid with
+``k = 1 + theta sin(pi x) sin(pi y)``, ``L`` uniform in ``levels`` and
+``theta`` uniform in ``[0, theta_max)`` (:func:`ttnumpy_amd.problems.diffusion_2d_cores`),
+by AMEn (``symmetric=True``, ``residual_tol``, ``max_sweeps``) from
+``x0 = rand([2]*2L, ranks=2, seed=i)``.
+
+The engine admits a request once it has arrived, a slot is free and free HBM
+covers the slot's arena; requests of different sizes run side by side and
+every tick advances all of them by one step of their own sweep (the same
+site-step kernels serve every request in one launch per kernel).  Latency is
+measured from arrival to result.  Operators and right-hand sides are built
+and copied to the device before the clock starts, as a client would hand
+them over.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import device as D
+from . import problems as P
+from .engine import Engine, Request
+from .tensor_train import TensorTrain, rand
+
+
+@dataclass
+class ServingConfig:
+    n_requests: int = 512
+    rate: float = 50.0            # arrivals per second
+    levels: Sequence[int] = (8, 9, 10, 11, 12)
+    theta_max: float = 0.9
+    seed: int = 0
+    residual_tol: float = 1e-8
+    max_sweeps: int = 20
+    nslots: int = 128
+    rmax: int = 64
+    poll: int = 16
+
+
+@dataclass
+class ServingItem:
+    index: int
+    level: int
+    theta: float
+    offset: float                 # arrival time after the start, seconds
+
+
+def request_stream(cfg: ServingConfig) -> List[ServingItem]:
+    """The seeded arrivals, levels and coefficients."""
+    rng = np.random.default_rng(cfg.seed)
+    gaps = rng.exponential(1.0 / cfg.rate, size=cfg.n_requests)
+    levels = rng.choice(np.asarray(cfg.levels), size=cfg.n_requests)
+    thetas = rng.uniform(0.0, cfg.theta_max, size=cfg.n_requests)
+    t = np.cumsum(gaps) - gaps[0]
+    return [ServingItem(i, int(levels[i]), float(thetas[i]), float(t[i]))
+            for i in range(cfg.n_requests)]
+
+
+def build_requests(cfg: ServingConfig, items: Sequence[ServingItem]) -> List[Request]:
+    """Device-resident operators, right-hand sides and initial guesses."""
+    parts: Dict[int, tuple] = {}
+    rhs: Dict[int, TensorTrain] = {}
+    out = []
+    for it in items:
+        L = it.level
+        if L not in parts:
+            parts[L] = P.diffusion_2d_parts(L)
+            rhs[L] = TensorTrain(P.ones_cores([2] * (2 * L)))
+        A = TensorTrain(P.diffusion_2d_cores(L, it.theta, parts[L]))
+        x0 = rand([2] * (2 * L), ranks=2, seed=it.index)
+        out.append(Request(A=A, b=rhs[L], x0=x0, symmetric=True, residual_tol=cfg.residual_tol,
+                           max_sweeps=cfg.max_sweeps, seed=it.index, tag=it))
+    return out
+
+
+@dataclass
+class ServingReport:
+    solves: int
+    errors: int
+    wall_s: float
+    solves_per_s: float
+    p50_ms: float
+    p99_ms: float
+    gflop: float
+    gflop_per_s: float
+    converged: int
+    stagnated: int
+    max_sweeps: int
+    worst_residual: float
+    by_level: Dict[int, dict] = field(default_factory=dict)
+
+
+def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
+          engine: Optional[Engine] = None, progress=None) -> tuple:
+    """Run the stream; returns ``(report, requests)``.  ``progress(n_done, req)``
+    is called after every finished request."""
+    items = request_stream(cfg)
+    if requests is None:
+        requests = build_requests(cfg, items)
+    eng = engine or Engine(cfg.nslots, rmax=cfg.rmax)
+    torch.cuda.synchronize()
+    flops0 = float(eng.kernel_flops.sum())
+    t0 = time.perf_counter()
+    for r, it in zip(requests, items):
+        r.arrival = t0 + it.offset
+        r.result = r.info = r.error = None
+    count = [0]
+
+    def on_done(r):
+        count[0] += 1
+        if progress is not None:
+            progress(count[0], r)
+
+    done = eng.run(sorted(requests, key=lambda r: r.arrival), poll=cfg.poll, clock=time.perf_counter,
+                   on_done=on_done)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([r.finished - r.arrival for r in done])
+    ok = [r for r in done if r.error is None]
+    stops = [str(r.info.stop_reason) for r in ok]
+    by_level: Dict[int, dict] = {}
+    for r in ok:
+        L = r.tag.level
+        e = by_level.setdefault(L, {"n": 0, "lat_ms": [], "sweeps": [], "max_bond": 0})
+        e["n"] += 1
+        e["lat_ms"].append(1e3 * (r.finished - r.arrival))
+        e["sweeps"].append(r.info.num_sweeps_run)
+        e["max_bond"] = max(e["max_bond"], max(r.result.bonds))
+    for L, e in by_level.items():
+        e["p50_ms"] = float(np.percentile(e.pop("lat_ms"), 50))
+        e["mean_sweeps"] = float(np.mean(e.pop("sweeps")))
+    gflop = (float(eng.kernel_flops.sum()) - flops0) / 1e9
+    rep = ServingReport(
+        solves=len(ok), errors=len(done) - len(ok), wall_s=wall, solves_per_s=len(ok) / wall,
+        p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
+        gflop=gflop, gflop_per_s=gflop / wall,
+        converged=stops.count("converged"), stagnated=stops.count("stagnated"),
+        max_sweeps=stops.count("max_sweeps"),
+        worst_residual=max((r.info.residuals[-1] for r in ok), default=float("nan")),
+        by_level=dict(sorted(by_level.items())))
+    return rep, done
