@@ -31,8 +31,9 @@ N_BITS = 8  # 256 x 256 grid
 RMAX = 48   # slot rank capacity (the C1 solution reaches bond 33)
 PEAK_FP64_TFLOPS = 78.6
 BATCH_PROFILED = 512  # batch of the committed counter summary
-KERNELS = ("ttx::eng_env_kernel", "ttx::eng_solve_kernel", "ttx::eng_chol_kernel",
-           "ttx::eng_lu_kernel", "ttx::eng_post_kernel", "ttx::eng_check_kernel")
+KERNELS = ("ttx::eng_env_kernel", "ttx::eng_solve_kernel", "ttx::eng_chol32_kernel",
+           "ttx::eng_chol_kernel", "ttx::eng_lu_kernel", "ttx::eng_post_kernel",
+           "ttx::eng_check_kernel")
 
 
 def build_inputs():
@@ -45,10 +46,28 @@ def build_inputs():
     return A, b, x0
 
 
-def requests(A, b, x0, n):
+def requests(A, b, x0, n, precision="fp64"):
     from ttnumpy_amd.engine import Request
-    return [Request(A=A, b=b, x0=x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0)
-            for _ in range(n)]
+    return [Request(A=A, b=b, x0=x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0,
+                    precision=precision) for _ in range(n)]
+
+
+def timed_batches(eng, A, b, x0, args, precision):
+    """``args.warmup`` untimed then ``args.steps`` timed batches; (solves/s, latencies, done, s)."""
+    import torch
+    for _ in range(args.warmup):
+        eng.run(requests(A, b, x0, args.batch, precision), poll=args.poll)
+    torch.cuda.synchronize()
+    lat, done = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = eng.run(requests(A, b, x0, args.batch, precision), poll=args.poll)
+        done += out
+        lat += [r.finished - r.admitted for r in out]
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    ok = [r for r in done if r.error is None]
+    return len(ok) / total, lat, done, total
 
 
 def measured_traffic(kernel: str):
@@ -111,6 +130,7 @@ def main():
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--poll", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--single", action="store_true",
                     help="also time one request alone (its launches are not part of the roofline)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -135,21 +155,10 @@ def main():
         return
     A, b, x0 = build_inputs()
     eng = Engine(args.batch, rmax=RMAX)
-    for _ in range(args.warmup):
-        eng.run(requests(A, b, x0, args.batch), poll=args.poll)
-    torch.cuda.synchronize()
-    lat, done = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = eng.run(requests(A, b, x0, args.batch), poll=args.poll)
-        done += out
-        lat += [r.finished - r.admitted for r in out]
-    torch.cuda.synchronize()
-    total = time.perf_counter() - t0
+    value, lat, done, total = timed_batches(eng, A, b, x0, args, "fp64")
     errors = [r.error for r in done if r.error]
     ok = [r for r in done if r.error is None]
     nsolved = len(ok)
-    value = nsolved / total
 
     single_ms = None
     if args.single:
@@ -183,6 +192,29 @@ def main():
                 for k, v in zip(KERNELS, prof.kernel_ms)}}
     r0 = ok[0] if ok else None
     verify = verify_result(A, b, r0) if r0 is not None else None
+    mixed = None
+    if not args.no_mixed:
+        # the mixed-precision engine on the same batches: FP32 Cholesky with FP64
+        # refinement for the direct local solves (N >= 256); everything else FP64
+        engm = Engine(args.batch, rmax=RMAX)
+        mval, mlat, mdone, _ = timed_batches(engm, A, b, x0, args, "mixed")
+        mok = [r for r in mdone if r.error is None]
+        m0 = mok[0] if mok else None
+        mixed = {"value": mval, "unit": "solves/s", "ratio_vs_fp64": mval / value if value else None,
+                 "p50_ms": 1e3 * float(np.percentile(mlat, 50)),
+                 "p99_ms": 1e3 * float(np.percentile(mlat, 99)),
+                 "solves_ok": len(mok), "errors": [r.error for r in mdone if r.error][:3],
+                 "mixed_direct_solves_per_solve": engm.mixed_solves / max(len(mdone), 1),
+                 "refinement_steps_per_mixed_solve": engm.mixed_refines / max(engm.mixed_solves, 1),
+                 "fp64_fallbacks": engm.mixed_fallbacks}
+        if m0 is not None and r0 is not None:
+            from oracle import tt_oracle as O
+            mv = verify_result(A, b, m0)
+            xm, x64 = O.Train(m0.result.to_numpy()), O.Train(r0.result.to_numpy())
+            diff = O.tt_norm(O.tt_add(xm, O.tt_scale(x64, -1.0))) / O.tt_norm(x64)
+            mixed.update({"final_residual": m0.info.residuals[-1], "sweeps": m0.info.num_sweeps_run,
+                          "stop_reason": str(m0.info.stop_reason), "verify": mv,
+                          "relative_difference_vs_fp64_solution": float(diff)})
     line = {
         "metric": "AMEn solves/s + p99 latency, mixed vs FP64 on same GPU at residual 1e-8",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -209,6 +241,7 @@ def main():
              "lu.block_rows", "lu.trailing", "lu.backsub"), prof.stage_ms)},
         "roofline": roof,
         "verify": verify,
+        "mixed": mixed,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()

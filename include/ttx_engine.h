@@ -9,7 +9,7 @@
 
 #define TTX_MAXD 64
 #define TTX_MAXSWEEPS 64
-#define TTX_ENGINE_KERNELS 6  /* per tick: env, solve, chol, lu, post, check */
+#define TTX_ENGINE_KERNELS 7  /* per tick: env, solve, chol32, chol, lu, post, check */
 
 enum ttx_phase {
   TTX_PH_XORTH = 1,     // right-orthonormalise x0, k = d-1 .. 1
@@ -49,6 +49,7 @@ typedef struct ttx_slot {
   int64_t d, kick, max_rank, direct_size, iterative, residual_mode, max_sweeps, track;
   int64_t rmax, nmax, ramax, rbmax, bneg, allow_fallback, chunk;
   int64_t chol;  // 1: direct solves try Cholesky first (symmetric problems), LU if not SPD
+  int64_t mixed; // 1: direct Cholesky solves of size >= 256 factor in FP32 with FP64 refinement
   int64_t n[TTX_MAXD];
   int64_t ra[TTX_MAXD + 1];
   int64_t rb[TTX_MAXD + 1];
@@ -70,7 +71,8 @@ typedef struct ttx_slot {
   int64_t phase, k, sweep, status, stop_reason, local_solves, local_inexact, local_fallbacks;
   int64_t rank_decisions, rank_capped, red_last, svd_noconv, ticks, minres_iters, lu_solves,
       chol_fallbacks;
-  int64_t sstage;   // local solve in flight: 1 Cholesky, 2 LU (this tick), 3 MINRES (resumes)
+  int64_t mixed_solves, mixed_refines, mixed_fallbacks;
+  int64_t sstage;   // local solve in flight: 1 Cholesky, 2 LU, 4 FP32 Cholesky (this tick), 3 MINRES (resumes)
   int64_t chk_core; // residual check in flight: next core (-1: not started)
   int64_t chk_cdim; // and the width of its carried R factor
   double flops[TTX_ENGINE_KERNELS];  // algorithmic flops banked per kernel
@@ -106,15 +108,15 @@ int64_t ttx_engine_field_offset(int which);
  */
 int ttx_engine_plan(ttx_slot* s);
 /*
- * Advance every slot of the device array `slots` by `ticks` ticks (six
- * kernels each: environments; local rhs and MINRES; Cholesky; LU;
- * truncation/enrichment/shift; residual check).  Slots with status != RUNNING are skipped.  ticks == 0
+ * Advance every slot of the device array `slots` by `ticks` ticks (seven
+ * kernels each: environments; local rhs and MINRES; FP32 Cholesky with FP64
+ * refinement; Cholesky; LU; truncation/enrichment/shift; residual check).  Slots with status != RUNNING are skipped.  ticks == 0
  * only prepares the kernels' LDS attributes.
  */
 int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks);
 /*
- * ttx_engine_tick with HIP events around each of the six kernels and a
- * synchronisation per tick; ms_out[0..5] accumulate the kernels' milliseconds.
+ * ttx_engine_tick with HIP events around each of the seven kernels and a
+ * synchronisation per tick; ms_out[0..6] accumulate the kernels' milliseconds.
  */
 int ttx_engine_profile(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
                        double* ms_out);

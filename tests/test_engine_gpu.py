@@ -146,3 +146,25 @@ def test_serving_diffusion_alone_equals_in_stream():
     assert alone.result.bonds == target.result.bonds
     for a, b in zip(alone.result.to_numpy(), target.result.to_numpy()):
         np.testing.assert_array_equal(a, b)
+
+
+def test_engine_mixed_precision_c1():
+    """Mixed precision (FP32 Cholesky + FP64 refinement for direct solves of
+    size >= 256): converges to the FP64 tolerance, FP64-verified, and lands on
+    the FP64 engine's solution within the residual-implied distance."""
+    from ttnumpy_amd.engine import Engine
+    from ttnumpy_amd.solvers.common import relative_residual
+    print("engine C1 mixed vs fp64", flush=True)
+    eng = Engine(2, rmax=48)
+    r64 = _poisson_request(8)
+    r32 = _poisson_request(8, precision="mixed")
+    out = eng.run([r64, r32], poll=16)
+    assert all(r.error is None for r in out)
+    print(r64.info, r32.info, eng.mixed_solves, eng.mixed_refines, eng.mixed_fallbacks, flush=True)
+    assert eng.mixed_solves > 0 and eng.mixed_fallbacks == 0
+    assert str(r32.info.stop_reason) == "converged"
+    res = float(relative_residual(r32.A, r32.b, r32.result))
+    assert res <= 1e-8 * 1.05
+    d = _rel_diff(r32.result, O.Train(r64.result.to_numpy()))
+    print("relative difference mixed vs fp64", d, "fp64-verified residual", res, flush=True)
+    assert d < 1e-5

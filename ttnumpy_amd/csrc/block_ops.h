@@ -1148,4 +1148,255 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
   lu_clock(clk, 4, t0);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 Cholesky factorization (no right-hand side) for the mixed-precision
+// local solve: the same blocking as block_chol_solve, in single precision on
+// v_mfma_f32_16x16x4f32 (A lane l = A[l%16][l/16], B lane l = B[l/16][l%16],
+// output lane l element e = D[4 (l/16) + e][l%16], established on the box by
+// an in-bounds probe).  W: row-major, row stride ld, lower triangle read and
+// overwritten by L.  *info = TTX_ENOTSPD on a pivot <= 0 or not finite.
+// ---------------------------------------------------------------------------
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+template <int NT>
+__device__ __forceinline__ void chol32_tile_update_t(int64_t Mr, int64_t Nc, int64_t K, float* W,
+                                                     int64_t ld, int64_t r0, int64_t c0, int64_t a0,
+                                                     float* lds) {
+  constexpr int TN = 64 * NT / 256;
+  constexpr int EA = kLuTileM * kLuK / NT;
+  constexpr int TPA = kLuK / EA;
+  constexpr int EB = TN * kLuK / NT;
+  constexpr int TPB = kLuK / EB;
+  if (Mr <= 0 || Nc <= 0 || K <= 0) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int wm = (wave & 3) * 32, wn = (wave >> 2) * 64;
+  float* As = lds;
+  float* Bs = lds + kLuK * kLuTileM;
+  const int ar = tid / TPA, ak = (tid % TPA) * EA;
+  const int br = tid / TPB, bk = (tid % TPB) * EB;
+  const int64_t tm = (Mr + kLuTileM - 1) / kLuTileM, tn = (Nc + TN - 1) / TN;
+  for (int64_t t = 0; t < tm * tn; ++t) {
+    const int64_t m0 = (t / tn) * kLuTileM, n0 = (t % tn) * TN;
+    if (r0 + m0 + kLuTileM - 1 < c0 + n0) continue;
+    // output rows of this lane: m0 + wm + 16 mt + 4 lq + e
+    float* crow[2][4];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t row = m0 + wm + 16 * mt + 4 * lq + e;
+        crow[mt][e] = row < Mr ? W + (r0 + row) * ld + c0 : nullptr;
+      }
+    float4_t acc[2][4];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const int64_t col = n0 + wn + 16 * nt + l16;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          acc[mt][nt][e] = (crow[mt][e] && col < Nc) ? crow[mt][e][col] : 0.0f;
+      }
+    const float* ap = (m0 + ar) < Mr ? W + (r0 + m0 + ar) * ld + a0 : nullptr;
+    const float* bp = (n0 + br) < Nc ? W + (c0 + n0 + br) * ld + a0 : nullptr;
+    float ra[EA], rb[EB];
+    auto gload = [&](int64_t k0) {
+#pragma unroll
+      for (int i = 0; i < EA; ++i) {
+        const int64_t kk = k0 + ak + i;
+        ra[i] = (ap && kk < K) ? ap[kk] : 0.0f;
+      }
+#pragma unroll
+      for (int i = 0; i < EB; ++i) {
+        const int64_t kk = k0 + bk + i;
+        rb[i] = (bp && kk < K) ? bp[kk] : 0.0f;
+      }
+    };
+    gload(0);
+    for (int64_t k0 = 0; k0 < K; k0 += kLuK) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < EA; ++i) As[(ak + i) * kLuTileM + ar] = ra[i];
+#pragma unroll
+      for (int i = 0; i < EB; ++i) Bs[(bk + i) * TN + br] = rb[i];
+      __syncthreads();
+      if (k0 + kLuK < K) gload(k0 + kLuK);
+#pragma unroll
+      for (int ks = 0; ks < kLuK; ks += 4) {
+        const int kr = ks + lq;
+        float a[2], b[4];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) a[mt] = -As[kr * kLuTileM + wm + 16 * mt + l16];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) b[nt] = Bs[kr * TN + wn + 16 * nt + l16];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt)
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const int64_t col = n0 + wn + 16 * nt + l16;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (crow[mt][e] && col < Nc) crow[mt][e][col] = acc[mt][nt][e];
+      }
+  }
+  __syncthreads();
+}
+
+template <int NT>
+__device__ __forceinline__ void block_chol32_factor(int64_t n, int64_t ld, float* W, float* lds,
+                                                    int32_t* info) {
+  __shared__ int s_bad32;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  {
+    const double dn = (double)n;
+    add_flops(dn * dn * dn / 3.0);
+  }
+  if (tid == 0) s_bad32 = 0;
+  float* Lk = lds;          // 16 x 17
+  float* stage = lds + 288;
+  constexpr int64_t kOuter = 128;
+  for (int64_t K0 = 0; K0 < n; K0 += kOuter) {
+    const int64_t ko = (n - K0) < kOuter ? (n - K0) : kOuter;
+    const int64_t K1 = K0 + ko;
+    for (int64_t k = K0; k < K1; k += 16) {
+      const int64_t nb = (K1 - k) < 16 ? (K1 - k) : 16;
+      if (wave == 0) {
+        float row[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+          row[c] = (lane < nb && c <= lane && c < nb) ? W[(k + lane) * ld + k + c] : 0.0f;
+        bool bad = false;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          if (j < nb) {
+            const float djj = __shfl(row[j], j, 64);
+            bad = bad || !(djj > 0.0f) || !(djj < INFINITY);
+            const float d = sqrtf(djj);
+            const float lj = lane >= j ? row[j] / d : 0.0f;
+            if (lane >= j) row[j] = lj;
+#pragma unroll
+            for (int c = j + 1; c < 16; ++c) {
+              const float lcj = __shfl(lj, c, 64);
+              if (lane > j && c <= lane) row[c] -= lj * lcj;
+            }
+          }
+        }
+        if (lane < nb) {
+#pragma unroll
+          for (int c = 0; c < 16; ++c)
+            if (c <= lane && c < nb) {
+              W[(k + lane) * ld + k + c] = row[c];
+              Lk[lane * 17 + c] = row[c];
+            }
+        }
+        if (lane == 0 && bad) s_bad32 = 1;
+      }
+      __syncthreads();
+      if (s_bad32) {
+        if (tid == 0) info[0] = TTX_ENOTSPD;
+        return;
+      }
+      for (int64_t i = k + nb + tid; i < n; i += NT) {
+        float* wr = W + i * ld + k;
+        float x[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) x[c] = c < nb ? wr[c] : 0.0f;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+          if (c < nb) {
+            float v = x[c];
+#pragma unroll
+            for (int t = 0; t < c; ++t) v -= x[t] * Lk[c * 17 + t];
+            x[c] = v / Lk[c * 17 + c];
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+          if (c < nb) wr[c] = x[c];
+      }
+      __syncthreads();
+      chol32_tile_update_t<NT>(n - (k + nb), K1 - (k + nb), nb, W, ld, k + nb, k + nb, k, stage);
+    }
+    chol32_tile_update_t<NT>(n - K1, n - K1, ko, W, ld, K1, K1, K0, stage);
+  }
+}
+
+// x <- (L L^T)^{-1} x in FP32 with the factor from block_chol32_factor; x in
+// global memory (n floats).  lds: 64 x 65 + 64 floats.  Ends synced.
+__device__ __forceinline__ void chol32_apply_inverse(int64_t n, int64_t ld, const float* W, float* x,
+                                                     float* lds) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwave = blockDim.x >> 6;
+  float* T = lds;
+  float* xb = lds + 64 * 65;
+  // forward: L y = x, 64-row blocks from the top; rows below updated by rows
+  for (int64_t j0 = 0; j0 < n; j0 += 64) {
+    const int64_t j1 = (j0 + 64) < n ? (j0 + 64) : n;
+    const int64_t nbk = j1 - j0;
+    if (lane < nbk)
+      for (int64_t r = wave; r < nbk; r += nwave)
+        T[r * 65 + lane] = lane <= r ? W[(j0 + r) * ld + j0 + lane] : 0.0f;
+    __syncthreads();
+    if (wave == 0) {
+      const int64_t row = j0 + lane;
+      float yv = row < j1 ? x[row] : 0.0f;
+      for (int64_t j = j0; j < j1; ++j) {
+        const int jj = (int)(j - j0);
+        const float yj = __shfl(yv, jj, 64) / T[jj * 65 + jj];
+        if (row == j) yv = yj;
+        if (row > j && row < j1) yv -= yj * T[lane * 65 + jj];
+      }
+      if (row < j1) { x[row] = yv; xb[lane] = yv; }
+    }
+    __syncthreads();
+    for (int64_t i = j1 + tid; i < n; i += blockDim.x) {
+      const float* wr = W + i * ld + j0;
+      float acc = 0.0f;
+      for (int64_t c = 0; c < nbk; ++c) acc += wr[c] * xb[c];
+      x[i] -= acc;
+    }
+    __syncthreads();
+  }
+  // back: L^T z = y, 64-row blocks from the bottom; rows above by columns
+  for (int64_t j1 = n; j1 > 0; j1 -= 64) {
+    const int64_t j0 = j1 - 64 > 0 ? j1 - 64 : 0;
+    const int64_t nbk = j1 - j0;
+    if (lane < nbk)
+      for (int64_t r = wave; r < nbk; r += nwave)
+        T[r * 65 + lane] = lane <= r ? W[(j0 + r) * ld + j0 + lane] : 0.0f;
+    __syncthreads();
+    if (wave == 0) {
+      const int64_t row = j0 + lane;
+      float yv = row < j1 ? x[row] : 0.0f;
+      for (int64_t j = j1 - 1; j >= j0; --j) {
+        const int jj = (int)(j - j0);
+        const float zj = __shfl(yv, jj, 64) / T[jj * 65 + jj];
+        if (row == j) yv = zj;
+        if (row < j) yv -= zj * T[jj * 65 + lane];
+      }
+      if (row < j1) { x[row] = yv; xb[lane] = yv; }
+    }
+    __syncthreads();
+    for (int64_t i = tid; i < j0; i += blockDim.x) {
+      float acc = 0.0f;
+      for (int64_t c = 0; c < nbk; ++c) acc += W[(j0 + c) * ld + i] * xb[c];
+      x[i] -= acc;
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace ttx

@@ -36,7 +36,8 @@ constexpr size_t kEngLds = 75 * 1024;
 // solve kernel to the Cholesky or LU kernel of the same tick; MINRES resumes
 // over ticks, kMinresSlice iterations at a time, so that a long solve of one
 // request does not hold up the tick of every other request
-enum { kSolveNone = 0, kSolveChol = 1, kSolveLu = 2, kSolveMinres = 3 };
+enum { kSolveNone = 0, kSolveChol = 1, kSolveLu = 2, kSolveMinres = 3, kSolveChol32 = 4 };
+constexpr int64_t kMixedMinN = 256;  // smaller direct solves stay in FP64
 constexpr int64_t kMinresSlice = 128;  // MINRES iterations per tick for one slot
 
 __device__ __forceinline__ double* xcore(const ttx_slot* s, int64_t k) { return s->x + k * s->cap_core; }
@@ -140,7 +141,8 @@ struct LocOp {
 // lower, only entries on or below the diagonal are written.
 // Matches local_matrix_entry's sums term for term (a ascending, zero psi
 // skipped, b ascending).  Ends synced.
-__device__ void local_matrix_blk(const LocOp& op, double* out, int64_t ld, double* lds,
+template <typename OutT>
+__device__ void local_matrix_blk(const LocOp& op, OutT* out, int64_t ld, double* lds,
                                  int64_t lds_cap, bool lower) {
   constexpr int kMaxRa = 16;
   const int rows = (int)op.rows(), cols = (int)op.cols();
@@ -166,7 +168,7 @@ __device__ void local_matrix_blk(const LocOp& op, double* out, int64_t ld, doubl
       for (int row = 0; row < rows; ++row) {
         const int rq = row / fr, f_r = row - rq * fr, f_l = rq / nr, i = rq - f_l * nr;
         if (lower && row < it) continue;
-        out[(int64_t)row * ld + it] = local_matrix_entry(
+        out[(int64_t)row * ld + it] = (OutT)local_matrix_entry(
             psi + (int64_t)t_l * Ra * fl, fl, f_l, A + (i * nc + j) * Rb, (int64_t)nr * nc * Rb, Ra,
             Rb, phi + (int64_t)t_r * Rb * fr, fr, f_r);
       }
@@ -195,7 +197,7 @@ __device__ void local_matrix_blk(const LocOp& op, double* out, int64_t ld, doubl
           T[a] = t;
         }
       }
-      double* o = out + (int64_t)(i * fr + f_r) * ld + c;
+      OutT* o = out + (int64_t)(i * fr + f_r) * ld + c;
       const int64_t ostep = (int64_t)nr * fr * ld;
       // with lower, only rows >= c: f_l >= ceil((c - i*fr - f_r) / (nr*fr))
       int f0 = 0;
@@ -212,7 +214,7 @@ __device__ void local_matrix_blk(const LocOp& op, double* out, int64_t ld, doubl
             if (p != 0.0) sum += p * T[a];
           }
         }
-        o[f_l * ostep] = sum;
+        o[f_l * ostep] = (OutT)sum;
       }
     }
   }
@@ -742,7 +744,8 @@ __device__ void solve_body(ttx_slot* s) {
     const bool iter = s->iterative && N > s->direct_size;
     if (!iter) {
       if (N > s->direct_cap) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
-      if (threadIdx.x == 0) s->sstage = s->chol ? kSolveChol : kSolveLu;
+      if (threadIdx.x == 0)
+        s->sstage = !s->chol ? kSolveLu : (s->mixed && N >= kMixedMinN ? kSolveChol32 : kSolveChol);
       return;
     }
     // first attempt, warm-started from the current core (one_site.py:151-153)
@@ -791,6 +794,96 @@ __device__ void solve_body(ttx_slot* s) {
   stamp(s, 3, t0);
 }
 
+// mixed precision: FP32 Cholesky on the lower triangle (entries formed in
+// FP64, rounded once) and FP64 iterative refinement, u_{i+1} = u_i +
+// (L L^T)^{-1} (b - A_loc u_i) with the residual through the matrix-free FP64
+// operator, until it stops shrinking; accepted once ||r|| <= kRefineAccept
+// ||b_loc||, else the FP64 Cholesky takes the step (same tick).
+constexpr int kRefineMax = 8;
+constexpr double kRefineStop = 1e-14;
+constexpr double kRefineAccept = 1e-11;
+
+__device__ void chol32_body(ttx_slot* s, double* lds) {
+  __shared__ double red[16];
+  __shared__ int32_t s_info;
+  if (!solve_step_active(s) || s->sstage != kSolveChol32) return;
+  const int64_t k = s->k;
+  const int64_t N = s->rx[k] * s->n[k] * s->rx[k + 1];
+  int64_t t0 = wall_clock64();
+  float* W = reinterpret_cast<float*>(scr(s, TTX_SCR_LU));  // N (N+1) floats of 2 N (N+1)
+  float* d32 = W + N * (N + 1);
+  const double* bloc = scr(s, TTX_SCR_BLOC);
+  double* u = scr(s, TTX_SCR_U);
+  double* r = scr(s, TTX_SCR_TMP);
+  const LocOp op = system_op(s, k);
+  local_matrix_blk<float>(op, W, N + 1, lds, kEngLds / 8, true);
+  if (threadIdx.x == 0) s_info = TTX_OK;
+  __syncthreads();
+  stamp(s, 1, t0);
+  block_chol32_factor<kEngThreads>(N, N + 1, W, reinterpret_cast<float*>(lds), &s_info);
+  __syncthreads();
+  bool ok = false;
+  int it = 0;
+  double rn = 0.0;
+  if (s_info == TTX_OK) {
+    const double bn = bnorm2(bloc, nullptr, N, red);
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+      u[i] = 0.0;
+      r[i] = bloc[i];
+    }
+    __syncthreads();
+    double prev = bn;
+    for (it = 1; it <= kRefineMax; ++it) {
+      for (int64_t i = threadIdx.x; i < N; i += blockDim.x) d32[i] = (float)r[i];
+      __syncthreads();
+      chol32_apply_inverse(N, N + 1, W, d32, reinterpret_cast<float*>(lds));
+      for (int64_t i = threadIdx.x; i < N; i += blockDim.x) u[i] += (double)d32[i];
+      __syncthreads();
+      op.apply(u, 1, r, -1.0, 0.0);  // r = -A u
+      double sq = 0.0;
+      for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+        const double ri = bloc[i] + r[i];
+        r[i] = ri;
+        sq += ri * ri;
+      }
+      rn = sqrt(block_sum(sq, red));
+      add_flops(4.0 * (double)N * (double)N);
+      if (rn <= kRefineStop * bn) { ok = true; break; }
+      if (rn > 0.25 * prev) { ok = rn <= kRefineAccept * bn; break; }
+      prev = rn;
+    }
+    if (it > kRefineMax) ok = rn <= kRefineAccept * bn;
+  }
+  stamp(s, 2, t0);
+  if (!ok) {  // not SPD in FP32, or the refinement stalled: FP64 Cholesky this tick
+    if (threadIdx.x == 0) {
+      s->mixed_fallbacks += 1;
+      s->sstage = kSolveChol;
+    }
+    return;
+  }
+  if (threadIdx.x == 0) {
+    double* scal = scr(s, TTX_SCR_SCAL);
+    scal[1] = 1.0;
+    scal[0] = s->residual_mode ? rn : 0.0;  // the refinement's last residual is exact
+    s->lu_solves += 1;
+    s->mixed_solves += 1;
+    s->mixed_refines += it;
+    s->sstage = kSolveNone;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kEngThreads, 2) void eng_chol32_kernel(ttx_slot* slots) {
+  extern __shared__ double lds[];
+  ttx_slot* s = slots + blockIdx.x;
+  if (threadIdx.x == 0) g_flops = 0.0;
+  __syncthreads();
+  chol32_body(s, lds);
+  __syncthreads();
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[2] += g_flops;
+}
+
 __device__ void chol_body(ttx_slot* s, double* lds) {
   __shared__ double red[16];
   __shared__ int32_t s_info;
@@ -800,7 +893,7 @@ __device__ void chol_body(ttx_slot* s, double* lds) {
   int64_t t0 = wall_clock64();
   double* W = scr(s, TTX_SCR_LU);
   const double* bloc = scr(s, TTX_SCR_BLOC);
-  local_matrix_blk(system_op(s, k), W, N + 1, lds, kEngLds / 8, true);
+  local_matrix_blk<double>(system_op(s, k), W, N + 1, lds, kEngLds / 8, true);
   for (int64_t r = threadIdx.x; r < N; r += blockDim.x) W[r * (N + 1) + N] = bloc[r];
   if (threadIdx.x == 0) s_info = TTX_OK;
   __syncthreads();
@@ -834,7 +927,7 @@ __device__ void lu_body(ttx_slot* s, double* lds) {
   double* W = scr(s, TTX_SCR_LU);
   const double* bloc = scr(s, TTX_SCR_BLOC);
   if ((int64_t)kEngLds < 4 * N + 8 * (N > 4224 ? N : 4224)) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
-  local_matrix_blk(system_op(s, k), W, N + 1, lds, kEngLds / 8, false);
+  local_matrix_blk<double>(system_op(s, k), W, N + 1, lds, kEngLds / 8, false);
   for (int64_t r = threadIdx.x; r < N; r += blockDim.x) W[r * (N + 1) + N] = bloc[r];
   if (threadIdx.x == 0) s_info = TTX_OK;
   __syncthreads();
@@ -868,7 +961,7 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_chol_kernel(ttx_slot* slot
   __syncthreads();
   chol_body(s, lds);
   __syncthreads();
-  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[2] += g_flops;
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[3] += g_flops;
 }
 
 __global__ __launch_bounds__(kEngThreads, 2) void eng_lu_kernel(ttx_slot* slots) {
@@ -878,7 +971,7 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_lu_kernel(ttx_slot* slots)
   __syncthreads();
   lu_body(s, lds);
   __syncthreads();
-  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[3] += g_flops;
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[4] += g_flops;
 }
 
 // truncation, enrichment and shifts after a solve; QR sweeps and z fit
@@ -1065,7 +1158,7 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_post_kernel(ttx_slot* slot
   post_body(s);
   __syncthreads();
   if (s->status == TTX_SLOT_RUNNING || s->status == TTX_SLOT_DONE) stamp(s, 5, t0);
-  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[4] += g_flops;
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[5] += g_flops;
 }
 
 // ||A x - b|| / ||b|| in TT arithmetic (common.py:118-146): the residual
@@ -1219,7 +1312,7 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_check_kernel(ttx_slot* slo
   check_body(s);
   __syncthreads();
   if (s->status == TTX_SLOT_RUNNING || s->status == TTX_SLOT_DONE) stamp(s, 6, t0);
-  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[5] += g_flops;
+  if (threadIdx.x == 0 && g_flops != 0.0) s->flops[6] += g_flops;
 }
 
 // the stopping test and the next phase after a completed sweep (thread 0)
@@ -1348,8 +1441,9 @@ extern "C" int ttx_engine_plan(ttx_slot* s) {
 static int engine_prepare() {
   static bool attr = false;
   if (!attr) {
-    const void* fns[4] = {(const void*)eng_chol_kernel, (const void*)eng_lu_kernel,
-                          (const void*)eng_post_kernel, (const void*)eng_check_kernel};
+    const void* fns[5] = {(const void*)eng_chol32_kernel, (const void*)eng_chol_kernel,
+                          (const void*)eng_lu_kernel, (const void*)eng_post_kernel,
+                          (const void*)eng_check_kernel};
     for (const void* f : fns)
       if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kEngLds) !=
           hipSuccess)
@@ -1359,15 +1453,16 @@ static int engine_prepare() {
   return TTX_OK;
 }
 
-// kernel i of a tick: env, solve (rhs + MINRES), Cholesky, LU, post, check
+// kernel i of a tick: env, solve (rhs + MINRES), FP32 Cholesky, Cholesky, LU, post, check
 static void launch_stage(int i, hipStream_t st, ttx_slot* slots, int64_t nslots) {
   const dim3 g((unsigned)nslots), b(kEngThreads);
   switch (i) {
     case 0: hipLaunchKernelGGL(eng_env_kernel, dim3((unsigned)nslots, 5), b, 0, st, slots); break;
     case 1: hipLaunchKernelGGL(eng_solve_kernel, g, b, 0, st, slots); break;
-    case 2: hipLaunchKernelGGL(eng_chol_kernel, g, b, kEngLds, st, slots); break;
-    case 3: hipLaunchKernelGGL(eng_lu_kernel, g, b, kEngLds, st, slots); break;
-    case 4: hipLaunchKernelGGL(eng_post_kernel, g, b, kEngLds, st, slots); break;
+    case 2: hipLaunchKernelGGL(eng_chol32_kernel, g, b, kEngLds, st, slots); break;
+    case 3: hipLaunchKernelGGL(eng_chol_kernel, g, b, kEngLds, st, slots); break;
+    case 4: hipLaunchKernelGGL(eng_lu_kernel, g, b, kEngLds, st, slots); break;
+    case 5: hipLaunchKernelGGL(eng_post_kernel, g, b, kEngLds, st, slots); break;
     default: hipLaunchKernelGGL(eng_check_kernel, g, b, kEngLds, st, slots); break;
   }
 }

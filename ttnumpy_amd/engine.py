@@ -33,9 +33,9 @@ from .tensor_train import TensorTrain
 
 MAXD = 64
 MAXSWEEPS = 64
-NKERNELS = 6  # per tick: env, solve (rhs + MINRES), chol, lu, post, check
-KERNEL_NAMES = ("eng_env_kernel", "eng_solve_kernel", "eng_chol_kernel", "eng_lu_kernel",
-                "eng_post_kernel", "eng_check_kernel")
+NKERNELS = 7  # per tick: env, solve (rhs + MINRES), chol32, chol, lu, post, check
+KERNEL_NAMES = ("eng_env_kernel", "eng_solve_kernel", "eng_chol32_kernel", "eng_chol_kernel",
+                "eng_lu_kernel", "eng_post_kernel", "eng_check_kernel")
 SCR_COUNT = 17
 
 PH_XORTH, PH_ZORTH, PH_DONE = 1, 2, 14
@@ -54,7 +54,7 @@ class Slot(ctypes.Structure):
     _fields_ = (
         [(nm, _i64) for nm in ("d", "kick", "max_rank", "direct_size", "iterative", "residual_mode",
                                "max_sweeps", "track", "rmax", "nmax", "ramax", "rbmax", "bneg",
-                               "allow_fallback", "chunk", "chol")]
+                               "allow_fallback", "chunk", "chol", "mixed")]
         + [("n", _i64 * MAXD), ("ra", _i64 * (MAXD + 1)), ("rb", _i64 * (MAXD + 1))]
         + [(nm, _f64) for nm in ("residual_tol", "trunc_tol", "local_rtol", "damp", "error_threshold",
                                  "stagnation", "bnorm", "pad1")]
@@ -67,7 +67,8 @@ class Slot(ctypes.Structure):
         + [(nm, _i64) for nm in ("phase", "k", "sweep", "status", "stop_reason", "local_solves",
                                  "local_inexact", "local_fallbacks", "rank_decisions", "rank_capped",
                                  "red_last", "svd_noconv", "ticks", "minres_iters", "lu_solves",
-                                 "chol_fallbacks", "sstage", "chk_core", "chk_cdim")]
+                                 "chol_fallbacks", "mixed_solves", "mixed_refines",
+                                 "mixed_fallbacks", "sstage", "chk_core", "chk_cdim")]
         + [("flops", _f64 * NKERNELS)]
         + [("residuals", _f64 * MAXSWEEPS), ("max_bonds", _i64 * MAXSWEEPS)]
         + [("clk", _i64 * 24)]
@@ -110,6 +111,7 @@ class Request:
     local_rtol: Optional[float] = None
     symmetric: bool = False
     seed: Optional[int] = None
+    precision: str = "fp64"  # "mixed": direct Cholesky solves (N >= 256) in FP32 + FP64 refinement
     tag: object = None
     arrival: float = 0.0
     # filled by the engine
@@ -152,6 +154,9 @@ class Engine:
         self.kernel_launches = 0
         self.minres_iters = 0
         self.lu_solves = 0
+        self.mixed_solves = 0
+        self.mixed_refines = 0
+        self.mixed_fallbacks = 0
         self.stage_ms = np.zeros(24)  # per-stage device wall clock summed over retired requests
         self._push_all()
 
@@ -276,6 +281,9 @@ class Engine:
         s = self._plan(req)
         s.iterative = 1 if st.iterative_solver is not None else 0
         s.chol = 1 if req.symmetric else 0
+        if req.precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {req.precision!r}")
+        s.mixed = 1 if req.precision == "mixed" else 0
         s.residual_mode = 1 if st.residual_tol is not None else 0
         s.max_sweeps = st.max_sweeps
         s.track = 1 if st.track_residual else 0
@@ -342,6 +350,9 @@ class Engine:
         self.kernel_flops += np.array(list(s.flops))
         self.minres_iters += int(s.minres_iters)
         self.lu_solves += int(s.lu_solves)
+        self.mixed_solves += int(s.mixed_solves)
+        self.mixed_refines += int(s.mixed_refines)
+        self.mixed_fallbacks += int(s.mixed_fallbacks)
         self.stage_ms += np.array(list(s.clk), dtype=np.float64) * 1e-5  # 100 MHz ticks
         if s.status == SLOT_DONE:
             d = s.d
