@@ -99,6 +99,46 @@ def verify_result(A, b, req):
             "engine_reported": req.info.residuals[-1]}
 
 
+def c2_rounding(with_cpu: bool):
+    """C2: (x1 + x2).truncate(error=1e-10, max_rank=96) for two d=40, mode-4,
+    rank-64 normal trains; FP64 and mixed (FP32 absorb contractions) on the GPU,
+    the oracle on one CPU thread."""
+    import torch
+    import ttnumpy_amd as tx
+    from ttnumpy_amd import problems as P
+    cores1 = P.random_train_cores([4] * 40, 64, 1, normal=True)
+    cores2 = P.random_train_cores([4] * 40, 64, 2, normal=True)
+    s = tx.TensorTrain(cores1).add(tx.TensorTrain(cores2))
+    out = {}
+    spectra = {}
+    for prec in ("fp64", "mixed"):
+        s.truncate(error=1e-10, max_rank=96, precision=prec)  # warm
+        torch.cuda.synchronize()
+        sp = []
+        t0 = time.perf_counter()
+        o = s.truncate(error=1e-10, max_rank=96, precision=prec, spectra=sp)
+        torch.cuda.synchronize()
+        out[prec] = {"ms": 1e3 * (time.perf_counter() - t0), "bonds_max": max(o.bonds)}
+        spectra[prec] = ([x.cpu().numpy() for x in sp], o.bonds)
+    b64 = spectra["fp64"][1]
+    worst = 0.0
+    for i, (a, b) in enumerate(zip(spectra["mixed"][0], spectra["fp64"][0])):
+        kept = b64[len(b64) - 2 - i]
+        worst = max(worst, float(np.max(np.abs(a[:kept] - b[:kept]) / b[:kept])))
+    res = {"gpu_fp64_ms": out["fp64"]["ms"], "gpu_mixed_ms": out["mixed"]["ms"],
+           "bonds_identical": spectra["mixed"][1] == b64, "max_bond": max(b64),
+           "mixed_max_rel_singular_value_diff": worst}
+    if with_cpu:
+        from threadpoolctl import threadpool_limits
+        from oracle import tt_oracle as O
+        so = O.tt_add(O.Train(cores1), O.Train(cores2))
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            O.truncate(so, error=1e-10, max_rank=96)
+            res["cpu_oracle_1thread_ms"] = 1e3 * (time.perf_counter() - t0)
+    return res
+
+
 def cpu_baseline(budget_s: float = 12.0):
     """The oracle's AMEn (FP64 NumPy/SciPy) on the same C1 inputs, one BLAS thread."""
     from threadpoolctl import threadpool_limits
@@ -131,6 +171,7 @@ def main():
     ap.add_argument("--poll", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-c2", action="store_true", help="skip the C2 rounding measurement")
     ap.add_argument("--single", action="store_true",
                     help="also time one request alone (its launches are not part of the roofline)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -243,6 +284,11 @@ def main():
         "verify": verify,
         "mixed": mixed,
     }
+    if not args.no_c2:
+        try:
+            line["c2_rounding"] = c2_rounding(not args.no_cpu)
+        except Exception as exc:  # the C1 line is reported whatever happens here
+            line["c2_rounding"] = {"error": repr(exc)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

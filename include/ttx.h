@@ -56,6 +56,21 @@ int ttx_gemm_f64(ttx_stream_t stream, int64_t batch, int64_t M, int64_t N, int64
                  double* C, int64_t c_ms, int64_t c_ns, int64_t c_bs);
 
 /*
+ * ttx_gemm_f64's contraction in FP32: A / a_scale and B / b_scale are rounded
+ * to FP32, products accumulate in FP32 on v_mfma_f32_16x16x4f32, and
+ * C = alpha * a_scale * b_scale * (product) + beta * C in FP64.  The scales
+ * are device scalars (e.g. the operands' norms, so that a train's norm does
+ * not leave FP32's range); NULL, zero or non-finite means 1.  The "FP32
+ * contraction" of the mixed rounding form of TensorTrain.truncate (the
+ * absorb products of ttnumpy/utils.py:72-83).
+ */
+int ttx_gemm_f32acc(ttx_stream_t stream, int64_t batch, int64_t M, int64_t N, int64_t K,
+                    double alpha, const double* A, int64_t a_ms, int64_t a_ks, int64_t a_bs,
+                    const double* B, int64_t b_ks, int64_t b_ns, int64_t b_bs, double beta,
+                    double* C, int64_t c_ms, int64_t c_ns, int64_t c_bs, const double* a_scale,
+                    const double* b_scale);
+
+/*
  * dst[i0..i5] = alpha * src[i0..i5] over an ndim-dimensional box (ndim <= 6),
  * with independent element strides.  Permutes, slices, block placement and
  * zero padding of cores (ttnumpy/tensor_train.py:254-322,

@@ -292,3 +292,34 @@ def test_truncated_bond_kernel_known_answers():
         one = _t(np.array([1.0]))
         D.truncated_bond_dev(S, one, delta, mr, out, 1)
         assert int(out[1].item()) == O.truncated_bond(np.array(s), delta, mr), (s, delta, mr)
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (37, 65, 129), (200, 96, 512)])
+def test_gemm_f32acc_matches_fp32_product(M, N, K):
+    """FP32 contraction of FP64 arrays: equals the product of the FP32-rounded
+    operands to FP32 accumulation error, and the FP64 product to FP32 error."""
+    print(f"gemm f32acc {M}x{N}x{K}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C = D.empty(M * N)
+    from ttnumpy_amd import _lib
+    rc = D.lib().ttx_gemm_f32acc(D.stream(), 1, M, N, K, 1.0, D.P(_t(A.reshape(-1))), K, 1, 0,
+                                 D.P(_t(B.reshape(-1))), N, 1, 0, 0.0, D.P(C), N, 1, 0, None, None)
+    _lib.check(rc, "ttx_gemm_f32acc")
+    got = _h(C).reshape(M, N)
+    ref32 = A.astype(np.float32).astype(np.float64) @ B.astype(np.float32).astype(np.float64)
+    scale = np.abs(A) @ np.abs(B)
+    assert np.all(np.abs(got - ref32) <= 2 * K * 6e-8 * scale + 1e-30)
+    assert np.abs(got - A @ B).max() <= 4 * K * 6e-8 * scale.max()
+    # with device scales (operands far outside FP32's range) the result is the same
+    big = 1e45
+    sc = _t(np.array([np.linalg.norm(A) * big, np.linalg.norm(B)]))
+    rc = D.lib().ttx_gemm_f32acc(D.stream(), 1, M, N, K, 1.0, D.P(_t((A * big).reshape(-1))), K, 1,
+                                 0, D.P(_t(B.reshape(-1))), N, 1, 0, 0.0, D.P(C), N, 1, 0,
+                                 D.P(sc), D.P(sc, 1))
+    _lib.check(rc, "ttx_gemm_f32acc")
+    got2 = _h(C).reshape(M, N) / big
+    assert np.all(np.isfinite(got2))
+    assert np.abs(got2 - A @ B).max() <= 4 * K * 6e-8 * scale.max()

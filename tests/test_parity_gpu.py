@@ -208,3 +208,22 @@ def test_amen_c1_poisson256_converges_like_reference():
     assert info.residuals[-1] <= 1e-8 * 1.05
     assert abs(max(x.bonds) - max(g["bonds"])) <= 2
     assert _rel_diff(x, _oracle_solution(g)) < 1e-5
+
+
+def test_truncate_c2_mixed_precision_vs_fp64():
+    """C2 with FP32 absorb contractions and FP64 QR/SVD: the same bonds as the
+    FP64 path and kept singular values within FP32 contraction error."""
+    print("truncate C2 mixed vs fp64", flush=True)
+    s = _tt(P.random_train_cores([4] * 40, 64, 1, normal=True)).add(
+        _tt(P.random_train_cores([4] * 40, 64, 2, normal=True)))
+    sp64, sp32 = [], []
+    o64 = s.truncate(error=1e-10, max_rank=96, spectra=sp64)
+    o32 = s.truncate(error=1e-10, max_rank=96, spectra=sp32, precision="mixed")
+    assert o32.bonds == o64.bonds
+    worst = 0.0
+    for i, (a, b) in enumerate(zip(sp32, sp64)):
+        kept = o64.bonds[len(o64.bonds) - 2 - i]
+        a, b = _h(a)[:kept], _h(b)[:kept]
+        worst = max(worst, float(np.max(np.abs(a - b) / b)))
+    print("max relative singular value difference mixed vs fp64:", worst, flush=True)
+    assert worst < 1e-4

@@ -32,6 +32,8 @@ _SIGNATURES = {
     "ttx_version": (_i32, []),
     "ttx_gemm_f64": (_i32, [_p, _i64, _i64, _i64, _i64, _f64, _p, _i64, _i64, _i64, _p, _i64,
                             _i64, _i64, _f64, _p, _i64, _i64, _i64]),
+    "ttx_gemm_f32acc": (_i32, [_p, _i64, _i64, _i64, _i64, _f64, _p, _i64, _i64, _i64, _p, _i64,
+                            _i64, _i64, _f64, _p, _i64, _i64, _i64, _p, _p]),
     "ttx_copy_f64": (_i32, [_p, _i32, _p, _p, _p, _p, _p, _f64]),
     "ttx_fill_f64": (_i32, [_p, _p, _i64, _f64]),
     "ttx_axpby_f64": (_i32, [_p, _i64, _f64, _p, _f64, _p, _p]),

@@ -170,6 +170,106 @@ using namespace ttx;
 
 extern "C" int ttx_version(void) { return 100; }
 
+// FP32 contraction of FP64 arrays: operands rounded to FP32 when staged,
+// products accumulated in FP32 on v_mfma_f32_16x16x4f32 (output slot (lane l,
+// element e) = row 4 (l/16) + e, column l % 16), the result widened to FP64.
+// Same tiling and strides as gemm_f64_kernel.
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void gemm_f32acc_kernel(GemmArgs p, const double* a_scale,
+                                                          const double* b_scale) {
+  __shared__ float As[GBK][GBM + 1];
+  __shared__ float Bs[GBK][GBN + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int64_t b = blockIdx.z;
+  const int64_t m0 = (int64_t)blockIdx.y * GBM;
+  const int64_t n0 = (int64_t)blockIdx.x * GBN;
+  const double* A = p.A + b * p.a_bs;
+  const double* B = p.B + b * p.b_bs;
+  double* C = p.C + b * p.c_bs;
+  const bool a_m_fast = p.a_ms <= p.a_ks;
+  const bool b_n_fast = p.b_ns <= p.b_ks;
+  const int wm = (wave >> 1) * 32;
+  const int wn = (wave & 1) * 32;
+  // operands are divided by their scales before the rounding to FP32 (so that
+  // large trains stay inside FP32's range) and the product multiplied back
+  const double sa = (a_scale && *a_scale > 0.0 && isfinite(*a_scale)) ? *a_scale : 1.0;
+  const double sb = (b_scale && *b_scale > 0.0 && isfinite(*b_scale)) ? *b_scale : 1.0;
+  const double ia = 1.0 / sa, ib = 1.0 / sb;
+  float4_t acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+  for (int64_t k0 = 0; k0 < p.K; k0 += GBK) {
+#pragma unroll
+    for (int e0 = 0; e0 < GBM * GBK; e0 += 256) {
+      const int e = e0 + tid;
+      int mm, kk;
+      if (a_m_fast) { mm = e % GBM; kk = e / GBM; } else { kk = e % GBK; mm = e / GBK; }
+      const int64_t gm = m0 + mm, gk = k0 + kk;
+      As[kk][mm] = (gm < p.M && gk < p.K) ? (float)(A[gm * p.a_ms + gk * p.a_ks] * ia) : 0.0f;
+    }
+#pragma unroll
+    for (int e0 = 0; e0 < GBN * GBK; e0 += 256) {
+      const int e = e0 + tid;
+      int nn, kk;
+      if (b_n_fast) { nn = e % GBN; kk = e / GBN; } else { kk = e % GBK; nn = e / GBK; }
+      const int64_t gn = n0 + nn, gk = k0 + kk;
+      Bs[kk][nn] = (gn < p.N && gk < p.K) ? (float)(B[gk * p.b_ks + gn * p.b_ns] * ib) : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GBK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      const int c = lane & 15;
+      const float a0 = As[kr][wm + c];
+      const float a1 = As[kr][wm + 16 + c];
+      const float b0 = Bs[kr][wn + c];
+      const float b1 = Bs[kr][wn + 16 + c];
+      acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc00, 0, 0, 0);
+      acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc01, 0, 0, 0);
+      acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc10, 0, 0, 0);
+      acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc11, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int col = lane & 15;
+  const int rq = lane >> 4;
+  float4_t* accs[4] = {&acc00, &acc01, &acc10, &acc11};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t gn = n0 + wn + 16 * (q & 1) + col;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t gm = m0 + wm + 16 * (q >> 1) + 4 * rq + i;
+      if (gm < p.M && gn < p.N) {
+        double* dst = C + gm * p.c_ms + gn * p.c_ns;
+        const double v = p.alpha * sa * sb * (double)(*accs[q])[i];
+        *dst = (p.beta == 0.0) ? v : v + p.beta * (*dst);
+      }
+    }
+  }
+}
+
+extern "C" int ttx_gemm_f32acc(ttx_stream_t stream, int64_t batch, int64_t M, int64_t N, int64_t K,
+                               double alpha, const double* A, int64_t a_ms, int64_t a_ks,
+                               int64_t a_bs, const double* B, int64_t b_ks, int64_t b_ns,
+                               int64_t b_bs, double beta, double* C, int64_t c_ms, int64_t c_ns,
+                               int64_t c_bs, const double* a_scale, const double* b_scale) {
+  if (batch < 0 || M < 0 || N < 0 || K < 0) return TTX_EINVAL;
+  if (batch == 0 || M == 0 || N == 0) return TTX_OK;
+  if (batch > 65535 || (M + GBM - 1) / GBM > 65535) return TTX_EINVAL;
+  if (K == 0) {
+    int64_t dims[3] = {batch, M, N};
+    int64_t ds[3] = {c_bs, c_ms, c_ns};
+    return ttx_copy_f64(stream, 3, dims, C, ds, C, ds, beta);
+  }
+  GemmArgs p{M, N, K, alpha, beta, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, C, c_ms, c_ns, c_bs};
+  dim3 grid((unsigned)((N + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM), (unsigned)batch);
+  hipLaunchKernelGGL(gemm_f32acc_kernel, grid, dim3(256), 0, (hipStream_t)stream, p, a_scale,
+                     b_scale);
+  return launch_status();
+}
+
 extern "C" int ttx_gemm_f64(ttx_stream_t stream, int64_t batch, int64_t M, int64_t N, int64_t K,
                             double alpha, const double* A, int64_t a_ms, int64_t a_ks,
                             int64_t a_bs, const double* B, int64_t b_ks, int64_t b_ns,

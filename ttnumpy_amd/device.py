@@ -75,6 +75,57 @@ def gemm(batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, beta, 
     check(rc, "ttx_gemm_f64")
 
 
+# Precision of the absorb contractions (``absorb_left``/``absorb_right``):
+# "fp64", or "fp32" for the mixed rounding form (operands rounded to FP32,
+# FP32 accumulation on MFMA, FP64 storage).  Set through ``contractions``.
+_CONTRACT = ["fp64"]
+
+
+class contractions:
+    """Context manager selecting the absorb-contraction precision."""
+
+    def __init__(self, precision: str):
+        if precision not in ("fp64", "fp32"):
+            raise ValueError(f"contraction precision must be 'fp64' or 'fp32', got {precision!r}")
+        self.precision = precision
+
+    def __enter__(self):
+        self.saved = _CONTRACT[0]
+        _CONTRACT[0] = self.precision
+        return self
+
+    def __exit__(self, *exc):
+        _CONTRACT[0] = self.saved
+        return False
+
+
+def gemm_contract(batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, beta, C, c_ms,
+                  c_ns, c_bs, a_count: int = 0, b_count: int = 0):
+    """``gemm`` in the precision selected by ``contractions``.  In FP32 the
+    operands are scaled by their norms over their ``a_count``/``b_count``
+    contiguous elements (0: unscaled) so the rounding stays in FP32's range."""
+    if _CONTRACT[0] == "fp64":
+        return gemm(batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, beta, C, c_ms,
+                    c_ns, c_bs)
+    scales = empty(2)
+    sa = sb = None
+    if a_count:
+        check(lib().ttx_nrm2_f64(stream(), a_count, A, P(scales)), "ttx_nrm2_f64")
+        sa = P(scales)
+    if b_count:
+        check(lib().ttx_nrm2_f64(stream(), b_count, B, P(scales, 1)), "ttx_nrm2_f64")
+        sb = P(scales, 1)
+    rc = lib().ttx_gemm_f32acc(stream(), batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns,
+                               b_bs, beta, C, c_ms, c_ns, c_bs, sa, sb)
+    check(rc, "ttx_gemm_f32acc")
+    _KEEP_SCALES.append(scales)
+    if len(_KEEP_SCALES) > 64:
+        del _KEEP_SCALES[:32]
+
+
+_KEEP_SCALES = []  # scale buffers stay alive until their kernels have long been enqueued
+
+
 def copy_nd(dims, src, src_strides, dst, dst_strides, alpha=1.0):
     nd = len(dims)
     arr = (ctypes_i64 * nd)
@@ -287,8 +338,8 @@ def absorb_left(Mx: int, m: int, r: int, core: torch.Tensor, out: Optional[torch
     rest = core.numel() // core.shape[0]
     if out is None:
         out = empty((m,) + tuple(core.shape[1:]))
-    gemm(1, m, rest, r, 1.0, Mx, r if m_rs is None else m_rs, m_cs, 0, P(core), rest, 1, 0, 0.0,
-         P(out), rest, 1, 0)
+    gemm_contract(1, m, rest, r, 1.0, Mx, r if m_rs is None else m_rs, m_cs, 0, P(core), rest, 1,
+                  0, 0.0, P(out), rest, 1, 0, a_count=m * r, b_count=core.numel())
     return out
 
 
@@ -297,8 +348,8 @@ def absorb_right(core: torch.Tensor, Mx: int, r: int, m: int, m_rs: Optional[int
     """``core.reshape(-1, r) @ (r x m)`` (``utils.py:79-83``)."""
     rows = core.numel() // r
     out = empty(tuple(core.shape[:-1]) + (m,))
-    gemm(1, rows, m, r, 1.0, P(core), r, 1, 0, Mx, m if m_rs is None else m_rs, m_cs, 0, 0.0,
-         P(out), m, 1, 0)
+    gemm_contract(1, rows, m, r, 1.0, P(core), r, 1, 0, Mx, m if m_rs is None else m_rs, m_cs, 0,
+                  0.0, P(out), m, 1, 0, a_count=core.numel(), b_count=r * m)
     return out
 
 

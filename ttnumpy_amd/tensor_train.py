@@ -285,8 +285,15 @@ class TensorTrain:
 
     def truncate(self, error: float = 0.0, max_rank: Optional[int] = None, canonical: str = "auto",
                  inplace: bool = False, method: str = "direct", seed: Optional[int] = None,
-                 options: Optional[dict] = None, spectra=None) -> "TensorTrain":
-        """Round to minimal ranks: QR sweep then SVD sweep (``tensor_train.py:852-940``)."""
+                 options: Optional[dict] = None, spectra=None,
+                 precision: str = "fp64") -> "TensorTrain":
+        """Round to minimal ranks: QR sweep then SVD sweep (``tensor_train.py:852-940``).
+
+        ``precision="mixed"`` runs the absorb contractions of both sweeps in
+        FP32 (FP32 MFMA, FP32 accumulation) and keeps the QR and SVD in FP64.
+        """
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
         if canonical not in ("auto", "left", "right"):
             raise ValueError(f"Canonical form must be 'left', 'right' or 'auto', but got {canonical}")
         if method not in ("direct", "randomized"):
@@ -299,8 +306,9 @@ class TensorTrain:
         form = "left" if canonical == "right" else "right"
         if canonical == "auto":
             form = "left" if (t.orth_center is None or t.orth_center >= t.n_cores / 2) else "right"
-        t.orthonormalize(canonical=form, inplace=True)
-        t.compress_svd(error, max_rank, inplace=True, spectra=spectra)
+        with D.contractions("fp32" if precision == "mixed" else "fp64"):
+            t.orthonormalize(canonical=form, inplace=True)
+            t.compress_svd(error, max_rank, inplace=True, spectra=spectra)
         return t
 
     # -- arithmetic used by the residual check
