@@ -50,6 +50,7 @@ typedef struct ttx_slot {
   int64_t rmax, nmax, ramax, rbmax, bneg, allow_fallback, chunk;
   int64_t chol;  // 1: direct solves try Cholesky first (symmetric problems), LU if not SPD
   int64_t mixed; // 1: direct Cholesky solves of size >= 256 factor in FP32 with FP64 refinement
+  int64_t minres_slice; // MINRES iterations per tick (a long local solve spans ticks)
   int64_t n[TTX_MAXD];
   int64_t ra[TTX_MAXD + 1];
   int64_t rb[TTX_MAXD + 1];

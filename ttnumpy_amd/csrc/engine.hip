@@ -34,11 +34,10 @@ constexpr size_t kEngLds = 75 * 1024;
 
 // local solve in flight (slot.sstage): a direct solve is handed from the
 // solve kernel to the Cholesky or LU kernel of the same tick; MINRES resumes
-// over ticks, kMinresSlice iterations at a time, so that a long solve of one
-// request does not hold up the tick of every other request
+// over ticks, slot.minres_slice iterations at a time, so that a long solve of
+// one request does not hold up the tick of every other request
 enum { kSolveNone = 0, kSolveChol = 1, kSolveLu = 2, kSolveMinres = 3, kSolveChol32 = 4 };
 constexpr int64_t kMixedMinN = 256;  // smaller direct solves stay in FP64
-constexpr int64_t kMinresSlice = 128;  // MINRES iterations per tick for one slot
 
 __device__ __forceinline__ double* xcore(const ttx_slot* s, int64_t k) { return s->x + k * s->cap_core; }
 __device__ __forceinline__ double* zcore(const ttx_slot* s, int64_t k) { return s->z + k * s->cap_zcore; }
@@ -760,8 +759,8 @@ __device__ void solve_body(ttx_slot* s) {
     minres_start(op, N, bloc, xcore(s, k), V, red, st);
   }
   // restarted MINRES with the true-residual check (common.py:776-800), at most
-  // kMinresSlice iterations in this tick
-  int64_t budget = kMinresSlice;
+  // slot.minres_slice iterations in this tick
+  int64_t budget = s->minres_slice > 0 ? s->minres_slice : 128;
   while (true) {
     budget -= minres_run(op, N, V, red, st, budget, threadIdx.x == 0 ? &s->minres_iters : nullptr);
     if (st[kMrIstop] == 0.0) break;  // out of budget: resume next tick

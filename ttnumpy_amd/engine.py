@@ -54,7 +54,7 @@ class Slot(ctypes.Structure):
     _fields_ = (
         [(nm, _i64) for nm in ("d", "kick", "max_rank", "direct_size", "iterative", "residual_mode",
                                "max_sweeps", "track", "rmax", "nmax", "ramax", "rbmax", "bneg",
-                               "allow_fallback", "chunk", "chol", "mixed")]
+                               "allow_fallback", "chunk", "chol", "mixed", "minres_slice")]
         + [("n", _i64 * MAXD), ("ra", _i64 * (MAXD + 1)), ("rb", _i64 * (MAXD + 1))]
         + [(nm, _f64) for nm in ("residual_tol", "trunc_tol", "local_rtol", "damp", "error_threshold",
                                  "stagnation", "bnorm", "pad1")]
@@ -125,7 +125,8 @@ class Request:
 class Engine:
     """A fixed set of slots on one GPU plus the continuous-batching loop."""
 
-    def __init__(self, nslots: int, rmax: int, chunk: int = 8, hbm_reserve_bytes: int = 2 << 30):
+    def __init__(self, nslots: int, rmax: int, chunk: int = 8, hbm_reserve_bytes: int = 2 << 30,
+                 minres_slice: int = 128):
         self.lib = _lib.load()
         _check_layout(self.lib)
         self.dev = D.device()
@@ -133,6 +134,9 @@ class Engine:
         self.rmax = int(rmax)
         self.chunk = int(chunk)
         self.reserve = int(hbm_reserve_bytes)
+        if minres_slice < 1:
+            raise ValueError("minres_slice must be positive")
+        self.minres_slice = int(minres_slice)
         self.host = (Slot * self.nslots)()
         for s in self.host:
             s.status = SLOT_EMPTY
@@ -336,6 +340,7 @@ class Engine:
         s.status = SLOT_RUNNING
         s.sstage = 0
         s.chk_core = -1
+        s.minres_slice = self.minres_slice
         ctypes.memmove(ctypes.addressof(self.host[i]), ctypes.addressof(s), ctypes.sizeof(Slot))
         self.slot_req[i] = req
         self.slot_mem[i] = tuple(keep)
@@ -451,6 +456,94 @@ class Engine:
                 self._push_all()
             if ticks > max_ticks:
                 raise RuntimeError("engine did not finish within the tick limit")
+        return done
+
+
+class EnginePool:
+    """Several engines served concurrently, each on its own HIP stream.
+
+    A tick advances every slot of one engine in lockstep, so a slot doing a
+    long step holds up the rest of its engine; requests of very different
+    sizes therefore go to different engines (``route(request) -> index``),
+    whose ticks run side by side on the GPU.  The host launches ``poll``
+    ticks plus an asynchronous copy of the slot descriptors on an engine's
+    stream and services whichever engine's copy has landed first: retire the
+    finished requests, admit arrived ones into free slots (within free HBM),
+    launch the next ticks.  Every operation of an engine runs on its stream.
+    """
+
+    def __init__(self, engines: Sequence[Engine], route):
+        self.engines = list(engines)
+        self.route = route
+        self.streams = [torch.cuda.Stream(device=e.dev) for e in self.engines]
+        self.events = [None] * len(self.engines)
+
+    def _launch(self, i: int, poll: int) -> None:
+        e, st = self.engines[i], self.streams[i]
+        with torch.cuda.stream(st):
+            e.tick(poll)
+            e.pinned.copy_(e.dev_slots, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        self.events[i] = ev
+
+    def run(self, queue: Sequence[Request], poll: int = 8, clock=None, on_done=None,
+            max_cycles: int = 10 ** 7) -> List[Request]:
+        """Serve ``queue`` (ordered by arrival); returns the requests as they finish."""
+        pending = [list() for _ in self.engines]
+        for r in queue:
+            pending[self.route(r)].append(r)
+        done: List[Request] = []
+        now = clock or (lambda: float("inf"))
+        cycles = 0
+        while any(pending) or any(ev is not None for ev in self.events):
+            progressed = False
+            for i, e in enumerate(self.engines):
+                ev = self.events[i]
+                if ev is not None:
+                    if not ev.query():
+                        continue
+                    self.events[i] = None
+                    with torch.cuda.stream(self.streams[i]):
+                        e._host_bytes()[:] = e.pinned.numpy()
+                        retired = False
+                        for j in range(e.nslots):
+                            if e.slot_req[j] is not None and e.host[j].status != SLOT_RUNNING:
+                                req = e.slot_req[j]
+                                e._retire(j, push=False)
+                                retired = True
+                                done.append(req)
+                                if on_done is not None:
+                                    on_done(req)
+                        if retired:
+                            e._push_all()
+                    progressed = True
+                with torch.cuda.stream(self.streams[i]):
+                    q = pending[i]
+                    if q and q[0].arrival <= now():
+                        free = [j for j in range(e.nslots) if e.slot_req[j] is None]
+                        admitted = False
+                        if free:
+                            free_b, _ = torch.cuda.mem_get_info(e.dev)
+                            while q and free and q[0].arrival <= now():
+                                j = free[0]
+                                held = e.arena[j].numel() * 8 if e.arena[j] is not None else 0
+                                need = max(e.slot_bytes_needed(q[0]) - held, 0)
+                                if free_b - need < e.reserve and e.active():
+                                    break
+                                e.admit(free.pop(0), q.pop(0), push=False)
+                                free_b -= need
+                                admitted = True
+                        if admitted:
+                            e._push_all()
+                    if e.active():
+                        self._launch(i, poll)
+                        progressed = True
+            cycles += 1
+            if cycles > max_cycles:
+                raise RuntimeError("engine pool did not finish within the cycle limit")
+            if not progressed:
+                time.sleep(0.0002)
         return done
 
 

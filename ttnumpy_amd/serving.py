@@ -9,9 +9,10 @@ by AMEn (``symmetric=True``, ``residual_tol``, ``max_sweeps``) from
 ``x0 = rand([2]*2L, ranks=2, seed=i)``.
 
 The engine admits a request once it has arrived, a slot is free and free HBM
-covers the slot's arena; requests of different sizes run side by side and
-every tick advances all of them by one step of their own sweep (the same
-site-step kernels serve every request in one launch per kernel).  Latency is
+covers the slot's arena; every tick advances all of its requests by one step
+of their own sweep (the same site-step kernels serve every request in one
+launch per kernel).  With ``per_level`` each grid level has its own engine on
+its own stream, so that small requests do not wait on the ticks of large ones.  Latency is
 measured from arrival to result.  Operators and right-hand sides are built
 and copied to the device before the clock starts, as a client would hand
 them over.
@@ -28,7 +29,7 @@ import torch
 
 from . import device as D
 from . import problems as P
-from .engine import Engine, Request
+from .engine import Engine, EnginePool, Request
 from .tensor_train import TensorTrain, rand
 
 
@@ -41,9 +42,11 @@ class ServingConfig:
     seed: int = 0
     residual_tol: float = 1e-8
     max_sweeps: int = 20
-    nslots: int = 128
+    nslots: int = 128             # per engine
     rmax: int = 64
     poll: int = 16
+    per_level: bool = True        # one engine (own stream) per grid level
+    minres_slice: int = 128       # MINRES iterations per tick
 
 
 @dataclass
@@ -106,9 +109,18 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
     items = request_stream(cfg)
     if requests is None:
         requests = build_requests(cfg, items)
-    eng = engine or Engine(cfg.nslots, rmax=cfg.rmax)
+    levels = sorted(set(int(x) for x in cfg.levels))
+    if engine is not None:
+        engines = [engine]
+        pool = None
+    elif cfg.per_level:
+        engines = [Engine(cfg.nslots, rmax=cfg.rmax, minres_slice=cfg.minres_slice) for _ in levels]
+        pool = EnginePool(engines, lambda r: levels.index(r.tag.level))
+    else:
+        engines = [Engine(cfg.nslots, rmax=cfg.rmax, minres_slice=cfg.minres_slice)]
+        pool = None
     torch.cuda.synchronize()
-    flops0 = float(eng.kernel_flops.sum())
+    flops0 = sum(float(e.kernel_flops.sum()) for e in engines)
     t0 = time.perf_counter()
     for r, it in zip(requests, items):
         r.arrival = t0 + it.offset
@@ -120,8 +132,11 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
         if progress is not None:
             progress(count[0], r)
 
-    done = eng.run(sorted(requests, key=lambda r: r.arrival), poll=cfg.poll, clock=time.perf_counter,
-                   on_done=on_done)
+    queue = sorted(requests, key=lambda r: r.arrival)
+    if pool is not None:
+        done = pool.run(queue, poll=cfg.poll, clock=time.perf_counter, on_done=on_done)
+    else:
+        done = engines[0].run(queue, poll=cfg.poll, clock=time.perf_counter, on_done=on_done)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     lat = np.array([r.finished - r.arrival for r in done])
@@ -138,7 +153,7 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
     for L, e in by_level.items():
         e["p50_ms"] = float(np.percentile(e.pop("lat_ms"), 50))
         e["mean_sweeps"] = float(np.mean(e.pop("sweeps")))
-    gflop = (float(eng.kernel_flops.sum()) - flops0) / 1e9
+    gflop = (sum(float(e.kernel_flops.sum()) for e in engines) - flops0) / 1e9
     rep = ServingReport(
         solves=len(ok), errors=len(done) - len(ok), wall_s=wall, solves_per_s=len(ok) / wall,
         p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
