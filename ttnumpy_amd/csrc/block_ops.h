@@ -382,15 +382,178 @@ struct SvdArgs {
   int32_t* info;
 };
 
-__device__ __forceinline__ int64_t rr_player(int64_t t, int64_t round, int64_t qq) {
-  return t == 0 ? 0 : 1 + (t - 1 + round) % (qq - 1);
+
+
+// One rotation of the one-sided Jacobi method on columns i < j of G (and of
+// the accumulated V) by the 16 lanes of a DPP row; `sub` is the lane within
+// the row.  Element r of a column is handled by lane r % 16, so the sums and
+// hence the result do not depend on which row or workgroup size runs it.
+// Returns true when the pair was rotated.  With REGS, columns of up to
+// 16 * kJacRegs elements are held in registers between the sums and the
+// rotation (the same operations in the same order, so the same bits).
+constexpr int kJacRegs = 8;  // column elements per lane held in registers
+// the rotated pair (cs x - sn y, sn x + cs y) with the roundings spelled out,
+// so that every code path that rotates produces the same bits
+__device__ __forceinline__ double jac_rot_lo(double cs, double sn, double x, double y) {
+  return __fma_rn(cs, x, -__dmul_rn(sn, y));
+}
+__device__ __forceinline__ double jac_rot_hi(double cs, double sn, double x, double y) {
+  return __fma_rn(sn, x, __dmul_rn(cs, y));
+}
+// Rotation of a column pair with squared norms a, b and inner product c
+// (Rutishauser's formulas, every rounding explicit); false when the pair is
+// orthogonal to tol.
+__device__ __forceinline__ bool jac_params(double a, double b, double c, double tol, double& cs,
+                                           double& sn) {
+  if (c == 0.0 || a == 0.0 || b == 0.0) return false;
+  if (fabs(c) <= __dmul_rn(__dmul_rn(tol, sqrt(a)), sqrt(b))) return false;
+  const double zeta = (b - a) / (2.0 * c);
+  const double tn = fabs(zeta) > 1e150
+                        ? 0.5 / zeta
+                        : (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(__fma_rn(zeta, zeta, 1.0)));
+  cs = 1.0 / sqrt(__fma_rn(tn, tn, 1.0));
+  sn = __dmul_rn(cs, tn);
+  return true;
 }
 
+template <bool REGS>
+__device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, double* vj, int Q,
+                                            int sub, double tol) {
+  double a = 0.0, b = 0.0, c = 0.0;
+  if (REGS && Q <= 16 * kJacRegs) {
+    // loads at clamped (in-bounds) indices, zeros selected past the end
+    double x[kJacRegs], y[kJacRegs], u[kJacRegs], w[kJacRegs];
+#pragma unroll
+    for (int t = 0; t < kJacRegs; ++t) {
+      const int r = sub + 16 * t;
+      const int rc = r < Q ? r : Q - 1;
+      const bool in = r < Q;
+      const double gx = gi[rc], gy = gj[rc], vx = vi[rc], vy = vj[rc];
+      x[t] = in ? gx : 0.0;
+      y[t] = in ? gy : 0.0;
+      u[t] = in ? vx : 0.0;
+      w[t] = in ? vy : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < kJacRegs; ++t) {
+      a = __fma_rn(x[t], x[t], a);
+      b = __fma_rn(y[t], y[t], b);
+      c = __fma_rn(x[t], y[t], c);
+    }
+    a = row16_sum(a);
+    b = row16_sum(b);
+    c = row16_sum(c);
+    double cs, sn;
+    if (!jac_params(a, b, c, tol, cs, sn)) return false;
+#pragma unroll
+    for (int t = 0; t < kJacRegs; ++t) {
+      const int r = sub + 16 * t;
+      if (r < Q) {
+        gi[r] = jac_rot_lo(cs, sn, x[t], y[t]);
+        gj[r] = jac_rot_hi(cs, sn, x[t], y[t]);
+        vi[r] = jac_rot_lo(cs, sn, u[t], w[t]);
+        vj[r] = jac_rot_hi(cs, sn, u[t], w[t]);
+      }
+    }
+    return true;
+  }
+  // four elements per lane per step, loads first (same order of operations)
+  for (int r0 = sub; r0 < Q; r0 += 64) {
+    double x[4], y[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int r = r0 + 16 * t;
+      const int rc = r < Q ? r : Q - 1;
+      const double gx = gi[rc], gy = gj[rc];
+      x[t] = r < Q ? gx : 0.0;
+      y[t] = r < Q ? gy : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      a = __fma_rn(x[t], x[t], a);
+      b = __fma_rn(y[t], y[t], b);
+      c = __fma_rn(x[t], y[t], c);
+    }
+  }
+  a = row16_sum(a);
+  b = row16_sum(b);
+  c = row16_sum(c);
+  double cs, sn;
+  if (!jac_params(a, b, c, tol, cs, sn)) return false;
+  for (int r0 = sub; r0 < Q; r0 += 64) {
+    double x[4], y[4], u[4], w[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int r = r0 + 16 * t;
+      const int rc = r < Q ? r : Q - 1;
+      x[t] = gi[rc];
+      y[t] = gj[rc];
+      u[t] = vi[rc];
+      w[t] = vj[rc];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int r = r0 + 16 * t;
+      if (r < Q) {
+        gi[r] = jac_rot_lo(cs, sn, x[t], y[t]);
+        gj[r] = jac_rot_hi(cs, sn, x[t], y[t]);
+        vi[r] = jac_rot_lo(cs, sn, u[t], w[t]);
+        vj[r] = jac_rot_hi(cs, sn, u[t], w[t]);
+      }
+    }
+  }
+  return true;
+}
+
+// Sweeps of the one-sided Jacobi method on the Q columns of G (Q x Q,
+// column-major) with the rotations accumulated into V, until a sweep
+// rotates nothing (true) or 80 sweeps (false).  Each round of the
+// round-robin schedule pairs every column once; a 16-lane DPP row takes one
+// pair.  `flag` is a block-shared int.  Ends synced.
+template <bool REGS>
+__device__ __forceinline__ bool jacobi_sweeps(double* G, double* V, int Q, int* flag) {
+  const int tid = threadIdx.x;
+  const int sub = tid & 15;
+  const int row = tid >> 4, nrow = blockDim.x >> 4;
+  const int QQ = Q + (Q & 1);
+  const double tol = (double)Q * kEps;
+  if (Q < 2) return true;
+  for (int sweep = 0; sweep < 80; ++sweep) {
+    if (tid == 0) *flag = 0;
+    __syncthreads();
+    for (int round = 0; round < QQ - 1; ++round) {
+      for (int pk = row; pk < QQ / 2; pk += nrow) {
+        // round-robin: player 0 stays, the others rotate by one per round
+        int i = 0, j;
+        if (pk != 0) {
+          i = pk - 1 + round;
+          if (i >= QQ - 1) i -= QQ - 1;
+          i += 1;
+        }
+        j = QQ - 2 - pk + round;
+        if (j >= QQ - 1) j -= QQ - 1;
+        j += 1;
+        if (i >= Q || j >= Q) continue;
+        if (i > j) { const int t = i; i = j; j = t; }
+        if (jacobi_pair<REGS>(G + i * Q, G + j * Q, V + i * Q, V + j * Q, Q, sub, tol) && sub == 0)
+          *flag = 1;
+      }
+      __syncthreads();
+    }
+    const bool done = *flag == 0;
+    __syncthreads();
+    if (done) return true;
+  }
+  return false;
+}
 
 // Thin SVD of one matrix by the whole block: QR of the tall side, then
-// one-sided Jacobi on the square factor (round-robin pairs, one wave per
-// pair), columns sorted by descending singular value.  `work` holds
-// 2*P*Q + 2*Q*Q + 3*Q doubles (P >= Q the long and short sides); it may be LDS.
+// one-sided Jacobi on the square factor (round-robin pairs, one 16-lane DPP
+// row per pair, the square factor in LDS when `lds` holds it), columns sorted
+// by descending singular value, and the tall factor's product on MFMA.
+// `work` holds 2*P*Q + 2*Q*Q + 3*Q doubles (P >= Q the long and short
+// sides); it may be LDS.  REGS: see jacobi_pair.  Ends synced.
+template <bool REGS = true>
 __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double* red,
                                           double* lds = nullptr, int64_t lds_cap = 0) {
   __shared__ int s_rot;
@@ -407,11 +570,11 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   const int64_t x_cs = wide ? p.a_rs : p.a_cs;
   // Jacobi on the Q x Q factor and U = Q_x U_r (the QR of the tall side counts itself)
   add_flops(20.0 * (double)Q * (double)Q * (double)Q + 2.0 * (double)P * (double)Q * (double)Q);
-  double* Wq = work;              // P x Q   (QR working matrix)
+  double* Wq = work;              // P x Q   (QR working matrix; later the sorted factor)
   double* tau = Wq + P * Q;       // Q
   double* Qx = tau + Q;           // P x Q   (orthonormal factor)
-  double* G = Qx + P * Q;         // Q x Q   (Jacobi matrix)
-  double* V = G + Q * Q;          // Q x Q   (accumulated rotations)
+  double* Gw = Qx + P * Q;        // Q x Q   (Jacobi matrix, when not in LDS)
+  double* V = Gw + Q * Q;         // Q x Q   (accumulated rotations)
   double* sig = V + Q * Q;        // Q
   double* ordd = sig + Q;         // Q       (sort permutation, stored as doubles)
 
@@ -429,8 +592,10 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
     return;
   }
   const bool use_qr = P > Q;
+  if (use_qr) block_householder_qr(P, Q, Wq, tau, Qx, red, true, lds, lds_cap);
+  // the QR is done with the LDS: the Jacobi matrix moves there if it fits
+  double* G = (lds != nullptr && Q * Q <= lds_cap) ? lds : Gw;
   if (use_qr) {
-    block_householder_qr(P, Q, Wq, tau, Qx, red, true, lds, lds_cap);
     for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
       const int64_t c = q32(e, Q), i = e - c * Q;
       G[e] = (i <= c) ? Wq[i + c * P] : 0.0;
@@ -441,62 +606,11 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   for (int64_t e = tid; e < Q * Q; e += blockDim.x) V[e] = (e == q32(e, Q) * (Q + 1)) ? 1.0 : 0.0;
   __syncthreads();
 
-  // one-sided Jacobi on the columns of G, round-robin pairs per round
-  const int64_t QQ = Q + (Q & 1);
-  const double tol = (double)Q * kEps;
-  int converged = (Q < 2);
-  for (int sweep = 0; sweep < 80 && !converged; ++sweep) {
-    if (tid == 0) s_rot = 0;
-    __syncthreads();
-    for (int64_t round = 0; round < QQ - 1; ++round) {
-      for (int64_t pk = wave; pk < QQ / 2; pk += nwave) {
-        int64_t i = rr_player(pk, round, QQ);
-        int64_t j = rr_player(QQ - 1 - pk, round, QQ);
-        if (i >= Q || j >= Q) continue;
-        if (i > j) { const int64_t t = i; i = j; j = t; }
-        double* gi = G + i * Q;
-        double* gj = G + j * Q;
-        double a = 0.0, b = 0.0, c = 0.0;
-        for (int64_t r = lane; r < Q; r += 64) {
-          const double x = gi[r], y = gj[r];
-          a += x * x;
-          b += y * y;
-          c += x * y;
-        }
-        a = wave_sum(a);
-        b = wave_sum(b);
-        c = wave_sum(c);
-        if (c == 0.0 || a == 0.0 || b == 0.0) continue;
-        if (fabs(c) <= tol * sqrt(a) * sqrt(b)) continue;
-        const double zeta = (b - a) / (2.0 * c);
-        double t;
-        if (fabs(zeta) > 1e150)
-          t = 0.5 / zeta;
-        else
-          t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        const double cs = 1.0 / sqrt(1.0 + t * t);
-        const double sn = cs * t;
-        for (int64_t r = lane; r < Q; r += 64) {
-          const double x = gi[r], y = gj[r];
-          gi[r] = cs * x - sn * y;
-          gj[r] = sn * x + cs * y;
-        }
-        double* vi = V + i * Q;
-        double* vj = V + j * Q;
-        for (int64_t r = lane; r < Q; r += 64) {
-          const double x = vi[r], y = vj[r];
-          vi[r] = cs * x - sn * y;
-          vj[r] = sn * x + cs * y;
-        }
-        if (lane == 0) s_rot = 1;
-      }
-      __syncthreads();
-    }
-    converged = (s_rot == 0);
-    __syncthreads();
-  }
+  // one-sided Jacobi on the columns of G; a separate call per location of G
+  // lets each access compile to an LDS or a global load
+  const int converged = G == lds ? jacobi_sweeps<REGS>(lds, V, (int)Q, &s_rot)
+                                 : jacobi_sweeps<REGS>(Gw, V, (int)Q, &s_rot);
   if (tid == 0) p.info[0] = converged ? TTX_OK : TTX_ENOCONV;
-  __syncthreads();
 
   // singular values = column norms
   for (int64_t j = wave; j < Q; j += nwave) {
@@ -518,30 +632,25 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
     ordd[pos] = (double)j;
   }
   __syncthreads();
-  // normalise the columns of G in place: G[:, j] /= sig[j]
+  // Gs (in Wq) column o = G[:, ord[o]] / sig[ord[o]]
+  double* Gs = Wq;
   for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
-    const int64_t j = q32(e, Q);
-    const double s = sig[j];
-    G[e] = s > 0.0 ? G[e] / s : 0.0;
+    const int64_t o = q32(e, Q), r = e - o * Q;
+    const int64_t src = (int64_t)ordd[o];
+    const double s = sig[src];
+    Gs[e] = s > 0.0 ? G[r + src * Q] / s : 0.0;
   }
   __syncthreads();
-  // Ux (P x Q) column o = (use_qr ? Qx : I) * G[:, ord[o]];  V column ord[o]
-  // tall: U_A = Ux, Vt_A = V^T.  wide: U_A = V, Vt_A = Ux^T.
-  for (int64_t e = tid; e < P * Q; e += blockDim.x) {
-    const int64_t o = q32(e, P), r = e - o * P;
-    const int64_t src = (int64_t)ordd[o];
-    double v;
-    if (use_qr) {
-      v = 0.0;
-      const double* gcol = G + src * Q;
-      for (int64_t l = 0; l < Q; ++l) v += Qx[r + l * P] * gcol[l];
-    } else {
-      v = G[r + src * Q];
+  // Ux (P x Q) = (use_qr ? Qx : I) Gs;  tall: U_A = Ux, Vt_A = V^T;  wide: U_A = V, Vt_A = Ux^T
+  double* ux = wide ? p.Vt : p.U;
+  const int64_t ux_rs = wide ? p.v_cs : p.u_rs, ux_cs = wide ? p.v_rs : p.u_cs;
+  if (use_qr) {
+    bgemm(1, P, Q, Q, 1.0, Qx, 1, P, 0, Gs, 1, Q, 0, 0.0, ux, ux_rs, ux_cs, 0, false);
+  } else {
+    for (int64_t e = tid; e < P * Q; e += blockDim.x) {
+      const int64_t o = q32(e, P), r = e - o * P;
+      ux[r * ux_rs + o * ux_cs] = Gs[r + o * Q];
     }
-    if (!wide)
-      p.U[r * p.u_rs + o * p.u_cs] = v;
-    else
-      p.Vt[o * p.v_rs + r * p.v_cs] = v;
   }
   for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
     const int64_t o = q32(e, Q), r = e - o * Q;
@@ -553,6 +662,7 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
       p.U[r * p.u_rs + o * p.u_cs] = v;
   }
   for (int64_t o = tid; o < Q; o += blockDim.x) p.S[o] = sig[(int64_t)ordd[o]];
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------

@@ -1061,7 +1061,7 @@ __device__ void post_body(ttx_slot* s) {
   if (threadIdx.x == 0) s_svd = TTX_OK;
   __syncthreads();
   SvdArgs sa{p, q, u, q, 1, U, K, 1, S, Vt, q, 1, scr(s, TTX_SCR_SVDW), &s_svd};
-  block_svd(sa, scr(s, TTX_SCR_SVDW), red, eng_lds, (int64_t)(kEngLds / 8));
+  block_svd<false>(sa, scr(s, TTX_SCR_SVDW), red, eng_lds, (int64_t)(kEngLds / 8));
   __syncthreads();
   stamp(s, 9, t0);
   if (s_svd == TTX_ENONFINITE) { fail(s, TTX_SLOT_ERR_NONFINITE); return; }

@@ -30,6 +30,27 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// One DPP lane move of a double within rows of 16 lanes (CTRL is a DPP16
+// control word: quad_perm, row_mirror, row_half_mirror).
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int mlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  const int mhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(mhi, mlo);
+}
+
+// Sum over the 16 lanes of a DPP row (lanes 16q..16q+15); every lane of the
+// row gets the same bits: each step adds a lane's value to its partner's,
+// and a + b == b + a exactly.
+__device__ __forceinline__ double row16_sum(double v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]: partner lane ^ 1
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]: partner lane ^ 2
+  v += dpp_move<0x141>(v);  // row_half_mirror: lane i <- 7 - i within 8
+  v += dpp_move<0x140>(v);  // row_mirror: lane i <- 15 - i within 16
+  return v;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
