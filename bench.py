@@ -139,6 +139,47 @@ def c2_rounding(with_cpu: bool):
     return res
 
 
+def c5_high_dim(with_cpu: bool):
+    """C5: AMEn on A = I + sum_i L_i (d=64, mode 8, rank-2 TTM, L = tridiag(-1,2,-1)),
+    b = rand([8]*64, ranks=16, seed=3), x0 = ones with b's bonds, max_rank=128,
+    residual_tol=1e-8, max_sweeps=30, symmetric, seed 0; the per-kernel device
+    path (MINRES local solves, multi-workgroup MFMA contractions).  Also the same
+    run with the stagnation stop off (30 sweeps), which drives the ranks up."""
+    import torch
+    import ttnumpy_amd as tx
+    from ttnumpy_amd import device as D
+    from ttnumpy_amd import problems as P
+    A_c = P.shifted_laplacian_sum_cores(64, 8, 1.0 / 64)
+    b_c = P.random_train_cores([8] * 64, 16, 3)
+    A, b = tx.TensorTrain(A_c), tx.TensorTrain(b_c)
+    res = {}
+    for name, stag in (("auto", "auto"), ("no_stagnation_stop", None)):
+        torch.cuda.synchronize()
+        f0 = D.flops()
+        t0 = time.perf_counter()
+        x, info = tx.amen(A, b, max_rank=128, residual_tol=1e-8, max_sweeps=30, symmetric=True,
+                          seed=0, return_info=True, stagnation_factor=stag)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        gf = (D.flops() - f0) / 1e9
+        res[name] = {"gpu_s": wall, "stop_reason": str(info.stop_reason),
+                     "sweeps": info.num_sweeps_run, "max_bond": max(x.bonds),
+                     "residual_fp64": info.residuals[-1], "local_solves": info.local_solves,
+                     "gflop": gf, "gflop_per_s": gf / wall}
+    if with_cpu:
+        from threadpoolctl import threadpool_limits
+        from oracle import tt_oracle as O
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            xo, io = O.amen(O.Train(A_c), O.Train(b_c), max_rank=128, residual_tol=1e-8,
+                            max_sweeps=30, symmetric=True, seed=0)
+            res["cpu_oracle_1thread"] = {"s": time.perf_counter() - t0,
+                                         "stop_reason": str(io.stop_reason),
+                                         "sweeps": io.num_sweeps_run, "max_bond": max(xo.bonds),
+                                         "residual_fp64": io.residuals[-1]}
+    return res
+
+
 def cpu_baseline(budget_s: float = 12.0):
     """The oracle's AMEn (FP64 NumPy/SciPy) on the same C1 inputs, one BLAS thread."""
     from threadpoolctl import threadpool_limits
@@ -172,6 +213,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the C2 rounding measurement")
+    ap.add_argument("--no-c5", action="store_true", help="skip the C5 high-dimensional measurement")
     ap.add_argument("--single", action="store_true",
                     help="also time one request alone (its launches are not part of the roofline)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -289,6 +331,11 @@ def main():
             line["c2_rounding"] = c2_rounding(not args.no_cpu)
         except Exception as exc:  # the C1 line is reported whatever happens here
             line["c2_rounding"] = {"error": repr(exc)}
+    if not args.no_c5:
+        try:
+            line["c5_high_dim"] = c5_high_dim(not args.no_cpu)
+        except Exception as exc:  # the C1 line is reported whatever happens here
+            line["c5_high_dim"] = {"error": repr(exc)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

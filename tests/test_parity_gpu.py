@@ -227,3 +227,31 @@ def test_truncate_c2_mixed_precision_vs_fp64():
         worst = max(worst, float(np.max(np.abs(a - b) / b)))
     print("max relative singular value difference mixed vs fp64:", worst, flush=True)
     assert worst < 1e-4
+
+
+def test_amen_c5_shape_small_matches_oracle():
+    """The C5 system at d = 6: mode 8, A = I + sum_i L_i (L = tridiag(-1, 2, -1))
+    as a rank-2 TTM, b = rand([8]*6, ranks=4, seed=3), x0 = ones, max_rank 64,
+    symmetric, seed 0.  Local sizes r*8*r cross direct_solve_size, so both the
+    Cholesky and the MINRES local paths run.
+
+    Round-off steers this trajectory from the first sweep on: the oracle alone
+    gives first-sweep residuals from 1.235e-3 to 1.345e-3 and stops converged
+    or stagnated (at 1.4e-8) depending on the BLAS thread count.  So both runs
+    must end within 3e-8, the first residuals agree to 15 %, and the solutions
+    to the distance a 3e-8 residual allows (kappa(A) < 30)."""
+    import ttnumpy_amd as tx
+    from threadpoolctl import threadpool_limits
+    print("amen C5 shape d=6", flush=True)
+    A_c = P.shifted_laplacian_sum_cores(6, 8, 1.0 / 6)
+    b_c = P.random_train_cores([8] * 6, 4, 3)
+    kw = dict(max_rank=64, residual_tol=1e-8, max_sweeps=20, symmetric=True, seed=0)
+    x, info = tx.amen(_tt(A_c), _tt(b_c), return_info=True, **kw)
+    with threadpool_limits(1):
+        xo, io = O.amen(O.Train(A_c), O.Train(b_c), **kw)
+    for stop, res in ((info.stop_reason, info.residuals), (io.stop_reason, io.residuals)):
+        assert str(stop) in ("converged", "stagnated")
+        assert res[-1] <= 3e-8
+    assert abs(info.residuals[0] - io.residuals[0]) <= 0.15 * io.residuals[0]
+    assert info.local_solves > 0
+    assert _rel_diff(x, xo) < 2e-6

@@ -67,9 +67,29 @@ def P(t: torch.Tensor, off: int = 0) -> int:
     return t.data_ptr() + 8 * off
 
 
+# ---------------------------------------------------------------- flop tally
+# Algorithmic flops of the launches made through this module, counted on the
+# host from each call's shapes with the closed forms of the routines (GEMM
+# 2mnk; Householder QR 2mnk - (m+n)k^2 + 2k^3/3 with Q formed as
+# 4mk^2 - 4k^3/3; SVD as the QR of the long side plus 20 Q^3 + 2 P Q^2; LU
+# 2n^3/3 + 2n^2; Cholesky n^3/3 + 2n^2).  ``flops()`` reads the running total.
+_FLOPS = [0.0]
+
+
+def flops() -> float:
+    return _FLOPS[0]
+
+
+def _qr_flops(m: int, n: int, form_q: bool) -> float:
+    k = min(m, n)
+    f = 2.0 * m * n * k - (m + n) * k * k + 2.0 * k ** 3 / 3.0
+    return f + (4.0 * m * k * k - 4.0 * k ** 3 / 3.0 if form_q else 0.0)
+
+
 # ---------------------------------------------------------------- primitives
 def gemm(batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, beta, C, c_ms, c_ns, c_bs):
     """C = alpha A B + beta C on MFMA; A/B/C are addresses (ints)."""
+    _FLOPS[0] += 2.0 * batch * M * N * K
     rc = lib().ttx_gemm_f64(stream(), batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns,
                             b_bs, beta, C, c_ms, c_ns, c_bs)
     check(rc, "ttx_gemm_f64")
@@ -107,6 +127,7 @@ def gemm_contract(batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_b
     if _CONTRACT[0] == "fp64":
         return gemm(batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, beta, C, c_ms,
                     c_ns, c_bs)
+    _FLOPS[0] += 2.0 * batch * M * N * K
     scales = empty(2)
     sa = sb = None
     if a_count:
@@ -277,6 +298,7 @@ def local_rhs(psi_b: torch.Tensor, b_core: torch.Tensor, phi_b: torch.Tensor, ou
 
 def lu_solve(Mb: torch.Tensor, n: int, u: torch.Tensor, info: torch.Tensor, ld: Optional[int] = None) -> None:
     """Solve the row-major [M | b] system in ``Mb`` (destroyed) into ``u`` (``common.py:767``)."""
+    _FLOPS[0] += 2.0 * n ** 3 / 3.0 + 2.0 * n * n
     work = torch.empty(int(lib().ttx_lu_work_size(n)) + 1, dtype=F64, device=device())
     check(lib().ttx_lu_solve_f64(stream(), n, P(Mb), n + 1 if ld is None else ld, P(u), P(work),
                                  info.data_ptr()), "ttx_lu_solve_f64")
@@ -286,6 +308,7 @@ def chol_solve(Mb: torch.Tensor, n: int, u: torch.Tensor, info: torch.Tensor,
                ld: Optional[int] = None) -> None:
     """Cholesky solve of the SPD row-major [M | b] in ``Mb`` (lower triangle read,
     destroyed); ``info`` gets TTX_ENOTSPD when M is not positive definite."""
+    _FLOPS[0] += n ** 3 / 3.0 + 2.0 * n * n
     check(lib().ttx_chol_solve_f64(stream(), n, P(Mb), n + 1 if ld is None else ld, P(u),
                                    info.data_ptr()), "ttx_chol_solve_f64")
 
@@ -293,6 +316,7 @@ def chol_solve(Mb: torch.Tensor, n: int, u: torch.Tensor, info: torch.Tensor,
 # ---------------------------------------------------------------- factorizations
 def qr(m: int, n: int, A: int, a_rs: int, a_cs: int, Q: Optional[int], q_rs: int, q_cs: int,
        R: int, r_rs: int, r_cs: int) -> None:
+    _FLOPS[0] += _qr_flops(m, n, Q is not None)
     work = empty(int(lib().ttx_qr_work_size(m, n)))
     rc = lib().ttx_qr_f64(stream(), m, n, A, a_rs, a_cs, Q if Q is not None else None, q_rs, q_cs,
                           R, r_rs, r_cs, P(work))
@@ -303,6 +327,8 @@ def svd(m: int, n: int, A: int, a_rs: int, a_cs: int, info: torch.Tensor, info_o
         ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
     """Thin SVD; returns contiguous U (m x k), S (k), Vt (k x n)."""
     k = min(m, n)
+    lp, sq = max(m, n), k
+    _FLOPS[0] += (_qr_flops(lp, sq, True) if lp > sq else 0.0) + 20.0 * sq ** 3 + 2.0 * lp * sq * sq
     U = empty(m, k)
     S = empty(k)
     Vt = empty(k, n)
