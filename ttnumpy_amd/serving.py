@@ -1,5 +1,7 @@
 """Serving loop for parametric 2-D diffusion solves on the batched engine.
 
+    python -m ttnumpy_amd.serving --requests 512 [--rate 50] [--precision mixed]
+
 A stream of requests arrives with exponential inter-arrival times (Poisson
 arrivals at ``rate`` per second).  Request ``i`` solves
 ``-div(k grad u) = 1`` on a ``2**L x 2**L`` grid with
@@ -47,6 +49,7 @@ class ServingConfig:
     poll: int = 16
     per_level: bool = True        # one engine (own stream) per grid level
     minres_slice: int = 128       # MINRES iterations per tick
+    precision: str = "fp64"       # "mixed": FP32 Cholesky with FP64 refinement for direct solves
 
 
 @dataclass
@@ -81,7 +84,8 @@ def build_requests(cfg: ServingConfig, items: Sequence[ServingItem]) -> List[Req
         A = TensorTrain(P.diffusion_2d_cores(L, it.theta, parts[L]))
         x0 = rand([2] * (2 * L), ranks=2, seed=it.index)
         out.append(Request(A=A, b=rhs[L], x0=x0, symmetric=True, residual_tol=cfg.residual_tol,
-                           max_sweeps=cfg.max_sweeps, seed=it.index, tag=it))
+                           max_sweeps=cfg.max_sweeps, seed=it.index, tag=it,
+                           precision=cfg.precision))
     return out
 
 
@@ -163,3 +167,46 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
         worst_residual=max((r.info.residuals[-1] for r in ok), default=float("nan")),
         by_level=dict(sorted(by_level.items())))
     return rep, done
+
+
+def main(argv=None) -> None:
+    """Serve a seeded stream and print progress lines and the report as JSON."""
+    import argparse
+    import dataclasses
+    import json
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--rate", type=float, default=50.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--slots", type=int, default=128, help="slots per engine")
+    ap.add_argument("--single-engine", action="store_true",
+                    help="one engine for all levels instead of one per level")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    a = ap.parse_args(argv)
+    cfg = ServingConfig(n_requests=a.requests, rate=a.rate, seed=a.seed, nslots=a.slots,
+                        per_level=not a.single_engine, precision=a.precision)
+    t = time.perf_counter()
+    reqs = build_requests(cfg, request_stream(cfg))
+    print(f"built {len(reqs)} requests in {time.perf_counter() - t:.1f} s", flush=True)
+    t0 = time.perf_counter()
+
+    def progress(k, r):
+        if k % 32 == 0 or k == a.requests:
+            print(f"{k} done at {time.perf_counter() - t0:.1f} s (last: L={r.tag.level}, "
+                  f"latency {r.finished - r.arrival:.2f} s)", flush=True)
+
+    rep, done = serve(cfg, reqs, progress=progress)
+    out = dataclasses.asdict(rep)
+    out["config"] = {"requests": a.requests, "rate_per_s": a.rate, "seed": a.seed,
+                     "levels": list(cfg.levels), "theta_max": cfg.theta_max,
+                     "slots_per_engine": a.slots, "per_level": cfg.per_level,
+                     "precision": a.precision, "residual_tol": cfg.residual_tol,
+                     "max_sweeps": cfg.max_sweeps}
+    errs = [r.error for r in done if r.error]
+    if errs:
+        out["first_errors"] = errs[:5]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
