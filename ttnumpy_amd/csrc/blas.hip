@@ -48,25 +48,35 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs p) {
   const int wm = (wave >> 1) * 32;
   const int wn = (wave & 1) * 32;
   double4_t acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
-
+  // each thread stages kPer elements of each slab; the next slab's loads are
+  // issued before the current slab's MFMAs
+  constexpr int kPer = GBM * GBK / 256;
+  int am[kPer], ak[kPer], bn[kPer], bk[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int e = q * 256 + tid;
+    if (a_m_fast) { am[q] = e % GBM; ak[q] = e / GBM; } else { ak[q] = e % GBK; am[q] = e / GBK; }
+    if (b_n_fast) { bn[q] = e % GBN; bk[q] = e / GBN; } else { bk[q] = e % GBK; bn[q] = e / GBK; }
+  }
+  double ra[kPer], rb[kPer];
+  auto load_slab = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int64_t gm = m0 + am[q], gka = k0 + ak[q];
+      ra[q] = (gm < p.M && gka < p.K) ? A[gm * p.a_ms + gka * p.a_ks] : 0.0;
+      const int64_t gn = n0 + bn[q], gkb = k0 + bk[q];
+      rb[q] = (gn < p.N && gkb < p.K) ? B[gkb * p.b_ks + gn * p.b_ns] : 0.0;
+    }
+  };
+  load_slab(0);
   for (int64_t k0 = 0; k0 < p.K; k0 += GBK) {
 #pragma unroll
-    for (int e0 = 0; e0 < GBM * GBK; e0 += 256) {
-      const int e = e0 + tid;
-      int mm, kk;
-      if (a_m_fast) { mm = e % GBM; kk = e / GBM; } else { kk = e % GBK; mm = e / GBK; }
-      const int64_t gm = m0 + mm, gk = k0 + kk;
-      As[kk][mm] = (gm < p.M && gk < p.K) ? A[gm * p.a_ms + gk * p.a_ks] : 0.0;
-    }
-#pragma unroll
-    for (int e0 = 0; e0 < GBN * GBK; e0 += 256) {
-      const int e = e0 + tid;
-      int nn, kk;
-      if (b_n_fast) { nn = e % GBN; kk = e / GBN; } else { kk = e % GBK; nn = e / GBK; }
-      const int64_t gn = n0 + nn, gk = k0 + kk;
-      Bs[kk][nn] = (gn < p.N && gk < p.K) ? B[gk * p.b_ks + gn * p.b_ns] : 0.0;
+    for (int q = 0; q < kPer; ++q) {
+      As[ak[q]][am[q]] = ra[q];
+      Bs[bk[q]][bn[q]] = rb[q];
     }
     __syncthreads();
+    if (k0 + GBK < p.K) load_slab(k0 + GBK);
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 4) {
       const int kr = kk + (lane >> 4);
