@@ -108,6 +108,65 @@ def test_qr_rank_deficient_and_transposed_view():
     np.testing.assert_allclose(_h(Q).T @ _h(Q), np.eye(10), atol=1e-12)
 
 
+@pytest.mark.parametrize("case", ["kappa1e4", "kappa1e9", "rank_deficient", "transposed_view"])
+def test_qr_tall_cholesky_qr2_and_fallback(case):
+    """Tall QRs with Q (m >= 256, 64 <= n <= 128) run CholeskyQR2 on the MFMA
+    GEMM; inputs too ill-conditioned for two passes (kappa ~1e9), or rank
+    deficient, are refused on the device and refactored by Householder.  Every
+    case must give an orthonormal Q and QR = A to rounding."""
+    print(f"qr tall {case}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(11)
+    m, n = 512, 128
+    A = rng.standard_normal((m, n))
+    if case == "kappa1e4":
+        A = A * np.logspace(0, -4, n)[None, :]
+    elif case == "kappa1e9":
+        A = A * np.logspace(0, -9, n)[None, :]
+    elif case == "rank_deficient":
+        A[:, 100:] = A[:, :28]
+    Q = D.empty(m, n)
+    R = D.empty(n, n)
+    if case == "transposed_view":
+        At = _t(np.ascontiguousarray(A.T))  # n x m storage, read as its transpose
+        D.qr(m, n, D.P(At), 1, m, D.P(Q), n, 1, D.P(R), n, 1)
+    else:
+        D.qr(m, n, D.P(_t(A)), n, 1, D.P(Q), n, 1, D.P(R), n, 1)
+    Qh, Rh = _h(Q), _h(R)
+    np.testing.assert_allclose(Qh @ Rh, A, atol=1e-13 * np.abs(A).max() * m)
+    np.testing.assert_allclose(Qh.T @ Qh, np.eye(n), atol=1e-13 * m)
+    assert np.allclose(np.tril(Rh, -1), 0.0)
+    if case != "rank_deficient":
+        Rref = np.linalg.qr(A, mode="reduced")[1]
+        np.testing.assert_allclose(np.abs(np.diag(Rh)), np.abs(np.diag(Rref)), rtol=1e-9)
+
+
+@pytest.mark.parametrize("m,n", [(384, 128), (128, 384), (512, 96), (96, 300)])
+def test_svd_tall_cholesky_qr2_path(m, n):
+    """Large tall/wide SVDs: CholeskyQR2 of the long side, the Jacobi kernel on
+    the square factor without accumulating rotations, V from the inverse
+    Cholesky factors, accepted by the orthogonality certificate (kappa ~ 30
+    here; the kappa ~ 1e9 cases of test_svd take the fallbacks)."""
+    print(f"svd tall {m}x{n}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(m + n)
+    k = min(m, n)
+    A = rng.standard_normal((m, n))
+    if m >= n:
+        A = A * np.logspace(0, -1.5, n)[None, :]
+    else:
+        A = np.logspace(0, -1.5, m)[:, None] * A
+    info = torch.zeros(2, dtype=torch.int32, device=D.device())
+    U, S, Vt = D.svd(m, n, D.P(_t(A)), n, 1, info)
+    assert int(info[0].item()) == 0
+    Uh, Sh, Vh = _h(U), _h(S), _h(Vt)
+    Sref = np.linalg.svd(A, compute_uv=False)
+    np.testing.assert_allclose(Sh, Sref, rtol=1e-12)
+    np.testing.assert_allclose((Uh * Sh) @ Vh, A, atol=1e-13 * Sref[0] * max(m, n))
+    np.testing.assert_allclose(Uh.T @ Uh, np.eye(k), atol=1e-13 * max(m, n))
+    np.testing.assert_allclose(Vh @ Vh.T, np.eye(k), atol=1e-13 * max(m, n))
+
+
 @pytest.mark.parametrize("m,n", [(1, 1), (1, 5), (5, 1), (8, 8), (66, 33), (33, 66), (128, 384),
                                  (384, 128), (200, 150), (2, 2)])
 def test_svd(m, n):

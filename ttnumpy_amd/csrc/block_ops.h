@@ -380,6 +380,7 @@ struct SvdArgs {
   int64_t v_rs, v_cs;
   double* work;
   int32_t* info;
+  const int* skip = nullptr;  // when non-null and *skip == 0 the kernel returns at once
 };
 
 
@@ -416,7 +417,7 @@ __device__ __forceinline__ bool jac_params(double a, double b, double c, double 
   return true;
 }
 
-template <bool REGS>
+template <bool REGS, bool ACCV>
 __device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, double* vj, int Q,
                                             int sub, double tol) {
   double a = 0.0, b = 0.0, c = 0.0;
@@ -428,7 +429,8 @@ __device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, 
       const int r = sub + 16 * t;
       const int rc = r < Q ? r : Q - 1;
       const bool in = r < Q;
-      const double gx = gi[rc], gy = gj[rc], vx = vi[rc], vy = vj[rc];
+      const double gx = gi[rc], gy = gj[rc];
+      const double vx = ACCV ? vi[rc] : 0.0, vy = ACCV ? vj[rc] : 0.0;
       x[t] = in ? gx : 0.0;
       y[t] = in ? gy : 0.0;
       u[t] = in ? vx : 0.0;
@@ -451,8 +453,10 @@ __device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, 
       if (r < Q) {
         gi[r] = jac_rot_lo(cs, sn, x[t], y[t]);
         gj[r] = jac_rot_hi(cs, sn, x[t], y[t]);
-        vi[r] = jac_rot_lo(cs, sn, u[t], w[t]);
-        vj[r] = jac_rot_hi(cs, sn, u[t], w[t]);
+        if (ACCV) {
+          vi[r] = jac_rot_lo(cs, sn, u[t], w[t]);
+          vj[r] = jac_rot_hi(cs, sn, u[t], w[t]);
+        }
       }
     }
     return true;
@@ -488,8 +492,8 @@ __device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, 
       const int rc = r < Q ? r : Q - 1;
       x[t] = gi[rc];
       y[t] = gj[rc];
-      u[t] = vi[rc];
-      w[t] = vj[rc];
+      u[t] = ACCV ? vi[rc] : 0.0;
+      w[t] = ACCV ? vj[rc] : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -497,8 +501,10 @@ __device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, 
       if (r < Q) {
         gi[r] = jac_rot_lo(cs, sn, x[t], y[t]);
         gj[r] = jac_rot_hi(cs, sn, x[t], y[t]);
-        vi[r] = jac_rot_lo(cs, sn, u[t], w[t]);
-        vj[r] = jac_rot_hi(cs, sn, u[t], w[t]);
+        if (ACCV) {
+          vi[r] = jac_rot_lo(cs, sn, u[t], w[t]);
+          vj[r] = jac_rot_hi(cs, sn, u[t], w[t]);
+        }
       }
     }
   }
@@ -510,7 +516,7 @@ __device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, 
 // rotates nothing (true) or 80 sweeps (false).  Each round of the
 // round-robin schedule pairs every column once; a 16-lane DPP row takes one
 // pair.  `flag` is a block-shared int.  Ends synced.
-template <bool REGS>
+template <bool REGS, bool ACCV>
 __device__ __forceinline__ bool jacobi_sweeps(double* G, double* V, int Q, int* flag) {
   const int tid = threadIdx.x;
   const int sub = tid & 15;
@@ -535,7 +541,8 @@ __device__ __forceinline__ bool jacobi_sweeps(double* G, double* V, int Q, int* 
         j += 1;
         if (i >= Q || j >= Q) continue;
         if (i > j) { const int t = i; i = j; j = t; }
-        if (jacobi_pair<REGS>(G + i * Q, G + j * Q, V + i * Q, V + j * Q, Q, sub, tol) && sub == 0)
+        if (jacobi_pair<REGS, ACCV>(G + i * Q, G + j * Q, V + i * Q, V + j * Q, Q, sub, tol) &&
+            sub == 0)
           *flag = 1;
       }
       __syncthreads();
@@ -552,8 +559,10 @@ __device__ __forceinline__ bool jacobi_sweeps(double* G, double* V, int Q, int* 
 // row per pair, the square factor in LDS when `lds` holds it), columns sorted
 // by descending singular value, and the tall factor's product on MFMA.
 // `work` holds 2*P*Q + 2*Q*Q + 3*Q doubles (P >= Q the long and short
-// sides); it may be LDS.  REGS: see jacobi_pair.  Ends synced.
-template <bool REGS = true>
+// sides); it may be LDS.  REGS: see jacobi_pair.  Without ACCV the rotations
+// are not accumulated: V (Vt of a tall, U of a wide input) is left unwritten
+// for the caller to form.  Ends synced.
+template <bool REGS = true, bool ACCV = true>
 __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double* red,
                                           double* lds = nullptr, int64_t lds_cap = 0) {
   __shared__ int s_rot;
@@ -603,13 +612,14 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   } else {
     for (int64_t e = tid; e < Q * Q; e += blockDim.x) G[e] = Wq[e];
   }
-  for (int64_t e = tid; e < Q * Q; e += blockDim.x) V[e] = (e == q32(e, Q) * (Q + 1)) ? 1.0 : 0.0;
+  if (ACCV)
+    for (int64_t e = tid; e < Q * Q; e += blockDim.x) V[e] = (e == q32(e, Q) * (Q + 1)) ? 1.0 : 0.0;
   __syncthreads();
 
   // one-sided Jacobi on the columns of G; a separate call per location of G
   // lets each access compile to an LDS or a global load
-  const int converged = G == lds ? jacobi_sweeps<REGS>(lds, V, (int)Q, &s_rot)
-                                 : jacobi_sweeps<REGS>(Gw, V, (int)Q, &s_rot);
+  const int converged = G == lds ? jacobi_sweeps<REGS, ACCV>(lds, V, (int)Q, &s_rot)
+                                 : jacobi_sweeps<REGS, ACCV>(Gw, V, (int)Q, &s_rot);
   if (tid == 0) p.info[0] = converged ? TTX_OK : TTX_ENOCONV;
 
   // singular values = column norms
@@ -652,15 +662,16 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
       ux[r * ux_rs + o * ux_cs] = Gs[r + o * Q];
     }
   }
-  for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
-    const int64_t o = q32(e, Q), r = e - o * Q;
-    const int64_t src = (int64_t)ordd[o];
-    const double v = V[r + src * Q];
-    if (!wide)
-      p.Vt[o * p.v_rs + r * p.v_cs] = v;
-    else
-      p.U[r * p.u_rs + o * p.u_cs] = v;
-  }
+  if (ACCV)
+    for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
+      const int64_t o = q32(e, Q), r = e - o * Q;
+      const int64_t src = (int64_t)ordd[o];
+      const double v = V[r + src * Q];
+      if (!wide)
+        p.Vt[o * p.v_rs + r * p.v_cs] = v;
+      else
+        p.U[r * p.u_rs + o * p.u_cs] = v;
+    }
   for (int64_t o = tid; o < Q; o += blockDim.x) p.S[o] = sig[(int64_t)ordd[o]];
   __syncthreads();
 }

@@ -29,6 +29,7 @@ struct QrArgs {
   double* R;
   int64_t r_rs, r_cs;
   double* work;
+  const int* skip;  // when non-null and *skip == 0, Q and R are already formed: return
 };
 
 // IN_LDS: the working matrix, tau and Q live in dynamic LDS (small cores);
@@ -37,6 +38,7 @@ template <bool IN_LDS>
 __global__ __launch_bounds__(1024) void qr_kernel(QrArgs p) {
   extern __shared__ double smem[];
   __shared__ double red[16];
+  if (p.skip != nullptr && *p.skip == 0) return;
   const int64_t m = p.m, n = p.n, k = m < n ? m : n;
   double* W = IN_LDS ? smem : p.work;
   double* tau = W + m * n;
@@ -63,14 +65,165 @@ __global__ __launch_bounds__(1024) void qr_kernel(QrArgs p) {
 }
 
 // ---------------------------------------------------------------------------
+// CholeskyQR2 for tall matrices whose Q is wanted: G = X^T X, L L^T = G,
+// Q1 = X L^{-T}; the same on Q1 (G2, L2, Q = Q1 L2^{-T}); R = L2^T L1^T.  The
+// Gram and triangular products run on the multi-workgroup MFMA GEMM, so a
+// 512 x 128 factorization spreads over the chip instead of one CU; the
+// Cholesky factor and its inverse are formed by one workgroup in LDS.
+// Two passes give orthogonality to rounding when kappa(X)^2 eps << 1
+// (Yamamoto, Nakatsukasa, Yanagisawa, Fukaya 2015).  The factorization is
+// refused (flag set) when a first-pass pivot falls below kCqrPivot times the
+// largest diagonal of G (kappa(X) beyond ~1e6) or is not finite, or when a
+// second-pass diagonal leaves [0.9, 1.1] (Q1 not close to orthonormal); the
+// Householder kernel queued after it then recomputes Q and R from X.  The
+// result equals Householder QR up to rounding and the signs of R's diagonal
+// (positive here), which every consumer is invariant to.
+// ---------------------------------------------------------------------------
+constexpr int64_t kCqrMinCols = 64, kCqrMaxCols = 128, kCqrMinRows = 256;
+constexpr double kCqrPivot = 1e-12;
+constexpr int kCqrCholThreads = 1024;
+
+static bool cqr_applies(int64_t m, int64_t n) {
+  return n >= kCqrMinCols && n <= kCqrMaxCols && m >= kCqrMinRows && m >= 2 * n;
+}
+
+// Work layout of the CholeskyQR2 path: G, L1, L1inv, L2, L2inv (n x n each,
+// row-major), Q1 (m x n row-major); the flag lives past the Householder
+// region so that a fallback kernel reads it before anything overwrites it.
+static int64_t cqr_work(int64_t m, int64_t n) { return 5 * n * n + m * n; }
+static int64_t hh_work(int64_t m, int64_t n) {
+  const int64_t k = m < n ? m : n;
+  return m * n + k + m * k;
+}
+
+// L L^T = G and Linv = L^{-1} for the row-major n x n G (its upper triangle
+// is read), both written row-major with zeros above the diagonal.  pass 0
+// sets *flag (0 accepted, 1 refused); pass 1 only sets it to 1 when the
+// second-pass diagonal is off.  n <= kCqrMaxCols.
+//
+// LDS holds U = L^T (upper, row stride n + 1): the right-looking update
+// A[i][k] -= A[j][i] A[j][k] / d then reads row j contiguously across the
+// lanes (a column of a row-major array would put every lane on one bank), and
+// row j is scaled during the next column's update, so each column costs one
+// barrier.  L^{-1} is formed in the same storage (LAPACK dtrti2 order, lower):
+// for j from the last column back, the trailing block holds its inverse T
+// (T[i][k] at A[k][i]) and column j becomes -(1/L_jj) T L[j+1:, j], with
+// L[j+1:, j] = row j of A; each row's sum is split over 8 lanes.
+__global__ __launch_bounds__(1024) void cqr_chol_kernel(int64_t n64, const double* G, double* L,
+                                                        double* Linv, int* flag, int pass) {
+  extern __shared__ double A[];  // n x (n + 1)
+  __shared__ int s_bad;
+  __shared__ double s_gmax;
+  __shared__ double s_y[kCqrMaxCols];
+  const int n = (int)n64, ld = n + 1;
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  const int tx = tid & 31, ty = tid >> 5, nty = nt >> 5;
+  if (tid == 0) {
+    s_bad = 0;
+    double gm = 0.0;
+    for (int i = 0; i < n; ++i) gm = fmax(gm, G[i * n + i]);
+    s_gmax = gm;
+  }
+  for (int e = tid; e < n * n; e += nt) {
+    const int i = e / n, k = e - i * n;
+    A[i * ld + k] = k >= i ? G[e] : 0.0;
+  }
+  __syncthreads();
+  const double floor_piv = pass == 0 ? kCqrPivot * s_gmax : 0.0;
+  double prev_l = 1.0;
+  int jend = n;
+  for (int j = 0; j < n; ++j) {
+    const double d = A[j * ld + j];
+    if (!(d > floor_piv) || !isfinite(d)) {  // uniform: every thread reads the same d
+      jend = j;
+      break;
+    }
+    const double rd = 1.0 / d;
+    if (j > 0) {
+      const double ri = 1.0 / prev_l;
+      for (int k = j + tid; k < n; k += nt) A[(j - 1) * ld + k] *= ri;
+    }
+    for (int i = j + 1 + ty; i < n; i += nty) {
+      const double aji = A[j * ld + i] * rd;
+      for (int k = i + tx; k < n; k += 32) A[i * ld + k] -= aji * A[j * ld + k];
+    }
+    prev_l = sqrt(d);
+    if (tid == 0) A[j * ld + j] = prev_l;
+    __syncthreads();
+  }
+  if (jend < n && tid == 0) s_bad = 1;  // the last row has nothing right of it to scale
+  __syncthreads();
+  if (pass == 1 && !s_bad && tid == 0) {
+    for (int j = 0; j < n; ++j)
+      if (fabs(A[j * ld + j] - 1.0) > 0.1) s_bad = 1;
+  }
+  for (int e = tid; e < n * n; e += nt) {
+    const int i = e / n, k = e - i * n;
+    L[e] = k <= i ? A[k * ld + i] : 0.0;
+  }
+  const int part = tid & 7, rowg = tid >> 3, nrowg = nt >> 3;
+  for (int j = n - 1; j >= 0; --j) {
+    __syncthreads();
+    const double inv = 1.0 / A[j * ld + j];
+    for (int i = j + 1 + rowg; i < n + rowg; i += nrowg) {
+      double acc = 0.0;
+      if (i < n)
+        for (int k = j + 1 + part; k <= i; k += 8) acc += A[k * ld + i] * A[j * ld + k];
+      acc += __shfl_xor(acc, 1, 8);
+      acc += __shfl_xor(acc, 2, 8);
+      acc += __shfl_xor(acc, 4, 8);
+      if (i < n && part == 0) s_y[i] = -inv * acc;
+    }
+    __syncthreads();
+    for (int i = j + 1 + tid; i < n; i += nt) A[j * ld + i] = s_y[i];
+    if (tid == 0) A[j * ld + j] = inv;
+  }
+  __syncthreads();
+  for (int e = tid; e < n * n; e += nt) {
+    const int i = e / n, k = e - i * n;
+    Linv[e] = k <= i ? A[k * ld + i] : 0.0;
+  }
+  if (tid == 0) {
+    if (pass == 0) *flag = s_bad;
+    else if (s_bad) *flag = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // SVD
 // ---------------------------------------------------------------------------
-template <bool IN_LDS>
+template <bool IN_LDS, bool ACCV = true>
 __global__ __launch_bounds__(1024) void svd_kernel(SvdArgs p) {
   extern __shared__ double smem[];
   __shared__ double red[16];
-  block_svd(p, IN_LDS ? smem : p.work, red, IN_LDS ? nullptr : smem,
-            IN_LDS ? 0 : (int64_t)(kQrLds / sizeof(double)));
+  if (p.skip != nullptr && *p.skip == 0) return;
+  block_svd<true, ACCV>(p, IN_LDS ? smem : p.work, red, IN_LDS ? nullptr : smem,
+                        IN_LDS ? 0 : (int64_t)(kQrLds / sizeof(double)));
+}
+
+// *flag = 0 when max |E - I| <= tol for the n x n row-major E, else 1
+__global__ __launch_bounds__(1024) void identity_check_kernel(int64_t n, const double* E,
+                                                              double tol, int* flag) {
+  __shared__ double red[16];
+  double mx = 0.0;
+  for (int64_t e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int64_t i = e / n, j = e - i * n;
+    const double d = fabs(E[e] - (i == j ? 1.0 : 0.0));
+    mx = (d > mx || d != d) ? d : mx;  // a NaN propagates
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(mx, off, 64);
+    mx = (o > mx || o != o) ? o : mx;
+  }
+  if (lane == 0) red[wave] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t = (red[w] > t || red[w] != red[w]) ? red[w] : t;
+    *flag = (t <= tol) ? 0 : 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -172,8 +325,10 @@ __global__ __launch_bounds__(256) void svd_terms_kernel(int64_t p, int64_t q, in
 using namespace ttx;
 
 extern "C" int64_t ttx_qr_work_size(int64_t m, int64_t n) {
-  const int64_t k = m < n ? m : n;
-  return m * n + k + m * k;
+  const int64_t h = hh_work(m, n);
+  if (!cqr_applies(m, n)) return h;
+  const int64_t c = cqr_work(m, n);
+  return (h > c ? h : c) + 2;  // + the flag
 }
 
 constexpr size_t kLdsLimit = 150 * 1024;
@@ -185,6 +340,57 @@ static int set_lds_attr(const void* fn) {
              : TTX_EHIP;
 }
 
+// CholeskyQR2 of X (m x n, strided) into Q and R, with the Householder
+// fallback queued behind it; work: ttx_qr_work_size(m, n) doubles.
+static int cqr_factor(ttx_stream_t stream, int64_t m, int64_t n, const double* A, int64_t a_rs,
+                      int64_t a_cs, double* Q, int64_t q_rs, int64_t q_cs, double* R,
+                      int64_t r_rs, int64_t r_cs, double* work) {
+  static bool attr_c = false, attr_g = false;
+  if (!attr_c) {
+    if (set_lds_attr((const void*)cqr_chol_kernel) != TTX_OK) return TTX_EHIP;
+    attr_c = true;
+  }
+  if (!attr_g) {
+    if (set_lds_attr((const void*)qr_kernel<false>) != TTX_OK) return TTX_EHIP;
+    attr_g = true;
+  }
+  const int64_t nn = n * n;
+  double* G = work;
+  double* L1 = G + nn;
+  double* L1i = L1 + nn;
+  double* L2 = L1i + nn;
+  double* L2i = L2 + nn;
+  double* Q1 = L2i + nn;
+  const int64_t h = hh_work(m, n), c = cqr_work(m, n);
+  int* flag = (int*)(work + (h > c ? h : c));
+  hipStream_t st = (hipStream_t)stream;
+  const size_t lds = (size_t)n * (n + 1) * sizeof(double);
+  int rc;
+  // pass 1: G = X^T X, L1, Q1 = X L1^{-T}
+  if ((rc = ttx_gemm_f64(stream, 1, n, n, m, 1.0, A, a_cs, a_rs, 0, A, a_rs, a_cs, 0, 0.0, G, n, 1,
+                         0)) != TTX_OK)
+    return rc;
+  hipLaunchKernelGGL(cqr_chol_kernel, dim3(1), dim3(kCqrCholThreads), lds, st, n, G, L1, L1i, flag, 0);
+  if ((rc = ttx_gemm_f64(stream, 1, m, n, n, 1.0, A, a_rs, a_cs, 0, L1i, 1, n, 0, 0.0, Q1, n, 1,
+                         0)) != TTX_OK)
+    return rc;
+  // pass 2: G = Q1^T Q1, L2, Q = Q1 L2^{-T}, R = L2^T L1^T
+  if ((rc = ttx_gemm_f64(stream, 1, n, n, m, 1.0, Q1, 1, n, 0, Q1, n, 1, 0, 0.0, G, n, 1, 0)) !=
+      TTX_OK)
+    return rc;
+  hipLaunchKernelGGL(cqr_chol_kernel, dim3(1), dim3(kCqrCholThreads), lds, st, n, G, L2, L2i, flag, 1);
+  if ((rc = ttx_gemm_f64(stream, 1, m, n, n, 1.0, Q1, n, 1, 0, L2i, 1, n, 0, 0.0, Q, q_rs, q_cs,
+                         0)) != TTX_OK)
+    return rc;
+  if ((rc = ttx_gemm_f64(stream, 1, n, n, n, 1.0, L2, 1, n, 0, L1, 1, n, 0, 0.0, R, r_rs, r_cs,
+                         0)) != TTX_OK)
+    return rc;
+  // refused: Householder from X (returns at once when the flag is 0)
+  QrArgs pf{m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work, flag};
+  hipLaunchKernelGGL(qr_kernel<false>, dim3(1), dim3(1024), kQrLds, st, pf);
+  return launch_status();
+}
+
 extern "C" int ttx_qr_f64(ttx_stream_t stream, int64_t m, int64_t n, const double* A, int64_t a_rs,
                           int64_t a_cs, double* Q, int64_t q_rs, int64_t q_cs, double* R,
                           int64_t r_rs, int64_t r_cs, double* work) {
@@ -194,8 +400,10 @@ extern "C" int ttx_qr_f64(ttx_stream_t stream, int64_t m, int64_t n, const doubl
     if (set_lds_attr((const void*)qr_kernel<true>) != TTX_OK) return TTX_EHIP;
     attr = true;
   }
-  QrArgs p{m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work};
-  const size_t bytes = (size_t)ttx_qr_work_size(m, n) * sizeof(double);
+  if (Q != nullptr && cqr_applies(m, n))
+    return cqr_factor(stream, m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work);
+  QrArgs p{m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work, nullptr};
+  const size_t bytes = (size_t)hh_work(m, n) * sizeof(double);
   const int64_t k = m < n ? m : n;
   // k >= kQrBlockMin: the blocked factorization (panels in LDS), as in the engine
   if (bytes <= kLdsLimit && k < kQrBlockMin) {
@@ -211,9 +419,16 @@ extern "C" int ttx_qr_f64(ttx_stream_t stream, int64_t m, int64_t n, const doubl
   return launch_status();
 }
 
+static int64_t svd_block_work(int64_t P, int64_t Q) { return 2 * P * Q + 2 * Q * Q + 3 * Q + 8; }
+
+// V_r from R^{-1} is accepted when max |V_r^T V_r - I| <= kSvdCert * Q * eps
+constexpr double kSvdCert = 16.0;
+
 extern "C" int64_t ttx_svd_work_size(int64_t m, int64_t n) {
   const int64_t P = m > n ? m : n, Q = m < n ? m : n;
-  return 2 * P * Q + 2 * Q * Q + 3 * Q + 8;
+  if (!cqr_applies(P, Q)) return svd_block_work(P, Q);
+  // Qx, R, U_r, Vt_r, two Q x Q scratch, the QR's work, the square SVD's work, a flag
+  return P * Q + 5 * Q * Q + ttx_qr_work_size(P, Q) + svd_block_work(Q, Q) + 2;
 }
 
 extern "C" int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n, const double* A,
@@ -226,8 +441,78 @@ extern "C" int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n, const doub
     if (set_lds_attr((const void*)svd_kernel<true>) != TTX_OK) return TTX_EHIP;
     attr = true;
   }
+  const int64_t P = m > n ? m : n, Qd = m < n ? m : n;
+  if (cqr_applies(P, Qd)) {
+    // X = A (tall) or A^T (wide), P x Qd: X = Qx R by CholeskyQR2 across the
+    // chip, then R = U_r S Vt_r, then tall: U = Qx U_r, Vt = Vt_r; wide:
+    // U = Vt_r^T, Vt = (Qx U_r)^T.  The Jacobi kernel first runs without
+    // accumulating rotations (its loop then touches only the square factor in
+    // LDS) and V_r = R^{-1} U_r S is formed from the inverse Cholesky factors
+    // the QR left behind, R^{-1} = L1^{-T} L2^{-T}; V_r is accepted when
+    // max |V_r^T V_r - I| <= kSvdCert (its error is about kappa(R) eps),
+    // otherwise the Jacobi kernel runs again accumulating the rotations.
+    static bool attr_g = false, attr_f = false;
+    if (!attr_g) {
+      if (set_lds_attr((const void*)svd_kernel<false, true>) != TTX_OK) return TTX_EHIP;
+      attr_g = true;
+    }
+    if (!attr_f) {
+      if (set_lds_attr((const void*)svd_kernel<false, false>) != TTX_OK) return TTX_EHIP;
+      attr_f = true;
+    }
+    const bool wide = m < n;
+    const int64_t x_rs = wide ? a_cs : a_rs, x_cs = wide ? a_rs : a_cs;
+    const int64_t qq = Qd * Qd;
+    double* Qx = work;
+    double* Rm = Qx + P * Qd;
+    double* Ur = Rm + qq;
+    double* Vtr = Ur + qq;
+    double* T = Vtr + qq;
+    double* E = T + qq;
+    double* wq = E + qq;
+    double* ws = wq + ttx_qr_work_size(P, Qd);
+    int* vflag = (int*)(ws + svd_block_work(Qd, Qd));
+    const double* L1i = wq + 2 * qq;  // the CholeskyQR2 work layout: G, L1, L1inv, L2, L2inv
+    const double* L2i = wq + 4 * qq;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = cqr_factor(stream, P, Qd, A, x_rs, x_cs, Qx, Qd, 1, Rm, Qd, 1, wq);
+    if (rc != TTX_OK) return rc;
+    SvdArgs pf{Qd, Qd, Rm, Qd, 1, Ur, Qd, 1, S, Vtr, Qd, 1, ws, info, nullptr};
+    hipLaunchKernelGGL((svd_kernel<false, false>), dim3(1), dim3(1024), kQrLds, st, pf);
+    if ((rc = launch_status()) != TTX_OK) return rc;
+    // T = L2inv^T U_r; V_r = L1inv^T T written transposed into Vt_r; rows of Vt_r times S
+    if ((rc = ttx_gemm_f64(stream, 1, Qd, Qd, Qd, 1.0, L2i, 1, Qd, 0, Ur, Qd, 1, 0, 0.0, T, Qd, 1,
+                           0)) != TTX_OK)
+      return rc;
+    if ((rc = ttx_gemm_f64(stream, 1, Qd, Qd, Qd, 1.0, L1i, 1, Qd, 0, T, Qd, 1, 0, 0.0, Vtr, 1, Qd,
+                           0)) != TTX_OK)
+      return rc;
+    if ((rc = ttx_scale_cols_f64(stream, Qd, Qd, Vtr, 1, Qd, S)) != TTX_OK) return rc;
+    // E = V_r^T V_r = Vt_r Vt_r^T
+    if ((rc = ttx_gemm_f64(stream, 1, Qd, Qd, Qd, 1.0, Vtr, Qd, 1, 0, Vtr, 1, Qd, 0, 0.0, E, Qd, 1,
+                           0)) != TTX_OK)
+      return rc;
+    hipLaunchKernelGGL(identity_check_kernel, dim3(1), dim3(1024), 0, st, Qd, E,
+                       kSvdCert * (double)Qd * kEps, vflag);
+    SvdArgs pa{Qd, Qd, Rm, Qd, 1, Ur, Qd, 1, S, Vtr, Qd, 1, ws, info, vflag};
+    hipLaunchKernelGGL((svd_kernel<false, true>), dim3(1), dim3(1024), kQrLds, st, pa);
+    if ((rc = launch_status()) != TTX_OK) return rc;
+    const int64_t dims[2] = {Qd, Qd};
+    if (!wide) {
+      if ((rc = ttx_gemm_f64(stream, 1, P, Qd, Qd, 1.0, Qx, Qd, 1, 0, Ur, Qd, 1, 0, 0.0, U, u_rs,
+                             u_cs, 0)) != TTX_OK)
+        return rc;
+      const int64_t ss[2] = {Qd, 1}, ds[2] = {v_rs, v_cs};
+      return ttx_copy_f64(stream, 2, dims, Vtr, ss, Vt, ds, 1.0);
+    }
+    const int64_t ss[2] = {1, Qd}, ds[2] = {u_rs, u_cs};  // U[i][o] = Vt_r[o][i]
+    if ((rc = ttx_copy_f64(stream, 2, dims, Vtr, ss, U, ds, 1.0)) != TTX_OK) return rc;
+    // Vt[o][r] = sum_l Qx[r][l] U_r[l][o]
+    return ttx_gemm_f64(stream, 1, P, Qd, Qd, 1.0, Qx, Qd, 1, 0, Ur, Qd, 1, 0, 0.0, Vt, v_cs, v_rs,
+                        0);
+  }
   SvdArgs p{m, n, A, a_rs, a_cs, U, u_rs, u_cs, S, Vt, v_rs, v_cs, work, info};
-  const size_t bytes = (size_t)ttx_svd_work_size(m, n) * sizeof(double);
+  const size_t bytes = (size_t)svd_block_work(m > n ? m : n, m < n ? m : n) * sizeof(double);
   const int64_t k = m < n ? m : n;
   if (bytes <= kLdsLimit && k < kQrBlockMin) {
     hipLaunchKernelGGL(svd_kernel<true>, dim3(1), dim3(1024), bytes, (hipStream_t)stream, p);
