@@ -96,64 +96,123 @@ static int64_t hh_work(int64_t m, int64_t n) {
   return m * n + k + m * k;
 }
 
+// Ordering of the LDS accesses of one wave between steps of a routine that
+// only that wave runs (the LDS serves a wave's accesses in issue order; the
+// fence keeps the compiler from moving loads and stores across the step).
+__device__ __forceinline__ void wave_step() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+constexpr int kCqrNb = 16;  // block width of the Cholesky factor and its inverse
+
+// Upper Cholesky of the b x b diagonal block at (j0, j0) of A (row stride ld)
+// by the calling wave; false when a pivot is <= floor_piv or not finite.
+__device__ bool wave_chol_block(double* A, int ld, int j0, int b, double floor_piv) {
+  const int lane = threadIdx.x & 63;
+  for (int j = 0; j < b; ++j) {
+    double* rj = A + (j0 + j) * ld + j0;
+    const double d = rj[j];
+    if (!(d > floor_piv) || !isfinite(d)) return false;  // uniform in the wave
+    const double l = sqrt(d), rl = 1.0 / l;
+    for (int k = j + 1 + lane; k < b; k += 64) rj[k] *= rl;
+    wave_step();
+    if (lane == 0) rj[j] = l;
+    const int w = b - j - 1;
+    for (int e = lane; e < w * w; e += 64) {
+      const int i = j + 1 + e / w, k = j + 1 + e % w;
+      if (k >= i) A[(j0 + i) * ld + j0 + k] -= rj[i] * rj[k];
+    }
+    wave_step();
+  }
+  return true;
+}
+
+// In-place inverse of the b x b upper block at (j0, j0) by the calling wave
+// (LAPACK dtrti2, upper: column j becomes -(1/U_jj) Inv[0:j,0:j] U[0:j, j]).
+__device__ void wave_inv_block(double* A, int ld, int j0, int b) {
+  const int lane = threadIdx.x & 63;
+  for (int j = 0; j < b; ++j) {
+    const double inv = 1.0 / A[(j0 + j) * ld + j0 + j];
+    double acc = 0.0;
+    if (lane < j)
+      for (int s = lane; s < j; ++s) acc += A[(j0 + lane) * ld + j0 + s] * A[(j0 + s) * ld + j0 + j];
+    wave_step();
+    if (lane < j) A[(j0 + lane) * ld + j0 + j] = -inv * acc;
+    if (lane == 0) A[(j0 + j) * ld + j0 + j] = inv;
+    wave_step();
+  }
+}
+
 // L L^T = G and Linv = L^{-1} for the row-major n x n G (its upper triangle
 // is read), both written row-major with zeros above the diagonal.  pass 0
 // sets *flag (0 accepted, 1 refused); pass 1 only sets it to 1 when the
 // second-pass diagonal is off.  n <= kCqrMaxCols.
 //
-// LDS holds U = L^T (upper, row stride n + 1): the right-looking update
-// A[i][k] -= A[j][i] A[j][k] / d then reads row j contiguously across the
-// lanes (a column of a row-major array would put every lane on one bank), and
-// row j is scaled during the next column's update, so each column costs one
-// barrier.  L^{-1} is formed in the same storage (LAPACK dtrti2 order, lower):
-// for j from the last column back, the trailing block holds its inverse T
-// (T[i][k] at A[k][i]) and column j becomes -(1/L_jj) T L[j+1:, j], with
-// L[j+1:, j] = row j of A; each row's sum is split over 8 lanes.
+// LDS holds U = L^T (upper, row stride n + 1, so that rows and columns both
+// spread over the banks).  Blocked by kCqrNb: wave 0 factors the diagonal
+// block, all threads solve the block row U12 = D^{-T} A12 (one thread per
+// column) and take the rank-kCqrNb update of the trailing upper triangle.
+// U^{-1} is then formed top-down in the same storage: T = Inv11 A12 by all
+// threads while wave 0 inverts the diagonal block, then A12 = -T D^{-1}; and
+// L^{-1} = (U^{-1})^T.  Five barriers per block.
 __global__ __launch_bounds__(1024) void cqr_chol_kernel(int64_t n64, const double* G, double* L,
                                                         double* Linv, int* flag, int pass) {
-  extern __shared__ double A[];  // n x (n + 1)
+  extern __shared__ double A[];  // n x (n + 1), then T (n x kCqrNb)
   __shared__ int s_bad;
   __shared__ double s_gmax;
-  __shared__ double s_y[kCqrMaxCols];
   const int n = (int)n64, ld = n + 1;
+  double* T = A + n * ld;
   const int tid = threadIdx.x;
   const int nt = blockDim.x;
+  const int wave = tid >> 6;
   const int tx = tid & 31, ty = tid >> 5, nty = nt >> 5;
-  if (tid == 0) {
-    s_bad = 0;
-    double gm = 0.0;
-    for (int i = 0; i < n; ++i) gm = fmax(gm, G[i * n + i]);
-    s_gmax = gm;
-  }
+  if (tid == 0) s_bad = 0;
   for (int e = tid; e < n * n; e += nt) {
     const int i = e / n, k = e - i * n;
     A[i * ld + k] = k >= i ? G[e] : 0.0;
   }
   __syncthreads();
+  if (wave == 0) {  // largest diagonal entry of G
+    double gm = 0.0;
+    for (int i = tid; i < n; i += 64) gm = fmax(gm, A[i * ld + i]);
+    gm = wave_max(gm);
+    if (tid == 0) s_gmax = gm;
+  }
+  __syncthreads();
   const double floor_piv = pass == 0 ? kCqrPivot * s_gmax : 0.0;
-  double prev_l = 1.0;
-  int jend = n;
-  for (int j = 0; j < n; ++j) {
-    const double d = A[j * ld + j];
-    if (!(d > floor_piv) || !isfinite(d)) {  // uniform: every thread reads the same d
-      jend = j;
-      break;
+  for (int jb = 0; jb < n; jb += kCqrNb) {
+    const int b = n - jb < kCqrNb ? n - jb : kCqrNb;
+    if (wave == 0 && !wave_chol_block(A, ld, jb, b, floor_piv) && tid == 0) s_bad = 1;
+    __syncthreads();
+    if (s_bad) break;  // uniform after the barrier
+    const int c0 = jb + b;
+    // U12 = D^{-T} A12: forward substitution down the block rows, one column per thread
+    for (int c = c0 + tid; c < n; c += nt) {
+      double x[kCqrNb];
+#pragma unroll
+      for (int r = 0; r < kCqrNb; ++r) {
+        if (r < b) {
+          double v = A[(jb + r) * ld + c];
+#pragma unroll
+          for (int q = 0; q < kCqrNb; ++q)
+            if (q < r) v -= A[(jb + q) * ld + jb + r] * x[q];
+          x[r] = v / A[(jb + r) * ld + jb + r];
+          A[(jb + r) * ld + c] = x[r];
+        }
+      }
     }
-    const double rd = 1.0 / d;
-    if (j > 0) {
-      const double ri = 1.0 / prev_l;
-      for (int k = j + tid; k < n; k += nt) A[(j - 1) * ld + k] *= ri;
+    __syncthreads();
+    // trailing upper triangle: A[i][k] -= sum_r U12[r][i] U12[r][k]
+    for (int i = c0 + ty; i < n; i += nty) {
+      for (int k = i + tx; k < n; k += 32) {
+        double acc = 0.0;
+        for (int r = 0; r < b; ++r) acc += A[(jb + r) * ld + i] * A[(jb + r) * ld + k];
+        A[i * ld + k] -= acc;
+      }
     }
-    for (int i = j + 1 + ty; i < n; i += nty) {
-      const double aji = A[j * ld + i] * rd;
-      for (int k = i + tx; k < n; k += 32) A[i * ld + k] -= aji * A[j * ld + k];
-    }
-    prev_l = sqrt(d);
-    if (tid == 0) A[j * ld + j] = prev_l;
     __syncthreads();
   }
-  if (jend < n && tid == 0) s_bad = 1;  // the last row has nothing right of it to scale
-  __syncthreads();
   if (pass == 1 && !s_bad && tid == 0) {
     for (int j = 0; j < n; ++j)
       if (fabs(A[j * ld + j] - 1.0) > 0.1) s_bad = 1;
@@ -162,24 +221,29 @@ __global__ __launch_bounds__(1024) void cqr_chol_kernel(int64_t n64, const doubl
     const int i = e / n, k = e - i * n;
     L[e] = k <= i ? A[k * ld + i] : 0.0;
   }
-  const int part = tid & 7, rowg = tid >> 3, nrowg = nt >> 3;
-  for (int j = n - 1; j >= 0; --j) {
-    __syncthreads();
-    const double inv = 1.0 / A[j * ld + j];
-    for (int i = j + 1 + rowg; i < n + rowg; i += nrowg) {
-      double acc = 0.0;
-      if (i < n)
-        for (int k = j + 1 + part; k <= i; k += 8) acc += A[k * ld + i] * A[j * ld + k];
-      acc += __shfl_xor(acc, 1, 8);
-      acc += __shfl_xor(acc, 2, 8);
-      acc += __shfl_xor(acc, 4, 8);
-      if (i < n && part == 0) s_y[i] = -inv * acc;
-    }
-    __syncthreads();
-    for (int i = j + 1 + tid; i < n; i += nt) A[j * ld + i] = s_y[i];
-    if (tid == 0) A[j * ld + j] = inv;
-  }
   __syncthreads();
+  if (!s_bad) {
+    for (int jb = 0; jb < n; jb += kCqrNb) {
+      const int b = n - jb < kCqrNb ? n - jb : kCqrNb;
+      // T = Inv11 A[0:jb, jb:jb+b] (Inv11 upper, already formed)
+      for (int e = tid; e < jb * b; e += nt) {
+        const int r = e / b, c = e - r * b;
+        double acc = 0.0;
+        for (int q = r; q < jb; ++q) acc += A[r * ld + q] * A[q * ld + jb + c];
+        T[r * kCqrNb + c] = acc;
+      }
+      if (wave == 0) wave_inv_block(A, ld, jb, b);
+      __syncthreads();
+      // A[0:jb, jb:jb+b] = -T D^{-1} (D^{-1} upper)
+      for (int e = tid; e < jb * b; e += nt) {
+        const int r = e / b, c = e - r * b;
+        double acc = 0.0;
+        for (int q = 0; q <= c; ++q) acc += T[r * kCqrNb + q] * A[(jb + q) * ld + jb + c];
+        A[r * ld + jb + c] = -acc;
+      }
+      __syncthreads();
+    }
+  }
   for (int e = tid; e < n * n; e += nt) {
     const int i = e / n, k = e - i * n;
     Linv[e] = k <= i ? A[k * ld + i] : 0.0;
@@ -364,7 +428,7 @@ static int cqr_factor(ttx_stream_t stream, int64_t m, int64_t n, const double* A
   const int64_t h = hh_work(m, n), c = cqr_work(m, n);
   int* flag = (int*)(work + (h > c ? h : c));
   hipStream_t st = (hipStream_t)stream;
-  const size_t lds = (size_t)n * (n + 1) * sizeof(double);
+  const size_t lds = (size_t)(n * (n + 1) + n * kCqrNb) * sizeof(double);
   int rc;
   // pass 1: G = X^T X, L1, Q1 = X L1^{-T}
   if ((rc = ttx_gemm_f64(stream, 1, n, n, m, 1.0, A, a_cs, a_rs, 0, A, a_rs, a_cs, 0, 0.0, G, n, 1,
