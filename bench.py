@@ -180,6 +180,62 @@ def c5_high_dim(with_cpu: bool):
     return res
 
 
+def c4_serving(n_requests: int, with_cpu: bool, cpu_per_level: int):
+    """C4: the seeded Poisson-arrival stream of parametric 2D diffusion solves
+    (``ttnumpy_amd.serving``) on one engine per grid level; with ``with_cpu``
+    the oracle (one thread) solves ``cpu_per_level`` requests of each level
+    and a FIFO single-server queue over the same arrival times, with each
+    request's service time the mean sampled time of its level, gives the
+    CPU's p50/p99 and solves/s."""
+    import dataclasses
+    from ttnumpy_amd import problems as P
+    from ttnumpy_amd.serving import ServingConfig, build_requests, request_stream, serve
+    cfg = ServingConfig(n_requests=n_requests)
+    items = request_stream(cfg)
+    reqs = build_requests(cfg, items)
+    t0 = time.perf_counter()
+
+    def progress(k, r):
+        if k % 64 == 0 or k == n_requests:
+            print(f"# c4: {k} done at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+    rep, _ = serve(cfg, reqs, progress=progress)
+    out = {"gpu": dataclasses.asdict(rep)}
+    if with_cpu:
+        from threadpoolctl import threadpool_limits
+        from oracle import tt_oracle as O
+        service = {}
+        stops = {}
+        for L in sorted(set(it.level for it in items)):
+            sample = [it for it in items if it.level == L][:cpu_per_level]
+            parts = P.diffusion_2d_parts(L)
+            times = []
+            for it in sample:
+                A = O.Train(P.diffusion_2d_cores(L, it.theta, parts))
+                b = O.Train(P.ones_cores([2] * (2 * L)))
+                x0 = O.Train(P.random_train_cores([2] * (2 * L), 2, it.index))
+                with threadpool_limits(1):
+                    ts = time.perf_counter()
+                    _, io = O.amen(A, b, x0, symmetric=True, residual_tol=cfg.residual_tol,
+                                   max_sweeps=cfg.max_sweeps, seed=it.index)
+                    times.append(time.perf_counter() - ts)
+                stops.setdefault(L, []).append(str(io.stop_reason))
+            service[L] = float(np.mean(times))
+        free = 0.0
+        lat = []
+        for it in items:
+            start = max(free, it.offset)
+            free = start + service[it.level]
+            lat.append(free - it.offset)
+        out["cpu_oracle_fifo_1thread"] = {
+            "solves_per_s": n_requests / free, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+            "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "service_s_by_level": service, "stops_by_level": stops,
+            "sample": f"{cpu_per_level} oracle solves per level, FIFO queue simulated over the "
+                      f"same {n_requests} arrivals"}
+    return out
+
+
 def cpu_baseline(budget_s: float = 12.0):
     """The oracle's AMEn (FP64 NumPy/SciPy) on the same C1 inputs, one BLAS thread."""
     from threadpoolctl import threadpool_limits
@@ -214,6 +270,10 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the C2 rounding measurement")
     ap.add_argument("--no-c5", action="store_true", help="skip the C5 high-dimensional measurement")
+    ap.add_argument("--c4", type=int, default=0, metavar="N",
+                    help="instead of the C1 line, serve the C4 stream of N requests and print its "
+                         "report (with the oracle FIFO estimate unless --no-cpu)")
+    ap.add_argument("--c4-cpu-per-level", type=int, default=2)
     ap.add_argument("--single", action="store_true",
                     help="also time one request alone (its launches are not part of the roofline)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -235,6 +295,10 @@ def main():
             D.copy_nd([n], D.P(src), [1], D.P(dst), [1])
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_each_way": 8 * n, "launches": 3}))
+        return
+    if args.c4:
+        print(json.dumps({"c4_serving": c4_serving(args.c4, not args.no_cpu, args.c4_cpu_per_level)}),
+              flush=True)
         return
     A, b, x0 = build_inputs()
     eng = Engine(args.batch, rmax=RMAX)
