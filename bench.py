@@ -189,7 +189,8 @@ def c4_serving(n_requests: int, with_cpu: bool, cpu_per_level: int):
     CPU's p50/p99 and solves/s."""
     import dataclasses
     from ttnumpy_amd import problems as P
-    from ttnumpy_amd.serving import ServingConfig, build_requests, request_stream, serve
+    from ttnumpy_amd.serving import (ServingConfig, build_requests, fifo_latencies, request_stream,
+                                     serve)
     cfg = ServingConfig(n_requests=n_requests)
     items = request_stream(cfg)
     reqs = build_requests(cfg, items)
@@ -221,14 +222,10 @@ def c4_serving(n_requests: int, with_cpu: bool, cpu_per_level: int):
                     times.append(time.perf_counter() - ts)
                 stops.setdefault(L, []).append(str(io.stop_reason))
             service[L] = float(np.mean(times))
-        free = 0.0
-        lat = []
-        for it in items:
-            start = max(free, it.offset)
-            free = start + service[it.level]
-            lat.append(free - it.offset)
+        finish, lat = fifo_latencies([it.offset for it in items],
+                                     [service[it.level] for it in items])
         out["cpu_oracle_fifo_1thread"] = {
-            "solves_per_s": n_requests / free, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+            "solves_per_s": n_requests / finish[-1], "p50_ms": 1e3 * float(np.percentile(lat, 50)),
             "p99_ms": 1e3 * float(np.percentile(lat, 99)),
             "service_s_by_level": service, "stops_by_level": stops,
             "sample": f"{cpu_per_level} oracle solves per level, FIFO queue simulated over the "

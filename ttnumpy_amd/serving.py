@@ -169,6 +169,19 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
     return rep, done
 
 
+def fifo_latencies(arrivals: Sequence[float], service: Sequence[float]) -> tuple:
+    """One server taking requests in arrival order: ``(finish times, latencies)``
+    for arrival offsets (non-decreasing) and per-request service times."""
+    free = 0.0
+    finish, lat = [], []
+    for a, s in zip(arrivals, service):
+        start = max(free, a)
+        free = start + s
+        finish.append(free)
+        lat.append(free - a)
+    return finish, lat
+
+
 def main(argv=None) -> None:
     """Serve a seeded stream and print progress lines and the report as JSON."""
     import argparse
