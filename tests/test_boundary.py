@@ -69,3 +69,13 @@ def test_settings_validation_messages():
     assert s.truncation_tol == pytest.approx(1e-8 / 30 ** 0.5)
     assert s.use_iterative(1001) and not s.use_iterative(1000)
     assert str(s.iterative_solver) == "minres"
+
+
+def test_solver_enums_stringify_to_their_value():
+    """str-enums: str() and format() give the bare value (the reference's
+    SolverInfo.stop_reason reaches reports through str())."""
+    from ttnumpy_amd.solvers.common import LocalSolver, StopReason, TruncationMode
+    for enum_type in (StopReason, LocalSolver, TruncationMode):
+        for member in enum_type:
+            assert str(member) == member.value
+            assert f"{member}" == member.value
