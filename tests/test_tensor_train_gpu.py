@@ -1,0 +1,112 @@
+"""TensorTrain orthonormalization, SVD compression and truncation on the
+device path: the contract of the reference's own tests
+(tests/test_tensor_train.py: orthonormalize forms and centres, copy
+semantics, partial sweeps, centre validation, compress_svd values and
+centres, truncate without inplace)."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _tx():
+    import ttnumpy_amd as tx
+    return tx
+
+
+def _left_ok(core, tol=1e-12):
+    m = core.reshape(-1, core.shape[-1])
+    np.testing.assert_allclose(m.T @ m, np.eye(m.shape[1]), atol=tol)
+
+
+def _right_ok(core, tol=1e-12):
+    m = core.reshape(core.shape[0], -1)
+    np.testing.assert_allclose(m @ m.T, np.eye(m.shape[0]), atol=tol)
+
+
+def test_orthonormalize_forms_and_centres():
+    print("tt orthonormalize forms", flush=True)
+    tx = _tx()
+    t = tx.rand(((2, 3), (2, 3), (2, 3), (2, 3)), ranks=2, seed=3)
+    full = t.get_full_tensor()
+    d = t.n_cores
+    left = t.orthonormalize("left")
+    assert left.orth_center == d - 1
+    for c in left.to_numpy()[:-1]:
+        _left_ok(c)
+    right = t.orthonormalize("right")
+    assert right.orth_center == 0
+    for c in right.to_numpy()[1:]:
+        _right_ok(c)
+    mix = t.orthonormalize("mix")
+    assert mix.orth_center == d // 2
+    cores = mix.to_numpy()
+    for i in range(mix.orth_center):
+        _left_ok(cores[i])
+    for i in range(mix.orth_center + 1, d):
+        _right_ok(cores[i])
+    for u in (left, right, mix):
+        np.testing.assert_allclose(u.get_full_tensor(), full, rtol=1e-12, atol=1e-12 * np.abs(full).max())
+
+
+def test_orthonormalize_copy_semantics():
+    tx = _tx()
+    t = tx.rand((2, 3, 4), ranks=5, seed=101)
+    before = [c.copy() for c in t.to_numpy()]
+    o = t.orthonormalize("left")
+    assert o is not t and t.orth_center is None
+    for c0, c1 in zip(before, t.to_numpy()):
+        np.testing.assert_array_equal(c0, c1)
+    again = o.orthonormalize("left")  # already satisfied: still an independent copy
+    assert again is not o and again.cores[0] is not o.cores[0]
+    assert again.orth_center == o.orth_center
+
+
+def test_orthonormalize_partial_sweeps_and_centre_checks():
+    print("tt orthonormalize partial", flush=True)
+    tx = _tx()
+    t = tx.rand((2, 3, 4, 5), ranks=6, seed=102)
+    full = t.get_full_tensor()
+    part = t.orthonormalize("left", center_position=1)
+    assert part.orth_center is None
+    for u in (part, t.orthonormalize("right", center_position=2),
+              t.orthonormalize("mix", center_position=1)):
+        assert u.bonds == [c.shape[0] for c in u.to_numpy()] + [1]
+        np.testing.assert_allclose(u.get_full_tensor(), full, rtol=1e-11, atol=1e-12 * np.abs(full).max())
+    moved = t.orthonormalize("mix", center_position=1).orthonormalize("left", center_position=2)
+    assert moved.orth_center == 2
+    assert moved.tt_norm() == pytest.approx(np.linalg.norm(full), rel=1e-12)
+    t3 = tx.rand((2, 3, 4), ranks=5, seed=103)
+    for canonical in ("left", "right", "mix"):
+        for bad in (3, -1):
+            with pytest.raises(ValueError):
+                t3.orthonormalize(canonical, center_position=bad)
+
+
+def test_compress_svd_values_bounds_and_centres():
+    print("tt compress_svd", flush=True)
+    tx = _tx()
+    t = tx.rand((2, 3, 4), ranks=5, seed=104)
+    s = t.orthonormalize("left").compress_svd(1e-10)
+    np.testing.assert_allclose(s.get_full_tensor(), t.get_full_tensor(), rtol=1e-9)
+    assert s.orth_center == 0
+    m = tx.rand(((2, 2), (3, 3), (4, 4)), ranks=6, seed=105)
+    s2 = m.orthonormalize("right").compress_svd(1e-10, 3)
+    assert max(s2.bonds) < 4 and s2.orth_center == 2
+
+
+def test_truncate_copy_and_left_leaning_centre():
+    print("tt truncate copy", flush=True)
+    tx = _tx()
+    t = tx.rand((3, 4, 5, 4, 3), ranks=4, seed=106)
+    before = [c.copy() for c in t.to_numpy()]
+    full = t.get_full_tensor()
+    out = t.truncate(error=1e-12)
+    assert out is not t
+    for c0, c1 in zip(before, t.to_numpy()):
+        np.testing.assert_array_equal(c0, c1)
+    np.testing.assert_allclose(out.get_full_tensor(), full, rtol=1e-10, atol=1e-12 * np.abs(full).max())
+    leaning = t.orthonormalize("left", center_position=1)
+    out2 = leaning.truncate(error=1e-12)
+    np.testing.assert_allclose(out2.get_full_tensor(), full, rtol=1e-10, atol=1e-12 * np.abs(full).max())
