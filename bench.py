@@ -101,8 +101,9 @@ def verify_result(A, b, req):
 
 def c2_rounding(with_cpu: bool):
     """C2: (x1 + x2).truncate(error=1e-10, max_rank=96) for two d=40, mode-4,
-    rank-64 normal trains; FP64 and mixed (FP32 absorb contractions) on the GPU,
-    the oracle on one CPU thread."""
+    rank-64 normal trains; FP64, mixed (FP32 absorb contractions) and
+    randomized (method="randomized", seed 0) on the GPU, the oracle on one CPU
+    thread."""
     import torch
     import ttnumpy_amd as tx
     from ttnumpy_amd import problems as P
@@ -128,6 +129,17 @@ def c2_rounding(with_cpu: bool):
     res = {"gpu_fp64_ms": out["fp64"]["ms"], "gpu_mixed_ms": out["mixed"]["ms"],
            "bonds_identical": spectra["mixed"][1] == b64, "max_bond": max(b64),
            "mixed_max_rel_singular_value_diff": worst}
+    # randomized rounding (sketch bond 96 + 10 < the train's 128): time and its
+    # distance to the direct rounding
+    s.truncate(error=1e-10, max_rank=96, method="randomized", seed=0)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rnd = s.truncate(error=1e-10, max_rank=96, method="randomized", seed=0)
+    torch.cuda.synchronize()
+    res["gpu_randomized_ms"] = 1e3 * (time.perf_counter() - t0)
+    direct = s.truncate(error=1e-10, max_rank=96)
+    res["randomized_rel_diff_vs_direct"] = float(
+        (rnd - direct).tt_norm() / direct.tt_norm())
     if with_cpu:
         from threadpoolctl import threadpool_limits
         from oracle import tt_oracle as O
@@ -136,6 +148,9 @@ def c2_rounding(with_cpu: bool):
             t0 = time.perf_counter()
             O.truncate(so, error=1e-10, max_rank=96)
             res["cpu_oracle_1thread_ms"] = 1e3 * (time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            O.truncate(so, error=1e-10, max_rank=96, method="randomized", seed=0)
+            res["cpu_oracle_randomized_1thread_ms"] = 1e3 * (time.perf_counter() - t0)
     return res
 
 

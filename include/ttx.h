@@ -207,6 +207,10 @@ int ttx_ttm_core_f64(ttx_stream_t stream, const double* A, int64_t Ra, int64_t n
 int ttx_scale_cols_f64(ttx_stream_t stream, int64_t rows, int64_t cols, double* X, int64_t rs,
                        int64_t cs, const double* s);
 
+/* x[i] /= max_j |x[j]| when that maximum is positive: the scaling of each
+ * random sketch of the randomized rounding (ttnumpy/utils.py:187-189). */
+int ttx_scale_maxabs_f64(ttx_stream_t stream, int64_t n, double* x);
+
 /* truncated_bond with delta = scale * nrm[0] read on the device (ttnumpy/utils.py:388-390,437-440). */
 int ttx_truncated_bond_dev_f64(ttx_stream_t stream, const double* S, int64_t len,
                                const double* nrm, double scale, int64_t max_rank, int32_t* out);

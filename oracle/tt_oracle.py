@@ -23,7 +23,10 @@ Pinning (what checks each function, beyond retyping its formula):
   (``tests/test_utils.py:57-130``) -- see ``tests/test_oracle.py``.
 * ``truncate``: golden fixtures written by the reference itself
   (``tests/golden/c2_round_d40.npz``, ``round_small_*.npz``), the two-core
-  closed form ``error = ||sigma[2:]||/||sigma||`` and orthonormality checks.
+  closed form ``error = ||sigma[2:]||/||sigma||`` and orthonormality checks;
+  ``method="randomized"``: the reference's fixtures ``round_rand_*.npz``
+  (sketches below and above the train's ranks, both sweep directions) and
+  the exactness of the sketch when no unfolding has rank above its bond.
 * environments / local operator / local rhs: brute force against dense
   interface matrices on tiny trains (``x^T A x`` projections).
 * ``amen`` / ``als`` / ``relative_residual``: the golden trajectories written
@@ -34,6 +37,7 @@ Pinning (what checks each function, beyond retyping its formula):
 from __future__ import annotations
 
 import copy as _copy
+import numbers
 from dataclasses import dataclass, field
 from math import prod
 from typing import List, Optional, Tuple
@@ -253,15 +257,84 @@ def compress_svd(t: Train, error: float = 0.0, max_rank=None, canonical="auto", 
     return t
 
 
-def truncate(t: Train, error: float = 0.0, max_rank=None, canonical="auto", spectra=None) -> Train:
-    """Copy, QR sweep, SVD sweep (``tensor_train.py:852-940``, ``method="direct"``)."""
+def sketch_rank(max_rank, options) -> int:
+    """Bond of the random sketch of ``method="randomized"`` (``tensor_train.py:122-140``):
+    ``max_rank`` plus ``options["oversampling"]`` (default 10)."""
+    if max_rank is None:
+        raise ValueError("method='randomized' needs max_rank")
+    options = {} if options is None else options
+    unknown = set(options) - {"oversampling"}
+    if unknown:
+        raise ValueError(f"Unknown options for method='randomized': {sorted(unknown)}")
+    over = options.get("oversampling", 10)
+    if not isinstance(over, numbers.Integral) or over < 0:
+        raise ValueError(f"oversampling must be a non-negative integer, but got {over!r}")
+    return max_rank + int(over)
+
+
+def randomized_sweep_left(cores: List[np.ndarray], rank: int, rng: np.random.Generator) -> None:
+    """Randomize-then-orthogonalize, left form, in place (``utils.py:139-198``).
+
+    From the last core back, W_i = (core_i contracted with W_{i+1}) times a
+    standard-normal block of at most ``rank`` columns drawn from ``rng``
+    (cores d-1 .. 1 in that order), scaled by its largest magnitude; then
+    from the first core on, core_i <- Q of (unfolding_i @ W_{i+1}) and Q^T
+    unfolding_i is absorbed into core_{i+1}."""
+    if rank < 1:
+        raise ValueError(f"rank must be positive, but got {rank}")
+    d = len(cores)
+    w = [None] * d + [np.ones((1, 1))]
+    for i in range(d - 1, 0, -1):
+        c = _absorb_right(cores[i], w[i + 1])
+        c = c.reshape(c.shape[0], -1)
+        bond = min(rank, c.shape[0], c.shape[1])
+        g = rng.standard_normal((c.shape[1], bond))
+        wi = c @ g
+        top = np.abs(wi).max()
+        w[i] = wi / top if top > 0 else wi
+    for i in range(d - 1):
+        s = cores[i].shape
+        m = cores[i].reshape(s[0] * s[1] * s[2], s[3])
+        q, _ = sla.qr(m @ w[i + 1], mode="economic")
+        cores[i] = q.reshape(s[0], s[1], s[2], -1)
+        cores[i + 1] = _absorb_left(q.T @ m, cores[i + 1])
+
+
+def randomized_sweep_right(cores: List[np.ndarray], rank: int, rng: np.random.Generator) -> None:
+    """Mirror image of :func:`randomized_sweep_left` (``utils.py:201-227``): the
+    left form on the reversed train with the bonds of each core swapped."""
+    mirrored = [c.transpose(3, 1, 2, 0) for c in reversed(cores)]
+    randomized_sweep_left(mirrored, rank, rng)
+    cores[:] = [np.ascontiguousarray(c.transpose(3, 1, 2, 0)) for c in reversed(mirrored)]
+
+
+def truncate(t: Train, error: float = 0.0, max_rank=None, canonical="auto", spectra=None,
+             method: str = "direct", seed=None, options=None) -> Train:
+    """Copy, orthonormalizing sweep, SVD sweep (``tensor_train.py:852-940``).
+
+    ``method="randomized"`` replaces the QR sweep by the randomized sweep on a
+    sketch of bond ``sketch_rank(max_rank, options)`` drawn from
+    ``default_rng(seed)``."""
     if canonical not in ("auto", "left", "right"):
         raise ValueError(f"Canonical form must be 'left', 'right' or 'auto', but got {canonical}")
+    if method not in ("direct", "randomized"):
+        raise ValueError(f"Method must be 'direct' or 'randomized', but got {method!r}")
+    if method == "randomized":
+        rank = sketch_rank(max_rank, options)
     out = t.copy()
     form = "left" if canonical == "right" else "right"
     if canonical == "auto":
         form = "left" if (out.orth_center is None or out.orth_center >= out.d / 2) else "right"
-    orthonormalize(out, form)
+    if method == "direct":
+        orthonormalize(out, form)
+    else:
+        rng = np.random.default_rng(seed)
+        if form == "left":
+            randomized_sweep_left(out.cores, rank, rng)
+            out.orth_center = out.d - 1
+        else:
+            randomized_sweep_right(out.cores, rank, rng)
+            out.orth_center = 0
     return compress_svd(out, error, max_rank, spectra=spectra)
 
 

@@ -175,5 +175,47 @@ def main(reference_root: str) -> None:
         json.dump(meta, fh, indent=1, sort_keys=True)
 
 
+def randomized(reference_root: str) -> None:
+    """Randomized rounding (``truncate(method="randomized")``): sketches below
+    and above the train's ranks, both sweep directions, full output tensors."""
+    sys.path.insert(0, reference_root)
+    from ttnumpy import tensor_train
+
+    TT = tensor_train.TensorTrain
+    meta_path = os.path.join(HERE, "golden_meta.json")
+    with open(meta_path) as fh:
+        meta = json.load(fh)
+    for name, modes, rank, max_rank, over, err, pre, seed in (
+        ("round_rand_a", [3, 4, 2, 5, 3, 2], 6, 3, 2, 0.0, None, 7),
+        ("round_rand_b", [3, 4, 2, 5, 3, 2], 6, 4, 3, 1e-8, "right", 8),
+        ("round_rand_c", [2] * 8, 7, 4, 10, 0.0, None, 9),
+    ):
+        c1 = problems.random_train_cores(modes, rank, seed=11, normal=True)
+        c2 = problems.random_train_cores(modes, rank, seed=12, normal=True)
+        s = TT([c.copy() for c in c1]).add(TT([c.copy() for c in c2]))
+        if pre is not None:
+            s = s.orthonormalize(pre)
+        out = s.truncate(error=err, max_rank=max_rank, method="randomized", seed=seed,
+                         options={"oversampling": over})
+        full = np.asarray(out.get_full_tensor()).reshape(-1)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"),
+                            bonds=np.asarray(out.bonds, dtype=np.int64), full=full,
+                            orth_center=np.int64(out.orth_center))
+        meta[name] = dict(modes=modes, rank=rank, seeds=[11, 12], normal=True, error=err,
+                          max_rank=max_rank, oversampling=over, seed=seed,
+                          orthonormalized_first=pre,
+                          call="(x1 + x2)[.orthonormalize(pre)].truncate(error, max_rank, "
+                               "method='randomized', seed, options={'oversampling': over})")
+        print(name, out.bonds, flush=True)
+    with open(meta_path, "w") as fh:
+        json.dump(meta, fh, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    root = args[0] if args else "/root/reference"
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "randomized":
+        randomized(root)
+    else:
+        main(root)
+        randomized(root)

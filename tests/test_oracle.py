@@ -230,3 +230,56 @@ def test_poisson_operator_is_the_kronecker_sum():
     dense = np.kron(L, np.eye(m)) + np.kron(np.eye(m), L)
     np.testing.assert_allclose(A, dense, rtol=1e-12, atol=1e-9)
     assert O.Train(P.poisson_2d_cores(nb)).bonds == [1, 4, 4, 2, 4, 4, 1]
+
+
+RAND_CASES = [
+    ("round_rand_a", [3, 4, 2, 5, 3, 2], 6, 3, 2, 0.0, None, 7),
+    ("round_rand_b", [3, 4, 2, 5, 3, 2], 6, 4, 3, 1e-8, "right", 8),
+    ("round_rand_c", [2] * 8, 7, 4, 10, 0.0, None, 9),
+]
+
+
+def _rand_input(modes, rank, pre):
+    s = O.tt_add(O.Train(P.random_train_cores(modes, rank, 11, normal=True)),
+                 O.Train(P.random_train_cores(modes, rank, 12, normal=True)))
+    if pre is not None:
+        O.orthonormalize(s, pre)
+    return s
+
+
+@pytest.mark.parametrize("name,modes,rank,mr,over,err,pre,seed", RAND_CASES)
+def test_randomized_truncate_matches_reference(name, modes, rank, mr, over, err, pre, seed):
+    """Sketch bonds below (a, b) and above (c) the train's ranks, both sweep
+    directions (b starts right-orthonormal): the reference's bonds, centre
+    and full tensor."""
+    g = load_golden(name)
+    out = O.truncate(_rand_input(modes, rank, pre), err, mr, method="randomized", seed=seed,
+                     options={"oversampling": over})
+    assert out.bonds == list(g["bonds"])
+    assert out.orth_center == int(g["orth_center"])
+    np.testing.assert_allclose(out.full(), g["full"], rtol=1e-9, atol=1e-11 * np.abs(g["full"]).max())
+
+
+def test_randomized_sketch_is_exact_when_ranks_fit():
+    """With every unfolding rank at most the sketch bond, the randomized sweep
+    spans the train exactly, so rounding gives the direct result."""
+    s = _rand_input([3, 4, 2, 5, 3], 3, None)   # sum of two rank-3 trains: ranks <= 6
+    direct = O.truncate(s, 1e-12, 6)
+    for seed in (0, 1):
+        rnd = O.truncate(s, 1e-12, 6, method="randomized", seed=seed, options={"oversampling": 0})
+        assert rnd.bonds == direct.bonds
+        np.testing.assert_allclose(rnd.full(), direct.full(), rtol=1e-10,
+                                   atol=1e-12 * np.abs(direct.full()).max())
+
+
+def test_randomized_options_are_checked():
+    s = _rand_input([2, 3, 2], 2, None)
+    with pytest.raises(ValueError, match="needs max_rank"):
+        O.truncate(s, method="randomized")
+    with pytest.raises(ValueError, match="Unknown options"):
+        O.truncate(s, max_rank=2, method="randomized", options={"over": 1})
+    for bad in (-1, 1.5, "2"):
+        with pytest.raises(ValueError, match="oversampling"):
+            O.truncate(s, max_rank=2, method="randomized", options={"oversampling": bad})
+    with pytest.raises(ValueError, match="Method"):
+        O.truncate(s, max_rank=2, method="sketchy")

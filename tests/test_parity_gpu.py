@@ -255,3 +255,52 @@ def test_amen_c5_shape_small_matches_oracle():
     assert abs(info.residuals[0] - io.residuals[0]) <= 0.15 * io.residuals[0]
     assert info.local_solves > 0
     assert _rel_diff(x, xo) < 2e-6
+
+
+RAND_CASES = [
+    ("round_rand_a", [3, 4, 2, 5, 3, 2], 6, 3, 2, 0.0, None, 7),
+    ("round_rand_b", [3, 4, 2, 5, 3, 2], 6, 4, 3, 1e-8, "right", 8),
+    ("round_rand_c", [2] * 8, 7, 4, 10, 0.0, None, 9),
+]
+
+
+@pytest.mark.parametrize("name,modes,rank,mr,over,err,pre,seed", RAND_CASES)
+def test_truncate_randomized_matches_reference(name, modes, rank, mr, over, err, pre, seed):
+    """truncate(method="randomized"): the reference's bonds, centre and full
+    tensor, with sketches below (a, b) and above (c) the train's ranks and
+    both sweep directions (b starts right-orthonormal)."""
+    print(f"truncate randomized {name}", flush=True)
+    g = load_golden(name)
+    s = _tt(P.random_train_cores(modes, rank, 11, normal=True)).add(
+        _tt(P.random_train_cores(modes, rank, 12, normal=True)))
+    if pre is not None:
+        s = s.orthonormalize(pre)
+    out = s.truncate(error=err, max_rank=mr, method="randomized", seed=seed,
+                     options={"oversampling": over})
+    assert out.bonds == list(g["bonds"])
+    assert out.orth_center == int(g["orth_center"])
+    full = out.get_full_tensor().reshape(-1)
+    np.testing.assert_allclose(full, g["full"], rtol=1e-9, atol=1e-11 * np.abs(g["full"]).max())
+
+
+def test_truncate_randomized_exact_when_ranks_fit_and_c2_shape():
+    """With the sketch bond above every unfolding rank the randomized rounding
+    equals the direct one; on the C2 inputs (ranks 128 > sketch 106) it keeps
+    the max_rank bond and stays within the sketch's error of the direct result."""
+    print("truncate randomized exactness / C2 shape", flush=True)
+    s = _tt(P.random_train_cores([3, 4, 2, 5, 3], 3, 11, normal=True)).add(
+        _tt(P.random_train_cores([3, 4, 2, 5, 3], 3, 12, normal=True)))
+    direct = s.truncate(error=1e-12, max_rank=6)
+    rnd = s.truncate(error=1e-12, max_rank=6, method="randomized", seed=0,
+                     options={"oversampling": 0})
+    assert rnd.bonds == direct.bonds
+    ref = direct.get_full_tensor()
+    np.testing.assert_allclose(rnd.get_full_tensor(), ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    c1 = P.random_train_cores([4] * 40, 64, 1, normal=True)
+    c2 = P.random_train_cores([4] * 40, 64, 2, normal=True)
+    big = _tt(c1).add(_tt(c2))
+    out = big.truncate(error=1e-10, max_rank=96, method="randomized", seed=0)
+    so = O.truncate(O.tt_add(O.Train(c1), O.Train(c2)), 1e-10, 96, method="randomized", seed=0)
+    assert out.bonds == so.bonds
+    diff = O.tt_norm(O.tt_add(O.Train(out.to_numpy()), O.tt_scale(so, -1.0))) / O.tt_norm(so)
+    assert diff < 1e-10

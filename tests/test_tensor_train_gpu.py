@@ -110,3 +110,19 @@ def test_truncate_copy_and_left_leaning_centre():
     leaning = t.orthonormalize("left", center_position=1)
     out2 = leaning.truncate(error=1e-12)
     np.testing.assert_allclose(out2.get_full_tensor(), full, rtol=1e-10, atol=1e-12 * np.abs(full).max())
+
+
+def test_add_with_randomized_truncation_passes_seed_and_options():
+    """add(truncate=True, method="randomized", seed, options) rounds like
+    truncate(method="randomized") with the same seed and options."""
+    print("tt add randomized", flush=True)
+    tx = _tx()
+    a = tx.rand((3, 4, 2, 5), ranks=4, seed=21)
+    b = tx.rand((3, 4, 2, 5), ranks=4, seed=22)
+    opts = {"oversampling": 1}
+    x = a.add(b, truncate=True, max_rank=3, method="randomized", seed=5, options=opts)
+    y = a.add(b).truncate(max_rank=3, method="randomized", seed=5, options=opts)
+    assert x.bonds == y.bonds
+    np.testing.assert_allclose(x.get_full_tensor(), y.get_full_tensor(), rtol=1e-12, atol=1e-14)
+    with pytest.raises(ValueError, match="needs max_rank"):
+        a.add(b, truncate=True, method="randomized")

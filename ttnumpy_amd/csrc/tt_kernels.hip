@@ -80,6 +80,27 @@ __global__ __launch_bounds__(256) void scale_cols_kernel(int64_t rows, int64_t c
   }
 }
 
+// x /= max |x| when that is positive; one workgroup (the sketches it scales
+// hold at most a few ten thousand entries)
+__global__ __launch_bounds__(1024) void scale_maxabs_kernel(int64_t n, double* x) {
+  __shared__ double red[16];
+  __shared__ double s_top;
+  double m = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(x[i]));
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t = fmax(t, red[w]);
+    s_top = t;
+  }
+  __syncthreads();
+  const double top = s_top;
+  if (top > 0.0)
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) x[i] /= top;
+}
+
 __global__ void truncated_bond_dev_kernel(const double* S, int64_t len, const double* nrm,
                                           double scale, int64_t max_rank, int32_t* out) {
   if (threadIdx.x != 0) return;
@@ -151,6 +172,13 @@ extern "C" int ttx_scale_cols_f64(ttx_stream_t stream, int64_t rows, int64_t col
   if (rows == 0 || cols == 0) return TTX_OK;
   hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_of(rows * cols)), dim3(256), 0,
                      (hipStream_t)stream, rows, cols, X, rs, cs, s);
+  return launch_status();
+}
+
+extern "C" int ttx_scale_maxabs_f64(ttx_stream_t stream, int64_t n, double* x) {
+  if (n < 0) return TTX_EINVAL;
+  if (n == 0) return TTX_OK;
+  hipLaunchKernelGGL(scale_maxabs_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, n, x);
   return launch_status();
 }
 

@@ -150,6 +150,85 @@ def svd_sweep_right(cores: List[torch.Tensor], error: float, max_rank=None, spec
         bond = nb
 
 
+def _sketch_rank(max_rank, options) -> int:
+    """Bond of the random sketch of ``method="randomized"``: ``max_rank`` plus
+    ``options["oversampling"]`` (default 10) (``tensor_train.py:122-140``)."""
+    if max_rank is None:
+        raise ValueError("method='randomized' needs max_rank")
+    options = {} if options is None else options
+    unknown = set(options) - {"oversampling"}
+    if unknown:
+        raise ValueError(f"Unknown options for method='randomized': {sorted(unknown)}")
+    over = options.get("oversampling", 10)
+    if not isinstance(over, numbers.Integral) or over < 0:
+        raise ValueError(f"oversampling must be a non-negative integer, but got {over!r}")
+    return max_rank + int(over)
+
+
+def randomized_sweep_left(cores: List[torch.Tensor], rank: int, rng: np.random.Generator) -> None:
+    """Randomize-then-orthogonalize, left form, in place (``utils.py:139-198``).
+
+    From the last core back, W_i = (core_i contracted with W_{i+1}) times a
+    standard-normal block of at most ``rank`` columns, scaled by its largest
+    magnitude; then from the first core on, core_i becomes the Q factor of
+    unfolding_i W_{i+1} and Q^T unfolding_i is absorbed into core_{i+1}.  The
+    normal blocks are drawn on the host from ``rng`` (cores d-1 .. 1, the
+    reference's generator and order) and copied to the device; the products
+    run on the MFMA GEMM and the QRs on ``ttx_qr_f64``.
+    """
+    if rank < 1:
+        raise ValueError(f"rank must be positive, but got {rank}")
+    d = len(cores)
+    L = D.lib()
+    w: List[Optional[torch.Tensor]] = [None] * d + [D.to_device(np.ones((1, 1)))]
+    for i in range(d - 1, 0, -1):
+        c = cores[i]
+        r0, nr, nc, r1 = c.shape
+        l1 = w[i + 1].shape[1]
+        cols = nr * nc * l1
+        contracted = D.empty(r0 * nr * nc, l1)          # = (r0, cols) row-major
+        D.gemm(1, r0 * nr * nc, l1, r1, 1.0, P(c), r1, 1, 0, P(w[i + 1]), l1, 1, 0, 0.0,
+               P(contracted), l1, 1, 0)
+        bond = min(rank, r0, cols)
+        g = D.to_device(rng.standard_normal((cols, bond)))
+        wi = D.empty(r0, bond)
+        D.gemm(1, r0, bond, cols, 1.0, P(contracted), cols, 1, 0, P(g), bond, 1, 0, 0.0, P(wi),
+               bond, 1, 0)
+        _lib.check(L.ttx_scale_maxabs_f64(D.stream(), r0 * bond, P(wi)), "ttx_scale_maxabs_f64")
+        w[i] = wi
+    for i in range(d - 1):
+        c = cores[i]
+        r0, nr, nc, r1 = c.shape
+        m = r0 * nr * nc
+        l1 = w[i + 1].shape[1]
+        mw = D.empty(m, l1)
+        D.gemm(1, m, l1, r1, 1.0, P(c), r1, 1, 0, P(w[i + 1]), l1, 1, 0, 0.0, P(mw), l1, 1, 0)
+        k = min(m, l1)
+        Q = D.empty(r0, nr, nc, k)
+        R = D.empty(k, l1)
+        D.qr(m, l1, P(mw), l1, 1, P(Q), k, 1, P(R), l1, 1)
+        qtm = D.empty(k, r1)                             # Q^T unfolding_i
+        D.gemm(1, k, r1, m, 1.0, P(Q), 1, k, 0, P(c), r1, 1, 0, 0.0, P(qtm), r1, 1, 0)
+        cores[i] = Q
+        cores[i + 1] = D.absorb_left(P(qtm), k, r1, cores[i + 1])
+
+
+def _mirror(c: torch.Tensor) -> torch.Tensor:
+    """The core with its bonds swapped, (r, n, m, r') -> (r', n, m, r)."""
+    r0, nr, nc, r1 = c.shape
+    out = D.empty(r1, nr, nc, r0)
+    D.copy_nd([r0, nr, nc, r1], P(c), [nr * nc * r1, nc * r1, r1, 1], P(out),
+              [1, nc * r0, r0, nr * nc * r0])
+    return out
+
+
+def randomized_sweep_right(cores: List[torch.Tensor], rank: int, rng: np.random.Generator) -> None:
+    """Mirror image of :func:`randomized_sweep_left` (``utils.py:201-227``)."""
+    mirrored = [_mirror(c) for c in reversed(cores)]
+    randomized_sweep_left(mirrored, rank, rng)
+    cores[:] = [_mirror(c) for c in reversed(mirrored)]
+
+
 # ---------------------------------------------------------------- container
 class TensorTrain:
     """Tensor train / tensor train matrix with device cores (``tensor_train.py:150-244``)."""
@@ -289,8 +368,13 @@ class TensorTrain:
                  precision: str = "fp64") -> "TensorTrain":
         """Round to minimal ranks: QR sweep then SVD sweep (``tensor_train.py:852-940``).
 
-        ``precision="mixed"`` runs the absorb contractions of both sweeps in
-        FP32 (FP32 MFMA, FP32 accumulation) and keeps the QR and SVD in FP64.
+        ``method="randomized"`` replaces the QR sweep by the randomized sweep
+        (:func:`randomized_sweep_left` or its mirror image) on a sketch of bond
+        ``max_rank + options["oversampling"]`` (default 10) drawn from
+        ``default_rng(seed)``; it needs ``max_rank`` and ``error`` bounds the
+        SVD truncation only.  ``precision="mixed"`` runs the absorb
+        contractions of both sweeps in FP32 (FP32 MFMA, FP32 accumulation) and
+        keeps the QR and SVD in FP64.
         """
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
@@ -299,7 +383,7 @@ class TensorTrain:
         if method not in ("direct", "randomized"):
             raise ValueError(f"Method must be 'direct' or 'randomized', but got {method!r}")
         if method == "randomized":
-            raise NotImplementedError("method='randomized' is not part of this build")
+            rank = _sketch_rank(max_rank, options)
         if max_rank is not None and max_rank <= 0:
             raise ValueError("max_rank must be a positive integer.")
         t = self if inplace else self.copy()
@@ -307,7 +391,17 @@ class TensorTrain:
         if canonical == "auto":
             form = "left" if (t.orth_center is None or t.orth_center >= t.n_cores / 2) else "right"
         with D.contractions("fp32" if precision == "mixed" else "fp64"):
-            t.orthonormalize(canonical=form, inplace=True)
+            if method == "direct":
+                t.orthonormalize(canonical=form, inplace=True)
+            else:
+                rng = np.random.default_rng(seed)
+                if form == "left":
+                    randomized_sweep_left(t.cores, rank, rng)
+                    t.orth_center = t.n_cores - 1
+                else:
+                    randomized_sweep_right(t.cores, rank, rng)
+                    t.orth_center = 0
+                t._sync_bonds()
             t.compress_svd(error, max_rank, inplace=True, spectra=spectra)
         return t
 
@@ -333,7 +427,7 @@ class TensorTrain:
             D.copy_nd(list(b.shape), P(b), [nr * nc * b.shape[3], nc * b.shape[3], b.shape[3], 1], P(c, off), st)
             cores.append(c)
         res = TensorTrain(cores)
-        return _round(res, truncate, error, max_rank, method)
+        return _round(res, truncate, error, max_rank, method, seed, options)
 
     def __add__(self, other):
         return self.add(other)
@@ -349,7 +443,7 @@ class TensorTrain:
         k = res.orth_center if res.orth_center is not None else 0
         c = res.cores[k]
         D.copy_nd([c.numel()], P(c), [1], P(c), [1], float(other))
-        return _round(res, truncate, error, max_rank, method)
+        return _round(res, truncate, error, max_rank, method, seed, options)
 
     def __mul__(self, other):
         return self.multiply(other)
@@ -380,7 +474,7 @@ class TensorTrain:
             c = D.empty(Ra * rx, nr, 1, Rb * rx2)
             D.ttm_core(a, x, P(c), nr * Rb * rx2, Rb * rx2, 1)
             cores.append(c)
-        return _round(TensorTrain(cores), truncate, error, max_rank, method)
+        return _round(TensorTrain(cores), truncate, error, max_rank, method, seed, options)
 
     def __matmul__(self, other):
         return self.dot(other)
@@ -412,13 +506,15 @@ class TensorTrain:
         return t.reshape(shape).cpu().numpy()
 
 
-def _round(res: TensorTrain, truncate: bool, error, max_rank, method) -> TensorTrain:
+def _round(res: TensorTrain, truncate: bool, error, max_rank, method, seed=None,
+           options=None) -> TensorTrain:
     """``_round_result`` (``tensor_train.py:94-119``)."""
     if not truncate:
         if method != "direct" or error is not None or max_rank is not None:
             raise ValueError("error, max_rank and method need truncate=True")
         return res
-    return res.truncate(error=0.0 if error is None else error, max_rank=max_rank, inplace=True, method=method)
+    return res.truncate(error=0.0 if error is None else error, max_rank=max_rank, inplace=True,
+                        method=method, seed=seed, options=options)
 
 
 def _ranks(d: int, ranks) -> List[int]:
