@@ -10,7 +10,8 @@ It restates, in FP64 NumPy/SciPy, the algorithm of ttnumpy 0.3.0 for
   (``ttnumpy/tensor_train.py:705-940``, ``ttnumpy/utils.py:22-136,336-460``);
 * the one-site AMEn and ALS solvers (``ttnumpy/solvers/amen.py:80-725``,
   ``ttnumpy/solvers/als.py:56-212``, ``ttnumpy/solvers/one_site.py:27-209``,
-  ``ttnumpy/solvers/common.py:118-929``);
+  ``ttnumpy/solvers/common.py:118-929``) and the two-site MALS
+  (``ttnumpy/solvers/mals.py:61-292``, ``ttnumpy/solvers/two_site.py:28-241``);
 * the per-sweep residual ``||A x - b|| / ||b||`` in TT arithmetic
   (``common.py:118-146`` via ``tensor_train.py:254-540``).
 
@@ -875,6 +876,117 @@ def amen(A: Train, b: Train, x0: Optional[Train] = None, kickrank=4, error_thres
     if redundant:
         orthonormalize(x, "left", redundant[-1])
         orthonormalize(x, "right", 0)
+    return x, info
+
+
+def _merge_pair(c0: np.ndarray, c1: np.ndarray) -> np.ndarray:
+    """Two neighbouring cores (r, m0, n0, s), (s, m1, n1, t) as one core
+    (r, m0 m1, n0 n1, t): a two-site system is the one-site system of this
+    core (``two_site.py:28-119`` index the rows (bond, m0, m1, bond) and the
+    columns (bond, n0, n1, bond) the same way)."""
+    r, m0, n0, _ = c0.shape
+    _, m1, n1, t = c1.shape
+    both = np.tensordot(c0, c1, axes=([3], [0]))          # r m0 n0 m1 n1 t
+    return both.transpose(0, 1, 3, 2, 4, 5).reshape(r, m0 * m1, n0 * n1, t)
+
+
+def _solve_pair(k, x, bonds, A, b, eL, eR, bL, bR, settings, info, center, error_threshold,
+                max_rank, damp):
+    """Assemble, solve and SVD-split the two-site system at (k, k+1)
+    (``two_site.py:147-241``).  ``bonds`` are the current bonds (the cores at
+    k+1 may be stale mid-sweep); ``center`` = (site, block) replaces the stale
+    core in the warm start (``two_site.py:122-144``)."""
+    rows = bonds[k] * x.row_modes[k]
+    cols = x.row_modes[k + 1] * bonds[k + 2]
+    n = rows * cols
+    Ak = _merge_pair(A.cores[k], A.cores[k + 1])
+    b_loc = local_rhs(bL[k], _c3(_merge_pair(b.cores[k], b.cores[k + 1])), bR[k + 1])
+    residual_mode = settings.residual_tol is not None
+    if settings.use_iterative(n):
+        op = _LocalOp(eL[k], Ak, eR[k + 1])
+        left, right = _c3(x.cores[k]), _c3(x.cores[k + 1])
+        if center is not None:
+            if center[0] == k:
+                left = center[1]
+            elif center[0] == k + 1:
+                right = center[1]
+        warm = None
+        if left.shape[-1] == right.shape[0]:
+            warm = np.tensordot(left, right, axes=(2, 0)).reshape(-1)
+            if warm.size != n:
+                warm = None
+        fb = (lambda: local_matrix(eL[k], Ak, eR[k + 1])) if settings.allow_dense_fallback else None
+        u, op = solve_local(op, b_loc, settings.iterative, settings.local_rtol, warm, fb, info)
+    else:
+        op = local_matrix(eL[k], Ak, eR[k + 1])
+        u, op = solve_local(op, b_loc, None, None)
+    exact = float(np.linalg.norm(op @ u - b_loc)) if residual_mode else 0.0
+    U, S, Vt = sla.svd(u.reshape(rows, cols), full_matrices=False, check_finite=False,
+                       lapack_driver="gesvd")
+    nb = _choose_rank(settings, op, b_loc, U, S, Vt, exact, error_threshold, max_rank, damp)
+    return U, S, Vt, nb
+
+
+def mals(A: Train, b: Train, x0: Optional[Train] = None, max_rank=None, max_sweeps=2,
+         residual_tol=None, residual_damp=2.0, stagnation_factor="auto", error_threshold=1e-7,
+         local_solver="auto", direct_solve_size=1000, local_rtol=None, allow_dense_fallback=False,
+         symmetric=False, return_info=True):
+    """Two-site MALS (``mals.py:61-292``).  Returns ``(x, Info)``.
+
+    Left-to-right over pairs (k, k+1), k < d-2: core k <- U_r, the block
+    S_r V_r kept as the centre; right-to-left over all pairs: core k+1 <- V_r,
+    the block U_r S_r kept, and at k = 0 stored as core 0.  Truncations per
+    sweep: 2d - 3 (the residual budget is shared among them)."""
+    if max_rank is not None and max_rank <= 0:
+        raise ValueError(f"max_rank must be a positive integer, got {max_rank}")
+    if residual_damp < 1.0:
+        raise ValueError(f"residual_damp must be >= 1, got {residual_damp}")
+    st = resolve_settings(max_sweeps, residual_tol, stagnation_factor, local_solver, symmetric,
+                          direct_solve_size, return_info, local_rtol, allow_dense_fallback,
+                          max(2 * A.d - 3, 1))
+    if x0 is None:
+        x0 = Train([np.ones((b.bonds[i], b.row_modes[i], 1, b.bonds[i + 1])) for i in range(b.d)])
+    _validate(A, b, x0)
+    x = x0.copy()
+    if x.orth_center != 0:
+        orthonormalize(x, "right")
+    d = x.d
+    bonds = list(x.bonds)
+    eA_L, eA_R, eb_L, eb_R = [None] * d, [None] * d, [None] * d, [None] * d
+    for i in range(d - 1, 0, -1):
+        build_env(eA_R, i, "right", x, x, A)
+        build_env(eb_R, i, "right", x, b)
+    info = Info()
+    b_norm = tt_norm(b) if st.track_residual else None
+    center = None
+    for sweep in range(st.max_sweeps):
+        for k in range(d - 1):
+            build_env(eA_L, k, "left", x, x, A)
+            build_env(eb_L, k, "left", x, b)
+            if k == d - 2:
+                continue
+            U, S, Vt, nb = _solve_pair(k, x, bonds, A, b, eA_L, eA_R, eb_L, eb_R, st, info, center,
+                                       error_threshold, max_rank, residual_damp)
+            n1 = x.row_modes[k + 1]
+            center = (k + 1, (S[:nb, None] * Vt[:nb]).reshape(nb, n1, bonds[k + 2]))
+            x.cores[k] = U[:, :nb].reshape(bonds[k], x.row_modes[k], 1, nb)
+            bonds[k + 1] = nb
+            x.orth_center = k + 1
+        for k in range(d - 2, -1, -1):
+            build_env(eA_R, k + 1, "right", x, x, A)
+            build_env(eb_R, k + 1, "right", x, b)
+            U, S, Vt, nb = _solve_pair(k, x, bonds, A, b, eA_L, eA_R, eb_L, eb_R, st, info, center,
+                                       error_threshold, max_rank, residual_damp)
+            n0, n1 = x.row_modes[k], x.row_modes[k + 1]
+            us = U[:, :nb] * S[:nb]
+            center = (k, us.reshape(bonds[k], n0, nb))
+            x.cores[k + 1] = Vt[:nb].reshape(nb, n1, 1, bonds[k + 2])
+            bonds[k + 1] = nb
+            if k == 0:
+                x.cores[0] = us.reshape(bonds[0], n0, 1, nb)
+            x.orth_center = k
+        if _record(st, info, A, b, x, b_norm, sweep, max_bond=max(x.bonds)):
+            break
     return x, info
 
 

@@ -211,11 +211,59 @@ def randomized(reference_root: str) -> None:
         json.dump(meta, fh, indent=1, sort_keys=True)
 
 
+def mals_cases(reference_root: str) -> None:
+    """Two-site MALS trajectories: residual mode (direct and MINRES local
+    solves) and threshold mode, with the full solutions."""
+    sys.path.insert(0, reference_root)
+    from ttnumpy import tensor_train
+    from ttnumpy.solvers.mals import mals
+
+    TT = tensor_train.TensorTrain
+    meta_path = os.path.join(HERE, "golden_meta.json")
+    with open(meta_path) as fh:
+        meta = json.load(fh)
+    h = 1.0 / (2**6 + 1)
+    cases = (
+        ("mals_lap1d6", TT(problems.laplacian_1d_cores(6, h)), TT(problems.ones_cores([2] * 6)),
+         dict(residual_tol=1e-10, max_sweeps=4, symmetric=True), "laplacian_1d_cores(6, 1/65)"),
+        ("mals_poisson16", TT(problems.poisson_2d_cores(4)), TT(problems.ones_cores([2] * 8)),
+         dict(residual_tol=1e-8, max_sweeps=6, symmetric=True), "poisson_2d_cores(4)"),
+        ("mals_poisson16_minres", TT(problems.poisson_2d_cores(4)),
+         TT(problems.ones_cores([2] * 8)),
+         dict(residual_tol=1e-8, max_sweeps=6, symmetric=True, direct_solve_size=16),
+         "poisson_2d_cores(4)"),
+        ("mals_poisson8_threshold", TT(problems.poisson_2d_cores(3)),
+         TT(problems.ones_cores([2] * 6)),
+         dict(error_threshold=1e-9, max_sweeps=2, symmetric=True, stagnation_factor=None),
+         "poisson_2d_cores(3)"),
+    )
+    for name, A, b, kw, op in cases:
+        d = A.n_cores
+        x0 = tensor_train.rand([2] * d, ranks=2, seed=0)
+        x, info = mals(A, b, x0=x0, return_info=True, **kw)
+        full = np.asarray(x.get_full_tensor()).reshape(-1)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"),
+                            bonds=np.asarray(x.bonds, dtype=np.int64), full=full,
+                            **_info_dict(info))
+        meta[name] = dict(operator=op, b="ones", x0=f"rand([2]*{d}, ranks=2, seed=0)",
+                          call=f"mals(x0=x0, return_info=True, **{kw})",
+                          bonds=list(map(int, x.bonds)), sweeps=int(info.num_sweeps_run),
+                          stop=str(info.stop_reason),
+                          residuals=[float(r) for r in info.residuals])
+        print(name, x.bonds, info.num_sweeps_run, info.stop_reason, info.residuals, flush=True)
+    with open(meta_path, "w") as fh:
+        json.dump(meta, fh, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     root = args[0] if args else "/root/reference"
-    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "randomized":
+    only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
+    if only == "randomized":
         randomized(root)
+    elif only == "mals":
+        mals_cases(root)
     else:
         main(root)
         randomized(root)
+        mals_cases(root)

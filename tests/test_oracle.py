@@ -8,6 +8,7 @@ answers are the values the reference's own tests assert
 
 import numpy as np
 import pytest
+from threadpoolctl import threadpool_limits
 
 from conftest import load_golden, unpack_cores
 from oracle import tt_oracle as O
@@ -283,3 +284,70 @@ def test_randomized_options_are_checked():
             O.truncate(s, max_rank=2, method="randomized", options={"oversampling": bad})
     with pytest.raises(ValueError, match="Method"):
         O.truncate(s, max_rank=2, method="sketchy")
+
+
+MALS_CASES = [
+    ("mals_lap1d6", "lap1d", 6, dict(residual_tol=1e-10, max_sweeps=4, symmetric=True)),
+    ("mals_poisson16", "poisson", 4, dict(residual_tol=1e-8, max_sweeps=6, symmetric=True)),
+    ("mals_poisson16_minres", "poisson", 4,
+     dict(residual_tol=1e-8, max_sweeps=6, symmetric=True, direct_solve_size=16)),
+    ("mals_poisson8_threshold", "poisson", 3,
+     dict(error_threshold=1e-9, max_sweeps=2, symmetric=True, stagnation_factor=None)),
+]
+
+
+def _mals_system(kind, n):
+    if kind == "lap1d":
+        return (O.Train(P.laplacian_1d_cores(n, 1.0 / (2 ** n + 1))), O.Train(P.ones_cores([2] * n)), n)
+    return O.Train(P.poisson_2d_cores(n)), O.Train(P.ones_cores([2] * (2 * n))), 2 * n
+
+
+@pytest.mark.parametrize("name,kind,n,kw", MALS_CASES)
+def test_mals_matches_reference(name, kind, n, kw):
+    """Two-site MALS: the reference's bonds, sweeps, stop reason and solution;
+    residual histories to the accuracy the local solves leave (the last
+    entries sit at round-off)."""
+    g = load_golden(name)
+    A, b, d = _mals_system(kind, n)
+    x0 = O.Train(P.random_train_cores([2] * d, 2, 0))
+    with threadpool_limits(1):
+        x, info = O.mals(A, b, x0, **kw)
+    assert x.bonds == list(g["bonds"])
+    assert info.num_sweeps_run == int(g["num_sweeps_run"])
+    assert info.stop_reason == str(g["stop_reason"])
+    r, rg = np.array(info.residuals), g["residuals"]
+    assert len(r) == len(rg)
+    for a, c in zip(r, rg):
+        if c > 1e-8:
+            assert abs(a - c) <= 1e-5 * c
+        elif c > 1e-12:  # set by the local solves' tolerance (1e-9 relative)
+            assert abs(a - c) <= 1e-2 * c
+        else:  # round-off level on both sides
+            assert a <= 1e-12
+    ref = g["full"]
+    assert np.linalg.norm(x.full() - ref) <= 1e-8 * np.linalg.norm(ref)
+
+
+def test_mals_two_site_operator_is_the_projection():
+    """Brute force: the two-site local matrix of a random frame equals the
+    dense operator projected onto span(left frame) x I x I x span(right frame)."""
+    rng = np.random.default_rng(4)
+    d, n = 4, 2
+    A = O.Train([rng.standard_normal((r0, n, n, r1))
+                 for r0, r1 in zip([1, 2, 3, 2], [2, 3, 2, 1])])
+    x = O.Train([rng.standard_normal((r0, n, 1, r1))
+                 for r0, r1 in zip([1, 2, 2, 2], [2, 2, 2, 1])])
+    O.orthonormalize(x, "mix", 1)
+    eL, eR = [None] * d, [None] * d
+    for k in range(d - 1, -1, -1):
+        O.build_env(eR, k, "right", x, x, A)
+    for k in range(d):
+        O.build_env(eL, k, "left", x, x, A)
+    k = 1
+    M = O.local_matrix(eL[k], O._merge_pair(A.cores[k], A.cores[k + 1]), eR[k + 1])
+    # frames: left = core 0 (2 x 2 -> (n, r1)), right = core 3 ((r3, n))
+    Lf = x.cores[0].reshape(n, -1)
+    Rf = x.cores[3].reshape(-1, n)
+    Pm = np.kron(np.kron(Lf, np.eye(n * n)), Rf.T)        # (16 x 2*4*2)
+    dense = A.dense_matrix()
+    np.testing.assert_allclose(M, Pm.T @ dense @ Pm, atol=1e-12)
