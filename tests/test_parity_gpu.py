@@ -304,3 +304,35 @@ def test_truncate_randomized_exact_when_ranks_fit_and_c2_shape():
     assert out.bonds == so.bonds
     diff = O.tt_norm(O.tt_add(O.Train(out.to_numpy()), O.tt_scale(so, -1.0))) / O.tt_norm(so)
     assert diff < 1e-10
+
+
+MALS_CASES = [
+    ("mals_lap1d6", "lap1d", 6, dict(residual_tol=1e-10, max_sweeps=4, symmetric=True)),
+    ("mals_poisson16", "poisson", 4, dict(residual_tol=1e-8, max_sweeps=6, symmetric=True)),
+    ("mals_poisson16_minres", "poisson", 4,
+     dict(residual_tol=1e-8, max_sweeps=6, symmetric=True, direct_solve_size=16)),
+    ("mals_poisson8_threshold", "poisson", 3,
+     dict(error_threshold=1e-9, max_sweeps=2, symmetric=True, stagnation_factor=None)),
+]
+
+
+@pytest.mark.parametrize("name,kind,n,kw", MALS_CASES)
+def test_mals_matches_reference(name, kind, n, kw):
+    """Two-site MALS on the device: the reference's bonds, sweep count, stop
+    reason and solution (direct and MINRES local solves, residual and
+    threshold rank rules)."""
+    import ttnumpy_amd as tx
+    print(f"mals {name}", flush=True)
+    g = load_golden(name)
+    if kind == "lap1d":
+        A, b, d = _tt(P.laplacian_1d_cores(n, 1.0 / (2 ** n + 1))), _tt(P.ones_cores([2] * n)), n
+    else:
+        A, b, d = _tt(P.poisson_2d_cores(n)), _tt(P.ones_cores([2] * (2 * n))), 2 * n
+    x0 = tx.rand([2] * d, ranks=2, seed=0)
+    x, info = tx.mals(A, b, x0=x0, return_info=True, **kw)
+    assert x.bonds == list(g["bonds"])
+    assert info.num_sweeps_run == int(g["num_sweeps_run"])
+    assert str(info.stop_reason) == str(g["stop_reason"])
+    ref = g["full"]
+    full = x.get_full_tensor().reshape(-1)
+    assert np.linalg.norm(full - ref) <= 1e-8 * np.linalg.norm(ref)
