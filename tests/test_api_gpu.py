@@ -15,7 +15,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 from oracle import tt_oracle as O  # noqa: E402
 from ttnumpy_amd import problems as P  # noqa: E402
 
-SOLVERS = ["amen", "als"]
+SOLVERS = ["amen", "als", "mals"]
 
 
 def _tx():
@@ -72,16 +72,18 @@ def test_zero_right_hand_side_gives_zero(name):
     assert info.stop_reason is not None
 
 
-def test_amen_rank_cap_and_bad_caps():
-    print("api amen max_rank", flush=True)
+@pytest.mark.parametrize("name", ["amen", "mals"])
+def test_rank_cap_and_bad_caps(name):
+    print(f"api {name} max_rank", flush=True)
     tx = _tx()
     A = tx.TensorTrain(P.poisson_2d_cores(3))
     b = tx.TensorTrain(P.ones_cores([2] * 6))
-    x = tx.amen(A, b, max_rank=3, max_sweeps=4, residual_tol=1e-12, symmetric=True, seed=0)
+    kw = dict(seed=0) if name == "amen" else {}
+    x = _solver(name)(A, b, max_rank=3, max_sweeps=4, residual_tol=1e-12, symmetric=True, **kw)
     assert max(x.bonds) <= 3
     for bad in (0, -2):
         with pytest.raises(ValueError):
-            tx.amen(A, b, max_rank=bad)
+            _solver(name)(A, b, max_rank=bad)
 
 
 def test_rejects_a_ttm_right_hand_side():
