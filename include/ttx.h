@@ -207,6 +207,13 @@ int ttx_ttm_core_f64(ttx_stream_t stream, const double* A, int64_t Ra, int64_t n
 int ttx_scale_cols_f64(ttx_stream_t stream, int64_t rows, int64_t cols, double* X, int64_t rs,
                        int64_t cs, const double* s);
 
+/* out[(a m), i, j, (b n)] = A[a, i, j, b] * B[m, i, j, n] for cores A (ra, ni, nj, rb)
+ * and B (sa, ni, nj, sb): one core of the element-wise product of two trains
+ * (ttnumpy/tensor_train.py:380-395, TensorTrain.multiply). */
+int ttx_hadamard_core_f64(ttx_stream_t stream, const double* A, int64_t ra, int64_t ni,
+                          int64_t nj, int64_t rb, const double* B, int64_t sa, int64_t sb,
+                          double* out);
+
 /* x[i] /= max_j |x[j]| when that maximum is positive: the scaling of each
  * random sketch of the randomized rounding (ttnumpy/utils.py:187-189). */
 int ttx_scale_maxabs_f64(ttx_stream_t stream, int64_t n, double* x);

@@ -359,6 +359,17 @@ def tt_add(a: Train, b: Train) -> Train:
     return Train(out)
 
 
+def tt_multiply(a: Train, b: Train) -> Train:
+    """Element-wise product: core (r s, m, n, r' s') with entries
+    a[x, m, n, y] b[u, m, n, v] at ((x, u), m, n, (y, v)) (``tensor_train.py:380-395``)."""
+    cores = []
+    for p, q in zip(a.cores, b.cores):
+        c = np.einsum("xmny,umnv->xumnyv", p, q)
+        cores.append(c.reshape(p.shape[0] * q.shape[0], p.shape[1], p.shape[2],
+                               p.shape[3] * q.shape[3]))
+    return Train(cores)
+
+
 def tt_scale(a: Train, alpha: float) -> Train:
     """Scalar multiple on the centre core, or core 0 (``tensor_train.py:396-401``)."""
     out = a.copy()

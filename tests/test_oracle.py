@@ -351,3 +351,14 @@ def test_mals_two_site_operator_is_the_projection():
     Pm = np.kron(np.kron(Lf, np.eye(n * n)), Rf.T)        # (16 x 2*4*2)
     dense = A.dense_matrix()
     np.testing.assert_allclose(M, Pm.T @ dense @ Pm, atol=1e-12)
+
+
+def test_elementwise_product_closed_form():
+    """full(a * b) = full(a) * full(b), for trains and for operator cores."""
+    a = O.Train(P.random_train_cores([3, 4, 2, 3], 3, 1, normal=True))
+    b = O.Train(P.random_train_cores([3, 4, 2, 3], 2, 2, normal=True))
+    ab = O.tt_multiply(a, b)
+    np.testing.assert_allclose(ab.full(), a.full() * b.full(), rtol=1e-12, atol=1e-13)
+    assert ab.bonds == [x * y for x, y in zip(a.bonds, b.bonds)]
+    A = O.Train(P.poisson_2d_cores(2))
+    np.testing.assert_allclose(O.tt_multiply(A, A).dense_matrix(), A.dense_matrix() ** 2, atol=1e-12)

@@ -80,6 +80,25 @@ __global__ __launch_bounds__(256) void scale_cols_kernel(int64_t rows, int64_t c
   }
 }
 
+// out[(a, m), i, j, (b, n)] = A[a, i, j, b] B[m, i, j, n]: the core of the
+// element-wise product of two trains (bonds multiply, modes shared)
+__global__ __launch_bounds__(256) void hadamard_core_kernel(const double* A, int64_t ra, int64_t ni,
+                                                           int64_t nj, int64_t rb, const double* B,
+                                                           int64_t sa, int64_t sb, double* out) {
+  const int64_t ob = rb * sb, total = ra * sa * ni * nj * ob;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    int64_t rem = e;
+    const int64_t n = rem % sb; rem /= sb;
+    const int64_t b = rem % rb; rem /= rb;
+    const int64_t j = rem % nj; rem /= nj;
+    const int64_t i = rem % ni; rem /= ni;
+    const int64_t m = rem % sa;
+    const int64_t a = rem / sa;
+    out[e] = A[((a * ni + i) * nj + j) * rb + b] * B[((m * ni + i) * nj + j) * sb + n];
+  }
+}
+
 // x /= max |x| when that is positive; one workgroup (the sketches it scales
 // hold at most a few ten thousand entries)
 __global__ __launch_bounds__(1024) void scale_maxabs_kernel(int64_t n, double* x) {
@@ -172,6 +191,15 @@ extern "C" int ttx_scale_cols_f64(ttx_stream_t stream, int64_t rows, int64_t col
   if (rows == 0 || cols == 0) return TTX_OK;
   hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_of(rows * cols)), dim3(256), 0,
                      (hipStream_t)stream, rows, cols, X, rs, cs, s);
+  return launch_status();
+}
+
+extern "C" int ttx_hadamard_core_f64(ttx_stream_t stream, const double* A, int64_t ra, int64_t ni,
+                                     int64_t nj, int64_t rb, const double* B, int64_t sa,
+                                     int64_t sb, double* out) {
+  if (ra < 1 || ni < 1 || nj < 1 || rb < 1 || sa < 1 || sb < 1) return TTX_EINVAL;
+  hipLaunchKernelGGL(hadamard_core_kernel, dim3(grid_of(ra * sa * ni * nj * rb * sb)), dim3(256),
+                     0, (hipStream_t)stream, A, ra, ni, nj, rb, B, sa, sb, out);
   return launch_status();
 }
 

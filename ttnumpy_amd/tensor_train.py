@@ -434,9 +434,23 @@ class TensorTrain:
 
     def multiply(self, other, truncate: bool = False, error=None, max_rank=None, method="direct",
                  seed=None, options=None) -> "TensorTrain":
-        """Scalar multiple on the centre core, or core 0 (``tensor_train.py:396-401``)."""
+        """Element-wise product with a train (cores of bonds r s, ``tensor_train.py:363-395``)
+        or a scalar multiple on the centre core, or core 0 (``tensor_train.py:396-401``)."""
         if isinstance(other, TensorTrain):
-            raise NotImplementedError("the Hadamard product of two trains is not part of this build")
+            if self.physical_dims() != other.physical_dims():
+                raise DimensionMismatch("Tensor trains must have the same dimensions")
+            if truncate and method == "randomized":
+                raise NotImplementedError("method='randomized' rounding of an element-wise product "
+                                          "is not part of this build")
+            cores = []
+            for a, b in zip(self.cores, other.cores):
+                ra, ni, nj, rb = a.shape
+                sa, sb = b.shape[0], b.shape[3]
+                c = D.empty(ra * sa, ni, nj, rb * sb)
+                _lib.check(D.lib().ttx_hadamard_core_f64(D.stream(), P(a), ra, ni, nj, rb, P(b), sa,
+                                                         sb, P(c)), "ttx_hadamard_core_f64")
+                cores.append(c)
+            return _round(TensorTrain(cores), truncate, error, max_rank, method, seed, options)
         if not isinstance(other, numbers.Number):
             raise TypeError(f"unsupported operand type(s) for *: 'TensorTrain' and '{type(other)}'")
         res = self.copy()
