@@ -309,6 +309,57 @@ def randomized_sweep_right(cores: List[np.ndarray], rank: int, rng: np.random.Ge
     cores[:] = [np.ascontiguousarray(c.transpose(3, 1, 2, 0)) for c in reversed(mirrored)]
 
 
+def randomized_hadamard(a_cores, b_cores, rank: int, rng: np.random.Generator) -> List[np.ndarray]:
+    """Left-orthonormal cores of the element-wise product of two trains from a
+    sketch, the product cores never formed (``utils.py:230-333``, HaTT-1).
+
+    The sketch and the sweep are those of :func:`randomized_sweep_left` on the
+    product train, whose core i is a_i (x) b_i over the bonds: contractions
+    with it are taken one factor at a time.  W_i is (r_i, s_i, l_i)."""
+    if rank < 1:
+        raise ValueError(f"rank must be positive, but got {rank}")
+    d = len(a_cores)
+    w = [None] * d + [np.ones((1, 1, 1))]
+    for i in range(d - 1, 0, -1):
+        a, b = a_cores[i], b_cores[i]
+        r, m, n, rp = a.shape
+        s, sp = b.shape[0], b.shape[3]
+        c = np.einsum("xky,ukv,yvl->xukl", a.reshape(r, m * n, rp), b.reshape(s, m * n, sp),
+                      w[i + 1]).reshape(r * s, -1)
+        bond = min(rank, c.shape[0], c.shape[1])
+        wi = c @ rng.standard_normal((c.shape[1], bond))
+        top = np.abs(wi).max()
+        if top > 0:
+            wi = wi / top
+        w[i] = wi.reshape(r, s, bond)
+    cores = []
+    carry = np.ones((1, 1, 1))
+    for i in range(d):
+        a, b = a_cores[i], b_cores[i]
+        r, m, n, rp = a.shape
+        s, sp = b.shape[0], b.shape[3]
+        c0 = carry.shape[0]
+        mat = np.einsum("cxu,xky,ukv->ckyv", carry, a.reshape(r, m * n, rp),
+                        b.reshape(s, m * n, sp)).reshape(c0 * m * n, rp * sp)
+        if i == d - 1:
+            cores.append(mat.reshape(c0, m, n, -1))
+            break
+        q, _ = sla.qr(mat @ w[i + 1].reshape(rp * sp, -1), mode="economic")
+        cores.append(q.reshape(c0, m, n, -1))
+        carry = (q.T @ mat).reshape(q.shape[1], rp, sp)
+    return cores
+
+
+def multiply_rounded(a: Train, b: Train, error=None, max_rank=None, method="direct", seed=None,
+                     options=None) -> Train:
+    """``a.multiply(b, truncate=True, ...)`` (``tensor_train.py:328-407``)."""
+    if method == "randomized":
+        out = Train(randomized_hadamard(a.cores, b.cores, sketch_rank(max_rank, options),
+                                        np.random.default_rng(seed)), orth_center=a.d - 1)
+        return compress_svd(out, 0.0 if error is None else error, max_rank)
+    return truncate(tt_multiply(a, b), 0.0 if error is None else error, max_rank)
+
+
 def truncate(t: Train, error: float = 0.0, max_rank=None, canonical="auto", spectra=None,
              method: str = "direct", seed=None, options=None) -> Train:
     """Copy, orthonormalizing sweep, SVD sweep (``tensor_train.py:852-940``).

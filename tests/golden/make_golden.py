@@ -207,6 +207,21 @@ def randomized(reference_root: str) -> None:
                           call="(x1 + x2)[.orthonormalize(pre)].truncate(error, max_rank, "
                                "method='randomized', seed, options={'oversampling': over})")
         print(name, out.bonds, flush=True)
+    # element-wise product rounded on a sketch (HaTT-1)
+    ca = problems.random_train_cores([3, 4, 2, 5, 3], 3, seed=31, normal=True)
+    cb = problems.random_train_cores([3, 4, 2, 5, 3], 4, seed=32, normal=True)
+    out = TT([c.copy() for c in ca]).multiply(TT([c.copy() for c in cb]), truncate=True,
+                                             max_rank=4, method="randomized", seed=3,
+                                             options={"oversampling": 2})
+    full = np.asarray(out.get_full_tensor()).reshape(-1)
+    np.savez_compressed(os.path.join(HERE, "multiply_rand.npz"),
+                        bonds=np.asarray(out.bonds, dtype=np.int64), full=full,
+                        orth_center=np.int64(out.orth_center))
+    meta["multiply_rand"] = dict(
+        modes=[3, 4, 2, 5, 3], ranks=[3, 4], seeds=[31, 32], normal=True,
+        call="x.multiply(y, truncate=True, max_rank=4, method='randomized', seed=3, "
+             "options={'oversampling': 2})")
+    print("multiply_rand", out.bonds, flush=True)
     with open(meta_path, "w") as fh:
         json.dump(meta, fh, indent=1, sort_keys=True)
 

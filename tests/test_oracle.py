@@ -362,3 +362,24 @@ def test_elementwise_product_closed_form():
     assert ab.bonds == [x * y for x, y in zip(a.bonds, b.bonds)]
     A = O.Train(P.poisson_2d_cores(2))
     np.testing.assert_allclose(O.tt_multiply(A, A).dense_matrix(), A.dense_matrix() ** 2, atol=1e-12)
+
+
+def test_randomized_elementwise_product_matches_reference():
+    """multiply(truncate=True, method="randomized"): the reference's bonds,
+    centre and full tensor; with a sketch covering the product's ranks it
+    equals the rounding of the exact product."""
+    g = load_golden("multiply_rand")
+    a = O.Train(P.random_train_cores([3, 4, 2, 5, 3], 3, 31, normal=True))
+    b = O.Train(P.random_train_cores([3, 4, 2, 5, 3], 4, 32, normal=True))
+    out = O.multiply_rounded(a, b, max_rank=4, method="randomized", seed=3,
+                             options={"oversampling": 2})
+    assert out.bonds == list(g["bonds"])
+    assert out.orth_center == int(g["orth_center"])
+    np.testing.assert_allclose(out.full(), g["full"], rtol=1e-9, atol=1e-11 * np.abs(g["full"]).max())
+    small_a = O.Train(P.random_train_cores([2, 3, 2], 2, 5, normal=True))
+    small_b = O.Train(P.random_train_cores([2, 3, 2], 2, 6, normal=True))
+    exact = O.multiply_rounded(small_a, small_b, error=1e-12, max_rank=4)
+    rnd = O.multiply_rounded(small_a, small_b, error=1e-12, max_rank=4, method="randomized",
+                             seed=0, options={"oversampling": 0})
+    assert rnd.bonds == exact.bonds
+    np.testing.assert_allclose(rnd.full(), exact.full(), rtol=1e-10, atol=1e-12)

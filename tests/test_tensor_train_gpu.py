@@ -152,3 +152,23 @@ def test_elementwise_product_and_its_rounding():
     A = tx.TensorTrain(P.poisson_2d_cores(2))
     with pytest.raises(tx.DimensionMismatch):
         x.multiply(A)
+
+
+def test_randomized_elementwise_product_matches_reference():
+    """multiply(truncate=True, method="randomized") on the device: the
+    reference's bonds, centre and full tensor (tests/golden/multiply_rand.npz)."""
+    print("tt randomized elementwise product", flush=True)
+    import os, sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from conftest import load_golden
+    from ttnumpy_amd import problems as P
+    tx = _tx()
+    g = load_golden("multiply_rand")
+    x = tx.TensorTrain(P.random_train_cores([3, 4, 2, 5, 3], 3, 31, normal=True))
+    y = tx.TensorTrain(P.random_train_cores([3, 4, 2, 5, 3], 4, 32, normal=True))
+    out = x.multiply(y, truncate=True, max_rank=4, method="randomized", seed=3,
+                     options={"oversampling": 2})
+    assert out.bonds == list(g["bonds"])
+    assert out.orth_center == int(g["orth_center"])
+    full = out.get_full_tensor().reshape(-1)
+    np.testing.assert_allclose(full, g["full"], rtol=1e-9, atol=1e-11 * np.abs(g["full"]).max())

@@ -213,6 +213,99 @@ def randomized_sweep_left(cores: List[torch.Tensor], rank: int, rng: np.random.G
         cores[i + 1] = D.absorb_left(P(qtm), k, r1, cores[i + 1])
 
 
+def _product_times_sketch(a: torch.Tensor, b: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
+    """C[(x, u), (k, l)] = sum_{y, v} a[x, k, y] b[u, k, v] w[y, v, l] for cores a
+    (r, m, n, r'), b (s, m, n, s') (k = (m, n)) and w (r', s', l): the product
+    core of the two trains contracted with the sketch without forming it."""
+    r, m, n, rp = a.shape
+    s, sp = b.shape[0], b.shape[3]
+    mn, l = m * n, w.shape[2]
+    h = D.empty(r * mn, sp * l)                      # h[x, k, v, l] = sum_y a w
+    D.gemm(1, r * mn, sp * l, rp, 1.0, P(a), rp, 1, 0, P(w), sp * l, 1, 0, 0.0, P(h), sp * l, 1, 0)
+    hp = D.empty(mn, sp, r, l)                       # hp[k, v, x, l]
+    D.copy_nd([r, mn, sp, l], P(h), [mn * sp * l, sp * l, l, 1], P(hp),
+              [l, sp * r * l, r * l, 1])
+    c = D.empty(mn, s, r * l)                        # per k: b[:, k, :] hp[k]
+    D.gemm(mn, s, r * l, sp, 1.0, P(b), mn * sp, 1, sp, P(hp), r * l, 1, sp * r * l, 0.0, P(c),
+           r * l, 1, s * r * l)
+    out = D.empty(r * s, mn * l)                     # out[(x, u), (k, l)] = c[k, u, x, l]
+    D.copy_nd([mn, s, r, l], P(c), [s * r * l, r * l, l, 1], P(out),
+              [l, mn * l, s * mn * l, 1])
+    return out
+
+
+def _carry_times_product(carry: torch.Tensor, a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """M[(c, k), (y, v)] = sum_{x, u} carry[c, x, u] a[x, k, y] b[u, k, v]: the carry
+    (c, r, s) contracted with the product core of a (r, m, n, r') and b (s, m, n, s')."""
+    r, m, n, rp = a.shape
+    s, sp = b.shape[0], b.shape[3]
+    mn, c0 = m * n, carry.shape[0]
+    cp = D.empty(c0, s, r)                           # cp[c, u, x]
+    D.copy_nd([c0, r, s], P(carry), [r * s, s, 1], P(cp), [s * r, 1, r])
+    t = D.empty(c0 * s, mn * rp)                     # t[c, u, k, y] = sum_x cp a
+    D.gemm(1, c0 * s, mn * rp, r, 1.0, P(cp), r, 1, 0, P(a), mn * rp, 1, 0, 0.0, P(t), mn * rp, 1, 0)
+    tp = D.empty(mn, c0 * rp, s)                     # tp[k, (c, y), u]
+    D.copy_nd([c0, s, mn, rp], P(t), [s * mn * rp, mn * rp, rp, 1], P(tp),
+              [rp * s, 1, c0 * rp * s, s])
+    cm = D.empty(mn, c0 * rp, sp)                    # per k: tp[k] b[:, k, :]
+    D.gemm(mn, c0 * rp, sp, s, 1.0, P(tp), s, 1, c0 * rp * s, P(b), mn * sp, 1, sp, 0.0, P(cm),
+           sp, 1, c0 * rp * sp)
+    out = D.empty(c0 * mn, rp * sp)                  # out[(c, k), (y, v)] = cm[k, c, y, v]
+    D.copy_nd([mn, c0, rp, sp], P(cm), [c0 * rp * sp, rp * sp, sp, 1], P(out),
+              [rp * sp, mn * rp * sp, sp, 1])
+    return out
+
+
+def randomized_hadamard(a_cores: List[torch.Tensor], b_cores: List[torch.Tensor], rank: int,
+                        rng: np.random.Generator) -> List[torch.Tensor]:
+    """Left-orthonormal cores of the element-wise product of two trains from a
+    sketch, the product cores never formed (``utils.py:230-333``, HaTT-1): the
+    sketch and sweep of :func:`randomized_sweep_left` on the product train,
+    each contraction with a product core taken one factor at a time."""
+    if rank < 1:
+        raise ValueError(f"rank must be positive, but got {rank}")
+    d = len(a_cores)
+    L = D.lib()
+    w: List[Optional[torch.Tensor]] = [None] * d + [D.to_device(np.ones((1, 1, 1)))]
+    for i in range(d - 1, 0, -1):
+        a, b = a_cores[i], b_cores[i]
+        r, s = a.shape[0], b.shape[0]
+        c = _product_times_sketch(a, b, w[i + 1])
+        cols = c.shape[1]
+        bond = min(rank, r * s, cols)
+        g = D.to_device(rng.standard_normal((cols, bond)))
+        wi = D.empty(r, s, bond)
+        D.gemm(1, r * s, bond, cols, 1.0, P(c), cols, 1, 0, P(g), bond, 1, 0, 0.0, P(wi), bond, 1, 0)
+        _lib.check(L.ttx_scale_maxabs_f64(D.stream(), r * s * bond, P(wi)), "ttx_scale_maxabs_f64")
+        w[i] = wi
+    cores = []
+    carry = D.to_device(np.ones((1, 1, 1)))
+    for i in range(d):
+        a, b = a_cores[i], b_cores[i]
+        _, m, n, rp = a.shape
+        sp = b.shape[3]
+        c0 = carry.shape[0]
+        mat = _carry_times_product(carry, a, b)          # (c0 m n, rp sp)
+        if i == d - 1:
+            cores.append(mat.reshape(c0, m, n, rp * sp))
+            break
+        rows = c0 * m * n
+        l1 = w[i + 1].shape[2]
+        mw = D.empty(rows, l1)
+        D.gemm(1, rows, l1, rp * sp, 1.0, P(mat), rp * sp, 1, 0, P(w[i + 1]), l1, 1, 0, 0.0, P(mw),
+               l1, 1, 0)
+        k = min(rows, l1)
+        Q = D.empty(c0, m, n, k)
+        R = D.empty(k, l1)
+        D.qr(rows, l1, P(mw), l1, 1, P(Q), k, 1, P(R), l1, 1)
+        cores.append(Q)
+        nxt = D.empty(k, rp, sp)                         # Q^T mat
+        D.gemm(1, k, rp * sp, rows, 1.0, P(Q), 1, k, 0, P(mat), rp * sp, 1, 0, 0.0, P(nxt), rp * sp,
+               1, 0)
+        carry = nxt
+    return cores
+
+
 def _mirror(c: torch.Tensor) -> torch.Tensor:
     """The core with its bonds swapped, (r, n, m, r') -> (r', n, m, r)."""
     r0, nr, nc, r1 = c.shape
@@ -440,8 +533,11 @@ class TensorTrain:
             if self.physical_dims() != other.physical_dims():
                 raise DimensionMismatch("Tensor trains must have the same dimensions")
             if truncate and method == "randomized":
-                raise NotImplementedError("method='randomized' rounding of an element-wise product "
-                                          "is not part of this build")
+                prod = TensorTrain(randomized_hadamard(self.cores, other.cores,
+                                                       _sketch_rank(max_rank, options),
+                                                       np.random.default_rng(seed)))
+                prod.orth_center = prod.n_cores - 1
+                return prod.compress_svd(0.0 if error is None else error, max_rank, inplace=True)
             cores = []
             for a, b in zip(self.cores, other.cores):
                 ra, ni, nj, rb = a.shape
