@@ -12,6 +12,8 @@ It restates, in FP64 NumPy/SciPy, the algorithm of ttnumpy 0.3.0 for
   ``ttnumpy/solvers/als.py:56-212``, ``ttnumpy/solvers/one_site.py:27-209``,
   ``ttnumpy/solvers/common.py:118-929``) and the two-site MALS
   (``ttnumpy/solvers/mals.py:61-292``, ``ttnumpy/solvers/two_site.py:28-241``);
+* the QTT conversions ``tt_to_qtt`` / ``qtt_to_tt``
+  (``ttnumpy/tensor_train.py:58-91,942-1135``);
 * the per-sweep residual ``||A x - b|| / ||b||`` in TT arithmetic
   (``common.py:118-146`` via ``tensor_train.py:254-540``).
 
@@ -445,6 +447,125 @@ def tt_norm(t: Train) -> float:
     c = [x.reshape(x.shape[0], x.shape[1] * x.shape[2], 1, x.shape[3]).copy() for x in t.cores]
     qr_sweep_right(c, len(c) - 1, 0)
     return float(np.linalg.norm(c[0].reshape(-1)))
+
+
+# --------------------------------------------------------------------------
+# QTT conversions  (tensor_train.py:58-91, 942-1135)
+# --------------------------------------------------------------------------
+
+
+def _power_of(mode: int, base: int) -> int:
+    """Exponent q with base**q == mode, or -1 (integer search, ``tensor_train.py:975-980``)."""
+    q = 0
+    while base ** q < mode:
+        q += 1
+    return q if base ** q == mode else -1
+
+
+def tt_to_qtt(t: Train, mode_size: int = 2, error: float = 1e-12, max_rank=None) -> Train:
+    """Split every core of mode ``mode_size**q`` into ``q`` cores of mode
+    ``mode_size`` (``tensor_train.py:942-1073``).
+
+    The train is first right-orthonormalised; each split is a ``gesvd`` of
+    the unfolding (left, m, m_c) x (rest), truncated by ``truncated_bond``
+    with ``delta = error / sqrt(#SVDs) * ||t||``, the kept ``U`` emitted and
+    ``U^T M`` carried on; at a group boundary the carry is pushed into the
+    next core by one more truncated SVD.  Returns a left-canonical train with
+    its centre at the last core; the input is not modified.
+    """
+    if mode_size < 2:
+        raise ValueError(f"mode_size must be at least 2, but got {mode_size}")
+    if max_rank is not None and max_rank <= 0:
+        raise ValueError("max_rank must be a positive integer.")
+    ttm = any(c > 1 for c in t.col_modes)
+    qs = []
+    for i, row in enumerate(t.row_modes):
+        q = _power_of(row, mode_size)
+        if q < 0:
+            raise ValueError(f"Row mode {row} not a power of {mode_size}")
+        if ttm and mode_size ** q != t.col_modes[i]:
+            raise ValueError("Column mode size of TTM not a power of mode size")
+        qs.append(q)
+    rs, cs = mode_size, (mode_size if ttm else 1)
+    n_svd = max(1, sum(max(0, q - 1) for q in qs) + t.d - 1)
+    w = orthonormalize(t.copy(), "right")
+    delta = error / np.sqrt(n_svd) * tt_norm(w)
+    out = []
+    for i, q in enumerate(qs):
+        core = w.cores[i]
+        left, right = core.shape[0], core.shape[-1]
+        for j in range(q - 1, 0, -1):
+            mat = core.reshape(left, rs, rs ** j, cs, cs ** j, right).transpose(0, 1, 3, 2, 4, 5)
+            mat = mat.reshape(left * rs * cs, -1)
+            u, s, _ = sla.svd(mat, full_matrices=False, check_finite=False, lapack_driver="gesvd")
+            k = truncated_bond(s, delta, max_rank)
+            out.append(u[:, :k].reshape(left, rs, cs, k))
+            core = u[:, :k].T @ mat
+            left = k
+        if i < t.d - 1:
+            mat = core.reshape(left * rs * cs, right)
+            u, s, _ = sla.svd(mat, full_matrices=False, check_finite=False, lapack_driver="gesvd")
+            k = truncated_bond(s, delta, max_rank)
+            out.append(u[:, :k].reshape(left, rs, cs, k))
+            w.cores[i + 1] = _absorb_left(u[:, :k].T @ mat, w.cores[i + 1])
+        else:
+            out.append(core.reshape(left, rs, cs, right))
+    return Train(out, orth_center=len(out) - 1)
+
+
+def merge_cores(cores: List[np.ndarray]) -> np.ndarray:
+    """Consecutive cores as one core (left, prod rows, prod cols, right), rows
+    and columns each in the cores' order (``tensor_train.py:58-91``)."""
+    left, right = cores[0].shape[0], cores[-1].shape[-1]
+    t = cores[0].reshape(-1, cores[0].shape[-1])
+    for c in cores[1:]:
+        t = (t @ c.reshape(c.shape[0], -1)).reshape(-1, c.shape[-1])
+    rows = [c.shape[1] for c in cores]
+    cols = [c.shape[2] for c in cores]
+    modes = [x for pair in zip(rows, cols) for x in pair]
+    kept = [a for a, m in enumerate(modes) if m > 1]     # size-1 modes need no axis
+    perm = [1 + i for i, a in enumerate(kept) if a % 2 == 0]
+    perm += [1 + i for i, a in enumerate(kept) if a % 2 == 1]
+    t = t.reshape([left] + [modes[a] for a in kept] + [right])
+    return t.transpose([0] + perm + [len(kept) + 1]).reshape(left, prod(rows), prod(cols), right)
+
+
+def qtt_groups(t: Train, physical_indexes) -> List[int]:
+    """Number of consecutive cores merged for each target index, with the
+    reference's dimension checks (``tensor_train.py:1100-1130``)."""
+    ttm = any(c > 1 for c in t.col_modes)
+    counts, pos = [], 0
+    for i, idx in enumerate(physical_indexes):
+        row_t, col_t = (idx if ttm else (idx, 1))
+        pr = pc = 1
+        n = 0
+        while n == 0 or pr != row_t or pc != col_t:
+            if pos + n >= t.d:
+                raise ValueError(f"Not enough QTT cores to match physical index for core {i}")
+            pr *= t.row_modes[pos + n]
+            pc *= t.col_modes[pos + n]
+            n += 1
+            if pr > row_t:
+                raise ValueError(f"Row mode size of QTT exceeds original row mode size for core {i}")
+            if pc > col_t:
+                raise ValueError(
+                    f"Column mode size of QTT exceeds original column mode size for core {i}")
+        counts.append(n)
+        pos += n
+    if pos != t.d:
+        raise ValueError(f"Physical indexes cover only {pos} of {t.d} QTT cores")
+    return counts
+
+
+def qtt_to_tt(t: Train, physical_indexes, error: float = 1e-12, max_rank=None) -> Train:
+    """Merge groups of QTT cores back into cores of the given modes, then
+    ``truncate(error, max_rank)`` (``tensor_train.py:1075-1135``)."""
+    counts = qtt_groups(t, physical_indexes)
+    cores, pos = [], 0
+    for n in counts:
+        cores.append(merge_cores(t.cores[pos:pos + n]))
+        pos += n
+    return truncate(Train(cores), error=error, max_rank=max_rank)
 
 
 def relative_residual(A: Train, b: Train, x: Train, b_norm: Optional[float] = None) -> float:

@@ -27,9 +27,9 @@ def load_golden(name):
     return np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
 
 
-def unpack_cores(g):
-    shapes = g["core_shapes"]
-    flat = g["core_data"]
+def unpack_cores(g, shapes_key="core_shapes", data_key="core_data"):
+    shapes = g[shapes_key]
+    flat = g[data_key]
     out, off = [], 0
     for s in shapes:
         n = int(np.prod(s))

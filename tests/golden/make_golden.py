@@ -270,6 +270,51 @@ def mals_cases(reference_root: str) -> None:
         json.dump(meta, fh, indent=1, sort_keys=True)
 
 
+def qtt_cases(reference_root: str) -> None:
+    """QTT conversions: ``tt_to_qtt`` with truncation (plain trains, a TTM,
+    mode 3) and ``qtt_to_tt`` back, regrouping included; inputs stored."""
+    sys.path.insert(0, reference_root)
+    from ttnumpy import tensor_train
+
+    TT = tensor_train.TensorTrain
+    meta_path = os.path.join(HERE, "golden_meta.json")
+    with open(meta_path) as fh:
+        meta = json.load(fh)
+
+    def cores(rows, cols, rank, seed):
+        rng = np.random.default_rng(seed)
+        bonds = [1] + [rank] * (len(rows) - 1) + [1]
+        return [rng.standard_normal((bonds[i], rows[i], cols[i], bonds[i + 1]))
+                for i in range(len(rows))]
+
+    cases = (
+        ("qtt_tt8", cores([8] * 4, [1] * 4, 5, 41), dict(error=0.2), [8] * 4, dict(error=0.05)),
+        ("qtt_ttm4", cores([4] * 3, [4] * 3, 3, 42), dict(error=1e-3, max_rank=4),
+         [(4, 4)] * 3, dict(error=1e-3, max_rank=3)),
+        ("qtt_mode3", cores([9, 27, 3], [1] * 3, 4, 43), dict(mode_size=3, error=1e-2),
+         [3, 3, 27, 3], dict(error=1e-2)),
+        ("qtt_regroup", cores([2] * 8, [1] * 8, 4, 44), None, [4, 16, 2, 2], dict(error=1e-2)),
+    )
+    for name, inp, to_kw, back_idx, back_kw in cases:
+        x = TT([c.copy() for c in inp])
+        q = x.tt_to_qtt(**to_kw) if to_kw is not None else x
+        back = q.qtt_to_tt(back_idx, **back_kw)
+        shapes, flat = _pack_cores(inp)
+        np.savez_compressed(
+            os.path.join(HERE, f"{name}.npz"), in_shapes=shapes, in_flat=flat,
+            qtt_bonds=np.asarray(q.bonds, dtype=np.int64),
+            qtt_full=np.asarray(q.get_full_tensor()).reshape(-1),
+            back_bonds=np.asarray(back.bonds, dtype=np.int64),
+            back_full=np.asarray(back.get_full_tensor()).reshape(-1),
+            back_orth_center=np.int64(back.orth_center))
+        meta[name] = dict(input="standard_normal cores, default_rng(seed) in site order",
+                          to_qtt=to_kw, back=dict(physical_indexes=back_idx, **back_kw),
+                          qtt_bonds=list(map(int, q.bonds)), back_bonds=list(map(int, back.bonds)))
+        print(name, q.bonds, back.bonds, flush=True)
+    with open(meta_path, "w") as fh:
+        json.dump(meta, fh, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     root = args[0] if args else "/root/reference"
@@ -278,7 +323,10 @@ if __name__ == "__main__":
         randomized(root)
     elif only == "mals":
         mals_cases(root)
+    elif only == "qtt":
+        qtt_cases(root)
     else:
         main(root)
         randomized(root)
         mals_cases(root)
+        qtt_cases(root)

@@ -383,3 +383,165 @@ def test_randomized_elementwise_product_matches_reference():
                              seed=0, options={"oversampling": 0})
     assert rnd.bonds == exact.bonds
     np.testing.assert_allclose(rnd.full(), exact.full(), rtol=1e-10, atol=1e-12)
+
+
+# ---------------------------------------------------------------- QTT conversions
+QTT_CASES = {
+    # name: (tt_to_qtt kwargs or None, qtt_to_tt physical indexes, qtt_to_tt kwargs)
+    "qtt_tt8": (dict(error=0.2), [8] * 4, dict(error=0.05)),
+    "qtt_ttm4": (dict(error=1e-3, max_rank=4), [(4, 4)] * 3, dict(error=1e-3, max_rank=3)),
+    "qtt_mode3": (dict(mode_size=3, error=1e-2), [3, 3, 27, 3], dict(error=1e-2)),
+    "qtt_regroup": (None, [4, 16, 2, 2], dict(error=1e-2)),
+}
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b)
+
+
+def _full(t):
+    """The reference's get_full_tensor order: a TTM as its dense matrix."""
+    return t.dense_matrix().reshape(-1) if any(c > 1 for c in t.col_modes) else t.full()
+
+
+@pytest.mark.parametrize("name", sorted(QTT_CASES))
+def test_qtt_conversions_match_reference(name):
+    """tt_to_qtt then qtt_to_tt reproduce the reference's bonds and tensors
+    (fixtures written by the reference, tests/golden/qtt_*.npz)."""
+    to_kw, idx, back_kw = QTT_CASES[name]
+    g = load_golden(name)
+    t = O.Train(unpack_cores(g, "in_shapes", "in_flat"))
+    with threadpool_limits(1):
+        q = O.tt_to_qtt(t, **to_kw) if to_kw is not None else t
+        back = O.qtt_to_tt(q, idx, **back_kw)
+    assert q.bonds == list(g["qtt_bonds"])
+    assert _rel(_full(q), g["qtt_full"]) < 1e-10
+    assert back.bonds == list(g["back_bonds"])
+    assert back.orth_center == int(g["back_orth_center"])
+    assert _rel(_full(back), g["back_full"]) < 1e-10
+
+
+def _dense_tt(tensor):
+    """Exact TT of a small dense tensor by successive SVDs (test input only)."""
+    dims = tensor.shape
+    cores, r, rest = [], 1, tensor.reshape(1, -1)
+    for n in dims[:-1]:
+        u, s, vt = np.linalg.svd(rest.reshape(r * n, -1), full_matrices=False)
+        k = int(np.sum(s > 1e-14 * s[0]))
+        cores.append(u[:, :k].reshape(r, n, 1, k))
+        rest, r = s[:k, None] * vt[:k], k
+    cores.append(rest.reshape(r, dims[-1], 1, 1))
+    return O.Train(cores)
+
+
+def test_qtt_round_trips_and_modes():
+    """Round trips at 1e-10 for mode 2, mode 3 and a TTM; five cycles keep the
+    tensor (the reference's test_qtt, tests/test_tensor_train.py:392-471)."""
+    for modes, ms, seed in (([8] * 4, 2, 115), ([27] * 4, 3, 116)):
+        t = O.Train(P.random_train_cores(modes, 5, seed))
+        q = O.tt_to_qtt(t, mode_size=ms, error=1e-10)
+        assert q.row_modes == [ms] * (3 * 4)
+        assert _rel(q.full(), t.full()) < 1e-10
+        back = O.qtt_to_tt(q, modes, error=1e-10)
+        assert back.row_modes == modes
+        assert _rel(back.full(), t.full()) < 1e-10
+    rng = np.random.default_rng(117)
+    b = [1, 5, 5, 1]
+    t = O.Train([rng.random((b[i], 8, 8, b[i + 1])) for i in range(3)])
+    q = O.tt_to_qtt(t, error=1e-10)
+    assert q.row_modes == [2] * 9 and q.col_modes == [2] * 9
+    assert _rel(_full(q), _full(t)) < 1e-10
+    assert _rel(_full(O.qtt_to_tt(q, [(8, 8)] * 3, error=1e-10)), _full(t)) < 1e-10
+    t = O.Train(P.random_train_cores([8] * 4, 8, 118))
+    x0 = t.full()
+    for _ in range(5):
+        t = O.qtt_to_tt(O.tt_to_qtt(t, error=1e-10), [8] * 4, error=1e-10)
+    assert _rel(t.full(), x0) < 1e-9
+
+
+def test_qtt_error_bound_canonical_form_and_floor():
+    """||qtt - A|| <= error ||A||; left-orthonormal cores with the centre last
+    and the norm in it; error 1e2 floors every bond at 1
+    (tests/test_tensor_train.py:473-517)."""
+    a = np.random.default_rng(0).standard_normal((8, 8, 8))
+    t = _dense_tt(a)
+    for err in (3e-1, 1e-1, 1e-2, 1e-4):
+        q = O.tt_to_qtt(t, error=err)
+        assert np.linalg.norm(q.full() - a.reshape(-1)) <= err * np.linalg.norm(a)
+    t = O.Train(P.random_train_cores([8, 8, 8], 5, 119))
+    q = O.tt_to_qtt(t, error=1e-10)
+    assert q.orth_center == q.d - 1
+    for c in q.cores[:-1]:
+        m = c.reshape(-1, c.shape[-1])
+        np.testing.assert_allclose(m.T @ m, np.eye(m.shape[1]), atol=1e-10)
+    assert abs(O.tt_norm(q) - np.linalg.norm(t.full())) < 1e-10 * np.linalg.norm(t.full())
+    before = [c.copy() for c in t.cores]
+    O.tt_to_qtt(t, error=1e-10)
+    assert all(np.array_equal(x, y) for x, y in zip(before, t.cores))
+    q = O.tt_to_qtt(O.Train(P.random_train_cores([16, 16], 4, 120)), error=1e2)
+    assert q.bonds == [1] * (q.d + 1)
+
+
+def test_qtt_mixed_modes_and_validation():
+    """Cores with no split, 243 = 3^5, and the argument checks
+    (tests/test_tensor_train.py:519-552,593-620)."""
+    t = O.Train(P.random_train_cores([2, 8], 2, 121))
+    q = O.tt_to_qtt(t, error=1e-10)
+    assert q.row_modes == [2, 2, 2, 2] and _rel(q.full(), t.full()) < 1e-10
+    t = O.Train(P.random_train_cores([243, 3], 2, 122))
+    q = O.tt_to_qtt(t, mode_size=3, error=1e-10)
+    assert q.row_modes == [3] * 6 and _rel(q.full(), t.full()) < 1e-10
+    t = O.Train(P.random_train_cores([4, 4], 2, 123))
+    for kw in (dict(max_rank=0), dict(mode_size=1), dict(mode_size=3)):
+        with pytest.raises(ValueError):
+            O.tt_to_qtt(t, **kw)
+    rng = np.random.default_rng(124)
+    rect = O.Train([rng.random((1, 4, 2, 2)), rng.random((2, 4, 2, 1))])
+    with pytest.raises(ValueError):
+        O.tt_to_qtt(rect)
+    q = O.tt_to_qtt(O.Train(P.random_train_cores([4, 4, 4], 3, 125)), error=1e-10)
+    for idx in ([4, 4], [8, 8, 8], [3, 8, 8]):
+        with pytest.raises(ValueError):
+            O.qtt_to_tt(q, idx, error=1e-10)
+
+
+def test_qtt_regrouping_and_long_groups():
+    """Groups of different sizes, and groups of 33+ cores with modes of 1
+    (tests/test_tensor_train.py:554-591)."""
+    a = np.random.default_rng(0).standard_normal((4, 4, 4))
+    q = O.tt_to_qtt(_dense_tt(a), error=1e-10)
+    for idx in ([8, 8], [2, 8, 4], [2] * 6):
+        back = O.qtt_to_tt(q, idx, error=1e-10)
+        assert back.row_modes == idx
+        assert _rel(back.full(), a.reshape(-1)) < 1e-10
+    t = O.Train(P.random_train_cores([2] * 10 + [1] * 30 + [2] * 3, 3, 7))
+    back = O.qtt_to_tt(t, [1024, 8])
+    assert back.row_modes == [1024, 8]
+    np.testing.assert_allclose(back.full(), t.full(), rtol=1e-12, atol=1e-14 * np.abs(t.full()).max())
+
+
+def test_merge_cores_against_dense_matrix():
+    """A merged TTM core is the dense matrix of the group: rows and columns in
+    the cores' order (brute force with Kronecker products)."""
+    rng = np.random.default_rng(5)
+    mats = [rng.standard_normal((2, 3)), rng.standard_normal((4, 1)), rng.standard_normal((3, 2))]
+    cores = [m.reshape(1, *m.shape, 1) for m in mats]
+    merged = O.merge_cores(cores)
+    np.testing.assert_allclose(merged[0, :, :, 0], np.kron(np.kron(mats[0], mats[1]), mats[2]),
+                               rtol=1e-14)
+
+
+def test_qtt_sine_of_sum_keeps_exact_ranks():
+    """sin(10 (x + y)) on 256 x 256 has TT and QTT rank 2: kept at the default
+    error, and at error 0 (tests/test_tensor_train.py:824-857)."""
+    size = 256
+    g = np.arange(size) / size
+    t = O.Train([np.stack([np.sin(10 * g), np.cos(10 * g)], 1).reshape(1, size, 1, 2),
+                 np.stack([np.cos(10 * g), np.sin(10 * g)]).reshape(2, size, 1, 1)])
+    q = O.tt_to_qtt(t)
+    assert max(q.bonds) == 2
+    assert _rel(q.full(), t.full()) < 1e-10
+    back = O.qtt_to_tt(q, [size, size])
+    assert back.bonds == [1, 2, 1]
+    assert _rel(back.full(), t.full()) < 1e-10
+    assert max(O.tt_to_qtt(O.Train([c[:, ::4] for c in t.cores]), error=0.0).bonds) == 2

@@ -3,7 +3,8 @@
 Mirrors the parts of ``ttnumpy.tensor_train.TensorTrain`` (ttnumpy 0.3.0,
 ``ttnumpy/tensor_train.py:150-940``) on the solver hot path: layout and
 validation, ``copy``, ``orthonormalize``, ``compress_svd``, ``truncate``,
-``add``, scalar ``multiply``, ``dot`` (TTM @ TT) and ``tt_norm``, plus the
+``add``, ``multiply`` (scalar and element-wise), ``dot`` (TTM @ TT), ``tt_norm``
+and the QTT conversions ``tt_to_qtt`` / ``qtt_to_tt``, plus the
 ``rand``/``ones``/``zeros``/``eye`` constructors.  Cores are contiguous FP64
 device tensors ``(r, n_row, n_col, r')``; every sweep runs on libttx kernels.
 """
@@ -602,7 +603,8 @@ class TensorTrain:
         return float(self.tt_norm_device().item())
 
     def get_full_tensor(self) -> np.ndarray:
-        """Dense tensor of a small train, contracted on the device (test helper)."""
+        """Dense tensor of a small train, contracted on the device: shape (rows...)
+        for a TT, (rows..., cols...) for a TTM (``tensor_train.py:599-624``)."""
         t = self.cores[0].reshape(-1, self.cores[0].shape[3])
         for c in self.cores[1:]:
             m, r = t.shape
@@ -610,10 +612,152 @@ class TensorTrain:
             out = D.empty(m, rest)
             D.gemm(1, m, rest, r, 1.0, P(t), r, 1, 0, P(c), rest, 1, 0, 0.0, P(out), rest, 1, 0)
             t = out.reshape(-1, c.shape[3])
-        shape = []
-        for r, c in zip(self.row_modes, self.col_modes):
-            shape += [r] if not self.is_ttm() else [r, c]
-        return t.reshape(shape).cpu().numpy()
+        if not self.is_ttm():
+            return t.reshape(self.row_modes).cpu().numpy()
+        d = self.n_cores
+        shape = [x for pair in zip(self.row_modes, self.col_modes) for x in pair]
+        perm = list(range(0, 2 * d, 2)) + list(range(1, 2 * d, 2))
+        return np.ascontiguousarray(t.reshape(shape).cpu().numpy().transpose(perm))
+
+    def matricize(self) -> np.ndarray:
+        """The full tensor as a vector (TT) or a (prod rows, prod cols) matrix (TTM)
+        (``tensor_train.py:626-638``)."""
+        m = _merge_group(self.cores)
+        if not self.is_ttm():
+            return m.reshape(-1).cpu().numpy()
+        return m.reshape(m.shape[1], m.shape[2]).cpu().numpy()
+
+    # -- QTT conversions
+    def tt_to_qtt(self, mode_size: int = 2, error: float = 1e-12,
+                  max_rank: Optional[int] = None) -> "TensorTrain":
+        """Quantized train (``tensor_train.py:942-1073``): a core of mode
+        ``mode_size**q`` becomes ``q`` cores of mode ``mode_size``.
+
+        After a right QR sweep, each split is a truncated SVD of the unfolding
+        (left, m, m_c) x (rest) with delta = error / sqrt(#SVDs) * ||x||; the kept
+        U is emitted and S V^T carried on (the reference carries U^T M, equal to
+        it up to rounding); at a core boundary one more SVD pushes the remainder
+        into the next core.  Left-canonical, centre at the last core; the
+        receiver is unchanged."""
+        if mode_size < 2:
+            raise ValueError(f"mode_size must be at least 2, but got {mode_size}")
+        if max_rank is not None and max_rank <= 0:
+            raise ValueError("max_rank must be a positive integer.")
+        qs = []
+        for i, row in enumerate(self.row_modes):
+            q = 0
+            while mode_size ** q < row:
+                q += 1
+            if mode_size ** q != row:
+                raise DimensionMismatch(f"Row mode {row} not a power of {mode_size}")
+            if self.is_ttm() and mode_size ** q != self.col_modes[i]:
+                raise DimensionMismatch("Column mode size of TTM not a power of mode size")
+            qs.append(q)
+        rs, cs = mode_size, (mode_size if self.is_ttm() else 1)
+        n_svd = max(1, sum(max(0, q - 1) for q in qs) + self.n_cores - 1)
+        tt = self.orthonormalize(canonical="right")
+        nrm = tt.tt_norm_device()
+        scale = error / math.sqrt(n_svd)
+        buf = torch.zeros(2, dtype=torch.int32, device=D.device())
+        out = []
+        for i, q in enumerate(qs):
+            cur = tt.cores[i]
+            left, right = cur.shape[0], cur.shape[3]
+            if q == 0:
+                raise ValueError(f"cannot split mode 1 of core {i} into modes of {mode_size}")
+            for j in range(q - 1, 0, -1):
+                R, C = rs ** j, cs ** j                   # what stays behind this split
+                rows, cols = left * rs * cs, R * C * right
+                mat = cur
+                if cs > 1:                                # (left, rs, R, cs, C, right) -> (left, rs, cs, R, C, right)
+                    mat = D.empty(rows, cols)
+                    D.copy_nd([left * rs, cs, R, C * right], P(cur),
+                              [R * cs * C * right, C * right, cs * C * right, 1], P(mat),
+                              [cs * R * C * right, R * C * right, C * right, 1])
+                u, sv, nb = _svd_split(rows, cols, mat, nrm, scale, max_rank, buf, i)
+                out.append(u.reshape(left, rs, cs, nb))
+                cur = sv.reshape(nb, R, C, right)
+                left = nb
+            if i < self.n_cores - 1:
+                rows = left * rs * cs
+                u, sv, nb = _svd_split(rows, right, cur, nrm, scale, max_rank, buf, i)
+                out.append(u.reshape(left, rs, cs, nb))
+                tt.cores[i + 1] = D.absorb_left(P(sv), nb, right, tt.cores[i + 1])
+            else:
+                out.append(cur.reshape(left, rs, cs, right))
+        qtt = TensorTrain(out)
+        qtt.orth_center = qtt.n_cores - 1
+        return qtt
+
+    def qtt_to_tt(self, physical_indexes, error: float = 1e-12,
+                  max_rank: Optional[int] = None) -> "TensorTrain":
+        """Merge runs of consecutive cores into cores of the given modes, then
+        ``truncate(error, max_rank)`` (``tensor_train.py:1075-1135``)."""
+        counts, pos = [], 0
+        for i, idx in enumerate(physical_indexes):
+            row_t, col_t = idx if self.is_ttm() else (idx, 1)
+            pr = pc = 1
+            n = 0
+            while n == 0 or pr != row_t or pc != col_t:
+                if pos + n >= self.n_cores:
+                    raise DimensionMismatch(
+                        f"Not enough QTT cores to match physical index for core {i}")
+                pr *= self.row_modes[pos + n]
+                pc *= self.col_modes[pos + n]
+                n += 1
+                if pr > row_t:
+                    raise DimensionMismatch(
+                        f"Row mode size of QTT exceeds original row mode size for core {i}")
+                if pc > col_t:
+                    raise DimensionMismatch(
+                        f"Column mode size of QTT exceeds original column mode size for core {i}")
+            counts.append(n)
+            pos += n
+        if pos != self.n_cores:
+            raise DimensionMismatch(f"Physical indexes cover only {pos} of {self.n_cores} QTT cores")
+        cores, pos = [], 0
+        for n in counts:
+            cores.append(_merge_group(self.cores[pos:pos + n]))
+            pos += n
+        return TensorTrain(cores).truncate(error=error, max_rank=max_rank, inplace=True)
+
+
+def _merge_group(cores: Sequence[torch.Tensor]) -> torch.Tensor:
+    """Consecutive cores as one core (left, prod rows, prod cols, right), rows and
+    columns each in the cores' order (``tensor_train.py:58-91``): one GEMM per
+    core, and where both the accumulated column mode and the new row mode exceed
+    1, a strided copy that moves the new row index ahead of the column index."""
+    m = cores[0]
+    if len(cores) == 1:
+        return m.clone()
+    for c in cores[1:]:
+        left, R, C, mid = m.shape
+        _, r, cc, right = c.shape
+        rest = r * cc * right
+        prod = D.empty(left, R, C, r, cc, right)
+        D.gemm(1, left * R * C, rest, mid, 1.0, P(m), mid, 1, 0, P(c), rest, 1, 0, 0.0, P(prod),
+               rest, 1, 0)
+        if C > 1 and r > 1:
+            swapped = D.empty(left, R, r, C, cc, right)
+            D.copy_nd([left * R, r, C, cc * right], P(prod), [C * rest, cc * right, rest, 1],
+                      P(swapped), [r * C * cc * right, C * cc * right, cc * right, 1])
+            prod = swapped
+        m = prod.reshape(left, R * r, C * cc, right)
+    return m
+
+
+def _svd_split(rows: int, cols: int, mat: torch.Tensor, nrm: torch.Tensor, scale: float,
+               max_rank, buf: torch.Tensor, core: int):
+    """Truncated SVD of a row-major (rows x cols) matrix with delta = scale * nrm:
+    returns the kept U (rows x k), S V^T (k x cols) and k."""
+    U, S, Vt = D.svd(rows, cols, P(mat), cols, 1, buf, 0)
+    k = min(rows, cols)
+    D.truncated_bond_dev(S, nrm, scale, max_rank, buf, 1)
+    nb = _sweep_status(buf, core)
+    u = D.empty(rows, nb)
+    D.copy_nd([rows, nb], P(U), [k, 1], P(u), [nb, 1])
+    D.scale_cols(cols, nb, P(Vt), 1, cols, P(S))
+    return u, Vt[:nb], nb
 
 
 def _round(res: TensorTrain, truncate: bool, error, max_rank, method, seed=None,
