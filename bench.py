@@ -195,6 +195,67 @@ def c5_high_dim(with_cpu: bool):
     return res
 
 
+def c3_diffusion_3d(with_cpu: bool, solve_levels=(4, 5), cpu_level: int = 4, full_level: int = 12):
+    """C3: -div(exp(g) grad u) = 1 on a (2**L)^3 grid (3L QTT cores), g a smooth
+    random rank-8 field over the three axes (two sine modes per factor, RMS 0.5,
+    seed 0), k = exp(g) at the faces by scaling and squaring with sketched
+    rounding (rank cap 48, error 1e-8), operator rounded at 1e-10.  The operator
+    is built at the full 4096^3 size (L=12, 36 cores) and timed; AMEn
+    (max_rank=64, residual_tol=1e-8, max_sweeps=30, symmetric, seed 0,
+    x0 = rand(ranks=2, seed=0)) runs at the reduced sizes, and the oracle's AMEn on
+    the same operator cores at ``cpu_level`` on all host cores."""
+    import os
+    import torch
+    import ttnumpy_amd as tx
+    from ttnumpy_amd import device as D
+    from ttnumpy_amd import diffusion3d as T3
+    kw_op = dict(k_error=1e-8, k_rank=48, op_error=1e-10, method="randomized")
+    kw = dict(max_rank=64, residual_tol=1e-8, max_sweeps=30, symmetric=True, seed=0)
+    res = {"field": {"modes": 2, "rank": 8, "rms": 0.5, "seed": 0}, "operator_build": kw_op,
+           "solve": kw}
+    A, info = T3.operator(T3.FieldSpec(n_bits=full_level), **kw_op)
+    res[f"build_L{full_level}"] = {"grid": f"{2**full_level}^3", "cores": 3 * full_level,
+                                   "build_s": info.build_s, "op_max_bond": max(info.bonds),
+                                   "op_bonds": info.bonds, "squarings": info.squarings,
+                                   "k_max_bond": max(max(v) for v in info.k_bonds.values())}
+    del A
+    ops = {}
+    for L in sorted(set(solve_levels) | ({cpu_level} if with_cpu else set())):
+        A, info = T3.operator(T3.FieldSpec(n_bits=L), **kw_op)
+        ops[L] = A
+        if L not in solve_levels:
+            continue
+        x0 = tx.rand([2] * (3 * L), ranks=2, seed=0)
+        torch.cuda.synchronize()
+        f0 = D.flops()
+        t0 = time.perf_counter()
+        x, inf = tx.amen(A, T3.rhs(L), x0=x0, return_info=True, **kw)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        gf = (D.flops() - f0) / 1e9
+        res[f"solve_L{L}"] = {"grid": f"{2**L}^3", "cores": 3 * L, "op_max_bond": max(info.bonds),
+                              "gpu_s": wall, "stop_reason": str(inf.stop_reason),
+                              "sweeps": inf.num_sweeps_run, "max_bond": max(x.bonds),
+                              "residual_fp64": inf.residuals[-1], "local_solves": inf.local_solves,
+                              "gflop": gf, "gflop_per_s": gf / wall}
+    if with_cpu:
+        from threadpoolctl import threadpool_limits
+        from oracle import tt_oracle as O
+        L = cpu_level
+        A = ops[L]
+        x0 = tx.rand([2] * (3 * L), ranks=2, seed=0)
+        cores = len(os.sched_getaffinity(0))
+        with threadpool_limits(cores):
+            t0 = time.perf_counter()
+            xo, io = O.amen(O.Train(A.to_numpy()), O.Train(T3.rhs(L).to_numpy()),
+                            O.Train(x0.to_numpy()), **kw)
+            res[f"cpu_oracle_L{L}"] = {"s": time.perf_counter() - t0, "threads": cores,
+                                       "stop_reason": str(io.stop_reason),
+                                       "sweeps": io.num_sweeps_run, "max_bond": max(xo.bonds),
+                                       "residual_fp64": io.residuals[-1]}
+    return res
+
+
 def c4_serving(n_requests: int, with_cpu: bool, cpu_per_level: int):
     """C4: the seeded Poisson-arrival stream of parametric 2D diffusion solves
     (``ttnumpy_amd.serving``) on one engine per grid level; with ``with_cpu``
@@ -282,6 +343,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the C2 rounding measurement")
     ap.add_argument("--no-c5", action="store_true", help="skip the C5 high-dimensional measurement")
+    ap.add_argument("--no-c3", action="store_true", help="skip the C3 3-D diffusion measurement")
     ap.add_argument("--c4", type=int, default=0, metavar="N",
                     help="instead of the C1 line, serve the C4 stream of N requests and print its "
                          "report (with the oracle FIFO estimate unless --no-cpu)")
@@ -407,11 +469,19 @@ def main():
             line["c2_rounding"] = c2_rounding(not args.no_cpu)
         except Exception as exc:  # the C1 line is reported whatever happens here
             line["c2_rounding"] = {"error": repr(exc)}
+        print("# c2 done", file=sys.stderr, flush=True)
     if not args.no_c5:
         try:
             line["c5_high_dim"] = c5_high_dim(not args.no_cpu)
         except Exception as exc:  # the C1 line is reported whatever happens here
             line["c5_high_dim"] = {"error": repr(exc)}
+        print("# c5 done", file=sys.stderr, flush=True)
+    if not args.no_c3:
+        try:
+            line["c3_diffusion_3d"] = c3_diffusion_3d(not args.no_cpu)
+        except Exception as exc:  # the C1 line is reported whatever happens here
+            line["c3_diffusion_3d"] = {"error": repr(exc)}
+        print("# c3 done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -214,6 +214,13 @@ int ttx_hadamard_core_f64(ttx_stream_t stream, const double* A, int64_t ra, int6
                           int64_t nj, int64_t rb, const double* B, int64_t sa, int64_t sb,
                           double* out);
 
+/* out[(a m), i, j, (b n)] = sum_k A[a, i, k, b] * B[m, k, j, n] for cores A
+ * (ra, ni, nk, rb) and B (sa, nk, nj, sb): one core of the matrix product of
+ * two trains, TTM @ TTM or TT @ TT^T (ttnumpy/tensor_train.py:427-499, dot). */
+int ttx_ttm_ttm_core_f64(ttx_stream_t stream, const double* A, int64_t ra, int64_t ni,
+                         int64_t nk, int64_t rb, const double* B, int64_t sa, int64_t nj,
+                         int64_t sb, double* out);
+
 /* x[i] /= max_j |x[j]| when that maximum is positive: the scaling of each
  * random sketch of the randomized rounding (ttnumpy/utils.py:187-189). */
 int ttx_scale_maxabs_f64(ttx_stream_t stream, int64_t n, double* x);

@@ -99,6 +99,30 @@ __global__ __launch_bounds__(256) void hadamard_core_kernel(const double* A, int
   }
 }
 
+// out[(a, m), i, j, (b, n)] = sum_k A[a, i, k, b] B[m, k, j, n]: one core of the
+// matrix product of two trains (bonds multiply, the shared mode k summed)
+__global__ __launch_bounds__(256) void ttm_ttm_core_kernel(const double* A, int64_t ra, int64_t ni,
+                                                          int64_t nk, int64_t rb, const double* B,
+                                                          int64_t sa, int64_t nj, int64_t sb,
+                                                          double* out) {
+  const int64_t total = ra * sa * ni * nj * rb * sb;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    int64_t rem = e;
+    const int64_t n = rem % sb; rem /= sb;
+    const int64_t b = rem % rb; rem /= rb;
+    const int64_t j = rem % nj; rem /= nj;
+    const int64_t i = rem % ni; rem /= ni;
+    const int64_t m = rem % sa;
+    const int64_t a = rem / sa;
+    const double* pa = A + (a * ni + i) * nk * rb + b;       // A[a, i, k, b], k stride rb
+    const double* pb = B + (m * nk * nj + j) * sb + n;       // B[m, k, j, n], k stride nj sb
+    double acc = 0.0;
+    for (int64_t k = 0; k < nk; ++k) acc = __fma_rn(pa[k * rb], pb[k * nj * sb], acc);
+    out[e] = acc;
+  }
+}
+
 // x /= max |x| when that is positive; one workgroup (the sketches it scales
 // hold at most a few ten thousand entries)
 __global__ __launch_bounds__(1024) void scale_maxabs_kernel(int64_t n, double* x) {
@@ -200,6 +224,15 @@ extern "C" int ttx_hadamard_core_f64(ttx_stream_t stream, const double* A, int64
   if (ra < 1 || ni < 1 || nj < 1 || rb < 1 || sa < 1 || sb < 1) return TTX_EINVAL;
   hipLaunchKernelGGL(hadamard_core_kernel, dim3(grid_of(ra * sa * ni * nj * rb * sb)), dim3(256),
                      0, (hipStream_t)stream, A, ra, ni, nj, rb, B, sa, sb, out);
+  return launch_status();
+}
+
+extern "C" int ttx_ttm_ttm_core_f64(ttx_stream_t stream, const double* A, int64_t ra,
+                                    int64_t ni, int64_t nk, int64_t rb, const double* B,
+                                    int64_t sa, int64_t nj, int64_t sb, double* out) {
+  if (ra < 1 || ni < 1 || nk < 1 || rb < 1 || sa < 1 || nj < 1 || sb < 1) return TTX_EINVAL;
+  hipLaunchKernelGGL(ttm_ttm_core_kernel, dim3(grid_of(ra * sa * ni * nj * rb * sb)), dim3(256),
+                     0, (hipStream_t)stream, A, ra, ni, nk, rb, B, sa, nj, sb, out);
   return launch_status();
 }
 

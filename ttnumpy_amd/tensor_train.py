@@ -570,22 +570,44 @@ class TensorTrain:
 
     def dot(self, other: "TensorTrain", truncate: bool = False, error=None, max_rank=None,
             method="direct", seed=None, options=None) -> "TensorTrain":
-        """TTM @ TT cores ``(Ra r, n, 1, Ra' r')`` (``tensor_train.py:427-499``)."""
+        """Matrix product of two trains, cores ``(Ra r, n_row, m_col, Ra' r')``
+        (``tensor_train.py:427-499``): TTM @ TT on the matvec kernel, TTM @ TTM,
+        TT @ TT^T and TT^T @ TT on the general core-product kernel."""
         if not isinstance(other, TensorTrain):
             raise TypeError(f"unsupported operand type(s) for @: 'TensorTrain' and '{type(other)}'")
         if self.col_modes != other.row_modes:
             raise DimensionMismatch(
                 "Tensor trains must have the same last and first physical dimensions respectively")
-        if other.is_ttm():
-            raise NotImplementedError("TTM @ TTM is not part of this build")
         cores = []
         for a, x in zip(self.cores, other.cores):
-            Ra, nr, _, Rb = a.shape
-            rx, _, _, rx2 = x.shape
-            c = D.empty(Ra * rx, nr, 1, Rb * rx2)
-            D.ttm_core(a, x, P(c), nr * Rb * rx2, Rb * rx2, 1)
+            Ra, nr, nk, Rb = a.shape
+            rx, _, nc, rx2 = x.shape
+            c = D.empty(Ra * rx, nr, nc, Rb * rx2)
+            if nc == 1:
+                D.ttm_core(a, x, P(c), nr * Rb * rx2, Rb * rx2, 1)
+            else:
+                _lib.check(D.lib().ttx_ttm_ttm_core_f64(D.stream(), P(a), Ra, nr, nk, Rb, P(x), rx,
+                                                        nc, rx2, P(c)), "ttx_ttm_ttm_core_f64")
             cores.append(c)
         return _round(TensorTrain(cores), truncate, error, max_rank, method, seed, options)
+
+    def transpose(self, cores_ind: Optional[List[int]] = None, conjugate: bool = False,
+                  inplace: bool = False) -> "TensorTrain":
+        """Swap row and column modes of the given cores, all by default
+        (``tensor_train.py:647-689``); cores are real, so ``conjugate`` changes nothing."""
+        t = self if inplace else self.copy()
+        idx = range(t.n_cores) if cores_ind is None else cores_ind
+        for i in idx:
+            c = t.cores[i]
+            r, nr, nc, r1 = c.shape
+            out = D.empty(r, nc, nr, r1)
+            D.copy_nd([r, nc, nr, r1], P(c), [nr * nc * r1, r1, nc * r1, 1], P(out),
+                      [nc * nr * r1, nr * r1, r1, 1])
+            t.cores[i] = out
+            t.row_modes[i], t.col_modes[i] = nc, nr
+        ttm = any(m > 1 for m in t.col_modes)
+        t.tt_type = TensorTrainType.TTM if ttm else TensorTrainType.TT
+        return t
 
     def __matmul__(self, other):
         return self.dot(other)
