@@ -32,6 +32,15 @@ Pinning (what checks each function, beyond retyping its formula):
   the exactness of the sketch when no unfolding has rank above its bond.
 * environments / local operator / local rhs: brute force against dense
   interface matrices on tiny trains (``x^T A x`` projections).
+* ``tt_to_qtt`` / ``qtt_to_tt`` / ``merge_cores``: the reference's fixtures
+  ``qtt_*.npz`` (truncating plain trains, a TTM with a rank cap, mode 3,
+  regrouping), the properties its own tests assert (round trips, error
+  bound, canonical form, bond floor, exact ranks) and, for the merge, dense
+  Kronecker products.
+* ``tt_multiply`` / ``randomized_hadamard``: the dense element-wise product,
+  the reference's fixture ``multiply_rand.npz``.
+* ``mals``: the reference's trajectories ``mals_*.npz`` and a brute-force
+  projection of the two-site system.
 * ``amen`` / ``als`` / ``relative_residual``: the golden trajectories written
   by the reference (bonds, residual history, stop reason, solution), and the
   dense solution of small systems.
