@@ -161,6 +161,32 @@ int ttx_minres_step_f64(ttx_stream_t stream, int64_t n, const double* y_in, doub
                         double* st);
 
 /*
+ * The one-site local matvec (ttnumpy/solvers/one_site.py:78-115, the
+ * LinearOperator of the local system): for B column vectors
+ *   out[z, fl, i, fr] = alpha * sum psi[tl, a, fl] A[a, i, j, b] y[z, tl, j, tr] phi[tr, b, fr]
+ *                       + beta * out[z, fl, i, fr]
+ * with psi (rl, Ra, fl), A (Ra, nr, nc, Rb), phi (rr, Rb, fr), y (B, rl, nc, rr)
+ * and out (B, fl, nr, fr), as three strided MFMA GEMMs through the scratch
+ * s1 (B rl nc Rb fr doubles) and s2 (B rl Ra nr fr doubles).
+ */
+int ttx_local_apply_f64(ttx_stream_t stream, const double* psi, int64_t rl, int64_t Ra,
+                        int64_t fl, const double* A, int64_t nr, int64_t nc, int64_t Rb,
+                        const double* phi, int64_t rr, int64_t fr, const double* y, int64_t B,
+                        double alpha, double beta, double* out, double* s1, double* s2);
+
+/*
+ * `count` MINRES iterations of the local system (the minres call at
+ * ttnumpy/solvers/common.py:784), issued from C: each applies the local
+ * matvec (as ttx_local_apply_f64, B = 1) to v = vec + n into y_in, then runs
+ * ttx_minres_step_f64.  n = fl nr fr must equal rl nc rr.  Steps after the
+ * stop test has fired leave the state unchanged.
+ */
+int ttx_minres_local_steps_f64(ttx_stream_t stream, const double* psi, int64_t rl, int64_t Ra,
+                               int64_t fl, const double* A, int64_t nr, int64_t nc, int64_t Rb,
+                               const double* phi, int64_t rr, int64_t fr, double* s1, double* s2,
+                               double* y_in, double* vec, double* st, int64_t count);
+
+/*
  * Rank rules.
  * truncated_bond: ttnumpy/utils.py:22-69 -> out[0] = bond.
  */

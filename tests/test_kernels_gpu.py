@@ -51,6 +51,24 @@ def test_gemm_strided(M, N, K, batch, layout):
     np.testing.assert_allclose(_h(Cd), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
 
 
+def test_gemm_tilings_agree_bitwise():
+    """A product large enough for the 64 x 64 tiling (525 tiles) and one row
+    block of it, which runs on the 32 x 32 tiling, give identical values."""
+    print("gemm 64x64 vs 32x32 tiling", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(11)
+    M, N, K = 1600, 1344, 40
+    A, B = rng.standard_normal((M, K)), rng.standard_normal((K, N))
+    Ad, Bd = _t(A), _t(B)
+    C = D.empty(M, N)
+    D.gemm(1, M, N, K, 1.0, D.P(Ad), K, 1, 0, D.P(Bd), N, 1, 0, 0.0, D.P(C), N, 1, 0)
+    part = D.empty(320, N)
+    D.gemm(1, 320, N, K, 1.0, D.P(Ad, 320 * K), K, 1, 0, D.P(Bd), N, 1, 0, 0.0, D.P(part), N, 1, 0)
+    full = _h(C)
+    np.testing.assert_allclose(full, A @ B, rtol=1e-12, atol=1e-12 * np.abs(full).max())
+    np.testing.assert_array_equal(_h(part), full[320:640])
+
+
 def test_gemm_beta_zero_ignores_nan():
     D = _dev()
     A = np.ones((4, 3))

@@ -49,6 +49,10 @@ _SIGNATURES = {
     "ttx_chol_solve_f64": (_i32, [_p, _i64, _p, _i64, _p, _p]),
     "ttx_minres_init_f64": (_i32, [_p, _i64, _p, _p, _p, _f64, _i64, _p, _p]),
     "ttx_minres_step_f64": (_i32, [_p, _i64, _p, _p, _p]),
+    "ttx_local_apply_f64": (_i32, [_p, _p, _i64, _i64, _i64, _p, _i64, _i64, _i64, _p, _i64, _i64,
+                                   _p, _i64, _f64, _f64, _p, _p, _p]),
+    "ttx_minres_local_steps_f64": (_i32, [_p, _p, _i64, _i64, _i64, _p, _i64, _i64, _i64, _p, _i64,
+                                          _i64, _p, _p, _p, _p, _p, _i64]),
     "ttx_truncated_bond_f64": (_i32, [_p, _p, _i64, _f64, _i64, _p]),
     "ttx_truncated_bond_dev_f64": (_i32, [_p, _p, _i64, _p, _f64, _i64, _p]),
     "ttx_rank_by_residual_f64": (_i32, [_p, _i64, _i64, _p, _p, _p, _p, _f64, _f64, _p, _p]),

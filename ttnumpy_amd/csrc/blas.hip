@@ -21,49 +21,68 @@ struct GemmArgs {
   int64_t c_ms, c_ns, c_bs;
 };
 
-// 64x64 output tile per 256-thread block; each of the 4 waves owns a 32x32
-// quadrant = 2x2 MFMA 16x16 blocks.  K is staged 16 at a time through LDS,
-// k-major so that the MFMA operand reads are consecutive across lanes.
+// BM x BN output tile per 256-thread block; the 4 waves split it 2 x 2, each
+// owning a (BM/2) x (BN/2) quadrant of (BM/32) x (BN/32) MFMA 16x16 blocks.
+// K is staged 16 at a time through LDS, k-major so that the MFMA operand
+// reads are consecutive across lanes.  64 x 64 is the default; 32 x 32 (one
+// MFMA block per wave, four times the waves and a quarter of the MFMA chain
+// each) serves the contractions whose 64 x 64 grid would leave most compute
+// units idle.  Both issue the same MFMA sequence per output element, so they
+// give identical results.
 // Operand layouts of v_mfma_f64_16x16x4f64 (wave64):
 //   A (16x4): lane l holds A[l % 16][l / 16]
 //   B (4x16): lane l holds B[l / 16][l % 16]
 //   D (16x16): lane l, element i holds D[4 * i + l / 16][l % 16]
 constexpr int GBM = 64, GBN = 64, GBK = 16;
+constexpr int GSM = 32, GSN = 32;
 
+template <int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs p) {
-  __shared__ double As[GBK][GBM + 1];
-  __shared__ double Bs[GBK][GBN + 1];
+  constexpr int FM = BM / 32, FN = BN / 32;     // MFMA blocks per wave along m and n
+  __shared__ double As[GBK][BM + 1];
+  __shared__ double Bs[GBK][BN + 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int64_t b = blockIdx.z;
-  const int64_t m0 = (int64_t)blockIdx.y * GBM;
-  const int64_t n0 = (int64_t)blockIdx.x * GBN;
+  const int64_t m0 = (int64_t)blockIdx.y * BM;
+  const int64_t n0 = (int64_t)blockIdx.x * BN;
   const double* A = p.A + b * p.a_bs;
   const double* B = p.B + b * p.b_bs;
   double* C = p.C + b * p.c_bs;
   // pick the fastest-varying tile index so that global reads coalesce
   const bool a_m_fast = p.a_ms <= p.a_ks;
   const bool b_n_fast = p.b_ns <= p.b_ks;
-  const int wm = (wave >> 1) * 32;
-  const int wn = (wave & 1) * 32;
-  double4_t acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
-  // each thread stages kPer elements of each slab; the next slab's loads are
-  // issued before the current slab's MFMAs
-  constexpr int kPer = GBM * GBK / 256;
-  int am[kPer], ak[kPer], bn[kPer], bk[kPer];
+  const int wm = (wave >> 1) * (BM / 2);
+  const int wn = (wave & 1) * (BN / 2);
+  double4_t acc[FM][FN];
 #pragma unroll
-  for (int q = 0; q < kPer; ++q) {
+  for (int x = 0; x < FM; ++x)
+#pragma unroll
+    for (int y = 0; y < FN; ++y) acc[x][y] = double4_t{0, 0, 0, 0};
+  // each thread stages kPerA / kPerB elements of each slab; the next slab's
+  // loads are issued before the current slab's MFMAs
+  constexpr int kPerA = BM * GBK / 256, kPerB = BN * GBK / 256;
+  int am[kPerA], ak[kPerA], bn[kPerB], bk[kPerB];
+#pragma unroll
+  for (int q = 0; q < kPerA; ++q) {
     const int e = q * 256 + tid;
-    if (a_m_fast) { am[q] = e % GBM; ak[q] = e / GBM; } else { ak[q] = e % GBK; am[q] = e / GBK; }
-    if (b_n_fast) { bn[q] = e % GBN; bk[q] = e / GBN; } else { bk[q] = e % GBK; bn[q] = e / GBK; }
+    if (a_m_fast) { am[q] = e % BM; ak[q] = e / BM; } else { ak[q] = e % GBK; am[q] = e / GBK; }
   }
-  double ra[kPer], rb[kPer];
+#pragma unroll
+  for (int q = 0; q < kPerB; ++q) {
+    const int e = q * 256 + tid;
+    if (b_n_fast) { bn[q] = e % BN; bk[q] = e / BN; } else { bk[q] = e % GBK; bn[q] = e / GBK; }
+  }
+  double ra[kPerA], rb[kPerB];
   auto load_slab = [&](int64_t k0) {
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
+    for (int q = 0; q < kPerA; ++q) {
       const int64_t gm = m0 + am[q], gka = k0 + ak[q];
       ra[q] = (gm < p.M && gka < p.K) ? A[gm * p.a_ms + gka * p.a_ks] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kPerB; ++q) {
       const int64_t gn = n0 + bn[q], gkb = k0 + bk[q];
       rb[q] = (gn < p.N && gkb < p.K) ? B[gkb * p.b_ks + gn * p.b_ns] : 0.0;
     }
@@ -71,44 +90,46 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs p) {
   load_slab(0);
   for (int64_t k0 = 0; k0 < p.K; k0 += GBK) {
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      As[ak[q]][am[q]] = ra[q];
-      Bs[bk[q]][bn[q]] = rb[q];
-    }
+    for (int q = 0; q < kPerA; ++q) As[ak[q]][am[q]] = ra[q];
+#pragma unroll
+    for (int q = 0; q < kPerB; ++q) Bs[bk[q]][bn[q]] = rb[q];
     __syncthreads();
     if (k0 + GBK < p.K) load_slab(k0 + GBK);
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 4) {
       const int kr = kk + (lane >> 4);
       const int c = lane & 15;
-      const double a0 = As[kr][wm + c];
-      const double a1 = As[kr][wm + 16 + c];
-      const double b0 = Bs[kr][wn + c];
-      const double b1 = Bs[kr][wn + 16 + c];
-      acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-      acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-      acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-      acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+      double av[FM], bv[FN];
+#pragma unroll
+      for (int x = 0; x < FM; ++x) av[x] = As[kr][wm + 16 * x + c];
+#pragma unroll
+      for (int y = 0; y < FN; ++y) bv[y] = Bs[kr][wn + 16 * y + c];
+#pragma unroll
+      for (int x = 0; x < FM; ++x)
+#pragma unroll
+        for (int y = 0; y < FN; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
     }
     __syncthreads();
   }
 
   const int col = lane & 15;
   const int rq = lane >> 4;
-  double4_t* accs[4] = {&acc00, &acc01, &acc10, &acc11};
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int64_t gn = n0 + wn + 16 * (q & 1) + col;
+  for (int x = 0; x < FM; ++x)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int64_t gm = m0 + wm + 16 * (q >> 1) + 4 * i + rq;
-      if (gm < p.M && gn < p.N) {
-        double* dst = C + gm * p.c_ms + gn * p.c_ns;
-        const double v = p.alpha * (*accs[q])[i];
-        *dst = (p.beta == 0.0) ? v : v + p.beta * (*dst);
+    for (int y = 0; y < FN; ++y) {
+      const int64_t gn = n0 + wn + 16 * y + col;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t gm = m0 + wm + 16 * x + 4 * i + rq;
+        if (gm < p.M && gn < p.N) {
+          double* dst = C + gm * p.c_ms + gn * p.c_ns;
+          const double v = p.alpha * acc[x][y][i];
+          *dst = (p.beta == 0.0) ? v : v + p.beta * (*dst);
+        }
       }
     }
-  }
 }
 
 struct CopyArgs {
@@ -280,6 +301,9 @@ extern "C" int ttx_gemm_f32acc(ttx_stream_t stream, int64_t batch, int64_t M, in
   return launch_status();
 }
 
+// below this many 64 x 64 tiles (two per compute unit) the 32 x 32 tiling runs
+constexpr int64_t kSmallTileBelow = 512;
+
 extern "C" int ttx_gemm_f64(ttx_stream_t stream, int64_t batch, int64_t M, int64_t N, int64_t K,
                             double alpha, const double* A, int64_t a_ms, int64_t a_ks,
                             int64_t a_bs, const double* B, int64_t b_ks, int64_t b_ns,
@@ -295,8 +319,14 @@ extern "C" int ttx_gemm_f64(ttx_stream_t stream, int64_t batch, int64_t M, int64
     return ttx_copy_f64(stream, 3, dims, C, ds, C, ds, beta);
   }
   GemmArgs p{M, N, K, alpha, beta, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, C, c_ms, c_ns, c_bs};
-  dim3 grid((unsigned)((N + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM), (unsigned)batch);
-  hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, (hipStream_t)stream, p);
+  const int64_t tiles = ((N + GBN - 1) / GBN) * ((M + GBM - 1) / GBM) * batch;
+  if (tiles < kSmallTileBelow) {
+    dim3 grid((unsigned)((N + GSN - 1) / GSN), (unsigned)((M + GSM - 1) / GSM), (unsigned)batch);
+    hipLaunchKernelGGL((gemm_f64_kernel<GSM, GSN>), grid, dim3(256), 0, (hipStream_t)stream, p);
+  } else {
+    dim3 grid((unsigned)((N + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM), (unsigned)batch);
+    hipLaunchKernelGGL((gemm_f64_kernel<GBM, GBN>), grid, dim3(256), 0, (hipStream_t)stream, p);
+  }
   return launch_status();
 }
 

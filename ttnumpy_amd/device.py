@@ -254,18 +254,27 @@ class LocalOp:
                                 empty(B * self.rl * self.Ra * self.nr * self.fr))
         return self._scratch[B]
 
-    def apply(self, y: int, B: int, out: int, alpha: float = 1.0, beta: float = 0.0) -> None:
+    def flops_per_apply(self, B: int = 1) -> float:
         rl, Ra, fl, rr, Rb, fr, nr, nc = self.rl, self.Ra, self.fl, self.rr, self.Rb, self.fr, self.nr, self.nc
+        return 2.0 * B * (rl * nc * Rb * fr * rr + rl * Ra * nr * fr * nc * Rb + fl * nr * fr * rl * Ra)
+
+    def apply(self, y: int, B: int, out: int, alpha: float = 1.0, beta: float = 0.0) -> None:
+        """s1 = y . phi, s2 = A_k . s1, out = psi^T . s2 (``ttx_local_apply_f64``)."""
         s1, s2 = self._bufs(B)
-        # s1[z, tl, j, b, fr] = sum_tr y[z, tl, j, tr] phi[tr, b, fr]
-        gemm(1, B * rl * nc, Rb * fr, rr, 1.0, y, rr, 1, 0, P(self.phi), Rb * fr, 1, 0, 0.0, P(s1),
-             Rb * fr, 1, 0)
-        # s2[(z, tl), (a, i), fr] = sum_{j, b} A[a, i, j, b] s1[(z, tl), j, b, fr]
-        gemm(B * rl, Ra * nr, fr, nc * Rb, 1.0, P(self.A), nc * Rb, 1, 0, P(s1), fr, 1, nc * Rb * fr,
-             0.0, P(s2), fr, 1, Ra * nr * fr)
-        # out[z, fl, (i, fr)] = sum_{tl, a} psi[tl, a, fl] s2[z, (tl, a), (i, fr)]
-        gemm(B, fl, nr * fr, rl * Ra, alpha, P(self.psi), 1, fl, 0, P(s2), nr * fr, 1,
-             rl * Ra * nr * fr, beta, out, nr * fr, 1, fl * nr * fr)
+        _FLOPS[0] += self.flops_per_apply(B)
+        rc = lib().ttx_local_apply_f64(stream(), P(self.psi), self.rl, self.Ra, self.fl, P(self.A),
+                                       self.nr, self.nc, self.Rb, P(self.phi), self.rr, self.fr, y,
+                                       B, alpha, beta, out, P(s1), P(s2))
+        check(rc, "ttx_local_apply_f64")
+
+    def minres_steps(self, y_in: int, vec: int, st: int, count: int) -> None:
+        """``count`` MINRES iterations with this operator (``ttx_minres_local_steps_f64``)."""
+        s1, s2 = self._bufs(1)
+        _FLOPS[0] += count * self.flops_per_apply(1)
+        rc = lib().ttx_minres_local_steps_f64(stream(), P(self.psi), self.rl, self.Ra, self.fl,
+                                              P(self.A), self.nr, self.nc, self.Rb, P(self.phi),
+                                              self.rr, self.fr, P(s1), P(s2), y_in, vec, st, count)
+        check(rc, "ttx_minres_local_steps_f64")
 
     def explicit(self, out: torch.Tensor, ld: int) -> None:
         """Write the explicit matrix row-major (row stride ``ld``) into ``out`` (``one_site.py:49-75``)."""

@@ -373,9 +373,12 @@ def minres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float,
         chunk = min(_POLL, maxiter - done)
         if chunk <= 0:
             break
-        for _ in range(chunk):
-            op.apply(P(vec, n), 1, P(yA))
-            _lib.check(L.ttx_minres_step_f64(s, n, P(yA), P(vec), P(st)), "ttx_minres_step_f64")
+        if hasattr(op, "minres_steps"):
+            op.minres_steps(P(yA), P(vec), P(st), chunk)
+        else:
+            for _ in range(chunk):
+                op.apply(P(vec, n), 1, P(yA))
+                _lib.check(L.ttx_minres_step_f64(s, n, P(yA), P(vec), P(st)), "ttx_minres_step_f64")
         done += chunk
     if state_out is not None:
         state_out.append(st.cpu().numpy())
