@@ -19,6 +19,7 @@ struct GemmArgs {
   int64_t b_ks, b_ns, b_bs;
   double* C;
   int64_t c_ms, c_ns, c_bs;
+  const double* skip;   // when set and *skip != 0, the launch does nothing
 };
 
 // BM x BN output tile per 256-thread block; the 4 waves split it 2 x 2, each
@@ -38,6 +39,7 @@ constexpr int GSM = 32, GSN = 32;
 
 template <int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs p) {
+  if (p.skip != nullptr && *p.skip != 0.0) return;  // uniform across the grid
   constexpr int FM = BM / 32, FN = BN / 32;     // MFMA blocks per wave along m and n
   __shared__ double As[GBK][BM + 1];
   __shared__ double Bs[GBK][BN + 1];
@@ -309,6 +311,14 @@ extern "C" int ttx_gemm_f64(ttx_stream_t stream, int64_t batch, int64_t M, int64
                             int64_t a_bs, const double* B, int64_t b_ks, int64_t b_ns,
                             int64_t b_bs, double beta, double* C, int64_t c_ms, int64_t c_ns,
                             int64_t c_bs) {
+  return gemm_f64_launch(stream, batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs,
+                         beta, C, c_ms, c_ns, c_bs, nullptr);
+}
+
+int ttx::gemm_f64_launch(ttx_stream_t stream, int64_t batch, int64_t M, int64_t N, int64_t K,
+                    double alpha, const double* A, int64_t a_ms, int64_t a_ks, int64_t a_bs,
+                    const double* B, int64_t b_ks, int64_t b_ns, int64_t b_bs, double beta,
+                    double* C, int64_t c_ms, int64_t c_ns, int64_t c_bs, const double* skip) {
   if (batch < 0 || M < 0 || N < 0 || K < 0) return TTX_EINVAL;
   if (batch == 0 || M == 0 || N == 0) return TTX_OK;
   if (batch > 65535 || (M + GBM - 1) / GBM > 65535) return TTX_EINVAL;
@@ -318,7 +328,8 @@ extern "C" int ttx_gemm_f64(ttx_stream_t stream, int64_t batch, int64_t M, int64
     int64_t ds[3] = {c_bs, c_ms, c_ns};
     return ttx_copy_f64(stream, 3, dims, C, ds, C, ds, beta);
   }
-  GemmArgs p{M, N, K, alpha, beta, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, C, c_ms, c_ns, c_bs};
+  GemmArgs p{M, N, K, alpha, beta, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, C, c_ms, c_ns, c_bs,
+              skip};
   const int64_t tiles = ((N + GBN - 1) / GBN) * ((M + GBM - 1) / GBM) * batch;
   if (tiles < kSmallTileBelow) {
     dim3 grid((unsigned)((N + GSN - 1) / GSN), (unsigned)((M + GSM - 1) / GSM), (unsigned)batch);

@@ -236,21 +236,29 @@ extern "C" int ttx_minres_step_f64(ttx_stream_t stream, int64_t n, const double*
 // out[z, fl, i, fr] = alpha * sum psi[tl, a, fl] A[a, i, j, b] y[z, tl, j, tr] phi[tr, b, fr]
 //                     + beta * out[z, fl, i, fr]
 // as three strided GEMMs: s1 = y . phi, s2 = A . s1, out = psi^T . s2.
+static int local_apply(ttx_stream_t stream, const double* psi, int64_t rl, int64_t Ra, int64_t fl,
+                       const double* A, int64_t nr, int64_t nc, int64_t Rb, const double* phi,
+                       int64_t rr, int64_t fr, const double* y, int64_t B, double alpha,
+                       double beta, double* out, double* s1, double* s2, const double* skip) {
+  if (rl < 1 || Ra < 1 || fl < 1 || nr < 1 || nc < 1 || Rb < 1 || rr < 1 || fr < 1 || B < 1)
+    return TTX_EINVAL;
+  int rc = gemm_f64_launch(stream, 1, B * rl * nc, Rb * fr, rr, 1.0, y, rr, 1, 0, phi, Rb * fr, 1,
+                           0, 0.0, s1, Rb * fr, 1, 0, skip);
+  if (rc != TTX_OK) return rc;
+  rc = gemm_f64_launch(stream, B * rl, Ra * nr, fr, nc * Rb, 1.0, A, nc * Rb, 1, 0, s1, fr, 1,
+                       nc * Rb * fr, 0.0, s2, fr, 1, Ra * nr * fr, skip);
+  if (rc != TTX_OK) return rc;
+  return gemm_f64_launch(stream, B, fl, nr * fr, rl * Ra, alpha, psi, 1, fl, 0, s2, nr * fr, 1,
+                         rl * Ra * nr * fr, beta, out, nr * fr, 1, fl * nr * fr, skip);
+}
+
 extern "C" int ttx_local_apply_f64(ttx_stream_t stream, const double* psi, int64_t rl, int64_t Ra,
                                    int64_t fl, const double* A, int64_t nr, int64_t nc, int64_t Rb,
                                    const double* phi, int64_t rr, int64_t fr, const double* y,
                                    int64_t B, double alpha, double beta, double* out, double* s1,
                                    double* s2) {
-  if (rl < 1 || Ra < 1 || fl < 1 || nr < 1 || nc < 1 || Rb < 1 || rr < 1 || fr < 1 || B < 1)
-    return TTX_EINVAL;
-  int rc = ttx_gemm_f64(stream, 1, B * rl * nc, Rb * fr, rr, 1.0, y, rr, 1, 0, phi, Rb * fr, 1, 0,
-                        0.0, s1, Rb * fr, 1, 0);
-  if (rc != TTX_OK) return rc;
-  rc = ttx_gemm_f64(stream, B * rl, Ra * nr, fr, nc * Rb, 1.0, A, nc * Rb, 1, 0, s1, fr, 1,
-                    nc * Rb * fr, 0.0, s2, fr, 1, Ra * nr * fr);
-  if (rc != TTX_OK) return rc;
-  return ttx_gemm_f64(stream, B, fl, nr * fr, rl * Ra, alpha, psi, 1, fl, 0, s2, nr * fr, 1,
-                      rl * Ra * nr * fr, beta, out, nr * fr, 1, fl * nr * fr);
+  return local_apply(stream, psi, rl, Ra, fl, A, nr, nc, Rb, phi, rr, fr, y, B, alpha, beta, out,
+                     s1, s2, nullptr);
 }
 
 extern "C" int ttx_minres_local_steps_f64(ttx_stream_t stream, const double* psi, int64_t rl,
@@ -261,8 +269,9 @@ extern "C" int ttx_minres_local_steps_f64(ttx_stream_t stream, const double* psi
   const int64_t n = fl * nr * fr;
   if (n < 1 || count < 0 || rl * nc * rr != n) return TTX_EINVAL;
   for (int64_t it = 0; it < count; ++it) {
-    int rc = ttx_local_apply_f64(stream, psi, rl, Ra, fl, A, nr, nc, Rb, phi, rr, fr, vec + n, 1,
-                                 1.0, 0.0, y_in, s1, s2);
+    // once istop is set the matvec launches return at once, as the step does
+    int rc = local_apply(stream, psi, rl, Ra, fl, A, nr, nc, Rb, phi, rr, fr, vec + n, 1, 1.0, 0.0,
+                         y_in, s1, s2, st + S_ISTOP);
     if (rc != TTX_OK) return rc;
     rc = ttx_minres_step_f64(stream, n, y_in, vec, st);
     if (rc != TTX_OK) return rc;

@@ -19,6 +19,14 @@ __device__ __forceinline__ int64_t q32(int64_t e, int64_t m) {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
+// ttx_gemm_f64 with an optional device flag: when `skip` is set and *skip is
+// nonzero at run time, the launch does nothing (Krylov steps queued after the
+// stop test fired)
+int gemm_f64_launch(ttx_stream_t stream, int64_t batch, int64_t M, int64_t N, int64_t K,
+                    double alpha, const double* A, int64_t a_ms, int64_t a_ks, int64_t a_bs,
+                    const double* B, int64_t b_ks, int64_t b_ns, int64_t b_bs, double beta,
+                    double* C, int64_t c_ms, int64_t c_ns, int64_t c_bs, const double* skip);
+
 inline int launch_status() {
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? TTX_OK : TTX_EHIP;

@@ -76,6 +76,10 @@ def P(t: torch.Tensor, off: int = 0) -> int:
 _FLOPS = [0.0]
 
 
+def add_flops(f: float) -> None:
+    _FLOPS[0] += f
+
+
 def flops() -> float:
     return _FLOPS[0]
 
@@ -268,9 +272,9 @@ class LocalOp:
         check(rc, "ttx_local_apply_f64")
 
     def minres_steps(self, y_in: int, vec: int, st: int, count: int) -> None:
-        """``count`` MINRES iterations with this operator (``ttx_minres_local_steps_f64``)."""
+        """``count`` MINRES iterations with this operator (``ttx_minres_local_steps_f64``);
+        the caller tallies the flops of the iterations that ran."""
         s1, s2 = self._bufs(1)
-        _FLOPS[0] += count * self.flops_per_apply(1)
         rc = lib().ttx_minres_local_steps_f64(stream(), P(self.psi), self.rl, self.Ra, self.fl,
                                               P(self.A), self.nr, self.nc, self.Rb, P(self.phi),
                                               self.rr, self.fr, P(s1), P(s2), y_in, vec, st, count)

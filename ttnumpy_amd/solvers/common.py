@@ -367,19 +367,23 @@ def minres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float,
                                      5 * n, P(vec), P(st)), "ttx_minres_init_f64")
     maxiter = 5 * n
     done = 0
+    chunked = hasattr(op, "minres_steps")
     while True:
-        if float(st[0].item()) != 0.0:
+        istop, itn = st[:2].tolist()
+        if istop != 0.0:
             break
         chunk = min(_POLL, maxiter - done)
         if chunk <= 0:
             break
-        if hasattr(op, "minres_steps"):
+        if chunked:
             op.minres_steps(P(yA), P(vec), P(st), chunk)
         else:
             for _ in range(chunk):
                 op.apply(P(vec, n), 1, P(yA))
                 _lib.check(L.ttx_minres_step_f64(s, n, P(yA), P(vec), P(st)), "ttx_minres_step_f64")
         done += chunk
+    if chunked:     # the matvecs queued after the stop test returned at once
+        D.add_flops(itn * op.flops_per_apply(1))
     if state_out is not None:
         state_out.append(st.cpu().numpy())
     return vec[:n].clone()
