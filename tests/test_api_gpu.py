@@ -106,7 +106,12 @@ def test_amen_default_guess_and_threshold_mode_match_oracle():
                       seed=0, return_info=True)
     xo, io = O.amen(O.Train(A_c), O.Train(b_c), max_sweeps=4, symmetric=True, seed=0)
     assert x.bonds == xo.bonds
-    assert info.num_sweeps_run == io.num_sweeps_run
+    if max(info.residuals[-1], io.residuals[-1]) < 1e-12:
+        # solved to round-off: the stagnation stop compares two rounding-level
+        # residuals, so where it fires can move by one sweep
+        assert abs(info.num_sweeps_run - io.num_sweeps_run) <= 1
+    else:
+        assert info.num_sweeps_run == io.num_sweeps_run
     ref = xo.full().reshape(-1)
     assert np.linalg.norm(_full(x) - ref) <= 1e-9 * np.linalg.norm(ref)
 

@@ -78,12 +78,21 @@ def qr_sweep_left(cores: List[torch.Tensor], start: int, end: int) -> None:
 
 
 def qr_sweep_right(cores: List[torch.Tensor], start: int, end: int, form_q: bool = True) -> None:
-    """Right QR sweep ``start..end+1`` on the transposed unfoldings (``utils.py:115-136``)."""
+    """Right QR sweep ``start..end+1`` on the transposed unfoldings (``utils.py:115-136``).
+
+    With ``form_q=False`` only the carried factors matter (the norm of the
+    train ends up in ``cores[end]``).  A transposed unfolding with no more rows
+    than columns (w <= r0) has a square orthogonal Q, so its R is the unfolding
+    up to a rotation that changes no norm: the unfolding itself is carried
+    instead, by one GEMM, and no QR is run."""
     for i in range(start, end, -1):
         c = cores[i]
         r0, nr, nc, r1 = c.shape
         w = nr * nc * r1
         k = min(w, r0)
+        if not form_q and w <= r0:
+            cores[i - 1] = D.absorb_right(cores[i - 1], P(c), r0, w, m_rs=w, m_cs=1)
+            continue
         R = D.empty(k, r0)
         if form_q:
             Qt = D.empty(k, nr, nc, r1)
