@@ -169,6 +169,8 @@ def c5_high_dim(with_cpu: bool):
     b_c = P.random_train_cores([8] * 64, 16, 3)
     A, b = tx.TensorTrain(A_c), tx.TensorTrain(b_c)
     res = {}
+    # one untimed run first: kernel loading and allocator growth for the new shapes
+    tx.amen(A, b, max_rank=128, residual_tol=1e-8, max_sweeps=30, symmetric=True, seed=0)
     for name, stag in (("auto", "auto"), ("no_stagnation_stop", None)):
         torch.cuda.synchronize()
         f0 = D.flops()
