@@ -121,6 +121,17 @@ int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_
  */
 int ttx_engine_profile(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
                        double* ms_out);
+/*
+ * `ticks` ticks of ttx_engine_tick recorded once as a HIP graph (captured on
+ * a private stream; *graph_out receives an opaque handle).  The kernels read
+ * everything through the device slot table `slots`, which must outlive the
+ * graph; admissions and retirements change the table's contents, not its
+ * address, so the graph serves every later poll.  launch replays it on
+ * `stream`; destroy frees it.
+ */
+int ttx_engine_graph_create(ttx_slot* slots, int64_t nslots, int64_t ticks, void** graph_out);
+int ttx_engine_graph_launch(ttx_stream_t stream, void* graph);
+int ttx_engine_graph_destroy(void* graph);
 
 #ifdef __cplusplus
 }

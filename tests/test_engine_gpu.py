@@ -168,3 +168,30 @@ def test_engine_mixed_precision_c1():
     d = _rel_diff(r32.result, O.Train(r64.result.to_numpy()))
     print("relative difference mixed vs fp64", d, "fp64-verified residual", res, flush=True)
     assert d < 1e-5
+
+
+def test_engine_graph_replays_equal_eager_ticks():
+    """Polls replayed from one recorded HIP graph give bit-identical results
+    to eager tick launches: a mix of requests admitted while others run (the
+    slot table changes between replays, with host work between them), served
+    twice on the same graph engine."""
+    from ttnumpy_amd.engine import Engine
+    print("engine graph replays vs eager", flush=True)
+
+    def mix():
+        return [_poisson_request(4, seed=2), _poisson_request(5, seed=1, x0_seed=1),
+                _poisson_request(6, seed=3, x0_seed=5), _poisson_request(5, seed=7, x0_seed=2)]
+
+    eager = Engine(3, rmax=40, graphs=False).run(mix(), poll=4)
+    geng = Engine(3, rmax=40, graphs=True)
+    for rep in range(2):
+        graph = geng.run(mix(), poll=4)
+        assert len(geng._graphs) == 1
+        for a, b in zip(sorted(eager, key=lambda r: (r.seed, len(r.A))),
+                        sorted(graph, key=lambda r: (r.seed, len(r.A)))):
+            assert a.error is None and b.error is None
+            assert a.info.residuals == b.info.residuals
+            assert a.result.bonds == b.result.bonds
+            for ca, cb in zip(a.result.to_numpy(), b.result.to_numpy()):
+                np.testing.assert_array_equal(ca, cb)
+    geng.close_graphs()

@@ -1475,6 +1475,57 @@ extern "C" int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nsl
   return launch_status();
 }
 
+// `ticks` ticks of the engine recorded once as a HIP graph.  The kernels'
+// only argument is the device slot table, which outlives the graph, and every
+// per-request change goes through that table, so one recording serves every
+// later poll of the engine.
+struct EngineGraph {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" int ttx_engine_graph_create(ttx_slot* slots, int64_t nslots, int64_t ticks,
+                                       void** graph_out) {
+  if (nslots < 1 || ticks < 1 || graph_out == nullptr) return TTX_EINVAL;
+  if (engine_prepare() != TTX_OK) return TTX_EHIP;
+  hipStream_t cs;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return TTX_EHIP;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int rc = TTX_OK;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = TTX_EHIP;
+  } else {
+    for (int64_t t = 0; t < ticks; ++t)
+      for (int i = 0; i < TTX_ENGINE_KERNELS; ++i) launch_stage(i, cs, slots, nslots);
+    if (hipStreamEndCapture(cs, &g) != hipSuccess) rc = TTX_EHIP;
+  }
+  if (rc == TTX_OK && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess)
+    rc = TTX_EHIP;
+  hipStreamDestroy(cs);
+  if (rc != TTX_OK) {
+    if (g != nullptr) hipGraphDestroy(g);
+    return rc;
+  }
+  *graph_out = new EngineGraph{g, exec};
+  return TTX_OK;
+}
+
+extern "C" int ttx_engine_graph_launch(ttx_stream_t stream, void* graph) {
+  if (graph == nullptr) return TTX_EINVAL;
+  EngineGraph* eg = (EngineGraph*)graph;
+  return hipGraphLaunch(eg->exec, (hipStream_t)stream) == hipSuccess ? TTX_OK : TTX_EHIP;
+}
+
+extern "C" int ttx_engine_graph_destroy(void* graph) {
+  if (graph == nullptr) return TTX_OK;
+  EngineGraph* eg = (EngineGraph*)graph;
+  hipGraphExecDestroy(eg->exec);
+  hipGraphDestroy(eg->graph);
+  delete eg;
+  return TTX_OK;
+}
+
 extern "C" int ttx_engine_profile(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
                                   double* ms_out) {
   if (nslots < 1 || ticks < 1) return TTX_EINVAL;

@@ -83,6 +83,7 @@ def _check_layout(lib) -> None:
     lib.ttx_engine_plan.restype = ctypes.c_int
     lib.ttx_engine_tick.argtypes = [_vp, _vp, _i64, _i64]
     lib.ttx_engine_tick.restype = ctypes.c_int
+
     lib.ttx_engine_profile.argtypes = [_vp, _vp, _i64, _i64, ctypes.POINTER(_f64)]
     lib.ttx_engine_profile.restype = ctypes.c_int
     if lib.ttx_engine_slot_bytes() != ctypes.sizeof(Slot):
@@ -126,8 +127,12 @@ class Engine:
     """A fixed set of slots on one GPU plus the continuous-batching loop."""
 
     def __init__(self, nslots: int, rmax: int, chunk: int = 8, hbm_reserve_bytes: int = 2 << 30,
-                 minres_slice: int = 128):
+                 minres_slice: int = 128, graphs: bool = True):
         self.lib = _lib.load()
+        # graphs: each poll's ticks replay one recorded HIP graph instead of
+        # seven launches per tick (same kernels, same order, same results)
+        self.graphs = bool(graphs)
+        self._graphs = {}
         _check_layout(self.lib)
         self.dev = D.device()
         self.nslots = int(nslots)
@@ -400,8 +405,29 @@ class Engine:
             self.kernel_ms += np.array(list(ms))
             self.kernel_launches += n
             return
+        if self.graphs:
+            g = self._graphs.get(n)
+            if g is None:
+                h = _vp()
+                _lib.check(self.lib.ttx_engine_graph_create(self.dev_slots.data_ptr(), self.nslots, n,
+                                                            ctypes.byref(h)), "ttx_engine_graph_create")
+                g = self._graphs[n] = h
+            _lib.check(self.lib.ttx_engine_graph_launch(D.stream(), g), "ttx_engine_graph_launch")
+            return
         rc = self.lib.ttx_engine_tick(D.stream(), self.dev_slots.data_ptr(), self.nslots, n)
         _lib.check(rc, "ttx_engine_tick")
+
+    def close_graphs(self) -> None:
+        """Free the recorded tick graphs (also done when the engine is collected)."""
+        for g in self._graphs.values():
+            self.lib.ttx_engine_graph_destroy(g)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            self.close_graphs()
+        except Exception:  # interpreter shutdown: the library may be gone already
+            pass
 
     def active(self) -> int:
         return sum(1 for r in self.slot_req if r is not None)

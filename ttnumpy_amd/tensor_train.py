@@ -77,20 +77,26 @@ def qr_sweep_left(cores: List[torch.Tensor], start: int, end: int) -> None:
         cores[i + 1] = D.absorb_left(P(R), k, r1, cores[i + 1])
 
 
+_WIDE_CARRY_MIN = 1 << 20
+
+
 def qr_sweep_right(cores: List[torch.Tensor], start: int, end: int, form_q: bool = True) -> None:
     """Right QR sweep ``start..end+1`` on the transposed unfoldings (``utils.py:115-136``).
 
     With ``form_q=False`` only the carried factors matter (the norm of the
     train ends up in ``cores[end]``).  A transposed unfolding with no more rows
     than columns (w <= r0) has a square orthogonal Q, so its R is the unfolding
-    up to a rotation that changes no norm: the unfolding itself is carried
-    instead, by one GEMM, and no QR is run."""
+    up to a rotation that changes no norm: for such unfoldings of at least
+    ``_WIDE_CARRY_MIN`` elements (one-workgroup QRs of milliseconds) the
+    unfolding itself is carried instead, by one GEMM.  Smaller ones keep the
+    QR, which the batched engine's residual check runs too, so both paths
+    compute the same bits."""
     for i in range(start, end, -1):
         c = cores[i]
         r0, nr, nc, r1 = c.shape
         w = nr * nc * r1
         k = min(w, r0)
-        if not form_q and w <= r0:
+        if not form_q and w <= r0 and w * r0 >= _WIDE_CARRY_MIN:
             cores[i - 1] = D.absorb_right(cores[i - 1], P(c), r0, w, m_rs=w, m_cs=1)
             continue
         R = D.empty(k, r0)
