@@ -240,6 +240,13 @@ int ttx_hadamard_core_f64(ttx_stream_t stream, const double* A, int64_t ra, int6
                           int64_t nj, int64_t rb, const double* B, int64_t sa, int64_t sb,
                           double* out);
 
+/* The tiling of ttx_gemm_f64 (no counterpart in the reference: a tuning knob of
+ * this library).  A contraction whose 64 x 64 tile grid (batch included) has
+ * fewer than `tiles` tiles runs on 32 x 32 tiles, which give identical values.
+ * Process-wide; default 512.  Set by ttnumpy_amd.autotune from timings. */
+int ttx_gemm_set_small_tile_limit(int64_t tiles);
+int64_t ttx_gemm_small_tile_limit(void);
+
 /* out[(a m), i, j, (b n)] = sum_k A[a, i, k, b] * B[m, k, j, n] for cores A
  * (ra, ni, nk, rb) and B (sa, nk, nj, sb): one core of the matrix product of
  * two trains, TTM @ TTM or TT @ TT^T (ttnumpy/tensor_train.py:427-499, dot). */

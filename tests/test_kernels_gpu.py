@@ -400,3 +400,24 @@ def test_gemm_f32acc_matches_fp32_product(M, N, K):
     got2 = _h(C).reshape(M, N) / big
     assert np.all(np.isfinite(got2))
     assert np.abs(got2 - A @ B).max() <= 4 * K * 6e-8 * scale.max()
+
+
+def test_autotune_sets_a_limit_and_keeps_results():
+    """Tuning the GEMM tiling on a workload changes its speed only: the
+    truncation gives the same bits before and after."""
+    print("gemm tiling autotune", flush=True)
+    import ttnumpy_amd as tx
+    from ttnumpy_amd import autotune as AT
+    from ttnumpy_amd import problems as Pb
+    x = tx.TensorTrain(Pb.random_train_cores([4] * 10, 24, 1, normal=True))
+    before = x.truncate(error=1e-10, max_rank=16).to_numpy()
+    old = AT.get_limit()
+    try:
+        rep = AT.tune(lambda: x.truncate(error=1e-10, max_rank=16), reps=3)
+        assert rep["distinct_shapes"] > 0 and AT.get_limit() == rep["limit"]
+        assert rep["predicted_ms"]["tuned"] <= rep["predicted_ms"]["previous"] + 1e-9
+        after = x.truncate(error=1e-10, max_rank=16).to_numpy()
+        for a, b in zip(before, after):
+            np.testing.assert_array_equal(a, b)
+    finally:
+        AT.set_limit(old)

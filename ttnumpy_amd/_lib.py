@@ -65,6 +65,8 @@ _SIGNATURES = {
     "ttx_scale_maxabs_f64": (_i32, [_p, _i64, _p]),
     "ttx_hadamard_core_f64": (_i32, [_p, _p, _i64, _i64, _i64, _i64, _p, _i64, _i64, _p]),
     "ttx_ttm_ttm_core_f64": (_i32, [_p, _p, _i64, _i64, _i64, _i64, _p, _i64, _i64, _i64, _p]),
+    "ttx_gemm_set_small_tile_limit": (_i32, [_i64]),
+    "ttx_gemm_small_tile_limit": (_i64, []),
     "ttx_rank_by_threshold_f64": (_i32, [_p, _p, _i64, _f64, _i64, _p]),
     "ttx_engine_slot_bytes": (_i64, []),
     "ttx_engine_field_offset": (_i64, [_i32]),

@@ -303,8 +303,17 @@ extern "C" int ttx_gemm_f32acc(ttx_stream_t stream, int64_t batch, int64_t M, in
   return launch_status();
 }
 
-// below this many 64 x 64 tiles (two per compute unit) the 32 x 32 tiling runs
-constexpr int64_t kSmallTileBelow = 512;
+// below this many 64 x 64 tiles the 32 x 32 tiling runs; two tiles per compute
+// unit unless an autotuner has set another limit (process-wide)
+static int64_t g_small_tile_below = 512;
+
+extern "C" int ttx_gemm_set_small_tile_limit(int64_t tiles) {
+  if (tiles < 0) return TTX_EINVAL;
+  g_small_tile_below = tiles;
+  return TTX_OK;
+}
+
+extern "C" int64_t ttx_gemm_small_tile_limit(void) { return g_small_tile_below; }
 
 extern "C" int ttx_gemm_f64(ttx_stream_t stream, int64_t batch, int64_t M, int64_t N, int64_t K,
                             double alpha, const double* A, int64_t a_ms, int64_t a_ks,
@@ -331,7 +340,7 @@ int ttx::gemm_f64_launch(ttx_stream_t stream, int64_t batch, int64_t M, int64_t 
   GemmArgs p{M, N, K, alpha, beta, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, C, c_ms, c_ns, c_bs,
               skip};
   const int64_t tiles = ((N + GBN - 1) / GBN) * ((M + GBM - 1) / GBM) * batch;
-  if (tiles < kSmallTileBelow) {
+  if (tiles < g_small_tile_below) {
     dim3 grid((unsigned)((N + GSN - 1) / GSN), (unsigned)((M + GSM - 1) / GSM), (unsigned)batch);
     hipLaunchKernelGGL((gemm_f64_kernel<GSM, GSN>), grid, dim3(256), 0, (hipStream_t)stream, p);
   } else {

@@ -13,7 +13,7 @@ operands in place, so no permuted copy of a core or environment is made.
 from __future__ import annotations
 
 import math
-from typing import Optional, Tuple
+from typing import List, Optional, Tuple
 
 import torch
 
@@ -91,9 +91,24 @@ def _qr_flops(m: int, n: int, form_q: bool) -> float:
 
 
 # ---------------------------------------------------------------- primitives
+# While a recorder dict is installed here, the FP64 GEMMs issued through this
+# module are counted by shape: (batch, M, N, K, A m-fastest, B n-fastest) -> calls
+# (ttnumpy_amd.autotune times those shapes).
+_GEMM_RECORD: List[Optional[dict]] = [None]
+
+
+def record_gemm(batch, M, N, K, a_m_fast: bool, b_n_fast: bool, count: int = 1) -> None:
+    rec = _GEMM_RECORD[0]
+    if rec is not None:
+        key = (int(batch), int(M), int(N), int(K), bool(a_m_fast), bool(b_n_fast))
+        rec[key] = rec.get(key, 0) + int(count)
+
+
 def gemm(batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns, b_bs, beta, C, c_ms, c_ns, c_bs):
     """C = alpha A B + beta C on MFMA; A/B/C are addresses (ints)."""
     _FLOPS[0] += 2.0 * batch * M * N * K
+    if _GEMM_RECORD[0] is not None:
+        record_gemm(batch, M, N, K, a_ms <= a_ks, b_ns <= b_ks)
     rc = lib().ttx_gemm_f64(stream(), batch, M, N, K, alpha, A, a_ms, a_ks, a_bs, B, b_ks, b_ns,
                             b_bs, beta, C, c_ms, c_ns, c_bs)
     check(rc, "ttx_gemm_f64")
@@ -262,10 +277,18 @@ class LocalOp:
         rl, Ra, fl, rr, Rb, fr, nr, nc = self.rl, self.Ra, self.fl, self.rr, self.Rb, self.fr, self.nr, self.nc
         return 2.0 * B * (rl * nc * Rb * fr * rr + rl * Ra * nr * fr * nc * Rb + fl * nr * fr * rl * Ra)
 
+    def _record(self, B: int, count: int) -> None:
+        rl, Ra, fl, rr, Rb, fr, nr, nc = self.rl, self.Ra, self.fl, self.rr, self.Rb, self.fr, self.nr, self.nc
+        record_gemm(1, B * rl * nc, Rb * fr, rr, rr <= 1, True, count)
+        record_gemm(B * rl, Ra * nr, fr, nc * Rb, nc * Rb <= 1, True, count)
+        record_gemm(B, fl, nr * fr, rl * Ra, True, True, count)
+
     def apply(self, y: int, B: int, out: int, alpha: float = 1.0, beta: float = 0.0) -> None:
         """s1 = y . phi, s2 = A_k . s1, out = psi^T . s2 (``ttx_local_apply_f64``)."""
         s1, s2 = self._bufs(B)
         _FLOPS[0] += self.flops_per_apply(B)
+        if _GEMM_RECORD[0] is not None:
+            self._record(B, 1)
         rc = lib().ttx_local_apply_f64(stream(), P(self.psi), self.rl, self.Ra, self.fl, P(self.A),
                                        self.nr, self.nc, self.Rb, P(self.phi), self.rr, self.fr, y,
                                        B, alpha, beta, out, P(s1), P(s2))
@@ -275,6 +298,8 @@ class LocalOp:
         """``count`` MINRES iterations with this operator (``ttx_minres_local_steps_f64``);
         the caller tallies the flops of the iterations that ran."""
         s1, s2 = self._bufs(1)
+        if _GEMM_RECORD[0] is not None:
+            self._record(1, count)
         rc = lib().ttx_minres_local_steps_f64(stream(), P(self.psi), self.rl, self.Ra, self.fl,
                                               P(self.A), self.nr, self.nc, self.Rb, P(self.phi),
                                               self.rr, self.fr, P(s1), P(s2), y_in, vec, st, count)
