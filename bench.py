@@ -377,6 +377,8 @@ def main():
                     help="instead of the C1 line, serve the C4 stream of N requests and print its "
                          "report (with the oracle FIFO estimate unless --no-cpu)")
     ap.add_argument("--c4-cpu-per-level", type=int, default=2)
+    ap.add_argument("--only", choices=("c2", "c3", "c5"), default=None,
+                    help="instead of the C1 line, run one of the other measurements and print it")
     ap.add_argument("--single", action="store_true",
                     help="also time one request alone (its launches are not part of the roofline)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -389,6 +391,11 @@ def main():
     import torch
     from ttnumpy_amd.engine import Engine
     assert torch.cuda.is_available(), "bench.py needs the MI355X"
+    if args.only is not None:
+        leg = {"c2": ("c2_rounding", c2_rounding), "c3": ("c3_diffusion_3d", c3_diffusion_3d),
+               "c5": ("c5_high_dim", c5_high_dim)}[args.only]
+        print(json.dumps({leg[0]: leg[1](not args.no_cpu)}), flush=True)
+        return
     if args.copy_calibration:
         from ttnumpy_amd import device as D
         n = args.copy_calibration * (1 << 20) // 8
