@@ -161,6 +161,31 @@ int ttx_minres_step_f64(ttx_stream_t stream, int64_t n, const double* y_in, doub
                         double* st);
 
 /*
+ * Restarted GMRES (the iteration of scipy.sparse.linalg.gmres with atol = 0,
+ * no preconditioner, restart m = min(20, n), maxiter = 10 n cycles, called at
+ * ttnumpy/solvers/common.py:786): modified Gram-Schmidt Arnoldi, LAPACK-style
+ * Givens rotations, the adaptive inner tolerance, the true residual after each
+ * cycle.  `work` holds ttx_gmres_work_size(n, m) doubles (the Krylov basis V,
+ * x at V + (m+1) n, A x at V + (m+2) n, ...), `st` TTX_GMRES_STATE doubles:
+ * st[0] done, st[2] info (0 or maxiter), st[10] |b - A x|, st[13] inner steps.
+ *   init    : x = x0 (or 0); `ax0` = A x0 when x0 is given, else NULL
+ *   arnoldi : step `col` of a cycle, after the caller wrote A V[col] into V[col+1]
+ *   finish  : end of a cycle, x += y V
+ *   check   : after the caller wrote A x into V + (m+2) n; next cycle or done
+ * Steps after the inner stop test (st[1] set) or after done leave the state alone.
+ */
+#define TTX_GMRES_STATE 16
+int64_t ttx_gmres_work_size(int64_t n, int64_t m);
+int ttx_gmres_init_f64(ttx_stream_t stream, int64_t n, int64_t m, const double* b,
+                       const double* x0, const double* ax0, double rtol, int64_t maxiter,
+                       double* work, double* st);
+int ttx_gmres_arnoldi_f64(ttx_stream_t stream, int64_t n, int64_t m, int64_t col, double* work,
+                          double* st);
+int ttx_gmres_finish_f64(ttx_stream_t stream, int64_t n, int64_t m, double* work, double* st);
+int ttx_gmres_check_f64(ttx_stream_t stream, int64_t n, int64_t m, const double* b, double* work,
+                        double* st);
+
+/*
  * The one-site local matvec (ttnumpy/solvers/one_site.py:78-115, the
  * LinearOperator of the local system): for B column vectors
  *   out[z, fl, i, fr] = alpha * sum psi[tl, a, fl] A[a, i, j, b] y[z, tl, j, tr] phi[tr, b, fr]
@@ -185,6 +210,16 @@ int ttx_minres_local_steps_f64(ttx_stream_t stream, const double* psi, int64_t r
                                int64_t fl, const double* A, int64_t nr, int64_t nc, int64_t Rb,
                                const double* phi, int64_t rr, int64_t fr, double* s1, double* s2,
                                double* y_in, double* vec, double* st, int64_t count);
+
+/*
+ * One GMRES cycle of the local system issued from C: m arnoldi steps, each
+ * after the local matvec of V[col] (skipped once the inner loop has stopped),
+ * then finish, the matvec of x and check.
+ */
+int ttx_gmres_local_cycle_f64(ttx_stream_t stream, const double* psi, int64_t rl, int64_t Ra,
+                              int64_t fl, const double* A, int64_t nr, int64_t nc, int64_t Rb,
+                              const double* phi, int64_t rr, int64_t fr, double* s1, double* s2,
+                              int64_t m, const double* b, double* work, double* st);
 
 /*
  * Rank rules.

@@ -140,3 +140,46 @@ def test_dense_fallback_is_counted_when_armed():
     assert info.local_solves > 0
     assert 0 < info.local_fallbacks <= info.local_solves
     assert info.local_inexact == 0
+
+
+def _convection_diffusion_cores(n_bits, c):
+    """-u'' + c u' on 2**n_bits points (Dirichlet, central differences): the QTT
+    Laplacian plus c (S - S^T) / (2h) with S the up-shift, summed and rounded."""
+    from ttnumpy_amd import diffusion3d as T3
+    h = 1.0 / (2**n_bits + 1)
+    S = T3.shift_cores(n_bits)
+    St = [np.ascontiguousarray(x.transpose(0, 2, 1, 3)) for x in S]
+    S = [x.copy() for x in S]
+    St[0] = St[0] * (-1.0)
+    S[0] = S[0] * (c / (2 * h))
+    St[0] = St[0] * (c / (2 * h))
+    lap = P.laplacian_1d_cores(n_bits, h)
+    return P.round_cores(P.block_sum_cores(P.block_sum_cores(lap, S), St), 1e-14)
+
+
+def test_amen_nonsymmetric_gmres_matches_oracle():
+    """symmetric=False with iterative local solves (GMRES, restart 20) on a
+    convection-diffusion operator: the device and the oracle (scipy's gmres,
+    as the reference calls it) both reach 1e-8, in sweep counts within two,
+    with solutions within the residual-implied distance of the dense solution."""
+    print("api amen nonsymmetric gmres", flush=True)
+    tx = _tx()
+    L = 6
+    A_c = _convection_diffusion_cores(L, 20.0)
+    b_c = P.ones_cores([2] * L)
+    x0 = tx.rand([2] * L, ranks=2, seed=0)
+    kw = dict(max_rank=16, residual_tol=1e-8, max_sweeps=10, symmetric=False, seed=0,
+              direct_solve_size=4)
+    x, info = tx.amen(tx.TensorTrain(A_c), tx.TensorTrain(b_c), x0=x0, return_info=True, **kw)
+    xo, io = O.amen(O.Train(A_c), O.Train(b_c), O.Train(x0.to_numpy()), **kw)
+    Ad = O.Train(A_c).dense_matrix()
+    assert np.linalg.norm(Ad - Ad.T) > 1.0        # really nonsymmetric
+    ref = np.linalg.solve(Ad, np.ones(2**L))
+    cond = np.linalg.cond(Ad)
+    for stop, res in ((info.stop_reason, info.residuals), (io.stop_reason, io.residuals)):
+        assert str(stop) in ("converged", "stagnated")
+        assert res[-1] <= 3e-8
+    assert info.local_solves > 0
+    assert abs(info.num_sweeps_run - io.num_sweeps_run) <= 2
+    for xf in (_full(x), xo.full().reshape(-1)):
+        assert np.linalg.norm(xf - ref) <= 3e-8 * cond * np.linalg.norm(ref)

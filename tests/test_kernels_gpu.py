@@ -421,3 +421,79 @@ def test_autotune_sets_a_limit_and_keeps_results():
             np.testing.assert_array_equal(a, b)
     finally:
         AT.set_limit(old)
+
+
+@pytest.mark.parametrize("n,seed,with_x0", [(60, 1, False), (300, 2, False), (300, 3, True),
+                                            (45, 4, True)])
+def test_gmres_matches_scipy(n, seed, with_x0):
+    """Restarted GMRES on dense nonsymmetric systems (restart 20, so several
+    cycles run) against scipy.sparse.linalg.gmres with the same rtol and x0:
+    both meet the tolerance and agree to the distance it allows."""
+    print(f"gmres n={n} seed={seed} x0={with_x0}", flush=True)
+    import scipy.sparse.linalg as spla
+    from ttnumpy_amd.solvers.common import gmres_device
+    D = _dev()
+    rng = np.random.default_rng(seed)
+    A = np.eye(n) * 4.0 + rng.standard_normal((n, n)) / np.sqrt(n)
+    b = rng.standard_normal(n)
+    x0 = rng.standard_normal(n) * 0.1 if with_x0 else None
+    rtol = 1e-10
+    ref, info = spla.gmres(A, b, x0=x0, rtol=rtol, atol=0.0)
+    assert info == 0
+    op = D.ExplicitOp(_t(A.copy()), n)
+    x = _h(gmres_device(op, _t(b), _t(x0) if x0 is not None else None, rtol))
+    assert np.linalg.norm(A @ x - b) <= rtol * np.linalg.norm(b) * 1.0001
+    cond = np.linalg.cond(A)
+    assert np.linalg.norm(x - ref) <= 2 * cond * rtol * np.linalg.norm(ref)
+
+
+def test_gmres_special_cases():
+    """b = 0 returns zeros; an exact x0 returns at once; the identity breaks
+    down after one step with the exact solution (scipy's early exits)."""
+    print("gmres special cases", flush=True)
+    from ttnumpy_amd.solvers.common import gmres_device
+    D = _dev()
+    n = 30
+    rng = np.random.default_rng(9)
+    A = np.eye(n) * 3.0 + rng.standard_normal((n, n)) * 0.1
+    op = D.ExplicitOp(_t(A.copy()), n)
+    np.testing.assert_array_equal(_h(gmres_device(op, _t(np.zeros(n)), None, 1e-8)), np.zeros(n))
+    xs = rng.standard_normal(n)
+    b = A @ xs
+    x = _h(gmres_device(op, _t(b), _t(xs), 1e-8))
+    assert np.linalg.norm(A @ x - b) <= 1e-8 * np.linalg.norm(b)
+    eye = D.ExplicitOp(_t(np.eye(n)), n)
+    x = _h(gmres_device(eye, _t(b), None, 1e-12))
+    np.testing.assert_allclose(x, b, rtol=1e-14)
+
+
+def test_gmres_local_cycle_matches_explicit_operator():
+    """The cycle issued from C on the matrix-free one-site operator and the
+    host-driven cycle on its explicit matrix solve the same (nonsymmetric,
+    well-conditioned) system to the tolerance."""
+    print("gmres local cycle vs explicit", flush=True)
+    from ttnumpy_amd.solvers.common import gmres_device
+    D = _dev()
+    rng = np.random.default_rng(12)
+    r, n_ = 4, 5
+    psi = np.zeros((r, 2, r))
+    phi = np.zeros((r, 2, r))
+    A = np.zeros((2, n_, n_, 2))
+    psi[:, 0, :] = np.eye(r)
+    phi[:, 0, :] = np.eye(r)
+    psi[:, 1, :] = 0.3 * rng.standard_normal((r, r))
+    phi[:, 1, :] = 0.3 * rng.standard_normal((r, r))
+    A[0, :, :, 0] = 4.0 * np.eye(n_) + rng.standard_normal((n_, n_)) / np.sqrt(n_)
+    A[1, :, :, 1] = 0.3 * rng.standard_normal((n_, n_))
+    op = D.LocalOp(_t(psi), _t(A), _t(phi))
+    N = op.rows
+    M = D.empty(N * N)
+    op.explicit(M, N)
+    Mh = _h(M).reshape(N, N)
+    b = rng.standard_normal(N)
+    rtol = 1e-11
+    x_ex = _h(gmres_device(D.ExplicitOp(_t(Mh.copy()), N), _t(b), None, rtol))
+    x_lo = _h(gmres_device(op, _t(b), None, rtol))
+    for x in (x_ex, x_lo):
+        assert np.linalg.norm(Mh @ x - b) <= 1.01 * rtol * np.linalg.norm(b)
+    assert np.linalg.norm(x_lo - x_ex) <= 2 * np.linalg.cond(Mh) * rtol * np.linalg.norm(x_ex)

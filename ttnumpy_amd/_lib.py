@@ -21,6 +21,7 @@ TTX_EHIP = 4
 TTX_ESINGULAR = 5
 TTX_ENOTSPD = 6
 TTX_MINRES_STATE = 32
+TTX_GMRES_STATE = 16
 
 _i64 = ctypes.c_int64
 _f64 = ctypes.c_double
@@ -53,6 +54,13 @@ _SIGNATURES = {
                                    _p, _i64, _f64, _f64, _p, _p, _p]),
     "ttx_minres_local_steps_f64": (_i32, [_p, _p, _i64, _i64, _i64, _p, _i64, _i64, _i64, _p, _i64,
                                           _i64, _p, _p, _p, _p, _p, _i64]),
+    "ttx_gmres_work_size": (_i64, [_i64, _i64]),
+    "ttx_gmres_init_f64": (_i32, [_p, _i64, _i64, _p, _p, _p, _f64, _i64, _p, _p]),
+    "ttx_gmres_arnoldi_f64": (_i32, [_p, _i64, _i64, _i64, _p, _p]),
+    "ttx_gmres_finish_f64": (_i32, [_p, _i64, _i64, _p, _p]),
+    "ttx_gmres_check_f64": (_i32, [_p, _i64, _i64, _p, _p, _p]),
+    "ttx_gmres_local_cycle_f64": (_i32, [_p, _p, _i64, _i64, _i64, _p, _i64, _i64, _i64, _p, _i64,
+                                         _i64, _p, _p, _i64, _p, _p, _p]),
     "ttx_truncated_bond_f64": (_i32, [_p, _p, _i64, _f64, _i64, _p]),
     "ttx_truncated_bond_dev_f64": (_i32, [_p, _p, _i64, _p, _f64, _i64, _p]),
     "ttx_rank_by_residual_f64": (_i32, [_p, _i64, _i64, _p, _p, _p, _p, _f64, _f64, _p, _p]),

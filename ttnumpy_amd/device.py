@@ -294,6 +294,17 @@ class LocalOp:
                                        B, alpha, beta, out, P(s1), P(s2))
         check(rc, "ttx_local_apply_f64")
 
+    def gmres_cycle(self, m: int, b: int, work: int, st: int) -> None:
+        """One GMRES cycle with this operator (``ttx_gmres_local_cycle_f64``); the
+        caller tallies the flops of the steps that ran."""
+        s1, s2 = self._bufs(1)
+        if _GEMM_RECORD[0] is not None:
+            self._record(1, m + 1)
+        rc = lib().ttx_gmres_local_cycle_f64(stream(), P(self.psi), self.rl, self.Ra, self.fl,
+                                             P(self.A), self.nr, self.nc, self.Rb, P(self.phi),
+                                             self.rr, self.fr, P(s1), P(s2), m, b, work, st)
+        check(rc, "ttx_gmres_local_cycle_f64")
+
     def minres_steps(self, y_in: int, vec: int, st: int, count: int) -> None:
         """``count`` MINRES iterations with this operator (``ttx_minres_local_steps_f64``);
         the caller tallies the flops of the iterations that ran."""

@@ -389,6 +389,51 @@ def minres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float,
     return vec[:n].clone()
 
 
+_GMRES_RESTART = 20   # scipy.sparse.linalg.gmres's default restart length
+
+
+def gmres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float) -> torch.Tensor:
+    """``scipy.sparse.linalg.gmres(op, b, x0=x0, rtol=rtol, atol=0.0)`` on the device.
+
+    The Arnoldi steps, rotations, cycle updates and stopping tests run in the
+    ``ttx_gmres_*`` kernels; the operator is applied between them (for the
+    one-site operator a whole cycle is issued by one
+    ``ttx_gmres_local_cycle_f64`` call); the host polls the done flag once
+    per cycle."""
+    n = op.rows
+    m = min(_GMRES_RESTART, n)
+    maxiter = 10 * n
+    L = D.lib()
+    s = D.stream()
+    work = D.empty(int(L.ttx_gmres_work_size(n, m)))
+    st = D.empty(_lib.TTX_GMRES_STATE)
+    xoff, axoff = (m + 1) * n, (m + 2) * n
+    ax0 = None
+    if x0 is not None:
+        op.apply(P(x0), 1, P(work, axoff))
+        ax0 = P(work, axoff)
+    _lib.check(L.ttx_gmres_init_f64(s, n, m, P(b), P(x0) if x0 is not None else None, ax0, rtol,
+                                    maxiter, P(work), P(st)), "ttx_gmres_init_f64")
+    chunked = hasattr(op, "gmres_cycle")
+    cycles = 0
+    while cycles < maxiter and float(st[0].item()) == 0.0:
+        if chunked:
+            op.gmres_cycle(m, P(b), P(work), P(st))
+        else:
+            for col in range(m):
+                op.apply(P(work, col * n), 1, P(work, (col + 1) * n))
+                _lib.check(L.ttx_gmres_arnoldi_f64(s, n, m, col, P(work), P(st)),
+                           "ttx_gmres_arnoldi_f64")
+            _lib.check(L.ttx_gmres_finish_f64(s, n, m, P(work), P(st)), "ttx_gmres_finish_f64")
+            op.apply(P(work, xoff), 1, P(work, axoff))
+            _lib.check(L.ttx_gmres_check_f64(s, n, m, P(b), P(work), P(st)), "ttx_gmres_check_f64")
+        cycles += 1
+    if chunked:     # the matvecs queued after an inner stop returned at once
+        inner, done_cycles = st[13].item(), st[11].item()
+        D.add_flops((inner + done_cycles) * op.flops_per_apply(1))
+    return work[xoff:xoff + n].clone()
+
+
 def solve_local_system(op, b_loc: torch.Tensor, *, iterative_solver, local_rtol, warm_start=None,
                        explicit_fallback=None, stats: Optional[_SweepStats] = None,
                        spd_first: bool = False):
@@ -399,9 +444,6 @@ def solve_local_system(op, b_loc: torch.Tensor, *, iterative_solver, local_rtol,
     ``explicit_fallback`` true a solve that misses its target after the
     restarts is redone that way (``common.py:794-809``).
     """
-    if iterative_solver is LocalSolver.GMRES:
-        raise NotImplementedError("the GMRES local solver is not part of this build; "
-                                  "pass symmetric=True (MINRES) or local_solver='direct'")
     n = op.rows
     tgt = D.norm(b_loc)
     target = local_rtol * float(tgt.item())
@@ -410,7 +452,10 @@ def solve_local_system(op, b_loc: torch.Tensor, *, iterative_solver, local_rtol,
     res = D.empty(1)
     au = D.empty(n)
     for _ in range(_MAX_LOCAL_RESTARTS):
-        u = minres_device(op, b_loc, u, rt)
+        if iterative_solver is LocalSolver.GMRES:
+            u = gmres_device(op, b_loc, u, rt)
+        else:
+            u = minres_device(op, b_loc, u, rt)
         op.apply(P(u), 1, P(au))
         D.diff_nrm2(n, P(au), P(b_loc), P(res))
         if float(res.item()) <= target:
