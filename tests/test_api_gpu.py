@@ -157,21 +157,27 @@ def _convection_diffusion_cores(n_bits, c):
     return P.round_cores(P.block_sum_cores(P.block_sum_cores(lap, S), St), 1e-14)
 
 
-def test_amen_nonsymmetric_gmres_matches_oracle():
+@pytest.mark.parametrize("name", SOLVERS)
+def test_nonsymmetric_gmres_matches_oracle(name):
     """symmetric=False with iterative local solves (GMRES, restart 20) on a
     convection-diffusion operator: the device and the oracle (scipy's gmres,
     as the reference calls it) both reach 1e-8, in sweep counts within two,
-    with solutions within the residual-implied distance of the dense solution."""
-    print("api amen nonsymmetric gmres", flush=True)
+    with solutions within the residual-implied distance of the dense solution.
+    ALS starts from a rank-8 guess (the solution's ranks fit it)."""
+    print(f"api {name} nonsymmetric gmres", flush=True)
     tx = _tx()
     L = 6
     A_c = _convection_diffusion_cores(L, 20.0)
     b_c = P.ones_cores([2] * L)
-    x0 = tx.rand([2] * L, ranks=2, seed=0)
-    kw = dict(max_rank=16, residual_tol=1e-8, max_sweeps=10, symmetric=False, seed=0,
-              direct_solve_size=4)
-    x, info = tx.amen(tx.TensorTrain(A_c), tx.TensorTrain(b_c), x0=x0, return_info=True, **kw)
-    xo, io = O.amen(O.Train(A_c), O.Train(b_c), O.Train(x0.to_numpy()), **kw)
+    kw = dict(residual_tol=1e-8, max_sweeps=10, symmetric=False, direct_solve_size=4)
+    if name == "amen":
+        kw.update(max_rank=16, seed=0)
+    elif name == "mals":
+        kw.update(max_rank=16)
+    x0 = tx.rand([2] * L, ranks=8 if name == "als" else 2, seed=0)
+    x, info = _solver(name)(tx.TensorTrain(A_c), tx.TensorTrain(b_c), x0=x0, return_info=True,
+                            **kw)
+    xo, io = getattr(O, name)(O.Train(A_c), O.Train(b_c), O.Train(x0.to_numpy()), **kw)
     Ad = O.Train(A_c).dense_matrix()
     assert np.linalg.norm(Ad - Ad.T) > 1.0        # really nonsymmetric
     ref = np.linalg.solve(Ad, np.ones(2**L))
