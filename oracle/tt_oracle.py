@@ -48,7 +48,6 @@ Pinning (what checks each function, beyond retyping its formula):
 
 from __future__ import annotations
 
-import copy as _copy
 import numbers
 from dataclasses import dataclass, field
 from math import prod

@@ -21,7 +21,7 @@ import argparse
 import contextlib
 import json
 import time
-from typing import Callable, Dict, Optional, Tuple
+from typing import Callable, Dict, Tuple
 
 import torch
 

@@ -12,7 +12,6 @@ operands in place, so no permuted copy of a core or environment is made.
 
 from __future__ import annotations
 
-import math
 from typing import List, Optional, Tuple
 
 import torch

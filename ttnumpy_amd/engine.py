@@ -18,10 +18,9 @@ no cross-slot reductions), so a request gives the same bits alone and in a mix.
 from __future__ import annotations
 
 import ctypes
-import math
 import time
-from dataclasses import dataclass, field
-from typing import Dict, List, Optional, Sequence
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
 
 import numpy as np
 import torch

@@ -22,7 +22,7 @@ import math
 import time
 from dataclasses import dataclass, field
 from enum import Enum
-from typing import List, Optional, Union
+from typing import List, Optional
 
 import torch
 

@@ -15,7 +15,7 @@ import math
 import numbers
 import warnings
 from enum import Enum
-from typing import List, Optional, Sequence, Union
+from typing import List, Optional, Sequence
 
 import numpy as np
 import torch
