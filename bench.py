@@ -10,7 +10,7 @@ such solves served concurrently by the device-resident engine (one slot per
 request, continuous batching); inputs
 are resident in HBM before the clock starts.  ``value`` is solves per second
 over the timed steps; p50/p99 are per-request latencies (admission to
-result).  Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build
+result).  Prints one JSON line.  ``--gpus N`` with N > 1 exits: the benchmark
 runs on one GPU.
 """
 
@@ -385,7 +385,7 @@ def main():
                     help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
               "(how the path would shard is described in the design notes)", file=sys.stderr)
         sys.exit(2)
     import torch
