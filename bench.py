@@ -386,7 +386,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
-              "(how the path would shard is described in the design notes)", file=sys.stderr)
+              "(independent solves would run one engine per GPU, without exchange)", file=sys.stderr)
         sys.exit(2)
     import torch
     from ttnumpy_amd.engine import Engine
