@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes
 import time
+import warnings
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
@@ -408,11 +409,18 @@ class Engine:
             g = self._graphs.get(n)
             if g is None:
                 h = _vp()
-                _lib.check(self.lib.ttx_engine_graph_create(self.dev_slots.data_ptr(), self.nslots, n,
-                                                            ctypes.byref(h)), "ttx_engine_graph_create")
-                g = self._graphs[n] = h
-            _lib.check(self.lib.ttx_engine_graph_launch(D.stream(), g), "ttx_engine_graph_launch")
-            return
+                rc = self.lib.ttx_engine_graph_create(self.dev_slots.data_ptr(), self.nslots, n,
+                                                      ctypes.byref(h))
+                if rc != _lib.TTX_OK:
+                    # the same kernels in the same order, launched one by one
+                    warnings.warn(f"HIP graph capture of the engine ticks failed (status {rc}); "
+                                  "launching the ticks directly", RuntimeWarning)
+                    self.graphs = False
+                else:
+                    g = self._graphs[n] = h
+            if g is not None:
+                _lib.check(self.lib.ttx_engine_graph_launch(D.stream(), g), "ttx_engine_graph_launch")
+                return
         rc = self.lib.ttx_engine_tick(D.stream(), self.dev_slots.data_ptr(), self.nslots, n)
         _lib.check(rc, "ttx_engine_tick")
 
