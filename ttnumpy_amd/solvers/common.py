@@ -343,7 +343,11 @@ def _validate_solver_inputs(A: TensorTrain, b: TensorTrain, x0: TensorTrain) -> 
 
 
 # ---------------------------------------------------------------- local solves
-_POLL = 16  # MINRES steps queued between polls of the device stop flag
+# MINRES steps queued between polls of the device stop flag: _POLL_FIRST, then
+# doubling up to _POLL (most local solves stop within a few steps, and the
+# steps queued past the stop still cost their launches)
+_POLL_FIRST = 2
+_POLL = 32
 
 
 def minres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float,
@@ -368,11 +372,13 @@ def minres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float,
     maxiter = 5 * n
     done = 0
     chunked = hasattr(op, "minres_steps")
+    poll = _POLL_FIRST
     while True:
         istop, itn = st[:2].tolist()
         if istop != 0.0:
             break
-        chunk = min(_POLL, maxiter - done)
+        chunk = min(poll, maxiter - done)
+        poll = min(2 * poll, _POLL)
         if chunk <= 0:
             break
         if chunked:
