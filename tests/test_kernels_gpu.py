@@ -434,15 +434,22 @@ def test_gmres_matches_scipy(n, seed, with_x0):
     from ttnumpy_amd.solvers.common import gmres_device
     D = _dev()
     rng = np.random.default_rng(seed)
-    A = np.eye(n) * 4.0 + rng.standard_normal((n, n)) / np.sqrt(n)
+    # shift 1.5: 43-56 inner steps, so three restart cycles of 20 run
+    A = np.eye(n) * 1.5 + rng.standard_normal((n, n)) / np.sqrt(n)
     b = rng.standard_normal(n)
     x0 = rng.standard_normal(n) * 0.1 if with_x0 else None
     rtol = 1e-10
     ref, info = spla.gmres(A, b, x0=x0, rtol=rtol, atol=0.0)
     assert info == 0
+    steps = []
+    spla.gmres(A, b, x0=x0, rtol=rtol, atol=0.0, callback=lambda pr: steps.append(pr),
+               callback_type="pr_norm")
     op = D.ExplicitOp(_t(A.copy()), n)
-    x = _h(gmres_device(op, _t(b), _t(x0) if x0 is not None else None, rtol))
+    st = []
+    x = _h(gmres_device(op, _t(b), _t(x0) if x0 is not None else None, rtol, state_out=st))
     assert np.linalg.norm(A @ x - b) <= rtol * np.linalg.norm(b) * 1.0001
+    # the same iteration: the same number of inner (Arnoldi) steps as scipy's
+    assert abs(int(st[0][13]) - len(steps)) <= 1, (int(st[0][13]), len(steps))
     cond = np.linalg.cond(A)
     assert np.linalg.norm(x - ref) <= 2 * cond * rtol * np.linalg.norm(ref)
 

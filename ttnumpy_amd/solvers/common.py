@@ -398,7 +398,8 @@ def minres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float,
 _GMRES_RESTART = 20   # scipy.sparse.linalg.gmres's default restart length
 
 
-def gmres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float) -> torch.Tensor:
+def gmres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float,
+                 state_out: Optional[list] = None) -> torch.Tensor:
     """``scipy.sparse.linalg.gmres(op, b, x0=x0, rtol=rtol, atol=0.0)`` on the device.
 
     The Arnoldi steps, rotations, cycle updates and stopping tests run in the
@@ -437,6 +438,8 @@ def gmres_device(op, b: torch.Tensor, x0: Optional[torch.Tensor], rtol: float) -
     if chunked:     # the matvecs queued after an inner stop returned at once
         inner, done_cycles = st[13].item(), st[11].item()
         D.add_flops((inner + done_cycles) * op.flops_per_apply(1))
+    if state_out is not None:
+        state_out.append(st.cpu().numpy())
     return work[xoff:xoff + n].clone()
 
 
