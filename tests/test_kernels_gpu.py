@@ -504,3 +504,31 @@ def test_gmres_local_cycle_matches_explicit_operator():
     for x in (x_ex, x_lo):
         assert np.linalg.norm(Mh @ x - b) <= 1.01 * rtol * np.linalg.norm(b)
     assert np.linalg.norm(x_lo - x_ex) <= 2 * np.linalg.cond(Mh) * rtol * np.linalg.norm(x_ex)
+
+
+@pytest.mark.parametrize("m,n,zero_cols", [(1500, 700, False), (9000, 128, False),
+                                            (4482, 737, False), (1200, 1200, False),
+                                            (1842, 265, False), (512, 1181, False),
+                                            (64, 7169, False), (2000, 200, True),
+                                            (300, 2000, True)])
+def test_qr_r_only_blocked_large(m, n, zero_cols):
+    """R without Q of large matrices (the blocked path across the chip, tall
+    and wide, with exactly zero columns for reflectors with tau = 0): R^T R
+    equals A^T A, R is upper triangular, and |diag R| equals LAPACK's."""
+    print(f"qr R-only blocked {m}x{n} zero_cols={zero_cols}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, n))
+    if zero_cols:
+        A[:, 40:75] = 0.0
+    k = min(m, n)
+    Ad = _t(A)
+    R = D.empty(k, n)
+    D.qr(m, n, D.P(Ad), n, 1, None, 1, n, D.P(R), n, 1)
+    Rh = _h(R)
+    assert np.allclose(np.tril(Rh, -1), 0.0)
+    scale = np.linalg.norm(A) ** 2
+    np.testing.assert_allclose(Rh.T @ Rh, A.T @ A, rtol=0, atol=1e-13 * scale)
+    ref = np.linalg.qr(A, mode="r")
+    np.testing.assert_allclose(np.abs(np.diag(Rh)), np.abs(np.diag(ref)), rtol=1e-10,
+                               atol=1e-12 * np.abs(np.diag(ref)).max())

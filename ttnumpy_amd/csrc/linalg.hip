@@ -388,8 +388,208 @@ __global__ __launch_bounds__(256) void svd_terms_kernel(int64_t p, int64_t q, in
 
 using namespace ttx;
 
+// ---------------------------------------------------------------------------
+// R (no Q) of a large matrix across the chip: LAPACK's blocked Householder
+// scheme (geqrf with larft/larfb) with kQrbNb-column panels.  One workgroup
+// factors a panel (columns in global memory, L2-resident) and forms V and the
+// compact-WY factor T; the trailing columns take C -= V (T^T (V^T C)) as three
+// MFMA GEMMs over the whole chip.
+// Work: W (m x n, column-major) | tau (k) | V (m x nb) | T (nb x nb) | Y (nb x n)
+// | Y2 (nb x n): qrb_work(m, n) doubles.
+// ---------------------------------------------------------------------------
+constexpr int64_t kQrbNb = 32;
+constexpr int kQrbThreads = 512;
+constexpr int64_t kQrbMinK = 64;
+constexpr int64_t kQrbMinElems = int64_t(1) << 17;
+
+static bool qrb_applies(int64_t m, int64_t n) {
+  const int64_t k = m < n ? m : n;
+  return k >= kQrbMinK && m * n >= kQrbMinElems;
+}
+static int64_t qrb_work(int64_t m, int64_t n) {
+  const int64_t k = m < n ? m : n;
+  return m * n + k + m * kQrbNb + kQrbNb * kQrbNb + 2 * kQrbNb * n;
+}
+
+// Sum over the wave of 32 values per lane (a[0..32)): recursive halving, so
+// lane l ends with the total of entry (l >> 1) & 31 (lanes l and l ^ 1 hold
+// the same bits).  Fixed order, so deterministic.
+__device__ __forceinline__ double wave_sum32(double (&a)[kQrbNb], int lane) {
+#pragma unroll
+  for (int s = 0; s < 5; ++s) {
+    const int half = 16 >> s, off = 32 >> s;
+    const bool hi = (lane & off) != 0;
+#pragma unroll
+    for (int j = 0; j < half; ++j) {
+      const double keep = hi ? a[half + j] : a[j];
+      const double send = hi ? a[j] : a[half + j];
+      a[j] = keep + __shfl_xor(send, off, kWave);
+    }
+  }
+  return a[0] + __shfl_xor(a[0], 1, kWave);
+}
+
+// One panel: columns j0 .. j0+jb-1 of W, rows j0 .. m-1 (mp rows).  Column c
+// is reduced by the reflector H = I - tau v v^T of LAPACK's dlarfg (v[c] = 1,
+// beta = -sign(alpha) ||x||); one pass over the rows then forms, for every
+// other panel column cc, the product v_c^T (column cc): the trailing columns'
+// (cc > c) give the update w = tau v^T P, the earlier reflectors' (cc < c)
+// give G[cc][c] = v_cc^T v_c, from which T's column c follows as in dlarft
+// (forward, columnwise): T[0:c, c] = -tau T[0:c, 0:c] G[0:c, c], T[c][c] = tau.
+// V (mp x jb, column-major, unit diagonal, zeros above) is written as the
+// reflectors form; R stays in W's upper triangle.
+__global__ __launch_bounds__(kQrbThreads) void qrb_panel_kernel(int64_t m, int64_t j0, int64_t jb,
+                                                               double* W, double* tau, double* V,
+                                                               double* T) {
+  __shared__ double red[kQrbThreads / 64];
+  __shared__ double part[kQrbThreads / 64][kQrbNb];
+  __shared__ double sG[kQrbNb][kQrbNb];
+  __shared__ double sT[kQrbNb][kQrbNb];
+  __shared__ double sW[kQrbNb];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nwave = kQrbThreads / 64;
+  const int64_t mp = m - j0;
+  double* P = W + j0 * m + j0;  // panel column c at P + c * m, rows 0 .. mp-1
+  double sq = 0.0;
+  for (int64_t i = 1 + tid; i < mp; i += kQrbThreads) sq += P[i] * P[i];
+  sq = block_sum(sq, red);
+  for (int64_t c = 0; c < jb; ++c) {
+    double* col = P + c * m;
+    const double alpha = col[c];
+    double t = 0.0, beta = alpha, scal = 1.0;
+    if (sq != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + sq), alpha);
+      scal = 1.0 / (alpha - beta);
+      t = (beta - alpha) / beta;
+    }
+    double* vc = V + c * mp;
+    if (tid < c) vc[tid] = 0.0;
+    double d[kQrbNb];
+#pragma unroll
+    for (int cc = 0; cc < kQrbNb; ++cc) d[cc] = 0.0;
+    // all 32 loads of a row are issued before any is used: columns past the
+    // panel read its last column (the partial is never used), column c reads
+    // itself (likewise)
+    for (int64_t i = c + tid; i < mp; i += kQrbThreads) {
+      double x[kQrbNb];
+#pragma unroll
+      for (int cc = 0; cc < kQrbNb; ++cc) {
+        const int64_t q = cc < jb ? cc : jb - 1;
+        x[cc] = cc < c ? V[q * mp + i] : P[q * m + i];
+      }
+      const double v = i == c ? 1.0 : col[i] * scal;
+      vc[i] = v;
+#pragma unroll
+      for (int cc = 0; cc < kQrbNb; ++cc) d[cc] += v * x[cc];
+    }
+    const double tot = wave_sum32(d, lane);
+    if ((lane & 1) == 0) part[wave][(lane >> 1) & 31] = tot;
+    __syncthreads();
+    if (tid < jb) {
+      double s = 0.0;
+      for (int w = 0; w < nwave; ++w) s += part[w][tid];
+      if (tid < c) sG[tid][c] = s;
+      sW[tid] = tid > c ? t * s : 0.0;
+    }
+    __syncthreads();
+    if (tid < jb) {
+      double v = 0.0;
+      if (tid < c) {
+        double acc = 0.0;
+        for (int64_t l = tid; l < c; ++l) acc += sT[tid][l] * sG[l][c];
+        v = -t * acc;
+      } else if (tid == c) {
+        v = t;
+      }
+      sT[tid][c] = v;
+    }
+    if (tid == 0) {
+      tau[j0 + c] = t;
+      col[c] = beta;
+    }
+    double w[kQrbNb];
+#pragma unroll
+    for (int cc = 0; cc < kQrbNb; ++cc) w[cc] = sW[cc];
+    double sqn = 0.0;
+    if (t != 0.0) {
+      for (int64_t i = c + tid; i < mp; i += kQrbThreads) {
+        double x[kQrbNb];
+#pragma unroll
+        for (int cc = 0; cc < kQrbNb; ++cc) x[cc] = P[(cc < jb ? cc : jb - 1) * m + i];
+        const double v = vc[i];
+#pragma unroll
+        for (int cc = 0; cc < kQrbNb; ++cc) {
+          if (cc > c && cc < jb) {
+            const double nv = x[cc] - w[cc] * v;
+            P[cc * m + i] = nv;
+            if (cc == c + 1 && i > c + 1) sqn += nv * nv;
+          }
+        }
+      }
+    } else if (c + 1 < jb) {
+      const double* nx = P + (c + 1) * m;
+      for (int64_t i = c + 2 + tid; i < mp; i += kQrbThreads) sqn += nx[i] * nx[i];
+    }
+    sq = block_sum(sqn, red);  // also orders this column's writes before the next
+  }
+  for (int64_t e = tid; e < jb * jb; e += kQrbThreads) {
+    const int64_t i = q32(e, jb), c = e - i * jb;
+    T[e] = sT[i][c];
+  }
+}
+
+__global__ __launch_bounds__(256) void qrb_extract_r_kernel(int64_t m, int64_t n, int64_t k,
+                                                           const double* W, double* R,
+                                                           int64_t r_rs, int64_t r_cs) {
+  const int64_t total = k * n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = q32(e, n), j = e - i * n;
+    R[i * r_rs + j * r_cs] = i <= j ? W[j * m + i] : 0.0;
+  }
+}
+
+static int qr_r_blocked(ttx_stream_t stream, int64_t m, int64_t n, const double* A, int64_t a_rs,
+                        int64_t a_cs, double* R, int64_t r_rs, int64_t r_cs, double* work) {
+  const int64_t k = m < n ? m : n;
+  double* W = work;
+  double* tau = W + m * n;
+  double* V = tau + k;
+  double* T = V + m * kQrbNb;
+  double* Y = T + kQrbNb * kQrbNb;
+  double* Y2 = Y + kQrbNb * n;
+  hipStream_t st = (hipStream_t)stream;
+  {
+    const int64_t dims[2] = {n, m}, ss[2] = {a_cs, a_rs}, ds[2] = {m, 1};
+    const int rc = ttx_copy_f64(stream, 2, dims, A, ss, W, ds, 1.0);
+    if (rc != TTX_OK) return rc;
+  }
+  for (int64_t j0 = 0; j0 < k; j0 += kQrbNb) {
+    const int64_t jb = (k - j0) < kQrbNb ? (k - j0) : kQrbNb;
+    const int64_t mp = m - j0, nt = n - j0 - jb;
+    hipLaunchKernelGGL(qrb_panel_kernel, dim3(1), dim3(kQrbThreads), 0, st, m, j0, jb, W, tau, V, T);
+    int rc = launch_status();
+    if (rc != TTX_OK) return rc;
+    if (nt <= 0) continue;
+    double* C = W + (j0 + jb) * m + j0;  // mp x nt, column-major (ld m)
+    // Y = V^T C (jb x nt, row-major)
+    rc = ttx_gemm_f64(stream, 1, jb, nt, mp, 1.0, V, mp, 1, 0, C, 1, m, 0, 0.0, Y, nt, 1, 0);
+    if (rc != TTX_OK) return rc;
+    // Y2 = T^T Y
+    rc = ttx_gemm_f64(stream, 1, jb, nt, jb, 1.0, T, 1, jb, 0, Y, nt, 1, 0, 0.0, Y2, nt, 1, 0);
+    if (rc != TTX_OK) return rc;
+    // C -= V Y2
+    rc = ttx_gemm_f64(stream, 1, mp, nt, jb, -1.0, V, 1, mp, 0, Y2, nt, 1, 0, 1.0, C, 1, m, 0);
+    if (rc != TTX_OK) return rc;
+  }
+  hipLaunchKernelGGL(qrb_extract_r_kernel, dim3((unsigned)((k * n + 255) / 256 < 4096 ? (k * n + 255) / 256 : 4096)),
+                     dim3(256), 0, st, m, n, k, W, R, r_rs, r_cs);
+  return launch_status();
+}
+
 extern "C" int64_t ttx_qr_work_size(int64_t m, int64_t n) {
-  const int64_t h = hh_work(m, n);
+  int64_t h = hh_work(m, n);
+  if (qrb_applies(m, n) && qrb_work(m, n) > h) h = qrb_work(m, n);
   if (!cqr_applies(m, n)) return h;
   const int64_t c = cqr_work(m, n);
   return (h > c ? h : c) + 2;  // + the flag
@@ -466,6 +666,9 @@ extern "C" int ttx_qr_f64(ttx_stream_t stream, int64_t m, int64_t n, const doubl
   }
   if (Q != nullptr && cqr_applies(m, n))
     return cqr_factor(stream, m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work);
+  // R only of a large matrix: blocked across the chip
+  if (Q == nullptr && qrb_applies(m, n))
+    return qr_r_blocked(stream, m, n, A, a_rs, a_cs, R, r_rs, r_cs, work);
   QrArgs p{m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work, nullptr};
   const size_t bytes = (size_t)hh_work(m, n) * sizeof(double);
   const int64_t k = m < n ? m : n;
