@@ -532,3 +532,46 @@ def test_qr_r_only_blocked_large(m, n, zero_cols):
     ref = np.linalg.qr(A, mode="r")
     np.testing.assert_allclose(np.abs(np.diag(Rh)), np.abs(np.diag(ref)), rtol=1e-10,
                                atol=1e-12 * np.abs(np.diag(ref)).max())
+
+
+@pytest.mark.parametrize("m,n,case", [(1500, 300, "graded"), (300, 1500, "graded"),
+                                      (800, 200, "deficient"), (200, 800, "deficient"),
+                                      (600, 150, "gauss")])
+def test_svd_large_preconditioned_jacobi(m, n, case):
+    """The large one-workgroup SVD (columns taken by decreasing norm, Jacobi on
+    the columns of R^T): singular values to LAPACK's, the reconstruction, and
+    orthonormal singular vectors for every singular value above the rounding
+    level; exactly zero columns and repeated columns (ties in the norm order,
+    singular values that are zero) included."""
+    print(f"svd preconditioned {m}x{n} {case}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(m + 3 * n)
+    k = min(m, n)
+    if case == "graded":
+        Ux = np.linalg.qr(rng.standard_normal((m, k)))[0]
+        Vx = np.linalg.qr(rng.standard_normal((n, k)))[0]
+        A = (Ux * np.logspace(0, -12, k)) @ Vx.T
+    else:
+        A = rng.standard_normal((m, n))
+        if case == "deficient":
+            if m >= n:
+                A[:, 20:40] = 0.0
+                A[:, 100:110] = A[:, 0:10]
+            else:
+                A[20:40, :] = 0.0
+                A[100:110, :] = A[0:10, :]
+    info = torch.zeros(2, dtype=torch.int32, device=D.device())
+    U, S, Vt = D.svd(m, n, D.P(_t(A)), n, 1, info)
+    assert int(info[0].item()) == 0
+    Uh, Sh, Vh = _h(U), _h(S), _h(Vt)
+    Sref = np.linalg.svd(A, compute_uv=False)
+    np.testing.assert_allclose(Sh, Sref, rtol=1e-10, atol=1e-13 * Sref[0])
+    assert np.all(np.diff(Sh) <= 0)
+    np.testing.assert_allclose((Uh * Sh) @ Vh, A, atol=1e-13 * Sref[0] * max(m, n))
+    live = Sh > 1e-10 * Sh[0]
+    kl = int(live.sum())
+    assert kl == int((Sref > 1e-10 * Sref[0]).sum())
+    if case == "deficient":
+        assert kl == k - 30
+    np.testing.assert_allclose(Uh[:, live].T @ Uh[:, live], np.eye(kl), atol=1e-12 * max(m, n))
+    np.testing.assert_allclose(Vh[live] @ Vh[live].T, np.eye(kl), atol=1e-12 * max(m, n))

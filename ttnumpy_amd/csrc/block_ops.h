@@ -562,7 +562,13 @@ __device__ __forceinline__ bool jacobi_sweeps(double* G, double* V, int Q, int* 
 // sides); it may be LDS.  REGS: see jacobi_pair.  Without ACCV the rotations
 // are not accumulated: V (Vt of a tall, U of a wide input) is left unwritten
 // for the caller to form.  Ends synced.
-template <bool REGS = true, bool ACCV = true>
+// PRE (with ACCV, on tall factors P > Q): Jacobi preconditioned as in Drmac
+// and Veselic's one-sided Jacobi SVD: the columns of X are taken in order of
+// decreasing norm (X Pi = Qx R), and the rotations act on the columns of R^T,
+// which for graded matrices are far closer to orthogonal than R's (about a
+// third of the sweeps).  With R^T V = Y (orthogonal columns, norms sigma):
+// X = (Qx V) Sigma (Pi Y Sigma^-1)^T, so U_X = Qx V and V_X = Pi Y Sigma^-1.
+template <bool REGS = true, bool ACCV = true, bool PRE = false>
 __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double* red,
                                           double* lds = nullptr, int64_t lds_cap = 0) {
   __shared__ int s_rot;
@@ -587,11 +593,36 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   double* sig = V + Q * Q;        // Q
   double* ordd = sig + Q;         // Q       (sort permutation, stored as doubles)
 
+  const bool pre = PRE && ACCV && P > Q;
   if (tid == 0) s_bad = 0;
+  if (pre) {
+    // squared column norms (in ordd), then the source column of each position
+    // (in sig): decreasing norm, ties by index; a non-finite norm sorts last
+    for (int64_t j = wave; j < Q; j += nwave) {
+      double s = 0.0;
+      for (int64_t r = lane; r < P; r += 64) {
+        const double v = p.A[r * x_rs + j * x_cs];
+        s += v * v;
+      }
+      s = wave_sum(s);
+      if (lane == 0) ordd[j] = isfinite(s) ? s : -1.0;
+    }
+    __syncthreads();
+    for (int64_t j = tid; j < Q; j += blockDim.x) {
+      const double sj = ordd[j];
+      int64_t pos = 0;
+      for (int64_t l = 0; l < Q; ++l) {
+        const double sl = ordd[l];
+        if (sl > sj || (sl == sj && l < j)) ++pos;
+      }
+      sig[pos] = (double)j;
+    }
+  }
   __syncthreads();
   for (int64_t e = tid; e < P * Q; e += blockDim.x) {
     const int64_t j = q32(e, P), i = e - j * P;
-    const double v = p.A[i * x_rs + j * x_cs];
+    const int64_t src = pre ? (int64_t)sig[j] : j;
+    const double v = p.A[i * x_rs + src * x_cs];
     if (!isfinite(v)) s_bad = 1;
     Wq[e] = v;
   }
@@ -604,7 +635,14 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   if (use_qr) block_householder_qr(P, Q, Wq, tau, Qx, red, true, lds, lds_cap);
   // the QR is done with the LDS: the Jacobi matrix moves there if it fits
   double* G = (lds != nullptr && Q * Q <= lds_cap) ? lds : Gw;
-  if (use_qr) {
+  if (pre) {
+    // G = R^T; the permutation moves to tau (free once Qx is formed)
+    for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
+      const int64_t c = q32(e, Q), i = e - c * Q;
+      G[e] = (c <= i) ? Wq[c + i * P] : 0.0;
+    }
+    for (int64_t j = tid; j < Q; j += blockDim.x) tau[j] = sig[j];
+  } else if (use_qr) {
     for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
       const int64_t c = q32(e, Q), i = e - c * Q;
       G[e] = (i <= c) ? Wq[i + c * P] : 0.0;
@@ -642,13 +680,16 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
     ordd[pos] = (double)j;
   }
   __syncthreads();
-  // Gs (in Wq) column o = G[:, ord[o]] / sig[ord[o]]
+  // Gs (in Wq) column o = G[:, ord[o]] / sig[ord[o]]  (pre: V[:, ord[o]])
   double* Gs = Wq;
   for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
     const int64_t o = q32(e, Q), r = e - o * Q;
     const int64_t src = (int64_t)ordd[o];
     const double s = sig[src];
-    Gs[e] = s > 0.0 ? G[r + src * Q] / s : 0.0;
+    if (pre)
+      Gs[e] = V[r + src * Q];
+    else
+      Gs[e] = s > 0.0 ? G[r + src * Q] / s : 0.0;
   }
   __syncthreads();
   // Ux (P x Q) = (use_qr ? Qx : I) Gs;  tall: U_A = Ux, Vt_A = V^T;  wide: U_A = V, Vt_A = Ux^T
@@ -664,9 +705,18 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   }
   if (ACCV)
     for (int64_t e = tid; e < Q * Q; e += blockDim.x) {
-      const int64_t o = q32(e, Q), r = e - o * Q;
+      const int64_t o = q32(e, Q), j = e - o * Q;
       const int64_t src = (int64_t)ordd[o];
-      const double v = V[r + src * Q];
+      double v;
+      int64_t r = j;
+      if (pre) {
+        // row Pi[j] of V_X = Y[j, ord[o]] / sigma
+        const double s = sig[src];
+        v = s > 0.0 ? G[j + src * Q] / s : 0.0;
+        r = (int64_t)tau[j];
+      } else {
+        v = V[j + src * Q];
+      }
       if (!wide)
         p.Vt[o * p.v_rs + r * p.v_cs] = v;
       else

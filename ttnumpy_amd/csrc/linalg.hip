@@ -262,7 +262,7 @@ __global__ __launch_bounds__(1024) void svd_kernel(SvdArgs p) {
   extern __shared__ double smem[];
   __shared__ double red[16];
   if (p.skip != nullptr && *p.skip == 0) return;
-  block_svd<true, ACCV>(p, IN_LDS ? smem : p.work, red, IN_LDS ? nullptr : smem,
+  block_svd<true, ACCV, !IN_LDS>(p, IN_LDS ? smem : p.work, red, IN_LDS ? nullptr : smem,
                         IN_LDS ? 0 : (int64_t)(kQrLds / sizeof(double)));
 }
 
