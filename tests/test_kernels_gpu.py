@@ -536,10 +536,13 @@ def test_qr_r_only_blocked_large(m, n, zero_cols):
 
 @pytest.mark.parametrize("m,n,case", [(1500, 300, "graded"), (300, 1500, "graded"),
                                       (800, 200, "deficient"), (200, 800, "deficient"),
-                                      (600, 150, "gauss")])
+                                      (600, 150, "gauss"), (700, 171, "gauss"),
+                                      (171, 700, "deficient"), (255, 255, "graded")])
 def test_svd_large_preconditioned_jacobi(m, n, case):
-    """The large one-workgroup SVD (columns taken by decreasing norm, Jacobi on
-    the columns of R^T): singular values to LAPACK's, the reconstruction, and
+    """The large SVDs (columns taken by decreasing norm, Jacobi on the columns
+    of R^T; on one workgroup below 160 columns, one workgroup per column pair
+    and one launch per round from 160 on, odd counts with a dummy column):
+    singular values to LAPACK's, the reconstruction, and
     orthonormal singular vectors for every singular value above the rounding
     level; exactly zero columns and repeated columns (ties in the norm order,
     singular values that are zero) included."""
@@ -575,3 +578,14 @@ def test_svd_large_preconditioned_jacobi(m, n, case):
         assert kl == k - 30
     np.testing.assert_allclose(Uh[:, live].T @ Uh[:, live], np.eye(kl), atol=1e-12 * max(m, n))
     np.testing.assert_allclose(Vh[live] @ Vh[live].T, np.eye(kl), atol=1e-12 * max(m, n))
+
+
+def test_svd_large_nonfinite_reports():
+    """A NaN in a factor of the multi-workgroup Jacobi path is reported as on
+    the small path (status 2), without running the sweeps."""
+    D = _dev()
+    A = np.random.default_rng(5).standard_normal((400, 200))
+    A[17, 123] = np.nan
+    info = torch.zeros(2, dtype=torch.int32, device=D.device())
+    D.svd(400, 200, D.P(_t(A)), 200, 1, info)
+    assert int(info[0].item()) == 2

@@ -691,8 +691,266 @@ static int64_t svd_block_work(int64_t P, int64_t Q) { return 2 * P * Q + 2 * Q *
 // V_r from R^{-1} is accepted when max |V_r^T V_r - I| <= kSvdCert * Q * eps
 constexpr double kSvdCert = 16.0;
 
+// ---------------------------------------------------------------------------
+// SVD of factors whose Q x Q Jacobi matrix does not fit LDS (Q >= kSvdmMinQ):
+// the one-sided Jacobi runs across the chip, one launch per round of the
+// round-robin schedule and one workgroup per column pair, instead of on one
+// workgroup whose rounds wait on global memory.  Preconditioned as the
+// one-workgroup kernel's large case (Drmac-Veselic): X's columns by
+// decreasing norm (X Pi = Qx R, Householder with Q), the rotations on the
+// columns of R^T (R's rows, which row-major R already stores as columns),
+// R^T V = Y, then U_X = Qx V and V_X = Pi Y Sigma^-1.  The host reads the
+// rotation count once per sweep (a stream synchronization per sweep), and
+// stops at the first sweep that rotates nothing or after kSvdmMaxSweeps.
+// Work: Xp (P x Q) | Qx (P x Q) | R (Q x Q) | V (Q x Q) | Vs (Q x Q) | sig (Q)
+// | the QR's work | ints: perm (Q), ord (Q), flags ((QQ-1) QQ/2), count, bad.
+// ---------------------------------------------------------------------------
+constexpr int64_t kSvdmMinQ = 160;
+constexpr int kSvdmMaxSweeps = 80;
+constexpr int kSvdmThreads = 256;
+
+static int64_t svdm_work(int64_t P, int64_t Q) {
+  const int64_t QQ = Q + (Q & 1);
+  const int64_t ints = 2 * Q + (QQ - 1) * (QQ / 2) + 2;
+  return 2 * P * Q + 3 * Q * Q + Q + ttx_qr_work_size(P, Q) + (ints + 1) / 2 + 1;
+}
+
+// squared column norms of X (P x Q, strides x_rs, x_cs), the source column of
+// each position by decreasing norm (ties by index; a non-finite norm sorts
+// last and sets *bad)
+__global__ __launch_bounds__(1024) void svdm_presort_kernel(int64_t P, int64_t Q, const double* A,
+                                                           int64_t x_rs, int64_t x_cs,
+                                                           double* keys, int* perm, int* bad) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  if (tid == 0) *bad = 0;
+  __syncthreads();
+  for (int64_t j = wave; j < Q; j += nwave) {
+    double s = 0.0;
+    for (int64_t r = lane; r < P; r += 64) {
+      const double v = A[r * x_rs + j * x_cs];
+      s += v * v;
+    }
+    s = wave_sum(s);
+    if (lane == 0) {
+      if (!isfinite(s)) *bad = 1;
+      keys[j] = isfinite(s) ? s : -1.0;
+    }
+  }
+  __syncthreads();
+  for (int64_t j = tid; j < Q; j += blockDim.x) {
+    const double sj = keys[j];
+    int64_t pos = 0;
+    for (int64_t l = 0; l < Q; ++l) {
+      const double sl = keys[l];
+      if (sl > sj || (sl == sj && l < j)) ++pos;
+    }
+    perm[pos] = (int)j;
+  }
+}
+
+// Xp (P x Q row-major) = X[:, perm]; V = I (Q x Q)
+__global__ void svdm_gather_kernel(int64_t P, int64_t Q, const double* A, int64_t x_rs,
+                                   int64_t x_cs, const int* perm, double* Xp, double* V) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < P * Q; e += stride) {
+    const int64_t i = q32(e, Q), c = e - i * Q;
+    Xp[e] = A[i * x_rs + (int64_t)perm[c] * x_cs];
+  }
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < Q * Q; e += stride)
+    V[e] = (e == q32(e, Q) * (Q + 1)) ? 1.0 : 0.0;
+}
+
+// One round: workgroup pk rotates its pair of columns of G and of V (both
+// column-major, ld Q) and records whether it rotated in flags[pk].
+__global__ __launch_bounds__(kSvdmThreads) void svdm_round_kernel(int64_t Q, int64_t QQ, int64_t round,
+                                                                  double* G, double* V, double tol,
+                                                                  int* flags) {
+  __shared__ double red[3][kSvdmThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t pk = blockIdx.x;
+  // round-robin: position 0 stays, the others rotate by one per round
+  int64_t i = 0, j;
+  if (pk != 0) {
+    i = pk - 1 + round;
+    if (i >= QQ - 1) i -= QQ - 1;
+    i += 1;
+  }
+  j = QQ - 2 - pk + round;
+  if (j >= QQ - 1) j -= QQ - 1;
+  j += 1;
+  if (i >= Q || j >= Q) {
+    if (tid == 0) flags[pk] = 0;
+    return;
+  }
+  if (i > j) { const int64_t t = i; i = j; j = t; }
+  double* gi = G + i * Q;
+  double* gj = G + j * Q;
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int64_t r = tid; r < Q; r += kSvdmThreads) {
+    const double x = gi[r], y = gj[r];
+    a += x * x;
+    b += y * y;
+    c += x * y;
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  if (lane == 0) {
+    red[0][wave] = a;
+    red[1][wave] = b;
+    red[2][wave] = c;
+  }
+  __syncthreads();
+  a = b = c = 0.0;
+  for (int w = 0; w < kSvdmThreads / 64; ++w) {
+    a += red[0][w];
+    b += red[1][w];
+    c += red[2][w];
+  }
+  double cs, sn;
+  const bool rot = jac_params(a, b, c, tol, cs, sn);
+  if (tid == 0) flags[pk] = rot ? 1 : 0;
+  if (!rot) return;
+  double* vi = V + i * Q;
+  double* vj = V + j * Q;
+  for (int64_t r = tid; r < Q; r += kSvdmThreads) {
+    const double x = gi[r], y = gj[r], u = vi[r], w = vj[r];
+    gi[r] = jac_rot_lo(cs, sn, x, y);
+    gj[r] = jac_rot_hi(cs, sn, x, y);
+    vi[r] = jac_rot_lo(cs, sn, u, w);
+    vj[r] = jac_rot_hi(cs, sn, u, w);
+  }
+}
+
+__global__ __launch_bounds__(1024) void svdm_count_kernel(int64_t n, const int* flags, int* count) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int64_t e = threadIdx.x; e < n; e += blockDim.x) s += (double)flags[e];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) *count = (int)s;
+}
+
+// sig[j] = ||G[:, j]||, ord = positions by decreasing sig (ties by index; a
+// NaN sorts last, so that ord is always a permutation)
+__global__ __launch_bounds__(1024) void svdm_sigma_kernel(int64_t Q, const double* G, double* sig,
+                                                         int* ord) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  for (int64_t j = wave; j < Q; j += nwave) {
+    const double* gj = G + j * Q;
+    double s = 0.0;
+    for (int64_t r = lane; r < Q; r += 64) s += gj[r] * gj[r];
+    s = wave_sum(s);
+    if (lane == 0) sig[j] = sqrt(s);
+  }
+  __syncthreads();
+  for (int64_t j = tid; j < Q; j += blockDim.x) {
+    const double sj = sig[j] == sig[j] ? sig[j] : -1.0;
+    int64_t pos = 0;
+    for (int64_t l = 0; l < Q; ++l) {
+      const double sl = sig[l] == sig[l] ? sig[l] : -1.0;
+      if (sl > sj || (sl == sj && l < j)) ++pos;
+    }
+    ord[pos] = (int)j;
+  }
+}
+
+// Vs (Q x Q row-major) = V[:, ord]; V_X[perm[j], o] = G[j, ord[o]] / sig into
+// out (row stride o_rs, column stride o_cs); S; the status
+__global__ void svdm_finish_kernel(int64_t Q, const double* G, const double* V, const double* sig,
+                                   const int* ord, const int* perm, double* Vs, double* out,
+                                   int64_t o_rs, int64_t o_cs, double* S, const int* bad,
+                                   int converged, int32_t* info) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < Q * Q; e += stride) {
+    const int64_t o = q32(e, Q), j = e - o * Q;
+    const int64_t src = ord[o];
+    const double s = sig[src];
+    Vs[j * Q + o] = V[j + src * Q];
+    out[(int64_t)perm[j] * o_rs + o * o_cs] = s > 0.0 ? G[j + src * Q] / s : 0.0;
+    if (j == 0) S[o] = s;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    info[0] = *bad ? TTX_ENONFINITE : (converged ? TTX_OK : TTX_ENOCONV);
+}
+
+__global__ void svdm_status_kernel(int32_t* info, int32_t code) { info[0] = code; }
+
+static int svd_multi_jacobi(ttx_stream_t stream, int64_t m, int64_t n, const double* A,
+                            int64_t a_rs, int64_t a_cs, double* U, int64_t u_rs, int64_t u_cs,
+                            double* S, double* Vt, int64_t v_rs, int64_t v_cs, double* work,
+                            int32_t* info) {
+  const bool wide = m < n;
+  const int64_t P = wide ? n : m, Q = wide ? m : n;
+  const int64_t x_rs = wide ? a_cs : a_rs, x_cs = wide ? a_rs : a_cs;
+  const int64_t QQ = Q + (Q & 1), npairs = QQ / 2;
+  double* Xp = work;
+  double* Qx = Xp + P * Q;
+  double* R = Qx + P * Q;
+  double* V = R + Q * Q;
+  double* Vs = V + Q * Q;
+  double* sig = Vs + Q * Q;
+  double* wq = sig + Q;
+  int* perm = (int*)(wq + ttx_qr_work_size(P, Q));
+  int* ord = perm + Q;
+  int* flags = ord + Q;
+  int* count = flags + (QQ - 1) * npairs;
+  int* bad = count + 1;
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  hipLaunchKernelGGL(svdm_presort_kernel, dim3(1), dim3(1024), 0, st, P, Q, A, x_rs, x_cs, sig,
+                     perm, bad);
+  if ((rc = launch_status()) != TTX_OK) return rc;
+  {
+    int h = 0;
+    if (hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return TTX_EHIP;
+    if (h != 0) {  // NaN/inf input: reported, nothing factored
+      hipLaunchKernelGGL(svdm_status_kernel, dim3(1), dim3(1), 0, st, info, (int32_t)TTX_ENONFINITE);
+      return launch_status();
+    }
+  }
+  const unsigned gb = (unsigned)((P * Q + 255) / 256 < 2048 ? (P * Q + 255) / 256 : 2048);
+  hipLaunchKernelGGL(svdm_gather_kernel, dim3(gb), dim3(256), 0, st, P, Q, A, x_rs, x_cs, perm, Xp,
+                     V);
+  if ((rc = launch_status()) != TTX_OK) return rc;
+  if ((rc = ttx_qr_f64(stream, P, Q, Xp, Q, 1, Qx, Q, 1, R, Q, 1, wq)) != TTX_OK) return rc;
+  // the Jacobi matrix is R^T column-major: column c is row c of row-major R
+  const double tol = (double)Q * kEps;
+  bool converged = false;
+  for (int sweep = 0; sweep < kSvdmMaxSweeps && !converged; ++sweep) {
+    for (int64_t round = 0; round < QQ - 1; ++round) {
+      hipLaunchKernelGGL(svdm_round_kernel, dim3((unsigned)npairs), dim3(kSvdmThreads), 0, st, Q,
+                         QQ, round, R, V, tol, flags + round * npairs);
+      if ((rc = launch_status()) != TTX_OK) return rc;
+    }
+    hipLaunchKernelGGL(svdm_count_kernel, dim3(1), dim3(1024), 0, st, (QQ - 1) * npairs, flags,
+                       count);
+    if ((rc = launch_status()) != TTX_OK) return rc;
+    int h = 0;
+    if (hipMemcpyAsync(&h, count, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return TTX_EHIP;
+    converged = h == 0;
+  }
+  hipLaunchKernelGGL(svdm_sigma_kernel, dim3(1), dim3(1024), 0, st, Q, R, sig, ord);
+  if ((rc = launch_status()) != TTX_OK) return rc;
+  // tall: U = U_X, Vt[o][perm j] = V_X;  wide: U[perm j][o] = V_X, Vt = U_X^T
+  double* out = wide ? U : Vt;
+  const int64_t o_rs = wide ? u_rs : v_cs, o_cs = wide ? u_cs : v_rs;
+  const unsigned fb = (unsigned)((Q * Q + 255) / 256 < 2048 ? (Q * Q + 255) / 256 : 2048);
+  hipLaunchKernelGGL(svdm_finish_kernel, dim3(fb), dim3(256), 0, st, Q, R, V, sig, ord, perm, Vs,
+                     out, o_rs, o_cs, S, bad, converged ? 1 : 0, info);
+  if ((rc = launch_status()) != TTX_OK) return rc;
+  // U_X = Qx Vs (P x Q)
+  if (!wide)
+    return ttx_gemm_f64(stream, 1, P, Q, Q, 1.0, Qx, Q, 1, 0, Vs, Q, 1, 0, 0.0, U, u_rs, u_cs, 0);
+  return ttx_gemm_f64(stream, 1, P, Q, Q, 1.0, Qx, Q, 1, 0, Vs, Q, 1, 0, 0.0, Vt, v_cs, v_rs, 0);
+}
+
 extern "C" int64_t ttx_svd_work_size(int64_t m, int64_t n) {
   const int64_t P = m > n ? m : n, Q = m < n ? m : n;
+  if (Q >= kSvdmMinQ) return svdm_work(P, Q);
   if (!cqr_applies(P, Q)) return svd_block_work(P, Q);
   // Qx, R, U_r, Vt_r, two Q x Q scratch, the QR's work, the square SVD's work, a flag
   return P * Q + 5 * Q * Q + ttx_qr_work_size(P, Q) + svd_block_work(Q, Q) + 2;
@@ -709,6 +967,9 @@ extern "C" int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n, const doub
     attr = true;
   }
   const int64_t P = m > n ? m : n, Qd = m < n ? m : n;
+  if (Qd >= kSvdmMinQ)
+    return svd_multi_jacobi(stream, m, n, A, a_rs, a_cs, U, u_rs, u_cs, S, Vt, v_rs, v_cs, work,
+                            info);
   if (cqr_applies(P, Qd)) {
     // X = A (tall) or A^T (wide), P x Qd: X = Qx R by CholeskyQR2 across the
     // chip, then R = U_r S Vt_r, then tall: U = Qx U_r, Vt = Vt_r; wide:
