@@ -536,12 +536,12 @@ def test_qr_r_only_blocked_large(m, n, zero_cols):
 
 @pytest.mark.parametrize("m,n,case", [(1500, 300, "graded"), (300, 1500, "graded"),
                                       (800, 200, "deficient"), (200, 800, "deficient"),
-                                      (600, 150, "gauss"), (700, 171, "gauss"),
+                                      (600, 150, "gauss"), (500, 135, "graded"), (700, 171, "gauss"),
                                       (171, 700, "deficient"), (255, 255, "graded")])
 def test_svd_large_preconditioned_jacobi(m, n, case):
     """The large SVDs (columns taken by decreasing norm, Jacobi on the columns
-    of R^T; on one workgroup below 160 columns, one workgroup per column pair
-    and one launch per round from 160 on, odd counts with a dummy column):
+    of R^T; on one workgroup below 139 columns, one workgroup per column pair
+    and one launch per round from 139 on, odd counts with a dummy column):
     singular values to LAPACK's, the reconstruction, and
     orthonormal singular vectors for every singular value above the rounding
     level; exactly zero columns and repeated columns (ties in the norm order,
@@ -589,3 +589,36 @@ def test_svd_large_nonfinite_reports():
     info = torch.zeros(2, dtype=torch.int32, device=D.device())
     D.svd(400, 200, D.P(_t(A)), 200, 1, info)
     assert int(info[0].item()) == 2
+
+
+@pytest.mark.parametrize("m,n,q_col_major,zero_cols", [(2000, 300, False, False),
+                                                       (300, 2000, False, False),
+                                                       (1000, 1000, False, False),
+                                                       (700, 171, True, False),
+                                                       (1500, 260, False, True)])
+def test_qr_blocked_with_q(m, n, q_col_major, zero_cols):
+    """Q and R of large matrices on the blocked path (Q accumulated from the
+    kept panels in reverse, as dorgqr): Q R = A, Q^T Q = I, R upper triangular,
+    |diag R| equal to LAPACK's; Q also written column-major (strided output)."""
+    print(f"qr blocked with Q {m}x{n} col_major={q_col_major} zero_cols={zero_cols}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(7 * m + n)
+    A = rng.standard_normal((m, n))
+    if zero_cols:
+        A[:, 50:80] = 0.0
+    k = min(m, n)
+    Q = D.empty(m, k)
+    R = D.empty(k, n)
+    if q_col_major:
+        D.qr(m, n, D.P(_t(A)), n, 1, D.P(Q), 1, m, D.P(R), n, 1)
+        Qh = _h(Q).reshape(-1)[:m * k].reshape(k, m).T
+    else:
+        D.qr(m, n, D.P(_t(A)), n, 1, D.P(Q), k, 1, D.P(R), n, 1)
+        Qh = _h(Q)
+    Rh = _h(R)
+    np.testing.assert_allclose(Qh @ Rh, A, atol=1e-13 * np.abs(A).max() * max(m, n))
+    np.testing.assert_allclose(Qh.T @ Qh, np.eye(k), atol=1e-13 * max(m, n))
+    assert np.allclose(np.tril(Rh, -1), 0.0)
+    ref = np.linalg.qr(A, mode="r")
+    np.testing.assert_allclose(np.abs(np.diag(Rh)), np.abs(np.diag(ref)), rtol=1e-10,
+                               atol=1e-12 * np.abs(np.diag(ref)).max())

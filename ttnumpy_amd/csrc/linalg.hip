@@ -389,13 +389,15 @@ __global__ __launch_bounds__(256) void svd_terms_kernel(int64_t p, int64_t q, in
 using namespace ttx;
 
 // ---------------------------------------------------------------------------
-// R (no Q) of a large matrix across the chip: LAPACK's blocked Householder
-// scheme (geqrf with larft/larfb) with kQrbNb-column panels.  One workgroup
-// factors a panel (columns in global memory, L2-resident) and forms V and the
+// QR of a large matrix across the chip: LAPACK's blocked Householder scheme
+// (geqrf with larft/larfb) with kQrbNb-column panels.  One workgroup factors
+// a panel (columns in global memory, L2-resident) and forms V and the
 // compact-WY factor T; the trailing columns take C -= V (T^T (V^T C)) as three
-// MFMA GEMMs over the whole chip.
-// Work: W (m x n, column-major) | tau (k) | V (m x nb) | T (nb x nb) | Y (nb x n)
-// | Y2 (nb x n): qrb_work(m, n) doubles.
+// MFMA GEMMs over the whole chip.  Q, when wanted, is accumulated as dorgqr
+// does: from [I; 0], the panels in reverse order, X[j0:, j0:] -= V (T (V^T X)).
+// Work: W (m x n, column-major) | tau (k) | V | T | Y (nb x n) | Y2 (nb x n),
+// V and T holding one panel (nb columns) without Q and every panel with it:
+// qrb_work(m, n) doubles.
 // ---------------------------------------------------------------------------
 constexpr int64_t kQrbNb = 32;
 constexpr int kQrbThreads = 512;
@@ -407,8 +409,8 @@ static bool qrb_applies(int64_t m, int64_t n) {
   return k >= kQrbMinK && m * n >= kQrbMinElems;
 }
 static int64_t qrb_work(int64_t m, int64_t n) {
-  const int64_t k = m < n ? m : n;
-  return m * n + k + m * kQrbNb + kQrbNb * kQrbNb + 2 * kQrbNb * n;
+  const int64_t k = m < n ? m : n, npan = (k + kQrbNb - 1) / kQrbNb;
+  return m * n + k + m * (k > kQrbNb ? k : kQrbNb) + npan * kQrbNb * kQrbNb + 2 * kQrbNb * n;
 }
 
 // Sum over the wave of 32 values per lane (a[0..32)): recursive halving, so
@@ -436,11 +438,11 @@ __device__ __forceinline__ double wave_sum32(double (&a)[kQrbNb], int lane) {
 // (cc > c) give the update w = tau v^T P, the earlier reflectors' (cc < c)
 // give G[cc][c] = v_cc^T v_c, from which T's column c follows as in dlarft
 // (forward, columnwise): T[0:c, c] = -tau T[0:c, 0:c] G[0:c, c], T[c][c] = tau.
-// V (mp x jb, column-major, unit diagonal, zeros above) is written as the
-// reflectors form; R stays in W's upper triangle.
+// V (mp x jb, column-major with leading dimension ldv, unit diagonal, zeros
+// above) is written as the reflectors form; R stays in W's upper triangle.
 __global__ __launch_bounds__(kQrbThreads) void qrb_panel_kernel(int64_t m, int64_t j0, int64_t jb,
                                                                double* W, double* tau, double* V,
-                                                               double* T) {
+                                                               int64_t ldv, double* T) {
   __shared__ double red[kQrbThreads / 64];
   __shared__ double part[kQrbThreads / 64][kQrbNb];
   __shared__ double sG[kQrbNb][kQrbNb];
@@ -462,7 +464,7 @@ __global__ __launch_bounds__(kQrbThreads) void qrb_panel_kernel(int64_t m, int64
       scal = 1.0 / (alpha - beta);
       t = (beta - alpha) / beta;
     }
-    double* vc = V + c * mp;
+    double* vc = V + c * ldv;
     if (tid < c) vc[tid] = 0.0;
     double d[kQrbNb];
 #pragma unroll
@@ -475,7 +477,7 @@ __global__ __launch_bounds__(kQrbThreads) void qrb_panel_kernel(int64_t m, int64
 #pragma unroll
       for (int cc = 0; cc < kQrbNb; ++cc) {
         const int64_t q = cc < jb ? cc : jb - 1;
-        x[cc] = cc < c ? V[q * mp + i] : P[q * m + i];
+        x[cc] = cc < c ? V[q * ldv + i] : P[q * m + i];
       }
       const double v = i == c ? 1.0 : col[i] * scal;
       vc[i] = v;
@@ -549,14 +551,31 @@ __global__ __launch_bounds__(256) void qrb_extract_r_kernel(int64_t m, int64_t n
   }
 }
 
-static int qr_r_blocked(ttx_stream_t stream, int64_t m, int64_t n, const double* A, int64_t a_rs,
-                        int64_t a_cs, double* R, int64_t r_rs, int64_t r_cs, double* work) {
-  const int64_t k = m < n ? m : n;
+// Q = [I_k; 0] (m x k, strides q_rs, q_cs)
+__global__ void qrb_eye_kernel(int64_t m, int64_t k, double* Q, int64_t q_rs, int64_t q_cs) {
+  const int64_t total = m * k;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = q32(e, k), c = e - i * k;
+    Q[i * q_rs + c * q_cs] = i == c ? 1.0 : 0.0;
+  }
+}
+
+static unsigned qrb_grid(int64_t elems) {
+  const int64_t b = (elems + 255) / 256;
+  return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
+}
+
+static int qr_blocked(ttx_stream_t stream, int64_t m, int64_t n, const double* A, int64_t a_rs,
+                      int64_t a_cs, double* Q, int64_t q_rs, int64_t q_cs, double* R, int64_t r_rs,
+                      int64_t r_cs, double* work) {
+  const int64_t k = m < n ? m : n, npan = (k + kQrbNb - 1) / kQrbNb;
+  const bool form_q = Q != nullptr;
   double* W = work;
   double* tau = W + m * n;
   double* V = tau + k;
-  double* T = V + m * kQrbNb;
-  double* Y = T + kQrbNb * kQrbNb;
+  double* T = V + m * (form_q && k > kQrbNb ? k : kQrbNb);
+  double* Y = T + (form_q ? npan : 1) * kQrbNb * kQrbNb;
   double* Y2 = Y + kQrbNb * n;
   hipStream_t st = (hipStream_t)stream;
   {
@@ -567,24 +586,51 @@ static int qr_r_blocked(ttx_stream_t stream, int64_t m, int64_t n, const double*
   for (int64_t j0 = 0; j0 < k; j0 += kQrbNb) {
     const int64_t jb = (k - j0) < kQrbNb ? (k - j0) : kQrbNb;
     const int64_t mp = m - j0, nt = n - j0 - jb;
-    hipLaunchKernelGGL(qrb_panel_kernel, dim3(1), dim3(kQrbThreads), 0, st, m, j0, jb, W, tau, V, T);
+    // this panel's V (rows j0.., leading dimension ldv) and T
+    double* Vp = form_q ? V + j0 * m + j0 : V;
+    const int64_t ldv = form_q ? m : mp;
+    double* Tp = form_q ? T + (j0 / kQrbNb) * kQrbNb * kQrbNb : T;
+    hipLaunchKernelGGL(qrb_panel_kernel, dim3(1), dim3(kQrbThreads), 0, st, m, j0, jb, W, tau, Vp,
+                       ldv, Tp);
     int rc = launch_status();
     if (rc != TTX_OK) return rc;
     if (nt <= 0) continue;
     double* C = W + (j0 + jb) * m + j0;  // mp x nt, column-major (ld m)
     // Y = V^T C (jb x nt, row-major)
-    rc = ttx_gemm_f64(stream, 1, jb, nt, mp, 1.0, V, mp, 1, 0, C, 1, m, 0, 0.0, Y, nt, 1, 0);
+    rc = ttx_gemm_f64(stream, 1, jb, nt, mp, 1.0, Vp, ldv, 1, 0, C, 1, m, 0, 0.0, Y, nt, 1, 0);
     if (rc != TTX_OK) return rc;
     // Y2 = T^T Y
-    rc = ttx_gemm_f64(stream, 1, jb, nt, jb, 1.0, T, 1, jb, 0, Y, nt, 1, 0, 0.0, Y2, nt, 1, 0);
+    rc = ttx_gemm_f64(stream, 1, jb, nt, jb, 1.0, Tp, 1, jb, 0, Y, nt, 1, 0, 0.0, Y2, nt, 1, 0);
     if (rc != TTX_OK) return rc;
     // C -= V Y2
-    rc = ttx_gemm_f64(stream, 1, mp, nt, jb, -1.0, V, 1, mp, 0, Y2, nt, 1, 0, 1.0, C, 1, m, 0);
+    rc = ttx_gemm_f64(stream, 1, mp, nt, jb, -1.0, Vp, 1, ldv, 0, Y2, nt, 1, 0, 1.0, C, 1, m, 0);
     if (rc != TTX_OK) return rc;
   }
-  hipLaunchKernelGGL(qrb_extract_r_kernel, dim3((unsigned)((k * n + 255) / 256 < 4096 ? (k * n + 255) / 256 : 4096)),
-                     dim3(256), 0, st, m, n, k, W, R, r_rs, r_cs);
-  return launch_status();
+  hipLaunchKernelGGL(qrb_extract_r_kernel, dim3(qrb_grid(k * n)), dim3(256), 0, st, m, n, k, W, R,
+                     r_rs, r_cs);
+  int rc = launch_status();
+  if (rc != TTX_OK || !form_q) return rc;
+  hipLaunchKernelGGL(qrb_eye_kernel, dim3(qrb_grid(m * k)), dim3(256), 0, st, m, k, Q, q_rs, q_cs);
+  if ((rc = launch_status()) != TTX_OK) return rc;
+  for (int64_t p = npan - 1; p >= 0; --p) {
+    const int64_t j0 = p * kQrbNb;
+    const int64_t jb = (k - j0) < kQrbNb ? (k - j0) : kQrbNb;
+    const int64_t mp = m - j0, nc = k - j0;
+    const double* Vp = V + j0 * m + j0;
+    const double* Tp = T + p * kQrbNb * kQrbNb;
+    double* X = Q + j0 * q_rs + j0 * q_cs;  // mp x nc
+    // Y = V^T X, Y2 = T Y, X -= V Y2
+    if ((rc = ttx_gemm_f64(stream, 1, jb, nc, mp, 1.0, Vp, m, 1, 0, X, q_rs, q_cs, 0, 0.0, Y, nc, 1,
+                           0)) != TTX_OK)
+      return rc;
+    if ((rc = ttx_gemm_f64(stream, 1, jb, nc, jb, 1.0, Tp, jb, 1, 0, Y, nc, 1, 0, 0.0, Y2, nc, 1,
+                           0)) != TTX_OK)
+      return rc;
+    if ((rc = ttx_gemm_f64(stream, 1, mp, nc, jb, -1.0, Vp, 1, m, 0, Y2, nc, 1, 0, 1.0, X, q_rs,
+                           q_cs, 0)) != TTX_OK)
+      return rc;
+  }
+  return TTX_OK;
 }
 
 extern "C" int64_t ttx_qr_work_size(int64_t m, int64_t n) {
@@ -666,9 +712,9 @@ extern "C" int ttx_qr_f64(ttx_stream_t stream, int64_t m, int64_t n, const doubl
   }
   if (Q != nullptr && cqr_applies(m, n))
     return cqr_factor(stream, m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work);
-  // R only of a large matrix: blocked across the chip
-  if (Q == nullptr && qrb_applies(m, n))
-    return qr_r_blocked(stream, m, n, A, a_rs, a_cs, R, r_rs, r_cs, work);
+  // large matrices: blocked across the chip
+  if (qrb_applies(m, n))
+    return qr_blocked(stream, m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work);
   QrArgs p{m, n, A, a_rs, a_cs, Q, q_rs, q_cs, R, r_rs, r_cs, work, nullptr};
   const size_t bytes = (size_t)hh_work(m, n) * sizeof(double);
   const int64_t k = m < n ? m : n;
@@ -705,7 +751,8 @@ constexpr double kSvdCert = 16.0;
 // Work: Xp (P x Q) | Qx (P x Q) | R (Q x Q) | V (Q x Q) | Vs (Q x Q) | sig (Q)
 // | the QR's work | ints: perm (Q), ord (Q), flags ((QQ-1) QQ/2), count, bad.
 // ---------------------------------------------------------------------------
-constexpr int64_t kSvdmMinQ = 160;
+// the smallest Q whose Q x Q Jacobi matrix exceeds the one-workgroup kernel's LDS
+constexpr int64_t kSvdmMinQ = 139;
 constexpr int kSvdmMaxSweeps = 80;
 constexpr int kSvdmThreads = 256;
 
