@@ -98,7 +98,9 @@ int ttx_diff_nrm2_f64(ttx_stream_t stream, int64_t n, const double* a, const dou
  * Q (m x k) and R (k x n), k = min(m, n), written with their own strides.
  * scipy.linalg.qr(mode="economic") at ttnumpy/utils.py:101,130 and
  * ttnumpy/solvers/one_site.py:188,204.  Q may be NULL (only R is formed, as the
- * norm sweep of ttnumpy/tensor_train.py:509-540 needs).  Work: ttx_qr_work_size(m, n).
+ * norm sweep of ttnumpy/tensor_train.py:509-540 needs).  Large matrices
+ * (min(m, n) >= 64, m n >= 2^17) take LAPACK's blocked scheme with the
+ * trailing updates on the GEMM across the chip.  Work: ttx_qr_work_size(m, n).
  */
 int64_t ttx_qr_work_size(int64_t m, int64_t n);
 int ttx_qr_f64(ttx_stream_t stream, int64_t m, int64_t n,
@@ -112,6 +114,9 @@ int ttx_qr_f64(ttx_stream_t stream, int64_t m, int64_t n,
  * side).  scipy.linalg.svd at ttnumpy/utils.py:347,359 (gesdd/gesvd) and
  * ttnumpy/solvers/amen.py:254,301 (gesvd).  info[0] receives TTX_OK,
  * TTX_ENOCONV (sweep limit) or TTX_ENONFINITE.  Work: ttx_svd_work_size.
+ * With k >= 139 the Jacobi runs across the chip, one launch per round, and
+ * the call synchronizes the stream once per sweep (and once after checking
+ * the input for NaN/inf): such calls cannot be captured in a HIP graph.
  */
 int64_t ttx_svd_work_size(int64_t m, int64_t n);
 int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n,
