@@ -562,12 +562,15 @@ __device__ __forceinline__ bool jacobi_sweeps(double* G, double* V, int Q, int* 
 // sides); it may be LDS.  REGS: see jacobi_pair.  Without ACCV the rotations
 // are not accumulated: V (Vt of a tall, U of a wide input) is left unwritten
 // for the caller to form.  Ends synced.
-// PRE (with ACCV, on tall factors P > Q): Jacobi preconditioned as in Drmac
+// PRE (with ACCV, on tall factors P > Q with Q >= kSvdPreMinQ; smaller factors
+// converge in few sweeps either way and keep the plain order): Jacobi
+// preconditioned as in Drmac
 // and Veselic's one-sided Jacobi SVD: the columns of X are taken in order of
 // decreasing norm (X Pi = Qx R), and the rotations act on the columns of R^T,
 // which for graded matrices are far closer to orthogonal than R's (about a
 // third of the sweeps).  With R^T V = Y (orthogonal columns, norms sigma):
 // X = (Qx V) Sigma (Pi Y Sigma^-1)^T, so U_X = Qx V and V_X = Pi Y Sigma^-1.
+constexpr int64_t kSvdPreMinQ = 64;
 template <bool REGS = true, bool ACCV = true, bool PRE = false>
 __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double* red,
                                           double* lds = nullptr, int64_t lds_cap = 0) {
@@ -593,7 +596,7 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
   double* sig = V + Q * Q;        // Q
   double* ordd = sig + Q;         // Q       (sort permutation, stored as doubles)
 
-  const bool pre = PRE && ACCV && P > Q;
+  const bool pre = PRE && ACCV && P > Q && Q >= kSvdPreMinQ;
   if (tid == 0) s_bad = 0;
   if (pre) {
     // squared column norms (in ordd), then the source column of each position
