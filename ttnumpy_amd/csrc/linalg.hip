@@ -399,8 +399,9 @@ using namespace ttx;
 // V and T holding one panel (nb columns) without Q and every panel with it:
 // qrb_work(m, n) doubles.
 // ---------------------------------------------------------------------------
-constexpr int64_t kQrbNb = 32;
-constexpr int kQrbThreads = 512;
+constexpr int64_t kQrbNb = 16;
+constexpr int kQrbLog = 4;  // log2(kQrbNb)
+constexpr int kQrbThreads = 1024;
 constexpr int64_t kQrbMinK = 64;
 constexpr int64_t kQrbMinElems = int64_t(1) << 17;
 
@@ -413,13 +414,14 @@ static int64_t qrb_work(int64_t m, int64_t n) {
   return m * n + k + m * (k > kQrbNb ? k : kQrbNb) + npan * kQrbNb * kQrbNb + 2 * kQrbNb * n;
 }
 
-// Sum over the wave of 32 values per lane (a[0..32)): recursive halving, so
-// lane l ends with the total of entry (l >> 1) & 31 (lanes l and l ^ 1 hold
-// the same bits).  Fixed order, so deterministic.
-__device__ __forceinline__ double wave_sum32(double (&a)[kQrbNb], int lane) {
+// Sum over the wave of kQrbNb values per lane: recursive halving over the
+// lane bits 5 .. 6-kQrbLog, so lane l ends with the total of entry
+// (l >> (6 - kQrbLog)) & (kQrbNb - 1) (the lanes that differ only in the low
+// bits hold the same bits).  Fixed order, so deterministic.
+__device__ __forceinline__ double wave_sum_nb(double (&a)[kQrbNb], int lane) {
 #pragma unroll
-  for (int s = 0; s < 5; ++s) {
-    const int half = 16 >> s, off = 32 >> s;
+  for (int s = 0; s < kQrbLog; ++s) {
+    const int half = (int)kQrbNb >> (s + 1), off = 32 >> s;
     const bool hi = (lane & off) != 0;
 #pragma unroll
     for (int j = 0; j < half; ++j) {
@@ -428,7 +430,10 @@ __device__ __forceinline__ double wave_sum32(double (&a)[kQrbNb], int lane) {
       a[j] = keep + __shfl_xor(send, off, kWave);
     }
   }
-  return a[0] + __shfl_xor(a[0], 1, kWave);
+  double v = a[0];
+#pragma unroll
+  for (int off = (1 << (6 - kQrbLog)) >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
 }
 
 // One panel: columns j0 .. j0+jb-1 of W, rows j0 .. m-1 (mp rows).  Column c
@@ -469,7 +474,7 @@ __global__ __launch_bounds__(kQrbThreads) void qrb_panel_kernel(int64_t m, int64
     double d[kQrbNb];
 #pragma unroll
     for (int cc = 0; cc < kQrbNb; ++cc) d[cc] = 0.0;
-    // all 32 loads of a row are issued before any is used: columns past the
+    // all kQrbNb loads of a row are issued before any is used: columns past the
     // panel read its last column (the partial is never used), column c reads
     // itself (likewise)
     for (int64_t i = c + tid; i < mp; i += kQrbThreads) {
@@ -484,8 +489,8 @@ __global__ __launch_bounds__(kQrbThreads) void qrb_panel_kernel(int64_t m, int64
 #pragma unroll
       for (int cc = 0; cc < kQrbNb; ++cc) d[cc] += v * x[cc];
     }
-    const double tot = wave_sum32(d, lane);
-    if ((lane & 1) == 0) part[wave][(lane >> 1) & 31] = tot;
+    const double tot = wave_sum_nb(d, lane);
+    if ((lane & ((1 << (6 - kQrbLog)) - 1)) == 0) part[wave][(lane >> (6 - kQrbLog)) & (kQrbNb - 1)] = tot;
     __syncthreads();
     if (tid < jb) {
       double s = 0.0;
