@@ -389,7 +389,8 @@ struct SvdArgs {
 // the accumulated V) by the 16 lanes of a DPP row; `sub` is the lane within
 // the row.  Element r of a column is handled by lane r % 16, so the sums and
 // hence the result do not depend on which row or workgroup size runs it.
-// Returns true when the pair was rotated.  With REGS, columns of up to
+// Returns 0 when the pair was not rotated, 2 when it was and |c| exceeded
+// thr sqrt(a b), else 1.  With REGS, columns of up to
 // 16 * kJacRegs elements are held in registers between the sums and the
 // rotation (the same operations in the same order, so the same bits).
 constexpr int kJacRegs = 8;  // column elements per lane held in registers
@@ -418,8 +419,8 @@ __device__ __forceinline__ bool jac_params(double a, double b, double c, double 
 }
 
 template <bool REGS, bool ACCV>
-__device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, double* vj, int Q,
-                                            int sub, double tol) {
+__device__ __forceinline__ int jacobi_pair(double* gi, double* gj, double* vi, double* vj, int Q,
+                                           int sub, double tol, double thr) {
   double a = 0.0, b = 0.0, c = 0.0;
   if (REGS && Q <= 16 * kJacRegs) {
     // loads at clamped (in-bounds) indices, zeros selected past the end
@@ -446,7 +447,7 @@ __device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, 
     b = row16_sum(b);
     c = row16_sum(c);
     double cs, sn;
-    if (!jac_params(a, b, c, tol, cs, sn)) return false;
+    if (!jac_params(a, b, c, tol, cs, sn)) return 0;
 #pragma unroll
     for (int t = 0; t < kJacRegs; ++t) {
       const int r = sub + 16 * t;
@@ -459,7 +460,7 @@ __device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, 
         }
       }
     }
-    return true;
+    return fabs(c) > __dmul_rn(__dmul_rn(thr, sqrt(a)), sqrt(b)) ? 2 : 1;
   }
   // four elements per lane per step, loads first (same order of operations)
   for (int r0 = sub; r0 < Q; r0 += 64) {
@@ -483,7 +484,7 @@ __device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, 
   b = row16_sum(b);
   c = row16_sum(c);
   double cs, sn;
-  if (!jac_params(a, b, c, tol, cs, sn)) return false;
+  if (!jac_params(a, b, c, tol, cs, sn)) return 0;
   for (int r0 = sub; r0 < Q; r0 += 64) {
     double x[4], y[4], u[4], w[4];
 #pragma unroll
@@ -508,14 +509,17 @@ __device__ __forceinline__ bool jacobi_pair(double* gi, double* gj, double* vi, 
       }
     }
   }
-  return true;
+  return fabs(c) > __dmul_rn(__dmul_rn(thr, sqrt(a)), sqrt(b)) ? 2 : 1;
 }
 
 // Sweeps of the one-sided Jacobi method on the Q columns of G (Q x Q,
-// column-major) with the rotations accumulated into V, until a sweep
-// rotates nothing (true) or 80 sweeps (false).  Each round of the
-// round-robin schedule pairs every column once; a 16-lane DPP row takes one
-// pair.  `flag` is a block-shared int.  Ends synced.
+// column-major) with the rotations accumulated into V (with ACCV), until a
+// sweep rotates nothing (true) or 80 sweeps (false).  Without ACCV a sweep
+// whose rotations all had |c| <= sqrt(tol)/4 sqrt(a b) also ends it: one-sided
+// Jacobi converges quadratically, so the next sweep would find every pair
+// orthogonal to about tol/16 and rotate by rounding-level angles at most.
+// Each round of the round-robin schedule pairs every column once; a 16-lane
+// DPP row takes one pair.  `flag` points to two block-shared ints.  Ends synced.
 template <bool REGS, bool ACCV>
 __device__ __forceinline__ bool jacobi_sweeps(double* G, double* V, int Q, int* flag) {
   const int tid = threadIdx.x;
@@ -524,8 +528,12 @@ __device__ __forceinline__ bool jacobi_sweeps(double* G, double* V, int Q, int* 
   const int QQ = Q + (Q & 1);
   const double tol = (double)Q * kEps;
   if (Q < 2) return true;
+  const double thr = ACCV ? 0.0 : 0.25 * sqrt(tol);
   for (int sweep = 0; sweep < 80; ++sweep) {
-    if (tid == 0) *flag = 0;
+    if (tid == 0) {
+      flag[0] = 0;
+      flag[1] = 0;
+    }
     __syncthreads();
     for (int round = 0; round < QQ - 1; ++round) {
       for (int pk = row; pk < QQ / 2; pk += nrow) {
@@ -541,13 +549,16 @@ __device__ __forceinline__ bool jacobi_sweeps(double* G, double* V, int Q, int* 
         j += 1;
         if (i >= Q || j >= Q) continue;
         if (i > j) { const int t = i; i = j; j = t; }
-        if (jacobi_pair<REGS, ACCV>(G + i * Q, G + j * Q, V + i * Q, V + j * Q, Q, sub, tol) &&
-            sub == 0)
-          *flag = 1;
+        const int rot = jacobi_pair<REGS, ACCV>(G + i * Q, G + j * Q, V + i * Q, V + j * Q, Q, sub,
+                                                tol, thr);
+        if (rot != 0 && sub == 0) {
+          flag[0] = 1;
+          if (rot == 2) flag[1] = 1;
+        }
       }
       __syncthreads();
     }
-    const bool done = *flag == 0;
+    const bool done = flag[0] == 0 || (!ACCV && flag[1] == 0);
     __syncthreads();
     if (done) return true;
   }
@@ -574,7 +585,7 @@ constexpr int64_t kSvdPreMinQ = 64;
 template <bool REGS = true, bool ACCV = true, bool PRE = false>
 __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double* red,
                                           double* lds = nullptr, int64_t lds_cap = 0) {
-  __shared__ int s_rot;
+  __shared__ int s_rot[2];
   __shared__ int s_bad;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -659,8 +670,8 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
 
   // one-sided Jacobi on the columns of G; a separate call per location of G
   // lets each access compile to an LDS or a global load
-  const int converged = G == lds ? jacobi_sweeps<REGS, ACCV>(lds, V, (int)Q, &s_rot)
-                                 : jacobi_sweeps<REGS, ACCV>(Gw, V, (int)Q, &s_rot);
+  const int converged = G == lds ? jacobi_sweeps<REGS, ACCV>(lds, V, (int)Q, s_rot)
+                                 : jacobi_sweeps<REGS, ACCV>(Gw, V, (int)Q, s_rot);
   if (tid == 0) p.info[0] = converged ? TTX_OK : TTX_ENOCONV;
 
   // singular values = column norms
