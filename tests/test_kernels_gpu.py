@@ -622,3 +622,34 @@ def test_qr_blocked_with_q(m, n, q_col_major, zero_cols):
     ref = np.linalg.qr(A, mode="r")
     np.testing.assert_allclose(np.abs(np.diag(Rh)), np.abs(np.diag(ref)), rtol=1e-10,
                                atol=1e-12 * np.abs(np.diag(ref)).max())
+
+
+@pytest.mark.parametrize("m,n", [(600, 200), (200, 600)])
+def test_large_svd_and_qr_on_transposed_views(m, n):
+    """The multi-workgroup SVD and the blocked QR read A through strides: a
+    transposed view (column-major storage) gives the same factors as the
+    row-major copy (the pre-sort and the QR see the same matrix)."""
+    print(f"large svd/qr transposed view {m}x{n}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(m * 3 + n)
+    A = rng.standard_normal((m, n)) * np.logspace(0, -6, n)[None, :]
+    At = _t(np.ascontiguousarray(A.T))  # n x m row-major = A column-major
+    Ar = _t(A)
+    k = min(m, n)
+    info = torch.zeros(2, dtype=torch.int32, device=D.device())
+    U1, S1, V1 = D.svd(m, n, D.P(Ar), n, 1, info)
+    assert int(info[0].item()) == 0
+    U2, S2, V2 = D.svd(m, n, D.P(At), 1, m, info)
+    assert int(info[0].item()) == 0
+    np.testing.assert_array_equal(_h(S1), _h(S2))
+    np.testing.assert_array_equal(_h(U1), _h(U2))
+    np.testing.assert_array_equal(_h(V1), _h(V2))
+    np.testing.assert_allclose(_h(S1), np.linalg.svd(A, compute_uv=False), rtol=1e-10,
+                               atol=1e-13 * np.linalg.norm(A, 2))
+    Q1, R1 = D.empty(m, k), D.empty(k, n)
+    Q2, R2 = D.empty(m, k), D.empty(k, n)
+    D.qr(m, n, D.P(Ar), n, 1, D.P(Q1), k, 1, D.P(R1), n, 1)
+    D.qr(m, n, D.P(At), 1, m, D.P(Q2), k, 1, D.P(R2), n, 1)
+    np.testing.assert_array_equal(_h(R1), _h(R2))
+    np.testing.assert_array_equal(_h(Q1), _h(Q2))
+    np.testing.assert_allclose(_h(Q1) @ _h(R1), A, atol=1e-13 * np.abs(A).max() * max(m, n))
