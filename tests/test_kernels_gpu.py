@@ -185,6 +185,29 @@ def test_svd_tall_cholesky_qr2_path(m, n):
     np.testing.assert_allclose(Vh @ Vh.T, np.eye(k), atol=1e-13 * max(m, n))
 
 
+@pytest.mark.parametrize("m,n", [(384, 128), (256, 64), (128, 384)])
+def test_svd_clustered_spectrum_orthonormal(m, n):
+    """Clustered singular values (X = orthonormal * (I + 1e-8 E)): the rotations
+    of such pairs keep O(1) angles while |c| is small, so the V-free Jacobi of
+    the CholeskyQR2 path must not stop on |c| alone; U and V stay orthonormal
+    to the rounding level and the singular values match LAPACK's."""
+    print(f"svd clustered {m}x{n}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(11 * m + n)
+    k = min(m, n)
+    Ux = np.linalg.qr(rng.standard_normal((m, k)))[0]
+    Vx = np.linalg.qr(rng.standard_normal((n, k)))[0]
+    A = Ux @ (np.eye(k) + 1e-8 * rng.standard_normal((k, k))) @ Vx.T
+    info = torch.zeros(2, dtype=torch.int32, device=D.device())
+    U, S, Vt = D.svd(m, n, D.P(_t(A)), n, 1, info)
+    assert int(info[0].item()) == 0
+    Uh, Sh, Vh = _h(U), _h(S), _h(Vt)
+    np.testing.assert_allclose(Sh, np.linalg.svd(A, compute_uv=False), rtol=1e-13)
+    np.testing.assert_allclose((Uh * Sh) @ Vh, A, atol=1e-13 * max(m, n))
+    assert np.max(np.abs(Uh.T @ Uh - np.eye(k))) <= 1e-13 * max(m, n)
+    assert np.max(np.abs(Vh @ Vh.T - np.eye(k))) <= 1e-13 * max(m, n)
+
+
 @pytest.mark.parametrize("m,n", [(1, 1), (1, 5), (5, 1), (8, 8), (66, 33), (33, 66), (128, 384),
                                  (384, 128), (200, 150), (2, 2)])
 def test_svd(m, n):

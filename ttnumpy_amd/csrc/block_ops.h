@@ -389,8 +389,8 @@ struct SvdArgs {
 // the accumulated V) by the 16 lanes of a DPP row; `sub` is the lane within
 // the row.  Element r of a column is handled by lane r % 16, so the sums and
 // hence the result do not depend on which row or workgroup size runs it.
-// Returns 0 when the pair was not rotated, 2 when it was and |c| exceeded
-// thr sqrt(a b), else 1.  With REGS, columns of up to
+// Returns 0 when the pair was not rotated, 2 when it was and either |c|
+// exceeded thr sqrt(a b) or the rotation's sine exceeded thr, else 1.  With REGS, columns of up to
 // 16 * kJacRegs elements are held in registers between the sums and the
 // rotation (the same operations in the same order, so the same bits).
 constexpr int kJacRegs = 8;  // column elements per lane held in registers
@@ -460,7 +460,7 @@ __device__ __forceinline__ int jacobi_pair(double* gi, double* gj, double* vi, d
         }
       }
     }
-    return fabs(c) > __dmul_rn(__dmul_rn(thr, sqrt(a)), sqrt(b)) ? 2 : 1;
+    return (fabs(c) > __dmul_rn(__dmul_rn(thr, sqrt(a)), sqrt(b)) || fabs(sn) > thr) ? 2 : 1;
   }
   // four elements per lane per step, loads first (same order of operations)
   for (int r0 = sub; r0 < Q; r0 += 64) {
@@ -509,15 +509,19 @@ __device__ __forceinline__ int jacobi_pair(double* gi, double* gj, double* vi, d
       }
     }
   }
-  return fabs(c) > __dmul_rn(__dmul_rn(thr, sqrt(a)), sqrt(b)) ? 2 : 1;
+  return (fabs(c) > __dmul_rn(__dmul_rn(thr, sqrt(a)), sqrt(b)) || fabs(sn) > thr) ? 2 : 1;
 }
 
 // Sweeps of the one-sided Jacobi method on the Q columns of G (Q x Q,
 // column-major) with the rotations accumulated into V (with ACCV), until a
 // sweep rotates nothing (true) or 80 sweeps (false).  Without ACCV a sweep
-// whose rotations all had |c| <= sqrt(tol)/4 sqrt(a b) also ends it: one-sided
-// Jacobi converges quadratically, so the next sweep would find every pair
-// orthogonal to about tol/16 and rotate by rounding-level angles at most.
+// whose rotations all had |c| <= sqrt(tol)/4 sqrt(a b) and a sine below
+// sqrt(tol)/4 also ends it: one-sided Jacobi converges quadratically, so the
+// next sweep would find every pair orthogonal to about tol/16 and rotate by
+// rounding-level angles at most.  The sine test matters for clustered
+// singular values: with a ~ b a small |c| can still give an O(1) angle, the
+// pair then converges only linearly, and a stop on |c| alone leaves the
+// columns orthogonal to ~sqrt(tol) rather than tol.
 // Each round of the round-robin schedule pairs every column once; a 16-lane
 // DPP row takes one pair.  `flag` points to two block-shared ints.  Ends synced.
 template <bool REGS, bool ACCV>
