@@ -31,7 +31,8 @@ enum {
   TTX_ENOCONV = 3,     /* iteration did not converge -> warning + retry      */
   TTX_EHIP = 4,        /* HIP launch/runtime error   -> RuntimeError          */
   TTX_ESINGULAR = 5,   /* exactly singular pivot     -> numpy LinAlgError    */
-  TTX_ENOTSPD = 6      /* Cholesky met a pivot <= 0 (or NaN): not positive definite */
+  TTX_ENOTSPD = 6,     /* Cholesky met a pivot <= 0 (or NaN): not positive definite */
+  TTX_ESCRATCH = 7     /* the call needs more than a fixed scratch array holds -> ValueError */
 };
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -123,6 +124,24 @@ int ttx_svd_f64(ttx_stream_t stream, int64_t m, int64_t n,
                 const double* A, int64_t a_rs, int64_t a_cs,
                 double* U, int64_t u_rs, int64_t u_cs, double* S,
                 double* Vt, int64_t v_rs, int64_t v_cs, double* work, int32_t* info);
+
+/*
+ * ttx_svd_f64 without a work argument and without any device allocation.
+ * No counterpart in the reference (whose SVD is LAPACK's, which allocates
+ * its own work): this entry is for callers that cannot allocate device
+ * memory, such as code under HIP graph capture or a serving loop that
+ * forbids hipMalloc.  The work buffer is a fixed, file-scope device array of
+ * ttx_svd_noalloc_max_work() doubles (32 MiB; each device holds its own copy
+ * until the process ends).  Shapes whose ttx_svd_work_size exceeds it return
+ * TTX_ESCRATCH and touch nothing.  Results are those of ttx_svd_f64 on the
+ * same input.  All calls on one device share the array, so calls that
+ * overlap on one device (two streams, two host threads) race: serialize them.
+ */
+int64_t ttx_svd_noalloc_max_work(void);
+int ttx_svd_f64_noalloc(ttx_stream_t stream, int64_t m, int64_t n,
+                        const double* A, int64_t a_rs, int64_t a_cs,
+                        double* U, int64_t u_rs, int64_t u_cs, double* S,
+                        double* Vt, int64_t v_rs, int64_t v_cs, int32_t* info);
 
 /*
  * Solve M u = b for a dense n x n system by LU with partial pivoting.

@@ -20,6 +20,7 @@ TTX_ENOCONV = 3
 TTX_EHIP = 4
 TTX_ESINGULAR = 5
 TTX_ENOTSPD = 6
+TTX_ESCRATCH = 7
 TTX_MINRES_STATE = 32
 TTX_GMRES_STATE = 16
 
@@ -45,6 +46,9 @@ _SIGNATURES = {
     "ttx_svd_work_size": (_i64, [_i64, _i64]),
     "ttx_svd_f64": (_i32, [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _i64,
                            _p, _p]),
+    "ttx_svd_noalloc_max_work": (_i64, []),
+    "ttx_svd_f64_noalloc": (_i32, [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64,
+                                   _i64, _p]),
     "ttx_lu_work_size": (_i64, [_i64]),
     "ttx_lu_solve_f64": (_i32, [_p, _i64, _p, _i64, _p, _p, _p]),
     "ttx_chol_solve_f64": (_i32, [_p, _i64, _p, _i64, _p, _p]),
@@ -121,6 +125,9 @@ def check(rc: int, what: str) -> None:
         return
     if rc == TTX_EINVAL:
         raise ValueError(f"{what}: invalid argument")
+    if rc == TTX_ESCRATCH:
+        raise ValueError(f"{what}: the problem needs more work memory than the fixed scratch "
+                         "array holds")
     if rc == TTX_ENONFINITE:
         raise ValueError(f"{what}: NaN or infinite entries")
     if rc == TTX_ENOCONV:

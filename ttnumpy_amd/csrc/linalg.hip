@@ -13,6 +13,8 @@
 
 #include <float.h>
 
+#include <mutex>
+
 #include "block_ops.h"
 
 namespace ttx {
@@ -1161,4 +1163,54 @@ extern "C" int ttx_svd_terms_f64(ttx_stream_t stream, int64_t p, int64_t q, int6
   hipLaunchKernelGGL(svd_terms_kernel, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream, p, q,
                      limit, U, u_rs, u_cs, S, Vt, v_rs, v_cs, terms);
   return launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// ttx_svd_f64_noalloc: ttx_svd_f64 with its work buffer in a file-scope device
+// array (`scratch`) instead of a caller allocation, for callers that cannot
+// allocate device memory.  Each device holds its own copy of the array from
+// the load of this library's code object until the process ends; the host
+// finds its address with hipGetSymbolAddress once per device (the array's
+// name is only a host-side stand-in, never a device address) and passes that
+// pointer on as ttx_svd_f64's `work`.  ttx_svd_f64 writes every part of its
+// work it reads, so the array needs no clearing between calls.  Calls on one
+// device share the array: two such calls in flight at once (on different
+// streams, or from different threads) race.
+constexpr int64_t kSvdScratchDoubles = int64_t(4) << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[kSvdScratchDoubles];
+
+namespace {
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchMaxDevices] = {};
+}  // namespace
+
+// Address of the current device's copy of `scratch` (looked up on the first
+// call on that device, with that device current).
+static int svd_scratch(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchMaxDevices) return TTX_EHIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr)
+      return TTX_EHIP;
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TTX_OK;
+}
+
+extern "C" int64_t ttx_svd_noalloc_max_work(void) { return kSvdScratchDoubles; }
+
+extern "C" int ttx_svd_f64_noalloc(ttx_stream_t stream, int64_t m, int64_t n, const double* A,
+                                   int64_t a_rs, int64_t a_cs, double* U, int64_t u_rs,
+                                   int64_t u_cs, double* S, double* Vt, int64_t v_rs,
+                                   int64_t v_cs, int32_t* info) {
+  if (m < 1 || n < 1) return TTX_EINVAL;
+  if (ttx_svd_work_size(m, n) > kSvdScratchDoubles) return TTX_ESCRATCH;
+  double* w = nullptr;
+  const int rc = svd_scratch(&w);
+  if (rc != TTX_OK) return rc;
+  return ttx_svd_f64(stream, m, n, A, a_rs, a_cs, U, u_rs, u_cs, S, Vt, v_rs, v_cs, w, info);
 }
