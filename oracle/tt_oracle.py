@@ -790,18 +790,31 @@ def solve_local(op, b_vec, iterative, local_rtol, warm=None, fallback=None, info
     return np.asarray(sla.solve(op, b_vec, check_finite=False)).ravel(), op
 
 
-def rank_by_residual(op, b_loc, U, S, Vt, exact_res, trunc_tol, damp, max_rank) -> Tuple[int, bool]:
-    """Smallest rank whose local residual meets the budget (``common.py:812-866``)."""
+def rank_by_residual(op, b_loc, U, S, Vt, exact_res, trunc_tol, damp, max_rank,
+                     trace: Optional[dict] = None) -> Tuple[int, bool]:
+    """Smallest rank whose local residual meets the budget (``common.py:812-866``).
+
+    With ``trace``, records the threshold and the norm of every partial sum
+    tested (``thr``, ``partials``), which the site-step parity tests use to
+    tell decisions that round-off can flip from those it cannot.
+    """
     if S[0] == 0.0:
+        if trace is not None:
+            trace.update(thr=0.0, partials=[], zero_spectrum=True)
         return 1, False
     limit = len(S) if max_rank is None else min(len(S), max_rank)
     thr = max(damp * exact_res, 0.5 * trunc_tol * np.linalg.norm(b_loc))
     terms = np.einsum("ik,kj->kij", U[:, :limit] * S[:limit], Vt[:limit, :], optimize="optimal")
     contrib = op @ terms.reshape(limit, -1).T
     partial = -b_loc.copy()
+    norms = []
+    if trace is not None:
+        trace.update(thr=thr, partials=norms, exact=exact_res, b_norm=float(np.linalg.norm(b_loc)))
     for r in range(limit):
         partial += contrib[:, r]
-        if np.linalg.norm(partial) <= thr:
+        nrm = np.linalg.norm(partial)
+        norms.append(float(nrm))
+        if nrm <= thr:
             return r + 1, False
     return limit, max_rank is not None and max_rank < len(S)
 
@@ -816,10 +829,17 @@ def rank_by_threshold(S, thr, max_rank) -> Tuple[int, bool]:
     return r, False
 
 
-def _choose_rank(settings, op, b_loc, U, S, Vt, exact, error_threshold, max_rank, damp):
+def _choose_rank(settings, op, b_loc, U, S, Vt, exact, error_threshold, max_rank, damp, trace=None):
+    if trace is not None:
+        trace["S"] = S
     if settings.residual_mode:
-        return rank_by_residual(op, b_loc, U, S, Vt, exact, settings.truncation_tol, damp, max_rank)[0]
-    return rank_by_threshold(S, error_threshold, max_rank)[0]
+        r = rank_by_residual(op, b_loc, U, S, Vt, exact, settings.truncation_tol, damp, max_rank,
+                             trace)[0]
+    else:
+        r = rank_by_threshold(S, error_threshold, max_rank)[0]
+    if trace is not None:
+        trace["rank"] = r
+    return r
 
 
 class _System:
@@ -913,21 +933,21 @@ def _exact_residual(op, u, b_loc):
     return float(np.linalg.norm(op @ u - b_loc))
 
 
-def truncate_core_right(core3, settings, error_threshold, max_rank, op, b_loc, damp):
+def truncate_core_right(core3, settings, error_threshold, max_rank, op, b_loc, damp, trace=None):
     """SVD split of (r_l n x r_r) (``amen.py:227-275``): (U_r, S V_r)."""
     rl, n, rr = core3.shape
     exact = _exact_residual(op, core3.ravel(), b_loc) if settings.residual_mode else 0.0
     U, S, Vt = sla.svd(core3.reshape(rl * n, rr), full_matrices=False, check_finite=False, lapack_driver="gesvd")
-    r = _choose_rank(settings, op, b_loc, U, S, Vt, exact, error_threshold, max_rank, damp)
+    r = _choose_rank(settings, op, b_loc, U, S, Vt, exact, error_threshold, max_rank, damp, trace)
     return U[:, :r].reshape(rl, n, r), S[:r, None] * Vt[:r]
 
 
-def truncate_core_left(core3, settings, error_threshold, max_rank, op, b_loc, damp):
+def truncate_core_left(core3, settings, error_threshold, max_rank, op, b_loc, damp, trace=None):
     """Mirror (``amen.py:278-322``): (V_r as (r,n,r_r), U_r S)."""
     rl, n, rr = core3.shape
     exact = _exact_residual(op, core3.ravel(), b_loc) if settings.residual_mode else 0.0
     U, S, Vt = sla.svd(core3.reshape(rl, n * rr), full_matrices=False, check_finite=False, lapack_driver="gesvd")
-    r = _choose_rank(settings, op, b_loc, U, S, Vt, exact, error_threshold, max_rank, damp)
+    r = _choose_rank(settings, op, b_loc, U, S, Vt, exact, error_threshold, max_rank, damp, trace)
     return Vt[:r].reshape(r, n, rr), U[:, :r] * S[:r]
 
 
@@ -984,11 +1004,86 @@ def _validate(A: Train, b: Train, x0: Train):
         raise ValueError("dimension mismatch")
 
 
+@dataclass
+class SiteState:
+    """Snapshot of the AMEn state before one site step (``amen.py:579-708``).
+
+    ``x`` and ``z`` are the cores of the solution and residual trains at
+    that moment (shared arrays: the solver rebinds cores, it never writes
+    into them), ``envs`` the ten environments at site ``k`` keyed
+    ``A_L``/``A_R`` (x,x,A), ``b_L``/``b_R`` (x,b), ``zb_L``/``zb_R`` (z,b),
+    ``zA_L``/``zA_R`` (z,x,A) and ``xz_L``/``xz_R`` (x,z).
+    """
+
+    sweep: int
+    direction: str
+    k: int
+    x: List[np.ndarray]
+    x_center: Optional[int]
+    z: List[np.ndarray]
+    envs: dict
+
+
+def site_step(state: SiteState, A: Train, b: Train, settings: Settings, error_threshold=1e-7,
+              max_rank=None, residual_damp=2.0) -> dict:
+    """Re-run one AMEn site step from a snapshot, as ``amen`` runs it.
+
+    Returns the step's intermediate results: ``b_loc``, the local solution
+    ``u``, the residual core ``z_new`` (with its two terms' norm
+    ``z_scale``), the truncation ``trace`` (spectrum, threshold, partial
+    norms, rank), ``core``/``carry``/``block`` and the cores of x and z at
+    ``k`` and its neighbour after the step.  Nothing in ``state`` changes.
+    """
+    k, d = state.k, A.d
+    x = Train(list(state.x))
+    x.orth_center = state.x_center
+    z = Train(list(state.z))
+    e = state.envs
+
+    def at(name):
+        lst = [None] * d
+        lst[k] = e[name]
+        return lst
+
+    out = {}
+    u, op, b_loc = _solve_site(k, x, A, b, at("A_L"), at("A_R"), at("b_L"), at("b_R"), settings, Info())
+    core3 = u.reshape(x.bonds[k], x.row_modes[k], x.bonds[k + 1])
+    z_new = residual_core(k, at("zb_L"), at("zb_R"), at("zA_L"), at("zA_R"), A, b, core3)
+    b_part = local_rhs(e["zb_L"], _c3(b.cores[k]), e["zb_R"])
+    out.update(b_loc=b_loc, u=u, z_new=z_new,
+               z_scale=float(np.linalg.norm(b_part) + np.linalg.norm(z_new.ravel() - b_part)))
+    trace = {}
+    if state.direction == "forward":
+        core, carry = truncate_core_right(core3, settings, error_threshold, max_rank, op, b_loc,
+                                          residual_damp, trace)
+        block = np.einsum("ba,bin->ain", e["xz_L"], z_new)
+        enrich_right(x, k, core, carry, block, max_rank)
+        shift_right(z, k, z_new)
+        nb = k + 1
+    elif k > 0:
+        core, carry = truncate_core_left(core3, settings, error_threshold, max_rank, op, b_loc,
+                                         residual_damp, trace)
+        block = _absorb_right(z_new, e["xz_R"])
+        enrich_left(x, k, core, carry, block, max_rank)
+        shift_left(z, k, z_new)
+        nb = k - 1
+    else:
+        return out
+    out.update(trace=trace, core=core, carry=carry, block=block, neighbour=nb,
+               x_k=x.cores[k], x_nb=x.cores[nb], z_k=z.cores[k], z_nb=z.cores[nb])
+    return out
+
+
 def amen(A: Train, b: Train, x0: Optional[Train] = None, kickrank=4, error_threshold=1e-7,
          max_rank=None, max_sweeps=1, residual_tol=None, residual_damp=2.0, stagnation_factor="auto",
          local_solver="auto", direct_solve_size=1000, local_rtol=None, allow_dense_fallback=False,
-         symmetric=False, seed=None, return_info=True):
-    """One-site AMEn (``amen.py:395-725``).  Returns ``(x, Info)``."""
+         symmetric=False, seed=None, return_info=True, hook=None):
+    """One-site AMEn (``amen.py:395-725``).  Returns ``(x, Info)``.
+
+    ``hook``, when given, is called before every site step with a
+    :class:`SiteState` snapshot of the full solver state at that step (x, z,
+    and every environment the step reads); it changes nothing in the run.
+    """
     if kickrank <= 0:
         raise ValueError(f"kickrank must be a positive integer, got {kickrank}")
     if max_rank is not None and max_rank <= 0:
@@ -1035,9 +1130,18 @@ def amen(A: Train, b: Train, x0: Optional[Train] = None, kickrank=4, error_thres
         build_env(zAR, k, "right", z, x, A)
         build_env(xzR, k, "right", x, z)
 
+    def snap(sweep, direction, k):
+        if hook is not None:
+            hook(SiteState(sweep, direction, k, list(x.cores), x.orth_center, list(z.cores),
+                           {n: e[k] for n, e in (("A_L", eA_L), ("A_R", eA_R), ("b_L", eb_L),
+                                                 ("b_R", eb_R), ("zb_L", zbL), ("zb_R", zbR),
+                                                 ("zA_L", zAL), ("zA_R", zAR), ("xz_L", xzL),
+                                                 ("xz_R", xzR))}))
+
     for sweep in range(st.max_sweeps):
         for k in range(d - 1):
             left_envs(k)
+            snap(sweep, "forward", k)
             u, op, b_loc = _solve_site(k, x, A, b, eA_L, eA_R, eb_L, eb_R, st, info)
             core3 = u.reshape(x.bonds[k], x.row_modes[k], x.bonds[k + 1])
             z_new = residual_core(k, zbL, zbR, zAL, zAR, A, b, core3)
@@ -1048,6 +1152,7 @@ def amen(A: Train, b: Train, x0: Optional[Train] = None, kickrank=4, error_thres
         left_envs(d - 1)
         for k in range(d - 1, -1, -1):
             right_envs(k)
+            snap(sweep, "backward", k)
             u, op, b_loc = _solve_site(k, x, A, b, eA_L, eA_R, eb_L, eb_R, st, info)
             core3 = u.reshape(x.bonds[k], x.row_modes[k], x.bonds[k + 1])
             z_new = residual_core(k, zbL, zbR, zAL, zAR, A, b, core3)

@@ -212,13 +212,50 @@ def test_als_lap1d_matches_reference():
     assert np.linalg.norm(x.full() - xd) / np.linalg.norm(xd) < 1e-8
 
 
+def _check_site_replay(states, A, b, settings):
+    """Every hooked state, replayed by ``site_step``, gives exactly the next
+    state's cores: the replay is the solver's own site step."""
+    steps = 0
+    for s, nxt in zip(states, states[1:]):
+        out = O.site_step(s, A, b, settings)
+        if "trace" not in out:
+            assert s.direction == "backward" and s.k == 0
+            continue
+        nb = out["neighbour"]
+        for got, want in ((out["x_k"], nxt.x[s.k]), (out["x_nb"], nxt.x[nb]),
+                          (out["z_k"], nxt.z[s.k]), (out["z_nb"], nxt.z[nb])):
+            assert got.shape == want.shape and np.array_equal(got, want), (s.sweep, s.direction, s.k)
+        steps += 1
+    return steps
+
+
 def test_amen_c1_poisson256_matches_reference():
     g = load_golden("c1_poisson256_amen")
     A = O.Train(P.poisson_2d_cores(8))
     b = O.Train(P.ones_cores([2] * 16))
     x0 = O.Train(P.random_train_cores([2] * 16, 2, 0))
-    x, info = O.amen(A, b, x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0)
+    states = []
+    x, info = O.amen(A, b, x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0,
+                     hook=states.append)
     _check_traj(x, info, g)
+    # the hook sees every site step of the golden run: 6 sweeps x (15 + 16)
+    assert len(states) == int(g["num_sweeps_run"]) * 31
+    st = O.resolve_settings(20, 1e-8, "auto", "auto", True, 1000, True, truncation_steps=30)
+    assert _check_site_replay(states, A, b, st) == len(states) - int(g["num_sweeps_run"])
+
+
+def test_site_replay_minres_path():
+    """The replay of the iterative (MINRES) local solves, on the 64 x 64 golden run."""
+    g = load_golden("poisson64_amen_minres")
+    A = O.Train(P.poisson_2d_cores(6))
+    b = O.Train(P.ones_cores([2] * 12))
+    x0 = O.Train(P.random_train_cores([2] * 12, 2, 0))
+    states = []
+    x, info = O.amen(A, b, x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0,
+                     direct_solve_size=16, hook=states.append)
+    _check_traj(x, info, g)
+    st = O.resolve_settings(20, 1e-8, "auto", "auto", True, 16, True, truncation_steps=22)
+    assert _check_site_replay(states, A, b, st) > 0
 
 
 def test_poisson_operator_is_the_kronecker_sum():
@@ -545,3 +582,30 @@ def test_qtt_sine_of_sum_keeps_exact_ranks():
     assert back.bonds == [1, 2, 1]
     assert _rel(back.full(), t.full()) < 1e-10
     assert max(O.tt_to_qtt(O.Train([c[:, ::4] for c in t.cores]), error=0.0).bonds) == 2
+
+
+def test_poisson_exact_solution_closed_form():
+    """The DST closed form used to pin AMEn solutions equals a dense solve of
+    the QTT operator (16 x 16 grid), and the stencil equals that operator."""
+    import poisson_exact as PE
+    A = O.Train(P.poisson_2d_cores(4))
+    M = A.dense_matrix()
+    rng = np.random.default_rng(1)
+    B = rng.standard_normal((16, 16))
+    np.testing.assert_allclose(PE.solve(B).reshape(-1), np.linalg.solve(M, B.reshape(-1)),
+                               rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(PE.apply(B).reshape(-1), M @ B.reshape(-1), rtol=1e-13, atol=1e-10)
+    ev = np.linalg.eigvalsh(M)
+    assert PE.lam_min(4) == pytest.approx(ev[0], rel=1e-12)
+
+
+def test_c1_reference_solution_within_residual_bound():
+    """The reference's own C1 solution (golden) is within ||Ax - b|| / lam_min
+    of the exact discrete solution: the bound the device tests apply."""
+    import poisson_exact as PE
+    g = load_golden("c1_poisson256_amen")
+    x = O.Train(unpack_cores(g)).full()
+    err, bound, nrm = PE.error_bound(x, np.ones(2**16), 8)
+    assert err <= bound * (1 + 1e-6)
+    assert bound <= 1e-8 * np.sqrt(2**16) / PE.lam_min(8) * 1.01  # residual 8.2e-9 <= 1e-8
+    assert err / nrm < 1e-7

@@ -1,9 +1,11 @@
 """Parity of the device hot path with the oracle and the reference-written goldens.
 
-Contractions (environments, local operator, local rhs, residual core) are
-compared with the oracle on seeded random trains; ``truncate`` with the
-reference's golden bonds and singular values; AMEn/ALS with the reference's
-golden trajectories.
+Contractions (environments, local operator, local rhs) are compared with the
+oracle on seeded random trains; ``truncate`` with the reference's golden
+bonds and singular values; AMEn/ALS with the reference's golden
+trajectories.  The AMEn site step itself (local solve, residual core,
+truncation and rank decision, enrichment) is compared on the oracle's
+in-situ states in ``test_site_parity_gpu.py``.
 """
 
 import numpy as np
@@ -196,18 +198,13 @@ def test_als_lap1d_matches_reference():
 
 
 def test_amen_c1_poisson256_converges_like_reference():
-    """C1: converged at 1e-8, solution within the residual-implied distance of the reference's.
-
-    The reference's own trajectory on C1 changes with BLAS round-off (bond 32
-    vs 33, 6 sweeps, even 'stagnated' with 8 OpenBLAS threads), so ranks are
-    compared within that band rather than exactly.
-    """
+    """C1 on the per-kernel path; see c1_check.check_c1_solution."""
+    from c1_check import check_c1_solution
     print("amen C1 poisson 256x256", flush=True)
     g = load_golden("c1_poisson256_amen")
     x, info = _amen_case(8)
-    assert info.residuals[-1] <= 1e-8 * 1.05
-    assert abs(max(x.bonds) - max(g["bonds"])) <= 2
-    assert _rel_diff(x, _oracle_solution(g)) < 1e-5
+    check_c1_solution(x.get_full_tensor(), info, g)
+    assert abs(max(x.bonds) - max(g["bonds"])) <= 1
 
 
 def test_truncate_c2_mixed_precision_vs_fp64():

@@ -129,7 +129,8 @@ def _read_decision(dec: _Decision, op, mode, max_rank, K):
 
 
 def _truncate_core_right(core3: torch.Tensor, mode, error_threshold, max_rank, op=None, b_loc=None,
-                         truncation_tol=None, residual_damp=2.0, stats: Optional[_SweepStats] = None):
+                         truncation_tol=None, residual_damp=2.0, stats: Optional[_SweepStats] = None,
+                         spectra: Optional[list] = None):
     """SVD split (r_l n x r_r) -> (U_r, S V_r) (``amen.py:227-275``)."""
     rl, n, rr = core3.shape
     m = rl * n
@@ -141,6 +142,8 @@ def _truncate_core_right(core3: torch.Tensor, mode, error_threshold, max_rank, o
         op.apply(P(core3), 1, P(tmp))
         D.diff_nrm2(N, P(tmp), P(b_loc), P(exact))
     U, S, Vt = D.svd(m, rr, P(core3), rr, 1, dec.buf, 0)
+    if spectra is not None:
+        spectra.append(S)
     K = min(m, rr)
     _choose_rank(mode, op, b_loc, U, S, Vt, m, rr, K, exact, error_threshold, max_rank,
                  truncation_tol, residual_damp, dec)
@@ -155,7 +158,8 @@ def _truncate_core_right(core3: torch.Tensor, mode, error_threshold, max_rank, o
 
 
 def _truncate_core_left(core3: torch.Tensor, mode, error_threshold, max_rank, op=None, b_loc=None,
-                        truncation_tol=None, residual_damp=2.0, stats: Optional[_SweepStats] = None):
+                        truncation_tol=None, residual_damp=2.0, stats: Optional[_SweepStats] = None,
+                        spectra: Optional[list] = None):
     """Mirror (``amen.py:278-322``): (V_r as (r, n, r_r), U_r S)."""
     rl, n, rr = core3.shape
     q = n * rr
@@ -167,6 +171,8 @@ def _truncate_core_left(core3: torch.Tensor, mode, error_threshold, max_rank, op
         op.apply(P(core3), 1, P(tmp))
         D.diff_nrm2(N, P(tmp), P(b_loc), P(exact))
     U, S, Vt = D.svd(rl, q, P(core3), q, 1, dec.buf, 0)
+    if spectra is not None:
+        spectra.append(S)
     K = min(rl, q)
     _choose_rank(mode, op, b_loc, U, S, Vt, rl, q, K, exact, error_threshold, max_rank,
                  truncation_tol, residual_damp, dec)
