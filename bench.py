@@ -391,11 +391,13 @@ def main():
     ap.add_argument("--c4", type=int, default=0, metavar="N",
                     help="instead of the C1 line, serve the C4 stream of N requests and print its "
                          "report (with the oracle FIFO estimate unless --no-cpu)")
+    ap.add_argument("--c4-requests", type=int, default=512,
+                    help="requests of the C4 serving leg of the full line (0 skips it)")
     ap.add_argument("--c4-cpu-per-level", type=int, default=2)
     ap.add_argument("--only", choices=("c2", "c3", "c5"), default=None,
                     help="instead of the C1 line, run one of the other measurements and print it")
-    ap.add_argument("--single", action="store_true",
-                    help="also time one request alone (its launches are not part of the roofline)")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip timing one request alone (its launches are not part of the roofline)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -433,7 +435,7 @@ def main():
     nsolved = len(ok)
 
     single_ms = None
-    if args.single:
+    if not args.no_single:
         # one request alone (latency floor of a single solve on this engine)
         single = Engine(1, rmax=RMAX)
         single.run(requests(A, b, x0, 1), poll=args.poll)
@@ -533,6 +535,13 @@ def main():
         except Exception as exc:  # the C1 line is reported whatever happens here
             line["c3_diffusion_3d"] = {"error": repr(exc)}
         print("# c3 done", file=sys.stderr, flush=True)
+    if args.c4_requests:
+        try:
+            line["c4_serving"] = c4_serving(args.c4_requests, not args.no_cpu,
+                                            args.c4_cpu_per_level)
+        except Exception as exc:  # the C1 line is reported whatever happens here
+            line["c4_serving"] = {"error": repr(exc)}
+        print("# c4 done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -61,3 +61,38 @@ def error_bound(x_full: np.ndarray, b_full: np.ndarray, n_bits: int):
     err = float(np.linalg.norm(X - exact))
     bound = float(np.linalg.norm(apply(X) - B)) / lam_min(n_bits)
     return err, bound, float(np.linalg.norm(exact))
+
+
+def diffusion_sparse(n_bits: int, theta: float):
+    """The operator of ``problems.diffusion_2d_cores(n_bits, theta)`` (C4's
+    ``-div(k grad u)``, ``k = 1 + theta sin(pi x) sin(pi y)``) as a sparse
+    matrix in the same index order: ``(T (x) I + I (x) T) / h^2 + theta (M (x) S
+    + S (x) M)`` with ``M = D^T diag(sin(pi x_edges)) D / h^2`` and
+    ``S = diag(sin(pi x_nodes))``.  ``M (x) S + S (x) M`` is positive
+    semidefinite for ``theta >= 0``, so ``lam_min`` of the Laplacian alone
+    bounds this operator's smallest eigenvalue from below."""
+    import scipy.sparse as sp
+    n = 2**n_bits
+    h = 1.0 / (n + 1)
+    x_nodes = (np.arange(n) + 1) * h
+    x_edges = (np.arange(n + 1) + 0.5) * h
+    s_nodes = np.sin(np.pi * x_nodes)
+    s_edges = np.sin(np.pi * x_edges)
+    T = sp.diags([-np.ones(n - 1), 2.0 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1]) / h**2
+    M = sp.diags([-s_edges[1:-1], s_edges[:-1] + s_edges[1:], -s_edges[1:-1]], [-1, 0, 1]) / h**2
+    S = sp.diags(s_nodes)
+    I = sp.identity(n)
+    return (sp.kron(T, I) + sp.kron(I, T) + theta * (sp.kron(M, S) + sp.kron(S, M))).tocsc()
+
+
+def diffusion_error_bound(x_full: np.ndarray, n_bits: int, theta: float):
+    """``(||x - A^-1 b||, ||A x - b|| / lam_min(-Laplace), ||A^-1 b||)`` for
+    C4's operator and ``b = ones``."""
+    import scipy.sparse.linalg as spla
+    A = diffusion_sparse(n_bits, theta)
+    b = np.ones(A.shape[0])
+    exact = spla.spsolve(A, b)
+    x = np.asarray(x_full).reshape(-1)
+    err = float(np.linalg.norm(x - exact))
+    bound = float(np.linalg.norm(A @ x - b)) / lam_min(n_bits)
+    return err, bound, float(np.linalg.norm(exact))

@@ -150,6 +150,52 @@ def test_serving_diffusion_alone_equals_in_stream():
         np.testing.assert_array_equal(a, b)
 
 
+def test_serving_c4_stream_matches_oracle():
+    """C4's parametric diffusion stream (``examples/parametric_diffusion.py:35-58``:
+    k = 1 + theta sin(pi x) sin(pi y), b = ones), here at L = 6..8, served with
+    continuous admission by one engine per level (EnginePool).  Each result is
+    compared with the oracle's AMEn on the same operator, right-hand side,
+    initial guess and seed, and with the exact discrete solution (sparse
+    direct solve): the same stop reason, sweeps within one, the error within
+    ||A x - b|| / lam_min for both, and the two solutions within 1e-9 relative
+    of each other."""
+    import poisson_exact as PE
+    from ttnumpy_amd.serving import ServingConfig, serve
+    print("serving C4 stream vs oracle", flush=True)
+    cfg = ServingConfig(n_requests=9, levels=(6, 7, 8), rate=100.0, nslots=3, rmax=64, poll=8,
+                        seed=5)
+    rep, done = serve(cfg)
+    print(rep, flush=True)
+    assert rep.errors == 0 and rep.solves == 9
+    worst = 0.0
+    for r in sorted(done, key=lambda r: r.tag.index):
+        it = r.tag
+        L = it.level
+        A = O.Train(P.diffusion_2d_cores(L, it.theta))
+        b = O.Train(P.ones_cores([2] * (2 * L)))
+        x0 = O.Train(P.random_train_cores([2] * (2 * L), 2, it.index))
+        xo, io = O.amen(A, b, x0, symmetric=True, residual_tol=cfg.residual_tol,
+                        max_sweeps=cfg.max_sweeps, seed=it.index)
+        xg = r.result.get_full_tensor().reshape(-1)
+        err, bound, nrm = PE.diffusion_error_bound(xg, L, it.theta)
+        err_o, bound_o, _ = PE.diffusion_error_bound(xo.full(), L, it.theta)
+        dist = float(np.linalg.norm(xg - xo.full().reshape(-1))) / nrm
+        worst = max(worst, dist)
+        print(f"  request {it.index} L={L} theta={it.theta:.3f}: gpu {r.info.stop_reason} "
+              f"{r.info.num_sweeps_run} sweeps res {r.info.residuals[-1]:.2e} bond "
+              f"{max(r.result.bonds)}; oracle {io.stop_reason} {io.num_sweeps_run} sweeps res "
+              f"{io.residuals[-1]:.2e} bond {max(xo.bonds)}; |x-x_exact| {err:.2e} (bound "
+              f"{bound:.2e}); |x_gpu-x_oracle|/|x| {dist:.2e}", flush=True)
+        assert str(r.info.stop_reason) == str(io.stop_reason)
+        if str(io.stop_reason) == "converged":
+            assert r.info.residuals[-1] <= cfg.residual_tol
+        assert abs(r.info.num_sweeps_run - io.num_sweeps_run) <= 1
+        assert err <= bound and err_o <= bound_o
+        assert dist * nrm <= bound + bound_o
+    print(f"worst relative distance gpu vs oracle {worst:.2e}", flush=True)
+    assert worst <= 1e-9
+
+
 def test_engine_mixed_precision_c1():
     """Mixed precision (FP32 Cholesky + FP64 refinement for direct solves of
     size >= 256): converges to the FP64 tolerance, FP64-verified, and lands on

@@ -609,3 +609,15 @@ def test_c1_reference_solution_within_residual_bound():
     assert err <= bound * (1 + 1e-6)
     assert bound <= 1e-8 * np.sqrt(2**16) / PE.lam_min(8) * 1.01  # residual 8.2e-9 <= 1e-8
     assert err / nrm < 1e-7
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.37, 0.9])
+def test_diffusion_sparse_operator_is_the_ttm(theta):
+    """C4's QTT operator (16 x 16 grid) equals the sparse finite-volume matrix the
+    device tests solve for the exact discrete solution, which is SPD with its
+    smallest eigenvalue above the Laplacian's."""
+    import poisson_exact as PE
+    A = O.Train(P.diffusion_2d_cores(4, theta)).dense_matrix()
+    S = PE.diffusion_sparse(4, theta).toarray()
+    np.testing.assert_allclose(A, S, rtol=1e-12, atol=1e-9)
+    assert np.linalg.eigvalsh(S)[0] >= PE.lam_min(4) * (1 - 1e-12)
