@@ -83,7 +83,8 @@ typedef struct ttx_slot {
   // 0 local rhs, 1 explicit matrix, 2 LU, 3 MINRES, 4 exact residual, 5 post, 6 check, 7 env,
   // inside post (sweep steps only): 8 residual core, 9 SVD, 10 rank choice, 11 enrichment
   // and QR, 12 z shift; inside the LU: 13 panels, 14 in-panel updates, 15 block rows,
-  // 16 trailing updates, 17 back substitution; 18-23 spare
+  // 16 trailing updates, 17 back substitution; mixed direct solves: 18 FP32 factor,
+  // 19 refinement; 20-23 spare
   int64_t clk[24];
 } ttx_slot;
 

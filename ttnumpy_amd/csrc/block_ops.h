@@ -1521,71 +1521,197 @@ __device__ __forceinline__ void block_chol32_factor(int64_t n, int64_t ld, float
   }
 }
 
-// x <- (L L^T)^{-1} x in FP32 with the factor from block_chol32_factor; x in
-// global memory (n floats).  lds: 64 x 65 + 64 floats.  Ends synced.
-__device__ __forceinline__ void chol32_apply_inverse(int64_t n, int64_t ld, const float* W, float* x,
-                                                     float* lds) {
+// Inverses of the 64 x 64 diagonal blocks of the FP32 Cholesky factor L
+// (row-major, row stride ld, lower triangle) into Dinv: block b at
+// Dinv + 4096 b, row-major 64 x 64, lower; a partial last block is padded
+// with the identity.  Two waves take a block (two blocks at a time with four
+// waves): column c of the inverse is formed by forward substitution on e_c
+// by two lanes of one wave, lane c % 32 holding rows 0..31 and lane
+// c % 32 + 32 rows 32..63 in 32 registers each; the pivot of each step is
+// passed between them by one shuffle, and every lane runs the same unrolled
+// steps (entries left of the diagonal stay zero).  lds: 2 * (64 * 65 + 64)
+// floats.  Ends synced.
+template <int NT>
+__device__ __forceinline__ void chol32_invert_diag(int64_t n, int64_t ld, const float* W, float* Dinv,
+                                                   float* lds) {
+  static_assert(NT == 256, "two blocks of two waves each");
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int nwave = blockDim.x >> 6;
-  float* T = lds;
-  float* xb = lds + 64 * 65;
-  // forward: L y = x, 64-row blocks from the top; rows below updated by rows
-  for (int64_t j0 = 0; j0 < n; j0 += 64) {
-    const int64_t j1 = (j0 + 64) < n ? (j0 + 64) : n;
-    const int64_t nbk = j1 - j0;
-    if (lane < nbk)
-      for (int64_t r = wave; r < nbk; r += nwave)
-        T[r * 65 + lane] = lane <= r ? W[(j0 + r) * ld + j0 + lane] : 0.0f;
-    __syncthreads();
-    if (wave == 0) {
-      const int64_t row = j0 + lane;
-      float yv = row < j1 ? x[row] : 0.0f;
-      for (int64_t j = j0; j < j1; ++j) {
-        const int jj = (int)(j - j0);
-        const float yj = __shfl(yv, jj, 64) / T[jj * 65 + jj];
-        if (row == j) yv = yj;
-        if (row > j && row < j1) yv -= yj * T[lane * 65 + jj];
+  const int nwave = NT >> 6;
+  const int per = nwave / 2;            // blocks at a time
+  const int slot = wave >> 1;           // which of them this wave works on
+  const int half = lane >> 5;           // rows 32 * half .. 32 * half + 31
+  const int col = (lane & 31) + 32 * (wave & 1);
+  const int64_t nblk = (n + 63) / 64;
+  float* T = lds + slot * (64 * 65 + 64);  // T[i * 65 + k] = L_bb[i][k]
+  float* rd = T + 64 * 65;                 // 1 / L_bb[k][k]
+  for (int64_t base = 0; base < nblk; base += per) {
+    const int64_t b = base + slot;
+    const int64_t j0 = 64 * b;
+    const int64_t nbk = b < nblk ? ((n - j0) < 64 ? (n - j0) : 64) : 0;
+    if (b < nblk) {
+      // 16 loads of this thread in flight, then their LDS stores (twice)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        float v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int r = (wave & 1) + 2 * (16 * h + j);
+          v[j] = (r < nbk && lane < nbk) ? (lane <= r ? W[(j0 + r) * ld + j0 + lane] : 0.0f)
+                                         : (r == lane ? 1.0f : 0.0f);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int r = (wave & 1) + 2 * (16 * h + j);
+          T[r * 65 + lane] = v[j];
+          if (lane == r) rd[r] = 1.0f / v[j];
+        }
       }
-      if (row < j1) { x[row] = yv; xb[lane] = yv; }
     }
     __syncthreads();
-    for (int64_t i = j1 + tid; i < n; i += blockDim.x) {
-      const float* wr = W + i * ld + j0;
-      float acc = 0.0f;
-      for (int64_t c = 0; c < nbk; ++c) acc += wr[c] * xb[c];
-      x[i] -= acc;
+    if (b < nblk) {
+      const float* Th = T + 32 * half * 65;  // this lane's rows
+      float x[32];
+#pragma unroll
+      for (int i = 0; i < 32; ++i) x[i] = (32 * half + i == col) ? 1.0f : 0.0f;
+#pragma unroll
+      for (int k = 0; k < 64; ++k) {
+        const int owner = k < 32 ? 0 : 1;
+        const float piv = x[k & 31] * rd[k];
+        const float xk = __shfl(piv, (lane & 31) + 32 * owner, 64);
+        if (half == owner) x[k & 31] = xk;
+#pragma unroll
+        for (int i = 0; i < 32; ++i)
+          if (32 * half + i > k) x[i] -= Th[i * 65 + k] * xk;
+      }
+      float* D = Dinv + b * 4096;
+#pragma unroll
+      for (int i = 0; i < 32; ++i) D[(32 * half + i) * 64 + col] = x[i];
     }
     __syncthreads();
   }
-  // back: L^T z = y, 64-row blocks from the bottom; rows above by columns
-  for (int64_t j1 = n; j1 > 0; j1 -= 64) {
-    const int64_t j0 = j1 - 64 > 0 ? j1 - 64 : 0;
-    const int64_t nbk = j1 - j0;
-    if (lane < nbk)
-      for (int64_t r = wave; r < nbk; r += nwave)
-        T[r * 65 + lane] = lane <= r ? W[(j0 + r) * ld + j0 + lane] : 0.0f;
-    __syncthreads();
-    if (wave == 0) {
-      const int64_t row = j0 + lane;
-      float yv = row < j1 ? x[row] : 0.0f;
-      for (int64_t j = j1 - 1; j >= j0; --j) {
-        const int jj = (int)(j - j0);
-        const float zj = __shfl(yv, jj, 64) / T[jj * 65 + jj];
-        if (row == j) yv = zj;
-        if (row < j) yv -= zj * T[jj * 65 + lane];
+}
+
+// x <- (L L^T)^{-1} x in FP32 with the factor from block_chol32_factor and
+// the diagonal-block inverses from chol32_invert_diag; x in global memory
+// (n floats, n <= 4096, so row offsets of 64 rows fit 32 bits).  Both triangular sweeps are left-looking over
+// 64-row blocks with the whole solution vector in LDS, so a block step is
+// one round trip to memory (its inverse block and its rows of L):
+//   forward  y_b = Dinv_b (x_b - L[b, <b] y_<b)        (row segments of L)
+//   back     z_b = Dinv_b^T (y_b - L[>b, b]^T z_>b)    (column blocks of L)
+// lds: 64 x 65 + 4 x 64 + 64 + n floats.  Ends synced.
+template <int NT>
+__device__ __forceinline__ void chol32_apply_inverse(int64_t n, int64_t ld, const float* W,
+                                                     const float* Dinv, float* x, float* lds) {
+  static_assert(NT == 256, "four waves: 16 rows of a block each, 4 threads per row");
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwave = NT >> 6;
+  const int nt = NT;
+  float* D = lds;               // 64 x 65: the block's inverse
+  float* part = D + 64 * 65;    // nwave x 64 partial sums
+  float* vin = part + 4 * 64;   // 64: the block's reduced right-hand side
+  float* y = vin + 64;          // n: the solution so far
+  const int q = tid & 3, rq = tid >> 2;  // 4 threads per row of the block product
+  const int64_t nblk = (n + 63) / 64;
+  for (int64_t i = tid; i < n; i += nt) y[i] = x[i];
+  __syncthreads();
+  // the 64 x 64 inverse block into D: each thread's 16 loads before its stores
+  auto stage = [&](const float* Db) {
+    float v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = Db[(wave + 4 * j) * 64 + lane];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) D[(wave + 4 * j) * 65 + lane] = v[j];
+  };
+  // forward: L y = x
+  for (int64_t b = 0; b < nblk; ++b) {
+    const int64_t j0 = 64 * b;
+    const int64_t nbk = (n - j0) < 64 ? (n - j0) : 64;
+    stage(Dinv + b * 4096);
+    // v_r = y_r - L[j0 + r, 0:j0] . y[0:j0]: wave w takes its 16 rows
+    // w + 4 j eight at a time (independent sums, so their loads are in
+    // flight together); its lanes stride along the rows
+    const float* wbase = W + (j0 + wave) * ld;
+    const int rs4 = (int)(4 * ld);  // n <= 4096 keeps 16 row steps in 32 bits
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      float acc[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] = 0.0f;
+      for (int c = lane; c < (int)j0; c += 64) {
+        const float yc = y[c];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int r = wave + 4 * (8 * h + j);
+          if (r < nbk) acc[j] += wbase[(8 * h + j) * rs4 + c] * yc;
+        }
       }
-      if (row < j1) { x[row] = yv; xb[lane] = yv; }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[j] += __shfl_xor(acc[j], o, 64);
+        const int r = wave + 4 * (8 * h + j);
+        if (lane == 0) vin[r] = r < nbk ? y[j0 + r] - acc[j] : 0.0f;
+      }
     }
     __syncthreads();
-    for (int64_t i = tid; i < j0; i += blockDim.x) {
+    for (int r = rq; r < 64; r += nt / 4) {
       float acc = 0.0f;
-      for (int64_t c = 0; c < nbk; ++c) acc += W[(j0 + c) * ld + i] * xb[c];
-      x[i] -= acc;
+#pragma unroll
+      for (int t = 0; t < 16; ++t) acc += D[r * 65 + 16 * q + t] * vin[16 * q + t];
+      acc += __shfl_xor(acc, 1, 64);
+      acc += __shfl_xor(acc, 2, 64);
+      if (q == 0 && r < nbk) y[j0 + r] = acc;
     }
     __syncthreads();
   }
+  // back: L^T z = y, blocks from the last
+  for (int64_t b = nblk - 1; b >= 0; --b) {
+    const int64_t j0 = 64 * b;
+    const int64_t nbk = (n - j0) < 64 ? (n - j0) : 64;
+    const int64_t j1 = j0 + nbk;
+    stage(Dinv + b * 4096);
+    // part[w][c] = sum over the rows i >= j1 that wave w takes of L[i][j0 + c] z_i,
+    // eight rows' loads in flight at a time
+    {
+      float acc[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] = 0.0f;
+      if (lane < nbk) {
+        for (int64_t i0 = j1 + wave; i0 < n; i0 += 8 * nwave) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int64_t i = i0 + j * nwave;
+            if (i < n) acc[j] += W[i * ld + j0 + lane] * y[i];
+          }
+        }
+      }
+      part[wave * 64 + lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) +
+                               ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    }
+    __syncthreads();
+    if (tid < 64) {
+      float acc = 0.0f;
+      for (int w = 0; w < nwave; ++w) acc += part[w * 64 + tid];
+      vin[tid] = tid < nbk ? y[j0 + tid] - acc : 0.0f;
+    }
+    __syncthreads();
+    // z = Dinv^T vin
+    for (int r = rq; r < 64; r += nt / 4) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 16; ++t) acc += D[(16 * q + t) * 65 + r] * vin[16 * q + t];
+      acc += __shfl_xor(acc, 1, 64);
+      acc += __shfl_xor(acc, 2, 64);
+      if (q == 0 && r < nbk) y[j0 + r] = acc;
+    }
+    __syncthreads();
+  }
+  for (int64_t i = tid; i < n; i += nt) x[i] = y[i];
+  __syncthreads();
 }
 
 }  // namespace ttx

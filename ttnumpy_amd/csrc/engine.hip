@@ -38,6 +38,11 @@ constexpr size_t kEngLds = 75 * 1024;
 // one request does not hold up the tick of every other request
 enum { kSolveNone = 0, kSolveChol = 1, kSolveLu = 2, kSolveMinres = 3, kSolveChol32 = 4 };
 constexpr int64_t kMixedMinN = 256;  // smaller direct solves stay in FP64
+// the FP32 triangular solves keep the solution vector in LDS beside 4480
+// floats of working space and index rows with 32-bit offsets
+// (chol32_apply_inverse)
+constexpr int64_t kMixedMaxN = 4096;
+static_assert(kMixedMaxN + 4480 <= (int64_t)(kEngLds / 4), "LDS of the FP32 triangular solves");
 
 __device__ __forceinline__ double* xcore(const ttx_slot* s, int64_t k) { return s->x + k * s->cap_core; }
 __device__ __forceinline__ double* zcore(const ttx_slot* s, int64_t k) { return s->z + k * s->cap_zcore; }
@@ -744,7 +749,9 @@ __device__ void solve_body(ttx_slot* s) {
     if (!iter) {
       if (N > s->direct_cap) { fail(s, TTX_SLOT_ERR_CAPACITY); return; }
       if (threadIdx.x == 0)
-        s->sstage = !s->chol ? kSolveLu : (s->mixed && N >= kMixedMinN ? kSolveChol32 : kSolveChol);
+        s->sstage = !s->chol ? kSolveLu
+                             : (s->mixed && N >= kMixedMinN && N <= kMixedMaxN ? kSolveChol32
+                                                                                : kSolveChol);
       return;
     }
     // first attempt, warm-started from the current core (one_site.py:151-153)
@@ -798,9 +805,18 @@ __device__ void solve_body(ttx_slot* s) {
 // (L L^T)^{-1} (b - A_loc u_i) with the residual through the matrix-free FP64
 // operator, until it stops shrinking; accepted once ||r|| <= kRefineAccept
 // ||b_loc||, else the FP64 Cholesky takes the step (same tick).
+// The refinement stops at ||r|| <= 1e-12 ||b_loc||: three orders below the
+// local tolerance the reference accepts from its iterative local solves
+// (local_rtol = 0.1 residual_tol = 1e-9, common.py:374-375), and far below
+// the residual-budget threshold 0.5 truncation_tol ||b_loc|| (~1e-9), so the
+// rank decisions it feeds are those of an exact solve.
 constexpr int kRefineMax = 8;
-constexpr double kRefineStop = 1e-14;
+constexpr double kRefineStop = 1e-12;
 constexpr double kRefineAccept = 1e-11;
+// the FP32 factor (N (N+1) floats), the right-hand side (N, padded to 64) and
+// the diagonal-block inverses (4096 per 64 rows) share the LU scratch of
+// 2 N (N+1) floats, which holds them for N >= 104
+static_assert(kMixedMinN >= 104, "mixed direct solves need N >= 104 for their scratch");
 
 __device__ void chol32_body(ttx_slot* s, double* lds) {
   __shared__ double red[16];
@@ -811,6 +827,7 @@ __device__ void chol32_body(ttx_slot* s, double* lds) {
   int64_t t0 = wall_clock64();
   float* W = reinterpret_cast<float*>(scr(s, TTX_SCR_LU));  // N (N+1) floats of 2 N (N+1)
   float* d32 = W + N * (N + 1);
+  float* Dinv = d32 + ((N + 63) / 64) * 64;  // 64 x 64 diagonal-block inverses of L
   const double* bloc = scr(s, TTX_SCR_BLOC);
   double* u = scr(s, TTX_SCR_U);
   double* r = scr(s, TTX_SCR_TMP);
@@ -821,6 +838,8 @@ __device__ void chol32_body(ttx_slot* s, double* lds) {
   stamp(s, 1, t0);
   block_chol32_factor<kEngThreads>(N, N + 1, W, reinterpret_cast<float*>(lds), &s_info);
   __syncthreads();
+  if (s_info == TTX_OK) chol32_invert_diag<kEngThreads>(N, N + 1, W, Dinv, reinterpret_cast<float*>(lds));
+  stamp(s, 18, t0);
   bool ok = false;
   int it = 0;
   double rn = 0.0;
@@ -835,7 +854,7 @@ __device__ void chol32_body(ttx_slot* s, double* lds) {
     for (it = 1; it <= kRefineMax; ++it) {
       for (int64_t i = threadIdx.x; i < N; i += blockDim.x) d32[i] = (float)r[i];
       __syncthreads();
-      chol32_apply_inverse(N, N + 1, W, d32, reinterpret_cast<float*>(lds));
+      chol32_apply_inverse<kEngThreads>(N, N + 1, W, Dinv, d32, reinterpret_cast<float*>(lds));
       for (int64_t i = threadIdx.x; i < N; i += blockDim.x) u[i] += (double)d32[i];
       __syncthreads();
       op.apply(u, 1, r, -1.0, 0.0);  // r = -A u
@@ -853,7 +872,7 @@ __device__ void chol32_body(ttx_slot* s, double* lds) {
     }
     if (it > kRefineMax) ok = rn <= kRefineAccept * bn;
   }
-  stamp(s, 2, t0);
+  stamp(s, 19, t0);
   if (!ok) {  // not SPD in FP32, or the refinement stalled: FP64 Cholesky this tick
     if (threadIdx.x == 0) {
       s->mixed_fallbacks += 1;
