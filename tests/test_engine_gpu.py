@@ -94,6 +94,37 @@ def test_engine_poisson64_minres_path():
     assert _rel_diff(out.result, O.Train(unpack_cores(g))) < 1e-6
 
 
+def test_engine_mixed_minres_path():
+    """Mixed precision on the MINRES path (64 x 64 Poisson, direct_solve_size=16:
+    every local solve iterative): the inner MINRES runs on the FP32 operator
+    (FP32 MFMA contractions), the outer FP64 correction meets the reference's
+    acceptance test ||A u - b|| <= local_rtol ||b||, so the request converges
+    to the FP64 tolerance (FP64-verified) and lands on the FP64 request's
+    solution within what the two residuals allow."""
+    import poisson_exact as PE
+    from ttnumpy_amd.engine import Engine
+    from ttnumpy_amd.solvers.common import relative_residual
+    print("engine mixed minres path", flush=True)
+    eng = Engine(2, rmax=40)
+    r64 = _poisson_request(6, direct_solve_size=16)
+    r32 = _poisson_request(6, direct_solve_size=16, precision="mixed")
+    eng.run([r64, r32], poll=8)
+    assert r64.error is None and r32.error is None, (r64.error, r32.error)
+    print(r64.info, r32.info, eng.mixed_solves, eng.mixed_refines, flush=True)
+    assert eng.mixed_solves >= r32.info.local_solves > 0
+    assert r32.info.local_inexact == 0
+    assert str(r32.info.stop_reason) == "converged"
+    res = float(relative_residual(r32.A, r32.b, r32.result))
+    assert res <= 1e-8
+    d = _rel_diff(r32.result, O.Train(r64.result.to_numpy()))
+    # both within ||A x - b|| / lam_min of the exact solution; relative to
+    # ||x||: cond(A) (r_32 + r_64)
+    bound = PE.kappa(6) * (res + r64.info.residuals[-1])
+    print("relative difference mixed vs fp64", d, "bound", bound, "fp64-verified residual", res,
+          flush=True)
+    assert d <= bound
+
+
 def test_engine_c1_poisson256():
     """C1 on the engine: the checks of c1_check.check_c1_solution
     (converged at 1e-8, 6 +- 1 sweeps, the exact discrete solution within the

@@ -99,6 +99,86 @@ __device__ __forceinline__ void bgemm(int64_t batch, int64_t M, int64_t N, int64
   }
 }
 
+// bgemm in single precision: FP32 operands and FP32 accumulation on
+// v_mfma_f32_16x16x4f32.  Its operand layouts are those of the f64
+// instruction (A lane l = A[l%16][l/16], B lane l = B[l/16][l%16]) but its
+// output is not: lane l element e is D[4 (l/16) + e][l%16] (f64: row
+// 4 e + l/16).  Same tiles, strides and load batching as bgemm.  No barrier
+// inside.
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void bgemm32(int64_t batch, int64_t M, int64_t N, int64_t K, float alpha,
+                                        const float* A, int64_t a_ms, int64_t a_ks, int64_t a_bs,
+                                        const float* B, int64_t b_ks, int64_t b_ns, int64_t b_bs,
+                                        float beta, float* C, int64_t c_ms, int64_t c_ns,
+                                        int64_t c_bs) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  const int r = lane & 15, kq = lane >> 4;
+  const int64_t tm = (M + 31) >> 5, tn = (N + 31) >> 5;
+  const int64_t per = tm * tn;
+  add_flops(2.0 * (double)batch * (double)M * (double)N * (double)K);
+  for (int64_t t = wave; t < batch * per; t += nw) {
+    const int64_t bt = t / per, tt = t % per;
+    const int64_t m0 = (tt / tn) << 5, n0 = (tt % tn) << 5;
+    const float* Ab = A + bt * a_bs;
+    const float* Bb = B + bt * b_bs;
+    float* Cb = C + bt * c_bs;
+    const int64_t ma = m0 + r, mb = m0 + 16 + r, na = n0 + r, nb = n0 + 16 + r;
+    const bool va = ma < M, vb = mb < M, wa = na < N, wb = nb < N;
+    floatx4_t c00 = {0.0f, 0.0f, 0.0f, 0.0f}, c01 = c00, c10 = c00, c11 = c00;
+    const float* pa0 = Ab + ma * a_ms + kq * a_ks;
+    const float* pa1 = Ab + mb * a_ms + kq * a_ks;
+    const float* pb0 = Bb + kq * b_ks + na * b_ns;
+    const float* pb1 = Bb + kq * b_ks + nb * b_ns;
+    const int64_t sa = 4 * a_ks, sb = 4 * b_ks;
+    for (int64_t k0 = 0; k0 < K; k0 += 8) {
+      float a0[2], a1[2], b0[2], b1[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const bool kin = k0 + 4 * q + kq < K;
+        a0[q] = (kin && va) ? pa0[q * sa] : 0.0f;
+        a1[q] = (kin && vb) ? pa1[q * sa] : 0.0f;
+        b0[q] = (kin && wa) ? pb0[q * sb] : 0.0f;
+        b1[q] = (kin && wb) ? pb1[q * sb] : 0.0f;
+      }
+      pa0 += 2 * sa;
+      pa1 += 2 * sa;
+      pb0 += 2 * sb;
+      pb1 += 2 * sb;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (k0 + 4 * q < K) {
+          c00 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[q], b0[q], c00, 0, 0, 0);
+          c01 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[q], b1[q], c01, 0, 0, 0);
+          c10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[q], b0[q], c10, 0, 0, 0);
+          c11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[q], b1[q], c11, 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t row0 = m0 + 4 * kq + e, row1 = row0 + 16;
+      if (row0 < M && wa) {
+        float* d = Cb + row0 * c_ms + na * c_ns;
+        *d = beta == 0.0f ? alpha * c00[e] : alpha * c00[e] + beta * (*d);
+      }
+      if (row0 < M && wb) {
+        float* d = Cb + row0 * c_ms + nb * c_ns;
+        *d = beta == 0.0f ? alpha * c01[e] : alpha * c01[e] + beta * (*d);
+      }
+      if (row1 < M && wa) {
+        float* d = Cb + row1 * c_ms + na * c_ns;
+        *d = beta == 0.0f ? alpha * c10[e] : alpha * c10[e] + beta * (*d);
+      }
+      if (row1 < M && wb) {
+        float* d = Cb + row1 * c_ms + nb * c_ns;
+        *d = beta == 0.0f ? alpha * c11[e] : alpha * c11[e] + beta * (*d);
+      }
+    }
+  }
+}
+
 // y = alpha * M x + beta * y for a row-major M (row stride ld); one wave per
 // row so the reads of a row coalesce.  No barrier inside.
 __device__ __forceinline__ void bgemv(int64_t rows, int64_t cols, double alpha, const double* M,

@@ -36,7 +36,8 @@ constexpr size_t kEngLds = 75 * 1024;
 // solve kernel to the Cholesky or LU kernel of the same tick; MINRES resumes
 // over ticks, slot.minres_slice iterations at a time, so that a long solve of
 // one request does not hold up the tick of every other request
-enum { kSolveNone = 0, kSolveChol = 1, kSolveLu = 2, kSolveMinres = 3, kSolveChol32 = 4 };
+enum { kSolveNone = 0, kSolveChol = 1, kSolveLu = 2, kSolveMinres = 3, kSolveChol32 = 4,
+       kSolveMinres32 = 5 };
 constexpr int64_t kMixedMinN = 256;  // smaller direct solves stay in FP64
 // the FP32 triangular solves keep the solution vector in LDS beside 4480
 // floats of working space and index rows with 32-bit offsets
@@ -132,6 +133,47 @@ struct LocOp {
     __syncthreads();
     bgemm(B, fl, nr * fr, rl * Ra, alpha, psi, 1, fl, 0, s2, nr * fr, 1, rl * Ra * nr * fr, beta,
           out, nr * fr, 1, fl * nr * fr);
+    __syncthreads();
+  }
+};
+
+// The one-site operator in FP32 for the inner solves of mixed requests:
+// FP32 copies of psi, A_k and phi, the three products of LocOp::apply on
+// FP32 MFMA (bgemm32, FP32 accumulation) through FP32 scratch.
+struct LocOp32 {
+  const float* psi;
+  const float* A;
+  const float* phi;
+  int64_t rl, Ra, fl, nr, nc, Rb, rr, fr;
+  float* s1;
+  float* s2;
+  __device__ void apply(const float* y, float* out) const {
+    bgemm32(1, rl * nc, Rb * fr, rr, 1.0f, y, rr, 1, 0, phi, Rb * fr, 1, 0, 0.0f, s1, Rb * fr, 1, 0);
+    __syncthreads();
+    bgemm32(rl, Ra * nr, fr, nc * Rb, 1.0f, A, nc * Rb, 1, 0, s1, fr, 1, nc * Rb * fr, 0.0f, s2, fr,
+            1, Ra * nr * fr);
+    __syncthreads();
+    bgemm32(1, fl, nr * fr, rl * Ra, 1.0f, psi, 1, fl, 0, s2, nr * fr, 1, 0, 0.0f, out, nr * fr, 1, 0);
+    __syncthreads();
+  }
+};
+
+// MINRES's operator for a mixed request: FP64 vectors in and out, the
+// product on LocOp32 (one column).  Ends synced.
+struct MixedOp {
+  LocOp32 op;
+  float* y32;
+  float* o32;
+  int64_t n;
+  __device__ void apply(const double* y, int64_t B, double* out, double alpha, double beta) const {
+    (void)B;  // MINRES applies one column at a time
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) y32[i] = (float)y[i];
+    __syncthreads();
+    op.apply(y32, o32);
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const double v = alpha * (double)o32[i];
+      out[i] = beta == 0.0 ? v : v + beta * out[i];
+    }
     __syncthreads();
   }
 };
@@ -304,7 +346,9 @@ __device__ int64_t shift_left_blk(double* core_k, double* core_km1, const double
 // it computes.
 enum {
   kMrBeta = 0, kMrOldb, kMrDbar, kMrEpsln, kMrPhibar, kMrTnorm2, kMrGmax, kMrGmin, kMrCs, kMrSn,
-  kMrBeta1, kMrItn, kMrIstop, kMrPr, kMrPw, kMrRtol, kMrMaxit, kMrAttempt, kMrTarget, kMrCount
+  kMrBeta1, kMrItn, kMrIstop, kMrPr, kMrPw, kMrRtol, kMrMaxit, kMrAttempt, kMrTarget,
+  kMrPhiStop,  // > 0: also stop once the residual estimate phibar <= st[kMrPhiStop] beta1
+  kMrCount
 };
 
 struct MrVec {
@@ -315,7 +359,8 @@ struct MrVec {
 };
 
 // begin an attempt from x0 (x0 may alias V's x); st[kMrRtol], st[kMrMaxit] set
-__device__ void minres_start(const LocOp& op, int64_t n, const double* b, const double* x0, double* V,
+template <class Op>
+__device__ void minres_start(const Op& op, int64_t n, const double* b, const double* x0, double* V,
                              double* red, double* st) {
   MrVec m(V, n, 0, 0);
   const double eps = DBL_EPSILON;
@@ -368,7 +413,8 @@ __device__ void minres_start(const LocOp& op, int64_t n, const double* b, const 
 }
 
 // up to `budget` iterations of the current attempt; returns the number done
-__device__ int64_t minres_run(const LocOp& op, int64_t n, double* V, double* red, double* st,
+template <class Op>
+__device__ int64_t minres_run(const Op& op, int64_t n, double* V, double* red, double* st,
                               int64_t budget, int64_t* iters) {
   constexpr int kB = 4;
   const double eps = DBL_EPSILON;
@@ -376,7 +422,7 @@ __device__ int64_t minres_run(const LocOp& op, int64_t n, double* V, double* red
   double beta = st[kMrBeta], oldb = st[kMrOldb], dbar = st[kMrDbar], epsln = st[kMrEpsln];
   double phibar = st[kMrPhibar], tnorm2 = st[kMrTnorm2], gmax = st[kMrGmax], gmin = st[kMrGmin];
   double cs = st[kMrCs], sn = st[kMrSn];
-  const double beta1 = st[kMrBeta1], rtol = st[kMrRtol];
+  const double beta1 = st[kMrBeta1], rtol = st[kMrRtol], phistop = st[kMrPhiStop];
   const int64_t maxiter = (int64_t)st[kMrMaxit];
   int64_t itn = (int64_t)st[kMrItn];
   int istop = (int)st[kMrIstop];
@@ -501,6 +547,7 @@ __device__ int64_t minres_run(const LocOp& op, int64_t n, double* V, double* red
       if (epsx >= beta1) istop = 3;
       if (test2 <= rtol) istop = 2;
       if (test1 <= rtol) istop = 1;
+      if (phistop > 0.0 && phibar <= phistop * beta1) istop = 1;
     }
     add_flops(18.0 * (double)n);
   }
@@ -705,6 +752,80 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_env_kernel(ttx_slot* slots
 // positive definite or the problem is not symmetric, to eng_lu_kernel (LU
 // with partial pivoting).  scal[0] receives the exact local residual.
 
+// ---- mixed iterative local solves
+// The inner solve runs MINRES on the correction equation A_32 d = r with
+// the operator applied in FP32 (MixedOp, FP32 copies of psi, A_k, phi; FP32
+// accumulation) until its residual estimate has fallen by
+// max(kMixedInnerRtol, target/2 ||r||) (SciPy's own test is relative to
+// ||A|| ||d|| and stops far too early on ill-conditioned systems); the outer loop adds d to u in FP64 and recomputes r = b_loc - A u
+// with the FP64 operator, until ||r|| meets the reference's acceptance test
+// (||r|| <= local_rtol ||b_loc||, common.py:787) or kMixedOuterMax passes,
+// the reference's budget of MINRES attempts before it accepts an inexact
+// solve (MAX_LOCAL_RESTARTS, common.py:779-800).
+// The environments stay FP64: the outer residual, the exact residual of the
+// rank rule and everything after the solve see the FP64 operator.
+constexpr int kMixedOuterMax = 4;
+constexpr double kMixedInnerRtol = 1e-5;
+enum { kMrOuter = kMrCount };
+
+// the FP32 operator of site k in the slot's M32 region into *m (the factors
+// are converted when fill is set); false when the region cannot hold it
+// (with m null: only that test)
+__device__ bool mixed_op(ttx_slot* s, int64_t k, MixedOp* mp, bool fill = false) {
+  const int64_t rl = s->rx[k], n = s->n[k], rr = s->rx[k + 1];
+  const int64_t Ra = s->ra[k], Rb = s->ra[k + 1];
+  const int64_t N = rl * n * rr;
+  const int64_t npsi = rl * Ra * rl, nphi = rr * Rb * rr, nA = Ra * n * n * Rb;
+  const int64_t ns1 = rl * n * Rb * rr, ns2 = rl * Ra * n * rr;
+  const int64_t have = 2 * (s->scr_size - s->scr_off[TTX_SCR_M32]);  // floats
+  if (!s->mixed || npsi + nphi + nA + ns1 + ns2 + 2 * N > have) return false;
+  if (mp == nullptr) return true;
+  MixedOp& m = *mp;
+  float* base = reinterpret_cast<float*>(scr(s, TTX_SCR_M32));
+  float* psi = base;
+  float* phi = psi + npsi;
+  float* A = phi + nphi;
+  m.op.psi = psi;
+  m.op.phi = phi;
+  m.op.A = A;
+  m.op.rl = rl;
+  m.op.Ra = Ra;
+  m.op.fl = rl;
+  m.op.nr = n;
+  m.op.nc = n;
+  m.op.Rb = Rb;
+  m.op.rr = rr;
+  m.op.fr = rr;
+  m.op.s1 = A + nA;
+  m.op.s2 = m.op.s1 + ns1;
+  m.y32 = m.op.s2 + ns2;
+  m.o32 = m.y32 + N;
+  m.n = N;
+  if (fill) {
+    const double* psi64 = envp(s, 0, 0, k);
+    const double* phi64 = envp(s, 0, 1, k);
+    const double* A64 = s->A[k];
+    for (int64_t i = threadIdx.x; i < npsi; i += blockDim.x) psi[i] = (float)psi64[i];
+    for (int64_t i = threadIdx.x; i < nphi; i += blockDim.x) phi[i] = (float)phi64[i];
+    for (int64_t i = threadIdx.x; i < nA; i += blockDim.x) A[i] = (float)A64[i];
+    __syncthreads();
+  }
+  return true;
+}
+
+// r = b_loc - A_64 u into r; returns ||r||
+__device__ double mixed_residual(const LocOp& op, const double* u, const double* bloc, double* r,
+                                 int64_t N, double* red) {
+  op.apply(u, 1, r, -1.0, 0.0);
+  double sq = 0.0;
+  for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+    const double ri = bloc[i] + r[i];
+    r[i] = ri;
+    sq += ri * ri;
+  }
+  return sqrt(block_sum(sq, red));
+}
+
 __device__ bool solve_step_active(const ttx_slot* s) {
   return s->status == TTX_SLOT_RUNNING && (s->phase == TTX_PH_FWD || s->phase == TTX_PH_BWD);
 }
@@ -738,7 +859,7 @@ __device__ void solve_body(ttx_slot* s) {
   double* st = scal + 8;  // MINRES state (kMrCount doubles)
   double* V = scr(s, TTX_SCR_MR);
   const int64_t stage = s->sstage;
-  if (stage != kSolveNone && stage != kSolveMinres) return;  // a direct solve is pending
+  if (stage != kSolveNone && stage != kSolveMinres) return;  // a direct or mixed solve is pending
   LocOp op = system_op(s, k);
   if (stage == kSolveNone) {
     local_rhs_blk(envp(s, 1, 0, k), s->rb[k], rl, s->b[k], n, s->rb[k + 1], envp(s, 1, 1, k), rr,
@@ -756,15 +877,26 @@ __device__ void solve_body(ttx_slot* s) {
     }
     // first attempt, warm-started from the current core (one_site.py:151-153)
     const double target = s->local_rtol * bnorm2(bloc, nullptr, N, red);
-    if (threadIdx.x == 0) {
+    if (s->mixed && mixed_op(s, k, nullptr)) {
+      // mixed: the FP32 inner solves run in eng_chol32_kernel
+      if (threadIdx.x == 0) {
+        st[kMrTarget] = target;
+        st[kMrOuter] = -1.0;  // not started
+        s->sstage = kSolveMinres32;
+      }
+      __syncthreads();
+      return;
+    } else if (threadIdx.x == 0) {
       st[kMrRtol] = s->local_rtol;
       st[kMrMaxit] = (double)(5 * N);
       st[kMrAttempt] = 0.0;
       st[kMrTarget] = target;
+      st[kMrPhiStop] = 0.0;  // SciPy's stopping tests only
       s->sstage = kSolveMinres;
     }
     minres_start(op, N, bloc, xcore(s, k), V, red, st);
   }
+
   // restarted MINRES with the true-residual check (common.py:776-800), at most
   // slot.minres_slice iterations in this tick
   int64_t budget = s->minres_slice > 0 ? s->minres_slice : 128;
@@ -892,12 +1024,98 @@ __device__ void chol32_body(ttx_slot* s, double* lds) {
   __syncthreads();
 }
 
+// The mixed iterative local solve (see kMixedOuterMax): started here on the
+// tick the solve kernel hands it over (slot.sstage = kSolveMinres32 with
+// st[kMrOuter] = -1), then resumed tick by tick.
+__device__ void minres32_body(ttx_slot* s) {
+  __shared__ double red[16];
+  if (!solve_step_active(s) || s->sstage != kSolveMinres32) return;
+  const int64_t k = s->k;
+  const int64_t N = s->rx[k] * s->n[k] * s->rx[k + 1];
+  int64_t t0 = wall_clock64();
+  double* u = scr(s, TTX_SCR_U);
+  const double* bloc = scr(s, TTX_SCR_BLOC);
+  double* tmp = scr(s, TTX_SCR_TMP);
+  double* scal = scr(s, TTX_SCR_SCAL);
+  double* st = scal + 8;
+  double* V = scr(s, TTX_SCR_MR);
+  const LocOp op = system_op(s, k);
+  MixedOp mop;
+  const bool start = st[kMrOuter] < 0.0;
+  mixed_op(s, k, &mop, start);  // its size was checked when the solve was handed over
+  if (start) {
+    // u = the warm start (the current core, one_site.py:151-153), r = b_loc - A_64 u
+    bcopy(u, xcore(s, k), N);
+    __syncthreads();
+    const double res = mixed_residual(op, u, bloc, tmp, N, red);
+    const double target = st[kMrTarget];
+    __syncthreads();  // every thread has read st before it changes
+    if (res <= target) {  // the warm start already meets the test
+      if (threadIdx.x == 0) {
+        s->local_solves += 1;
+        s->mixed_solves += 1;
+        scal[1] = 0.0;
+        scal[0] = s->residual_mode ? res : 0.0;
+        s->sstage = kSolveNone;
+      }
+      __syncthreads();
+      stamp(s, 3, t0);
+      return;
+    }
+    if (threadIdx.x == 0) {
+      st[kMrOuter] = 0.0;
+      st[kMrRtol] = 1e-15;  // the residual-estimate test below ends the inner solve
+      st[kMrPhiStop] = fmax(kMixedInnerRtol, 0.5 * target / res);
+      st[kMrMaxit] = (double)(5 * N);
+    }
+    minres_start(mop, N, tmp, nullptr, V, red, st);
+  }
+
+  // FP32 inner solves, FP64 corrections; at most slot.minres_slice inner
+  // iterations in this tick
+  int64_t budget = s->minres_slice > 0 ? s->minres_slice : 128;
+  while (true) {
+    budget -= minres_run(mop, N, V, red, st, budget, threadIdx.x == 0 ? &s->minres_iters : nullptr);
+    if (st[kMrIstop] == 0.0) break;  // out of budget: resume next tick
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) u[i] += V[i];
+    __syncthreads();
+    const double res = mixed_residual(op, u, bloc, tmp, N, red);
+    const double target = st[kMrTarget];
+    const int outer = (int)st[kMrOuter] + 1;
+    const bool ok = res <= target;
+    __syncthreads();  // everyone has read st before it is rewritten
+    if (ok || outer >= kMixedOuterMax) {
+      if (threadIdx.x == 0) {
+        s->local_solves += 1;
+        if (!ok) s->local_inexact += 1;
+        s->mixed_solves += 1;
+        s->mixed_refines += outer;
+        scal[1] = 0.0;
+        scal[0] = s->residual_mode ? res : 0.0;  // ||A_64 u - b_loc||, the exact residual
+        s->sstage = kSolveNone;
+      }
+      __syncthreads();
+      stamp(s, 3, t0);
+      return;
+    }
+    if (threadIdx.x == 0) {
+      st[kMrOuter] = (double)outer;
+      st[kMrPhiStop] = fmax(kMixedInnerRtol, 0.5 * target / res);
+    }
+  minres_start(mop, N, tmp, nullptr, V, red, st);
+    if (budget <= 0) break;
+  }
+  stamp(s, 3, t0);
+}
+
 __global__ __launch_bounds__(kEngThreads, 2) void eng_chol32_kernel(ttx_slot* slots) {
   extern __shared__ double lds[];
   ttx_slot* s = slots + blockIdx.x;
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
   chol32_body(s, lds);
+  __syncthreads();
+  minres32_body(s);
   __syncthreads();
   if (threadIdx.x == 0 && g_flops != 0.0) s->flops[2] += g_flops;
 }
@@ -1447,6 +1665,10 @@ extern "C" int ttx_engine_plan(ttx_slot* s) {
   sz[TTX_SCR_CHK] = chk_res > chk_rank ? chk_res : chk_rank;
   sz[TTX_SCR_ENVS] = 5 * 2 * s->cap_core * ramax;
   sz[TTX_SCR_SCAL] = 64;
+  // mixed requests: FP32 psi, phi, A_k, the two products' scratch and two vectors
+  sz[TTX_SCR_M32] = s->mixed ? (2 * s->cap_env[0] + ramax * nmax * nmax * ramax + 2 * cl * ramax +
+                                2 * cl + 1) / 2
+                             : 0;
   off = 0;
   for (int i = 0; i < TTX_SCR_COUNT; ++i) {
     s->scr_off[i] = off;

@@ -36,7 +36,7 @@ MAXSWEEPS = 64
 NKERNELS = 7  # per tick: env, solve (rhs + MINRES), chol32, chol, lu, post, check
 KERNEL_NAMES = ("eng_env_kernel", "eng_solve_kernel", "eng_chol32_kernel", "eng_chol_kernel",
                 "eng_lu_kernel", "eng_post_kernel", "eng_check_kernel")
-SCR_COUNT = 17
+SCR_COUNT = 18
 
 PH_XORTH, PH_ZORTH, PH_DONE = 1, 2, 14
 SLOT_RUNNING, SLOT_DONE, SLOT_EMPTY = 0, 1, -100
@@ -112,7 +112,9 @@ class Request:
     local_rtol: Optional[float] = None
     symmetric: bool = False
     seed: Optional[int] = None
-    precision: str = "fp64"  # "mixed": direct Cholesky solves (N >= 256) in FP32 + FP64 refinement
+    # "mixed": direct Cholesky solves (N >= 256) factor in FP32 with FP64 refinement, and
+    # MINRES solves run on the FP32 operator with FP64 defect correction
+    precision: str = "fp64"
     tag: object = None
     arrival: float = 0.0
     # filled by the engine
@@ -192,8 +194,10 @@ class Engine:
         d = len(A)
         if d > MAXD:
             raise ValueError(f"at most {MAXD} cores per request")
+        if req.precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {req.precision!r}")
         key = (tuple(A.row_modes), tuple(A.bonds), tuple(b.bonds), req.kickrank,
-               req.max_rank, int(req.direct_solve_size))
+               req.max_rank, int(req.direct_solve_size), req.precision)
         hit = self._plans.get(key)
         if hit is not None:
             s = Slot()
@@ -206,6 +210,8 @@ class Engine:
         s.direct_size = int(req.direct_solve_size)
         s.rmax = self.rmax
         s.chunk = self.chunk
+        # mixed requests get the scratch of the FP32 operator
+        s.mixed = 1 if req.precision == "mixed" else 0
         for i in range(d):
             s.n[i] = A.row_modes[i]
         for i in range(d + 1):
@@ -290,9 +296,6 @@ class Engine:
         s = self._plan(req)
         s.iterative = 1 if st.iterative_solver is not None else 0
         s.chol = 1 if req.symmetric else 0
-        if req.precision not in ("fp64", "mixed"):
-            raise ValueError(f"precision must be 'fp64' or 'mixed', got {req.precision!r}")
-        s.mixed = 1 if req.precision == "mixed" else 0
         s.residual_mode = 1 if st.residual_tol is not None else 0
         s.max_sweeps = st.max_sweeps
         s.track = 1 if st.track_residual else 0
