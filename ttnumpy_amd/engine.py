@@ -115,6 +115,9 @@ class Request:
     # "mixed": direct Cholesky solves (N >= 256) factor in FP32 with FP64 refinement, and
     # MINRES solves run on the FP32 operator with FP64 defect correction
     precision: str = "fp64"
+    # rank capacity of the request's slot (None: the engine's rmax); raised by
+    # the engine when a bond outgrows it
+    rank_capacity: Optional[int] = None
     tag: object = None
     arrival: float = 0.0
     # filled by the engine
@@ -129,8 +132,16 @@ class Engine:
     """A fixed set of slots on one GPU plus the continuous-batching loop."""
 
     def __init__(self, nslots: int, rmax: int, chunk: int = 8, hbm_reserve_bytes: int = 2 << 30,
-                 minres_slice: int = 128, graphs: bool = True):
+                 minres_slice: int = 128, graphs: bool = True, max_rank_capacity: int = 1024):
         self.lib = _lib.load()
+        # A request whose bond outgrows its slot's rank capacity is planned
+        # again with twice the capacity (up to max_rank_capacity, and never
+        # above its max_rank) and served again from its x0: the slot arena
+        # grows with the plan, and the solve is deterministic, so the result
+        # is the one a slot of the larger capacity gives from the start.  The
+        # reference's ranks are bounded only by max_rank (amen.py:441-451).
+        self.max_rank_capacity = int(max_rank_capacity)
+        self.regrown = 0
         # graphs: each poll's ticks replay one recorded HIP graph instead of
         # seven launches per tick (same kernels, same order, same results)
         self.graphs = bool(graphs)
@@ -196,8 +207,9 @@ class Engine:
             raise ValueError(f"at most {MAXD} cores per request")
         if req.precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {req.precision!r}")
+        cap = self.capacity(req)
         key = (tuple(A.row_modes), tuple(A.bonds), tuple(b.bonds), req.kickrank,
-               req.max_rank, int(req.direct_solve_size), req.precision)
+               req.max_rank, int(req.direct_solve_size), req.precision, cap)
         hit = self._plans.get(key)
         if hit is not None:
             s = Slot()
@@ -208,7 +220,7 @@ class Engine:
         s.kick = req.kickrank
         s.max_rank = 0 if req.max_rank is None else int(req.max_rank)
         s.direct_size = int(req.direct_solve_size)
-        s.rmax = self.rmax
+        s.rmax = cap
         s.chunk = self.chunk
         # mixed requests get the scratch of the FP32 operator
         s.mixed = 1 if req.precision == "mixed" else 0
@@ -267,6 +279,24 @@ class Engine:
             return make()[0]
         return self._cached(self._z0_images, (seed, tuple(n), kick, cap), make)
 
+    def capacity(self, req: Request) -> int:
+        """The rank capacity a slot for ``req`` is planned with."""
+        return int(req.rank_capacity) if req.rank_capacity is not None else self.rmax
+
+    def _regrow(self, req: Request) -> bool:
+        """After ``req`` failed on rank capacity: raise its capacity and reset it
+        for another admission; False when it cannot grow."""
+        cap = self.capacity(req)
+        limit = self.max_rank_capacity
+        if req.max_rank is not None:
+            limit = min(limit, int(req.max_rank))
+        if req.error != _STATUS_TEXT[-1] or cap >= limit:
+            return False
+        req.rank_capacity = min(2 * cap, limit)
+        req.result = req.info = req.error = None
+        self.regrown += 1
+        return True
+
     def slot_bytes_needed(self, req: Request) -> int:
         s = self._plan(req)
         d = s.d
@@ -312,8 +342,9 @@ class Engine:
         if x0 is None:
             from .tensor_train import ones
             x0 = ones(b.physical_dims(), ranks=b.bonds)
-        if max(x0.bonds) > self.rmax:
-            raise ValueError(f"x0 bond {max(x0.bonds)} exceeds the slot capacity {self.rmax}")
+        if max(x0.bonds) > self.capacity(req):
+            raise ValueError(f"x0 bond {max(x0.bonds)} exceeds the slot capacity "
+                             f"{self.capacity(req)}")
         # arena: x | z | env | scratch, kept by the slot across requests
         nx, nz = d * s.cap_core, d * s.cap_zcore
         need = nx + nz + s.env_size + s.scr_size
@@ -485,6 +516,9 @@ class Engine:
                     req = self.slot_req[i]
                     self._retire(i, push=False)
                     retired = True
+                    if self._regrow(req):
+                        pending.insert(0, req)
+                        continue
                     done.append(req)
                     if on_done is not None:
                         on_done(req)
@@ -548,6 +582,9 @@ class EnginePool:
                                 req = e.slot_req[j]
                                 e._retire(j, push=False)
                                 retired = True
+                                if e._regrow(req):
+                                    pending[i].insert(0, req)
+                                    continue
                                 done.append(req)
                                 if on_done is not None:
                                     on_done(req)
