@@ -140,6 +140,32 @@ def test_engine_c1_poisson256():
     assert abs(max(out.result.bonds) - max(g["bonds"])) <= 1
 
 
+def test_engine_rank_capacity_grows():
+    """A bond that outgrows the slot's rank capacity re-plans the slot with
+    twice the capacity (the reference bounds ranks only by max_rank,
+    amen.py:441-451) and serves the request again from its x0; the result is
+    the one a slot of that capacity gives from the start, and the oracle's."""
+    from ttnumpy_amd.engine import Engine
+    print("engine rank capacity growth", flush=True)
+    g = load_golden("poisson64_amen")
+    small = Engine(2, rmax=8)
+    r1 = _poisson_request(6)                 # max bond 24
+    r2 = _poisson_request(5, seed=3, max_rank=10)
+    small.run([r1, r2], poll=8)
+    print(small.regrown, r1.rank_capacity, r2.rank_capacity, r1.info, flush=True)
+    assert r1.error is None and r2.error is None, (r1.error, r2.error)
+    assert small.regrown >= 3 and r1.rank_capacity == 32 and r2.rank_capacity == 10
+    assert max(r2.result.bonds) <= 10
+    ref = _poisson_request(6)
+    Engine(1, rmax=32).run([ref], poll=8)
+    assert r1.info.residuals == ref.info.residuals and r1.result.bonds == ref.result.bonds
+    for a, b in zip(r1.result.to_numpy(), ref.result.to_numpy()):
+        np.testing.assert_array_equal(a, b)
+    assert str(r1.info.stop_reason) == "converged"
+    assert r1.info.num_sweeps_run == int(g["num_sweeps_run"])
+    assert _rel_diff(r1.result, O.Train(unpack_cores(g))) < 1e-7
+
+
 def test_engine_alone_equals_in_mix():
     """A request's result is bit-identical alone and batched with different requests."""
     from ttnumpy_amd.engine import Engine
