@@ -601,6 +601,36 @@ def test_svd_large_preconditioned_jacobi(m, n, case):
         assert kl == k - 30
     np.testing.assert_allclose(Uh[:, live].T @ Uh[:, live], np.eye(kl), atol=1e-12 * max(m, n))
     np.testing.assert_allclose(Vh[live] @ Vh[live].T, np.eye(kl), atol=1e-12 * max(m, n))
+    # every singular vector, those of zero singular values included (LAPACK's are orthonormal)
+    np.testing.assert_allclose(Uh.T @ Uh, np.eye(k), atol=1e-12 * max(m, n))
+    np.testing.assert_allclose(Vh @ Vh.T, np.eye(k), atol=1e-12 * max(m, n))
+
+
+@pytest.mark.parametrize("m,n", [(40, 12), (12, 40), (384, 128), (128, 384), (300, 160),
+                                 (160, 300), (7, 7)])
+def test_svd_exact_zero_singular_values_orthonormal(m, n):
+    """Exactly zero singular values (zero columns or rows): the singular vectors
+    that go with them are completed to orthonormal sets, as LAPACK's are, on
+    every path (one workgroup, CholeskyQR2 with its Householder fallback,
+    multi-workgroup Jacobi, wide and tall)."""
+    print(f"svd zero singular values {m}x{n}", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(5 * m + n)
+    k = min(m, n)
+    A = rng.standard_normal((m, n))
+    if m >= n:
+        A[:, 3:7] = 0.0
+    else:
+        A[3:7, :] = 0.0
+    info = torch.zeros(2, dtype=torch.int32, device=D.device())
+    U, S, Vt = D.svd(m, n, D.P(_t(A)), n, 1, info)
+    assert int(info[0].item()) == 0
+    Uh, Sh, Vh = _h(U), _h(S), _h(Vt)
+    assert np.count_nonzero(Sh == 0.0) == 4
+    np.testing.assert_allclose(Sh, np.linalg.svd(A, compute_uv=False), rtol=1e-11, atol=1e-13 * Sh[0])
+    np.testing.assert_allclose((Uh * Sh) @ Vh, A, atol=1e-13 * Sh[0] * max(m, n))
+    np.testing.assert_allclose(Uh.T @ Uh, np.eye(k), atol=1e-12 * max(m, n))
+    np.testing.assert_allclose(Vh @ Vh.T, np.eye(k), atol=1e-12 * max(m, n))
 
 
 def test_svd_large_nonfinite_reports():

@@ -112,6 +112,29 @@ def test_truncate_copy_and_left_leaning_centre():
     np.testing.assert_allclose(out2.get_full_tensor(), full, rtol=1e-10, atol=1e-12 * np.abs(full).max())
 
 
+@pytest.mark.parametrize("max_rank", [None, 3])
+def test_truncate_zero_train_keeps_orthonormal_cores(max_rank):
+    """A train with a zero core: every SVD of the rounding has only zero
+    singular values and keeps one (truncated_bond's floor, utils.py:22-69);
+    its singular vectors are orthonormal, as LAPACK's are, so every core but
+    the centre stays orthonormal, with the oracle's bonds and a zero tensor."""
+    print(f"tt truncate zero train max_rank={max_rank}", flush=True)
+    from oracle import tt_oracle as O
+    tx = _tx()
+    rng = np.random.default_rng(23)
+    modes, ranks = [3, 4, 2, 5, 3], [1, 3, 4, 4, 2, 1]
+    cores = [rng.standard_normal((ranks[i], modes[i], 1, ranks[i + 1])) for i in range(5)]
+    cores[2][:] = 0.0
+    out = tx.TensorTrain(cores).truncate(error=0.0, max_rank=max_rank)
+    ref = O.truncate(O.Train(cores), 0.0, max_rank)
+    assert out.bonds == ref.bonds == [1] * 6
+    assert out.orth_center == 0
+    got = out.to_numpy()
+    for k in range(1, 5):
+        _right_ok(got[k])
+    assert np.all(out.get_full_tensor() == 0.0)
+
+
 def test_add_with_randomized_truncation_passes_seed_and_options():
     """add(truncate=True, method="randomized", seed, options) rounds like
     truncate(method="randomized") with the same seed and options."""

@@ -649,6 +649,73 @@ __device__ __forceinline__ bool jacobi_sweeps(double* G, double* V, int Q, int* 
   return false;
 }
 
+// The columns c of X (rows x cols, X[r * rs + c * cs], rows >= cols) whose
+// singular value S[c] is exactly zero are replaced by unit vectors
+// orthogonal to every other column, so that an SVD with zero singular
+// values still has orthonormal singular vectors, as LAPACK's gesdd/gesvd
+// give (the sweeps here leave such columns zero).  The other columns must be
+// orthonormal; S is descending, so the zero ones come last.  For each zero
+// column, in order, the candidate is the e_j least inside the span of the
+// columns before it (the smallest sum_q X[j][q]^2; its distance from the span
+// is at least sqrt((rows - c) / rows)), orthogonalized twice and normalized.
+// Block-wide (red: blockDim.x / 64 doubles); a single read of S[cols - 1]
+// when no singular value is zero.  Ends synced.
+__device__ void complete_zero_columns(int64_t rows, int64_t cols, double* X, int64_t rs, int64_t cs,
+                                      const double* S, double* red) {
+  __shared__ double s_val[16];
+  __shared__ int64_t s_idx[16];
+  if (cols < 1 || rows < cols || !(S[cols - 1] == 0.0)) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nwave = blockDim.x >> 6;
+  int64_t c0 = cols;
+  while (c0 > 0 && S[c0 - 1] == 0.0) --c0;
+  for (int64_t c = c0; c < cols; ++c) {
+    // the least-covered row j: argmin over rows of sum_{q < c} X[j][q]^2
+    double best = INFINITY;
+    int64_t bj = rows;
+    for (int64_t j = tid; j < rows; j += blockDim.x) {
+      double sq = 0.0;
+      for (int64_t q = 0; q < c; ++q) {
+        const double v = X[j * rs + q * cs];
+        sq += v * v;
+      }
+      if (sq < best) { best = sq; bj = j; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int64_t oj = __shfl_xor(bj, off, 64);
+      if (ob < best || (ob == best && oj < bj)) { best = ob; bj = oj; }
+    }
+    __syncthreads();
+    if (lane == 0) { s_val[wave] = best; s_idx[wave] = bj; }
+    __syncthreads();
+    best = s_val[0];
+    bj = s_idx[0];
+    for (int w = 1; w < nwave; ++w)
+      if (s_val[w] < best || (s_val[w] == best && s_idx[w] < bj)) { best = s_val[w]; bj = s_idx[w]; }
+    // v = e_bj, orthogonalized twice against the columns before c, normalized
+    for (int64_t r = tid; r < rows; r += blockDim.x) X[r * rs + c * cs] = (r == bj) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass)
+      for (int64_t q = 0; q < c; ++q) {
+        double d = 0.0;
+        for (int64_t r = tid; r < rows; r += blockDim.x) d += X[r * rs + q * cs] * X[r * rs + c * cs];
+        d = block_sum(d, red);
+        for (int64_t r = tid; r < rows; r += blockDim.x) X[r * rs + c * cs] -= d * X[r * rs + q * cs];
+        __syncthreads();
+      }
+    double nn = 0.0;
+    for (int64_t r = tid; r < rows; r += blockDim.x) {
+      const double v = X[r * rs + c * cs];
+      nn += v * v;
+    }
+    nn = sqrt(block_sum(nn, red));
+    for (int64_t r = tid; r < rows; r += blockDim.x) X[r * rs + c * cs] /= nn;
+    __syncthreads();
+  }
+}
+
 // Thin SVD of one matrix by the whole block: QR of the tall side, then
 // one-sided Jacobi on the square factor (round-robin pairs, one 16-lane DPP
 // row per pair, the square factor in LDS when `lds` holds it), columns sorted
@@ -822,6 +889,11 @@ __device__ __forceinline__ void block_svd(const SvdArgs& p, double* work, double
     }
   for (int64_t o = tid; o < Q; o += blockDim.x) p.S[o] = sig[(int64_t)ordd[o]];
   __syncthreads();
+  if (ACCV) {
+    // zero singular values: complete U's columns and Vt's rows to orthonormal sets
+    complete_zero_columns(p.m, Q, p.U, p.u_rs, p.u_cs, p.S, red);
+    complete_zero_columns(p.n, Q, p.Vt, p.v_cs, p.v_rs, p.S, red);
+  }
 }
 
 // ---------------------------------------------------------------------------

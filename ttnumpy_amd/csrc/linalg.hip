@@ -929,6 +929,19 @@ __global__ void svdm_finish_kernel(int64_t Q, const double* G, const double* V, 
 
 __global__ void svdm_status_kernel(int32_t* info, int32_t code) { info[0] = code; }
 
+// Orthonormal completion of the singular vectors of exactly zero singular
+// values (complete_zero_columns on U's columns and Vt's rows); returns after
+// one read of S when none is zero.
+__global__ __launch_bounds__(1024) void svd_complete_kernel(int64_t m, int64_t n, double* U,
+                                                            int64_t u_rs, int64_t u_cs,
+                                                            const double* S, double* Vt,
+                                                            int64_t v_rs, int64_t v_cs) {
+  __shared__ double red[16];
+  const int64_t k = m < n ? m : n;
+  complete_zero_columns(m, k, U, u_rs, u_cs, S, red);
+  complete_zero_columns(n, k, Vt, v_cs, v_rs, S, red);
+}
+
 static int svd_multi_jacobi(ttx_stream_t stream, int64_t m, int64_t n, const double* A,
                             int64_t a_rs, int64_t a_cs, double* U, int64_t u_rs, int64_t u_cs,
                             double* S, double* Vt, int64_t v_rs, int64_t v_cs, double* work,
@@ -998,8 +1011,14 @@ static int svd_multi_jacobi(ttx_stream_t stream, int64_t m, int64_t n, const dou
   if ((rc = launch_status()) != TTX_OK) return rc;
   // U_X = Qx Vs (P x Q)
   if (!wide)
-    return ttx_gemm_f64(stream, 1, P, Q, Q, 1.0, Qx, Q, 1, 0, Vs, Q, 1, 0, 0.0, U, u_rs, u_cs, 0);
-  return ttx_gemm_f64(stream, 1, P, Q, Q, 1.0, Qx, Q, 1, 0, Vs, Q, 1, 0, 0.0, Vt, v_cs, v_rs, 0);
+    rc = ttx_gemm_f64(stream, 1, P, Q, Q, 1.0, Qx, Q, 1, 0, Vs, Q, 1, 0, 0.0, U, u_rs, u_cs, 0);
+  else
+    rc = ttx_gemm_f64(stream, 1, P, Q, Q, 1.0, Qx, Q, 1, 0, Vs, Q, 1, 0, 0.0, Vt, v_cs, v_rs, 0);
+  if (rc != TTX_OK) return rc;
+  // the finish kernel leaves zero singular vectors for zero singular values
+  hipLaunchKernelGGL(svd_complete_kernel, dim3(1), dim3(1024), 0, st, m, n, U, u_rs, u_cs, S, Vt,
+                     v_rs, v_cs);
+  return launch_status();
 }
 
 extern "C" int64_t ttx_svd_work_size(int64_t m, int64_t n) {
