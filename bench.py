@@ -206,6 +206,33 @@ def c5_high_dim(with_cpu: bool):
                          "rmax": 64}
     else:
         res["engine"] = {"error": r.error}
+    # the mixed form on the same engine: MINRES local solves on the FP32 operator
+    # (FP32 MFMA contractions) with FP64 defect correction, FP32 Cholesky with FP64
+    # refinement for the direct ones; residual check, QR/SVD and rank rule in FP64
+    from ttnumpy_amd.solvers.common import relative_residual
+    engm = Engine(1, rmax=64)
+    for rep in range(2):
+        reqm = Request(A=A, b=b, max_rank=128, residual_tol=1e-8, max_sweeps=30, symmetric=True,
+                       seed=0, precision="mixed")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rm = engm.run([reqm])[0]
+        torch.cuda.synchronize()
+        wallm = time.perf_counter() - t0
+    if rm.error is None and r.error is None:
+        from oracle import tt_oracle as O
+        x64, xm = O.Train(r.result.to_numpy()), O.Train(rm.result.to_numpy())
+        res["engine_mixed"] = {
+            "gpu_s": wallm, "ratio_vs_fp64": wall / wallm, "stop_reason": str(rm.info.stop_reason),
+            "sweeps": rm.info.num_sweeps_run, "max_bond": max(rm.result.bonds),
+            "residual_fp64_verified": float(relative_residual(A, b, rm.result)),
+            "residual_fp64_verified_fp64_path": float(relative_residual(A, b, r.result)),
+            "relative_difference_vs_fp64_solution": float(
+                O.tt_norm(O.tt_add(xm, O.tt_scale(x64, -1.0))) / O.tt_norm(x64)),
+            "local_solves": rm.info.local_solves, "minres_iters": engm.minres_iters // 2,
+            "minres_iters_fp64": eng.minres_iters // 2}
+    else:
+        res["engine_mixed"] = {"error": rm.error or r.error}
     if with_cpu:
         from threadpoolctl import threadpool_limits
         from oracle import tt_oracle as O
@@ -469,7 +496,9 @@ def main():
     mixed = None
     if not args.no_mixed:
         # the mixed-precision engine on the same batches: FP32 Cholesky with FP64
-        # refinement for the direct local solves (N >= 256); everything else FP64
+        # refinement for the direct local solves (N >= 256), MINRES on the FP32
+        # operator with FP64 defect correction for the iterative ones; the
+        # environments, residual check, QR/SVD and rank rule stay FP64
         engm = Engine(args.batch, rmax=RMAX)
         mval, mlat, mdone, _ = timed_batches(engm, A, b, x0, args, "mixed")
         mok = [r for r in mdone if r.error is None]
@@ -488,7 +517,21 @@ def main():
             diff = O.tt_norm(O.tt_add(xm, O.tt_scale(x64, -1.0))) / O.tt_norm(x64)
             mixed.update({"final_residual": m0.info.residuals[-1], "sweeps": m0.info.num_sweeps_run,
                           "stop_reason": str(m0.info.stop_reason), "verify": mv,
-                          "relative_difference_vs_fp64_solution": float(diff)})
+                          "relative_difference_vs_fp64_solution": float(diff),
+                          "mixed_minres_iters_per_solve": engm.minres_iters / max(len(mdone), 1)})
+            if not args.no_cpu:
+                # distance to the oracle's FP64 solution of C1 (one BLAS thread, the
+                # trajectory of the reference's own C1 run)
+                from threadpoolctl import threadpool_limits
+                from ttnumpy_amd import problems as P
+                with threadpool_limits(1):
+                    xo, _ = O.amen(O.Train(P.poisson_2d_cores(N_BITS)),
+                                   O.Train(P.ones_cores([2] * (2 * N_BITS))),
+                                   O.Train(x0.to_numpy()), symmetric=True, residual_tol=1e-8,
+                                   max_sweeps=20, seed=0)
+                for key, xx in (("mixed", xm), ("fp64", x64)):
+                    mixed[f"relative_difference_vs_oracle_solution_{key}"] = float(
+                        O.tt_norm(O.tt_add(xx, O.tt_scale(xo, -1.0))) / O.tt_norm(xo))
     line = {
         "metric": "AMEn solves/s + p99 latency, mixed vs FP64 on same GPU at residual 1e-8",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -526,6 +569,8 @@ def main():
     if not args.no_c5:
         try:
             line["c5_high_dim"] = c5_high_dim(not args.no_cpu)
+            if mixed is not None:
+                mixed["c5_engine_one_slot"] = line["c5_high_dim"].get("engine_mixed")
         except Exception as exc:  # the C1 line is reported whatever happens here
             line["c5_high_dim"] = {"error": repr(exc)}
         print("# c5 done", file=sys.stderr, flush=True)
