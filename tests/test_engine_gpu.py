@@ -125,6 +125,34 @@ def test_engine_mixed_minres_path():
     assert d <= bound
 
 
+def test_engine_dense_fallback_matches_oracle():
+    """allow_dense_fallback (common.py:793-809): when the four MINRES attempts
+    miss local_rtol (here 1e-15, below what MINRES reaches), the engine solves
+    the explicit local matrix instead, as the reference does; the solve
+    converges, counts its fallbacks, and lands on the oracle's solution
+    (which takes the same fallbacks) within the residual bound."""
+    import poisson_exact as PE
+    from ttnumpy_amd.engine import Engine
+    print("engine dense fallback", flush=True)
+    kw = dict(direct_solve_size=16, local_rtol=1e-15, allow_dense_fallback=True)
+    req = _poisson_request(6, **kw)
+    Engine(1, rmax=40).run([req], poll=8)
+    assert req.error is None, req.error
+    A = O.Train(P.poisson_2d_cores(6))
+    b = O.Train(P.ones_cores([2] * 12))
+    x0 = O.Train(P.random_train_cores([2] * 12, 2, 0))
+    xo, io = O.amen(A, b, x0, symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=0, **kw)
+    print(req.info, io, flush=True)
+    assert req.info.local_fallbacks > 0 and io.local_fallbacks > 0
+    assert req.info.local_inexact == 0 and io.local_inexact == 0
+    assert str(req.info.stop_reason) == str(io.stop_reason) == "converged"
+    assert req.info.residuals[-1] <= 1e-8
+    d = _rel_diff(req.result, xo)
+    bound = PE.kappa(6) * (req.info.residuals[-1] + io.residuals[-1])
+    print("relative difference vs oracle", d, "bound", bound, flush=True)
+    assert d <= bound
+
+
 def test_engine_c1_poisson256():
     """C1 on the engine: the checks of c1_check.check_c1_solution
     (converged at 1e-8, 6 +- 1 sweeps, the exact discrete solution within the

@@ -908,6 +908,19 @@ __device__ void solve_body(ttx_slot* s) {
     const double target = st[kMrTarget];
     const int attempt = (int)st[kMrAttempt];
     const bool ok = res <= target;
+    if (!ok && attempt >= 3 && s->allow_fallback && N <= s->direct_cap) {
+      // the explicit-matrix fallback (common.py:793-809): the direct kernels
+      // of this tick solve the system on the explicit local matrix
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        s->local_solves += 1;
+        s->local_fallbacks += 1;
+        s->sstage = s->chol ? kSolveChol : kSolveLu;
+      }
+      __syncthreads();
+      stamp(s, 3, t0);
+      return;
+    }
     if (ok || attempt >= 3) {
       bcopy(u, V, N);
       if (threadIdx.x == 0) {
@@ -1084,6 +1097,18 @@ __device__ void minres32_body(ttx_slot* s) {
     const int outer = (int)st[kMrOuter] + 1;
     const bool ok = res <= target;
     __syncthreads();  // everyone has read st before it is rewritten
+    if (!ok && outer >= kMixedOuterMax && s->allow_fallback && N <= s->direct_cap) {
+      // the explicit-matrix fallback (common.py:793-809) in FP64
+      if (threadIdx.x == 0) {
+        s->local_solves += 1;
+        s->local_fallbacks += 1;
+        s->mixed_refines += outer;
+        s->sstage = kSolveChol;
+      }
+      __syncthreads();
+      stamp(s, 3, t0);
+      return;
+    }
     if (ok || outer >= kMixedOuterMax) {
       if (threadIdx.x == 0) {
         s->local_solves += 1;
@@ -1625,7 +1650,9 @@ extern "C" int ttx_engine_plan(ttx_slot* s) {
   s->cap_core = rmax * nmax * rmax;
   s->cap_zcore = kick * nmax * kick;
   s->cap_loc = s->cap_core;
-  s->direct_cap = s->direct_size < s->cap_loc ? s->direct_size : s->cap_loc;
+  // the explicit-matrix fallback of an iterative solve (allow_fallback) may
+  // factor a local system of any size the slot holds
+  s->direct_cap = (s->allow_fallback || s->direct_size > s->cap_loc) ? s->cap_loc : s->direct_size;
   if (s->direct_cap < 1) s->direct_cap = 1;
   const int64_t zr = kick > rmax ? kick : rmax;
   s->cap_env[0] = rmax * ramax * rmax;

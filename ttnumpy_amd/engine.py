@@ -110,6 +110,7 @@ class Request:
     local_solver: str = "auto"
     direct_solve_size: int = 1000
     local_rtol: Optional[float] = None
+    allow_dense_fallback: bool = False
     symmetric: bool = False
     seed: Optional[int] = None
     # "mixed": direct Cholesky solves (N >= 256) factor in FP32 with FP64 refinement, and
@@ -209,7 +210,8 @@ class Engine:
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {req.precision!r}")
         cap = self.capacity(req)
         key = (tuple(A.row_modes), tuple(A.bonds), tuple(b.bonds), req.kickrank,
-               req.max_rank, int(req.direct_solve_size), req.precision, cap)
+               req.max_rank, int(req.direct_solve_size), req.precision, cap,
+               bool(req.allow_dense_fallback))
         hit = self._plans.get(key)
         if hit is not None:
             s = Slot()
@@ -224,6 +226,8 @@ class Engine:
         s.chunk = self.chunk
         # mixed requests get the scratch of the FP32 operator
         s.mixed = 1 if req.precision == "mixed" else 0
+        # the explicit-matrix fallback needs the direct solve's buffer for any local size
+        s.allow_fallback = 1 if req.allow_dense_fallback else 0
         for i in range(d):
             s.n[i] = A.row_modes[i]
         for i in range(d + 1):
@@ -320,7 +324,8 @@ class Engine:
             max_sweeps=req.max_sweeps, residual_tol=req.residual_tol,
             stagnation_factor=req.stagnation_factor, local_solver=req.local_solver,
             symmetric=req.symmetric, direct_solve_size=req.direct_solve_size, return_info=True,
-            local_rtol=req.local_rtol, truncation_steps=max(2 * (d - 1), 1))
+            local_rtol=req.local_rtol, allow_dense_fallback=req.allow_dense_fallback,
+            truncation_steps=max(2 * (d - 1), 1))
         if st.iterative_solver is not None and str(st.iterative_solver) == "gmres":
             raise NotImplementedError("the engine's iterative local solver is MINRES (symmetric=True)")
         s = self._plan(req)
