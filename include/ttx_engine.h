@@ -41,13 +41,14 @@ enum ttx_slot_status {
 enum {
   TTX_SCR_U = 0, TTX_SCR_BLOC, TTX_SCR_TMP, TTX_SCR_ZNEW, TTX_SCR_MAT, TTX_SCR_LU, TTX_SCR_MR,
   TTX_SCR_S1, TTX_SCR_S2, TTX_SCR_SVDW, TTX_SCR_USVD, TTX_SCR_SSVD, TTX_SCR_VSVD, TTX_SCR_ENR,
-  TTX_SCR_CHK, TTX_SCR_ENVS, TTX_SCR_SCAL, TTX_SCR_M32, TTX_SCR_COUNT
+  TTX_SCR_CHK, TTX_SCR_ENVS, TTX_SCR_SCAL, TTX_SCR_M32, TTX_SCR_GMRES, TTX_SCR_COUNT
 };
 
 typedef struct ttx_slot {
   // ---- problem (host-written)
   int64_t d, kick, max_rank, direct_size, iterative, residual_mode, max_sweeps, track;
   int64_t rmax, nmax, ramax, rbmax, bneg, allow_fallback, chunk;
+  // (iterative: 0 direct local solves only, 1 MINRES, 2 GMRES above direct_size)
   int64_t chol;  // 1: direct solves try Cholesky first (symmetric problems), LU if not SPD
   int64_t mixed; // 1: direct Cholesky solves of size >= 256 factor in FP32 with FP64 refinement,
                  //    MINRES solves run on the FP32 operator with FP64 defect correction

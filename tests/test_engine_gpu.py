@@ -153,6 +153,44 @@ def test_engine_dense_fallback_matches_oracle():
     assert d <= bound
 
 
+def test_engine_gmres_nonsymmetric_matches_device_path_and_oracle():
+    """symmetric=False with iterative local solves (convection-diffusion, the
+    operator of test_api_gpu's GMRES test): the engine runs SciPy's restarted
+    GMRES (gmres_blk.h, the routines of the per-kernel path's GMRES kernels)
+    with the reference's four attempts (common.py:776-809).  It converges as
+    the per-kernel path and the oracle (scipy's gmres) do, and all three land
+    within the residual-implied distance of the dense solution."""
+    import ttnumpy_amd as tx
+    from test_api_gpu import _convection_diffusion_cores
+    from ttnumpy_amd.engine import Engine, Request
+    print("engine gmres nonsymmetric", flush=True)
+    L = 6
+    A_c = _convection_diffusion_cores(L, 20.0)
+    b_c = P.ones_cores([2] * L)
+    kw = dict(residual_tol=1e-8, max_sweeps=10, symmetric=False, direct_solve_size=4, max_rank=16,
+              seed=0)
+    x0 = tx.rand([2] * L, ranks=2, seed=0)
+    req = Request(A=_tt(A_c), b=_tt(b_c), x0=x0, **kw)
+    eng = Engine(1, rmax=32)
+    eng.run([req], poll=8)
+    assert req.error is None, req.error
+    x, info = tx.amen(_tt(A_c), _tt(b_c), x0=x0, return_info=True, **kw)
+    xo, io = O.amen(O.Train(A_c), O.Train(b_c), O.Train(x0.to_numpy()), **kw)
+    print(req.info, info, io, eng.minres_iters, flush=True)
+    Ad = O.Train(A_c).dense_matrix()
+    ref = np.linalg.solve(Ad, np.ones(2**L))
+    cond = np.linalg.cond(Ad)
+    assert req.info.local_solves > 0 and eng.minres_iters > 0
+    for stop, res in ((req.info.stop_reason, req.info.residuals), (info.stop_reason, info.residuals),
+                      (io.stop_reason, io.residuals)):
+        assert str(stop) in ("converged", "stagnated")
+        assert res[-1] <= 3e-8
+    assert abs(req.info.num_sweeps_run - io.num_sweeps_run) <= 2
+    assert abs(req.info.num_sweeps_run - info.num_sweeps_run) <= 2
+    xe = req.result.get_full_tensor().reshape(-1)
+    assert np.linalg.norm(xe - ref) <= 3e-8 * cond * np.linalg.norm(ref)
+
+
 def test_engine_c1_poisson256():
     """C1 on the engine: the checks of c1_check.check_c1_solution
     (converged at 1e-8, 6 +- 1 sweeps, the exact discrete solution within the

@@ -22,6 +22,7 @@
 // reference lines it follows.
 
 #include "block_ops.h"
+#include "gmres_blk.h"
 #include "ttx_engine.h"
 
 namespace ttx {
@@ -37,7 +38,7 @@ constexpr size_t kEngLds = 75 * 1024;
 // over ticks, slot.minres_slice iterations at a time, so that a long solve of
 // one request does not hold up the tick of every other request
 enum { kSolveNone = 0, kSolveChol = 1, kSolveLu = 2, kSolveMinres = 3, kSolveChol32 = 4,
-       kSolveMinres32 = 5 };
+       kSolveMinres32 = 5, kSolveGmres = 6 };
 constexpr int64_t kMixedMinN = 256;  // smaller direct solves stay in FP64
 // the FP32 triangular solves keep the solution vector in LDS beside 4480
 // floats of working space and index rows with 32-bit offsets
@@ -877,12 +878,17 @@ __device__ void solve_body(ttx_slot* s) {
     }
     // first attempt, warm-started from the current core (one_site.py:151-153)
     const double target = s->local_rtol * bnorm2(bloc, nullptr, N, red);
-    if (s->mixed && mixed_op(s, k, nullptr)) {
-      // mixed: the FP32 inner solves run in eng_chol32_kernel
+    // GMRES (common.py:786) runs in eng_lu_kernel, the mixed FP32 inner
+    // solves in eng_chol32_kernel
+    const int64_t hand = s->iterative == 2 ? kSolveGmres
+                         : (s->mixed && mixed_op(s, k, nullptr)) ? kSolveMinres32 : kSolveNone;
+    if (hand != kSolveNone) {
       if (threadIdx.x == 0) {
         st[kMrTarget] = target;
+        st[kMrRtol] = s->local_rtol;
+        st[kMrAttempt] = 0.0;
         st[kMrOuter] = -1.0;  // not started
-        s->sstage = kSolveMinres32;
+        s->sstage = hand;
       }
       __syncthreads();
       return;
@@ -1225,10 +1231,101 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_chol_kernel(ttx_slot* slot
   if (threadIdx.x == 0 && g_flops != 0.0) s->flops[3] += g_flops;
 }
 
+// GMRES local solves (symmetric=False, or local_solver="gmres") with the
+// reference's restarts (common.py:776-809): up to four gmres calls, each from
+// the last iterate with rtol shrunk 1e-2 (floor 1e-15), accepted once
+// ||A u - b_loc|| <= local_rtol ||b_loc||; the routines of gmres_blk.h on the
+// slot's GMRES work, whole restart cycles (m + 1 operator applies) while the
+// tick's budget of minres_slice applies lasts.
+constexpr int64_t kGmresRestart = 20;
+
+__device__ void gmres_body(ttx_slot* s) {
+  __shared__ double red[16];
+  if (!solve_step_active(s) || s->sstage != kSolveGmres) return;
+  const int64_t k = s->k;
+  const int64_t N = s->rx[k] * s->n[k] * s->rx[k + 1];
+  const int64_t m = N < kGmresRestart ? N : kGmresRestart;
+  int64_t t0 = wall_clock64();
+  double* u = scr(s, TTX_SCR_U);
+  const double* bloc = scr(s, TTX_SCR_BLOC);
+  double* scal = scr(s, TTX_SCR_SCAL);
+  double* st = scal + 8;    // attempt bookkeeping (the MINRES slots)
+  double* gst = scal + 40;  // GMRES state (TTX_GMRES_STATE doubles)
+  double* work = scr(s, TTX_SCR_GMRES);
+  GmresView g(work, N, m);
+  const LocOp op = system_op(s, k);
+  if (st[kMrOuter] < 0.0) {
+    // first call, warm-started from the current core (one_site.py:151-153)
+    op.apply(xcore(s, k), 1, g.ax, 1.0, 0.0);
+    gmres_init_blk(N, m, bloc, xcore(s, k), g.ax, st[kMrRtol], 10 * N, work, gst, red);
+    if (threadIdx.x == 0) st[kMrOuter] = 0.0;
+    __syncthreads();
+  }
+  int64_t budget = s->minres_slice > 0 ? s->minres_slice : 128;
+  while (true) {
+    while (gst[G_DONE] == 0.0 && budget > 0) {
+      for (int64_t col = 0; col < m; ++col) {
+        if (gst[G_SKIP] != 0.0) break;  // the inner loop has stopped
+        op.apply(g.V + col * N, 1, g.V + (col + 1) * N, 1.0, 0.0);
+        gmres_arnoldi_blk(N, m, col, work, gst, red);
+        --budget;
+      }
+      const int64_t steps = (int64_t)gst[G_COL] + 1;  // Arnoldi steps of this cycle
+      gmres_finish_blk(N, m, work, gst);
+      op.apply(g.x, 1, g.ax, 1.0, 0.0);
+      gmres_check_blk(N, m, bloc, work, gst, red);
+      --budget;
+      if (threadIdx.x == 0) s->minres_iters += steps;
+    }
+    if (gst[G_DONE] == 0.0) break;  // out of budget: resume next tick
+    // the attempt is over: true residual ||b - A x|| from its last check
+    const double res = gst[G_RNORM];
+    const double target = st[kMrTarget];
+    const int attempt = (int)st[kMrAttempt];
+    const bool ok = res <= target;
+    __syncthreads();  // everyone has read st before it is rewritten
+    if (!ok && attempt >= 3 && s->allow_fallback && N <= s->direct_cap) {
+      // the explicit-matrix fallback (common.py:793-809): LU in this kernel
+      if (threadIdx.x == 0) {
+        s->local_solves += 1;
+        s->local_fallbacks += 1;
+        s->sstage = s->chol ? kSolveChol : kSolveLu;
+      }
+      __syncthreads();
+      stamp(s, 3, t0);
+      return;
+    }
+    if (ok || attempt >= 3) {
+      bcopy(u, g.x, N);
+      if (threadIdx.x == 0) {
+        s->local_solves += 1;
+        if (!ok) s->local_inexact += 1;
+        scal[1] = 0.0;
+        scal[0] = s->residual_mode ? res : 0.0;  // ||A u - b_loc||, the exact residual
+        s->sstage = kSolveNone;
+      }
+      __syncthreads();
+      stamp(s, 3, t0);
+      return;
+    }
+    // next attempt from the last iterate (x and A x are the check's)
+    const double rt = fmax(st[kMrRtol] * 1e-2, 1e-15);
+    if (threadIdx.x == 0) {
+      st[kMrRtol] = rt;
+      st[kMrAttempt] = (double)(attempt + 1);
+    }
+    gmres_init_blk(N, m, bloc, g.x, g.ax, rt, 10 * N, work, gst, red);
+    if (budget <= 0) break;
+  }
+  stamp(s, 3, t0);
+}
+
 __global__ __launch_bounds__(kEngThreads, 2) void eng_lu_kernel(ttx_slot* slots) {
   extern __shared__ double lds[];
   ttx_slot* s = slots + blockIdx.x;
   if (threadIdx.x == 0) g_flops = 0.0;
+  __syncthreads();
+  gmres_body(s);
   __syncthreads();
   lu_body(s, lds);
   __syncthreads();
@@ -1696,6 +1793,10 @@ extern "C" int ttx_engine_plan(ttx_slot* s) {
   sz[TTX_SCR_M32] = s->mixed ? (2 * s->cap_env[0] + ramax * nmax * nmax * ramax + 2 * cl * ramax +
                                 2 * cl + 1) / 2
                              : 0;
+  // GMRES slots: the Krylov basis and the rest of the GMRES work for the
+  // largest local system (restart min(20, N))
+  sz[TTX_SCR_GMRES] = s->iterative == 2 ? gmres_work_doubles(cl, cl < kGmresRestart ? cl : kGmresRestart)
+                                        : 0;
   off = 0;
   for (int i = 0; i < TTX_SCR_COUNT; ++i) {
     s->scr_off[i] = off;

@@ -36,7 +36,7 @@ MAXSWEEPS = 64
 NKERNELS = 7  # per tick: env, solve (rhs + MINRES), chol32, chol, lu, post, check
 KERNEL_NAMES = ("eng_env_kernel", "eng_solve_kernel", "eng_chol32_kernel", "eng_chol_kernel",
                 "eng_lu_kernel", "eng_post_kernel", "eng_check_kernel")
-SCR_COUNT = 18
+SCR_COUNT = 19
 
 PH_XORTH, PH_ZORTH, PH_DONE = 1, 2, 14
 SLOT_RUNNING, SLOT_DONE, SLOT_EMPTY = 0, 1, -100
@@ -211,7 +211,7 @@ class Engine:
         cap = self.capacity(req)
         key = (tuple(A.row_modes), tuple(A.bonds), tuple(b.bonds), req.kickrank,
                req.max_rank, int(req.direct_solve_size), req.precision, cap,
-               bool(req.allow_dense_fallback))
+               bool(req.allow_dense_fallback), str(req.local_solver), bool(req.symmetric))
         hit = self._plans.get(key)
         if hit is not None:
             s = Slot()
@@ -228,6 +228,12 @@ class Engine:
         s.mixed = 1 if req.precision == "mixed" else 0
         # the explicit-matrix fallback needs the direct solve's buffer for any local size
         s.allow_fallback = 1 if req.allow_dense_fallback else 0
+        # the local solvers (common.py:330-360): direct only, MINRES or GMRES above
+        # direct_solve_size; GMRES slots get its Krylov basis in their scratch
+        ls = str(req.local_solver)
+        s.iterative = (0 if ls == "direct" else
+                       1 if ls == "minres" or (ls == "auto" and req.symmetric) else 2)
+        s.chol = 1 if req.symmetric else 0
         for i in range(d):
             s.n[i] = A.row_modes[i]
         for i in range(d + 1):
@@ -326,11 +332,7 @@ class Engine:
             symmetric=req.symmetric, direct_solve_size=req.direct_solve_size, return_info=True,
             local_rtol=req.local_rtol, allow_dense_fallback=req.allow_dense_fallback,
             truncation_steps=max(2 * (d - 1), 1))
-        if st.iterative_solver is not None and str(st.iterative_solver) == "gmres":
-            raise NotImplementedError("the engine's iterative local solver is MINRES (symmetric=True)")
         s = self._plan(req)
-        s.iterative = 1 if st.iterative_solver is not None else 0
-        s.chol = 1 if req.symmetric else 0
         s.residual_mode = 1 if st.residual_tol is not None else 0
         s.max_sweeps = st.max_sweeps
         s.track = 1 if st.track_residual else 0
