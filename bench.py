@@ -312,6 +312,59 @@ def c3_diffusion_3d(with_cpu: bool, solve_levels=(4, 5), cpu_level: int = 4, ful
     return res
 
 
+def c3_full(sweeps: int, level: int = 12):
+    """C3 at its configured size: the 4096^3 operator (36 cores) built as in
+    c3_diffusion_3d, then AMEn (max_rank=64, residual_tol=1e-8, symmetric,
+    seed 0, x0 = rand(ranks=2, seed=0)) for at most ``sweeps`` sweeps on the
+    per-kernel path.  Each sweep's FP64 residual check forms the residual
+    train A x - b (bonds up to R r + 1 ~ 1.4e4) and norms it by a Q-less QR
+    sweep.  The solver's own per-sweep log lines (sweep, residual, max bond,
+    seconds) go to stderr as the run's progress and into the result."""
+    import logging
+    import torch
+    import ttnumpy_amd as tx
+    from ttnumpy_amd import device as D
+    from ttnumpy_amd import diffusion3d as T3
+    kw_op = dict(k_error=1e-8, k_rank=48, op_error=1e-10, method="randomized")
+    kw = dict(max_rank=64, residual_tol=1e-8, max_sweeps=sweeps, symmetric=True, seed=0)
+    lines = []
+
+    class Keep(logging.Handler):
+        def emit(self, record):
+            msg = record.getMessage()
+            lines.append(msg)
+            print(f"# c3 full: {msg}", file=sys.stderr, flush=True)
+
+    log = logging.getLogger("ttnumpy_amd")
+    h = Keep()
+    log.addHandler(h)
+    old_level = log.level
+    log.setLevel(logging.DEBUG)
+    try:
+        A, info = T3.operator(T3.FieldSpec(n_bits=level), **kw_op)
+        print(f"# c3 full: operator built in {info.build_s:.1f} s, max bond {max(info.bonds)}",
+              file=sys.stderr, flush=True)
+        x0 = tx.rand([2] * (3 * level), ranks=2, seed=0)
+        b = T3.rhs(level)
+        torch.cuda.synchronize()
+        f0 = D.flops()
+        t0 = time.perf_counter()
+        x, inf = tx.amen(A, b, x0=x0, return_info=True, **kw)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        gf = (D.flops() - f0) / 1e9
+        return {"grid": f"{2**level}^3", "cores": 3 * level, "op_max_bond": max(info.bonds),
+                "op_build_s": info.build_s, "solve": kw, "gpu_s": wall,
+                "stop_reason": str(inf.stop_reason), "sweeps": inf.num_sweeps_run,
+                "residuals_fp64": inf.residuals, "max_bonds": inf.max_bonds,
+                "final_bonds": x.bonds, "local_solves": inf.local_solves,
+                "local_inexact": inf.local_inexact, "gflop": gf, "gflop_per_s": gf / wall,
+                "sweep_log": lines}
+    finally:
+        log.removeHandler(h)
+        log.setLevel(old_level)
+
+
 def c4_serving(n_requests: int, with_cpu: bool, cpu_per_level: int):
     """C4: the seeded Poisson-arrival stream of parametric 2D diffusion solves
     (``ttnumpy_amd.serving``) on one engine per grid level; with ``with_cpu``
@@ -418,6 +471,9 @@ def main():
     ap.add_argument("--c4", type=int, default=0, metavar="N",
                     help="instead of the C1 line, serve the C4 stream of N requests and print its "
                          "report (with the oracle FIFO estimate unless --no-cpu)")
+    ap.add_argument("--c3-full-sweeps", type=int, default=0, metavar="N",
+                    help="instead of the C1 line, run C3 at 4096^3 (36 cores) for at most N sweeps "
+                         "and print its report")
     ap.add_argument("--c4-requests", type=int, default=512,
                     help="requests of the C4 serving leg of the full line (0 skips it)")
     ap.add_argument("--c4-cpu-per-level", type=int, default=2)
@@ -453,6 +509,9 @@ def main():
     if args.c4:
         print(json.dumps({"c4_serving": c4_serving(args.c4, not args.no_cpu, args.c4_cpu_per_level)}),
               flush=True)
+        return
+    if args.c3_full_sweeps:
+        print(json.dumps({"c3_full": c3_full(args.c3_full_sweeps)}), flush=True)
         return
     A, b, x0 = build_inputs()
     eng = Engine(args.batch, rmax=RMAX)

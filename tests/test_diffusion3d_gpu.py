@@ -101,3 +101,39 @@ def test_amen_on_variable_coefficient_3d_matches_dense_and_oracle():
     assert io.residuals[-1] <= 3e-8
     assert abs(info.num_sweeps_run - io.num_sweeps_run) <= 2
     assert np.linalg.norm(O.Train(xo.cores).full() - ref) <= 3e-8 * cond * np.linalg.norm(ref)
+
+
+def test_amen_3d_minres_regime_matches_dense_and_oracle():
+    """C3's iterative regime: L=4 (4096 unknowns, 12 cores) with
+    direct_solve_size=64, so every local system above 64 unknowns takes the
+    matrix-free MINRES path with the reference's restarts
+    (one_site.py:144-162, common.py:776-809), as C3's systems of up to 8192
+    unknowns do at full size.  Device and oracle converge, in sweep counts
+    within two, with MINRES solves on both, and both solutions lie within the
+    residual-implied distance of the dense solution."""
+    print("diffusion3d amen L=4 minres", flush=True)
+    from threadpoolctl import threadpool_limits
+    from ttnumpy_amd import diffusion3d as T3
+    import ttnumpy_amd as tx
+    spec = T3.FieldSpec(n_bits=4, modes=2, seed=0)
+    A, _ = T3.operator(spec, k_error=1e-12, op_error=1e-13)
+    b = T3.rhs(4)
+    x0 = tx.rand([2] * 12, ranks=2, seed=0)
+    kw = dict(max_rank=64, residual_tol=1e-8, max_sweeps=30, symmetric=True, seed=0,
+              direct_solve_size=64)
+    x, info = tx.amen(A, b, x0=x0, return_info=True, **kw)
+    Ad = A.get_full_tensor().reshape(4096, 4096)
+    ref = np.linalg.solve(Ad, np.ones(4096))
+    cond = np.linalg.cond(Ad)
+    xd = x.get_full_tensor().reshape(-1)
+    with threadpool_limits(8):
+        xo, io = O.amen(O.Train(A.to_numpy()), O.Train(b.to_numpy()),
+                        O.Train(x0.to_numpy()), **kw)
+    print(info, io, cond, flush=True)
+    assert info.local_solves > 0 and io.local_solves > 0
+    for stop, res in ((info.stop_reason, info.residuals), (io.stop_reason, io.residuals)):
+        assert str(stop) == "converged" and res[-1] <= 1e-8
+    assert abs(info.num_sweeps_run - io.num_sweeps_run) <= 2
+    for xf, res in ((xd, info.residuals[-1]), (O.Train(xo.cores).full().reshape(-1),
+                                               io.residuals[-1])):
+        assert np.linalg.norm(xf - ref) <= cond * res * np.linalg.norm(ref)

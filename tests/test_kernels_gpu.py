@@ -69,6 +69,20 @@ def test_gemm_tilings_agree_bitwise():
     np.testing.assert_array_equal(_h(part), full[320:640])
 
 
+def test_gemm_taller_than_one_launch():
+    """More rows than one launch's grid holds (65535 row tiles): the product
+    runs over row blocks and equals the reference everywhere, the last
+    partial block included."""
+    print("gemm tall M=2.2e6", flush=True)
+    D = _dev()
+    rng = np.random.default_rng(3)
+    M, N, K = 2_200_003, 3, 5
+    A, B = rng.standard_normal((M, K)), rng.standard_normal((K, N))
+    C = D.empty(M, N)
+    D.gemm(1, M, N, K, 1.0, D.P(_t(A)), K, 1, 0, D.P(_t(B)), N, 1, 0, 0.0, D.P(C), N, 1, 0)
+    np.testing.assert_allclose(_h(C), A @ B, rtol=1e-12, atol=1e-12)
+
+
 def test_gemm_beta_zero_ignores_nan():
     D = _dev()
     A = np.ones((4, 3))

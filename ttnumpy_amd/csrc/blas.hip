@@ -290,7 +290,20 @@ extern "C" int ttx_gemm_f32acc(ttx_stream_t stream, int64_t batch, int64_t M, in
                                int64_t c_bs, const double* a_scale, const double* b_scale) {
   if (batch < 0 || M < 0 || N < 0 || K < 0) return TTX_EINVAL;
   if (batch == 0 || M == 0 || N == 0) return TTX_OK;
-  if (batch > 65535 || (M + GBM - 1) / GBM > 65535) return TTX_EINVAL;
+  if (batch > 65535) return TTX_EINVAL;
+  // the row tiles go on the grid's y dimension (at most 65535): taller
+  // products run as several launches over row blocks
+  constexpr int64_t kRowsPerLaunch = (int64_t)65535 * GBM;
+  if (M > kRowsPerLaunch) {
+    for (int64_t r0 = 0; r0 < M; r0 += kRowsPerLaunch) {
+      const int64_t mc = (M - r0) < kRowsPerLaunch ? (M - r0) : kRowsPerLaunch;
+      const int rc = ttx_gemm_f32acc(stream, batch, mc, N, K, alpha, A + r0 * a_ms, a_ms, a_ks, a_bs,
+                                     B, b_ks, b_ns, b_bs, beta, C + r0 * c_ms, c_ms, c_ns, c_bs,
+                                     a_scale, b_scale);
+      if (rc != TTX_OK) return rc;
+    }
+    return TTX_OK;
+  }
   if (K == 0) {
     int64_t dims[3] = {batch, M, N};
     int64_t ds[3] = {c_bs, c_ms, c_ns};
@@ -330,7 +343,20 @@ int ttx::gemm_f64_launch(ttx_stream_t stream, int64_t batch, int64_t M, int64_t 
                     double* C, int64_t c_ms, int64_t c_ns, int64_t c_bs, const double* skip) {
   if (batch < 0 || M < 0 || N < 0 || K < 0) return TTX_EINVAL;
   if (batch == 0 || M == 0 || N == 0) return TTX_OK;
-  if (batch > 65535 || (M + GBM - 1) / GBM > 65535) return TTX_EINVAL;
+  if (batch > 65535) return TTX_EINVAL;
+  // the row tiles go on the grid's y dimension (at most 65535): taller
+  // products run as several launches over row blocks (either tiling gives
+  // the same values, so the split does not change them)
+  constexpr int64_t kRowsPerLaunch = (int64_t)65535 * GSM;
+  if (M > kRowsPerLaunch) {
+    for (int64_t r0 = 0; r0 < M; r0 += kRowsPerLaunch) {
+      const int64_t mc = (M - r0) < kRowsPerLaunch ? (M - r0) : kRowsPerLaunch;
+      const int rc = gemm_f64_launch(stream, batch, mc, N, K, alpha, A + r0 * a_ms, a_ms, a_ks, a_bs,
+                                     B, b_ks, b_ns, b_bs, beta, C + r0 * c_ms, c_ms, c_ns, c_bs, skip);
+      if (rc != TTX_OK) return rc;
+    }
+    return TTX_OK;
+  }
   if (K == 0) {
     // empty contraction: C = beta * C (beta == 0 writes zeros without reading C)
     int64_t dims[3] = {batch, M, N};

@@ -8,6 +8,7 @@ small device->host copy per rank decision (and per sweep for the residual).
 
 from __future__ import annotations
 
+import logging
 from math import prod
 from typing import Optional, Tuple, Union
 
@@ -22,6 +23,8 @@ from .common import (LocalSolver, SolverInfo, TruncationMode, _SolverSettings, _
                      prepare_right_environment)
 from .one_site import (_shift_center_left, _shift_center_right, _solve_local_one_site,
                        local_rhs_one_site)
+
+_log = logging.getLogger("ttnumpy_amd")
 
 
 def init_residual_train(b: TensorTrain, kickrank: int, seed: Optional[int] = None) -> TensorTrain:
@@ -309,6 +312,8 @@ def amen(A: TensorTrain, b: TensorTrain, x0: Optional[TensorTrain] = None, inpla
         prepare_left_environment(d - 1, lzb, z, b)
         prepare_left_environment(d - 1, lzA, z, x, A)
         prepare_left_environment(d - 1, lxz, x, z)
+        if _log.isEnabledFor(logging.DEBUG):
+            _log.debug("AMEn sweep %d: forward half-sweep done, max bond %d", sweep + 1, max(x.bonds))
 
         for k in range(d - 1, -1, -1):
             prepare_right_environment(k, rA, x, x, A)
@@ -332,6 +337,9 @@ def amen(A: TensorTrain, b: TensorTrain, x0: Optional[TensorTrain] = None, inpla
                 _enrich_left(x, k, core, carry, block, max_rank, stats=stats)
                 _shift_center_left(z, k, z_new)
 
+        if _log.isEnabledFor(logging.DEBUG):
+            _log.debug("AMEn sweep %d: backward half-sweep done, max bond %d; residual check",
+                       sweep + 1, max(x.bonds))
         if monitor.record(sweep, x, max_bond=max(x.bonds), stats=stats):
             break
 
