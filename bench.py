@@ -382,11 +382,21 @@ def c4_serving(n_requests: int, with_cpu: bool, cpu_per_level: int):
     t0 = time.perf_counter()
 
     def progress(k, r):
-        if k % 64 == 0 or k == n_requests:
+        if r is None:
+            print(f"# c4: engine tuned at {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
+        elif k % 64 == 0 or k == n_requests:
             print(f"# c4: {k} done at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     rep, _ = serve(cfg, reqs, progress=progress)
     out = {"gpu": dataclasses.asdict(rep)}
+    # the same arrivals in the mixed form (FP32 Cholesky / FP32-operator MINRES
+    # with FP64 refinement; environments, residual check, QR/SVD and rank rule FP64)
+    cfgm = ServingConfig(n_requests=n_requests, precision="mixed")
+    t0 = time.perf_counter()
+    repm, _ = serve(cfgm, build_requests(cfgm, items), progress=progress)
+    out["gpu_mixed"] = dataclasses.asdict(repm)
+    out["mixed_ratio_vs_fp64"] = repm.solves_per_s / rep.solves_per_s
     if with_cpu:
         from threadpoolctl import threadpool_limits
         from oracle import tt_oracle as O
@@ -515,6 +525,12 @@ def main():
         return
     A, b, x0 = build_inputs()
     eng = Engine(args.batch, rmax=RMAX)
+    # tile variant of the engine's kernels, picked on one untimed batch by
+    # the engine tuner (both variants compute the same values)
+    from ttnumpy_amd.autotune import tune_engine
+    tiles = tune_engine(eng, requests(A, b, x0, args.batch))
+    print(f"# engine tiles {tiles['tiles']} (gain {tiles['gain_vs_default']:.3f})",
+          file=sys.stderr, flush=True)
     value, lat, done, total = timed_batches(eng, A, b, x0, args, "fp64")
     errors = [r.error for r in done if r.error]
     ok = [r for r in done if r.error is None]
@@ -534,6 +550,7 @@ def main():
     # events around each kernel on the engine's stream; algorithmic flops are
     # banked per kernel by the device counters of each slot
     prof = Engine(args.batch, rmax=RMAX)
+    prof.tiles = eng.tiles
     prof.profiling = True
     prof.run(requests(A, b, x0, args.batch), poll=args.poll)
     j = int(np.argmax(prof.kernel_ms))
@@ -559,6 +576,7 @@ def main():
         # operator with FP64 defect correction for the iterative ones; the
         # environments, residual check, QR/SVD and rank rule stay FP64
         engm = Engine(args.batch, rmax=RMAX)
+        engm.tiles = eng.tiles
         mval, mlat, mdone, _ = timed_batches(engm, A, b, x0, args, "mixed")
         mok = [r for r in mdone if r.error is None]
         m0 = mok[0] if mok else None
@@ -603,6 +621,11 @@ def main():
                    "rank_capacity": RMAX},
         "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
         "single_request_ms": single_ms,
+        "engine_tiles": {"tiles": tiles["tiles"], "variant": tiles["variant"],
+                         "gain_vs_default": tiles["gain_vs_default"],
+                         "kernel_ms_total": {v: k["total"] for v, k in tiles["kernel_ms"].items()},
+                         "chol_kernel_ms": {v: k["eng_chol_kernel"]
+                                            for v, k in tiles["kernel_ms"].items()}},
         "solves_ok": nsolved, "errors": errors[:3],
         "final_residual": r0.info.residuals[-1] if r0 else None,
         "sweeps": r0.info.num_sweeps_run if r0 else None,

@@ -58,7 +58,8 @@ typedef struct ttx_slot {
   int64_t rb[TTX_MAXD + 1];
   double residual_tol, trunc_tol, local_rtol, damp, error_threshold, stagnation, bnorm, pad1;
   // ---- layout (planner-written, doubles)
-  int64_t cap_core, cap_zcore, cap_loc, direct_cap, lmax, scr_size, env_size, pad2;
+  int64_t cap_core, cap_zcore, cap_loc, direct_cap, lmax, scr_size, env_size;
+  int64_t tiles;  // tile variants (host-written): bit 0, Cholesky updates on 128 x 32 tiles
   int64_t cap_env[5], pad3[3];
   int64_t env_off[10], pad4[6];
   int64_t scr_off[TTX_SCR_COUNT], pad5[32 - TTX_SCR_COUNT];

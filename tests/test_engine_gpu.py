@@ -368,3 +368,34 @@ def test_engine_graph_replays_equal_eager_ticks():
             for ca, cb in zip(a.result.to_numpy(), b.result.to_numpy()):
                 np.testing.assert_array_equal(ca, cb)
     geng.close_graphs()
+
+
+def test_engine_tile_variants_bitwise_equal_and_tuner():
+    """The Cholesky tile variants (128 x 64 and 128 x 32 trailing-update tiles)
+    sum every element over the same k in the same order, so a mix with direct
+    and MINRES local solves gives identical residual histories and cores under
+    either; the engine tuner then picks one of them from measured kernel time."""
+    from ttnumpy_amd.autotune import ENGINE_TILE_VARIANTS, tune_engine
+    from ttnumpy_amd.engine import Engine
+    print("engine tile variants, poisson 32x32 and 64x64", flush=True)
+
+    def mix():
+        return [_poisson_request(6, direct_solve_size=600), _poisson_request(5, seed=1, x0_seed=1)]
+
+    runs = []  # local_solves counts the iterative ones: the 64 x 64 case has both kinds
+    for t in sorted(ENGINE_TILE_VARIANTS):
+        eng = Engine(2, rmax=40)
+        eng.tiles = t
+        runs.append(sorted(eng.run(mix(), poll=8), key=lambda r: len(r.A)))
+    assert runs[0][-1].info.local_solves > 0
+    for a, b in zip(*runs):
+        assert a.error is None and b.error is None
+        assert a.info.residuals == b.info.residuals
+        assert a.result.bonds == b.result.bonds
+        for ca, cb in zip(a.result.to_numpy(), b.result.to_numpy()):
+            np.testing.assert_array_equal(ca, cb)
+    eng = Engine(2, rmax=40)
+    rep = tune_engine(eng, mix(), poll=8)
+    print(rep, flush=True)
+    assert eng.tiles == rep["tiles"] and rep["tiles"] in ENGINE_TILE_VARIANTS
+    assert all(rep["kernel_ms"][t]["total"] > 0 for t in ENGINE_TILE_VARIANTS)

@@ -1,4 +1,4 @@
-"""Autotuner of the GEMM tiling: which contractions run on 32 x 32 tiles.
+"""Autotuners: the GEMM tiling of the per-kernel path, the engine's tile variants.
 
 ``ttx_gemm_f64`` runs a contraction on 64 x 64 output tiles, or on 32 x 32 tiles
 (four times the waves, a quarter of the MFMA chain each, identical values) when
@@ -13,6 +13,10 @@ depends on the shape and on the GPU, so the limit is chosen from timings:
    the timings, and ``tune`` sets it (``ttx_gemm_set_small_tile_limit``).
 
     python -m ttnumpy_amd.autotune [--workload c5|c3|c2] [--reps 20] [--save FILE]
+
+``tune_engine`` picks the batched engine's tile variant (the Cholesky
+trailing update's 128 x 64 or 128 x 32 tiles; identical values) from the
+engine's own kernel times on a sample of the requests it will serve.
 """
 
 from __future__ import annotations
@@ -21,7 +25,7 @@ import argparse
 import contextlib
 import json
 import time
-from typing import Callable, Dict, Tuple
+from typing import Callable, Dict, Optional, Tuple
 
 import torch
 
@@ -124,6 +128,63 @@ def tune(workload: Callable[[], object], reps: int = 20, apply: bool = True) -> 
                                "all_32x32": predicted_ms(times, rec, ALL_32)}}
     if apply:
         set_limit(limit)
+    return report
+
+
+# ------------------------------------------------------------------ engine tiles
+# The engine's kernels take their tile variants from each slot's `tiles`
+# bits; the variants give identical values, so they can be told apart by
+# speed alone, on the requests an engine is about to serve.
+ENGINE_TILE_VARIANTS = {
+    0: "Cholesky trailing updates on 128 x 64 tiles",
+    1: "Cholesky trailing updates on 128 x 32 tiles",
+}
+
+
+def tune_engine(engine, sample, poll: int = 64, ticks: Optional[int] = None,
+                apply: bool = True) -> dict:
+    """Pick ``engine``'s tile variant from its own kernels' times on ``sample``.
+
+    For each variant a scratch engine of the same slot count and rank
+    capacity serves copies of the ``sample`` requests (up to its slots) with
+    HIP events around each of the seven kernels of every tick: to completion,
+    or for ``ticks`` ticks (a bound on the tuning time for long solves).  The
+    variants compute the same values, so both do the same work, and the
+    variant with the least total kernel time wins.  Returns a report
+    (per-variant kernel milliseconds, the choice, the gain over variant 0)
+    and, with ``apply``, sets ``engine.tiles``."""
+    import dataclasses
+    import numpy as np
+    from .engine import KERNEL_NAMES, Engine
+    t0 = time.perf_counter()
+    reqs = list(sample)[:engine.nslots]
+    res = {}
+    for v in ENGINE_TILE_VARIANTS:
+        eng = Engine(engine.nslots, rmax=engine.rmax, chunk=engine.chunk,
+                     hbm_reserve_bytes=engine.reserve, minres_slice=engine.minres_slice,
+                     graphs=False, max_rank_capacity=engine.max_rank_capacity)
+        eng.tiles = v
+        eng.profiling = True
+        copies = [dataclasses.replace(r, result=None, info=None, error=None, admitted=0.0,
+                                      finished=0.0, arrival=0.0) for r in reqs]
+        if ticks is None:
+            eng.run(copies, poll=poll)
+        else:  # finished slots idle through the remaining ticks
+            for i, r in enumerate(copies):
+                eng.admit(i, r, push=False)
+            eng._push_all()
+            eng.tick(ticks)
+        torch.cuda.synchronize()
+        res[v] = {k: round(float(x), 3) for k, x in zip(KERNEL_NAMES, eng.kernel_ms)}
+        res[v]["total"] = round(float(np.sum(eng.kernel_ms)), 3)
+        del eng
+    best = min(res, key=lambda v: res[v]["total"])
+    report = {"tiles": best, "variant": ENGINE_TILE_VARIANTS[best], "requests": len(reqs),
+              "ticks": ticks,
+              "kernel_ms": res, "gain_vs_default": res[0]["total"] / res[best]["total"],
+              "tuning_s": time.perf_counter() - t0}
+    if apply:
+        engine.tiles = best
     return report
 
 

@@ -1252,11 +1252,16 @@ __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W,
 // For i < Mr, j < Nc with r0 + i >= c0 + j (tiles wholly above the diagonal
 // are skipped; entries above it inside a crossing tile are written but never
 // read):  W[r0+i][c0+j] -= sum_{t<K} W[r0+i][a0+t] * W[c0+j][a0+t]
-template <int NT>
+// TN: output tile columns (64 NT / 256, or 32 with 256 threads); each wave
+// owns 32 rows x TN / (NT / 256) columns.  Every output accumulates k
+// ascending, four per MFMA step, so the tile shape does not change a value.
+template <int NT, int TN = 64 * NT / 256>
 __device__ __forceinline__ void chol_tile_update_t(int64_t Mr, int64_t Nc, int64_t K, double* W,
                                                    int64_t ld, int64_t r0, int64_t c0, int64_t a0,
                                                    double* lds) {
-  constexpr int TN = 64 * NT / 256;
+  constexpr int WN = TN / (NT / 256);  // columns per wave
+  constexpr int NB = WN / 16;          // 16-column MFMA blocks per wave
+  static_assert(NB >= 1 && NB * 16 * (NT / 256) == TN, "tile columns");
   constexpr int EA = kLuTileM * kLuK / NT;  // A slab: consecutive k per thread
   constexpr int TPA = kLuK / EA;
   constexpr int EB = TN * kLuK / NT;        // B slab: consecutive k of one B row per thread
@@ -1265,7 +1270,7 @@ __device__ __forceinline__ void chol_tile_update_t(int64_t Mr, int64_t Nc, int64
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, lq = lane >> 4;
-  const int wm = (wave & 3) * 32, wn = (wave >> 2) * 64;
+  const int wm = (wave & 3) * 32, wn = (wave >> 2) * WN;
   double* As = lds;                    // [kLuK][kLuTileM]
   double* Bs = lds + kLuK * kLuTileM;  // [kLuK][TN]
   const int ar = tid / TPA, ak = (tid % TPA) * EA;
@@ -1283,11 +1288,11 @@ __device__ __forceinline__ void chol_tile_update_t(int64_t Mr, int64_t Nc, int64
         const int64_t row = m0 + wm + 16 * mt + 4 * e + lq;
         crow[mt][e] = row < Mr ? W + (r0 + row) * ld + c0 : nullptr;
       }
-    double4_t acc[2][4];
+    double4_t acc[2][NB];
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
+      for (int nt = 0; nt < NB; ++nt) {
         const int64_t col = n0 + wn + 16 * nt + l16;
 #pragma unroll
         for (int e = 0; e < 4; ++e)
@@ -1320,22 +1325,22 @@ __device__ __forceinline__ void chol_tile_update_t(int64_t Mr, int64_t Nc, int64
 #pragma unroll
       for (int ks = 0; ks < kLuK; ks += 4) {
         const int kr = ks + lq;
-        double a[2], b[4];
+        double a[2], b[NB];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) a[mt] = -As[kr * kLuTileM + wm + 16 * mt + l16];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) b[nt] = Bs[kr * TN + wn + 16 * nt + l16];
+        for (int nt = 0; nt < NB; ++nt) b[nt] = Bs[kr * TN + wn + 16 * nt + l16];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-          for (int nt = 0; nt < 4; ++nt)
+          for (int nt = 0; nt < NB; ++nt)
             acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
       }
     }
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
+      for (int nt = 0; nt < NB; ++nt) {
         const int64_t col = n0 + wn + 16 * nt + l16;
 #pragma unroll
         for (int e = 0; e < 4; ++e)
@@ -1345,7 +1350,7 @@ __device__ __forceinline__ void chol_tile_update_t(int64_t Mr, int64_t Nc, int64
   __syncthreads();
 }
 
-template <int NT>
+template <int NT, int TN = 64 * NT / 256>
 __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* W, double* u,
                                                  double* lds, int32_t* info,
                                                  int64_t* clk = nullptr) {
@@ -1439,11 +1444,11 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
       __syncthreads();
       lu_clock(clk, 0, t0);
       // 3. the rest of the outer panel (lower part), rank nb
-      chol_tile_update_t<NT>(n - (k + nb), K1 - (k + nb), nb, W, ld, k + nb, k + nb, k, stage);
+      chol_tile_update_t<NT, TN>(n - (k + nb), K1 - (k + nb), nb, W, ld, k + nb, k + nb, k, stage);
       lu_clock(clk, 1, t0);
     }
     // trailing lower triangle, rank ko
-    chol_tile_update_t<NT>(n - K1, n - K1, ko, W, ld, K1, K1, K0, stage);
+    chol_tile_update_t<NT, TN>(n - K1, n - K1, ko, W, ld, K1, K1, K0, stage);
     lu_clock(clk, 3, t0);
   }
   // L^T u = y, 64-row blocks from the bottom

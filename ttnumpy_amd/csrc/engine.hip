@@ -1165,7 +1165,12 @@ __device__ void chol_body(ttx_slot* s, double* lds) {
   if (threadIdx.x == 0) s_info = TTX_OK;
   __syncthreads();
   stamp(s, 1, t0);
-  block_chol_solve<kEngThreads>(N, N + 1, W, scr(s, TTX_SCR_U), lds, &s_info, &s->clk[13]);
+  // the tile shape of the trailing updates is the slot's (picked by the
+  // host's tuner); both give the same values
+  if (s->tiles & 1)
+    block_chol_solve<kEngThreads, 32>(N, N + 1, W, scr(s, TTX_SCR_U), lds, &s_info, &s->clk[13]);
+  else
+    block_chol_solve<kEngThreads>(N, N + 1, W, scr(s, TTX_SCR_U), lds, &s_info, &s->clk[13]);
   __syncthreads();
   stamp(s, 2, t0);
   if (s_info == TTX_ENOTSPD) {  // not positive definite: LU takes over this tick

@@ -59,7 +59,7 @@ class Slot(ctypes.Structure):
         + [(nm, _f64) for nm in ("residual_tol", "trunc_tol", "local_rtol", "damp", "error_threshold",
                                  "stagnation", "bnorm", "pad1")]
         + [(nm, _i64) for nm in ("cap_core", "cap_zcore", "cap_loc", "direct_cap", "lmax", "scr_size",
-                                 "env_size", "pad2")]
+                                 "env_size", "tiles")]
         + [("cap_env", _i64 * 5), ("pad3", _i64 * 3), ("env_off", _i64 * 10), ("pad4", _i64 * 6),
            ("scr_off", _i64 * 32)]
         + [("A", _vp * MAXD), ("b", _vp * MAXD), ("x", _vp), ("z", _vp), ("env", _vp), ("scr", _vp)]
@@ -143,6 +143,9 @@ class Engine:
         # reference's ranks are bounded only by max_rank (amen.py:441-451).
         self.max_rank_capacity = int(max_rank_capacity)
         self.regrown = 0
+        # tile variants of the engine's kernels (the slot's `tiles` bits; equal
+        # values, different speed), chosen by ttnumpy_amd.autotune.tune_engine
+        self.tiles = 0
         # graphs: each poll's ticks replay one recorded HIP graph instead of
         # seven launches per tick (same kernels, same order, same results)
         self.graphs = bool(graphs)
@@ -333,6 +336,7 @@ class Engine:
             local_rtol=req.local_rtol, allow_dense_fallback=req.allow_dense_fallback,
             truncation_steps=max(2 * (d - 1), 1))
         s = self._plan(req)
+        s.tiles = self.tiles
         s.residual_mode = 1 if st.residual_tol is not None else 0
         s.max_sweeps = st.max_sweeps
         s.track = 1 if st.track_residual else 0

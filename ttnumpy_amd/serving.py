@@ -50,6 +50,11 @@ class ServingConfig:
     per_level: bool = True        # one engine (own stream) per grid level
     minres_slice: int = 128       # MINRES iterations per tick
     precision: str = "fp64"       # "mixed": FP32 Cholesky with FP64 refinement for direct solves
+    # before the clock starts, each engine's tile variant is picked from its
+    # kernels' times over `tune_ticks` ticks of up to `tune_sample` requests
+    # of its own level (0: variant 0)
+    tune_sample: int = 8
+    tune_ticks: int = 192
 
 
 @dataclass
@@ -104,12 +109,16 @@ class ServingReport:
     max_sweeps: int
     worst_residual: float
     by_level: Dict[int, dict] = field(default_factory=dict)
+    # per engine (keyed by grid level, or "all"): the tile variant chosen on
+    # the warm-up sample, its measured kernel-time gain over variant 0, tuning seconds
+    tiles: Dict[object, dict] = field(default_factory=dict)
 
 
 def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
           engine: Optional[Engine] = None, progress=None) -> tuple:
     """Run the stream; returns ``(report, requests)``.  ``progress(n_done, req)``
-    is called after every finished request."""
+    is called after every finished request (and as ``progress(0, None)`` after
+    each engine's tuning)."""
     items = request_stream(cfg)
     if requests is None:
         requests = build_requests(cfg, items)
@@ -123,6 +132,21 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
     else:
         engines = [Engine(cfg.nslots, rmax=cfg.rmax, minres_slice=cfg.minres_slice)]
         pool = None
+    tiles: Dict[object, dict] = {}
+    if cfg.tune_sample > 0:
+        from .autotune import tune_engine
+        keys = levels if pool is not None else ["all"]
+        for key, eng in zip(keys, engines):
+            sample = [r for r, it in zip(requests, items) if key == "all" or it.level == key]
+            if sample:
+                t = tune_engine(eng, sample[:cfg.tune_sample], ticks=cfg.tune_ticks)
+                if progress is not None:
+                    progress(0, None)
+                tiles[key] = {"tiles": t["tiles"], "variant": t["variant"],
+                              "gain_vs_default": t["gain_vs_default"], "sample": t["requests"],
+                              "ticks": t["ticks"],
+                              "kernel_ms_total": {v: k["total"] for v, k in t["kernel_ms"].items()},
+                              "tuning_s": t["tuning_s"]}
     torch.cuda.synchronize()
     flops0 = sum(float(e.kernel_flops.sum()) for e in engines)
     t0 = time.perf_counter()
@@ -165,7 +189,7 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
         converged=stops.count("converged"), stagnated=stops.count("stagnated"),
         max_sweeps=stops.count("max_sweeps"),
         worst_residual=max((r.info.residuals[-1] for r in ok), default=float("nan")),
-        by_level=dict(sorted(by_level.items())))
+        by_level=dict(sorted(by_level.items())), tiles=tiles)
     return rep, done
 
 
@@ -195,16 +219,21 @@ def main(argv=None) -> None:
     ap.add_argument("--single-engine", action="store_true",
                     help="one engine for all levels instead of one per level")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--tune-sample", type=int, default=32,
+                    help="warm-up requests per engine for the tile tuner (0: no tuning)")
     a = ap.parse_args(argv)
     cfg = ServingConfig(n_requests=a.requests, rate=a.rate, seed=a.seed, nslots=a.slots,
-                        per_level=not a.single_engine, precision=a.precision)
+                        per_level=not a.single_engine, precision=a.precision,
+                        tune_sample=a.tune_sample)
     t = time.perf_counter()
     reqs = build_requests(cfg, request_stream(cfg))
     print(f"built {len(reqs)} requests in {time.perf_counter() - t:.1f} s", flush=True)
     t0 = time.perf_counter()
 
     def progress(k, r):
-        if k % 32 == 0 or k == a.requests:
+        if r is None:
+            print(f"tuned an engine at {time.perf_counter() - t0:.1f} s", flush=True)
+        elif k % 32 == 0 or k == a.requests:
             print(f"{k} done at {time.perf_counter() - t0:.1f} s (last: L={r.tag.level}, "
                   f"latency {r.finished - r.arrival:.2f} s)", flush=True)
 
@@ -214,7 +243,7 @@ def main(argv=None) -> None:
                      "levels": list(cfg.levels), "theta_max": cfg.theta_max,
                      "slots_per_engine": a.slots, "per_level": cfg.per_level,
                      "precision": a.precision, "residual_tol": cfg.residual_tol,
-                     "max_sweeps": cfg.max_sweeps}
+                     "max_sweeps": cfg.max_sweeps, "tune_sample": cfg.tune_sample}
     errs = [r.error for r in done if r.error]
     if errs:
         out["first_errors"] = errs[:5]
