@@ -59,7 +59,7 @@ typedef struct ttx_slot {
   double residual_tol, trunc_tol, local_rtol, damp, error_threshold, stagnation, bnorm, pad1;
   // ---- layout (planner-written, doubles)
   int64_t cap_core, cap_zcore, cap_loc, direct_cap, lmax, scr_size, env_size;
-  int64_t tiles;  // tile variants (host-written): bit 0, Cholesky updates on 128 x 32 tiles
+  int64_t tiles;  // tile variants (host-written): bit 0, Cholesky rows below the diagonal in 64-row tiles
   int64_t cap_env[5], pad3[3];
   int64_t env_off[10], pad4[6];
   int64_t scr_off[TTX_SCR_COUNT], pad5[32 - TTX_SCR_COUNT];
@@ -86,8 +86,9 @@ typedef struct ttx_slot {
   // 0 local rhs, 1 explicit matrix, 2 LU, 3 MINRES, 4 exact residual, 5 post, 6 check, 7 env,
   // inside post (sweep steps only): 8 residual core, 9 SVD, 10 rank choice, 11 enrichment
   // and QR, 12 z shift; inside the LU: 13 panels, 14 in-panel updates, 15 block rows,
-  // 16 trailing updates, 17 back substitution; mixed direct solves: 18 FP32 factor,
-  // 19 refinement; 20-23 spare
+  // 16 trailing updates, 17 back substitution (inside the Cholesky: 13 diagonal-block
+  // factorizations, 14 diagonal-block updates, 16 rows below them, 17 back
+  // substitution); mixed direct solves: 18 FP32 factor, 19 refinement; 20-23 spare
   int64_t clk[24];
 } ttx_slot;
 

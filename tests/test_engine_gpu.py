@@ -371,8 +371,8 @@ def test_engine_graph_replays_equal_eager_ticks():
 
 
 def test_engine_tile_variants_bitwise_equal_and_tuner():
-    """The Cholesky tile variants (128 x 64 and 128 x 32 trailing-update tiles)
-    sum every element over the same k in the same order, so a mix with direct
+    """The Cholesky tile variants (128- and 64-row tiles below the diagonal
+    blocks) sum every element over the same k in the same order, so a mix with direct
     and MINRES local solves gives identical residual histories and cores under
     either; the engine tuner then picks one of them from measured kernel time."""
     from ttnumpy_amd.autotune import ENGINE_TILE_VARIANTS, tune_engine

@@ -14,9 +14,9 @@ depends on the shape and on the GPU, so the limit is chosen from timings:
 
     python -m ttnumpy_amd.autotune [--workload c5|c3|c2] [--reps 20] [--save FILE]
 
-``tune_engine`` picks the batched engine's tile variant (the Cholesky
-trailing update's 128 x 64 or 128 x 32 tiles; identical values) from the
-engine's own kernel times on a sample of the requests it will serve.
+``tune_engine`` picks the batched engine's tile variant (128- or 64-row
+tiles below the Cholesky factorization's diagonal blocks; identical values)
+from the engine's own kernel times on a sample of the requests it will serve.
 """
 
 from __future__ import annotations
@@ -136,8 +136,8 @@ def tune(workload: Callable[[], object], reps: int = 20, apply: bool = True) -> 
 # bits; the variants give identical values, so they can be told apart by
 # speed alone, on the requests an engine is about to serve.
 ENGINE_TILE_VARIANTS = {
-    0: "Cholesky trailing updates on 128 x 64 tiles",
-    1: "Cholesky trailing updates on 128 x 32 tiles",
+    0: "Cholesky rows below each diagonal block in tiles of 128 rows x 64 columns",
+    1: "Cholesky rows below each diagonal block in tiles of 64 rows x 64 columns",
 }
 
 

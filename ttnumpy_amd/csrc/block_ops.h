@@ -1234,181 +1234,189 @@ __device__ __forceinline__ void block_lu_solve(int64_t n, int64_t ld, double* W,
 
 // ---------------------------------------------------------------------------
 // Blocked Cholesky solve of a symmetric positive definite [M | b] (row-major,
-// row stride ld >= n + 1, only the lower triangle of M is read) by one block
-// of NT (256 or 512) threads.
+// row stride ld >= n + 1; only the lower triangle of M and column n = b are
+// read, L overwrites the lower triangle) by one block of 256 threads, with
+// u = M^{-1} b.
 //
-// Right-looking with two levels: per 16-column sub-panel the diagonal block
-// is factored by one wave, the rows below are solved against it one row per
-// thread (rows are contiguous, no pivoting, so no block-wide reduction per
-// column), and the forward substitution of b rides along; the rest of the
-// 128-column outer panel takes rank-16 updates, the trailing lower triangle
-// one rank-128 update per outer panel through the LDS-staged MFMA tiles.
-// Then L^T u = y from the bottom in 64-row blocks.  Needs
-// 8 * 4352 + 256 bytes of lds at most (independent of n).
-// *info = TTX_ENOTSPD on a pivot <= 0 or not finite: the routine stops early
-// and W is left partly overwritten.  Ends synced.
+// Left-looking over 64-column block columns J = [J0, J0 + jb), with
+// L_J< = L[J][0:J0] and y_< = y[0:J0] final from the block columns before:
+//  1. D = M_JJ - L_J< L_J<^T (its ten lower 16 x 16 blocks on MFMA) and
+//     c = b_J - L_J< y_< into LDS;
+//  2. D = L_JJ L_JJ^T in LDS, 16 columns at a time: wave 0 (lane = row of the
+//     block) factors the 16-column sub-panel with y_J = L_JJ^{-1} c riding
+//     along, then the lower blocks right of it take its rank-16 update on
+//     MFMA; L_JJ goes to W and y_J to u;
+//  3. per tile of 64 NTB rows R below J: V = (M_RJ - L_R< L_J<^T)^T on MFMA
+//     into registers (each wave owns 16 NTB rows of R and all of J), then
+//     X L_JJ^T = V^T sixteen columns at a time: the columns' update by the
+//     solved ones on MFMA (with -L_JJ from LDS as the A operand; V in the
+//     accumulator layout is the B operand as it stands), then one lane per
+//     row solves against the 16 x 16 diagonal block by substitution through
+//     LDS; X = L_RJ is stored.
+// Every entry of L is written once and read back only as an operand:
+// L_R< once per block column and L_J< once per tile of R, about
+// (n^3 / 384)(1 + 64 / RT) doubles over the factorization, besides reading
+// M and writing L once (n^2 / 2 doubles each), where a right-looking
+// factorization reads and writes the trailing matrix once per panel.
+// Each entry sees the operations of the right-looking blocked form in the
+// same order: one chain of MFMA steps (four k at a time, k ascending) over
+// every 16-column block left of its own, then the 16-column block's
+// substitution (a pivot lane's sqrt and divisions inside a diagonal block,
+// division by the stored diagonal below it), so the tile shape (NTB) and the
+// schedule change no value.  Then L^T u = y from the bottom in 64-row
+// blocks.  lds: kCholLds doubles.  *info = TTX_ENOTSPD on a pivot <= 0 or
+// not finite: the routine stops early and W is left partly overwritten.
+// Ends synced.
 // ---------------------------------------------------------------------------
 
-// For i < Mr, j < Nc with r0 + i >= c0 + j (tiles wholly above the diagonal
-// are skipped; entries above it inside a crossing tile are written but never
-// read):  W[r0+i][c0+j] -= sum_{t<K} W[r0+i][a0+t] * W[c0+j][a0+t]
-// TN: output tile columns (64 NT / 256, or 32 with 256 threads); each wave
-// owns 32 rows x TN / (NT / 256) columns.  Every output accumulates k
-// ascending, four per MFMA step, so the tile shape does not change a value.
-template <int NT, int TN = 64 * NT / 256>
-__device__ __forceinline__ void chol_tile_update_t(int64_t Mr, int64_t Nc, int64_t K, double* W,
-                                                   int64_t ld, int64_t r0, int64_t c0, int64_t a0,
-                                                   double* lds) {
-  constexpr int WN = TN / (NT / 256);  // columns per wave
-  constexpr int NB = WN / 16;          // 16-column MFMA blocks per wave
-  static_assert(NB >= 1 && NB * 16 * (NT / 256) == TN, "tile columns");
-  constexpr int EA = kLuTileM * kLuK / NT;  // A slab: consecutive k per thread
-  constexpr int TPA = kLuK / EA;
-  constexpr int EB = TN * kLuK / NT;        // B slab: consecutive k of one B row per thread
-  constexpr int TPB = kLuK / EB;
-  if (Mr <= 0 || Nc <= 0 || K <= 0) return;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int l16 = lane & 15, lq = lane >> 4;
-  const int wm = (wave & 3) * 32, wn = (wave >> 2) * WN;
-  double* As = lds;                    // [kLuK][kLuTileM]
-  double* Bs = lds + kLuK * kLuTileM;  // [kLuK][TN]
-  const int ar = tid / TPA, ak = (tid % TPA) * EA;
-  const int br = tid / TPB, bk = (tid % TPB) * EB;
-  const int64_t tm = (Mr + kLuTileM - 1) / kLuTileM, tn = (Nc + TN - 1) / TN;
-  for (int64_t t = 0; t < tm * tn; ++t) {
-    const int64_t m0 = (t / tn) * kLuTileM, n0 = (t % tn) * TN;
-    // skip tiles strictly above the diagonal: last row < first column
-    if (r0 + m0 + kLuTileM - 1 < c0 + n0) continue;
-    double* crow[2][4];
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int64_t row = m0 + wm + 16 * mt + 4 * e + lq;
-        crow[mt][e] = row < Mr ? W + (r0 + row) * ld + c0 : nullptr;
-      }
-    double4_t acc[2][NB];
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-      for (int nt = 0; nt < NB; ++nt) {
-        const int64_t col = n0 + wn + 16 * nt + l16;
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          acc[mt][nt][e] = (crow[mt][e] && col < Nc) ? crow[mt][e][col] : 0.0;
-      }
-    const double* ap = (m0 + ar) < Mr ? W + (r0 + m0 + ar) * ld + a0 : nullptr;
-    const double* bp = (n0 + br) < Nc ? W + (c0 + n0 + br) * ld + a0 : nullptr;
-    double ra[EA], rb[EB];
-    auto gload = [&](int64_t k0) {
-#pragma unroll
-      for (int i = 0; i < EA; ++i) {
-        const int64_t kk = k0 + ak + i;
-        ra[i] = (ap && kk < K) ? ap[kk] : 0.0;
-      }
-#pragma unroll
-      for (int i = 0; i < EB; ++i) {
-        const int64_t kk = k0 + bk + i;
-        rb[i] = (bp && kk < K) ? bp[kk] : 0.0;
-      }
-    };
-    gload(0);
-    for (int64_t k0 = 0; k0 < K; k0 += kLuK) {
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < EA; ++i) As[(ak + i) * kLuTileM + ar] = ra[i];
-#pragma unroll
-      for (int i = 0; i < EB; ++i) Bs[(bk + i) * TN + br] = rb[i];
-      __syncthreads();
-      if (k0 + kLuK < K) gload(k0 + kLuK);
-#pragma unroll
-      for (int ks = 0; ks < kLuK; ks += 4) {
-        const int kr = ks + lq;
-        double a[2], b[NB];
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt) a[mt] = -As[kr * kLuTileM + wm + 16 * mt + l16];
-#pragma unroll
-        for (int nt = 0; nt < NB; ++nt) b[nt] = Bs[kr * TN + wn + 16 * nt + l16];
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-          for (int nt = 0; nt < NB; ++nt)
-            acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-      for (int nt = 0; nt < NB; ++nt) {
-        const int64_t col = n0 + wn + 16 * nt + l16;
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          if (crow[mt][e] && col < Nc) crow[mt][e][col] = acc[mt][nt][e];
-      }
-  }
-  __syncthreads();
+constexpr int kCholKs = 16;  // k slab of the MFMA updates
+constexpr int kCholLds = 64 * 65 + 64 + kCholKs * 64 + kCholKs * 128;
+
+// v of lane src (uniform across the wave) in every lane
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
 }
 
-template <int NT, int TN = 64 * NT / 256>
+template <int NT, int NTB = 2>
 __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* W, double* u,
                                                  double* lds, int32_t* info,
                                                  int64_t* clk = nullptr) {
+  static_assert(NT == 256, "four waves: the 16-row blocks of D and of the tiles below");
+  static_assert(NTB == 1 || NTB == 2, "rows of a tile below the diagonal per wave: 16 NTB");
+  constexpr int KS = kCholKs;
+  constexpr int RT = 64 * NTB;            // rows of R per tile
+  constexpr int EB = RT * KS / NT;        // L_R< slab: consecutive k of one row per thread
+  constexpr int TPB = KS / EB;
+  static_assert(4 * 16 * NTB * 17 <= KS * (64 + RT), "substitution rows fit the slab space");
   __shared__ int s_bad;
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int nwave = NT >> 6;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, lq = lane >> 4;
   int64_t t0 = wall_clock64();
   {
     const double dn = (double)n;
     add_flops(dn * dn * dn / 3.0 + 2.0 * dn * dn);
   }
+  double* D = lds;                // 64 x 65: M_JJ, then L_JJ (lower)
+  double* ys = D + 64 * 65;       // 64: c, then y_J
+  double* As = ys + 64;           // [KS][64]: slab of L_J<
+  double* Bs = As + KS * 64;      // [KS][RT]: slab of L_R< (step 1: of y_<)
+  double* Sw = As + wave * 16 * NTB * 17;  // step 3: this wave's rows, 16 columns (stride 17)
   if (tid == 0) s_bad = 0;
-  double* Lk = lds;        // 16 x 17, the factored diagonal block
-  double* yk = lds + 272;  // its 16 entries of y
-  double* stage = lds + 320;
-  constexpr int64_t kOuter = 128;
-  for (int64_t K0 = 0; K0 < n; K0 += kOuter) {
-    const int64_t ko = (n - K0) < kOuter ? (n - K0) : kOuter;
-    const int64_t K1 = K0 + ko;
-    for (int64_t k = K0; k < K1; k += 16) {
-      const int64_t nb = (K1 - k) < 16 ? (K1 - k) : 16;
-      // 1. the diagonal block and y_k by wave 0 (lane = row)
+  const int ar = tid >> 2, ak = (tid & 3) * 4;   // L_J< slab: 64 rows x 4 threads x 4 k
+  const int br = tid / TPB, bk = (tid % TPB) * EB;
+  // this wave's lower 16 x 16 blocks of D: w, w + 4, w + 8 of
+  // (0,0) (1,0) (1,1) (2,0) (2,1) (2,2) (3,0) (3,1) (3,2) (3,3)
+  int qi[3], qj[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int b = wave + 4 * q;
+    const int i = b < 1 ? 0 : b < 3 ? 1 : b < 6 ? 2 : 3;
+    qi[q] = i;
+    qj[q] = b - i * (i + 1) / 2;
+  }
+  const int nq = wave < 2 ? 3 : 2;  // blocks 0..9
+  for (int64_t J0 = 0; J0 < n; J0 += 64) {
+    const int jb = (int)((n - J0) < 64 ? (n - J0) : 64);
+    const double* aprow = ar < jb ? W + (J0 + ar) * ld : nullptr;
+    // ---- 1. D = M_JJ - L_J< L_J<^T, c = b_J - L_J< y_<
+    {
+      double4_t acc[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = 16 * qi[q] + 4 * e + lq, c = 16 * qj[q] + l16;
+          acc[q][e] = (q < nq && r < jb && c <= r) ? W[(J0 + r) * ld + J0 + c] : 0.0;
+        }
+      double cacc = (wave == 0 && lane < jb) ? W[(J0 + lane) * ld + n] : 0.0;
+      double ra[4], ry = 0.0;
+      auto gload = [&](int64_t k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t kk = k0 + ak + i;
+          ra[i] = (aprow && kk < J0) ? aprow[kk] : 0.0;
+        }
+        if (tid < KS) ry = (k0 + tid < J0) ? u[k0 + tid] : 0.0;
+      };
+      if (J0 > 0) gload(0);
+      for (int64_t k0 = 0; k0 < J0; k0 += KS) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) As[(ak + i) * 64 + ar] = ra[i];
+        if (tid < KS) Bs[tid] = ry;
+        __syncthreads();
+        if (k0 + KS < J0) gload(k0 + KS);
+#pragma unroll
+        for (int ks = 0; ks < KS; ks += 4) {
+          const int kr = ks + lq;
+#pragma unroll
+          for (int q = 0; q < 3; ++q)
+            if (q < nq) {  // uniform across the wave
+              const double a = -As[kr * 64 + 16 * qi[q] + l16];
+              const double b = As[kr * 64 + 16 * qj[q] + l16];
+              acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+            }
+        }
+        if (wave == 0) {
+#pragma unroll
+          for (int kk = 0; kk < KS; ++kk) cacc -= As[kk * 64 + lane] * Bs[kk];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if (q < nq) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            D[(16 * qi[q] + 4 * e + lq) * 65 + 16 * qj[q] + l16] = acc[q][e];
+        }
+      if (wave == 0) ys[lane] = cacc;
+    }
+    __syncthreads();
+    lu_clock(clk, 1, t0);
+    // ---- 2. D = L_JJ L_JJ^T, y_J = L_JJ^{-1} c
+    for (int c0 = 0; c0 < jb; c0 += 16) {
+      const int nb = jb - c0 < 16 ? jb - c0 : 16;
       if (wave == 0) {
+        // lane r >= c0 holds D[r][c0 .. c0 + 16) (on or below the diagonal);
+        // rows of the diagonal block divide by the pivot's sqrt, the rows
+        // below it by the stored diagonal entry
+        const bool act = lane >= c0 && lane < jb;
+        const bool below = lane >= c0 + 16;
         double row[16];
 #pragma unroll
         for (int c = 0; c < 16; ++c)
-          row[c] = (lane < nb && c <= lane && c < nb) ? W[(k + lane) * ld + k + c] : 0.0;
-        double yv = lane < nb ? W[(k + lane) * ld + n] : 0.0;
+          row[c] = (act && c < nb && c0 + c <= lane) ? D[lane * 65 + c0 + c] : 0.0;
+        double yv = act ? ys[lane] : 0.0;
         bool bad = false;
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-          if (j < nb) {  // uniform: every shuffle below runs on all 64 lanes
-            const double djj = __shfl(row[j], j, 64);
+          if (j < nb) {  // uniform
+            const int pj = c0 + j;
+            const double djj = lane_value(row[j], pj);
             bad = bad || !(djj > 0.0) || !(djj < INFINITY);
             const double d = sqrt(djj);
-            const double lj = lane >= j ? row[j] / d : 0.0;  // column j of L, rows >= j
-            if (lane >= j) row[j] = lj;
-            const double yj = __shfl(yv, j, 64) / d;
-            if (lane == j) yv = yj;
+            const double ljj = djj / d;  // the pivot lane's own row[j] / d
+            const double lj = lane >= pj ? row[j] / (below ? ljj : d) : 0.0;
+            if (lane >= pj) row[j] = lj;
+            const double yj = lane_value(yv, pj) / d;
+            if (lane == pj) yv = yj;
 #pragma unroll
-            for (int c = j + 1; c < 16; ++c) {
-              const double lcj = __shfl(lj, c, 64);
-              if (lane > j && c <= lane) row[c] -= lj * lcj;
-            }
-            if (lane > j) yv -= lj * yj;
+            for (int c = j + 1; c < 16; ++c)
+              if (c < nb) {
+                const double lcj = lane_value(lj, c0 + c);
+                if (lane > pj && c0 + c <= lane) row[c] -= lj * lcj;
+              }
+            if (lane > pj) yv -= lj * yj;
           }
         }
-        if (lane < nb) {
+        if (act) {
 #pragma unroll
           for (int c = 0; c < 16; ++c)
-            if (c <= lane && c < nb) {
-              W[(k + lane) * ld + k + c] = row[c];
-              Lk[lane * 17 + c] = row[c];
-            }
-          W[(k + lane) * ld + n] = yv;
-          yk[lane] = yv;
+            if (c < nb && c0 + c <= lane) D[lane * 65 + c0 + c] = row[c];
+          ys[lane] = yv;
         }
         if (lane == 0 && bad) s_bad = 1;
       }
@@ -1417,61 +1425,167 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
         if (tid == 0) info[0] = TTX_ENOTSPD;
         return;
       }
-      // 2. rows below: x L_kk^T = w (forward substitution along the row),
-      //    then b_i -= x . y_k
-      for (int64_t i = k + nb + tid; i < n; i += NT) {
-        double* wr = W + i * ld + k;
-        double x[16];
+      // rank-16 update of D's lower blocks right of this sub-panel
+      const int sb = c0 >> 4;
+      if (c0 + 16 < jb) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c) x[c] = c < nb ? wr[c] : 0.0;
-        double bi = W[i * ld + n];
+        for (int q = 0; q < 3; ++q)
+          if (q < nq && qj[q] > sb) {  // uniform
+            double4_t acc;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-          if (c < nb) {
-            double v = x[c];
+            for (int e = 0; e < 4; ++e)
+              acc[e] = D[(16 * qi[q] + 4 * e + lq) * 65 + 16 * qj[q] + l16];
 #pragma unroll
-            for (int t = 0; t < c; ++t) v -= x[t] * Lk[c * 17 + t];
-            v /= Lk[c * 17 + c];
-            x[c] = v;
-            bi -= v * yk[c];
+            for (int ks = 0; ks < 16; ks += 4) {
+              const double a = -D[(16 * qi[q] + l16) * 65 + c0 + ks + lq];
+              const double b = D[(16 * qj[q] + l16) * 65 + c0 + ks + lq];
+              acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              D[(16 * qi[q] + 4 * e + lq) * 65 + 16 * qj[q] + l16] = acc[e];
           }
+        __syncthreads();
+      }
+    }
+    // L_JJ to W, y_J to u
+    for (int p = tid; p < 64 * 64; p += NT) {
+      const int r = p >> 6, c = p & 63;
+      if (r < jb && c <= r) W[(J0 + r) * ld + J0 + c] = D[r * 65 + c];
+    }
+    if (tid < jb) u[J0 + tid] = ys[tid];
+    lu_clock(clk, 0, t0);
+    // ---- 3. the rows below: X L_JJ^T = M_RJ - L_R< L_J<^T
+    for (int64_t R0 = J0 + 64; R0 < n; R0 += RT) {
+      const int64_t wr0 = R0 + wave * 16 * NTB;
+      // lane l's rows of the tile (null past n); its columns are J0 + 16 mt + 4 e + l / 16
+      double* rp[NTB];
+#pragma unroll
+      for (int nt = 0; nt < NTB; ++nt) {
+        const int64_t row = wr0 + 16 * nt + l16;
+        rp[nt] = row < n ? W + row * ld + J0 + lq : nullptr;
+      }
+      double4_t v[4][NTB];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NTB; ++nt)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            v[mt][nt][e] = (rp[nt] && 16 * mt + 4 * e + lq < jb) ? rp[nt][16 * mt + 4 * e] : 0.0;
+      const double* bprow = (R0 + br) < n ? W + (R0 + br) * ld : nullptr;
+      double ra[4], rb[EB];
+      auto gload = [&](int64_t k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t kk = k0 + ak + i;
+          ra[i] = (aprow && kk < J0) ? aprow[kk] : 0.0;
         }
 #pragma unroll
-        for (int c = 0; c < 16; ++c)
-          if (c < nb) wr[c] = x[c];
-        W[i * ld + n] = bi;
+        for (int i = 0; i < EB; ++i) {
+          const int64_t kk = k0 + bk + i;
+          rb[i] = (bprow && kk < J0) ? bprow[kk] : 0.0;
+        }
+      };
+      if (J0 > 0) gload(0);
+      for (int64_t k0 = 0; k0 < J0; k0 += KS) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) As[(ak + i) * 64 + ar] = ra[i];
+#pragma unroll
+        for (int i = 0; i < EB; ++i) Bs[(bk + i) * RT + br] = rb[i];
+        __syncthreads();
+        if (k0 + KS < J0) gload(k0 + KS);
+#pragma unroll
+        for (int ks = 0; ks < KS; ks += 4) {
+          const int kr = ks + lq;
+          double a[4], b[NTB];
+#pragma unroll
+          for (int mt = 0; mt < 4; ++mt) a[mt] = -As[kr * 64 + 16 * mt + l16];
+#pragma unroll
+          for (int nt = 0; nt < NTB; ++nt) b[nt] = Bs[kr * RT + wave * 16 * NTB + 16 * nt + l16];
+#pragma unroll
+          for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NTB; ++nt)
+              v[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], b[nt], v[mt][nt], 0, 0, 0);
+        }
       }
-      __syncthreads();
-      lu_clock(clk, 0, t0);
-      // 3. the rest of the outer panel (lower part), rank nb
-      chol_tile_update_t<NT, TN>(n - (k + nb), K1 - (k + nb), nb, W, ld, k + nb, k + nb, k, stage);
-      lu_clock(clk, 1, t0);
+      __syncthreads();  // the slab space becomes this step's substitution rows
+      // 16 columns at a time: lane l's element e of block (mt, nt) is
+      // V[16 mt + 4 e + l / 16][16 nt + l % 16], the B operand of k-step e
+      // of block column mt
+      const double* dl = D + l16 * 65 + lq;
+#pragma unroll
+      for (int m2 = 0; m2 < 4; ++m2) {
+#pragma unroll
+        for (int mt = 0; mt < m2; ++mt)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const double a = -dl[16 * m2 * 65 + 16 * mt + 4 * e];
+#pragma unroll
+            for (int nt = 0; nt < NTB; ++nt)
+              v[m2][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, v[mt][nt][e], v[m2][nt], 0, 0, 0);
+          }
+        if (16 * m2 < jb) {  // uniform
+#pragma unroll
+          for (int nt = 0; nt < NTB; ++nt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Sw[(16 * nt + l16) * 17 + 4 * e + lq] = v[m2][nt][e];
+          __builtin_amdgcn_wave_barrier();
+          if (lane < 16 * NTB) {
+            // row lane of the wave: x L_bb^T = w against the diagonal block
+            // (mbcnt_lo(0, 0) is 0 in every lane; it keeps the compiler from
+            // holding the block's 120 uniform entries in scalar registers)
+            const double* Lb = D + 16 * m2 * 65 + 16 * m2 + __builtin_amdgcn_mbcnt_lo(0u, 0u);
+            double* sr = Sw + lane * 17;
+            double x[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+              double t = sr[c];
+#pragma unroll
+              for (int p = 0; p < c; ++p) t -= x[p] * Lb[c * 65 + p];
+              x[c] = 16 * m2 + c < jb ? t / Lb[c * 65 + c] : 0.0;
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) sr[c] = x[c];
+          }
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int nt = 0; nt < NTB; ++nt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[m2][nt][e] = Sw[(16 * nt + l16) * 17 + 4 * e + lq];
+        }
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NTB; ++nt)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (rp[nt] && 16 * mt + 4 * e + lq < jb) rp[nt][16 * mt + 4 * e] = v[mt][nt][e];
     }
-    // trailing lower triangle, rank ko
-    chol_tile_update_t<NT, TN>(n - K1, n - K1, ko, W, ld, K1, K1, K0, stage);
+    __syncthreads();  // D is read above until here
     lu_clock(clk, 3, t0);
   }
-  // L^T u = y, 64-row blocks from the bottom
+  // L^T u = y, 64-row blocks from the bottom (y is in u)
   double* T = lds;  // 64 x 65: T[r][c] = L[j0 + r][j0 + c], r >= c
   double* ub = lds + 64 * 65;
-  for (int64_t i = tid; i < n; i += NT) u[i] = W[i * ld + n];
-  __syncthreads();
   for (int64_t j1 = n; j1 > 0; j1 -= 64) {
     const int64_t j0 = j1 - 64 > 0 ? j1 - 64 : 0;
     const int64_t nbk = j1 - j0;
     if (lane < nbk) {
-      for (int64_t r = wave; r < nbk; r += nwave)
+      for (int64_t r = wave; r < nbk; r += NT / 64)
         T[r * 65 + lane] = lane <= r ? W[(j0 + r) * ld + j0 + lane] : 0.0;
     }
     __syncthreads();
     if (wave == 0) {
       const int64_t row = j0 + lane;
       double yv = row < j1 ? u[row] : 0.0;
-      for (int64_t j = j1 - 1; j >= j0; --j) {
-        const int jj = (int)(j - j0);
-        const double xj = __shfl(yv, jj, 64) / T[jj * 65 + jj];
-        if (row == j) yv = xj;
-        if (row < j) yv -= xj * T[jj * 65 + lane];  // L[j][row] = L^T[row][j]
+      for (int jj = (int)nbk - 1; jj >= 0; --jj) {
+        const double xj = lane_value(yv, jj) / T[jj * 65 + jj];
+        if (lane == jj) yv = xj;
+        if (lane < jj) yv -= xj * T[jj * 65 + lane];  // L[j][row] = L^T[row][j]
       }
       if (row < j1) { u[row] = yv; ub[lane] = yv; }
     }

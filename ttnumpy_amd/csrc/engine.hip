@@ -45,6 +45,7 @@ constexpr int64_t kMixedMinN = 256;  // smaller direct solves stay in FP64
 // (chol32_apply_inverse)
 constexpr int64_t kMixedMaxN = 4096;
 static_assert(kMixedMaxN + 4480 <= (int64_t)(kEngLds / 4), "LDS of the FP32 triangular solves");
+static_assert(kCholLds * 8 <= (int)kEngLds, "LDS of the Cholesky solve");
 
 __device__ __forceinline__ double* xcore(const ttx_slot* s, int64_t k) { return s->x + k * s->cap_core; }
 __device__ __forceinline__ double* zcore(const ttx_slot* s, int64_t k) { return s->z + k * s->cap_zcore; }
@@ -1151,7 +1152,7 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_chol32_kernel(ttx_slot* sl
   if (threadIdx.x == 0 && g_flops != 0.0) s->flops[2] += g_flops;
 }
 
-__device__ void chol_body(ttx_slot* s, double* lds) {
+__device__ __forceinline__ void chol_body(ttx_slot* s, double* lds) {
   __shared__ double red[16];
   __shared__ int32_t s_info;
   if (!solve_step_active(s) || s->sstage != kSolveChol) return;
@@ -1165,12 +1166,12 @@ __device__ void chol_body(ttx_slot* s, double* lds) {
   if (threadIdx.x == 0) s_info = TTX_OK;
   __syncthreads();
   stamp(s, 1, t0);
-  // the tile shape of the trailing updates is the slot's (picked by the
+  // the tile height below the diagonal blocks is the slot's (picked by the
   // host's tuner); both give the same values
   if (s->tiles & 1)
-    block_chol_solve<kEngThreads, 32>(N, N + 1, W, scr(s, TTX_SCR_U), lds, &s_info, &s->clk[13]);
+    block_chol_solve<kEngThreads, 1>(N, N + 1, W, scr(s, TTX_SCR_U), lds, &s_info, &s->clk[13]);
   else
-    block_chol_solve<kEngThreads>(N, N + 1, W, scr(s, TTX_SCR_U), lds, &s_info, &s->clk[13]);
+    block_chol_solve<kEngThreads, 2>(N, N + 1, W, scr(s, TTX_SCR_U), lds, &s_info, &s->clk[13]);
   __syncthreads();
   stamp(s, 2, t0);
   if (s_info == TTX_ENOTSPD) {  // not positive definite: LU takes over this tick

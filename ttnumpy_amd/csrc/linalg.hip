@@ -296,12 +296,12 @@ __global__ __launch_bounds__(1024) void identity_check_kernel(int64_t n, const d
 // LU with partial pivoting and the solve of [M | b] (row-major) by one block:
 // the same blocked routine the solver engine runs per request.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(512) void chol_solve_kernel(int64_t n, int64_t ld, double* W,
+__global__ __launch_bounds__(256) void chol_solve_kernel(int64_t n, int64_t ld, double* W,
                                                          double* u, int32_t* info) {
-  __shared__ double lds[4480];
+  extern __shared__ double lds[];
   if (threadIdx.x == 0) info[0] = TTX_OK;
   __syncthreads();
-  block_chol_solve<512>(n, ld, W, u, lds, info);
+  block_chol_solve<256>(n, ld, W, u, lds, info);
 }
 
 __global__ __launch_bounds__(512) void lu_solve_kernel(int64_t n, int64_t ld, double* W, double* u,
@@ -1150,7 +1150,13 @@ extern "C" int ttx_lu_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, int6
 extern "C" int ttx_chol_solve_f64(ttx_stream_t stream, int64_t n, double* Mb, int64_t ld, double* u,
                                   int32_t* info) {
   if (n < 1 || ld < n + 1) return TTX_EINVAL;
-  hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(512), 0, (hipStream_t)stream, n, ld, Mb, u, info);
+  static bool attr_set = false;
+  if (!attr_set) {
+    if (set_lds_attr((const void*)chol_solve_kernel) != TTX_OK) return TTX_EHIP;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(256), kCholLds * sizeof(double),
+                     (hipStream_t)stream, n, ld, Mb, u, info);
   return launch_status();
 }
 
