@@ -44,6 +44,8 @@ constexpr int64_t kMixedMinN = 256;  // smaller direct solves stay in FP64
 // floats of working space and index rows with 32-bit offsets
 // (chol32_apply_inverse)
 constexpr int64_t kMixedMaxN = 4096;
+// ranks whose partial residuals the rank rule forms per operator application
+constexpr int kRankChunkMax = 16;
 static_assert(kMixedMaxN + 4480 <= (int64_t)(kEngLds / 4), "LDS of the FP32 triangular solves");
 static_assert(kCholLds * 8 <= (int)kEngLds, "LDS of the Cholesky solve");
 
@@ -644,6 +646,7 @@ __device__ int64_t choose_rank_blk(ttx_slot* s, const LocOp& op, const double* b
   const double bn = bnorm2(bloc, nullptr, N, red);
   const double a = s->damp * exact, f = 0.5 * s->trunc_tol * bn;
   const double thr = a > f ? a : f;
+  __shared__ double red_many[kRankChunkMax * (kEngThreads / 64)];
   double* partial = scr(s, TTX_SCR_TMP);
   double* terms = scr(s, TTX_SCR_CHK);
   double* contrib = terms + s->chunk * s->cap_loc;
@@ -658,17 +661,27 @@ __device__ int64_t choose_rank_blk(ttx_slot* s, const LocOp& op, const double* b
     }
     __syncthreads();
     op.apply(terms, cb, contrib, 1.0, 0.0);
-    for (int64_t j = 0; j < cb; ++j) {
-      const double* cj = contrib + j * N;
-      double t = 0.0;
-      for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
-        const double v = partial[i] + cj[i];
-        partial[i] = v;
-        t += v * v;
-      }
-      const double nrm = sqrt(block_sum(t, red));
-      if (nrm <= thr) return c0 + j + 1;
+    // the cb partial residuals' norms in one pass and one reduction; each
+    // norm sums the same terms in the same order as a reduction per rank
+    double t[kRankChunkMax];
+#pragma unroll
+    for (int j = 0; j < kRankChunkMax; ++j) t[j] = 0.0;
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+      double v = partial[i];
+#pragma unroll
+      for (int j = 0; j < kRankChunkMax; ++j)
+        if (j < cb) {
+          v += contrib[j * N + i];
+          t[j] += v * v;
+        }
+      partial[i] = v;
     }
+    block_sum_many<kRankChunkMax>(t, (int)cb, red_many);
+    int hit = -1;  // the first rank whose residual meets thr
+#pragma unroll
+    for (int j = kRankChunkMax - 1; j >= 0; --j)
+      if (j < cb && sqrt(t[j]) <= thr) hit = j;
+    if (hit >= 0) return c0 + hit + 1;
   }
   *capped = (mr > 0 && mr < K) ? 1 : 0;
   return limit;
@@ -681,7 +694,7 @@ __device__ __forceinline__ void stamp(ttx_slot* s, int i, int64_t& t0) {
   t0 = t;
 }
 
-__device__ void fail(ttx_slot* s, int64_t code) {
+__device__ __forceinline__ void fail(ttx_slot* s, int64_t code) {
   if (threadIdx.x == 0) s->status = code;
 }
 
@@ -828,7 +841,7 @@ __device__ double mixed_residual(const LocOp& op, const double* u, const double*
   return sqrt(block_sum(sq, red));
 }
 
-__device__ bool solve_step_active(const ttx_slot* s) {
+__device__ __forceinline__ bool solve_step_active(const ttx_slot* s) {
   return s->status == TTX_SLOT_RUNNING && (s->phase == TTX_PH_FWD || s->phase == TTX_PH_BWD);
 }
 
@@ -1190,7 +1203,7 @@ __device__ __forceinline__ void chol_body(ttx_slot* s, double* lds) {
   exact_local_residual(s, k, N, red, t0);
 }
 
-__device__ void lu_body(ttx_slot* s, double* lds) {
+__device__ __forceinline__ void lu_body(ttx_slot* s, double* lds) {
   __shared__ double red[16];
   __shared__ int32_t s_info;
   if (!solve_step_active(s) || s->sstage != kSolveLu) return;
@@ -1339,7 +1352,7 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_lu_kernel(ttx_slot* slots)
 }
 
 // truncation, enrichment and shifts after a solve; QR sweeps and z fit
-__device__ void post_body(ttx_slot* s) {
+__device__ __forceinline__ void post_body(ttx_slot* s) {
   __shared__ double red[16];
   if (s->status != TTX_SLOT_RUNNING) return;
   const int64_t ph = s->phase, d = s->d, k = s->k;
@@ -1739,7 +1752,9 @@ extern "C" int64_t ttx_engine_field_offset(int which) {
 
 // Fill the layout fields of a host-side slot from its problem fields.
 extern "C" int ttx_engine_plan(ttx_slot* s) {
-  if (s->d < 2 || s->d > TTX_MAXD || s->rmax < 1 || s->kick < 1 || s->chunk < 1) return TTX_EINVAL;
+  if (s->d < 2 || s->d > TTX_MAXD || s->rmax < 1 || s->kick < 1 || s->chunk < 1 ||
+      s->chunk > kRankChunkMax)
+    return TTX_EINVAL;
   int64_t nmax = 1, ramax = 1, rbmax = 1;
   for (int64_t i = 0; i < s->d; ++i) nmax = s->n[i] > nmax ? s->n[i] : nmax;
   for (int64_t i = 0; i <= s->d; ++i) {

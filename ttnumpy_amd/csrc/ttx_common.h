@@ -65,6 +65,34 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// block_sum of v[0..m) at once (m <= M): each value is reduced by the same
+// lane tree and wave order as block_sum, behind one pair of barriers.  `red`
+// must hold M * blockDim.x / 64 doubles of shared memory.
+template <int M>
+__device__ __forceinline__ void block_sum_many(double* v, int m, double* red) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int nwave = (blockDim.x + kWave - 1) / kWave;
+#pragma unroll
+  for (int j = 0; j < M; ++j)
+    if (j < m) v[j] = wave_sum(v[j]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+      if (j < m) red[wave * M + j] = v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < M; ++j)
+    if (j < m) {
+      double t = 0.0;
+      for (int w = 0; w < nwave; ++w) t += red[w * M + j];
+      v[j] = t;
+    }
+  __syncthreads();  // red may be reused at once
+}
+
 // Sum over the whole block; every thread gets the result.  `red` must hold
 // blockDim.x / 64 doubles of shared memory.  Fixed order, so deterministic.
 __device__ __forceinline__ double block_sum(double v, double* red) {

@@ -154,6 +154,8 @@ class Engine:
         self.dev = D.device()
         self.nslots = int(nslots)
         self.rmax = int(rmax)
+        if not 1 <= chunk <= 16:
+            raise ValueError("chunk (ranks tested per operator application) must be in 1..16")
         self.chunk = int(chunk)
         self.reserve = int(hbm_reserve_bytes)
         if minres_slice < 1:
