@@ -404,24 +404,23 @@ class Engine:
     def _retire(self, i: int, push: bool = True) -> None:
         s = self.host[i]
         req = self.slot_req[i]
-        self.kernel_flops += np.array(list(s.flops))
+        self.kernel_flops += np.ctypeslib.as_array(s.flops)
         self.minres_iters += int(s.minres_iters)
         self.lu_solves += int(s.lu_solves)
         self.mixed_solves += int(s.mixed_solves)
         self.mixed_refines += int(s.mixed_refines)
         self.mixed_fallbacks += int(s.mixed_fallbacks)
-        self.stage_ms += np.array(list(s.clk), dtype=np.float64) * 1e-5  # 100 MHz ticks
+        self.stage_ms += np.ctypeslib.as_array(s.clk) * 1e-5  # 100 MHz ticks
         if s.status == SLOT_DONE:
             d = s.d
-            xs = self.arena[i][:d * s.cap_core].clone()
-            cores = []
-            for k in range(d):
-                r0, n, r1 = s.rx[k], s.n[k], s.rx[k + 1]
-                off = k * s.cap_core
-                cores.append(xs[off:off + r0 * n * r1].view(r0, n, 1, r1))
-            x = TensorTrain(cores)
+            cap = s.cap_core
+            bonds = list(s.rx[:d + 1])
+            modes = list(s.n[:d])
+            xs = self.arena[i][:d * cap].clone()
+            cores = [xs[k * cap:k * cap + bonds[k] * modes[k] * bonds[k + 1]].view(
+                bonds[k], modes[k], 1, bonds[k + 1]) for k in range(d)]
+            x = TensorTrain._from_device_cores(cores, bonds, modes)
             x.orth_center = 0
-            x.max_rank = max(x.bonds)
             info = SolverInfo()
             nsw = int(s.sweep) + 1
             info.num_sweeps_run = nsw

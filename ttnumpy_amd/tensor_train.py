@@ -379,6 +379,23 @@ class TensorTrain:
         self.bonds = bonds
         self.tt_type = tt_type
 
+    @classmethod
+    def _from_device_cores(cls, cores: Sequence[torch.Tensor], bonds: Sequence[int],
+                           modes: Sequence[int]) -> "TensorTrain":
+        """A TT over FP64 device cores of shape (bonds[k], modes[k], 1, bonds[k+1])
+        that their producer already laid out and checked (the engine's results):
+        the constructor's validation and copies are skipped."""
+        self = cls.__new__(cls)
+        self.n_cores = len(cores)
+        self.cores = list(cores)
+        self.bonds = list(bonds)
+        self.max_rank = max(self.bonds)
+        self.orth_center = None
+        self.row_modes = list(modes)
+        self.col_modes = [1] * len(cores)
+        self.tt_type = TensorTrainType.TT
+        return self
+
     # -- bookkeeping
     def __len__(self) -> int:
         return self.n_cores
