@@ -88,7 +88,8 @@ typedef struct ttx_slot {
   // and QR, 12 z shift; inside the LU: 13 panels, 14 in-panel updates, 15 block rows,
   // 16 trailing updates, 17 back substitution (inside the Cholesky: 13 diagonal-block
   // factorizations, 14 diagonal-block updates, 16 rows below them, 17 back
-  // substitution); mixed direct solves: 18 FP32 factor, 19 refinement; 20-23 spare
+  // substitution); mixed direct solves: 18 FP32 factor, 19 refinement; inside FP64
+  // MINRES: 20 operator applications, 21 vector updates and reductions; 22-23 spare
   int64_t clk[24];
 } ttx_slot;
 

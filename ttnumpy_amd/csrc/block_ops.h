@@ -19,9 +19,10 @@ __device__ __forceinline__ void add_flops(double f) {
 
 // ---------------------------------------------------------------------------
 // Batched strided GEMM by one block: C[b] = alpha A[b] B[b] + beta C[b].
-// Each wave owns 32x32 output tiles (2x2 MFMA 16x16x4 f64 blocks) and reads
-// its operands straight from memory with the given strides.  No barrier
-// inside: callers separate dependent calls with __syncthreads().
+// Each wave owns 32x32 output tiles (2x2 MFMA 16x16x4 f64 blocks; blocks
+// wholly past M or N are skipped) and reads its operands straight from memory
+// with the given strides.  No barrier inside: callers separate dependent
+// calls with __syncthreads().
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void bgemm(int64_t batch, int64_t M, int64_t N, int64_t K, double alpha,
                                       const double* A, int64_t a_ms, int64_t a_ks, int64_t a_bs,
@@ -43,6 +44,9 @@ __device__ __forceinline__ void bgemm(int64_t batch, int64_t M, int64_t N, int64
     double* Cb = C + bt * c_bs;
     const int64_t ma = m0 + r, mb = m0 + 16 + r, na = n0 + r, nb = n0 + 16 + r;
     const bool va = ma < M, vb = mb < M, wa = na < N, wb = nb < N;
+    // 16 x 16 blocks of the tile wholly past M or N are not computed (uniform
+    // across the wave; their outputs are never stored)
+    const bool m2 = m0 + 16 < M, n2 = n0 + 16 < N;
     double4_t c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
     // two k-steps of operands are loaded before their MFMAs (8 loads per
     // lane in flight); the accumulation order is k ascending
@@ -69,9 +73,9 @@ __device__ __forceinline__ void bgemm(int64_t batch, int64_t M, int64_t N, int64
       for (int q = 0; q < 2; ++q) {
         if (k0 + 4 * q < K) {
           c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b0[q], c00, 0, 0, 0);
-          c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b1[q], c01, 0, 0, 0);
-          c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b0[q], c10, 0, 0, 0);
-          c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b1[q], c11, 0, 0, 0);
+          if (n2) c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b1[q], c01, 0, 0, 0);
+          if (m2) c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b0[q], c10, 0, 0, 0);
+          if (m2 && n2) c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b1[q], c11, 0, 0, 0);
         }
       }
     }
@@ -126,6 +130,7 @@ __device__ __forceinline__ void bgemm32(int64_t batch, int64_t M, int64_t N, int
     float* Cb = C + bt * c_bs;
     const int64_t ma = m0 + r, mb = m0 + 16 + r, na = n0 + r, nb = n0 + 16 + r;
     const bool va = ma < M, vb = mb < M, wa = na < N, wb = nb < N;
+    const bool m2 = m0 + 16 < M, n2 = n0 + 16 < N;  // as in bgemm
     floatx4_t c00 = {0.0f, 0.0f, 0.0f, 0.0f}, c01 = c00, c10 = c00, c11 = c00;
     const float* pa0 = Ab + ma * a_ms + kq * a_ks;
     const float* pa1 = Ab + mb * a_ms + kq * a_ks;
@@ -150,9 +155,9 @@ __device__ __forceinline__ void bgemm32(int64_t batch, int64_t M, int64_t N, int
       for (int q = 0; q < 2; ++q) {
         if (k0 + 4 * q < K) {
           c00 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[q], b0[q], c00, 0, 0, 0);
-          c01 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[q], b1[q], c01, 0, 0, 0);
-          c10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[q], b0[q], c10, 0, 0, 0);
-          c11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[q], b1[q], c11, 0, 0, 0);
+          if (n2) c01 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[q], b1[q], c01, 0, 0, 0);
+          if (m2) c10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[q], b0[q], c10, 0, 0, 0);
+          if (m2 && n2) c11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[q], b1[q], c11, 0, 0, 0);
         }
       }
     }

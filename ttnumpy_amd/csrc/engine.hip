@@ -419,7 +419,7 @@ __device__ void minres_start(const Op& op, int64_t n, const double* b, const dou
 // up to `budget` iterations of the current attempt; returns the number done
 template <class Op>
 __device__ int64_t minres_run(const Op& op, int64_t n, double* V, double* red, double* st,
-                              int64_t budget, int64_t* iters) {
+                              int64_t budget, int64_t* iters, int64_t* clk = nullptr) {
   constexpr int kB = 4;
   const double eps = DBL_EPSILON;
   const int64_t T = blockDim.x;
@@ -434,10 +434,16 @@ __device__ int64_t minres_run(const Op& op, int64_t n, double* V, double* red, d
   __syncthreads();  // every thread has read st before thread 0 rewrites it
   MrVec m(V, n, pr, pw);
   int64_t done = 0;
+  int64_t tc = clk ? wall_clock64() : 0;
   while (istop == 0 && itn < maxiter && done < budget) {
     ++itn;
     ++done;
     op.apply(m.v, 1, m.yA, 1.0, 0.0);
+    if (clk) {  // clk[0]: operator applications, clk[1]: the rest of the iterations
+      const int64_t t = wall_clock64();
+      if (threadIdx.x == 0) clk[0] += t - tc;
+      tc = t;
+    }
     const double f1 = itn >= 2 ? beta / oldb : 0.0;
     double s = 0.0;
     for (int64_t i0 = threadIdx.x; i0 < n; i0 += kB * T) {
@@ -554,6 +560,11 @@ __device__ int64_t minres_run(const Op& op, int64_t n, double* V, double* red, d
       if (phistop > 0.0 && phibar <= phistop * beta1) istop = 1;
     }
     add_flops(18.0 * (double)n);
+    if (clk) {
+      const int64_t t = wall_clock64();
+      if (threadIdx.x == 0) clk[1] += t - tc;
+      tc = t;
+    }
   }
   if (istop == 0 && itn >= maxiter) istop = 6;
   if (threadIdx.x == 0) {
@@ -921,7 +932,8 @@ __device__ void solve_body(ttx_slot* s) {
   // slot.minres_slice iterations in this tick
   int64_t budget = s->minres_slice > 0 ? s->minres_slice : 128;
   while (true) {
-    budget -= minres_run(op, N, V, red, st, budget, threadIdx.x == 0 ? &s->minres_iters : nullptr);
+    budget -= minres_run(op, N, V, red, st, budget, threadIdx.x == 0 ? &s->minres_iters : nullptr,
+                         &s->clk[20]);
     if (st[kMrIstop] == 0.0) break;  // out of budget: resume next tick
     op.apply(V, 1, tmp, 1.0, 0.0);
     const double res = bnorm2(tmp, bloc, N, red);
