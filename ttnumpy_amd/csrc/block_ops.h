@@ -103,6 +103,79 @@ __device__ __forceinline__ void bgemm(int64_t batch, int64_t M, int64_t N, int64
   }
 }
 
+// C[b][m][n] = alpha sum_k A[m][k] B[b][k][n] + beta C[b][m][n] for M <= 16
+// with A shared by the batch (the operator-core product of the local matvec:
+// M = R n, K = n R'): the batch's columns are numbered as one index
+// j = b N + n, so 16 x 32 output tiles run across batch boundaries instead of
+// leaving most of a tile per small product empty, and a wave needs half or
+// fewer of bgemm's dependent load round trips.  Each output is the same
+// MFMA chain (k ascending, four per step) as in bgemm.  No barrier inside.
+__device__ __forceinline__ void bgemm_shared_a_m16(int64_t batch, int64_t M, int64_t N, int64_t K,
+                                                   double alpha, const double* A, int64_t a_ms,
+                                                   int64_t a_ks, const double* B, int64_t b_ks,
+                                                   int64_t b_ns, int64_t b_bs, double beta,
+                                                   double* C, int64_t c_ms, int64_t c_ns,
+                                                   int64_t c_bs, bool count_flops = true) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  const int r = lane & 15, kq = lane >> 4;
+  const int64_t cols = batch * N;
+  if (count_flops) add_flops(2.0 * (double)batch * (double)M * (double)N * (double)K);
+  const bool va = r < M;
+  for (int64_t t = wave; t < ((cols + 31) >> 5); t += nw) {
+    const int64_t j0 = t << 5;
+    const bool n2 = j0 + 16 < cols;  // uniform
+    int64_t jb[2];
+    bool wv[2];
+    const double* pb[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t j = j0 + 16 * h + r;
+      wv[h] = j < cols;
+      const int64_t bt = wv[h] ? j / N : 0, n = wv[h] ? j - bt * N : 0;
+      jb[h] = bt * c_bs + n * c_ns;
+      pb[h] = B + bt * b_bs + n * b_ns + kq * b_ks;
+    }
+    const double* pa = A + r * a_ms + kq * a_ks;
+    double4_t c0 = {0.0, 0.0, 0.0, 0.0}, c1 = c0;
+    const int64_t sa = 4 * a_ks, sb = 4 * b_ks;
+    for (int64_t k0 = 0; k0 < K; k0 += 8) {
+      double a[2], b0[2], b1[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const bool kin = k0 + 4 * q + kq < K;
+        a[q] = (kin && va) ? pa[q * sa] : 0.0;
+        b0[q] = (kin && wv[0]) ? pb[0][q * sb] : 0.0;
+        b1[q] = (kin && wv[1]) ? pb[1][q * sb] : 0.0;
+      }
+      pa += 2 * sa;
+      pb[0] += 2 * sb;
+      pb[1] += 2 * sb;
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        if (k0 + 4 * q < K) {
+          c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], b0[q], c0, 0, 0, 0);
+          if (n2) c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], b1[q], c1, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t row = 4 * e + kq;
+      if (row < M) {
+        if (wv[0]) {
+          double* d = C + jb[0] + row * c_ms;
+          *d = beta == 0.0 ? alpha * c0[e] : alpha * c0[e] + beta * (*d);
+        }
+        if (wv[1]) {
+          double* d = C + jb[1] + row * c_ms;
+          *d = beta == 0.0 ? alpha * c1[e] : alpha * c1[e] + beta * (*d);
+        }
+      }
+    }
+  }
+}
+
 // bgemm in single precision: FP32 operands and FP32 accumulation on
 // v_mfma_f32_16x16x4f32.  Its operand layouts are those of the f64
 // instruction (A lane l = A[l%16][l/16], B lane l = B[l/16][l%16]) but its

@@ -132,8 +132,12 @@ struct LocOp {
   __device__ void apply(const double* y, int64_t B, double* out, double alpha, double beta) const {
     bgemm(1, B * rl * nc, Rb * fr, rr, 1.0, y, rr, 1, 0, phi, Rb * fr, 1, 0, 0.0, s1, Rb * fr, 1, 0);
     __syncthreads();
-    bgemm(B * rl, Ra * nr, fr, nc * Rb, 1.0, A, nc * Rb, 1, 0, s1, fr, 1, nc * Rb * fr, 0.0, s2, fr,
-          1, Ra * nr * fr);
+    if (Ra * nr <= 16)
+      bgemm_shared_a_m16(B * rl, Ra * nr, fr, nc * Rb, 1.0, A, nc * Rb, 1, s1, fr, 1, nc * Rb * fr,
+                         0.0, s2, fr, 1, Ra * nr * fr);
+    else
+      bgemm(B * rl, Ra * nr, fr, nc * Rb, 1.0, A, nc * Rb, 1, 0, s1, fr, 1, nc * Rb * fr, 0.0, s2,
+            fr, 1, Ra * nr * fr);
     __syncthreads();
     bgemm(B, fl, nr * fr, rl * Ra, alpha, psi, 1, fl, 0, s2, nr * fr, 1, rl * Ra * nr * fr, beta,
           out, nr * fr, 1, fl * nr * fr);
