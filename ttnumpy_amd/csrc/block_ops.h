@@ -1696,147 +1696,143 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
 // ---------------------------------------------------------------------------
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
-template <int NT>
-__device__ __forceinline__ void chol32_tile_update_t(int64_t Mr, int64_t Nc, int64_t K, float* W,
-                                                     int64_t ld, int64_t r0, int64_t c0, int64_t a0,
-                                                     float* lds) {
-  constexpr int TN = 64 * NT / 256;
-  constexpr int EA = kLuTileM * kLuK / NT;
-  constexpr int TPA = kLuK / EA;
-  constexpr int EB = TN * kLuK / NT;
-  constexpr int TPB = kLuK / EB;
-  if (Mr <= 0 || Nc <= 0 || K <= 0) return;
+// Left-looking, like block_chol_solve: for each 64-column block column J
+//   1. D = M_JJ - L_J< L_J<^T on MFMA (ten lower 16 x 16 blocks over four waves)
+//   2. D = L_JJ L_JJ^T by wave 0 (lane = row), 16-column sub-panels, the lower
+//      blocks right of each sub-panel taking its rank-16 update on MFMA
+//   3. each tile of 64 NTB rows R below: V^T = (M_RJ - L_R< L_J<^T)^T on MFMA
+//      into registers, then X L_JJ^T = V solved in registers (off-diagonal
+//      16-column blocks on MFMA, the diagonal ones by one lane per row).
+// Every L entry is written once and the columns left of J are read once per
+// tile.  The FP32 accumulator (lane l, element e) holds row 4 (l/16) + e of
+// its 16 x 16 block, so in step 3 lane l holds V[row l%16][16 mt + 4 (l/16) + e]:
+// as the B operand of k-step e the lanes of quarter q supply column
+// 16 mt + 4 q + e, and the A operand (L_JJ) is read with the same k order.
+// lds: kChol32Lds floats.  Ends synced.
+constexpr int kChol32Ks = 32;  // k slab of the MFMA updates
+constexpr int kChol32Lds = 64 * 65 + kChol32Ks * 64 + kChol32Ks * 128;
+
+template <int NT, int NTB = 2>
+__device__ __forceinline__ void block_chol32_factor(int64_t n, int64_t ld, float* W, float* lds,
+                                                    int32_t* info) {
+  static_assert(NT == 256, "four waves: the 16-row blocks of D and of the tiles below");
+  static_assert(NTB == 1 || NTB == 2, "rows of a tile below the diagonal per wave: 16 NTB");
+  constexpr int KS = kChol32Ks;
+  constexpr int RT = 64 * NTB;            // rows of R per tile
+  constexpr int EA = 64 * KS / NT;        // L_J< slab: consecutive k of one row per thread
+  constexpr int TPA = KS / EA;
+  constexpr int EB = RT * KS / NT;        // L_R< slab
+  constexpr int TPB = KS / EB;
+  static_assert(TPA == 4, "four threads per row of the L_J< slab");
+  static_assert(4 * 16 * NTB * 17 <= KS * (64 + RT), "substitution rows fit the slab space");
+  static_assert(64 * 65 + KS * (64 + RT) <= kChol32Lds, "LDS plan");
+  __shared__ int s_bad32;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, lq = lane >> 4;
-  const int wm = (wave & 3) * 32, wn = (wave >> 2) * 64;
-  float* As = lds;
-  float* Bs = lds + kLuK * kLuTileM;
-  const int ar = tid / TPA, ak = (tid % TPA) * EA;
-  const int br = tid / TPB, bk = (tid % TPB) * EB;
-  const int64_t tm = (Mr + kLuTileM - 1) / kLuTileM, tn = (Nc + TN - 1) / TN;
-  for (int64_t t = 0; t < tm * tn; ++t) {
-    const int64_t m0 = (t / tn) * kLuTileM, n0 = (t % tn) * TN;
-    if (r0 + m0 + kLuTileM - 1 < c0 + n0) continue;
-    // output rows of this lane: m0 + wm + 16 mt + 4 lq + e
-    float* crow[2][4];
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int64_t row = m0 + wm + 16 * mt + 4 * lq + e;
-        crow[mt][e] = row < Mr ? W + (r0 + row) * ld + c0 : nullptr;
-      }
-    float4_t acc[2][4];
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-        const int64_t col = n0 + wn + 16 * nt + l16;
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          acc[mt][nt][e] = (crow[mt][e] && col < Nc) ? crow[mt][e][col] : 0.0f;
-      }
-    const float* ap = (m0 + ar) < Mr ? W + (r0 + m0 + ar) * ld + a0 : nullptr;
-    const float* bp = (n0 + br) < Nc ? W + (c0 + n0 + br) * ld + a0 : nullptr;
-    float ra[EA], rb[EB];
-    auto gload = [&](int64_t k0) {
-#pragma unroll
-      for (int i = 0; i < EA; ++i) {
-        const int64_t kk = k0 + ak + i;
-        ra[i] = (ap && kk < K) ? ap[kk] : 0.0f;
-      }
-#pragma unroll
-      for (int i = 0; i < EB; ++i) {
-        const int64_t kk = k0 + bk + i;
-        rb[i] = (bp && kk < K) ? bp[kk] : 0.0f;
-      }
-    };
-    gload(0);
-    for (int64_t k0 = 0; k0 < K; k0 += kLuK) {
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < EA; ++i) As[(ak + i) * kLuTileM + ar] = ra[i];
-#pragma unroll
-      for (int i = 0; i < EB; ++i) Bs[(bk + i) * TN + br] = rb[i];
-      __syncthreads();
-      if (k0 + kLuK < K) gload(k0 + kLuK);
-#pragma unroll
-      for (int ks = 0; ks < kLuK; ks += 4) {
-        const int kr = ks + lq;
-        float a[2], b[4];
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt) a[mt] = -As[kr * kLuTileM + wm + 16 * mt + l16];
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) b[nt] = Bs[kr * TN + wn + 16 * nt + l16];
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-          for (int nt = 0; nt < 4; ++nt)
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-        const int64_t col = n0 + wn + 16 * nt + l16;
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          if (crow[mt][e] && col < Nc) crow[mt][e][col] = acc[mt][nt][e];
-      }
-  }
-  __syncthreads();
-}
-
-template <int NT>
-__device__ __forceinline__ void block_chol32_factor(int64_t n, int64_t ld, float* W, float* lds,
-                                                    int32_t* info) {
-  __shared__ int s_bad32;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
   {
     const double dn = (double)n;
     add_flops(dn * dn * dn / 3.0);
   }
+  float* D = lds;                // 64 x 65: M_JJ, then L_JJ (lower)
+  float* As = D + 64 * 65;       // [KS][64]: slab of L_J<
+  float* Bs = As + KS * 64;      // [KS][RT]: slab of L_R<
+  float* Sw = As + wave * 16 * NTB * 17;  // step 3: this wave's rows, 16 columns (stride 17)
   if (tid == 0) s_bad32 = 0;
-  float* Lk = lds;          // 16 x 17
-  float* stage = lds + 288;
-  constexpr int64_t kOuter = 128;
-  for (int64_t K0 = 0; K0 < n; K0 += kOuter) {
-    const int64_t ko = (n - K0) < kOuter ? (n - K0) : kOuter;
-    const int64_t K1 = K0 + ko;
-    for (int64_t k = K0; k < K1; k += 16) {
-      const int64_t nb = (K1 - k) < 16 ? (K1 - k) : 16;
+  const int ar = tid / TPA, ak = (tid % TPA) * EA;
+  const int br = tid / TPB, bk = (tid % TPB) * EB;
+  // this wave's lower 16 x 16 blocks of D: w, w + 4, w + 8 of
+  // (0,0) (1,0) (1,1) (2,0) (2,1) (2,2) (3,0) (3,1) (3,2) (3,3)
+  int qi[3], qj[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int b = wave + 4 * q;
+    const int i = b < 1 ? 0 : b < 3 ? 1 : b < 6 ? 2 : 3;
+    qi[q] = i;
+    qj[q] = b - i * (i + 1) / 2;
+  }
+  const int nq = wave < 2 ? 3 : 2;  // blocks 0..9
+  for (int64_t J0 = 0; J0 < n; J0 += 64) {
+    const int jb = (int)((n - J0) < 64 ? (n - J0) : 64);
+    const float* aprow = ar < jb ? W + (J0 + ar) * ld : nullptr;
+    float ra[EA];
+    auto aload = [&](int64_t k0) {
+#pragma unroll
+      for (int i = 0; i < EA; ++i) {
+        const int64_t kk = k0 + ak + i;
+        ra[i] = (aprow && kk < J0) ? aprow[kk] : 0.0f;
+      }
+    };
+    // ---- 1. D = M_JJ - L_J< L_J<^T
+    {
+      float4_t acc[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = 16 * qi[q] + 4 * lq + e, c = 16 * qj[q] + l16;
+          acc[q][e] = (q < nq && r < jb && c <= r) ? W[(J0 + r) * ld + J0 + c] : 0.0f;
+        }
+      if (J0 > 0) aload(0);
+      for (int64_t k0 = 0; k0 < J0; k0 += KS) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < EA; ++i) As[(ak + i) * 64 + ar] = ra[i];
+        __syncthreads();
+        if (k0 + KS < J0) aload(k0 + KS);
+#pragma unroll
+        for (int ks = 0; ks < KS; ks += 4) {
+          const int kr = ks + lq;
+#pragma unroll
+          for (int q = 0; q < 3; ++q)
+            if (q < nq) {  // uniform across the wave
+              const float a = -As[kr * 64 + 16 * qi[q] + l16];
+              const float b = As[kr * 64 + 16 * qj[q] + l16];
+              acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[q], 0, 0, 0);
+            }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if (q < nq) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            D[(16 * qi[q] + 4 * lq + e) * 65 + 16 * qj[q] + l16] = acc[q][e];
+        }
+    }
+    __syncthreads();
+    // ---- 2. D = L_JJ L_JJ^T
+    for (int c0 = 0; c0 < jb; c0 += 16) {
+      const int nb = jb - c0 < 16 ? jb - c0 : 16;
       if (wave == 0) {
+        // lane r >= c0 holds D[r][c0 .. c0 + 16) (on or below the diagonal)
+        const bool act = lane >= c0 && lane < jb;
         float row[16];
 #pragma unroll
         for (int c = 0; c < 16; ++c)
-          row[c] = (lane < nb && c <= lane && c < nb) ? W[(k + lane) * ld + k + c] : 0.0f;
+          row[c] = (act && c < nb && c0 + c <= lane) ? D[lane * 65 + c0 + c] : 0.0f;
         bool bad = false;
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-          if (j < nb) {
-            const float djj = __shfl(row[j], j, 64);
+          if (j < nb) {  // uniform
+            const int pj = c0 + j;
+            const float djj = __shfl(row[j], pj, 64);
             bad = bad || !(djj > 0.0f) || !(djj < INFINITY);
             const float d = sqrtf(djj);
-            const float lj = lane >= j ? row[j] / d : 0.0f;
-            if (lane >= j) row[j] = lj;
+            const float lj = lane >= pj ? row[j] / d : 0.0f;
+            if (lane >= pj) row[j] = lj;
 #pragma unroll
-            for (int c = j + 1; c < 16; ++c) {
-              const float lcj = __shfl(lj, c, 64);
-              if (lane > j && c <= lane) row[c] -= lj * lcj;
-            }
+            for (int c = j + 1; c < 16; ++c)
+              if (c < nb) {
+                const float lcj = __shfl(lj, c0 + c, 64);
+                if (lane > pj && c0 + c <= lane) row[c] -= lj * lcj;
+              }
           }
         }
-        if (lane < nb) {
+        if (act) {
 #pragma unroll
           for (int c = 0; c < 16; ++c)
-            if (c <= lane && c < nb) {
-              W[(k + lane) * ld + k + c] = row[c];
-              Lk[lane * 17 + c] = row[c];
-            }
+            if (c < nb && c0 + c <= lane) D[lane * 65 + c0 + c] = row[c];
         }
         if (lane == 0 && bad) s_bad32 = 1;
       }
@@ -1845,28 +1841,142 @@ __device__ __forceinline__ void block_chol32_factor(int64_t n, int64_t ld, float
         if (tid == 0) info[0] = TTX_ENOTSPD;
         return;
       }
-      for (int64_t i = k + nb + tid; i < n; i += NT) {
-        float* wr = W + i * ld + k;
-        float x[16];
+      // rank-16 update of D's lower blocks right of this sub-panel
+      const int sb = c0 >> 4;
+      if (c0 + 16 < jb) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c) x[c] = c < nb ? wr[c] : 0.0f;
+        for (int q = 0; q < 3; ++q)
+          if (q < nq && qj[q] > sb) {  // uniform
+            float4_t acc;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-          if (c < nb) {
-            float v = x[c];
+            for (int e = 0; e < 4; ++e)
+              acc[e] = D[(16 * qi[q] + 4 * lq + e) * 65 + 16 * qj[q] + l16];
 #pragma unroll
-            for (int t = 0; t < c; ++t) v -= x[t] * Lk[c * 17 + t];
-            x[c] = v / Lk[c * 17 + c];
+            for (int ks = 0; ks < 16; ks += 4) {
+              const float a = -D[(16 * qi[q] + l16) * 65 + c0 + ks + lq];
+              const float b = D[(16 * qj[q] + l16) * 65 + c0 + ks + lq];
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              D[(16 * qi[q] + 4 * lq + e) * 65 + 16 * qj[q] + l16] = acc[e];
           }
+        __syncthreads();
+      }
+    }
+    // L_JJ to W
+    for (int p = tid; p < 64 * 64; p += NT) {
+      const int r = p >> 6, c = p & 63;
+      if (r < jb && c <= r) W[(J0 + r) * ld + J0 + c] = D[r * 65 + c];
+    }
+    // ---- 3. the rows below: X L_JJ^T = M_RJ - L_R< L_J<^T
+    for (int64_t R0 = J0 + 64; R0 < n; R0 += RT) {
+      const int64_t wr0 = R0 + wave * 16 * NTB;
+      // lane l's rows of the tile (null past n); its columns are J0 + 16 mt + 4 (l / 16) + e
+      float* rp[NTB];
+#pragma unroll
+      for (int nt = 0; nt < NTB; ++nt) {
+        const int64_t row = wr0 + 16 * nt + l16;
+        rp[nt] = row < n ? W + row * ld + J0 + 4 * lq : nullptr;
+      }
+      float4_t v[4][NTB];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NTB; ++nt)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            v[mt][nt][e] = (rp[nt] && 16 * mt + 4 * lq + e < jb) ? rp[nt][16 * mt + e] : 0.0f;
+      const float* bprow = (R0 + br) < n ? W + (R0 + br) * ld : nullptr;
+      float rb[EB];
+      auto bload = [&](int64_t k0) {
+#pragma unroll
+        for (int i = 0; i < EB; ++i) {
+          const int64_t kk = k0 + bk + i;
+          rb[i] = (bprow && kk < J0) ? bprow[kk] : 0.0f;
+        }
+      };
+      if (J0 > 0) {
+        aload(0);
+        bload(0);
+      }
+      for (int64_t k0 = 0; k0 < J0; k0 += KS) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < EA; ++i) As[(ak + i) * 64 + ar] = ra[i];
+#pragma unroll
+        for (int i = 0; i < EB; ++i) Bs[(bk + i) * RT + br] = rb[i];
+        __syncthreads();
+        if (k0 + KS < J0) {
+          aload(k0 + KS);
+          bload(k0 + KS);
         }
 #pragma unroll
-        for (int c = 0; c < 16; ++c)
-          if (c < nb) wr[c] = x[c];
+        for (int ks = 0; ks < KS; ks += 4) {
+          const int kr = ks + lq;
+          float a[4], b[NTB];
+#pragma unroll
+          for (int mt = 0; mt < 4; ++mt) a[mt] = -As[kr * 64 + 16 * mt + l16];
+#pragma unroll
+          for (int nt = 0; nt < NTB; ++nt) b[nt] = Bs[kr * RT + wave * 16 * NTB + 16 * nt + l16];
+#pragma unroll
+          for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NTB; ++nt)
+              v[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mt], b[nt], v[mt][nt], 0, 0, 0);
+        }
       }
-      __syncthreads();
-      chol32_tile_update_t<NT>(n - (k + nb), K1 - (k + nb), nb, W, ld, k + nb, k + nb, k, stage);
+      __syncthreads();  // the slab space becomes this step's substitution rows
+      // k-step e of block column mt: lane l supplies column 16 mt + 4 (l/16) + e
+      const float* dl = D + l16 * 65 + 4 * lq;
+#pragma unroll
+      for (int m2 = 0; m2 < 4; ++m2) {
+#pragma unroll
+        for (int mt = 0; mt < m2; ++mt)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float a = -dl[16 * m2 * 65 + 16 * mt + e];
+#pragma unroll
+            for (int nt = 0; nt < NTB; ++nt)
+              v[m2][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, v[mt][nt][e], v[m2][nt], 0, 0, 0);
+          }
+        if (16 * m2 < jb) {  // uniform
+#pragma unroll
+          for (int nt = 0; nt < NTB; ++nt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Sw[(16 * nt + l16) * 17 + 4 * lq + e] = v[m2][nt][e];
+          __builtin_amdgcn_wave_barrier();
+          if (lane < 16 * NTB) {
+            // row lane of the wave: x L_bb^T = w against the diagonal block
+            const float* Lb = D + 16 * m2 * 65 + 16 * m2 + __builtin_amdgcn_mbcnt_lo(0u, 0u);
+            float* sr = Sw + lane * 17;
+            float x[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+              float t = sr[c];
+#pragma unroll
+              for (int p = 0; p < c; ++p) t -= x[p] * Lb[c * 65 + p];
+              x[c] = 16 * m2 + c < jb ? t / Lb[c * 65 + c] : 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) sr[c] = x[c];
+          }
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int nt = 0; nt < NTB; ++nt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[m2][nt][e] = Sw[(16 * nt + l16) * 17 + 4 * lq + e];
+        }
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NTB; ++nt)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (rp[nt] && 16 * mt + 4 * lq + e < jb) rp[nt][16 * mt + e] = v[mt][nt][e];
     }
-    chol32_tile_update_t<NT>(n - K1, n - K1, ko, W, ld, K1, K1, K0, stage);
+    __syncthreads();  // D is read above until here
   }
 }
 
@@ -1944,122 +2054,138 @@ __device__ __forceinline__ void chol32_invert_diag(int64_t n, int64_t ld, const 
 
 // x <- (L L^T)^{-1} x in FP32 with the factor from block_chol32_factor and
 // the diagonal-block inverses from chol32_invert_diag; x in global memory
-// (n floats, n <= 4096, so row offsets of 64 rows fit 32 bits).  Both triangular sweeps are left-looking over
-// 64-row blocks with the whole solution vector in LDS, so a block step is
-// one round trip to memory (its inverse block and its rows of L):
-//   forward  y_b = Dinv_b (x_b - L[b, <b] y_<b)        (row segments of L)
-//   back     z_b = Dinv_b^T (y_b - L[>b, b]^T z_>b)    (column blocks of L)
-// lds: 64 x 65 + 4 x 64 + 64 + n floats.  Ends synced.
+// (n floats, n <= 4096), the solution so far in LDS.  The row stride ld is a
+// multiple of 4, so that 16-byte loads of 4 columns are aligned.  Both sweeps
+// step over 64-row blocks and read their rows of L as one pass with 32 floats
+// of every lane in flight (16 lanes per 64-column row segment, 4 rows per
+// wave per load, 128 rows per pass):
+//   forward (right-looking)  y_b = Dinv_b y_b, then y_{>b} -= L[>b, b] y_b
+//   back    (left-looking)   z_b = Dinv_b^T (y_b - L[>b, b]^T z_>b)
+// The next block's inverse is loaded while the current one is applied.
+// lds: 64 x 65 + 4 x 64 + n floats.  Ends synced.
 template <int NT>
 __device__ __forceinline__ void chol32_apply_inverse(int64_t n, int64_t ld, const float* W,
                                                      const float* Dinv, float* x, float* lds) {
   static_assert(NT == 256, "four waves: 16 rows of a block each, 4 threads per row");
+  constexpr int P = 8;  // rows per lane per pass
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int nwave = NT >> 6;
-  const int nt = NT;
+  const int g = lane >> 4, m = lane & 15;  // row of the wave's four, column quad
   float* D = lds;               // 64 x 65: the block's inverse
-  float* part = D + 64 * 65;    // nwave x 64 partial sums
-  float* vin = part + 4 * 64;   // 64: the block's reduced right-hand side
-  float* y = vin + 64;          // n: the solution so far
+  float* part = D + 64 * 65;    // 4 x 64 partial sums of the waves
+  float* y = part + 4 * 64;     // n: the solution so far
   const int q = tid & 3, rq = tid >> 2;  // 4 threads per row of the block product
   const int64_t nblk = (n + 63) / 64;
-  for (int64_t i = tid; i < n; i += nt) y[i] = x[i];
-  __syncthreads();
-  // the 64 x 64 inverse block into D: each thread's 16 loads before its stores
-  auto stage = [&](const float* Db) {
-    float v[16];
+  for (int64_t i = tid; i < n; i += NT) y[i] = x[i];
+  float dv[16];
+  auto dload = [&](int64_t b) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = Db[(wave + 4 * j) * 64 + lane];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) D[(wave + 4 * j) * 65 + lane] = v[j];
+    for (int j = 0; j < 16; ++j) dv[j] = Dinv[b * 4096 + (wave + 4 * j) * 64 + lane];
   };
+  auto dstore = [&]() {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) D[(wave + 4 * j) * 65 + lane] = dv[j];
+  };
+  // y_b <- D y_b (trans: D^T y_b), four threads per row; ends synced
+  auto dprod = [&](int64_t j0, int64_t nbk, bool trans) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int c = 16 * q + t;
+      const float dval = trans ? D[c * 65 + rq] : D[rq * 65 + c];
+      acc += dval * (c < nbk ? y[j0 + c] : 0.0f);
+    }
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    __syncthreads();  // every read of y_b before it is overwritten
+    if (q == 0 && rq < nbk) y[j0 + rq] = acc;
+    __syncthreads();
+  };
+  // this lane's 4 columns of rows i0 + 4 p + g (zero past n)
+  auto rows_load = [&](int64_t i0, int64_t j0, float4* l) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int64_t i = i0 + 4 * p + g;
+      l[p] = i < n ? *reinterpret_cast<const float4*>(W + i * ld + j0 + 4 * m)
+                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+  };
+  dload(0);
+  __syncthreads();
   // forward: L y = x
   for (int64_t b = 0; b < nblk; ++b) {
     const int64_t j0 = 64 * b;
     const int64_t nbk = (n - j0) < 64 ? (n - j0) : 64;
-    stage(Dinv + b * 4096);
-    // v_r = y_r - L[j0 + r, 0:j0] . y[0:j0]: wave w takes its 16 rows
-    // w + 4 j eight at a time (independent sums, so their loads are in
-    // flight together); its lanes stride along the rows
-    const float* wbase = W + (j0 + wave) * ld;
-    const int rs4 = (int)(4 * ld);  // n <= 4096 keeps 16 row steps in 32 bits
+    const int64_t j1 = j0 + nbk;
+    dstore();
+    __syncthreads();
+    if (b + 1 < nblk) dload(b + 1);
+    dprod(j0, nbk, false);
+    if (j1 < n) {  // uniform; the block is full (nbk = 64)
+      const float y0 = y[j0 + 4 * m], y1 = y[j0 + 4 * m + 1], y2 = y[j0 + 4 * m + 2],
+                  y3 = y[j0 + 4 * m + 3];
+      for (int64_t i0 = j1 + 4 * P * wave; i0 < n; i0 += 4 * P * (NT / 64)) {
+        float4 l[P];
+        rows_load(i0, j0, l);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      float acc[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) acc[j] = 0.0f;
-      for (int c = lane; c < (int)j0; c += 64) {
-        const float yc = y[c];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int r = wave + 4 * (8 * h + j);
-          if (r < nbk) acc[j] += wbase[(8 * h + j) * rs4 + c] * yc;
+        for (int p = 0; p < P; ++p) {
+          float sm = (l[p].x * y0 + l[p].y * y1) + (l[p].z * y2 + l[p].w * y3);
+          sm += __shfl_xor(sm, 1, 64);
+          sm += __shfl_xor(sm, 2, 64);
+          sm += __shfl_xor(sm, 4, 64);
+          sm += __shfl_xor(sm, 8, 64);
+          const int64_t i = i0 + 4 * p + g;
+          if (m == 0 && i < n) y[i] -= sm;
         }
       }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) acc[j] += __shfl_xor(acc[j], o, 64);
-        const int r = wave + 4 * (8 * h + j);
-        if (lane == 0) vin[r] = r < nbk ? y[j0 + r] - acc[j] : 0.0f;
-      }
+      __syncthreads();
     }
-    __syncthreads();
-    for (int r = rq; r < 64; r += nt / 4) {
-      float acc = 0.0f;
-#pragma unroll
-      for (int t = 0; t < 16; ++t) acc += D[r * 65 + 16 * q + t] * vin[16 * q + t];
-      acc += __shfl_xor(acc, 1, 64);
-      acc += __shfl_xor(acc, 2, 64);
-      if (q == 0 && r < nbk) y[j0 + r] = acc;
-    }
-    __syncthreads();
   }
   // back: L^T z = y, blocks from the last
+  dload(nblk - 1);
   for (int64_t b = nblk - 1; b >= 0; --b) {
     const int64_t j0 = 64 * b;
     const int64_t nbk = (n - j0) < 64 ? (n - j0) : 64;
     const int64_t j1 = j0 + nbk;
-    stage(Dinv + b * 4096);
-    // part[w][c] = sum over the rows i >= j1 that wave w takes of L[i][j0 + c] z_i,
-    // eight rows' loads in flight at a time
-    {
-      float acc[8];
+    dstore();
+    // part[w][c] = sum over the wave's rows i >= j1 of L[i][j0 + c] z_i
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    for (int64_t i0 = j1 + 4 * P * wave; i0 < n; i0 += 4 * P * (NT / 64)) {
+      float4 l[P];
+      rows_load(i0, j0, l);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) acc[j] = 0.0f;
-      if (lane < nbk) {
-        for (int64_t i0 = j1 + wave; i0 < n; i0 += 8 * nwave) {
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const int64_t i = i0 + j * nwave;
-            if (i < n) acc[j] += W[i * ld + j0 + lane] * y[i];
-          }
-        }
+      for (int p = 0; p < P; ++p) {
+        const int64_t i = i0 + 4 * p + g;
+        const float zi = i < n ? y[i] : 0.0f;
+        a0 += l[p].x * zi;
+        a1 += l[p].y * zi;
+        a2 += l[p].z * zi;
+        a3 += l[p].w * zi;
       }
-      part[wave * 64 + lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) +
-                               ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    }
+    a0 += __shfl_xor(a0, 16, 64);
+    a1 += __shfl_xor(a1, 16, 64);
+    a2 += __shfl_xor(a2, 16, 64);
+    a3 += __shfl_xor(a3, 16, 64);
+    a0 += __shfl_xor(a0, 32, 64);
+    a1 += __shfl_xor(a1, 32, 64);
+    a2 += __shfl_xor(a2, 32, 64);
+    a3 += __shfl_xor(a3, 32, 64);
+    if (g == 0) {
+      part[wave * 64 + 4 * m] = a0;
+      part[wave * 64 + 4 * m + 1] = a1;
+      part[wave * 64 + 4 * m + 2] = a2;
+      part[wave * 64 + 4 * m + 3] = a3;
     }
     __syncthreads();
-    if (tid < 64) {
-      float acc = 0.0f;
-      for (int w = 0; w < nwave; ++w) acc += part[w * 64 + tid];
-      vin[tid] = tid < nbk ? y[j0 + tid] - acc : 0.0f;
-    }
+    if (b > 0) dload(b - 1);
+    if (tid < 64 && tid < nbk)
+      y[j0 + tid] -= (part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]);
     __syncthreads();
-    // z = Dinv^T vin
-    for (int r = rq; r < 64; r += nt / 4) {
-      float acc = 0.0f;
-#pragma unroll
-      for (int t = 0; t < 16; ++t) acc += D[(16 * q + t) * 65 + r] * vin[16 * q + t];
-      acc += __shfl_xor(acc, 1, 64);
-      acc += __shfl_xor(acc, 2, 64);
-      if (q == 0 && r < nbk) y[j0 + r] = acc;
-    }
-    __syncthreads();
+    dprod(j0, nbk, true);
   }
-  for (int64_t i = tid; i < n; i += nt) x[i] = y[i];
+  for (int64_t i = tid; i < n; i += NT) x[i] = y[i];
   __syncthreads();
 }
 

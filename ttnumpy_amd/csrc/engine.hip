@@ -48,6 +48,7 @@ constexpr int64_t kMixedMaxN = 4096;
 constexpr int kRankChunkMax = 16;
 static_assert(kMixedMaxN + 4480 <= (int64_t)(kEngLds / 4), "LDS of the FP32 triangular solves");
 static_assert(kCholLds * 8 <= (int)kEngLds, "LDS of the Cholesky solve");
+static_assert(kChol32Lds * 4 <= (int)kEngLds, "LDS of the FP32 Cholesky factor");
 
 __device__ __forceinline__ double* xcore(const ttx_slot* s, int64_t k) { return s->x + k * s->cap_core; }
 __device__ __forceinline__ double* zcore(const ttx_slot* s, int64_t k) { return s->z + k * s->cap_zcore; }
@@ -994,10 +995,11 @@ __device__ void solve_body(ttx_slot* s) {
 constexpr int kRefineMax = 8;
 constexpr double kRefineStop = 1e-12;
 constexpr double kRefineAccept = 1e-11;
-// the FP32 factor (N (N+1) floats), the right-hand side (N, padded to 64) and
-// the diagonal-block inverses (4096 per 64 rows) share the LU scratch of
-// 2 N (N+1) floats, which holds them for N >= 104
-static_assert(kMixedMinN >= 104, "mixed direct solves need N >= 104 for their scratch");
+// the FP32 factor (N rows of stride ld32 = N rounded up to 4 floats, so that
+// 16-byte loads of its rows are aligned), the right-hand side (N, padded to
+// 64) and the diagonal-block inverses (4096 per 64 rows) share the LU scratch
+// of 2 N (N+1) floats, which holds them for N >= 113
+static_assert(kMixedMinN >= 113, "mixed direct solves need N >= 113 for their scratch");
 
 __device__ void chol32_body(ttx_slot* s, double* lds) {
   __shared__ double red[16];
@@ -1006,20 +1008,21 @@ __device__ void chol32_body(ttx_slot* s, double* lds) {
   const int64_t k = s->k;
   const int64_t N = s->rx[k] * s->n[k] * s->rx[k + 1];
   int64_t t0 = wall_clock64();
-  float* W = reinterpret_cast<float*>(scr(s, TTX_SCR_LU));  // N (N+1) floats of 2 N (N+1)
-  float* d32 = W + N * (N + 1);
+  const int64_t ld32 = (N + 3) & ~(int64_t)3;
+  float* W = reinterpret_cast<float*>(scr(s, TTX_SCR_LU));  // N ld32 floats of 2 N (N+1)
+  float* d32 = W + N * ld32;
   float* Dinv = d32 + ((N + 63) / 64) * 64;  // 64 x 64 diagonal-block inverses of L
   const double* bloc = scr(s, TTX_SCR_BLOC);
   double* u = scr(s, TTX_SCR_U);
   double* r = scr(s, TTX_SCR_TMP);
   const LocOp op = system_op(s, k);
-  local_matrix_blk<float>(op, W, N + 1, lds, kEngLds / 8, true);
+  local_matrix_blk<float>(op, W, ld32, lds, kEngLds / 8, true);
   if (threadIdx.x == 0) s_info = TTX_OK;
   __syncthreads();
   stamp(s, 1, t0);
-  block_chol32_factor<kEngThreads>(N, N + 1, W, reinterpret_cast<float*>(lds), &s_info);
+  block_chol32_factor<kEngThreads>(N, ld32, W, reinterpret_cast<float*>(lds), &s_info);
   __syncthreads();
-  if (s_info == TTX_OK) chol32_invert_diag<kEngThreads>(N, N + 1, W, Dinv, reinterpret_cast<float*>(lds));
+  if (s_info == TTX_OK) chol32_invert_diag<kEngThreads>(N, ld32, W, Dinv, reinterpret_cast<float*>(lds));
   stamp(s, 18, t0);
   bool ok = false;
   int it = 0;
@@ -1035,7 +1038,7 @@ __device__ void chol32_body(ttx_slot* s, double* lds) {
     for (it = 1; it <= kRefineMax; ++it) {
       for (int64_t i = threadIdx.x; i < N; i += blockDim.x) d32[i] = (float)r[i];
       __syncthreads();
-      chol32_apply_inverse<kEngThreads>(N, N + 1, W, Dinv, d32, reinterpret_cast<float*>(lds));
+      chol32_apply_inverse<kEngThreads>(N, ld32, W, Dinv, d32, reinterpret_cast<float*>(lds));
       for (int64_t i = threadIdx.x; i < N; i += blockDim.x) u[i] += (double)d32[i];
       __syncthreads();
       op.apply(u, 1, r, -1.0, 0.0);  // r = -A u
