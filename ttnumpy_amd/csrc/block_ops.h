@@ -1383,7 +1383,9 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
   double* Bs = As + KS * 64;      // [KS][RT]: slab of L_R< (step 1: of y_<)
   double* Sw = As + wave * 16 * NTB * 17;  // step 3: this wave's rows, 16 columns (stride 17)
   if (tid == 0) s_bad = 0;
-  const int ar = tid >> 2, ak = (tid & 3) * 4;   // L_J< slab: 64 rows x 4 threads x 4 k
+  constexpr int EA = 64 * KS / NT;        // L_J< slab: 64 rows x 4 threads x EA k
+  static_assert(KS / EA == 4, "four threads per row of the L_J< slab");
+  const int ar = tid >> 2, ak = (tid & 3) * EA;
   const int br = tid / TPB, bk = (tid % TPB) * EB;
   // this wave's lower 16 x 16 blocks of D: w, w + 4, w + 8 of
   // (0,0) (1,0) (1,1) (2,0) (2,1) (2,2) (3,0) (3,1) (3,2) (3,3)
@@ -1410,10 +1412,10 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
           acc[q][e] = (q < nq && r < jb && c <= r) ? W[(J0 + r) * ld + J0 + c] : 0.0;
         }
       double cacc = (wave == 0 && lane < jb) ? W[(J0 + lane) * ld + n] : 0.0;
-      double ra[4], ry = 0.0;
+      double ra[EA], ry = 0.0;
       auto gload = [&](int64_t k0) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < EA; ++i) {
           const int64_t kk = k0 + ak + i;
           ra[i] = (aprow && kk < J0) ? aprow[kk] : 0.0;
         }
@@ -1423,7 +1425,7 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
       for (int64_t k0 = 0; k0 < J0; k0 += KS) {
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 4; ++i) As[(ak + i) * 64 + ar] = ra[i];
+        for (int i = 0; i < EA; ++i) As[(ak + i) * 64 + ar] = ra[i];
         if (tid < KS) Bs[tid] = ry;
         __syncthreads();
         if (k0 + KS < J0) gload(k0 + KS);
@@ -1552,10 +1554,10 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
           for (int e = 0; e < 4; ++e)
             v[mt][nt][e] = (rp[nt] && 16 * mt + 4 * e + lq < jb) ? rp[nt][16 * mt + 4 * e] : 0.0;
       const double* bprow = (R0 + br) < n ? W + (R0 + br) * ld : nullptr;
-      double ra[4], rb[EB];
+      double ra[EA], rb[EB];
       auto gload = [&](int64_t k0) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < EA; ++i) {
           const int64_t kk = k0 + ak + i;
           ra[i] = (aprow && kk < J0) ? aprow[kk] : 0.0;
         }
@@ -1569,7 +1571,7 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
       for (int64_t k0 = 0; k0 < J0; k0 += KS) {
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 4; ++i) As[(ak + i) * 64 + ar] = ra[i];
+        for (int i = 0; i < EA; ++i) As[(ak + i) * 64 + ar] = ra[i];
 #pragma unroll
         for (int i = 0; i < EB; ++i) Bs[(bk + i) * RT + br] = rb[i];
         __syncthreads();
@@ -1652,9 +1654,21 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
   for (int64_t j1 = n; j1 > 0; j1 -= 64) {
     const int64_t j0 = j1 - 64 > 0 ? j1 - 64 : 0;
     const int64_t nbk = j1 - j0;
-    if (lane < nbk) {
-      for (int64_t r = wave; r < nbk; r += NT / 64)
-        T[r * 65 + lane] = lane <= r ? W[(j0 + r) * ld + j0 + lane] : 0.0;
+    {
+      // the block's 16 rows of this wave: loads in flight together, then stored
+      double tv[64 * 64 / NT];
+#pragma unroll
+      for (int j = 0; j < 64 * 64 / NT; ++j) {
+        const int r = wave + (NT / 64) * j;
+        tv[j] = (lane < nbk && r < nbk && lane <= r) ? W[(j0 + r) * ld + j0 + lane] : 0.0;
+      }
+      if (lane < nbk) {
+#pragma unroll
+        for (int j = 0; j < 64 * 64 / NT; ++j) {
+          const int r = wave + (NT / 64) * j;
+          if (r < nbk) T[r * 65 + lane] = tv[j];
+        }
+      }
     }
     __syncthreads();
     if (wave == 0) {
@@ -1669,14 +1683,15 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
     }
     __syncthreads();
     // y_i -= sum_{c in block} L[c][i] u_c for the rows above (coalesced over i)
+    // (32 loads of a thread in flight; the sum runs over c in order)
     for (int64_t i = tid; i < j0; i += NT) {
       double acc = 0.0;
-      for (int64_t c = 0; c < nbk; c += 8) {
-        double v[8];
+      for (int64_t c = 0; c < nbk; c += 32) {
+        double v[32];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = (c + q < nbk) ? W[(j0 + c + q) * ld + i] : 0.0;
+        for (int q = 0; q < 32; ++q) v[q] = (c + q < nbk) ? W[(j0 + c + q) * ld + i] : 0.0;
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
+        for (int q = 0; q < 32; ++q)
           if (c + q < nbk) acc += v[q] * ub[c + q];
       }
       u[i] -= acc;
@@ -1714,7 +1729,7 @@ constexpr int kChol32Lds = 64 * 65 + kChol32Ks * 64 + kChol32Ks * 128;
 
 template <int NT, int NTB = 2>
 __device__ __forceinline__ void block_chol32_factor(int64_t n, int64_t ld, float* W, float* lds,
-                                                    int32_t* info) {
+                                                    int32_t* info, int64_t* clk = nullptr) {
   static_assert(NT == 256, "four waves: the 16-row blocks of D and of the tiles below");
   static_assert(NTB == 1 || NTB == 2, "rows of a tile below the diagonal per wave: 16 NTB");
   constexpr int KS = kChol32Ks;
@@ -1752,6 +1767,7 @@ __device__ __forceinline__ void block_chol32_factor(int64_t n, int64_t ld, float
     qj[q] = b - i * (i + 1) / 2;
   }
   const int nq = wave < 2 ? 3 : 2;  // blocks 0..9
+  int64_t t0 = wall_clock64();
   for (int64_t J0 = 0; J0 < n; J0 += 64) {
     const int jb = (int)((n - J0) < 64 ? (n - J0) : 64);
     const float* aprow = ar < jb ? W + (J0 + ar) * ld : nullptr;
@@ -1869,6 +1885,7 @@ __device__ __forceinline__ void block_chol32_factor(int64_t n, int64_t ld, float
       const int r = p >> 6, c = p & 63;
       if (r < jb && c <= r) W[(J0 + r) * ld + J0 + c] = D[r * 65 + c];
     }
+    lu_clock(clk, 0, t0);
     // ---- 3. the rows below: X L_JJ^T = M_RJ - L_R< L_J<^T
     for (int64_t R0 = J0 + 64; R0 < n; R0 += RT) {
       const int64_t wr0 = R0 + wave * 16 * NTB;
@@ -1977,6 +1994,7 @@ __device__ __forceinline__ void block_chol32_factor(int64_t n, int64_t ld, float
             if (rp[nt] && 16 * mt + 4 * lq + e < jb) rp[nt][16 * mt + e] = v[mt][nt][e];
     }
     __syncthreads();  // D is read above until here
+    lu_clock(clk, 1, t0);
   }
 }
 

@@ -1020,7 +1020,8 @@ __device__ void chol32_body(ttx_slot* s, double* lds) {
   if (threadIdx.x == 0) s_info = TTX_OK;
   __syncthreads();
   stamp(s, 1, t0);
-  block_chol32_factor<kEngThreads>(N, ld32, W, reinterpret_cast<float*>(lds), &s_info);
+  block_chol32_factor<kEngThreads>(N, ld32, W, reinterpret_cast<float*>(lds), &s_info,
+                                  &s->clk[22]);
   __syncthreads();
   if (s_info == TTX_OK) chol32_invert_diag<kEngThreads>(N, ld32, W, Dinv, reinterpret_cast<float*>(lds));
   stamp(s, 18, t0);
