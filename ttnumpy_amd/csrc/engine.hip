@@ -192,45 +192,27 @@ struct MixedOp {
 // Each thread owns a column (t_l, j, t_r) and a share of the (i, f_r) pairs;
 // T_a does not depend on f_l, so it is formed once per pair and reused down
 // the fl rows that share it.  Stores of a row stay coalesced across threads.
-// psi, A_k and phi are staged in LDS when they fit in lds_cap doubles.  With
-// lower, only entries on or below the diagonal are written.
-// Matches local_matrix_entry's sums term for term (a ascending, zero psi
-// skipped, b ascending).  Ends synced.
-template <typename OutT>
-__device__ void local_matrix_blk(const LocOp& op, OutT* out, int64_t ld, double* lds,
-                                 int64_t lds_cap, bool lower) {
-  constexpr int kMaxRa = 16;
-  const int rows = (int)op.rows(), cols = (int)op.cols();
-  add_flops(2.0 * (double)rows * (double)cols * (double)op.Ra * (double)op.Rb);
+// psi, A_k and phi are staged in LDS when they fit in lds_cap doubles (phi as
+// [b][f_r][t_r], so that consecutive threads, consecutive t_r, read
+// consecutive words).  With lower, only entries on or below the diagonal are
+// written.  Matches local_matrix_entry's sums term for term (a ascending,
+// zero psi skipped, b ascending).  Ends synced.
+//
+// The column loop takes its three operands as separate pointers with their
+// strides, so that on the staged path every operand pointer derives from the
+// LDS array alone and the compiler reads them with LDS instructions.  It is
+// instantiated for operator bonds up to KR: every load of an entry's terms
+// (KR values, indices clamped to the bond so they stay in bounds) is issued
+// before the sums that use them, so a thread waits once per entry rather than
+// once per term; the sums skip the terms past the bonds.
+template <typename OutT, int KR>
+__device__ __forceinline__ void local_matrix_cols(const LocOp& op, OutT* out, int64_t ld, bool lower,
+                                                  const double* psi, const double* phi,
+                                                  const double* A, int pt, int pb, int pf) {
+  const int cols = (int)op.cols();
   const int Ra = (int)op.Ra, Rb = (int)op.Rb, fl = (int)op.fl, fr = (int)op.fr, nr = (int)op.nr,
             nc = (int)op.nc, rr = (int)op.rr;
-  const int npsi = (int)(op.rl * Ra * fl), nphi = (int)(rr * Rb * fr), nA = Ra * nr * nc * Rb;
-  const double* psi = op.psi;
-  const double* phi = op.phi;
-  const double* A = op.A;
-  if ((int64_t)(npsi + nphi + nA) <= lds_cap) {
-    for (int e = threadIdx.x; e < npsi; e += blockDim.x) lds[e] = psi[e];
-    for (int e = threadIdx.x; e < nphi; e += blockDim.x) lds[npsi + e] = phi[e];
-    for (int e = threadIdx.x; e < nA; e += blockDim.x) lds[npsi + nphi + e] = A[e];
-    __syncthreads();
-    psi = lds;
-    phi = lds + npsi;
-    A = lds + npsi + nphi;
-  }
-  if (Ra > kMaxRa) {  // wide operator bonds: entry by entry
-    for (int it = threadIdx.x; it < cols; it += blockDim.x) {
-      const int cq = it / rr, t_r = it - cq * rr, t_l = cq / nc, j = cq - t_l * nc;
-      for (int row = 0; row < rows; ++row) {
-        const int rq = row / fr, f_r = row - rq * fr, f_l = rq / nr, i = rq - f_l * nr;
-        if (lower && row < it) continue;
-        out[(int64_t)row * ld + it] = (OutT)local_matrix_entry(
-            psi + (int64_t)t_l * Ra * fl, fl, f_l, A + (i * nc + j) * Rb, (int64_t)nr * nc * Rb, Ra,
-            Rb, phi + (int64_t)t_r * Rb * fr, fr, f_r);
-      }
-    }
-    __syncthreads();
-    return;
-  }
+  const int am1 = Ra - 1, bm1 = Rb - 1;
   const int npairs = nr * fr;
   const int S = cols >= (int)blockDim.x ? 1 : (int)blockDim.x / cols;
   const int a_stride = nr * nc * Rb;
@@ -238,19 +220,25 @@ __device__ void local_matrix_blk(const LocOp& op, OutT* out, int64_t ld, double*
     const int sl = it / cols, c = it - sl * cols;
     const int cq = c / rr, t_r = c - cq * rr, t_l = cq / nc, j = cq - t_l * nc;
     const double* ps = psi + t_l * Ra * fl;
-    const double* ph = phi + t_r * Rb * fr;
+    const double* ph = phi + t_r * pt;
     const int pbeg = npairs * sl / S, pend = npairs * (sl + 1) / S;
     for (int pq = pbeg; pq < pend; ++pq) {
       const int i = pq / fr, f_r = pq - i * fr;
-      double T[kMaxRa];
+      double phv[KR];  // phi[t_r, b, f_r], read once for every a
 #pragma unroll
-      for (int a = 0; a < kMaxRa; ++a) {
-        if (a < Ra) {
-          const double* arow = A + a * a_stride + (i * nc + j) * Rb;
-          double t = 0.0;
-          for (int b = 0; b < Rb; ++b) t += arow[b] * ph[b * fr + f_r];
-          T[a] = t;
-        }
+      for (int b = 0; b < KR; ++b) phv[b] = ph[(b < Rb ? b : bm1) * pb + f_r * pf];
+      double T[KR];
+#pragma unroll
+      for (int a = 0; a < KR; ++a) {
+        const double* arow = A + (a < Ra ? a : am1) * a_stride + (i * nc + j) * Rb;
+        double av[KR];
+#pragma unroll
+        for (int b = 0; b < KR; ++b) av[b] = arow[b < Rb ? b : bm1];
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < KR; ++b)
+          if (b < Rb) t += av[b] * phv[b];
+        T[a] = t;
       }
       OutT* o = out + (int64_t)(i * fr + f_r) * ld + c;
       const int64_t ostep = (int64_t)nr * fr * ld;
@@ -261,17 +249,61 @@ __device__ void local_matrix_blk(const LocOp& op, OutT* out, int64_t ld, double*
         f0 = need > 0 ? (need + nr * fr - 1) / (nr * fr) : 0;
       }
       for (int f_l = f0; f_l < fl; ++f_l) {
+        double pv[KR];
+#pragma unroll
+        for (int a = 0; a < KR; ++a) pv[a] = ps[(a < Ra ? a : am1) * fl + f_l];
         double sum = 0.0;
 #pragma unroll
-        for (int a = 0; a < kMaxRa; ++a) {
-          if (a < Ra) {
-            const double p = ps[a * fl + f_l];
-            if (p != 0.0) sum += p * T[a];
-          }
-        }
+        for (int a = 0; a < KR; ++a)
+          if (a < Ra && pv[a] != 0.0) sum += pv[a] * T[a];
         o[f_l * ostep] = (OutT)sum;
       }
     }
+  }
+}
+
+template <typename OutT>
+__device__ __forceinline__ void local_matrix_blk(const LocOp& op, OutT* out, int64_t ld, double* lds,
+                                 int64_t lds_cap, bool lower) {
+  constexpr int kMaxRa = 16;
+  const int rows = (int)op.rows(), cols = (int)op.cols();
+  add_flops(2.0 * (double)rows * (double)cols * (double)op.Ra * (double)op.Rb);
+  const int Ra = (int)op.Ra, Rb = (int)op.Rb, fl = (int)op.fl, fr = (int)op.fr, nr = (int)op.nr,
+            nc = (int)op.nc, rr = (int)op.rr;
+  const int npsi = (int)(op.rl * Ra * fl), nphi = (int)(rr * Rb * fr), nA = Ra * nr * nc * Rb;
+  if (Ra > kMaxRa || Rb > kMaxRa) {  // wide operator bonds: entry by entry
+    for (int it = threadIdx.x; it < cols; it += blockDim.x) {
+      const int cq = it / rr, t_r = it - cq * rr, t_l = cq / nc, j = cq - t_l * nc;
+      for (int row = 0; row < rows; ++row) {
+        const int rq = row / fr, f_r = row - rq * fr, f_l = rq / nr, i = rq - f_l * nr;
+        if (lower && row < it) continue;
+        out[(int64_t)row * ld + it] = (OutT)local_matrix_entry(
+            op.psi + (int64_t)t_l * Ra * fl, fl, f_l, op.A + (i * nc + j) * Rb,
+            (int64_t)nr * nc * Rb, Ra, Rb, op.phi + (int64_t)t_r * Rb * fr, fr, f_r);
+      }
+    }
+    __syncthreads();
+    return;
+  }
+  if ((int64_t)(npsi + nphi + nA) <= lds_cap) {
+    for (int e = threadIdx.x; e < npsi; e += blockDim.x) lds[e] = op.psi[e];
+    for (int e = threadIdx.x; e < nphi; e += blockDim.x) {
+      const int t_r = e / (Rb * fr), bf = e - t_r * (Rb * fr);
+      lds[npsi + bf * rr + t_r] = op.phi[e];
+    }
+    for (int e = threadIdx.x; e < nA; e += blockDim.x) lds[npsi + nphi + e] = op.A[e];
+    __syncthreads();
+    const double* sp = lds;
+    const double* sf = lds + npsi;
+    const double* sa = lds + npsi + nphi;
+    if (Ra <= 4 && Rb <= 4)
+      local_matrix_cols<OutT, 4>(op, out, ld, lower, sp, sf, sa, 1, fr * rr, rr);
+    else if (Ra <= 8 && Rb <= 8)
+      local_matrix_cols<OutT, 8>(op, out, ld, lower, sp, sf, sa, 1, fr * rr, rr);
+    else
+      local_matrix_cols<OutT, kMaxRa>(op, out, ld, lower, sp, sf, sa, 1, fr * rr, rr);
+  } else {
+    local_matrix_cols<OutT, kMaxRa>(op, out, ld, lower, op.psi, op.phi, op.A, Rb * fr, fr, 1);
   }
   __syncthreads();
 }
