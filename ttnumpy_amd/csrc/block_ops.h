@@ -1398,6 +1398,17 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
     qj[q] = b - i * (i + 1) / 2;
   }
   const int nq = wave < 2 ? 3 : 2;  // blocks 0..9
+  // this lane's entries of a packed 16 x 16 lower triangle (136 entries):
+  // index lane + 64 j is entry (pk_c[j], pk_p[j]); pk_c = -1 past the end
+  int pk_c[3], pk_p[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int q = lane + 64 * j;
+    int c = 0;
+    while ((c + 1) * (c + 2) / 2 <= q && c < 16) ++c;
+    pk_c[j] = q < 136 ? c : -1;
+    pk_p[j] = q - c * (c + 1) / 2;
+  }
   for (int64_t J0 = 0; J0 < n; J0 += 64) {
     const int jb = (int)((n - J0) < 64 ? (n - J0) : 64);
     const double* aprow = ar < jb ? W + (J0 + ar) * ld : nullptr;
@@ -1612,20 +1623,31 @@ __device__ __forceinline__ void block_chol_solve(int64_t n, int64_t ld, double* 
           for (int nt = 0; nt < NTB; ++nt)
 #pragma unroll
             for (int e = 0; e < 4; ++e) Sw[(16 * nt + l16) * 17 + 4 * e + lq] = v[m2][nt][e];
+          // the lower triangle of the diagonal block L_bb, packed (entry
+          // (c, p) at c (c + 1) / 2 + p) three entries a lane: read from LDS
+          // once, then each one broadcast by v_readlane where the solve uses it
+          double lt[3];
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            lt[j] = pk_c[j] >= 0 ? D[(16 * m2 + pk_c[j]) * 65 + 16 * m2 + pk_p[j]] : 0.0;
           __builtin_amdgcn_wave_barrier();
-          if (lane < 16 * NTB) {
-            // row lane of the wave: x L_bb^T = w against the diagonal block
-            // (mbcnt_lo(0, 0) is 0 in every lane; it keeps the compiler from
-            // holding the block's 120 uniform entries in scalar registers)
-            const double* Lb = D + 16 * m2 * 65 + 16 * m2 + __builtin_amdgcn_mbcnt_lo(0u, 0u);
-            double* sr = Sw + lane * 17;
+          {
+            // row lane % (16 NTB) of the wave: x L_bb^T = w against the
+            // diagonal block.  Every lane runs it (lanes past 16 NTB repeat a
+            // row and store the same bits): the entries are read from other
+            // lanes, so no lane may skip the code that forms them.
+            double* sr = Sw + (lane & (16 * NTB - 1)) * 17;
             double x[16];
 #pragma unroll
             for (int c = 0; c < 16; ++c) {
               double t = sr[c];
 #pragma unroll
-              for (int p = 0; p < c; ++p) t -= x[p] * Lb[c * 65 + p];
-              x[c] = 16 * m2 + c < jb ? t / Lb[c * 65 + c] : 0.0;
+              for (int p = 0; p < c; ++p) {
+                const int q = c * (c + 1) / 2 + p;
+                t -= x[p] * lane_value(lt[q >> 6], q & 63);
+              }
+              const int qd = c * (c + 1) / 2 + c;
+              x[c] = 16 * m2 + c < jb ? t / lane_value(lt[qd >> 6], qd & 63) : 0.0;
             }
 #pragma unroll
             for (int c = 0; c < 16; ++c) sr[c] = x[c];
@@ -1767,6 +1789,17 @@ __device__ __forceinline__ void block_chol32_factor(int64_t n, int64_t ld, float
     qj[q] = b - i * (i + 1) / 2;
   }
   const int nq = wave < 2 ? 3 : 2;  // blocks 0..9
+  // this lane's entries of a packed 16 x 16 lower triangle (136 entries):
+  // index lane + 64 j is entry (pk_c[j], pk_p[j]); pk_c = -1 past the end
+  int pk_c[3], pk_p[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int q = lane + 64 * j;
+    int c = 0;
+    while ((c + 1) * (c + 2) / 2 <= q && c < 16) ++c;
+    pk_c[j] = q < 136 ? c : -1;
+    pk_p[j] = q - c * (c + 1) / 2;
+  }
   int64_t t0 = wall_clock64();
   for (int64_t J0 = 0; J0 < n; J0 += 64) {
     const int jb = (int)((n - J0) < 64 ? (n - J0) : 64);
@@ -1962,18 +1995,34 @@ __device__ __forceinline__ void block_chol32_factor(int64_t n, int64_t ld, float
           for (int nt = 0; nt < NTB; ++nt)
 #pragma unroll
             for (int e = 0; e < 4; ++e) Sw[(16 * nt + l16) * 17 + 4 * lq + e] = v[m2][nt][e];
+          // the lower triangle of L_bb, packed (entry (c, p) at c (c + 1) / 2 + p)
+          // three entries a lane, the diagonal ones inverted: read from LDS
+          // once, then each one broadcast by v_readlane where the solve uses it
+          float lt[3];
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const float v = pk_c[j] >= 0 ? D[(16 * m2 + pk_c[j]) * 65 + 16 * m2 + pk_p[j]] : 1.0f;
+            lt[j] = pk_c[j] == pk_p[j] ? __builtin_amdgcn_rcpf(v) : v;
+          }
           __builtin_amdgcn_wave_barrier();
-          if (lane < 16 * NTB) {
-            // row lane of the wave: x L_bb^T = w against the diagonal block
-            const float* Lb = D + 16 * m2 * 65 + 16 * m2 + __builtin_amdgcn_mbcnt_lo(0u, 0u);
-            float* sr = Sw + lane * 17;
+          {
+            // row lane % (16 NTB) of the wave: x L_bb^T = w against the
+            // diagonal block.  Every lane runs it (lanes past 16 NTB repeat a
+            // row and store the same bits): the entries are read from other
+            // lanes, so no lane may skip the code that forms them.
+            float* sr = Sw + (lane & (16 * NTB - 1)) * 17;
             float x[16];
 #pragma unroll
             for (int c = 0; c < 16; ++c) {
               float t = sr[c];
 #pragma unroll
-              for (int p = 0; p < c; ++p) t -= x[p] * Lb[c * 65 + p];
-              x[c] = 16 * m2 + c < jb ? t / Lb[c * 65 + c] : 0.0f;
+              for (int p = 0; p < c; ++p) {
+                const int q = c * (c + 1) / 2 + p;
+                t -= x[p] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lt[q >> 6]), q & 63));
+              }
+              const int qd = c * (c + 1) / 2 + c;
+              const float rd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lt[qd >> 6]), qd & 63));
+              x[c] = 16 * m2 + c < jb ? t * rd : 0.0f;
             }
 #pragma unroll
             for (int c = 0; c < 16; ++c) sr[c] = x[c];
