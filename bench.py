@@ -208,31 +208,37 @@ def c5_high_dim(with_cpu: bool):
         res["engine"] = {"error": r.error}
     # the mixed form on the same engine: MINRES local solves on the FP32 operator
     # (FP32 MFMA contractions) with FP64 defect correction, FP32 Cholesky with FP64
-    # refinement for the direct ones; residual check, QR/SVD and rank rule in FP64
+    # refinement for the direct ones; residual check, QR/SVD and rank rule in FP64.
+    # "bf16": the same with the MINRES operator's products on BF16 operands (FP32
+    # accumulation, v_mfma_f32_16x16x32_bf16)
     from ttnumpy_amd.solvers.common import relative_residual
-    engm = Engine(1, rmax=64)
-    for rep in range(2):
-        reqm = Request(A=A, b=b, max_rank=128, residual_tol=1e-8, max_sweeps=30, symmetric=True,
-                       seed=0, precision="mixed")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        rm = engm.run([reqm])[0]
-        torch.cuda.synchronize()
-        wallm = time.perf_counter() - t0
-    if rm.error is None and r.error is None:
-        from oracle import tt_oracle as O
-        x64, xm = O.Train(r.result.to_numpy()), O.Train(rm.result.to_numpy())
-        res["engine_mixed"] = {
-            "gpu_s": wallm, "ratio_vs_fp64": wall / wallm, "stop_reason": str(rm.info.stop_reason),
-            "sweeps": rm.info.num_sweeps_run, "max_bond": max(rm.result.bonds),
-            "residual_fp64_verified": float(relative_residual(A, b, rm.result)),
-            "residual_fp64_verified_fp64_path": float(relative_residual(A, b, r.result)),
-            "relative_difference_vs_fp64_solution": float(
-                O.tt_norm(O.tt_add(xm, O.tt_scale(x64, -1.0))) / O.tt_norm(x64)),
-            "local_solves": rm.info.local_solves, "minres_iters": engm.minres_iters // 2,
-            "minres_iters_fp64": eng.minres_iters // 2}
-    else:
-        res["engine_mixed"] = {"error": rm.error or r.error}
+    for prec in ("mixed", "bf16"):
+        engm = Engine(1, rmax=64)
+        for rep in range(2):
+            reqm = Request(A=A, b=b, max_rank=128, residual_tol=1e-8, max_sweeps=30,
+                           symmetric=True, seed=0, precision=prec)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rm = engm.run([reqm])[0]
+            torch.cuda.synchronize()
+            wallm = time.perf_counter() - t0
+        if rm.error is None and r.error is None:
+            from oracle import tt_oracle as O
+            x64, xm = O.Train(r.result.to_numpy()), O.Train(rm.result.to_numpy())
+            res[f"engine_{prec}"] = {
+                "gpu_s": wallm, "ratio_vs_fp64": wall / wallm,
+                "stop_reason": str(rm.info.stop_reason),
+                "sweeps": rm.info.num_sweeps_run, "max_bond": max(rm.result.bonds),
+                "residual_fp64_verified": float(relative_residual(A, b, rm.result)),
+                "residual_fp64_verified_fp64_path": float(relative_residual(A, b, r.result)),
+                "relative_difference_vs_fp64_solution": float(
+                    O.tt_norm(O.tt_add(xm, O.tt_scale(x64, -1.0))) / O.tt_norm(x64)),
+                "local_solves": rm.info.local_solves, "local_inexact": rm.info.local_inexact,
+                "minres_iters": engm.minres_iters // 2,
+                "outer_corrections": engm.mixed_refines // 2,
+                "minres_iters_fp64": eng.minres_iters // 2}
+        else:
+            res[f"engine_{prec}"] = {"error": rm.error or r.error}
     if with_cpu:
         from threadpoolctl import threadpool_limits
         from oracle import tt_oracle as O
@@ -653,6 +659,7 @@ def main():
             line["c5_high_dim"] = c5_high_dim(not args.no_cpu)
             if mixed is not None:
                 mixed["c5_engine_one_slot"] = line["c5_high_dim"].get("engine_mixed")
+                mixed["c5_engine_one_slot_bf16"] = line["c5_high_dim"].get("engine_bf16")
         except Exception as exc:  # the C1 line is reported whatever happens here
             line["c5_high_dim"] = {"error": repr(exc)}
         print("# c5 done", file=sys.stderr, flush=True)

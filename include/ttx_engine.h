@@ -51,7 +51,8 @@ typedef struct ttx_slot {
   // (iterative: 0 direct local solves only, 1 MINRES, 2 GMRES above direct_size)
   int64_t chol;  // 1: direct solves try Cholesky first (symmetric problems), LU if not SPD
   int64_t mixed; // 1: direct Cholesky solves of size >= 256 factor in FP32 with FP64 refinement,
-                 //    MINRES solves run on the FP32 operator with FP64 defect correction
+                 //    MINRES solves run on the FP32 operator with FP64 defect correction;
+                 // 2: the same with the MINRES operator's products on BF16 operands
   int64_t minres_slice; // MINRES iterations per tick (a long local solve spans ticks)
   int64_t n[TTX_MAXD];
   int64_t ra[TTX_MAXD + 1];

@@ -125,6 +125,42 @@ def test_engine_mixed_minres_path():
     assert d <= bound
 
 
+def test_engine_bf16_minres_c5_shape():
+    """BF16 operands on the MINRES path (precision="bf16"): the C5 system at
+    d = 6 (A = I + sum_i L_i, kappa(A) < 30, mode 8), direct_solve_size 64 so
+    that most local solves are iterative.  The inner MINRES applies the local
+    operator with its three products on BF16 operands and FP32 accumulation
+    (v_mfma_f32_16x16x32_bf16), the outer FP64 correction meets the reference's
+    acceptance test, so the request ends like the FP64 one (residual <= 3e-8,
+    FP64-verified) and its solution is within what the two residuals allow of
+    the FP64 request's: kappa(A) (r_bf16 + r_64), kappa(A) < 30."""
+    from ttnumpy_amd.engine import Engine, Request
+    from ttnumpy_amd.solvers.common import relative_residual
+    print("engine bf16 minres C5 shape d=6", flush=True)
+    A, b = _tt(P.shifted_laplacian_sum_cores(6, 8, 1.0 / 6)), _tt(P.random_train_cores([8] * 6, 4, 3))
+    kw = dict(max_rank=64, residual_tol=1e-8, max_sweeps=20, symmetric=True, seed=0,
+              direct_solve_size=64)
+    reqs = {p: Request(A=A, b=b, precision=p, **kw) for p in ("fp64", "mixed", "bf16")}
+    out = {}
+    for p, r in reqs.items():  # one engine each: the counters are per engine
+        eng = Engine(1, rmax=64)
+        eng.run([r], poll=8)
+        assert r.error is None, (p, r.error)
+        out[p] = (r, eng.minres_iters, eng.mixed_solves, eng.mixed_refines)
+        print(p, r.info, "minres iterations", eng.minres_iters, "mixed solves", eng.mixed_solves,
+              "outer passes", eng.mixed_refines, flush=True)
+    r64 = out["fp64"][0]
+    for p in ("mixed", "bf16"):
+        r = out[p][0]
+        assert out[p][2] > 0  # mixed solves ran
+        assert str(r.info.stop_reason) in ("converged", "stagnated")
+        res = float(relative_residual(A, b, r.result))
+        assert res <= 3e-8, (p, res)
+        d = _rel_diff(r.result, O.Train(r64.result.to_numpy()))
+        print(p, "relative difference vs fp64", d, "fp64-verified residual", res, flush=True)
+        assert d <= 30.0 * (res + r64.info.residuals[-1]), (p, d)
+
+
 def test_engine_dense_fallback_matches_oracle():
     """allow_dense_fallback (common.py:793-809): when the four MINRES attempts
     miss local_rtol (here 1e-15, below what MINRES reaches), the engine solves

@@ -257,6 +257,84 @@ __device__ __forceinline__ void bgemm32(int64_t batch, int64_t M, int64_t N, int
   }
 }
 
+// bgemm32 with BF16 operands: the entries of A and B are rounded to BF16
+// (round to nearest even, v_cvt_pk_bf16_f32) as they are loaded and the
+// products are accumulated in FP32 on v_mfma_f32_16x16x32_bf16.  Lane l
+// supplies the 8 operand values of k = 8 (l/16) + j, j < 8, for row (A) or
+// column (B) l % 16 of its 16 x 16 block, the same k slots in A and B;
+// output lane l element e = C[4 (l/16) + e][l % 16], as for the FP32 MFMA.
+// Each wave owns 32 x 32 output tiles.  No barrier inside.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void bgemm_bf16(int64_t batch, int64_t M, int64_t N, int64_t K,
+                                           float alpha, const float* A, int64_t a_ms, int64_t a_ks,
+                                           int64_t a_bs, const float* B, int64_t b_ks, int64_t b_ns,
+                                           int64_t b_bs, float beta, float* C, int64_t c_ms,
+                                           int64_t c_ns, int64_t c_bs) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  const int r = lane & 15, kq = lane >> 4;
+  const int64_t tm = (M + 31) >> 5, tn = (N + 31) >> 5;
+  const int64_t per = tm * tn;
+  add_flops(2.0 * (double)batch * (double)M * (double)N * (double)K);
+  for (int64_t t = wave; t < batch * per; t += nw) {
+    const int64_t bt = t / per, tt = t % per;
+    const int64_t m0 = (tt / tn) << 5, n0 = (tt % tn) << 5;
+    const float* Ab = A + bt * a_bs;
+    const float* Bb = B + bt * b_bs;
+    float* Cb = C + bt * c_bs;
+    const int64_t ma = m0 + r, mb = m0 + 16 + r, na = n0 + r, nb = n0 + 16 + r;
+    const bool va = ma < M, vb = mb < M, wa = na < N, wb = nb < N;
+    const bool m2 = m0 + 16 < M, n2 = n0 + 16 < N;
+    floatx4_t c00 = {0.0f, 0.0f, 0.0f, 0.0f}, c01 = c00, c10 = c00, c11 = c00;
+    for (int64_t k0 = 0; k0 < K; k0 += 32) {
+      float fa0[8], fa1[8], fb0[8], fb1[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int64_t kk = k0 + 8 * kq + j;
+        const bool kin = kk < K;
+        fa0[j] = (kin && va) ? Ab[ma * a_ms + kk * a_ks] : 0.0f;
+        fa1[j] = (kin && vb) ? Ab[mb * a_ms + kk * a_ks] : 0.0f;
+        fb0[j] = (kin && wa) ? Bb[kk * b_ks + na * b_ns] : 0.0f;
+        fb1[j] = (kin && wb) ? Bb[kk * b_ks + nb * b_ns] : 0.0f;
+      }
+      bf16x8_t a0, a1, b0, b1;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        a0[j] = (__bf16)fa0[j];
+        a1[j] = (__bf16)fa1[j];
+        b0[j] = (__bf16)fb0[j];
+        b1[j] = (__bf16)fb1[j];
+      }
+      c00 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, c00, 0, 0, 0);
+      if (n2) c01 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, c01, 0, 0, 0);
+      if (m2) c10 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, c10, 0, 0, 0);
+      if (m2 && n2) c11 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, c11, 0, 0, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t row0 = m0 + 4 * kq + e, row1 = row0 + 16;
+      if (row0 < M && wa) {
+        float* d = Cb + row0 * c_ms + na * c_ns;
+        *d = beta == 0.0f ? alpha * c00[e] : alpha * c00[e] + beta * (*d);
+      }
+      if (row0 < M && wb) {
+        float* d = Cb + row0 * c_ms + nb * c_ns;
+        *d = beta == 0.0f ? alpha * c01[e] : alpha * c01[e] + beta * (*d);
+      }
+      if (row1 < M && wa) {
+        float* d = Cb + row1 * c_ms + na * c_ns;
+        *d = beta == 0.0f ? alpha * c10[e] : alpha * c10[e] + beta * (*d);
+      }
+      if (row1 < M && wb) {
+        float* d = Cb + row1 * c_ms + nb * c_ns;
+        *d = beta == 0.0f ? alpha * c11[e] : alpha * c11[e] + beta * (*d);
+      }
+    }
+  }
+}
+
 // y = alpha * M x + beta * y for a row-major M (row stride ld); one wave per
 // row so the reads of a row coalesce.  No barrier inside.
 __device__ __forceinline__ void bgemv(int64_t rows, int64_t cols, double alpha, const double* M,

@@ -148,7 +148,9 @@ struct LocOp {
 
 // The one-site operator in FP32 for the inner solves of mixed requests:
 // FP32 copies of psi, A_k and phi, the three products of LocOp::apply on
-// FP32 MFMA (bgemm32, FP32 accumulation) through FP32 scratch.
+// FP32 MFMA (bgemm32, FP32 accumulation) through FP32 scratch; with bf16 the
+// operands of each product are rounded to BF16 as they are loaded
+// (bgemm_bf16, FP32 accumulation).
 struct LocOp32 {
   const float* psi;
   const float* A;
@@ -156,14 +158,22 @@ struct LocOp32 {
   int64_t rl, Ra, fl, nr, nc, Rb, rr, fr;
   float* s1;
   float* s2;
+  bool bf16;
   __device__ void apply(const float* y, float* out) const {
-    bgemm32(1, rl * nc, Rb * fr, rr, 1.0f, y, rr, 1, 0, phi, Rb * fr, 1, 0, 0.0f, s1, Rb * fr, 1, 0);
-    __syncthreads();
-    bgemm32(rl, Ra * nr, fr, nc * Rb, 1.0f, A, nc * Rb, 1, 0, s1, fr, 1, nc * Rb * fr, 0.0f, s2, fr,
-            1, Ra * nr * fr);
-    __syncthreads();
-    bgemm32(1, fl, nr * fr, rl * Ra, 1.0f, psi, 1, fl, 0, s2, nr * fr, 1, 0, 0.0f, out, nr * fr, 1, 0);
-    __syncthreads();
+    auto gemm = [&](int64_t batch, int64_t M, int64_t N, int64_t K, const float* A_, int64_t a_ms,
+                    int64_t a_ks, int64_t a_bs, const float* B_, int64_t b_ks, int64_t b_ns,
+                    int64_t b_bs, float* C_, int64_t c_ms, int64_t c_ns, int64_t c_bs) {
+      if (bf16)
+        bgemm_bf16(batch, M, N, K, 1.0f, A_, a_ms, a_ks, a_bs, B_, b_ks, b_ns, b_bs, 0.0f, C_, c_ms,
+                   c_ns, c_bs);
+      else
+        bgemm32(batch, M, N, K, 1.0f, A_, a_ms, a_ks, a_bs, B_, b_ks, b_ns, b_bs, 0.0f, C_, c_ms, c_ns,
+                c_bs);
+      __syncthreads();
+    };
+    gemm(1, rl * nc, Rb * fr, rr, y, rr, 1, 0, phi, Rb * fr, 1, 0, s1, Rb * fr, 1, 0);
+    gemm(rl, Ra * nr, fr, nc * Rb, A, nc * Rb, 1, 0, s1, fr, 1, nc * Rb * fr, s2, fr, 1, Ra * nr * fr);
+    gemm(1, fl, nr * fr, rl * Ra, psi, 1, fl, 0, s2, nr * fr, 1, 0, out, nr * fr, 1, 0);
   }
 };
 
@@ -829,6 +839,11 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_env_kernel(ttx_slot* slots
 // rank rule and everything after the solve see the FP64 operator.
 constexpr int kMixedOuterMax = 4;
 constexpr double kMixedInnerRtol = 1e-5;
+// BF16 operands (slot.mixed = 2): the operator's rounding error, ~1e-3 of its
+// norm, bounds what one correction can gain, so the inner solve stops at 1e-3
+// and the outer loop gets more passes
+constexpr int kBf16OuterMax = 16;
+constexpr double kBf16InnerRtol = 1e-3;
 enum { kMrOuter = kMrCount };
 
 // the FP32 operator of site k in the slot's M32 region into *m (the factors
@@ -861,6 +876,7 @@ __device__ bool mixed_op(ttx_slot* s, int64_t k, MixedOp* mp, bool fill = false)
   m.op.fr = rr;
   m.op.s1 = A + nA;
   m.op.s2 = m.op.s1 + ns1;
+  m.op.bf16 = s->mixed == 2;
   m.y32 = m.op.s2 + ns2;
   m.o32 = m.y32 + N;
   m.n = N;
@@ -1125,6 +1141,8 @@ __device__ void minres32_body(ttx_slot* s) {
   double* st = scal + 8;
   double* V = scr(s, TTX_SCR_MR);
   const LocOp op = system_op(s, k);
+  const int outer_max = s->mixed == 2 ? kBf16OuterMax : kMixedOuterMax;
+  const double inner_rtol = s->mixed == 2 ? kBf16InnerRtol : kMixedInnerRtol;
   MixedOp mop;
   const bool start = st[kMrOuter] < 0.0;
   mixed_op(s, k, &mop, start);  // its size was checked when the solve was handed over
@@ -1150,7 +1168,7 @@ __device__ void minres32_body(ttx_slot* s) {
     if (threadIdx.x == 0) {
       st[kMrOuter] = 0.0;
       st[kMrRtol] = 1e-15;  // the residual-estimate test below ends the inner solve
-      st[kMrPhiStop] = fmax(kMixedInnerRtol, 0.5 * target / res);
+      st[kMrPhiStop] = fmax(inner_rtol, 0.5 * target / res);
       st[kMrMaxit] = (double)(5 * N);
     }
     minres_start(mop, N, tmp, nullptr, V, red, st);
@@ -1169,7 +1187,7 @@ __device__ void minres32_body(ttx_slot* s) {
     const int outer = (int)st[kMrOuter] + 1;
     const bool ok = res <= target;
     __syncthreads();  // everyone has read st before it is rewritten
-    if (!ok && outer >= kMixedOuterMax && s->allow_fallback && N <= s->direct_cap) {
+    if (!ok && outer >= outer_max && s->allow_fallback && N <= s->direct_cap) {
       // the explicit-matrix fallback (common.py:793-809) in FP64
       if (threadIdx.x == 0) {
         s->local_solves += 1;
@@ -1181,7 +1199,7 @@ __device__ void minres32_body(ttx_slot* s) {
       stamp(s, 3, t0);
       return;
     }
-    if (ok || outer >= kMixedOuterMax) {
+    if (ok || outer >= outer_max) {
       if (threadIdx.x == 0) {
         s->local_solves += 1;
         if (!ok) s->local_inexact += 1;
@@ -1197,7 +1215,7 @@ __device__ void minres32_body(ttx_slot* s) {
     }
     if (threadIdx.x == 0) {
       st[kMrOuter] = (double)outer;
-      st[kMrPhiStop] = fmax(kMixedInnerRtol, 0.5 * target / res);
+      st[kMrPhiStop] = fmax(inner_rtol, 0.5 * target / res);
     }
   minres_start(mop, N, tmp, nullptr, V, red, st);
     if (budget <= 0) break;

@@ -114,7 +114,8 @@ class Request:
     symmetric: bool = False
     seed: Optional[int] = None
     # "mixed": direct Cholesky solves (N >= 256) factor in FP32 with FP64 refinement, and
-    # MINRES solves run on the FP32 operator with FP64 defect correction
+    # MINRES solves run on the FP32 operator with FP64 defect correction; "bf16": the
+    # same, with the MINRES operator's products on BF16 operands (FP32 accumulation)
     precision: str = "fp64"
     # rank capacity of the request's slot (None: the engine's rmax); raised by
     # the engine when a bond outgrows it
@@ -211,8 +212,8 @@ class Engine:
         d = len(A)
         if d > MAXD:
             raise ValueError(f"at most {MAXD} cores per request")
-        if req.precision not in ("fp64", "mixed"):
-            raise ValueError(f"precision must be 'fp64' or 'mixed', got {req.precision!r}")
+        if req.precision not in ("fp64", "mixed", "bf16"):
+            raise ValueError(f"precision must be 'fp64', 'mixed' or 'bf16', got {req.precision!r}")
         cap = self.capacity(req)
         key = (tuple(A.row_modes), tuple(A.bonds), tuple(b.bonds), req.kickrank,
                req.max_rank, int(req.direct_solve_size), req.precision, cap,
@@ -229,8 +230,8 @@ class Engine:
         s.direct_size = int(req.direct_solve_size)
         s.rmax = cap
         s.chunk = self.chunk
-        # mixed requests get the scratch of the FP32 operator
-        s.mixed = 1 if req.precision == "mixed" else 0
+        # mixed and bf16 requests get the scratch of the FP32 operator
+        s.mixed = {"fp64": 0, "mixed": 1, "bf16": 2}[req.precision]
         # the explicit-matrix fallback needs the direct solve's buffer for any local size
         s.allow_fallback = 1 if req.allow_dense_fallback else 0
         # the local solvers (common.py:330-360): direct only, MINRES or GMRES above

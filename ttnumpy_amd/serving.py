@@ -218,7 +218,7 @@ def main(argv=None) -> None:
     ap.add_argument("--slots", type=int, default=128, help="slots per engine")
     ap.add_argument("--single-engine", action="store_true",
                     help="one engine for all levels instead of one per level")
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--precision", choices=("fp64", "mixed", "bf16"), default="fp64")
     ap.add_argument("--tune-sample", type=int, default=32,
                     help="warm-up requests per engine for the tile tuner (0: no tuning)")
     a = ap.parse_args(argv)
