@@ -218,6 +218,65 @@ struct MixedOp {
 template <typename OutT, int KR>
 __device__ __forceinline__ void local_matrix_cols(const LocOp& op, OutT* out, int64_t ld, bool lower,
                                                   const double* psi, const double* phi,
+                                                  const double* A, int pt, int pb, int pf);
+
+// operator bonds up to 16: the terms are read where the sums use them (KR
+// clamped loads per term would not fit in registers)
+template <typename OutT>
+__device__ __forceinline__ void local_matrix_cols_wide(const LocOp& op, OutT* out, int64_t ld,
+                                                       bool lower, const double* psi,
+                                                       const double* phi, const double* A, int pt,
+                                                       int pb, int pf) {
+  constexpr int kMaxRa = 16;
+  const int cols = (int)op.cols();
+  const int Ra = (int)op.Ra, Rb = (int)op.Rb, fl = (int)op.fl, fr = (int)op.fr, nr = (int)op.nr,
+            nc = (int)op.nc, rr = (int)op.rr;
+  const int npairs = nr * fr;
+  const int S = cols >= (int)blockDim.x ? 1 : (int)blockDim.x / cols;
+  const int a_stride = nr * nc * Rb;
+  for (int it = threadIdx.x; it < cols * S; it += blockDim.x) {
+    const int sl = it / cols, c = it - sl * cols;
+    const int cq = c / rr, t_r = c - cq * rr, t_l = cq / nc, j = cq - t_l * nc;
+    const double* ps = psi + t_l * Ra * fl;
+    const double* ph = phi + t_r * pt;
+    const int pbeg = npairs * sl / S, pend = npairs * (sl + 1) / S;
+    for (int pq = pbeg; pq < pend; ++pq) {
+      const int i = pq / fr, f_r = pq - i * fr;
+      double T[kMaxRa];
+#pragma unroll
+      for (int a = 0; a < kMaxRa; ++a) {
+        if (a < Ra) {
+          const double* arow = A + a * a_stride + (i * nc + j) * Rb;
+          double t = 0.0;
+          for (int b = 0; b < Rb; ++b) t += arow[b] * ph[b * pb + f_r * pf];
+          T[a] = t;
+        }
+      }
+      OutT* o = out + (int64_t)(i * fr + f_r) * ld + c;
+      const int64_t ostep = (int64_t)nr * fr * ld;
+      int f0 = 0;
+      if (lower) {
+        const int need = c - (i * fr + f_r);
+        f0 = need > 0 ? (need + nr * fr - 1) / (nr * fr) : 0;
+      }
+      for (int f_l = f0; f_l < fl; ++f_l) {
+        double sum = 0.0;
+#pragma unroll
+        for (int a = 0; a < kMaxRa; ++a) {
+          if (a < Ra) {
+            const double p = ps[a * fl + f_l];
+            if (p != 0.0) sum += p * T[a];
+          }
+        }
+        o[f_l * ostep] = (OutT)sum;
+      }
+    }
+  }
+}
+
+template <typename OutT, int KR>
+__device__ __forceinline__ void local_matrix_cols(const LocOp& op, OutT* out, int64_t ld, bool lower,
+                                                  const double* psi, const double* phi,
                                                   const double* A, int pt, int pb, int pf) {
   const int cols = (int)op.cols();
   const int Ra = (int)op.Ra, Rb = (int)op.Rb, fl = (int)op.fl, fr = (int)op.fr, nr = (int)op.nr,
@@ -311,9 +370,9 @@ __device__ __forceinline__ void local_matrix_blk(const LocOp& op, OutT* out, int
     else if (Ra <= 8 && Rb <= 8)
       local_matrix_cols<OutT, 8>(op, out, ld, lower, sp, sf, sa, 1, fr * rr, rr);
     else
-      local_matrix_cols<OutT, kMaxRa>(op, out, ld, lower, sp, sf, sa, 1, fr * rr, rr);
+      local_matrix_cols_wide<OutT>(op, out, ld, lower, sp, sf, sa, 1, fr * rr, rr);
   } else {
-    local_matrix_cols<OutT, kMaxRa>(op, out, ld, lower, op.psi, op.phi, op.A, Rb * fr, fr, 1);
+    local_matrix_cols_wide<OutT>(op, out, ld, lower, op.psi, op.phi, op.A, Rb * fr, fr, 1);
   }
   __syncthreads();
 }
