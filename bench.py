@@ -112,23 +112,29 @@ def c2_rounding(with_cpu: bool):
     s = tx.TensorTrain(cores1).add(tx.TensorTrain(cores2))
     out = {}
     spectra = {}
+    from ttnumpy_amd import device as D
     for prec in ("fp64", "mixed"):
         s.truncate(error=1e-10, max_rank=96, precision=prec)  # warm
         torch.cuda.synchronize()
         sp = []
+        f0 = D.flops()
         t0 = time.perf_counter()
         o = s.truncate(error=1e-10, max_rank=96, precision=prec, spectra=sp)
         torch.cuda.synchronize()
-        out[prec] = {"ms": 1e3 * (time.perf_counter() - t0), "bonds_max": max(o.bonds)}
+        out[prec] = {"ms": 1e3 * (time.perf_counter() - t0), "bonds_max": max(o.bonds),
+                     "gflop": (D.flops() - f0) / 1e9}
         spectra[prec] = ([x.cpu().numpy() for x in sp], o.bonds)
     b64 = spectra["fp64"][1]
     worst = 0.0
     for i, (a, b) in enumerate(zip(spectra["mixed"][0], spectra["fp64"][0])):
         kept = b64[len(b64) - 2 - i]
         worst = max(worst, float(np.max(np.abs(a[:kept] - b[:kept]) / b[:kept])))
+    gf = out["fp64"]["gflop"]
     res = {"gpu_fp64_ms": out["fp64"]["ms"], "gpu_mixed_ms": out["mixed"]["ms"],
            "bonds_identical": spectra["mixed"][1] == b64, "max_bond": max(b64),
-           "mixed_max_rel_singular_value_diff": worst}
+           "mixed_max_rel_singular_value_diff": worst, "gflop": gf,
+           "gflop_per_s": gf / (1e-3 * out["fp64"]["ms"]),
+           "frac_fp64_peak": gf / (1e-3 * out["fp64"]["ms"]) / (1e3 * PEAK_FP64_TFLOPS)}
     # randomized rounding (sketch bond 96 + 10 < the train's 128): time and its
     # distance to the direct rounding
     s.truncate(error=1e-10, max_rank=96, method="randomized", seed=0)  # warm
@@ -186,7 +192,8 @@ def c5_high_dim(with_cpu: bool):
         res[name] = {"gpu_s": wall, "stop_reason": str(info.stop_reason),
                      "sweeps": info.num_sweeps_run, "max_bond": max(x.bonds),
                      "residual_fp64": info.residuals[-1], "local_solves": info.local_solves,
-                     "gflop": gf, "gflop_per_s": gf / wall}
+                     "gflop": gf, "gflop_per_s": gf / wall,
+                     "frac_fp64_peak": gf / wall / (1e3 * PEAK_FP64_TFLOPS)}
     # the same solve on the batched engine (one slot, rank capacity 64): every
     # site step of the sweep stays on the device between the host's polls
     from ttnumpy_amd.engine import Engine, Request
@@ -299,7 +306,8 @@ def c3_diffusion_3d(with_cpu: bool, solve_levels=(4, 5), cpu_level: int = 4, ful
                               "gpu_s": wall, "stop_reason": str(inf.stop_reason),
                               "sweeps": inf.num_sweeps_run, "max_bond": max(x.bonds),
                               "residual_fp64": inf.residuals[-1], "local_solves": inf.local_solves,
-                              "gflop": gf, "gflop_per_s": gf / wall}
+                              "gflop": gf, "gflop_per_s": gf / wall,
+                     "frac_fp64_peak": gf / wall / (1e3 * PEAK_FP64_TFLOPS)}
     if with_cpu:
         from threadpoolctl import threadpool_limits
         from oracle import tt_oracle as O
@@ -365,6 +373,7 @@ def c3_full(sweeps: int, level: int = 12):
                 "residuals_fp64": inf.residuals, "max_bonds": inf.max_bonds,
                 "final_bonds": x.bonds, "local_solves": inf.local_solves,
                 "local_inexact": inf.local_inexact, "gflop": gf, "gflop_per_s": gf / wall,
+                "frac_fp64_peak": gf / wall / (1e3 * PEAK_FP64_TFLOPS),
                 "sweep_log": lines}
     finally:
         log.removeHandler(h)
@@ -403,6 +412,8 @@ def c4_serving(n_requests: int, with_cpu: bool, cpu_per_level: int):
     repm, _ = serve(cfgm, build_requests(cfgm, items), progress=progress)
     out["gpu_mixed"] = dataclasses.asdict(repm)
     out["mixed_ratio_vs_fp64"] = repm.solves_per_s / rep.solves_per_s
+    for key in ("gpu", "gpu_mixed"):
+        out[key]["frac_fp64_peak"] = out[key]["gflop_per_s"] / (1e3 * PEAK_FP64_TFLOPS)
     if with_cpu:
         from threadpoolctl import threadpool_limits
         from oracle import tt_oracle as O
