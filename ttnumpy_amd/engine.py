@@ -177,6 +177,15 @@ class Engine:
         self._bnorms = {}
         self._x0_images = {}
         self._z0_images = {}
+        # prepared descriptors of admitted requests, keyed by everything the
+        # descriptor depends on except the slot's arena (requests that repeat
+        # an earlier one are admitted by a copy and four pointers)
+        self._templates = {}
+        # device work of admissions (arena clears and x0/z0 images), issued in
+        # a few multi-tensor launches when the slot array is pushed
+        self._dev_zero: List[torch.Tensor] = []
+        self._dev_dst: List[torch.Tensor] = []
+        self._dev_src: List[torch.Tensor] = []
         self.profiling = False
         self.kernel_ms = np.zeros(NKERNELS)     # per tick kernel (profiling runs only)
         self.kernel_flops = np.zeros(NKERNELS)  # algorithmic flops of retired requests
@@ -193,11 +202,22 @@ class Engine:
     def _host_bytes(self) -> np.ndarray:
         return np.frombuffer(self.host, dtype=np.uint8)
 
+    def _flush_dev(self) -> None:
+        """Issue the admissions' queued arena clears and image copies (on the
+        current stream, before the slot descriptors that point at them)."""
+        if self._dev_zero:
+            torch._foreach_zero_(self._dev_zero)
+        if self._dev_dst:
+            torch._foreach_copy_(self._dev_dst, self._dev_src)
+        self._dev_zero, self._dev_dst, self._dev_src = [], [], []
+
     def _push_all(self) -> None:
+        self._flush_dev()
         self.pinned.numpy()[:] = self._host_bytes()
         self.dev_slots.copy_(self.pinned, non_blocking=False)
 
     def _push(self, i: int) -> None:
+        self._flush_dev()
         a, b = i * self.nbytes, (i + 1) * self.nbytes
         self.pinned.numpy()[a:b] = self._host_bytes()[a:b]
         self.dev_slots[a:b].copy_(self.pinned[a:b], non_blocking=False)
@@ -322,8 +342,24 @@ class Engine:
         """Place ``req`` in free slot ``i``: arena, descriptor, initial trains.
 
         With ``push=False`` the descriptor stays on the host until the caller
-        pushes the slot array (one copy for a whole admission round).
+        pushes the slot array (one copy for a whole admission round); the
+        arena's clear and the x0/z0 copies are issued with that push.
         """
+        tkey = None
+        if req.x0 is not None and req.seed is not None:
+            tkey = (self._train_key(req.A), self._train_key(req.b), self._train_key(req.x0),
+                    req.kickrank, req.max_rank, int(req.direct_solve_size), req.precision,
+                    self.capacity(req), bool(req.allow_dense_fallback), str(req.local_solver),
+                    bool(req.symmetric), req.residual_tol, req.max_sweeps, req.stagnation_factor,
+                    req.local_rtol, req.residual_damp, req.error_threshold, req.seed, self.tiles,
+                    self.minres_slice)
+            hit = self._templates.get(tkey)
+            if hit is not None:
+                tmpl, nx, nz, need, x0_img, z0_img = hit
+                s = Slot()
+                ctypes.memmove(ctypes.addressof(s), ctypes.addressof(tmpl), ctypes.sizeof(Slot))
+                self._place(i, req, s, nx, nz, need, x0_img, z0_img, push)
+                return
         if req.kickrank <= 0:
             raise ValueError(f"kickrank must be a positive integer, got {req.kickrank}")
         if req.max_rank is not None and req.max_rank <= 0:
@@ -362,20 +398,9 @@ class Engine:
         # arena: x | z | env | scratch, kept by the slot across requests
         nx, nz = d * s.cap_core, d * s.cap_zcore
         need = nx + nz + s.env_size + s.scr_size
-        arena = self.arena[i]
-        if arena is None or arena.numel() < need:
-            self.arena[i] = None
-            arena = self.arena[i] = torch.empty(need, dtype=D.F64, device=self.dev)
-        arena[nx + nz:need].zero_()
-        arena[:nx].copy_(self._x0_image(x0, s.cap_core))
+        x0_img = self._x0_image(x0, s.cap_core)
         zn = [int(s.n[k]) for k in range(d)]
-        arena[nx:nx + nz].copy_(self._z0_image(req.seed, zn, req.kickrank, s.cap_zcore))
-        base = arena.data_ptr()
-        s.x = base
-        s.z = base + 8 * nx
-        s.env = base + 8 * (nx + nz)
-        s.scr = base + 8 * (nx + nz + s.env_size)
-        keep = [A, b]
+        z0_img = self._z0_image(req.seed, zn, req.kickrank, s.cap_zcore)
         for k in range(d):
             s.A[k] = A.cores[k].data_ptr()
             s.b[k] = b.cores[k].data_ptr()
@@ -394,15 +419,53 @@ class Engine:
         s.sstage = 0
         s.chk_core = -1
         s.minres_slice = self.minres_slice
+        if tkey is not None:
+            if len(self._templates) >= 64:
+                self._templates.pop(next(iter(self._templates)))
+            tmpl = Slot()
+            ctypes.memmove(ctypes.addressof(tmpl), ctypes.addressof(s), ctypes.sizeof(Slot))
+            self._templates[tkey] = (tmpl, nx, nz, need, x0_img, z0_img)
+        self._place(i, req, s, nx, nz, need, x0_img, z0_img, push)
+
+    def _place(self, i: int, req: Request, s: Slot, nx: int, nz: int, need: int,
+               x0_img: torch.Tensor, z0_img: torch.Tensor, push: bool) -> None:
+        """Bind the prepared descriptor ``s`` to slot ``i``'s arena (grown if too
+        small) and queue the arena's clear and its x0/z0 images."""
+        arena = self.arena[i]
+        if arena is None or arena.numel() < need:
+            self.arena[i] = None
+            arena = self.arena[i] = torch.empty(need, dtype=D.F64, device=self.dev)
+        self._dev_zero.append(arena[nx + nz:need])
+        self._dev_dst += [arena[:nx], arena[nx:nx + nz]]
+        self._dev_src += [x0_img, z0_img]
+        base = arena.data_ptr()
+        s.x = base
+        s.z = base + 8 * nx
+        s.env = base + 8 * (nx + nz)
+        s.scr = base + 8 * (nx + nz + s.env_size)
         ctypes.memmove(ctypes.addressof(self.host[i]), ctypes.addressof(s), ctypes.sizeof(Slot))
         self.slot_req[i] = req
-        self.slot_mem[i] = tuple(keep)
+        self.slot_mem[i] = (req.A, req.b)
         req.admitted = time.perf_counter()
         if push:
             self._push(i)
 
     # ------------------------------------------------------------ retirement
-    def _retire(self, i: int, push: bool = True) -> None:
+    def _retire_many(self, slots: Sequence[int]) -> None:
+        """Retire the finished ``slots`` (not pushed): the x blocks of the done
+        ones leave their arenas in one concatenating copy, each result's cores
+        are views of its part of it."""
+        blocks = {}
+        done = [i for i in slots if self.host[i].status == SLOT_DONE]
+        if done:
+            sizes = [int(self.host[i].d) * int(self.host[i].cap_core) for i in done]
+            big = torch.cat([self.arena[i][:n] for i, n in zip(done, sizes)])
+            for i, part in zip(done, torch.split(big, sizes)):
+                blocks[i] = part
+        for i in slots:
+            self._retire(i, push=False, xs=blocks.get(i))
+
+    def _retire(self, i: int, push: bool = True, xs: Optional[torch.Tensor] = None) -> None:
         s = self.host[i]
         req = self.slot_req[i]
         self.kernel_flops += np.ctypeslib.as_array(s.flops)
@@ -417,9 +480,15 @@ class Engine:
             cap = s.cap_core
             bonds = list(s.rx[:d + 1])
             modes = list(s.n[:d])
-            xs = self.arena[i][:d * cap].clone()
-            cores = [xs[k * cap:k * cap + bonds[k] * modes[k] * bonds[k + 1]].view(
-                bonds[k], modes[k], 1, bonds[k + 1]) for k in range(d)]
+            if xs is None:
+                xs = self.arena[i][:d * cap].clone()
+            # the block holds core k at offset k cap, padded to cap
+            sizes = []
+            for k in range(d):
+                n_k = bonds[k] * modes[k] * bonds[k + 1]
+                sizes += [n_k, cap - n_k]
+            parts = torch.split(xs, sizes)
+            cores = [parts[2 * k].view(bonds[k], modes[k], 1, bonds[k + 1]) for k in range(d)]
             x = TensorTrain._from_device_cores(cores, bonds, modes)
             x.orth_center = 0
             info = SolverInfo()
@@ -523,19 +592,18 @@ class Engine:
             self.tick(poll)
             ticks += poll
             self._pull()
-            retired = False
-            for i in range(self.nslots):
-                if self.slot_req[i] is not None and self.host[i].status != SLOT_RUNNING:
-                    req = self.slot_req[i]
-                    self._retire(i, push=False)
-                    retired = True
-                    if self._regrow(req):
-                        pending.insert(0, req)
-                        continue
-                    done.append(req)
-                    if on_done is not None:
-                        on_done(req)
-            if retired:
+            fin = [i for i in range(self.nslots)
+                   if self.slot_req[i] is not None and self.host[i].status != SLOT_RUNNING]
+            reqs = [self.slot_req[i] for i in fin]
+            self._retire_many(fin)
+            for req in reqs:
+                if self._regrow(req):
+                    pending.insert(0, req)
+                    continue
+                done.append(req)
+                if on_done is not None:
+                    on_done(req)
+            if fin:
                 self._push_all()
             if ticks > max_ticks:
                 raise RuntimeError("engine did not finish within the tick limit")
@@ -589,19 +657,18 @@ class EnginePool:
                     self.events[i] = None
                     with torch.cuda.stream(self.streams[i]):
                         e._host_bytes()[:] = e.pinned.numpy()
-                        retired = False
-                        for j in range(e.nslots):
-                            if e.slot_req[j] is not None and e.host[j].status != SLOT_RUNNING:
-                                req = e.slot_req[j]
-                                e._retire(j, push=False)
-                                retired = True
-                                if e._regrow(req):
-                                    pending[i].insert(0, req)
-                                    continue
-                                done.append(req)
-                                if on_done is not None:
-                                    on_done(req)
-                        if retired:
+                        fin = [j for j in range(e.nslots)
+                               if e.slot_req[j] is not None and e.host[j].status != SLOT_RUNNING]
+                        reqs = [e.slot_req[j] for j in fin]
+                        e._retire_many(fin)
+                        for req in reqs:
+                            if e._regrow(req):
+                                pending[i].insert(0, req)
+                                continue
+                            done.append(req)
+                            if on_done is not None:
+                                on_done(req)
+                        if fin:
                             e._push_all()
                     progressed = True
                 with torch.cuda.stream(self.streams[i]):
