@@ -10,6 +10,10 @@
 #define TTX_MAXD 64
 #define TTX_MAXSWEEPS 64
 #define TTX_ENGINE_KERNELS 7  /* per tick: env, solve, chol32, chol, lu, post, check */
+/* stage masks of a tick: bit i launches kernel i of the list above */
+#define TTX_ENGINE_STAGES_ALL 0x7f
+#define TTX_ENGINE_STAGE_CHOL32 0x04 /* FP32 Cholesky and FP32 MINRES (mixed requests) */
+#define TTX_ENGINE_STAGE_LU 0x10     /* LU and GMRES */
 
 enum ttx_phase {
   TTX_PH_XORTH = 1,     // right-orthonormalise x0, k = d-1 .. 1
@@ -119,24 +123,31 @@ int ttx_engine_plan(ttx_slot* s);
  * Advance every slot of the device array `slots` by `ticks` ticks (seven
  * kernels each: environments; local rhs and MINRES; FP32 Cholesky with FP64
  * refinement; Cholesky; LU; truncation/enrichment/shift; residual check).  Slots with status != RUNNING are skipped.  ticks == 0
- * only prepares the kernels' LDS attributes.
+ * only prepares the kernels' LDS attributes.  `stages` (a subset of
+ * TTX_ENGINE_STAGES_ALL, else TTX_EINVAL) names the kernels a tick launches:
+ * a slot whose local solve waits on a kernel left out stays where it is,
+ * untouched, until a tick launches that kernel, so leaving out kernels no
+ * slot needs changes no result.
  */
-int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks);
+int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
+                    uint32_t stages);
 /*
- * ttx_engine_tick with HIP events around each of the seven kernels and a
+ * ttx_engine_tick (all seven kernels) with HIP events around each kernel and a
  * synchronisation per tick; ms_out[0..6] accumulate the kernels' milliseconds.
  */
 int ttx_engine_profile(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
                        double* ms_out);
 /*
- * `ticks` ticks of ttx_engine_tick recorded once as a HIP graph (captured on
+ * `ticks` ticks of ttx_engine_tick with the kernels of `stages` recorded once
+ * as a HIP graph (captured on
  * a private stream; *graph_out receives an opaque handle).  The kernels read
  * everything through the device slot table `slots`, which must outlive the
  * graph; admissions and retirements change the table's contents, not its
  * address, so the graph serves every later poll.  launch replays it on
  * `stream`; destroy frees it.
  */
-int ttx_engine_graph_create(ttx_slot* slots, int64_t nslots, int64_t ticks, void** graph_out);
+int ttx_engine_graph_create(ttx_slot* slots, int64_t nslots, int64_t ticks, uint32_t stages,
+                            void** graph_out);
 int ttx_engine_graph_launch(ttx_stream_t stream, void* graph);
 int ttx_engine_graph_destroy(void* graph);
 

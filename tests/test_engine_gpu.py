@@ -406,6 +406,56 @@ def test_engine_graph_replays_equal_eager_ticks():
     geng.close_graphs()
 
 
+def test_engine_idle_stage_skipping_bitwise_equal():
+    """Ticks that leave out the FP32 and LU kernels while no resident slot
+    needs them give the same bits as ticks that launch all seven.  Three slots
+    serve a queue whose mix changes as requests retire: symmetric FP64
+    requests, a negated Poisson operator (negative definite: every Cholesky
+    fails and the solve turns to LU between polls, then waits for the next
+    poll's LU kernel), a mixed-precision request and a nonsymmetric GMRES
+    request."""
+    import ttnumpy_amd as tx
+    from test_api_gpu import _convection_diffusion_cores
+    from ttnumpy_amd.engine import STAGES_ALL, STAGE_CHOL32, STAGE_LU, Engine, Request
+    print("engine idle stage skipping", flush=True)
+
+    def queue():
+        neg_a, neg_b = P.poisson_2d_cores(4), P.ones_cores([2] * 8)
+        neg_a[0], neg_b[0] = -neg_a[0], -neg_b[0]
+        cd = _convection_diffusion_cores(6, 20.0)
+        return [_poisson_request(4, seed=2),
+                Request(A=_tt(neg_a), b=_tt(neg_b), x0=tx.rand([2] * 8, ranks=2, seed=3),
+                        symmetric=True, residual_tol=1e-8, max_sweeps=20, seed=3),
+                _poisson_request(5, seed=1, x0_seed=1),
+                _poisson_request(5, seed=4, x0_seed=4, precision="mixed"),
+                Request(A=_tt(cd), b=_tt(P.ones_cores([2] * 6)), x0=tx.rand([2] * 6, ranks=2, seed=0),
+                        symmetric=False, residual_tol=1e-8, max_sweeps=10, direct_solve_size=4,
+                        max_rank=16, seed=0),
+                _poisson_request(4, seed=5, x0_seed=6)]
+
+    out = {}
+    for skip in (False, True):
+        eng = Engine(3, rmax=40)
+        eng.skip_idle_stages = skip
+        done = sorted(eng.run(queue(), poll=4), key=lambda r: r.seed)
+        out[skip] = (done, {k[1] for k in eng._graphs}, eng.chol_fallbacks, eng.mixed_solves)
+        eng.close_graphs()
+    (full, full_masks, full_lu, full_mixed), (skip, skip_masks, skip_lu, skip_mixed) = out[False], out[True]
+    print("masks", sorted(skip_masks), "Cholesky to LU", full_lu, skip_lu, "mixed", full_mixed,
+          skip_mixed, flush=True)
+    assert full_masks == {STAGES_ALL}
+    assert STAGES_ALL & ~(STAGE_CHOL32 | STAGE_LU) in skip_masks and len(skip_masks) > 1
+    assert full_lu == skip_lu > 0 and full_mixed == skip_mixed
+    for a, b in zip(full, skip):
+        assert a.seed == b.seed and a.error is None and b.error is None, (a.error, b.error)
+        assert a.info.residuals == b.info.residuals
+        assert a.info.num_sweeps_run == b.info.num_sweeps_run
+        assert a.result.bonds == b.result.bonds
+        for ca, cb in zip(a.result.to_numpy(), b.result.to_numpy()):
+            np.testing.assert_array_equal(ca, cb)
+    assert all(str(r.info.stop_reason) in ("converged", "stagnated") for r in skip)
+
+
 def test_engine_tile_variants_bitwise_equal_and_tuner():
     """The Cholesky tile variants (128- and 64-row tiles below the diagonal
     blocks) sum every element over the same k in the same order, so a mix with direct

@@ -1985,12 +1985,21 @@ static void launch_stage(int i, hipStream_t st, ttx_slot* slots, int64_t nslots)
   }
 }
 
-extern "C" int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks) {
-  if (nslots < 1 || ticks < 0) return TTX_EINVAL;
+// The kernels of a tick hand a local solve on through slot.sstage, and every
+// kernel returns at once for a slot whose sstage names another kernel's solve
+// (env, post and check wait for kSolveNone), so a slot waiting on a kernel
+// that a tick leaves out is untouched by that tick.
+static void launch_tick(hipStream_t st, ttx_slot* slots, int64_t nslots, uint32_t stages) {
+  for (int i = 0; i < TTX_ENGINE_KERNELS; ++i)
+    if ((stages >> i) & 1u) launch_stage(i, st, slots, nslots);
+}
+
+extern "C" int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
+                               uint32_t stages) {
+  if (nslots < 1 || ticks < 0 || (stages & ~(uint32_t)TTX_ENGINE_STAGES_ALL) != 0) return TTX_EINVAL;
   if (engine_prepare() != TTX_OK) return TTX_EHIP;
   hipStream_t st = (hipStream_t)stream;
-  for (int64_t t = 0; t < ticks; ++t)
-    for (int i = 0; i < TTX_ENGINE_KERNELS; ++i) launch_stage(i, st, slots, nslots);
+  for (int64_t t = 0; t < ticks; ++t) launch_tick(st, slots, nslots, stages);
   return launch_status();
 }
 
@@ -2004,8 +2013,10 @@ struct EngineGraph {
 };
 
 extern "C" int ttx_engine_graph_create(ttx_slot* slots, int64_t nslots, int64_t ticks,
-                                       void** graph_out) {
-  if (nslots < 1 || ticks < 1 || graph_out == nullptr) return TTX_EINVAL;
+                                       uint32_t stages, void** graph_out) {
+  if (nslots < 1 || ticks < 1 || graph_out == nullptr ||
+      (stages & ~(uint32_t)TTX_ENGINE_STAGES_ALL) != 0)
+    return TTX_EINVAL;
   if (engine_prepare() != TTX_OK) return TTX_EHIP;
   hipStream_t cs;
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return TTX_EHIP;
@@ -2015,8 +2026,7 @@ extern "C" int ttx_engine_graph_create(ttx_slot* slots, int64_t nslots, int64_t 
   if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
     rc = TTX_EHIP;
   } else {
-    for (int64_t t = 0; t < ticks; ++t)
-      for (int i = 0; i < TTX_ENGINE_KERNELS; ++i) launch_stage(i, cs, slots, nslots);
+    for (int64_t t = 0; t < ticks; ++t) launch_tick(cs, slots, nslots, stages);
     if (hipStreamEndCapture(cs, &g) != hipSuccess) rc = TTX_EHIP;
   }
   if (rc == TTX_OK && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess)

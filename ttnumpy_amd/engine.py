@@ -34,6 +34,9 @@ from .tensor_train import TensorTrain
 MAXD = 64
 MAXSWEEPS = 64
 NKERNELS = 7  # per tick: env, solve (rhs + MINRES), chol32, chol, lu, post, check
+# stage masks of a tick (bit i launches kernel i)
+STAGES_ALL, STAGE_CHOL32, STAGE_LU = 0x7F, 0x04, 0x10
+_SSTAGE_LU, _SSTAGE_GMRES = 2, 6  # slot.sstage of a solve waiting on the LU kernel
 KERNEL_NAMES = ("eng_env_kernel", "eng_solve_kernel", "eng_chol32_kernel", "eng_chol_kernel",
                 "eng_lu_kernel", "eng_post_kernel", "eng_check_kernel")
 SCR_COUNT = 19
@@ -81,7 +84,7 @@ def _check_layout(lib) -> None:
     lib.ttx_engine_field_offset.argtypes = [ctypes.c_int]
     lib.ttx_engine_plan.argtypes = [ctypes.c_void_p]
     lib.ttx_engine_plan.restype = ctypes.c_int
-    lib.ttx_engine_tick.argtypes = [_vp, _vp, _i64, _i64]
+    lib.ttx_engine_tick.argtypes = [_vp, _vp, _i64, _i64, ctypes.c_uint32]
     lib.ttx_engine_tick.restype = ctypes.c_int
 
     lib.ttx_engine_profile.argtypes = [_vp, _vp, _i64, _i64, ctypes.POINTER(_f64)]
@@ -151,6 +154,9 @@ class Engine:
         # seven launches per tick (same kernels, same order, same results)
         self.graphs = bool(graphs)
         self._graphs = {}
+        # launch the FP32 and LU kernels only in polls where a resident slot
+        # can use them (see _stages); False launches all seven every tick
+        self.skip_idle_stages = True
         _check_layout(self.lib)
         self.dev = D.device()
         self.nslots = int(nslots)
@@ -165,6 +171,14 @@ class Engine:
         self.host = (Slot * self.nslots)()
         for s in self.host:
             s.status = SLOT_EMPTY
+        # numpy views of each descriptor's counter arrays (made once: building
+        # an array view of a ctypes array costs more than the sum it feeds)
+        self._v_flops = [np.ctypeslib.as_array(s.flops) for s in self.host]
+        self._v_clk = [np.ctypeslib.as_array(s.clk) for s in self.host]
+        # the descriptors as a (slot, 8-byte field) table, for the stage mask
+        self._table = np.frombuffer(self.host, dtype=np.int64).reshape(self.nslots, -1)
+        self._col = {nm: getattr(Slot, nm).offset // 8
+                     for nm in ("status", "chol", "mixed", "iterative", "sstage")}
         self.nbytes = ctypes.sizeof(Slot)
         self.dev_slots = torch.zeros(self.nslots * self.nbytes, dtype=torch.uint8, device=self.dev)
         self.pinned = torch.zeros(self.nslots * self.nbytes, dtype=torch.uint8).pin_memory()
@@ -192,6 +206,7 @@ class Engine:
         self.kernel_launches = 0
         self.minres_iters = 0
         self.lu_solves = 0
+        self.chol_fallbacks = 0  # Cholesky solves that found no positive pivot and ran LU
         self.mixed_solves = 0
         self.mixed_refines = 0
         self.mixed_fallbacks = 0
@@ -468,13 +483,14 @@ class Engine:
     def _retire(self, i: int, push: bool = True, xs: Optional[torch.Tensor] = None) -> None:
         s = self.host[i]
         req = self.slot_req[i]
-        self.kernel_flops += np.ctypeslib.as_array(s.flops)
+        self.kernel_flops += self._v_flops[i]
         self.minres_iters += int(s.minres_iters)
         self.lu_solves += int(s.lu_solves)
+        self.chol_fallbacks += int(s.chol_fallbacks)
         self.mixed_solves += int(s.mixed_solves)
         self.mixed_refines += int(s.mixed_refines)
         self.mixed_fallbacks += int(s.mixed_fallbacks)
-        self.stage_ms += np.ctypeslib.as_array(s.clk) * 1e-5  # 100 MHz ticks
+        self.stage_ms += self._v_clk[i] * 1e-5  # 100 MHz ticks
         if s.status == SLOT_DONE:
             d = s.d
             cap = s.cap_core
@@ -482,13 +498,13 @@ class Engine:
             modes = list(s.n[:d])
             if xs is None:
                 xs = self.arena[i][:d * cap].clone()
-            # the block holds core k at offset k cap, padded to cap
-            sizes = []
-            for k in range(d):
-                n_k = bonds[k] * modes[k] * bonds[k + 1]
-                sizes += [n_k, cap - n_k]
-            parts = torch.split(xs, sizes)
-            cores = [parts[2 * k].view(bonds[k], modes[k], 1, bonds[k + 1]) for k in range(d)]
+            # the block holds core k at offset k cap, padded to cap: each core
+            # is a contiguous (r_k, n_k, 1, r_k+1) view of it (one as_strided
+            # per core: the cheapest view torch builds from Python)
+            base, view = xs.storage_offset(), xs.as_strided
+            cores = [view((bonds[k], modes[k], 1, bonds[k + 1]),
+                          (modes[k] * bonds[k + 1], bonds[k + 1], bonds[k + 1], 1), base + k * cap)
+                     for k in range(d)]
             x = TensorTrain._from_device_cores(cores, bonds, modes)
             x.orth_center = 0
             info = SolverInfo()
@@ -513,6 +529,28 @@ class Engine:
             self._push(i)
 
     # ------------------------------------------------------------ loop
+    def _stages(self) -> int:
+        """The kernels the next ticks launch, from the host mirror of the slots.
+
+        The FP32 kernel (FP32 Cholesky, FP32 MINRES) runs while a mixed
+        request is resident; the LU kernel (LU, GMRES) while a resident
+        request solves with it (``symmetric=False``, GMRES) or, as the last
+        poll saw, waits on it after a failed Cholesky.  A solve that turns to
+        LU between polls waits untouched until the next poll launches the LU
+        kernel: the slot's results do not depend on when its kernels run."""
+        if not self.skip_idle_stages:
+            return STAGES_ALL
+        t, c = self._table, self._col
+        run = t[:, c["status"]] == SLOT_RUNNING
+        m = STAGES_ALL & ~(STAGE_CHOL32 | STAGE_LU)
+        if np.any(run & (t[:, c["mixed"]] != 0)):
+            m |= STAGE_CHOL32
+        ss = t[:, c["sstage"]]
+        if np.any(run & ((t[:, c["chol"]] == 0) | (t[:, c["iterative"]] == 2)
+                         | (ss == _SSTAGE_LU) | (ss == _SSTAGE_GMRES))):
+            m |= STAGE_LU
+        return m
+
     def tick(self, n: int = 1) -> None:
         if self.profiling:
             ms = (_f64 * NKERNELS)(*([0.0] * NKERNELS))
@@ -521,23 +559,24 @@ class Engine:
             self.kernel_ms += np.array(list(ms))
             self.kernel_launches += n
             return
+        stages = self._stages()
         if self.graphs:
-            g = self._graphs.get(n)
+            g = self._graphs.get((n, stages))
             if g is None:
                 h = _vp()
                 rc = self.lib.ttx_engine_graph_create(self.dev_slots.data_ptr(), self.nslots, n,
-                                                      ctypes.byref(h))
+                                                      stages, ctypes.byref(h))
                 if rc != _lib.TTX_OK:
                     # the same kernels in the same order, launched one by one
                     warnings.warn(f"HIP graph capture of the engine ticks failed (status {rc}); "
                                   "launching the ticks directly", RuntimeWarning)
                     self.graphs = False
                 else:
-                    g = self._graphs[n] = h
+                    g = self._graphs[(n, stages)] = h
             if g is not None:
                 _lib.check(self.lib.ttx_engine_graph_launch(D.stream(), g), "ttx_engine_graph_launch")
                 return
-        rc = self.lib.ttx_engine_tick(D.stream(), self.dev_slots.data_ptr(), self.nslots, n)
+        rc = self.lib.ttx_engine_tick(D.stream(), self.dev_slots.data_ptr(), self.nslots, n, stages)
         _lib.check(rc, "ttx_engine_tick")
 
     def close_graphs(self) -> None:
