@@ -14,6 +14,9 @@
 #define TTX_ENGINE_STAGES_ALL 0x7f
 #define TTX_ENGINE_STAGE_CHOL32 0x04 /* FP32 Cholesky and FP32 MINRES (mixed requests) */
 #define TTX_ENGINE_STAGE_LU 0x10     /* LU and GMRES */
+/* one kernel per tick that runs all seven stages for each slot, the slots
+ * not waiting for each other between stages (same bits as the seven) */
+#define TTX_ENGINE_FUSED 0x80
 
 enum ttx_phase {
   TTX_PH_XORTH = 1,     // right-orthonormalise x0, k = d-1 .. 1
@@ -124,7 +127,8 @@ int ttx_engine_plan(ttx_slot* s);
  * kernels each: environments; local rhs and MINRES; FP32 Cholesky with FP64
  * refinement; Cholesky; LU; truncation/enrichment/shift; residual check).  Slots with status != RUNNING are skipped.  ticks == 0
  * only prepares the kernels' LDS attributes.  `stages` (a subset of
- * TTX_ENGINE_STAGES_ALL, else TTX_EINVAL) names the kernels a tick launches:
+ * TTX_ENGINE_STAGES_ALL, else TTX_EINVAL; or TTX_ENGINE_FUSED, which
+ * launches the one fused kernel instead) names the kernels a tick launches:
  * a slot whose local solve waits on a kernel left out stays where it is,
  * untouched, until a tick launches that kernel, so leaving out kernels no
  * slot needs changes no result.

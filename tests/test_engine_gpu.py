@@ -288,8 +288,9 @@ def test_engine_alone_equals_in_mix():
 
 def test_serving_diffusion_alone_equals_in_stream():
     """Parametric diffusion requests of different grid sizes served as a timed
-    stream (continuous admission, MINRES slices over ticks) give the same bits
-    as the same request served alone."""
+    stream (continuous admission, MINRES slices over ticks, the fused tick
+    kernel) give the same bits as the same request served alone on the seven
+    tick kernels."""
     from ttnumpy_amd.engine import Engine
     from ttnumpy_amd.serving import ServingConfig, ServingItem, build_requests, serve
     print("serving diffusion alone vs stream", flush=True)
@@ -406,17 +407,18 @@ def test_engine_graph_replays_equal_eager_ticks():
     geng.close_graphs()
 
 
-def test_engine_idle_stage_skipping_bitwise_equal():
+def test_engine_idle_stage_skipping_and_fused_bitwise_equal():
     """Ticks that leave out the FP32 and LU kernels while no resident slot
-    needs them give the same bits as ticks that launch all seven.  Three slots
-    serve a queue whose mix changes as requests retire: symmetric FP64
-    requests, a negated Poisson operator (negative definite: every Cholesky
-    fails and the solve turns to LU between polls, then waits for the next
-    poll's LU kernel), a mixed-precision request and a nonsymmetric GMRES
-    request."""
+    needs them, and ticks as one fused kernel (each slot runs the seven stages
+    without waiting for the others), give the same bits as ticks that launch
+    all seven kernels.  Three slots serve a queue whose mix changes as
+    requests retire: symmetric FP64 requests, a negated Poisson operator
+    (negative definite: every Cholesky fails and the solve turns to LU
+    between polls, then waits for the next poll's LU kernel), a
+    mixed-precision request and a nonsymmetric GMRES request."""
     import ttnumpy_amd as tx
     from test_api_gpu import _convection_diffusion_cores
-    from ttnumpy_amd.engine import STAGES_ALL, STAGE_CHOL32, STAGE_LU, Engine, Request
+    from ttnumpy_amd.engine import FUSED, STAGES_ALL, STAGE_CHOL32, STAGE_LU, Engine, Request
     print("engine idle stage skipping", flush=True)
 
     def queue():
@@ -434,26 +436,30 @@ def test_engine_idle_stage_skipping_bitwise_equal():
                 _poisson_request(4, seed=5, x0_seed=6)]
 
     out = {}
-    for skip in (False, True):
-        eng = Engine(3, rmax=40)
-        eng.skip_idle_stages = skip
+    for mode in ("all", "skip", "fused"):
+        eng = Engine(3, rmax=40, fused=mode == "fused")
+        eng.skip_idle_stages = mode == "skip"
         done = sorted(eng.run(queue(), poll=4), key=lambda r: r.seed)
-        out[skip] = (done, {k[1] for k in eng._graphs}, eng.chol_fallbacks, eng.mixed_solves)
+        out[mode] = (done, {k[1] for k in eng._graphs}, eng.chol_fallbacks, eng.mixed_solves)
         eng.close_graphs()
-    (full, full_masks, full_lu, full_mixed), (skip, skip_masks, skip_lu, skip_mixed) = out[False], out[True]
-    print("masks", sorted(skip_masks), "Cholesky to LU", full_lu, skip_lu, "mixed", full_mixed,
-          skip_mixed, flush=True)
+    full, full_masks, full_lu, full_mixed = out["all"]
+    print({m: (sorted(o[1]), o[2], o[3]) for m, o in out.items()}, flush=True)
     assert full_masks == {STAGES_ALL}
+    assert out["fused"][1] == {FUSED}
+    skip_masks = out["skip"][1]
     assert STAGES_ALL & ~(STAGE_CHOL32 | STAGE_LU) in skip_masks and len(skip_masks) > 1
-    assert full_lu == skip_lu > 0 and full_mixed == skip_mixed
-    for a, b in zip(full, skip):
-        assert a.seed == b.seed and a.error is None and b.error is None, (a.error, b.error)
-        assert a.info.residuals == b.info.residuals
-        assert a.info.num_sweeps_run == b.info.num_sweeps_run
-        assert a.result.bonds == b.result.bonds
-        for ca, cb in zip(a.result.to_numpy(), b.result.to_numpy()):
-            np.testing.assert_array_equal(ca, cb)
-    assert all(str(r.info.stop_reason) in ("converged", "stagnated") for r in skip)
+    assert full_lu > 0
+    for mode in ("skip", "fused"):
+        other, _, lu, mixed = out[mode]
+        assert lu == full_lu and mixed == full_mixed
+        for a, b in zip(full, other):
+            assert a.seed == b.seed and a.error is None and b.error is None, (mode, a.error, b.error)
+            assert a.info.residuals == b.info.residuals
+            assert a.info.num_sweeps_run == b.info.num_sweeps_run
+            assert a.result.bonds == b.result.bonds
+            for ca, cb in zip(a.result.to_numpy(), b.result.to_numpy()):
+                np.testing.assert_array_equal(ca, cb)
+    assert all(str(r.info.stop_reason) in ("converged", "stagnated") for r in full)
 
 
 def test_engine_tile_variants_bitwise_equal_and_tuner():

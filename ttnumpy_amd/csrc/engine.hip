@@ -864,16 +864,35 @@ __device__ void env_body(ttx_slot* s, int kind) {
                   with_op ? s->ra[c] : 1, with_op ? s->ra[c + 1] : 1, out, s1, s2);
 }
 
-__global__ __launch_bounds__(kEngThreads, 2) void eng_env_kernel(ttx_slot* slots) {
-  ttx_slot* s = slots + blockIdx.x;
+// ---- the stages of a tick.  Each runs one kernel's share of a tick for one
+// slot and banks its flops in s->flops[stage]; the tick kernels run one stage
+// over all slots, eng_run_kernel all seven in turn for its own slot.
+
+// environments of one kind (0..4); `several`: other workgroups update the
+// same slot's counters at the same time (the tick kernel's five kinds)
+__device__ __forceinline__ void stage_env(ttx_slot* s, int kind, bool several) {
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
   const int64_t t0 = wall_clock64();
-  env_body(s, blockIdx.y);
+  env_body(s, kind);
   __syncthreads();
-  if (threadIdx.x == 0 && s->status == TTX_SLOT_RUNNING) atomicAdd((unsigned long long*)&s->clk[7],
-                                                                   (unsigned long long)(wall_clock64() - t0));
-  if (threadIdx.x == 0 && g_flops != 0.0) atomicAdd(&s->flops[0], g_flops);
+  if (threadIdx.x == 0 && s->status == TTX_SLOT_RUNNING) {
+    const unsigned long long dt = (unsigned long long)(wall_clock64() - t0);
+    if (several)
+      atomicAdd((unsigned long long*)&s->clk[7], dt);
+    else
+      s->clk[7] += (int64_t)dt;
+  }
+  if (threadIdx.x == 0 && g_flops != 0.0) {
+    if (several)
+      atomicAdd(&s->flops[0], g_flops);
+    else
+      s->flops[0] += g_flops;
+  }
+}
+
+__global__ __launch_bounds__(kEngThreads, 2) void eng_env_kernel(ttx_slot* slots) {
+  stage_env(slots + blockIdx.x, blockIdx.y, true);
 }
 
 // The local solve of a sweep step (one_site.py:118-178, common.py:729-809)
@@ -1282,9 +1301,7 @@ __device__ void minres32_body(ttx_slot* s) {
   stamp(s, 3, t0);
 }
 
-__global__ __launch_bounds__(kEngThreads, 2) void eng_chol32_kernel(ttx_slot* slots) {
-  extern __shared__ double lds[];
-  ttx_slot* s = slots + blockIdx.x;
+__device__ __forceinline__ void stage_chol32(ttx_slot* s, double* lds) {
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
   chol32_body(s, lds);
@@ -1292,6 +1309,11 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_chol32_kernel(ttx_slot* sl
   minres32_body(s);
   __syncthreads();
   if (threadIdx.x == 0 && g_flops != 0.0) s->flops[2] += g_flops;
+}
+
+__global__ __launch_bounds__(kEngThreads, 2) void eng_chol32_kernel(ttx_slot* slots) {
+  extern __shared__ double lds[];
+  stage_chol32(slots + blockIdx.x, lds);
 }
 
 __device__ __forceinline__ void chol_body(ttx_slot* s, double* lds) {
@@ -1360,8 +1382,7 @@ __device__ __forceinline__ void lu_body(ttx_slot* s, double* lds) {
   exact_local_residual(s, k, N, red, t0);
 }
 
-__global__ __launch_bounds__(kEngThreads, 2) void eng_solve_kernel(ttx_slot* slots) {
-  ttx_slot* s = slots + blockIdx.x;
+__device__ __forceinline__ void stage_solve(ttx_slot* s) {
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
   solve_body(s);
@@ -1369,14 +1390,21 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_solve_kernel(ttx_slot* slo
   if (threadIdx.x == 0 && g_flops != 0.0) s->flops[1] += g_flops;
 }
 
-__global__ __launch_bounds__(kEngThreads, 2) void eng_chol_kernel(ttx_slot* slots) {
-  extern __shared__ double lds[];
-  ttx_slot* s = slots + blockIdx.x;
+__global__ __launch_bounds__(kEngThreads, 2) void eng_solve_kernel(ttx_slot* slots) {
+  stage_solve(slots + blockIdx.x);
+}
+
+__device__ __forceinline__ void stage_chol(ttx_slot* s, double* lds) {
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
   chol_body(s, lds);
   __syncthreads();
   if (threadIdx.x == 0 && g_flops != 0.0) s->flops[3] += g_flops;
+}
+
+__global__ __launch_bounds__(kEngThreads, 2) void eng_chol_kernel(ttx_slot* slots) {
+  extern __shared__ double lds[];
+  stage_chol(slots + blockIdx.x, lds);
 }
 
 // GMRES local solves (symmetric=False, or local_solver="gmres") with the
@@ -1468,9 +1496,7 @@ __device__ void gmres_body(ttx_slot* s) {
   stamp(s, 3, t0);
 }
 
-__global__ __launch_bounds__(kEngThreads, 2) void eng_lu_kernel(ttx_slot* slots) {
-  extern __shared__ double lds[];
-  ttx_slot* s = slots + blockIdx.x;
+__device__ __forceinline__ void stage_lu(ttx_slot* s, double* lds) {
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
   gmres_body(s);
@@ -1478,6 +1504,11 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_lu_kernel(ttx_slot* slots)
   lu_body(s, lds);
   __syncthreads();
   if (threadIdx.x == 0 && g_flops != 0.0) s->flops[4] += g_flops;
+}
+
+__global__ __launch_bounds__(kEngThreads, 2) void eng_lu_kernel(ttx_slot* slots) {
+  extern __shared__ double lds[];
+  stage_lu(slots + blockIdx.x, lds);
 }
 
 // truncation, enrichment and shifts after a solve; QR sweeps and z fit
@@ -1656,8 +1687,7 @@ __device__ __forceinline__ void post_body(ttx_slot* s) {
   stamp(s, 12, t0);
 }
 
-__global__ __launch_bounds__(kEngThreads, 2) void eng_post_kernel(ttx_slot* slots) {
-  ttx_slot* s = slots + blockIdx.x;
+__device__ __forceinline__ void stage_post(ttx_slot* s) {
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
   int64_t t0 = wall_clock64();
@@ -1665,6 +1695,10 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_post_kernel(ttx_slot* slot
   __syncthreads();
   if (s->status == TTX_SLOT_RUNNING || s->status == TTX_SLOT_DONE) stamp(s, 5, t0);
   if (threadIdx.x == 0 && g_flops != 0.0) s->flops[5] += g_flops;
+}
+
+__global__ __launch_bounds__(kEngThreads, 2) void eng_post_kernel(ttx_slot* slots) {
+  stage_post(slots + blockIdx.x);
 }
 
 // ||A x - b|| / ||b|| in TT arithmetic (common.py:118-146): the residual
@@ -1810,8 +1844,7 @@ __device__ void check_body(ttx_slot* s) {
   if (threadIdx.x == 0) check_decide(s, sw, res);
 }
 
-__global__ __launch_bounds__(kEngThreads, 2) void eng_check_kernel(ttx_slot* slots) {
-  ttx_slot* s = slots + blockIdx.x;
+__device__ __forceinline__ void stage_check(ttx_slot* s) {
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
   int64_t t0 = wall_clock64();
@@ -1819,6 +1852,29 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_check_kernel(ttx_slot* slo
   __syncthreads();
   if (s->status == TTX_SLOT_RUNNING || s->status == TTX_SLOT_DONE) stamp(s, 6, t0);
   if (threadIdx.x == 0 && g_flops != 0.0) s->flops[6] += g_flops;
+}
+
+__global__ __launch_bounds__(kEngThreads, 2) void eng_check_kernel(ttx_slot* slots) {
+  stage_check(slots + blockIdx.x);
+}
+
+// A whole tick in one launch: workgroup b runs the seven stages of a tick for
+// slot b, in the tick kernels' order (the five environment kinds one after
+// another), without waiting for the other slots between stages.  A slot's
+// stages read and write only that slot, so a pass gives the bits a tick of
+// the seven kernels gives.  (One pass per launch: a loop of passes inside
+// the kernel spills thousands of registers.)
+__global__ __launch_bounds__(kEngThreads, 2) void eng_pass_kernel(ttx_slot* slots) {
+  extern __shared__ double lds[];
+  ttx_slot* s = slots + blockIdx.x;
+  if (s->status != TTX_SLOT_RUNNING) return;  // uniform across the workgroup
+  for (int kind = 0; kind < 5; ++kind) stage_env(s, kind, false);
+  stage_solve(s);
+  stage_chol32(s, lds);
+  stage_chol(s, lds);
+  stage_lu(s, lds);
+  stage_post(s);
+  stage_check(s);
 }
 
 // the stopping test and the next phase after a completed sweep (thread 0)
@@ -1959,9 +2015,9 @@ extern "C" int ttx_engine_plan(ttx_slot* s) {
 static int engine_prepare() {
   static bool attr = false;
   if (!attr) {
-    const void* fns[5] = {(const void*)eng_chol32_kernel, (const void*)eng_chol_kernel,
-                          (const void*)eng_lu_kernel, (const void*)eng_post_kernel,
-                          (const void*)eng_check_kernel};
+    const void* fns[6] = {(const void*)eng_chol32_kernel, (const void*)eng_chol_kernel,
+                          (const void*)eng_lu_kernel,     (const void*)eng_post_kernel,
+                          (const void*)eng_check_kernel,  (const void*)eng_pass_kernel};
     for (const void* f : fns)
       if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kEngLds) !=
           hipSuccess)
@@ -1989,14 +2045,19 @@ static void launch_stage(int i, hipStream_t st, ttx_slot* slots, int64_t nslots)
 // kernel returns at once for a slot whose sstage names another kernel's solve
 // (env, post and check wait for kSolveNone), so a slot waiting on a kernel
 // that a tick leaves out is untouched by that tick.
+// With TTX_ENGINE_FUSED the whole tick is one eng_pass_kernel launch.
 static void launch_tick(hipStream_t st, ttx_slot* slots, int64_t nslots, uint32_t stages) {
+  if (stages & TTX_ENGINE_FUSED) {
+    hipLaunchKernelGGL(eng_pass_kernel, dim3((unsigned)nslots), dim3(kEngThreads), kEngLds, st, slots);
+    return;
+  }
   for (int i = 0; i < TTX_ENGINE_KERNELS; ++i)
     if ((stages >> i) & 1u) launch_stage(i, st, slots, nslots);
 }
 
 extern "C" int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
                                uint32_t stages) {
-  if (nslots < 1 || ticks < 0 || (stages & ~(uint32_t)TTX_ENGINE_STAGES_ALL) != 0) return TTX_EINVAL;
+  if (nslots < 1 || ticks < 0 || (stages & ~(uint32_t)(TTX_ENGINE_STAGES_ALL | TTX_ENGINE_FUSED)) != 0) return TTX_EINVAL;
   if (engine_prepare() != TTX_OK) return TTX_EHIP;
   hipStream_t st = (hipStream_t)stream;
   for (int64_t t = 0; t < ticks; ++t) launch_tick(st, slots, nslots, stages);
@@ -2015,7 +2076,7 @@ struct EngineGraph {
 extern "C" int ttx_engine_graph_create(ttx_slot* slots, int64_t nslots, int64_t ticks,
                                        uint32_t stages, void** graph_out) {
   if (nslots < 1 || ticks < 1 || graph_out == nullptr ||
-      (stages & ~(uint32_t)TTX_ENGINE_STAGES_ALL) != 0)
+      (stages & ~(uint32_t)(TTX_ENGINE_STAGES_ALL | TTX_ENGINE_FUSED)) != 0)
     return TTX_EINVAL;
   if (engine_prepare() != TTX_OK) return TTX_EHIP;
   hipStream_t cs;

@@ -35,7 +35,7 @@ MAXD = 64
 MAXSWEEPS = 64
 NKERNELS = 7  # per tick: env, solve (rhs + MINRES), chol32, chol, lu, post, check
 # stage masks of a tick (bit i launches kernel i)
-STAGES_ALL, STAGE_CHOL32, STAGE_LU = 0x7F, 0x04, 0x10
+STAGES_ALL, STAGE_CHOL32, STAGE_LU, FUSED = 0x7F, 0x04, 0x10, 0x80
 _SSTAGE_LU, _SSTAGE_GMRES = 2, 6  # slot.sstage of a solve waiting on the LU kernel
 KERNEL_NAMES = ("eng_env_kernel", "eng_solve_kernel", "eng_chol32_kernel", "eng_chol_kernel",
                 "eng_lu_kernel", "eng_post_kernel", "eng_check_kernel")
@@ -137,7 +137,8 @@ class Engine:
     """A fixed set of slots on one GPU plus the continuous-batching loop."""
 
     def __init__(self, nslots: int, rmax: int, chunk: int = 8, hbm_reserve_bytes: int = 2 << 30,
-                 minres_slice: int = 128, graphs: bool = True, max_rank_capacity: int = 1024):
+                 minres_slice: int = 128, graphs: bool = True, max_rank_capacity: int = 1024,
+                 fused: bool = False):
         self.lib = _lib.load()
         # A request whose bond outgrows its slot's rank capacity is planned
         # again with twice the capacity (up to max_rank_capacity, and never
@@ -157,6 +158,9 @@ class Engine:
         # launch the FP32 and LU kernels only in polls where a resident slot
         # can use them (see _stages); False launches all seven every tick
         self.skip_idle_stages = True
+        # fused: each tick is one kernel in which every slot runs the seven
+        # stages without waiting for the other slots between them (same bits)
+        self.fused = bool(fused)
         _check_layout(self.lib)
         self.dev = D.device()
         self.nslots = int(nslots)
@@ -538,6 +542,8 @@ class Engine:
         poll saw, waits on it after a failed Cholesky.  A solve that turns to
         LU between polls waits untouched until the next poll launches the LU
         kernel: the slot's results do not depend on when its kernels run."""
+        if self.fused:
+            return FUSED
         if not self.skip_idle_stages:
             return STAGES_ALL
         t, c = self._table, self._col

@@ -55,6 +55,10 @@ class ServingConfig:
     # of its own level (0: variant 0)
     tune_sample: int = 8
     tune_ticks: int = 192
+    # one kernel per tick in which each slot runs the seven stages without
+    # waiting for the other slots between them (Engine(fused=True), same
+    # bits): the slots of a level hold requests at different sites and sweeps
+    fused: bool = True
 
 
 @dataclass
@@ -112,6 +116,7 @@ class ServingReport:
     # per engine (keyed by grid level, or "all"): the tile variant chosen on
     # the warm-up sample, its measured kernel-time gain over variant 0, tuning seconds
     tiles: Dict[object, dict] = field(default_factory=dict)
+    fused: bool = True            # ticks as one fused kernel (else seven kernels)
 
 
 def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
@@ -127,10 +132,11 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
         engines = [engine]
         pool = None
     elif cfg.per_level:
-        engines = [Engine(cfg.nslots, rmax=cfg.rmax, minres_slice=cfg.minres_slice) for _ in levels]
+        engines = [Engine(cfg.nslots, rmax=cfg.rmax, minres_slice=cfg.minres_slice, fused=cfg.fused)
+                   for _ in levels]
         pool = EnginePool(engines, lambda r: levels.index(r.tag.level))
     else:
-        engines = [Engine(cfg.nslots, rmax=cfg.rmax, minres_slice=cfg.minres_slice)]
+        engines = [Engine(cfg.nslots, rmax=cfg.rmax, minres_slice=cfg.minres_slice, fused=cfg.fused)]
         pool = None
     tiles: Dict[object, dict] = {}
     if cfg.tune_sample > 0:
@@ -189,7 +195,7 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
         converged=stops.count("converged"), stagnated=stops.count("stagnated"),
         max_sweeps=stops.count("max_sweeps"),
         worst_residual=max((r.info.residuals[-1] for r in ok), default=float("nan")),
-        by_level=dict(sorted(by_level.items())), tiles=tiles)
+        by_level=dict(sorted(by_level.items())), tiles=tiles, fused=all(e.fused for e in engines))
     return rep, done
 
 
@@ -221,10 +227,12 @@ def main(argv=None) -> None:
     ap.add_argument("--precision", choices=("fp64", "mixed", "bf16"), default="fp64")
     ap.add_argument("--tune-sample", type=int, default=32,
                     help="warm-up requests per engine for the tile tuner (0: no tuning)")
+    ap.add_argument("--tick-kernels", action="store_true",
+                    help="seven kernels per tick instead of the fused one")
     a = ap.parse_args(argv)
     cfg = ServingConfig(n_requests=a.requests, rate=a.rate, seed=a.seed, nslots=a.slots,
                         per_level=not a.single_engine, precision=a.precision,
-                        tune_sample=a.tune_sample)
+                        tune_sample=a.tune_sample, fused=not a.tick_kernels)
     t = time.perf_counter()
     reqs = build_requests(cfg, request_stream(cfg))
     print(f"built {len(reqs)} requests in {time.perf_counter() - t:.1f} s", flush=True)
@@ -243,7 +251,8 @@ def main(argv=None) -> None:
                      "levels": list(cfg.levels), "theta_max": cfg.theta_max,
                      "slots_per_engine": a.slots, "per_level": cfg.per_level,
                      "precision": a.precision, "residual_tol": cfg.residual_tol,
-                     "max_sweeps": cfg.max_sweeps, "tune_sample": cfg.tune_sample}
+                     "max_sweeps": cfg.max_sweeps, "tune_sample": cfg.tune_sample,
+                     "fused": cfg.fused}
     errs = [r.error for r in done if r.error]
     if errs:
         out["first_errors"] = errs[:5]
