@@ -136,6 +136,16 @@ int ttx_engine_plan(ttx_slot* s);
 int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t ticks,
                     uint32_t stages);
 /*
+ * Ticks of every slot in one launch, each slot at its own pace: the
+ * workgroup of a slot runs the seven stages of a tick in order, pass after
+ * pass, and stops when its slot is no longer running, after `passes` passes,
+ * or (budget > 0) at the first pass boundary `budget` ticks of the 100 MHz
+ * wall clock after it started (one pass at least).  A slot's results are
+ * those of as many ttx_engine_tick ticks, bit for bit.
+ */
+int ttx_engine_run(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t passes,
+                   int64_t budget);
+/*
  * ttx_engine_tick (all seven kernels) with HIP events around each kernel and a
  * synchronisation per tick; ms_out[0..6] accumulate the kernels' milliseconds.
  */

@@ -409,9 +409,10 @@ def test_engine_graph_replays_equal_eager_ticks():
 
 def test_engine_idle_stage_skipping_and_fused_bitwise_equal():
     """Ticks that leave out the FP32 and LU kernels while no resident slot
-    needs them, and ticks as one fused kernel (each slot runs the seven stages
-    without waiting for the others), give the same bits as ticks that launch
-    all seven kernels.  Three slots serve a queue whose mix changes as
+    needs them, ticks as one fused kernel (each slot runs the seven stages
+    without waiting for the others), and paced polls (each slot runs ticks at
+    its own pace for a wall-clock budget) give the same bits as ticks that
+    launch all seven kernels.  Three slots serve a queue whose mix changes as
     requests retire: symmetric FP64 requests, a negated Poisson operator
     (negative definite: every Cholesky fails and the solve turns to LU
     between polls, then waits for the next poll's LU kernel), a
@@ -436,20 +437,21 @@ def test_engine_idle_stage_skipping_and_fused_bitwise_equal():
                 _poisson_request(4, seed=5, x0_seed=6)]
 
     out = {}
-    for mode in ("all", "skip", "fused"):
-        eng = Engine(3, rmax=40, fused=mode == "fused")
+    for mode in ("all", "skip", "fused", "paced"):
+        eng = Engine(3, rmax=40, fused=mode in ("fused", "paced"))
         eng.skip_idle_stages = mode == "skip"
+        eng.pace_us = 3000.0 if mode == "paced" else 0.0
         done = sorted(eng.run(queue(), poll=4), key=lambda r: r.seed)
         out[mode] = (done, {k[1] for k in eng._graphs}, eng.chol_fallbacks, eng.mixed_solves)
         eng.close_graphs()
     full, full_masks, full_lu, full_mixed = out["all"]
     print({m: (sorted(o[1]), o[2], o[3]) for m, o in out.items()}, flush=True)
     assert full_masks == {STAGES_ALL}
-    assert out["fused"][1] == {FUSED}
+    assert out["fused"][1] == {FUSED} and out["paced"][1] == set()
     skip_masks = out["skip"][1]
     assert STAGES_ALL & ~(STAGE_CHOL32 | STAGE_LU) in skip_masks and len(skip_masks) > 1
     assert full_lu > 0
-    for mode in ("skip", "fused"):
+    for mode in ("skip", "fused", "paced"):
         other, _, lu, mixed = out[mode]
         assert lu == full_lu and mixed == full_mixed
         for a, b in zip(full, other):

@@ -84,6 +84,7 @@ _SIGNATURES = {
     "ttx_engine_field_offset": (_i64, [_i32]),
     "ttx_engine_plan": (_i32, [_p]),
     "ttx_engine_tick": (_i32, [_p, _p, _i64, _i64, ctypes.c_uint32]),
+    "ttx_engine_run": (_i32, [_p, _p, _i64, _i64, _i64]),
     "ttx_engine_profile": (_i32, [_p, _p, _i64, _i64, ctypes.POINTER(_f64)]),
     "ttx_engine_graph_create": (_i32, [_p, _i64, _i64, ctypes.c_uint32, ctypes.POINTER(_p)]),
     "ttx_engine_graph_launch": (_i32, [_p, _p]),

@@ -1858,23 +1858,61 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_check_kernel(ttx_slot* slo
   stage_check(slots + blockIdx.x);
 }
 
-// A whole tick in one launch: workgroup b runs the seven stages of a tick for
-// slot b, in the tick kernels' order (the five environment kinds one after
-// another), without waiting for the other slots between stages.  A slot's
-// stages read and write only that slot, so a pass gives the bits a tick of
-// the seven kernels gives.  (One pass per launch: a loop of passes inside
-// the kernel spills thousands of registers.)
+// A pass: the seven stages of a tick for slot s, in the tick kernels' order
+// (the five environment kinds one after another).  A slot's stages read and
+// write only that slot, so a pass gives the bits a tick of the seven kernels
+// gives.  The LDS operand of the stages is the kernel's dynamic LDS, named
+// here (not passed in) so that a pass compiled as a function of its own
+// still reads it with LDS instructions.
+// Each stage is a function of its own, so that its registers are allocated
+// as in its tick kernel (one function for all seven spills hundreds).
+__device__ __noinline__ void call_env(ttx_slot* s, int kind) { stage_env(s, kind, false); }
+__device__ __noinline__ void call_solve(ttx_slot* s) { stage_solve(s); }
+__device__ __noinline__ void call_chol32(ttx_slot* s) { stage_chol32(s, eng_lds); }
+__device__ __noinline__ void call_chol(ttx_slot* s) { stage_chol(s, eng_lds); }
+__device__ __noinline__ void call_lu(ttx_slot* s) { stage_lu(s, eng_lds); }
+__device__ __noinline__ void call_post(ttx_slot* s) { stage_post(s); }
+__device__ __noinline__ void call_check(ttx_slot* s) { stage_check(s); }
+
+__device__ __forceinline__ void run_pass(ttx_slot* s) {
+  for (int kind = 0; kind < 5; ++kind) call_env(s, kind);
+  call_solve(s);
+  call_chol32(s);
+  call_chol(s);
+  call_lu(s);
+  call_post(s);
+  call_check(s);
+}
+
+// A whole tick in one launch: workgroup b runs a pass for slot b without
+// waiting for the other slots between stages.
 __global__ __launch_bounds__(kEngThreads, 2) void eng_pass_kernel(ttx_slot* slots) {
-  extern __shared__ double lds[];
   ttx_slot* s = slots + blockIdx.x;
   if (s->status != TTX_SLOT_RUNNING) return;  // uniform across the workgroup
-  for (int kind = 0; kind < 5; ++kind) stage_env(s, kind, false);
-  stage_solve(s);
-  stage_chol32(s, lds);
-  stage_chol(s, lds);
-  stage_lu(s, lds);
-  stage_post(s);
-  stage_check(s);
+  run_pass(s);
+}
+
+
+
+// Many passes per launch, each slot at its own pace: workgroup b runs passes
+// for slot b until the slot stops running, `passes` passes are done, or (with
+// budget > 0) `budget` ticks of the 100 MHz wall clock have passed since it
+// started (checked between passes; one pass at least).  Thread 0 decides and
+// every thread reads the decision after a barrier.
+__global__ __launch_bounds__(kEngThreads, 2) void eng_run_kernel(ttx_slot* slots, int64_t passes,
+                                                                 int64_t budget) {
+  __shared__ int s_go;
+  ttx_slot* s = slots + blockIdx.x;
+  const int64_t start = wall_clock64();
+  for (int64_t p = 0; p < passes; ++p) {
+    __syncthreads();  // every thread has read the last pass's s_go
+    if (threadIdx.x == 0)
+      s_go = s->status == TTX_SLOT_RUNNING &&
+             (p == 0 || budget <= 0 || wall_clock64() - start < budget);
+    __syncthreads();
+    if (!s_go) break;
+    run_pass(s);
+  }
 }
 
 // the stopping test and the next phase after a completed sweep (thread 0)
@@ -2015,9 +2053,10 @@ extern "C" int ttx_engine_plan(ttx_slot* s) {
 static int engine_prepare() {
   static bool attr = false;
   if (!attr) {
-    const void* fns[6] = {(const void*)eng_chol32_kernel, (const void*)eng_chol_kernel,
+    const void* fns[7] = {(const void*)eng_chol32_kernel, (const void*)eng_chol_kernel,
                           (const void*)eng_lu_kernel,     (const void*)eng_post_kernel,
-                          (const void*)eng_check_kernel,  (const void*)eng_pass_kernel};
+                          (const void*)eng_check_kernel,  (const void*)eng_pass_kernel,
+                          (const void*)eng_run_kernel};
     for (const void* f : fns)
       if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kEngLds) !=
           hipSuccess)
@@ -2061,6 +2100,16 @@ extern "C" int ttx_engine_tick(ttx_stream_t stream, ttx_slot* slots, int64_t nsl
   if (engine_prepare() != TTX_OK) return TTX_EHIP;
   hipStream_t st = (hipStream_t)stream;
   for (int64_t t = 0; t < ticks; ++t) launch_tick(st, slots, nslots, stages);
+  return launch_status();
+}
+
+extern "C" int ttx_engine_run(ttx_stream_t stream, ttx_slot* slots, int64_t nslots, int64_t passes,
+                              int64_t budget) {
+  if (nslots < 1 || passes < 0) return TTX_EINVAL;
+  if (engine_prepare() != TTX_OK) return TTX_EHIP;
+  if (passes == 0) return TTX_OK;
+  hipLaunchKernelGGL(eng_run_kernel, dim3((unsigned)nslots), dim3(kEngThreads), kEngLds,
+                     (hipStream_t)stream, slots, passes, budget);
   return launch_status();
 }
 

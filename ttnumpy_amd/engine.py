@@ -161,6 +161,10 @@ class Engine:
         # fused: each tick is one kernel in which every slot runs the seven
         # stages without waiting for the other slots between them (same bits)
         self.fused = bool(fused)
+        # with fused and pace_us > 0 a poll is one launch in which every slot
+        # runs ticks at its own pace for about pace_us microseconds
+        # (ttx_engine_run), instead of `poll` ticks in lockstep
+        self.pace_us = 0.0
         _check_layout(self.lib)
         self.dev = D.device()
         self.nslots = int(nslots)
@@ -565,6 +569,12 @@ class Engine:
             self.kernel_ms += np.array(list(ms))
             self.kernel_launches += n
             return
+        if self.fused and self.pace_us > 0:
+            budget = max(1, int(self.pace_us * 100))  # 100 MHz wall clock; never 0 (no limit)
+            rc = self.lib.ttx_engine_run(D.stream(), self.dev_slots.data_ptr(), self.nslots, 1 << 40,
+                                         budget)
+            _lib.check(rc, "ttx_engine_run")
+            return
         stages = self._stages()
         if self.graphs:
             g = self._graphs.get((n, stages))
@@ -658,14 +668,15 @@ class Engine:
 class EnginePool:
     """Several engines served concurrently, each on its own HIP stream.
 
-    A tick advances every slot of one engine in lockstep, so a slot doing a
-    long step holds up the rest of its engine; requests of very different
-    sizes therefore go to different engines (``route(request) -> index``),
-    whose ticks run side by side on the GPU.  The host launches ``poll``
-    ticks plus an asynchronous copy of the slot descriptors on an engine's
-    stream and services whichever engine's copy has landed first: retire the
-    finished requests, admit arrived ones into free slots (within free HBM),
-    launch the next ticks.  Every operation of an engine runs on its stream.
+    Requests of very different sizes go to different engines
+    (``route(request) -> index``), whose launches run side by side on the
+    GPU: with lockstep ticks a slot doing a long step holds up the rest of its
+    engine (paced engines, ``Engine.pace_us``, do not have that problem).  The
+    host launches a poll (``poll`` ticks, or one paced launch) plus an
+    asynchronous copy of the slot descriptors on an engine's stream and
+    services whichever engine's copy has landed first: retire the finished
+    requests, admit arrived ones into free slots (within free HBM), launch the
+    next poll.  Every operation of an engine runs on its stream.
     """
 
     def __init__(self, engines: Sequence[Engine], route):

@@ -11,11 +11,12 @@ by AMEn (``symmetric=True``, ``residual_tol``, ``max_sweeps``) from
 ``x0 = rand([2]*2L, ranks=2, seed=i)``.
 
 The engine admits a request once it has arrived, a slot is free and free HBM
-covers the slot's arena; every tick advances all of its requests by one step
-of their own sweep (the same site-step kernels serve every request in one
-launch per kernel).  With ``per_level`` each grid level has its own engine on
-its own stream, so that small requests do not wait on the ticks of large ones.  Latency is
-measured from arrival to result.  Operators and right-hand sides are built
+covers the slot's arena.  Each poll is one launch in which every slot runs the
+steps of its own sweep at its own pace for about ``pace_us`` (the fused kernel
+of all seven stages, called pass after pass; ``pace_us=0``: ``poll`` ticks in
+lockstep, one launch per tick); the results are the bits the tick kernels give.
+With ``per_level`` each grid level has its own engine on its own stream.
+Latency is measured from arrival to result.  Operators and right-hand sides are built
 and copied to the device before the clock starts, as a client would hand
 them over.
 """
@@ -59,6 +60,10 @@ class ServingConfig:
     # waiting for the other slots between them (Engine(fused=True), same
     # bits): the slots of a level hold requests at different sites and sweeps
     fused: bool = True
+    # with fused, pace_us > 0: each poll is one launch in which every slot runs
+    # ticks at its own pace for about pace_us microseconds (Engine.pace_us;
+    # same bits): no slot waits for a slower one at tick boundaries either
+    pace_us: float = 100000.0
 
 
 @dataclass
@@ -117,6 +122,7 @@ class ServingReport:
     # the warm-up sample, its measured kernel-time gain over variant 0, tuning seconds
     tiles: Dict[object, dict] = field(default_factory=dict)
     fused: bool = True            # ticks as one fused kernel (else seven kernels)
+    pace_us: float = 0.0          # paced polls of about this many microseconds (0: ticks)
 
 
 def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
@@ -138,6 +144,8 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
     else:
         engines = [Engine(cfg.nslots, rmax=cfg.rmax, minres_slice=cfg.minres_slice, fused=cfg.fused)]
         pool = None
+    for e in engines:
+        e.pace_us = cfg.pace_us if e.fused else 0.0
     tiles: Dict[object, dict] = {}
     if cfg.tune_sample > 0:
         from .autotune import tune_engine
@@ -195,7 +203,8 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
         converged=stops.count("converged"), stagnated=stops.count("stagnated"),
         max_sweeps=stops.count("max_sweeps"),
         worst_residual=max((r.info.residuals[-1] for r in ok), default=float("nan")),
-        by_level=dict(sorted(by_level.items())), tiles=tiles, fused=all(e.fused for e in engines))
+        by_level=dict(sorted(by_level.items())), tiles=tiles, fused=all(e.fused for e in engines),
+        pace_us=max(e.pace_us for e in engines))
     return rep, done
 
 
@@ -229,10 +238,13 @@ def main(argv=None) -> None:
                     help="warm-up requests per engine for the tile tuner (0: no tuning)")
     ap.add_argument("--tick-kernels", action="store_true",
                     help="seven kernels per tick instead of the fused one")
+    ap.add_argument("--pace-ms", type=float, default=100.0,
+                    help="paced polls of this many milliseconds (0: polls of ticks)")
     a = ap.parse_args(argv)
     cfg = ServingConfig(n_requests=a.requests, rate=a.rate, seed=a.seed, nslots=a.slots,
                         per_level=not a.single_engine, precision=a.precision,
-                        tune_sample=a.tune_sample, fused=not a.tick_kernels)
+                        tune_sample=a.tune_sample, fused=not a.tick_kernels,
+                        pace_us=1e3 * a.pace_ms)
     t = time.perf_counter()
     reqs = build_requests(cfg, request_stream(cfg))
     print(f"built {len(reqs)} requests in {time.perf_counter() - t:.1f} s", flush=True)
@@ -252,7 +264,7 @@ def main(argv=None) -> None:
                      "slots_per_engine": a.slots, "per_level": cfg.per_level,
                      "precision": a.precision, "residual_tol": cfg.residual_tol,
                      "max_sweeps": cfg.max_sweeps, "tune_sample": cfg.tune_sample,
-                     "fused": cfg.fused}
+                     "fused": cfg.fused, "pace_us": cfg.pace_us}
     errs = [r.error for r in done if r.error]
     if errs:
         out["first_errors"] = errs[:5]
