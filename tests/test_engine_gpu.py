@@ -288,14 +288,14 @@ def test_engine_alone_equals_in_mix():
 
 def test_serving_diffusion_alone_equals_in_stream():
     """Parametric diffusion requests of different grid sizes served as a timed
-    stream (continuous admission, MINRES slices over ticks, the fused tick
-    kernel) give the same bits as the same request served alone on the seven
-    tick kernels."""
+    stream (one engine per level on its own stream, continuous admission,
+    MINRES slices over ticks, paced polls of the fused stages) give the same
+    bits as the same request served alone on the seven tick kernels."""
     from ttnumpy_amd.engine import Engine
     from ttnumpy_amd.serving import ServingConfig, ServingItem, build_requests, serve
     print("serving diffusion alone vs stream", flush=True)
     cfg = ServingConfig(n_requests=6, levels=(4, 5, 6), rate=200.0, nslots=3, rmax=40, poll=4,
-                        seed=11)
+                        seed=11, per_level=True)
     rep, done = serve(cfg)
     print(rep, flush=True)
     assert rep.errors == 0 and rep.solves == 6
@@ -313,7 +313,8 @@ def test_serving_diffusion_alone_equals_in_stream():
 def test_serving_c4_stream_matches_oracle():
     """C4's parametric diffusion stream (``examples/parametric_diffusion.py:35-58``:
     k = 1 + theta sin(pi x) sin(pi y), b = ones), here at L = 6..8, served with
-    continuous admission by one engine per level (EnginePool).  Each result is
+    continuous admission and paced polls by one engine for all levels, as the
+    serving loop does by default.  Each result is
     compared with the oracle's AMEn on the same operator, right-hand side,
     initial guess and seed, and with the exact discrete solution (sparse
     direct solve): the same stop reason, sweeps within one, the error within

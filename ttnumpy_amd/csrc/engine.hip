@@ -1127,7 +1127,11 @@ constexpr double kRefineAccept = 1e-11;
 // of 2 N (N+1) floats, which holds them for N >= 113
 static_assert(kMixedMinN >= 113, "mixed direct solves need N >= 113 for their scratch");
 
-__device__ void chol32_body(ttx_slot* s, double* lds) {
+// (a function of its own, called from two kernels: it names the dynamic LDS
+// itself, so that its LDS operands are read with LDS instructions whatever
+// pointer a caller holds)
+__device__ void chol32_body(ttx_slot* s) {
+  double* lds = eng_lds;
   __shared__ double red[16];
   __shared__ int32_t s_info;
   if (!solve_step_active(s) || s->sstage != kSolveChol32) return;
@@ -1301,10 +1305,10 @@ __device__ void minres32_body(ttx_slot* s) {
   stamp(s, 3, t0);
 }
 
-__device__ __forceinline__ void stage_chol32(ttx_slot* s, double* lds) {
+__device__ __forceinline__ void stage_chol32(ttx_slot* s) {
   if (threadIdx.x == 0) g_flops = 0.0;
   __syncthreads();
-  chol32_body(s, lds);
+  chol32_body(s);
   __syncthreads();
   minres32_body(s);
   __syncthreads();
@@ -1312,8 +1316,7 @@ __device__ __forceinline__ void stage_chol32(ttx_slot* s, double* lds) {
 }
 
 __global__ __launch_bounds__(kEngThreads, 2) void eng_chol32_kernel(ttx_slot* slots) {
-  extern __shared__ double lds[];
-  stage_chol32(slots + blockIdx.x, lds);
+  stage_chol32(slots + blockIdx.x);
 }
 
 __device__ __forceinline__ void chol_body(ttx_slot* s, double* lds) {
@@ -1868,7 +1871,7 @@ __global__ __launch_bounds__(kEngThreads, 2) void eng_check_kernel(ttx_slot* slo
 // as in its tick kernel (one function for all seven spills hundreds).
 __device__ __noinline__ void call_env(ttx_slot* s, int kind) { stage_env(s, kind, false); }
 __device__ __noinline__ void call_solve(ttx_slot* s) { stage_solve(s); }
-__device__ __noinline__ void call_chol32(ttx_slot* s) { stage_chol32(s, eng_lds); }
+__device__ __noinline__ void call_chol32(ttx_slot* s) { stage_chol32(s); }
 __device__ __noinline__ void call_chol(ttx_slot* s) { stage_chol(s, eng_lds); }
 __device__ __noinline__ void call_lu(ttx_slot* s) { stage_lu(s, eng_lds); }
 __device__ __noinline__ void call_post(ttx_slot* s) { stage_post(s); }

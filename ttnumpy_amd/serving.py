@@ -45,10 +45,13 @@ class ServingConfig:
     seed: int = 0
     residual_tol: float = 1e-8
     max_sweeps: int = 20
-    nslots: int = 128             # per engine
+    nslots: int = 512             # per engine
     rmax: int = 64
     poll: int = 16
-    per_level: bool = True        # one engine (own stream) per grid level
+    # one engine (own stream) per grid level: with lockstep ticks it keeps small
+    # requests from waiting on large ones; paced polls need no such split, and
+    # one engine whose slots are all resident at once serves more
+    per_level: bool = False
     minres_slice: int = 128       # MINRES iterations per tick
     precision: str = "fp64"       # "mixed": FP32 Cholesky with FP64 refinement for direct solves
     # before the clock starts, each engine's tile variant is picked from its
@@ -230,9 +233,9 @@ def main(argv=None) -> None:
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--rate", type=float, default=50.0)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--slots", type=int, default=128, help="slots per engine")
-    ap.add_argument("--single-engine", action="store_true",
-                    help="one engine for all levels instead of one per level")
+    ap.add_argument("--slots", type=int, default=512, help="slots per engine")
+    ap.add_argument("--per-level", action="store_true",
+                    help="one engine per grid level instead of one for all")
     ap.add_argument("--precision", choices=("fp64", "mixed", "bf16"), default="fp64")
     ap.add_argument("--tune-sample", type=int, default=32,
                     help="warm-up requests per engine for the tile tuner (0: no tuning)")
@@ -242,7 +245,7 @@ def main(argv=None) -> None:
                     help="paced polls of this many milliseconds (0: polls of ticks)")
     a = ap.parse_args(argv)
     cfg = ServingConfig(n_requests=a.requests, rate=a.rate, seed=a.seed, nslots=a.slots,
-                        per_level=not a.single_engine, precision=a.precision,
+                        per_level=a.per_level, precision=a.precision,
                         tune_sample=a.tune_sample, fused=not a.tick_kernels,
                         pace_us=1e3 * a.pace_ms)
     t = time.perf_counter()
