@@ -443,17 +443,21 @@ def test_engine_idle_stage_skipping_and_fused_bitwise_equal():
         eng.skip_idle_stages = mode == "skip"
         eng.pace_us = 3000.0 if mode == "paced" else 0.0
         done = sorted(eng.run(queue(), poll=4), key=lambda r: r.seed)
-        out[mode] = (done, {k[1] for k in eng._graphs}, eng.chol_fallbacks, eng.mixed_solves)
+        out[mode] = (done, {k[1] for k in eng._graphs}, eng.chol_fallbacks, eng.mixed_solves,
+                     eng.paced_polls)
         eng.close_graphs()
-    full, full_masks, full_lu, full_mixed = out["all"]
-    print({m: (sorted(o[1]), o[2], o[3]) for m, o in out.items()}, flush=True)
+    full, full_masks, full_lu, full_mixed, _ = out["all"]
+    print({m: (sorted(o[1]), o[2], o[3], o[4]) for m, o in out.items()}, flush=True)
     assert full_masks == {STAGES_ALL}
-    assert out["fused"][1] == {FUSED} and out["paced"][1] == set()
+    # the paced engine serves polls with one slot left (or slots in step) by
+    # lockstep ticks of the seven kernels, the others by paced launches
+    assert out["fused"][1] == {FUSED} and FUSED not in out["paced"][1]
+    assert out["paced"][4] > 0 and out["fused"][4] == 0
     skip_masks = out["skip"][1]
     assert STAGES_ALL & ~(STAGE_CHOL32 | STAGE_LU) in skip_masks and len(skip_masks) > 1
     assert full_lu > 0
     for mode in ("skip", "fused", "paced"):
-        other, _, lu, mixed = out[mode]
+        other, _, lu, mixed, _ = out[mode]
         assert lu == full_lu and mixed == full_mixed
         for a, b in zip(full, other):
             assert a.seed == b.seed and a.error is None and b.error is None, (mode, a.error, b.error)
@@ -463,6 +467,38 @@ def test_engine_idle_stage_skipping_and_fused_bitwise_equal():
             for ca, cb in zip(a.result.to_numpy(), b.result.to_numpy()):
                 np.testing.assert_array_equal(ca, cb)
     assert all(str(r.info.stop_reason) in ("converged", "stagnated") for r in full)
+
+
+def test_engine_paced_aligned_polls_use_tick_kernels():
+    """A paced engine serves polls whose running slots all hold the same
+    request at the same step with lockstep ticks of the seven kernels (tick
+    graphs get recorded, no paced launch), and polls of a mixed queue with
+    paced launches; either way the results are the bits of a lockstep
+    engine."""
+    import dataclasses
+    from ttnumpy_amd.engine import STAGES_ALL, STAGE_CHOL32, STAGE_LU, Engine
+    print("engine paced aligned polls", flush=True)
+    one = _poisson_request(5, seed=1, x0_seed=1)
+    same = [dataclasses.replace(one) for _ in range(3)]  # the same trains: one descriptor
+    ref = Engine(3, rmax=40).run([_poisson_request(5, seed=1, x0_seed=1)], poll=4)[0]
+    eng = Engine(3, rmax=40, fused=True)
+    eng.pace_us = 2000.0
+    eng.run(same, poll=4)
+    assert {k[1] for k in eng._graphs} == {STAGES_ALL & ~(STAGE_CHOL32 | STAGE_LU)}
+    assert eng.paced_polls == 0
+    mix = [_poisson_request(4, seed=2), _poisson_request(5, seed=1, x0_seed=1),
+           _poisson_request(6, seed=3, x0_seed=5)]
+    eng2 = Engine(3, rmax=40, fused=True)
+    eng2.pace_us = 2000.0
+    eng2.run(mix, poll=4)
+    assert eng2.paced_polls > 0
+    for r in same + [mix[1]]:
+        assert r.error is None
+        assert r.info.residuals == ref.info.residuals
+        assert r.result.bonds == ref.result.bonds
+        for a, b in zip(r.result.to_numpy(), ref.result.to_numpy()):
+            np.testing.assert_array_equal(a, b)
+    eng.close_graphs()
 
 
 def test_engine_tile_variants_bitwise_equal_and_tuner():

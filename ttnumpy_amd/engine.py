@@ -163,7 +163,9 @@ class Engine:
         self.fused = bool(fused)
         # with fused and pace_us > 0 a poll is one launch in which every slot
         # runs ticks at its own pace for about pace_us microseconds
-        # (ttx_engine_run), instead of `poll` ticks in lockstep
+        # (ttx_engine_run), instead of `poll` ticks in lockstep -- except in
+        # polls whose resident slots all hold the same request at the same
+        # step, which the seven tick kernels serve fastest (see _aligned)
         self.pace_us = 0.0
         _check_layout(self.lib)
         self.dev = D.device()
@@ -186,7 +188,13 @@ class Engine:
         # the descriptors as a (slot, 8-byte field) table, for the stage mask
         self._table = np.frombuffer(self.host, dtype=np.int64).reshape(self.nslots, -1)
         self._col = {nm: getattr(Slot, nm).offset // 8
-                     for nm in ("status", "chol", "mixed", "iterative", "sstage")}
+                     for nm in ("status", "chol", "mixed", "iterative", "sstage", "phase", "k",
+                                "sweep")}
+        # per slot: the id of the prepared descriptor its request came from
+        # (equal ids: equal requests), or a unique negative id
+        self._slot_tid = np.zeros(self.nslots, dtype=np.int64)
+        self._tids = {}
+        self._tid_next = 0
         self.nbytes = ctypes.sizeof(Slot)
         self.dev_slots = torch.zeros(self.nslots * self.nbytes, dtype=torch.uint8, device=self.dev)
         self.pinned = torch.zeros(self.nslots * self.nbytes, dtype=torch.uint8).pin_memory()
@@ -212,6 +220,7 @@ class Engine:
         self.kernel_ms = np.zeros(NKERNELS)     # per tick kernel (profiling runs only)
         self.kernel_flops = np.zeros(NKERNELS)  # algorithmic flops of retired requests
         self.kernel_launches = 0
+        self.paced_polls = 0  # polls served by one paced launch (ttx_engine_run)
         self.minres_iters = 0
         self.lu_solves = 0
         self.chol_fallbacks = 0  # Cholesky solves that found no positive pivot and ran LU
@@ -369,6 +378,8 @@ class Engine:
         arena's clear and the x0/z0 copies are issued with that push.
         """
         tkey = None
+        self._tid_next += 1
+        tid = -self._tid_next  # unique unless the request has a prepared descriptor
         if req.x0 is not None and req.seed is not None:
             tkey = (self._train_key(req.A), self._train_key(req.b), self._train_key(req.x0),
                     req.kickrank, req.max_rank, int(req.direct_solve_size), req.precision,
@@ -376,12 +387,14 @@ class Engine:
                     bool(req.symmetric), req.residual_tol, req.max_sweeps, req.stagnation_factor,
                     req.local_rtol, req.residual_damp, req.error_threshold, req.seed, self.tiles,
                     self.minres_slice)
+            tid = self._tids.setdefault(tkey, len(self._tids) + 1)
             hit = self._templates.get(tkey)
             if hit is not None:
                 tmpl, nx, nz, need, x0_img, z0_img = hit
                 s = Slot()
                 ctypes.memmove(ctypes.addressof(s), ctypes.addressof(tmpl), ctypes.sizeof(Slot))
                 self._place(i, req, s, nx, nz, need, x0_img, z0_img, push)
+                self._slot_tid[i] = tid
                 return
         if req.kickrank <= 0:
             raise ValueError(f"kickrank must be a positive integer, got {req.kickrank}")
@@ -449,6 +462,7 @@ class Engine:
             ctypes.memmove(ctypes.addressof(tmpl), ctypes.addressof(s), ctypes.sizeof(Slot))
             self._templates[tkey] = (tmpl, nx, nz, need, x0_img, z0_img)
         self._place(i, req, s, nx, nz, need, x0_img, z0_img, push)
+        self._slot_tid[i] = tid
 
     def _place(self, i: int, req: Request, s: Slot, nx: int, nz: int, need: int,
                x0_img: torch.Tensor, z0_img: torch.Tensor, push: bool) -> None:
@@ -537,7 +551,7 @@ class Engine:
             self._push(i)
 
     # ------------------------------------------------------------ loop
-    def _stages(self) -> int:
+    def _stages(self, lockstep: bool = False) -> int:
         """The kernels the next ticks launch, from the host mirror of the slots.
 
         The FP32 kernel (FP32 Cholesky, FP32 MINRES) runs while a mixed
@@ -545,8 +559,10 @@ class Engine:
         request solves with it (``symmetric=False``, GMRES) or, as the last
         poll saw, waits on it after a failed Cholesky.  A solve that turns to
         LU between polls waits untouched until the next poll launches the LU
-        kernel: the slot's results do not depend on when its kernels run."""
-        if self.fused:
+        kernel: the slot's results do not depend on when its kernels run.
+        A fused engine's ticks are the one fused kernel, except in the
+        lockstep polls of a paced engine (``lockstep``: the seven kernels)."""
+        if self.fused and not lockstep:
             return FUSED
         if not self.skip_idle_stages:
             return STAGES_ALL
@@ -561,6 +577,24 @@ class Engine:
             m |= STAGE_LU
         return m
 
+    def _aligned(self) -> bool:
+        """True when every running slot holds the same request (same prepared
+        descriptor) at the same step (phase, site, sweep, solve stage), as the
+        host mirror last saw them: such slots stay in step, and lockstep ticks
+        of the seven kernels serve them faster than paced polls."""
+        t, c = self._table, self._col
+        run = t[:, c["status"]] == SLOT_RUNNING
+        if not np.any(run):
+            return True
+        ids = self._slot_tid[run]
+        if np.any(ids != ids[0]):
+            return False
+        for nm in ("phase", "k", "sweep", "sstage"):
+            v = t[run, c[nm]]
+            if np.any(v != v[0]):
+                return False
+        return True
+
     def tick(self, n: int = 1) -> None:
         if self.profiling:
             ms = (_f64 * NKERNELS)(*([0.0] * NKERNELS))
@@ -569,13 +603,15 @@ class Engine:
             self.kernel_ms += np.array(list(ms))
             self.kernel_launches += n
             return
-        if self.fused and self.pace_us > 0:
+        paced = self.fused and self.pace_us > 0
+        if paced and not self._aligned():
             budget = max(1, int(self.pace_us * 100))  # 100 MHz wall clock; never 0 (no limit)
             rc = self.lib.ttx_engine_run(D.stream(), self.dev_slots.data_ptr(), self.nslots, 1 << 40,
                                          budget)
             _lib.check(rc, "ttx_engine_run")
+            self.paced_polls += 1
             return
-        stages = self._stages()
+        stages = self._stages(lockstep=paced)
         if self.graphs:
             g = self._graphs.get((n, stages))
             if g is None:
