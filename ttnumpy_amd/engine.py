@@ -193,8 +193,9 @@ class Engine:
         # per slot: the id of the prepared descriptor its request came from
         # (equal ids: equal requests), or a unique negative id
         self._slot_tid = np.zeros(self.nslots, dtype=np.int64)
-        self._tids = {}
-        self._tid_next = 0
+        self._tids = {}       # descriptor key -> id (ids are never reused)
+        self._tid_seq = 0     # last positive id handed out
+        self._tid_next = 0    # last negative id handed out
         self.nbytes = ctypes.sizeof(Slot)
         self.dev_slots = torch.zeros(self.nslots * self.nbytes, dtype=torch.uint8, device=self.dev)
         self.pinned = torch.zeros(self.nslots * self.nbytes, dtype=torch.uint8).pin_memory()
@@ -387,7 +388,12 @@ class Engine:
                     bool(req.symmetric), req.residual_tol, req.max_sweeps, req.stagnation_factor,
                     req.local_rtol, req.residual_damp, req.error_threshold, req.seed, self.tiles,
                     self.minres_slice)
-            tid = self._tids.setdefault(tkey, len(self._tids) + 1)
+            tid = self._tids.get(tkey)
+            if tid is None:
+                if len(self._tids) >= 256:  # forgetting a key only loses alignment
+                    self._tids.pop(next(iter(self._tids)))
+                self._tid_seq += 1
+                tid = self._tids[tkey] = self._tid_seq
             hit = self._templates.get(tkey)
             if hit is not None:
                 tmpl, nx, nz, need, x0_img, z0_img = hit
