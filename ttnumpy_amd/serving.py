@@ -126,6 +126,10 @@ class ServingReport:
     tiles: Dict[object, dict] = field(default_factory=dict)
     fused: bool = True            # ticks as one fused kernel (else seven kernels)
     pace_us: float = 0.0          # paced polls of about this many microseconds (0: ticks)
+    # slots holding a request, averaged over the run (sum of admission-to-result
+    # times / wall clock), against the slots the engines have
+    mean_busy_slots: float = 0.0
+    slots: int = 0
 
 
 def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
@@ -207,7 +211,9 @@ def serve(cfg: ServingConfig, requests: Optional[List[Request]] = None,
         max_sweeps=stops.count("max_sweeps"),
         worst_residual=max((r.info.residuals[-1] for r in ok), default=float("nan")),
         by_level=dict(sorted(by_level.items())), tiles=tiles, fused=all(e.fused for e in engines),
-        pace_us=max(e.pace_us for e in engines))
+        pace_us=max(e.pace_us for e in engines),
+        mean_busy_slots=float(sum(r.finished - r.admitted for r in done)) / wall,
+        slots=sum(e.nslots for e in engines))
     return rep, done
 
 
