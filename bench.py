@@ -651,6 +651,11 @@ def main():
         "minres_iters_per_solve": prof.minres_iters / max(args.batch, 1),
         "lu_solves_per_solve": prof.lu_solves / max(args.batch, 1),
         "gflop_per_solve": float(prof.kernel_flops.sum()) / max(args.batch, 1) / 1e9,
+        # every engine kernel's algorithmic flops at the timed solve rate, against
+        # the FP64 MFMA peak (the roofline object is the dominant kernel alone)
+        "engine_tflops": float(prof.kernel_flops.sum()) / max(args.batch, 1) * value / 1e12,
+        "engine_frac_fp64_peak": float(prof.kernel_flops.sum()) / max(args.batch, 1) * value
+                                 / (1e12 * PEAK_FP64_TFLOPS),
         "stage_ms_per_solve": {nm: round(float(v) / max(args.batch, 1), 3) for nm, v in zip(
             ("rhs", "matrix", "lu", "minres", "exact_res", "post", "check", "env", "post.residual_core",
              "post.svd", "post.rank", "post.enrich_qr", "post.zshift", "lu.panels", "lu.in_panel",
